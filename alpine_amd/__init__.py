@@ -1,0 +1,356 @@
+"""alpine_amd -- MI355X (gfx950) back end for alpine's Accelerator / LightSampler plug-ins.
+
+The package is a thin host layer over ``libapg.so`` (hand-written HIP for
+gfx950, C ABI in ``include/apg.h``).  It never computes results on the CPU:
+when the library or the GPU is missing, the entry points raise.
+
+``GpuAccelerator`` mirrors the reference's ``Accelerator`` interface
+(src/accelerators/accelerator.h:13-39): ``append_mesh``, ``get_vertex_buffer``,
+``update_shape``, ``update_scene``, ``intersect`` and ``intersect_any`` -- the
+last two batched over device-resident ray arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libapg.so")
+
+APG_OK = 0
+APG_INVALID_ID = 0xFFFFFFFF
+BUILDER_GPU = 0
+BUILDER_HOST = 1
+
+_ERRORS = {
+    -1: "invalid argument", -2: "no GPU device", -3: "HIP runtime error",
+    -4: "call out of order", -5: "capacity exceeded", -6: "traversal stack overflow",
+}
+
+
+class ApgError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"apg error {code} ({_ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class _Rays(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("ox", "oy", "oz", "dx", "dy", "dz", "tmax")]
+
+
+class _Hits(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("prim", "t", "b0", "b1")]
+
+
+class _BuildParams(C.Structure):
+    _fields_ = [("leaf_threshold", C.c_uint32), ("builder", C.c_uint32),
+                ("reserved", C.c_uint32 * 6)]
+
+
+class _SceneStats(C.Structure):
+    _fields_ = [
+        ("triangles", C.c_uint32), ("shapes", C.c_uint32), ("binary_nodes", C.c_uint32),
+        ("binary_leaves", C.c_uint32), ("wide_nodes", C.c_uint32),
+        ("max_depth_binary", C.c_uint32), ("sah_cost", C.c_double),
+        ("sah_cost_leaf", C.c_double), ("build_ms", C.c_double), ("device_bytes", C.c_uint64),
+    ]
+
+
+class TraceConfig(C.Structure):
+    _fields_ = [("block_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+                ("fetch_threshold", C.c_uint32), ("reserved", C.c_uint32 * 5)]
+
+
+_lib = None
+
+# Every symbol the C ABI headers declare (checked by tests/test_abi.py).
+ABI_SYMBOLS = [
+    "apg_create", "apg_destroy", "apg_last_error", "apg_mesh_append", "apg_mesh_vertices",
+    "apg_mesh_update", "apg_mesh_prim_offset", "apg_sphere_append", "apg_scene_build",
+    "apg_scene_get_stats", "apg_trace_config_default", "apg_set_trace_config", "apg_intersect",
+    "apg_occluded", "apg_intersect_counted", "apg_scene_triangles", "apg_scene_leaf_keys",
+    "apg_scene_wide_nodes",
+]
+DEBUG_SYMBOLS = ["apg_debug_host_build", "apg_debug_scene_info", "apg_debug_scene_copy",
+                 "apg_debug_host_trace", "apg_debug_scene_free"]
+
+
+def lib():
+    """Load libapg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ApgError(-2, f"{LIB_PATH} is missing: run `python -m alpine_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int
+    sig = {
+        "apg_create": (i32, [C.POINTER(vp), i32]),
+        "apg_destroy": (i32, [vp]),
+        "apg_last_error": (C.c_char_p, [vp]),
+        "apg_mesh_append": (i32, [vp, vp, u32, vp, u32, C.POINTER(u32)]),
+        "apg_mesh_vertices": (i32, [vp, u32, C.POINTER(C.POINTER(C.c_float)), C.POINTER(u32)]),
+        "apg_mesh_update": (i32, [vp, u32]),
+        "apg_mesh_prim_offset": (i32, [vp, u32, C.POINTER(u32)]),
+        "apg_sphere_append": (i32, [vp, vp, u32]),
+        "apg_scene_build": (i32, [vp, C.POINTER(_BuildParams)]),
+        "apg_scene_get_stats": (i32, [vp, C.POINTER(_SceneStats)]),
+        "apg_trace_config_default": (i32, [vp, C.POINTER(TraceConfig)]),
+        "apg_set_trace_config": (i32, [vp, C.POINTER(TraceConfig)]),
+        "apg_intersect": (i32, [vp, C.POINTER(_Rays), u32, C.POINTER(_Hits), vp]),
+        "apg_occluded": (i32, [vp, C.POINTER(_Rays), u32, vp, vp]),
+        "apg_intersect_counted": (i32, [vp, C.POINTER(_Rays), u32, C.POINTER(_Hits), i32, vp,
+                                        C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), vp]),
+        "apg_scene_triangles": (i32, [vp, vp, vp, u32]),
+        "apg_scene_leaf_keys": (i32, [vp, vp, u32]),
+        "apg_scene_wide_nodes": (i32, [vp, vp, u32]),
+        "apg_debug_host_build": (i32, [vp, u32, vp, u32, u32, C.POINTER(vp)]),
+        "apg_debug_scene_info": (i32, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), C.POINTER(u32), C.POINTER(u32)]),
+        "apg_debug_scene_copy": (i32, [vp, vp, vp, vp, vp]),
+        "apg_debug_host_trace": (i32, [vp, vp, vp, vp, u32, i32, vp, vp, vp, vp, vp,
+                                       C.POINTER(u32)]),
+        "apg_debug_scene_free": (None, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if isinstance(a, np.ndarray) else C.c_void_p(a.data_ptr())
+
+
+def _check_cuda(t, name):
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ApgError(-1, f"{name} must be a CUDA (HIP) tensor")
+    if not t.is_contiguous():
+        raise ApgError(-1, f"{name} must be contiguous")
+
+
+class GpuAccelerator:
+    """The ``gpu`` Accelerator (reference interface: accelerator.h:13-39)."""
+
+    INVALID_SHAPE_ID = APG_INVALID_ID
+
+    def __init__(self, device: int = 0):
+        self._lib = lib()
+        h = C.c_void_p()
+        rc = self._lib.apg_create(C.byref(h), device)
+        if rc != APG_OK:
+            raise ApgError(rc, self._lib.apg_last_error(None).decode())
+        self._h = h
+        self.device = device
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.apg_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def _rc(self, rc):
+        if rc != APG_OK:
+            raise ApgError(rc, self._lib.apg_last_error(self._h).decode())
+
+    # ---- geometry ----
+    def append_mesh(self, vertices, prims) -> int:
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(prims, np.uint32).reshape(-1, 3)
+        sid = C.c_uint32()
+        self._rc(self._lib.apg_mesh_append(self._h, _ptr(v), len(v), _ptr(t), len(t),
+                                           C.byref(sid)))
+        return sid.value
+
+    def append_sphere(self, vertices):
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 4)
+        self._rc(self._lib.apg_sphere_append(self._h, _ptr(v), len(v)))
+
+    def get_vertex_buffer(self, shape_id: int) -> np.ndarray:
+        p = C.POINTER(C.c_float)()
+        nv = C.c_uint32()
+        self._rc(self._lib.apg_mesh_vertices(self._h, shape_id, C.byref(p), C.byref(nv)))
+        return np.ctypeslib.as_array(p, shape=(nv.value, 3))
+
+    def update_shape(self, shape_id: int):
+        self._rc(self._lib.apg_mesh_update(self._h, shape_id))
+
+    def prim_offset(self, shape_id: int) -> int:
+        o = C.c_uint32()
+        self._rc(self._lib.apg_mesh_prim_offset(self._h, shape_id, C.byref(o)))
+        return o.value
+
+    def update_scene(self, leaf_threshold: int = 4, builder: int = BUILDER_GPU):
+        p = _BuildParams(leaf_threshold, builder)
+        self._rc(self._lib.apg_scene_build(self._h, C.byref(p)))
+
+    def stats(self) -> dict:
+        s = _SceneStats()
+        self._rc(self._lib.apg_scene_get_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _SceneStats._fields_}
+
+    def trace_config(self) -> TraceConfig:
+        c = TraceConfig()
+        self._rc(self._lib.apg_trace_config_default(self._h, C.byref(c)))
+        return c
+
+    def set_trace_config(self, cfg: TraceConfig):
+        self._rc(self._lib.apg_set_trace_config(self._h, C.byref(cfg)))
+
+    # ---- traversal ----
+    @staticmethod
+    def _rays(org, dir, tmax=None):
+        for name, t in (("org", org), ("dir", dir)):
+            _check_cuda(t, name)
+        if org.shape[0] != 3 or dir.shape[0] != 3:
+            raise ApgError(-1, "rays are SoA: org and dir must have shape (3, n)")
+        r = _Rays(org[0].data_ptr(), org[1].data_ptr(), org[2].data_ptr(),
+                  dir[0].data_ptr(), dir[1].data_ptr(), dir[2].data_ptr(),
+                  tmax.data_ptr() if tmax is not None else None)
+        return r
+
+    @staticmethod
+    def _stream(stream):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        return C.c_void_p(s.cuda_stream)
+
+    def intersect(self, org, dir, out=None, stream=None):
+        """Closest hit for rays org/dir of shape (3, n) float32 on the GPU.
+
+        Returns (prim uint32-as-int32, t, b0, b1) device tensors.
+        """
+        import torch
+        n = org.shape[1]
+        rays = self._rays(org, dir)
+        if out is None:
+            out = (torch.empty(n, dtype=torch.int32, device=org.device),
+                   torch.empty(n, dtype=torch.float32, device=org.device),
+                   torch.empty(n, dtype=torch.float32, device=org.device),
+                   torch.empty(n, dtype=torch.float32, device=org.device))
+        hits = _Hits(*(t.data_ptr() for t in out))
+        self._rc(self._lib.apg_intersect(self._h, C.byref(rays), n, C.byref(hits),
+                                         self._stream(stream)))
+        return out
+
+    def intersect_any(self, org, dir, tmax=None, out=None, stream=None):
+        """Any hit with t < tmax (FLT_MAX when None); returns a uint8 device tensor."""
+        import torch
+        n = org.shape[1]
+        if tmax is not None:
+            _check_cuda(tmax, "tmax")
+        rays = self._rays(org, dir, tmax)
+        if out is None:
+            out = torch.empty(n, dtype=torch.uint8, device=org.device)
+        self._rc(self._lib.apg_occluded(self._h, C.byref(rays), n, C.c_void_p(out.data_ptr()),
+                                        self._stream(stream)))
+        return out
+
+    def intersect_counted(self, org, dir, tmax=None, any_hit=False, stream=None):
+        """Instrumented run: results plus total node and triangle fetches."""
+        import torch
+        n = org.shape[1]
+        rays = self._rays(org, dir, tmax)
+        nodes, tris = C.c_uint64(), C.c_uint64()
+        if any_hit:
+            occ = torch.empty(n, dtype=torch.uint8, device=org.device)
+            self._rc(self._lib.apg_intersect_counted(self._h, C.byref(rays), n, None, 1,
+                                                     C.c_void_p(occ.data_ptr()), C.byref(nodes),
+                                                     C.byref(tris), self._stream(stream)))
+            return occ, nodes.value, tris.value
+        out = (torch.empty(n, dtype=torch.int32, device=org.device),
+               torch.empty(n, dtype=torch.float32, device=org.device),
+               torch.empty(n, dtype=torch.float32, device=org.device),
+               torch.empty(n, dtype=torch.float32, device=org.device))
+        hits = _Hits(*(t.data_ptr() for t in out))
+        self._rc(self._lib.apg_intersect_counted(self._h, C.byref(rays), n, C.byref(hits), 0,
+                                                 None, C.byref(nodes), C.byref(tris),
+                                                 self._stream(stream)))
+        return out, nodes.value, tris.value
+
+    # ---- introspection ----
+    def leaf_keys(self) -> np.ndarray:
+        n = self.stats()["triangles"]
+        out = np.empty(n, np.uint32)
+        self._rc(self._lib.apg_scene_leaf_keys(self._h, _ptr(out), n))
+        return out
+
+    def triangles(self):
+        n = self.stats()["triangles"]
+        tri9 = np.empty((n, 9), np.float32)
+        ids = np.empty(n, np.uint32)
+        self._rc(self._lib.apg_scene_triangles(self._h, _ptr(tri9), _ptr(ids), n))
+        return tri9, ids
+
+    def wide_nodes(self) -> np.ndarray:
+        n = self.stats()["wide_nodes"]
+        out = np.empty((n, 16), np.uint32)
+        self._rc(self._lib.apg_scene_wide_nodes(self._h, _ptr(out), n))
+        return out
+
+
+class DebugScene:
+    """Host build + host traversal replay (include/apg_debug.h); CPU tests only."""
+
+    def __init__(self, verts, tris, leaf_threshold=4):
+        L = lib()
+        v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+        h = C.c_void_p()
+        rc = L.apg_debug_host_build(_ptr(v), len(v), _ptr(t), len(t), leaf_threshold,
+                                    C.byref(h))
+        if rc != APG_OK:
+            raise ApgError(rc, "debug host build failed")
+        self._h = h
+        self.nt = len(t)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().apg_debug_scene_free(self._h)
+            self._h = None
+
+    def info(self):
+        nn, nt, bn, md = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        sah, sahl = C.c_double(), C.c_double()
+        lib().apg_debug_scene_info(self._h, C.byref(nn), C.byref(nt), C.byref(sah),
+                                   C.byref(sahl), C.byref(bn), C.byref(md))
+        return {"wide_nodes": nn.value, "triangles": nt.value, "sah_cost": sah.value,
+                "sah_cost_leaf": sahl.value, "binary_nodes": bn.value, "max_depth": md.value}
+
+    def arrays(self):
+        i = self.info()
+        nodes = np.empty((i["wide_nodes"], 16), np.uint32)
+        tri9 = np.empty((i["triangles"], 9), np.float32)
+        ids = np.empty(i["triangles"], np.uint32)
+        keys = np.empty(self.nt, np.uint32)
+        lib().apg_debug_scene_copy(self._h, _ptr(nodes), _ptr(tri9), _ptr(ids), _ptr(keys))
+        return nodes, tri9, ids, keys
+
+    def trace(self, org, dir, tmax=None, any_hit=False):
+        org = np.ascontiguousarray(org, np.float32)
+        dir = np.ascontiguousarray(dir, np.float32)
+        n = len(org)
+        tm = None if tmax is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(tmax, np.float32), (n,)))
+        depth = C.c_uint32()
+        if any_hit:
+            occ = np.empty(n, np.uint8)
+            lib().apg_debug_host_trace(self._h, _ptr(org), _ptr(dir),
+                                       None if tm is None else _ptr(tm), n, 1, None, None, None,
+                                       None, _ptr(occ), C.byref(depth))
+            return occ, depth.value
+        prim = np.empty(n, np.uint32)
+        t = np.empty(n, np.float32)
+        b0 = np.empty(n, np.float32)
+        b1 = np.empty(n, np.float32)
+        lib().apg_debug_host_trace(self._h, _ptr(org), _ptr(dir),
+                                   None if tm is None else _ptr(tm), n, 0, _ptr(prim), _ptr(t),
+                                   _ptr(b0), _ptr(b1), None, C.byref(depth))
+        return (prim, t, b0, b1), depth.value

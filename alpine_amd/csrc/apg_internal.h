@@ -1,0 +1,114 @@
+// Internal state of an apg_context (not part of the C ABI).
+#pragma once
+
+#include "../../include/apg.h"
+#include "wide_node.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace apg {
+
+struct Shape
+{
+    std::vector<float> xyz;      // the library's copy (Accelerator::getVertexBuffer)
+    std::vector<uint32_t> idx;   // 3 per triangle
+    uint32_t prim_offset = 0;
+};
+
+// Binary tree produced by either builder (host mirror for stats and collapse).
+struct BinNode
+{
+    float lo[3], hi[3];
+    uint32_t left = 0, right = 0;  // inner: children
+    uint32_t first = 0, count = 0; // leaf: range in the ordered triangle list (count > 0)
+};
+
+struct DeviceScene
+{
+    ApgWideNode* nodes = nullptr;
+    float* tris = nullptr;      // 9 floats per triangle, tree order
+    uint32_t* tri_ids = nullptr; // tree order -> global triangle index
+    uint32_t node_count = 0;
+    uint32_t tri_count = 0;
+};
+
+struct Context
+{
+    int device = 0;
+    int cu_count = 0;
+    std::string last_error;
+    std::vector<Shape> shapes;
+    uint32_t total_tris = 0;
+
+    // scene
+    bool built = false;
+    DeviceScene scene;
+    apg_scene_stats stats{};
+    std::vector<uint32_t> leaf_keys;  // global triangle -> smallest index in its leaf
+
+    // traversal launch state
+    apg_trace_config cfg{};
+    std::mutex launch_mutex;
+    hipEvent_t last_launch = nullptr;
+    uint32_t* counters = nullptr;     // [0] ray cursor, [1] overflow flag, [2..5] fetch counts
+    uint64_t* spill = nullptr;        // per persistent thread stack spill
+    size_t spill_threads = 0;
+
+    ~Context();
+    int fail(int code, const std::string& msg)
+    {
+        last_error = msg;
+        return code;
+    }
+};
+
+// Triangle data in reference form (bvh_common.cpp:105-125): v0, e0, e1 and
+// the vertex bounding box.
+struct TriPrep
+{
+    float v0[3], e0[3], e1[3];
+    float lo[3], hi[3];
+};
+
+void prepare_triangles(const Context& ctx, std::vector<TriPrep>& out);
+
+// Host binned-SAH builder (same split rule as the reference, bvh_common.cpp:185-343).
+// Produces the binary tree over `order` (ordered -> global index), then
+// splits leaves above APG_MAX_LEAF into balanced sub-trees.
+struct BinaryTree
+{
+    std::vector<BinNode> nodes;   // nodes[0] is the root
+    std::vector<uint32_t> order;  // tree order -> global triangle index
+    uint32_t sah_leaves = 0;
+    uint32_t sah_nodes = 0;
+    uint32_t max_depth = 0;
+    double sah_cost = 0.0, sah_cost_leaf = 0.0;
+    std::vector<uint32_t> leaf_keys;
+};
+void build_binary_host(const std::vector<TriPrep>& tris, uint32_t leaf_threshold,
+                       BinaryTree& out);
+
+// Collapse a binary tree into 64 B wide nodes (reference collapse policy,
+// wide_bvh.cpp:200-243) with octant-ordered slots and quantised boxes.
+void collapse_to_wide(const BinaryTree& bt, const std::vector<TriPrep>& tris,
+                      std::vector<ApgWideNode>& nodes, std::vector<float>& tri9,
+                      std::vector<uint32_t>& tri_ids);
+
+// Scene-wide quantisation slack (see wide_node.h).
+float quant_slack(const std::vector<TriPrep>& tris);
+
+} // namespace apg
+
+struct apg_context : apg::Context {};
+
+#define APG_HIP_CHECK(ctx, expr)                                                   \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess)                                                      \
+            return (ctx)->fail(APG_E_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
+    } while (0)

@@ -1,0 +1,276 @@
+// C ABI: context lifetime, geometry ingest and scene build orchestration.
+//
+// Reference behaviour mirrored here:
+//  - appendMesh copies the caller's vertices and indices and assigns the next
+//    shape id (bvh.cpp:83-106); the per-triangle data (v0, e0, e1, bbox) is
+//    derived exactly as Primitive::updateVertices (bvh_common.cpp:105-125).
+//  - getVertexBuffer hands out the library's own vertex copy; updateShape
+//    marks it for re-derivation; updateScene rebuilds the whole tree
+//    (bvh.cpp:114-142).
+//  - appendSphere is rejected like the reference Bvh (bvh.cpp:108-112).
+#include "apg_internal.h"
+
+#include <chrono>
+#include <cstring>
+
+namespace apg {
+
+int build_scene_gpu(Context* ctx, const std::vector<TriPrep>& tris, uint32_t leaf_threshold,
+                    BinaryTree& bt, std::vector<ApgWideNode>& nodes, std::vector<float>& tri9,
+                    std::vector<uint32_t>& ids);
+
+Context::~Context()
+{
+    (void)hipSetDevice(device);
+    if (scene.nodes) (void)hipFree(scene.nodes);
+    if (scene.tris) (void)hipFree(scene.tris);
+    if (scene.tri_ids) (void)hipFree(scene.tri_ids);
+    if (counters) (void)hipFree(counters);
+    if (spill) (void)hipFree(spill);
+    if (last_launch) (void)hipEventDestroy(last_launch);
+}
+
+static void free_scene(Context* ctx)
+{
+    if (ctx->scene.nodes) (void)hipFree(ctx->scene.nodes);
+    if (ctx->scene.tris) (void)hipFree(ctx->scene.tris);
+    if (ctx->scene.tri_ids) (void)hipFree(ctx->scene.tri_ids);
+    ctx->scene = DeviceScene{};
+}
+
+static int upload_scene(Context* ctx, const std::vector<ApgWideNode>& nodes,
+                        const std::vector<float>& tri9, const std::vector<uint32_t>& ids)
+{
+    free_scene(ctx);
+    DeviceScene& s = ctx->scene;
+    s.node_count = uint32_t(nodes.size());
+    s.tri_count = uint32_t(ids.size());
+    if (s.node_count == 0) return APG_OK;
+    APG_HIP_CHECK(ctx, hipMalloc(&s.nodes, nodes.size() * sizeof(ApgWideNode)));
+    APG_HIP_CHECK(ctx, hipMalloc(&s.tris, tri9.size() * sizeof(float)));
+    APG_HIP_CHECK(ctx, hipMalloc(&s.tri_ids, ids.size() * sizeof(uint32_t)));
+    APG_HIP_CHECK(ctx, hipMemcpy(s.nodes, nodes.data(), nodes.size() * sizeof(ApgWideNode), hipMemcpyHostToDevice));
+    APG_HIP_CHECK(ctx, hipMemcpy(s.tris, tri9.data(), tri9.size() * sizeof(float), hipMemcpyHostToDevice));
+    APG_HIP_CHECK(ctx, hipMemcpy(s.tri_ids, ids.data(), ids.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return APG_OK;
+}
+
+} // namespace apg
+
+namespace {
+thread_local std::string g_create_error;
+}
+
+extern "C" {
+
+int apg_create(apg_context** out, int device)
+{
+    if (!out) return APG_E_INVALID;
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0)
+    {
+        g_create_error = "ERROR: GPU accelerator is not available (no HIP device)";
+        return APG_E_NO_DEVICE;
+    }
+    if (device < 0 || device >= count)
+    {
+        g_create_error = "ERROR: device index out of range";
+        return APG_E_INVALID;
+    }
+    e = hipSetDevice(device);
+    if (e != hipSuccess)
+    {
+        g_create_error = std::string("hipSetDevice: ") + hipGetErrorString(e);
+        return APG_E_HIP;
+    }
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess)
+    {
+        g_create_error = std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
+        return APG_E_HIP;
+    }
+    auto* ctx = new apg_context;
+    ctx->device = device;
+    ctx->cu_count = prop.multiProcessorCount;
+    apg_trace_config_default(ctx, &ctx->cfg);
+    *out = ctx;
+    return APG_OK;
+}
+
+int apg_destroy(apg_context* ctx)
+{
+    delete ctx;
+    return APG_OK;
+}
+
+const char* apg_last_error(const apg_context* ctx)
+{
+    return ctx ? ctx->last_error.c_str() : g_create_error.c_str();
+}
+
+int apg_mesh_append(apg_context* ctx, const float* xyz, uint32_t nv, const uint32_t* idx,
+                    uint32_t nt, uint32_t* shape_id)
+{
+    if (!ctx) return APG_E_INVALID;
+    if (shape_id) *shape_id = APG_INVALID_ID;
+    if ((nv && !xyz) || (nt && !idx)) return ctx->fail(APG_E_INVALID, "null geometry pointer");
+    if (uint64_t(ctx->total_tris) + nt > 0x7fffffffull)
+        return ctx->fail(APG_E_CAPACITY, "scene exceeds 2^31 - 1 triangles");
+    for (uint64_t i = 0; i < uint64_t(nt) * 3; ++i)
+        if (idx[i] >= nv) return ctx->fail(APG_E_INVALID, "triangle index out of range");
+    apg::Shape sh;
+    sh.xyz.assign(xyz, xyz + size_t(nv) * 3);
+    sh.idx.assign(idx, idx + size_t(nt) * 3);
+    sh.prim_offset = ctx->total_tris;
+    ctx->total_tris += nt;
+    ctx->shapes.push_back(std::move(sh));
+    if (shape_id) *shape_id = uint32_t(ctx->shapes.size() - 1);
+    return APG_OK;
+}
+
+int apg_mesh_vertices(apg_context* ctx, uint32_t shape_id, float** xyz, uint32_t* nv)
+{
+    if (!ctx || !xyz) return APG_E_INVALID;
+    if (shape_id >= ctx->shapes.size()) return ctx->fail(APG_E_INVALID, "unknown shape id");
+    *xyz = ctx->shapes[shape_id].xyz.data();
+    if (nv) *nv = uint32_t(ctx->shapes[shape_id].xyz.size() / 3);
+    return APG_OK;
+}
+
+int apg_mesh_update(apg_context* ctx, uint32_t shape_id)
+{
+    if (!ctx) return APG_E_INVALID;
+    if (shape_id >= ctx->shapes.size()) return ctx->fail(APG_E_INVALID, "unknown shape id");
+    // Triangle data is re-derived from the vertex copy at the next build.
+    return APG_OK;
+}
+
+int apg_mesh_prim_offset(const apg_context* ctx, uint32_t shape_id, uint32_t* offset)
+{
+    if (!ctx || !offset) return APG_E_INVALID;
+    if (shape_id >= ctx->shapes.size()) return APG_E_INVALID;
+    *offset = ctx->shapes[shape_id].prim_offset;
+    return APG_OK;
+}
+
+int apg_sphere_append(apg_context* ctx, const float*, uint32_t)
+{
+    if (!ctx) return APG_E_INVALID;
+    return ctx->fail(APG_E_INVALID, "ERROR: Sphere intersection has not been implemented yet.");
+}
+
+int apg_scene_build(apg_context* ctx, const apg_build_params* params)
+{
+    if (!ctx) return APG_E_INVALID;
+    APG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    auto t0 = std::chrono::steady_clock::now();
+    uint32_t leaf_threshold = params && params->leaf_threshold ? params->leaf_threshold : 4u;
+    uint32_t builder = params ? params->builder : APG_BUILDER_GPU;
+    if (leaf_threshold > 255) return ctx->fail(APG_E_INVALID, "leaf_threshold must be <= 255");
+
+    // Wait for kernels still reading the old scene.
+    if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+
+    std::vector<apg::TriPrep> tris;
+    apg::prepare_triangles(*ctx, tris);
+    apg::BinaryTree bt;
+    std::vector<ApgWideNode> nodes;
+    std::vector<float> tri9;
+    std::vector<uint32_t> ids;
+    int rc = APG_OK;
+    if (builder == APG_BUILDER_HOST)
+    {
+        apg::build_binary_host(tris, leaf_threshold, bt);
+        apg::collapse_to_wide(bt, tris, nodes, tri9, ids);
+        rc = apg::upload_scene(ctx, nodes, tri9, ids);
+    }
+    else if (builder == APG_BUILDER_GPU)
+    {
+        rc = apg::build_scene_gpu(ctx, tris, leaf_threshold, bt, nodes, tri9, ids);
+    }
+    else
+        return ctx->fail(APG_E_INVALID, "unknown builder");
+    if (rc) return rc;
+
+    ctx->leaf_keys = std::move(bt.leaf_keys);
+    apg_scene_stats& st = ctx->stats;
+    st = apg_scene_stats{};
+    st.triangles = ctx->total_tris;
+    st.shapes = uint32_t(ctx->shapes.size());
+    st.binary_nodes = bt.sah_nodes;
+    st.binary_leaves = bt.sah_leaves;
+    st.wide_nodes = ctx->scene.node_count;
+    st.max_depth_binary = bt.max_depth;
+    st.sah_cost = bt.sah_cost;
+    st.sah_cost_leaf = bt.sah_cost_leaf;
+    st.device_bytes = uint64_t(ctx->scene.node_count) * 64 + uint64_t(ctx->scene.tri_count) * 40;
+    st.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ctx->built = true;
+    return APG_OK;
+}
+
+int apg_scene_get_stats(const apg_context* ctx, apg_scene_stats* stats)
+{
+    if (!ctx || !stats) return APG_E_INVALID;
+    *stats = ctx->stats;
+    return APG_OK;
+}
+
+int apg_trace_config_default(const apg_context* ctx, apg_trace_config* cfg)
+{
+    if (!cfg) return APG_E_INVALID;
+    (void)ctx;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->block_size = 256;
+    cfg->blocks_per_cu = 8;
+    cfg->fetch_threshold = 16;
+    return APG_OK;
+}
+
+int apg_set_trace_config(apg_context* ctx, const apg_trace_config* cfg)
+{
+    if (!ctx || !cfg) return APG_E_INVALID;
+    if (cfg->block_size == 0 || cfg->block_size % 64 || cfg->block_size > 256)
+        return ctx->fail(APG_E_INVALID, "block_size must be 64, 128, 192 or 256");
+    if (cfg->blocks_per_cu == 0 || cfg->blocks_per_cu > 64)
+        return ctx->fail(APG_E_INVALID, "blocks_per_cu must be in 1..64");
+    if (cfg->fetch_threshold == 0 || cfg->fetch_threshold > 64)
+        return ctx->fail(APG_E_INVALID, "fetch_threshold must be in 1..64");
+    ctx->cfg = *cfg;
+    return APG_OK;
+}
+
+int apg_scene_triangles(const apg_context* ctx, float* tri9, uint32_t* ids, uint32_t n)
+{
+    if (!ctx || !tri9 || !ids) return APG_E_INVALID;
+    if (!ctx->built || n != ctx->scene.tri_count) return APG_E_STATE;
+    if (n == 0) return APG_OK;
+    if (hipMemcpy(tri9, ctx->scene.tris, size_t(n) * 9 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        return APG_E_HIP;
+    if (hipMemcpy(ids, ctx->scene.tri_ids, size_t(n) * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+        return APG_E_HIP;
+    return APG_OK;
+}
+
+int apg_scene_leaf_keys(const apg_context* ctx, uint32_t* keys, uint32_t n)
+{
+    if (!ctx || !keys) return APG_E_INVALID;
+    if (!ctx->built || n != ctx->leaf_keys.size()) return APG_E_STATE;
+    std::memcpy(keys, ctx->leaf_keys.data(), size_t(n) * sizeof(uint32_t));
+    return APG_OK;
+}
+
+int apg_scene_wide_nodes(const apg_context* ctx, void* nodes64, uint32_t n)
+{
+    if (!ctx || !nodes64) return APG_E_INVALID;
+    if (!ctx->built || n != ctx->scene.node_count) return APG_E_STATE;
+    if (n == 0) return APG_OK;
+    if (hipMemcpy(nodes64, ctx->scene.nodes, size_t(n) * 64, hipMemcpyDeviceToHost) != hipSuccess)
+        return APG_E_HIP;
+    return APG_OK;
+}
+
+} // extern "C"

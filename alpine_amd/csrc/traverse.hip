@@ -1,0 +1,342 @@
+// Persistent wide-BVH traversal for gfx950 (closest hit and any hit).
+//
+// One ray per lane.  Each wave keeps its lanes busy from a global ray cursor:
+// when enough lanes are idle the wave takes a contiguous batch with one
+// atomicAdd and hands the indices to its idle lanes in lane order (ballot +
+// popcount prefix), so short and long rays share a wave without waiting for
+// each other.
+//
+// Per lane the traversal state is a "node group" (child base, inner-slot mask,
+// still-to-visit hit mask in octant traversal order) and a "triangle group"
+// (first triangle, 32-bit hit mask), after the compressed wide BVH traversal
+// of Ylitie, Karras and Laine (HPG 2017).  Pending node groups go to a
+// per-lane stack whose top APG_STACK_LDS entries live in LDS (lane-major, so a
+// wave's accesses are bank-conflict free) and whose deeper entries spill to a
+// per-thread HBM region.
+//
+// The box test works on the quantised 64 B nodes (wide_node.h) with one FMA
+// per plane; the boxes carry a slack that covers that test's rounding.  The
+// triangle test is the reference's Moller-Trumbore (bvh_common.cpp:66-103)
+// with the same operation order and IEEE divisions; this file is compiled with
+// -ffp-contract=off so it rounds exactly as the reference does.
+#include "apg_internal.h"
+#include "traverse_core.h"
+
+#include <cfloat>
+
+namespace apg {
+
+constexpr uint32_t APG_STACK_LDS = 8;
+constexpr uint32_t APG_STACK_SPILL = 56;
+constexpr uint32_t APG_STACK_MAX = APG_STACK_LDS + APG_STACK_SPILL;
+
+struct TraceArgs
+{
+    const ApgWideNode* nodes;
+    const float* tris;
+    const uint32_t* tri_ids;
+    apg_rays rays;
+    uint32_t n;
+    uint32_t has_scene;
+    apg_hits hits;
+    uint8_t* occluded;
+    uint32_t* counters;               // [0] ray cursor, [1] stack overflow flag
+    uint64_t* spill;
+    unsigned long long* fetch_counts; // [0] nodes, [1] triangles
+    uint32_t fetch_threshold;
+};
+
+__global__ void reset_counters_kernel(uint32_t* counters)
+{
+    if (threadIdx.x < 2) counters[threadIdx.x] = 0u;
+}
+
+template <bool ANY, bool COUNT>
+__global__ __launch_bounds__(256) void trace_kernel(TraceArgs A)
+{
+    extern __shared__ uint32_t lds[];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = threadIdx.x >> 6;
+    uint32_t* st_base = lds + wave * (APG_STACK_LDS * 64u * 2u);
+    uint32_t* st_bits = st_base + APG_STACK_LDS * 64u;
+    uint64_t* spill = A.spill + size_t(blockIdx.x * blockDim.x + threadIdx.x) * APG_STACK_SPILL;
+
+    bool active = false;
+    bool exhausted = false;  // wave-uniform
+    uint32_t rid = 0;
+    float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0;
+    RayBox rb{};
+    float tmax = 0;
+    uint32_t best = 0xFFFFFFFFu;
+    float hb0 = 0.0f, hb1 = 0.0f;
+    uint32_t g_base = 0, g_bits = 0;
+    uint32_t t_base = 0, t_mask = 0;
+    uint32_t sp = 0;
+    bool found = false;
+    uint32_t n_node = 0, n_tri = 0;
+
+    for (;;)
+    {
+        // ---- ray fetch: idle lanes take the next indices in lane order ----
+        const uint64_t idle = __ballot(!active);
+        const uint32_t nidle = __popcll(idle);
+        if (!exhausted && (nidle >= A.fetch_threshold || nidle == 64u))
+        {
+            const uint32_t leader = __ffsll((unsigned long long)idle) - 1u;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&A.counters[0], nidle);
+            base = __shfl(base, leader);
+            if (base + nidle >= A.n) exhausted = true;
+            if (!active)
+            {
+                const uint32_t my = base + __popcll(idle & ((1ull << lane) - 1ull));
+                if (my < A.n)
+                {
+                    rid = my;
+                    active = true;
+                    ox = A.rays.ox[rid]; oy = A.rays.oy[rid]; oz = A.rays.oz[rid];
+                    dx = A.rays.dx[rid]; dy = A.rays.dy[rid]; dz = A.rays.dz[rid];
+                    tmax = A.rays.tmax ? A.rays.tmax[rid] : FLT_MAX;
+                    rb = make_raybox(ox, oy, oz, dx, dy, dz);
+                    best = 0xFFFFFFFFu;
+                    hb0 = hb1 = 0.0f;
+                    found = false;
+                    sp = 0;
+                    t_mask = 0;
+                    g_base = 0;
+                    // Root as a group holding one inner child in slot 0.
+                    g_bits = A.has_scene ? root_group_bits(rb.oct) : 0u;
+                }
+            }
+        }
+        if (__ballot(active) == 0ull)
+        {
+            if (exhausted) break;
+            continue;
+        }
+        if (!active) continue;
+
+        // ---- one inner node ----
+        if ((g_bits >> 8) != 0u)
+        {
+            uint32_t hits = 0;
+            const uint32_t node_idx = group_next_node(g_base, g_bits, rb.oct, hits);
+            if (hits)
+            {
+                // push the remaining siblings
+                if (sp >= APG_STACK_LDS)
+                {
+                    const uint32_t deep = sp - APG_STACK_LDS;
+                    if (deep >= APG_STACK_SPILL)
+                    {
+                        A.counters[1] = 1u;
+                        hits = 0;  // dropped: the overflow flag marks the results unreliable
+                    }
+                    else
+                    {
+                        const uint32_t ring = (sp % APG_STACK_LDS) * 64u + lane;
+                        spill[deep] = (uint64_t(st_bits[ring]) << 32) | st_base[ring];
+                    }
+                }
+                if (hits)
+                {
+                    const uint32_t ring = (sp % APG_STACK_LDS) * 64u + lane;
+                    st_base[ring] = g_base;
+                    st_bits[ring] = (hits << 8) | (g_bits & 0xffu);
+                    sp++;
+                }
+            }
+            const uint4* np = reinterpret_cast<const uint4*>(A.nodes + node_idx);
+            const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+            const uint32_t w[16] = { w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w,
+                                     w2.x, w2.y, w2.z, w2.w, w3.x, w3.y, w3.z, w3.w };
+            if (COUNT) n_node++;
+            node_visit(w, rb, tmax, g_base, g_bits, t_base, t_mask);
+        }
+
+        // ---- triangles of the current group ----
+        while (t_mask)
+        {
+            const uint32_t k = __ffs(t_mask) - 1u;
+            t_mask &= t_mask - 1u;
+            const uint32_t tri = t_base + k;
+            const float* tp = A.tris + size_t(tri) * APG_TRI_FLOATS;
+            if (COUNT) n_tri++;
+            float t, b0, b1;
+            if (!tri_hit(tp, ox, oy, oz, dx, dy, dz, tmax, t, b0, b1)) continue;
+            if (ANY)
+            {
+                found = true;
+                t_mask = 0;
+                g_bits = 0;
+                sp = 0;
+                break;
+            }
+            tmax = t;
+            best = tri;
+            hb0 = b0;
+            hb1 = b1;
+        }
+
+        // ---- next group ----
+        if ((g_bits >> 8) == 0u)
+        {
+            if (sp == 0u)
+            {
+                if (ANY)
+                    A.occluded[rid] = found ? 1u : 0u;
+                else
+                {
+                    const bool hit = best != 0xFFFFFFFFu;
+                    A.hits.prim[rid] = hit ? A.tri_ids[best] : 0xFFFFFFFFu;
+                    A.hits.t[rid] = hit ? tmax : FLT_MAX;
+                    A.hits.b0[rid] = hit ? hb0 : 0.0f;
+                    A.hits.b1[rid] = hit ? hb1 : 0.0f;
+                }
+                active = false;
+            }
+            else
+            {
+                sp--;
+                const uint32_t ring = (sp % APG_STACK_LDS) * 64u + lane;
+                g_base = st_base[ring];
+                g_bits = st_bits[ring];
+                if (sp >= APG_STACK_LDS)
+                {
+                    const uint64_t e = spill[sp - APG_STACK_LDS];
+                    st_base[ring] = uint32_t(e);
+                    st_bits[ring] = uint32_t(e >> 32);
+                }
+            }
+        }
+    }
+    if (COUNT)
+    {
+        atomicAdd(&A.fetch_counts[0], (unsigned long long)n_node);
+        atomicAdd(&A.fetch_counts[1], (unsigned long long)n_tri);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+static int ensure_trace_buffers(Context* ctx, uint32_t threads)
+{
+    if (!ctx->counters)
+    {
+        APG_HIP_CHECK(ctx, hipMalloc(&ctx->counters, 64));
+        APG_HIP_CHECK(ctx, hipMemset(ctx->counters, 0, 64));
+    }
+    if (ctx->spill_threads < threads)
+    {
+        if (ctx->spill) (void)hipFree(ctx->spill);
+        ctx->spill = nullptr;
+        APG_HIP_CHECK(ctx, hipMalloc(&ctx->spill, size_t(threads) * APG_STACK_SPILL * sizeof(uint64_t)));
+        ctx->spill_threads = threads;
+    }
+    if (!ctx->last_launch) APG_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->last_launch, hipEventDisableTiming));
+    return APG_OK;
+}
+
+int launch_trace(Context* ctx, const apg_rays* rays, uint32_t n, const apg_hits* hits, bool any,
+                 uint8_t* occ, unsigned long long* counts, hipStream_t stream)
+{
+    if (!rays || (!any && (!hits || !hits->prim || !hits->t || !hits->b0 || !hits->b1)) ||
+        (any && !occ))
+        return ctx->fail(APG_E_INVALID, "null ray or result pointer");
+    if (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz)
+        return ctx->fail(APG_E_INVALID, "null ray component pointer");
+    if (!ctx->built) return ctx->fail(APG_E_STATE, "apg_scene_build has not been called");
+    if (n == 0) return APG_OK;
+    if (n > 0x7fffffffu) return ctx->fail(APG_E_CAPACITY, "more than 2^31 - 1 rays in one call");
+
+    const apg_trace_config& cfg = ctx->cfg;
+    const uint32_t block = cfg.block_size;
+    uint32_t grid = uint32_t(ctx->cu_count) * cfg.blocks_per_cu;
+    const uint32_t need = (n + block - 1) / block;
+    if (grid > need) grid = need;
+    if (grid == 0) grid = 1;
+
+    std::lock_guard<std::mutex> lock(ctx->launch_mutex);
+    int rc = ensure_trace_buffers(ctx, grid * block);
+    if (rc) return rc;
+
+    TraceArgs A;
+    A.nodes = ctx->scene.nodes;
+    A.tris = ctx->scene.tris;
+    A.tri_ids = ctx->scene.tri_ids;
+    A.rays = *rays;
+    A.n = n;
+    A.has_scene = ctx->scene.node_count > 0 ? 1u : 0u;
+    if (hits) A.hits = *hits; else A.hits = apg_hits{ nullptr, nullptr, nullptr, nullptr };
+    A.occluded = occ;
+    A.counters = ctx->counters;
+    A.spill = ctx->spill;
+    A.fetch_counts = counts;
+    A.fetch_threshold = cfg.fetch_threshold;
+
+    // Kernels from all streams share the ray cursor and the spill area:
+    // order this launch after the previous one.
+    APG_HIP_CHECK(ctx, hipStreamWaitEvent(stream, ctx->last_launch, 0));
+    reset_counters_kernel<<<1, 64, 0, stream>>>(ctx->counters);
+    const size_t shmem = size_t(block / 64) * APG_STACK_LDS * 64 * 2 * sizeof(uint32_t);
+    if (any)
+    {
+        if (counts) trace_kernel<true, true><<<grid, block, shmem, stream>>>(A);
+        else trace_kernel<true, false><<<grid, block, shmem, stream>>>(A);
+    }
+    else
+    {
+        if (counts) trace_kernel<false, true><<<grid, block, shmem, stream>>>(A);
+        else trace_kernel<false, false><<<grid, block, shmem, stream>>>(A);
+    }
+    APG_HIP_CHECK(ctx, hipGetLastError());
+    APG_HIP_CHECK(ctx, hipEventRecord(ctx->last_launch, stream));
+    return APG_OK;
+}
+
+} // namespace apg
+
+extern "C" {
+
+int apg_intersect(apg_context* ctx, const apg_rays* rays, uint32_t n, const apg_hits* hits,
+                  void* stream)
+{
+    if (!ctx) return APG_E_INVALID;
+    return apg::launch_trace(ctx, rays, n, hits, false, nullptr, nullptr,
+                             static_cast<hipStream_t>(stream));
+}
+
+int apg_occluded(apg_context* ctx, const apg_rays* rays, uint32_t n, uint8_t* occluded,
+                 void* stream)
+{
+    if (!ctx) return APG_E_INVALID;
+    return apg::launch_trace(ctx, rays, n, nullptr, true, occluded, nullptr,
+                             static_cast<hipStream_t>(stream));
+}
+
+int apg_intersect_counted(apg_context* ctx, const apg_rays* rays, uint32_t n,
+                          const apg_hits* hits, int any, uint8_t* occluded,
+                          uint64_t* node_fetches, uint64_t* tri_fetches, void* stream)
+{
+    if (!ctx || !node_fetches || !tri_fetches) return APG_E_INVALID;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    unsigned long long* counts = nullptr;
+    APG_HIP_CHECK(ctx, hipMalloc(&counts, 2 * sizeof(unsigned long long)));
+    APG_HIP_CHECK(ctx, hipMemsetAsync(counts, 0, 2 * sizeof(unsigned long long), s));
+    int rc = apg::launch_trace(ctx, rays, n, hits, any != 0, occluded, counts, s);
+    unsigned long long host[2] = { 0, 0 };
+    if (rc == APG_OK)
+    {
+        hipError_t e = hipMemcpyAsync(host, counts, sizeof(host), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = ctx->fail(APG_E_HIP, hipGetErrorString(e));
+    }
+    (void)hipFree(counts);
+    *node_fetches = host[0];
+    *tri_fetches = host[1];
+    return rc;
+}
+
+} // extern "C"

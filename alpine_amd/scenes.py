@@ -1,0 +1,185 @@
+"""Procedural scenes and ray sets for the traversal microbench and the parity tests.
+
+Every generator is deterministic (numpy PCG64 with a fixed seed), so the same
+arrays are produced on any host.  Geometry is returned as (verts float32
+(nv, 3), tris uint32 (nt, 3)); ray sets as (org float32 (n, 3), dir float32
+(n, 3)).
+
+Workloads
+---------
+* ``displaced_icosphere(subdiv=8, seed=7)``: icosphere (20 * 4**subdiv
+  triangles) whose vertices are pushed along the normal to radius
+  1 + 0.1 * value_noise(p), value noise in [-1, 1] on an integer lattice of
+  frequency 4 (smoothstep-blended trilinear interpolation of hashed lattice
+  values).
+* ``random_triangles(n=200_000, seed=7)``: equilateral triangles, centres
+  uniform in [-3, 3]^3, edge length log-uniform in [0.005, 0.1], uniformly
+  random orientation.
+* ``microbench_scene()``: the two above concatenated (1,510,720 triangles at
+  the default sizes).
+* ``incoherent_rays(n, lo, hi, seed)``: origins uniform in the box, directions
+  uniform on the sphere (z = 2u - 1, phi = 2 pi v).
+* ``coherent_rays(w, h, ...)``: pinhole camera rays through pixel centres.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_PHI = (1.0 + 5.0 ** 0.5) / 2.0
+
+
+def icosahedron():
+    v = np.array([
+        [-1, _PHI, 0], [1, _PHI, 0], [-1, -_PHI, 0], [1, -_PHI, 0],
+        [0, -1, _PHI], [0, 1, _PHI], [0, -1, -_PHI], [0, 1, -_PHI],
+        [_PHI, 0, -1], [_PHI, 0, 1], [-_PHI, 0, -1], [-_PHI, 0, 1],
+    ], dtype=np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([
+        [0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11],
+        [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8],
+        [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+        [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1],
+    ], dtype=np.int64)
+    return v, f
+
+
+def subdivide(v, f):
+    """Split every triangle into four; edge midpoints are projected to the unit sphere."""
+    nv = len(v)
+    e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]], axis=0)
+    e.sort(axis=1)
+    key = e[:, 0] * nv + e[:, 1]
+    uniq, inv = np.unique(key, return_inverse=True)
+    a, b = uniq // nv, uniq % nv
+    mid = v[a] + v[b]
+    mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+    m = nv + inv.reshape(3, -1).T  # (nf, 3): midpoints of edges 01, 12, 20
+    v2 = np.concatenate([v, mid], axis=0)
+    f0, f1, f2 = f[:, 0], f[:, 1], f[:, 2]
+    m01, m12, m20 = m[:, 0], m[:, 1], m[:, 2]
+    f2_ = np.concatenate([
+        np.stack([f0, m01, m20], 1), np.stack([m01, f1, m12], 1),
+        np.stack([m20, m12, f2], 1), np.stack([m01, m12, m20], 1),
+    ], axis=0)
+    return v2, f2_
+
+
+def _hash3(ix, iy, iz, seed):
+    h = (ix.astype(np.uint64) * np.uint64(0x9E3779B185EBCA87)
+         ^ iy.astype(np.uint64) * np.uint64(0xC2B2AE3D27D4EB4F)
+         ^ iz.astype(np.uint64) * np.uint64(0x165667B19E3779F9)
+         ^ np.uint64(seed) * np.uint64(0x27D4EB2F165667C5))
+    h ^= h >> np.uint64(33)
+    h *= np.uint64(0xFF51AFD7ED558CCD)
+    h ^= h >> np.uint64(33)
+    h *= np.uint64(0xC4CEB9FE1A85EC53)
+    h ^= h >> np.uint64(33)
+    return (h >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+
+
+def value_noise(p, seed=7, freq=4.0):
+    """Trilinear value noise in [-1, 1] with smoothstep weights."""
+    q = p * freq
+    i0 = np.floor(q).astype(np.int64)
+    fr = q - i0
+    w = fr * fr * (3.0 - 2.0 * fr)
+    out = np.zeros(len(p))
+    with np.errstate(over="ignore"):
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    val = _hash3(i0[:, 0] + dx, i0[:, 1] + dy, i0[:, 2] + dz, seed)
+                    wx = w[:, 0] if dx else 1.0 - w[:, 0]
+                    wy = w[:, 1] if dy else 1.0 - w[:, 1]
+                    wz = w[:, 2] if dz else 1.0 - w[:, 2]
+                    out += wx * wy * wz * val
+    return out * 2.0 - 1.0
+
+
+def displaced_icosphere(subdiv=8, seed=7, amp=0.1):
+    v, f = icosahedron()
+    for _ in range(subdiv):
+        v, f = subdivide(v, f)
+    r = 1.0 + amp * value_noise(v, seed)
+    v = v * r[:, None]
+    return v.astype(np.float32), f.astype(np.uint32)
+
+
+def _random_rotations(rng, n):
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1),
+    ], axis=1)
+
+
+def random_triangles(n=200_000, seed=7, extent=3.0, lmin=0.005, lmax=0.1):
+    rng = np.random.default_rng(seed + 1000)
+    c = rng.uniform(-extent, extent, (n, 3))
+    L = np.exp(rng.uniform(np.log(lmin), np.log(lmax), n))
+    R = _random_rotations(rng, n)
+    ang = np.array([0.0, 2.0 * np.pi / 3.0, 4.0 * np.pi / 3.0])
+    local = np.stack([np.cos(ang), np.sin(ang), np.zeros(3)], -1) / np.sqrt(3.0)  # unit side
+    pts = c[:, None, :] + L[:, None, None] * np.einsum("nij,kj->nki", R, local)
+    verts = pts.reshape(-1, 3).astype(np.float32)
+    tris = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    return verts, tris
+
+
+def concat(*meshes):
+    vs, ts, off = [], [], 0
+    for v, t in meshes:
+        vs.append(v)
+        ts.append(t + np.uint32(off))
+        off += len(v)
+    return np.concatenate(vs).astype(np.float32), np.concatenate(ts).astype(np.uint32)
+
+
+def microbench_scene(subdiv=8, n_random=200_000, seed=7):
+    return concat(displaced_icosphere(subdiv, seed), random_triangles(n_random, seed))
+
+
+def scene_bounds(verts):
+    return verts.min(axis=0), verts.max(axis=0)
+
+
+def incoherent_rays(n, lo, hi, seed=8):
+    rng = np.random.default_rng(seed)
+    org = rng.uniform(np.asarray(lo, np.float64), np.asarray(hi, np.float64), (n, 3))
+    u, v = rng.random(n), rng.random(n)
+    z = 2.0 * u - 1.0
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    phi = 2.0 * np.pi * v
+    d = np.stack([r * np.cos(phi), r * np.sin(phi), z], -1)
+    return org.astype(np.float32), d.astype(np.float32)
+
+
+def coherent_rays(width=2048, height=2048, eye=(0.0, 0.0, -6.5), target=(0.0, 0.0, 0.0),
+                  up=(0.0, 1.0, 0.0), fovy_deg=50.0):
+    eye = np.asarray(eye, np.float64)
+    fwd = np.asarray(target, np.float64) - eye
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(np.asarray(up, np.float64), fwd)
+    right /= np.linalg.norm(right)
+    upv = np.cross(fwd, right)
+    h = np.tan(np.radians(fovy_deg) * 0.5)
+    w = h * width / height
+    xs = ((np.arange(width) + 0.5) / width - 0.5) * 2.0 * w
+    ys = (0.5 - (np.arange(height) + 0.5) / height) * 2.0 * h
+    X, Y = np.meshgrid(xs, ys)
+    d = X[..., None] * right + Y[..., None] * upv + fwd
+    d = d.reshape(-1, 3)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    org = np.broadcast_to(eye, d.shape)
+    return np.ascontiguousarray(org, np.float32), d.astype(np.float32)
+
+
+def small_scene(n_random=2000, subdiv=3, seed=7):
+    """A few thousand triangles with the microbench's structure, for brute-force checks."""
+    return concat(displaced_icosphere(subdiv, seed), random_triangles(n_random, seed,
+                                                                      extent=1.5, lmin=0.02,
+                                                                      lmax=0.4))

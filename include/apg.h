@@ -1,0 +1,142 @@
+/*
+ * apg.h -- C ABI of the MI355X (gfx950) ray-tracing back end for alpine.
+ *
+ * This is the drop-in boundary for alpine's two plug-in interfaces
+ *   Accelerator   (reference: src/accelerators/accelerator.h:13-39)
+ *   LightSampler  (reference: src/light_samplers/light_sampler.h:12-25)
+ * Every entry point below names the reference member it replaces.  The per-ray
+ * reference calls (intersect(const Ray&), intersectAny(const Ray&, float)) map
+ * to the batched calls with n = 1; the render path submits whole queues.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only; no C++ or torch types.
+ *  - Every call returns an int status: APG_OK (0) or a negative APG_E_* code.
+ *    apg_last_error(ctx) returns a message for the most recent failure on the
+ *    context (apg_last_error(NULL): the most recent failure of apg_create).
+ *  - "device" pointers are HBM allocations on the context's GPU; "host"
+ *    pointers are ordinary process memory.  `stream` is a hipStream_t (NULL =
+ *    the default stream).  Batched calls are asynchronous on `stream`.
+ *  - Calls that launch kernels may be issued from several host threads;
+ *    the library serialises their kernels on the GPU in call order.
+ *  - Triangle indices in results are *global*: meshes are numbered in append
+ *    order and their triangles concatenated (global = prim_offset(shape) +
+ *    mesh-local primId, the reference's Bvh layout, bvh.cpp:83-106).
+ */
+#ifndef APG_H
+#define APG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define APG_OK 0
+#define APG_E_INVALID (-1)     /* bad argument */
+#define APG_E_NO_DEVICE (-2)   /* no usable GPU: behaves like wide_bvh_dummy.cpp */
+#define APG_E_HIP (-3)         /* HIP runtime error */
+#define APG_E_STATE (-4)       /* call out of order (e.g. intersect before build) */
+#define APG_E_CAPACITY (-5)    /* size beyond a documented limit */
+#define APG_E_OVERFLOW (-6)    /* traversal stack overflow reported by a kernel */
+
+#define APG_INVALID_ID 0xFFFFFFFFu /* Accelerator::INVALID_SHAPE_ID, accelerator.h:38 */
+
+typedef struct apg_context apg_context;
+
+/* Rays in HBM, structure of arrays, n entries each (28 B per ray).
+ * tmax may be NULL: closest-hit then uses FLT_MAX (bvh.cpp:230-233). */
+typedef struct apg_rays {
+    const float* ox; const float* oy; const float* oz;
+    const float* dx; const float* dy; const float* dz;
+    const float* tmax;
+} apg_rays;
+
+/* Closest-hit results in HBM (16 B per ray): prim = APG_INVALID_ID on a miss
+ * (t = FLT_MAX, b0 = b1 = 0), otherwise the global triangle index, the hit
+ * distance and the barycentrics weighting vertices 1 and 2 (the reference
+ * Intersection fields primId, t, barycentric; intersection.h:8-16). */
+typedef struct apg_hits {
+    uint32_t* prim; float* t; float* b0; float* b1;
+} apg_hits;
+
+/* Which builder and which tree shape scene_build produces. */
+typedef struct apg_build_params {
+    uint32_t leaf_threshold;  /* binned-SAH leaf threshold (reference Bvh: 8); 0 = default 4 */
+    uint32_t builder;         /* APG_BUILDER_GPU (default) or APG_BUILDER_HOST */
+    uint32_t reserved[6];
+} apg_build_params;
+#define APG_BUILDER_GPU 0u
+#define APG_BUILDER_HOST 1u
+
+typedef struct apg_scene_stats {
+    uint32_t triangles;
+    uint32_t shapes;
+    uint32_t binary_nodes;     /* nodes of the binary binned-SAH tree */
+    uint32_t binary_leaves;
+    uint32_t wide_nodes;       /* 64 B quantised 8-wide nodes */
+    uint32_t max_depth_binary;
+    double sah_cost;           /* binary tree, C_trav = C_isect = 1, / SA(root) */
+    double sah_cost_leaf;      /* binary tree, leaf term only */
+    double build_ms;           /* last scene_build, wall clock */
+    uint64_t device_bytes;     /* nodes + triangles + index maps */
+} apg_scene_stats;
+
+/* Traversal launch shape (filled with defaults by apg_trace_config_default;
+ * the autotuner in alpine_amd/autotune.py writes these). */
+typedef struct apg_trace_config {
+    uint32_t block_size;       /* threads per workgroup (multiple of 64) */
+    uint32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu */
+    uint32_t fetch_threshold;  /* idle lanes a wave waits for before fetching rays */
+    uint32_t reserved[5];
+} apg_trace_config;
+
+/* ---- lifetime (Alpine::resetScene creates the accelerator, alpine.cpp:158-176) ---- */
+int apg_create(apg_context** ctx, int device);
+int apg_destroy(apg_context* ctx);
+const char* apg_last_error(const apg_context* ctx);
+
+/* ---- geometry (Accelerator::appendMesh / getVertexBuffer / updateShape / updateScene,
+ *      accelerator.h:19-31) ---- */
+/* Copies nv vertices (xyz, host) and nt triangles (3 x uint32, host). */
+int apg_mesh_append(apg_context* ctx, const float* xyz, uint32_t nv, const uint32_t* idx,
+                    uint32_t nt, uint32_t* shape_id);
+/* Mutable host view of the library's copy of the shape's vertices
+ * (Accelerator::getVertexBuffer).  Write it, then call apg_mesh_update. */
+int apg_mesh_vertices(apg_context* ctx, uint32_t shape_id, float** xyz, uint32_t* nv);
+int apg_mesh_update(apg_context* ctx, uint32_t shape_id);           /* updateShape */
+int apg_mesh_prim_offset(const apg_context* ctx, uint32_t shape_id, uint32_t* offset);
+/* Appending a sphere is rejected like the reference Bvh (bvh.cpp:108-112). */
+int apg_sphere_append(apg_context* ctx, const float* xyzr, uint32_t n);
+int apg_scene_build(apg_context* ctx, const apg_build_params* params); /* updateScene */
+int apg_scene_get_stats(const apg_context* ctx, apg_scene_stats* stats);
+
+/* ---- traversal (Accelerator::intersect / intersectAny, accelerator.h:33-35) ---- */
+int apg_trace_config_default(const apg_context* ctx, apg_trace_config* cfg);
+int apg_set_trace_config(apg_context* ctx, const apg_trace_config* cfg);
+/* Closest hit for n rays (device SoA in, device SoA out). */
+int apg_intersect(apg_context* ctx, const apg_rays* rays, uint32_t n, const apg_hits* hits,
+                  void* stream);
+/* Any hit with t < tmax[i] (FLT_MAX when tmax is NULL); occluded[i] = 0 or 1. */
+int apg_occluded(apg_context* ctx, const apg_rays* rays, uint32_t n, uint8_t* occluded,
+                 void* stream);
+/* Instrumented closest/any-hit run: same results, plus total 64 B node
+ * fetches and 36 B triangle fetches over all rays (host uint64 out).  Used for
+ * the byte model of the roofline. */
+int apg_intersect_counted(apg_context* ctx, const apg_rays* rays, uint32_t n,
+                          const apg_hits* hits, int any, uint8_t* occluded,
+                          uint64_t* node_fetches, uint64_t* tri_fetches, void* stream);
+/* The triangle data the kernels read, in tree order: 9 floats per triangle
+ * (v0, e0 = v1 - v0, e1 = v2 - v0; bvh_common.cpp:105-125) and the global
+ * index of each.  Host copies out, for tests. */
+int apg_scene_triangles(const apg_context* ctx, float* tri9, uint32_t* ids, uint32_t n);
+/* The binary tree's leaf sets: for each global triangle, the smallest global
+ * triangle index in its leaf (canonical form; host out). */
+int apg_scene_leaf_keys(const apg_context* ctx, uint32_t* keys, uint32_t n);
+/* Raw 64 B wide nodes (host out), for structural tests. */
+int apg_scene_wide_nodes(const apg_context* ctx, void* nodes64, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* APG_H */

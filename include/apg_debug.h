@@ -1,0 +1,38 @@
+/*
+ * apg_debug.h -- test hooks of libapg.so (not part of the drop-in boundary).
+ *
+ * They run the host build of the library's own traversal steps
+ * (alpine_amd/csrc/traverse_core.h) over a host-built scene, so CPU-only tests
+ * can check the wide-node layout and the traversal logic against the oracle
+ * without a GPU.  The product path (apg.h) never calls them.
+ */
+#ifndef APG_DEBUG_H
+#define APG_DEBUG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct apg_debug_scene apg_debug_scene;
+
+int apg_debug_host_build(const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                         uint32_t leaf_threshold, apg_debug_scene** out);
+int apg_debug_scene_info(const apg_debug_scene* s, uint32_t* n_nodes, uint32_t* n_tris,
+                         double* sah_cost, double* sah_cost_leaf, uint32_t* binary_nodes,
+                         uint32_t* max_depth);
+int apg_debug_scene_copy(const apg_debug_scene* s, void* nodes64, float* tri9, uint32_t* ids,
+                         uint32_t* leaf_keys);
+/* org3/dir3: n x 3 floats (host).  tmax may be NULL (FLT_MAX).  max_stack
+ * (optional) receives the deepest stack the traversals used. */
+int apg_debug_host_trace(const apg_debug_scene* s, const float* org3, const float* dir3,
+                         const float* tmax, uint32_t n, int any, uint32_t* prim, float* t,
+                         float* b0, float* b1, uint8_t* occluded, uint32_t* max_stack);
+void apg_debug_scene_free(apg_debug_scene* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,100 @@
+/*
+ * ORACLE -- test infrastructure only.
+ *
+ * A clean-room CPU restatement, in plain C, of the reference's binary BVH
+ * (alpine src/accelerators/bvh.cpp, bvh_common.cpp, utils/bounding_box.h,
+ * utils/bvh_util.h, math/vector.h).  It is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product library (alpine_amd/libapg.so) never links it.
+ *
+ * Build: `make -C oracle` -> oracle/liboracle.so, compiled with
+ * -O2 -ffp-contract=off and no -march, so float expressions round exactly as
+ * the reference's default x86-64 build does (no FMA contraction).
+ *
+ * Pinning (see tests/test_oracle.py):
+ *   - closest/any hit, t, barycentrics and tree leaf sets against golden
+ *     vectors written by the reference's own bvh.cpp/bvh_common.cpp compiled
+ *     here (oracle/_ref, tests/golden/gen_golden.py);
+ *   - brute-force O(N) intersection (tree-independent ground truth);
+ *   - dyadic Moller-Trumbore known answers; SAH brute force on small nodes;
+ *   - tree invariants (each primitive in exactly one leaf, child boxes inside
+ *     parents, leaf sizes).
+ */
+#ifndef ALPINE_ORACLE_H
+#define ALPINE_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_bvh oracle_bvh;
+
+/* Build the reference binary BVH over nt triangles (global indices 0..nt-1,
+ * meshes concatenated in append order).  leaf_threshold: 8 for Bvh
+ * (bvh.cpp:14), 16 for WideBvh (wide_bvh.cpp:21). */
+oracle_bvh* oracle_bvh_create(const float* xyz, uint32_t nv, const uint32_t* idx,
+                              uint32_t nt, uint32_t leaf_threshold);
+void oracle_bvh_destroy(oracle_bvh* b);
+
+/* Closest hit (bvh.cpp:230-233, tFar = FLT_MAX).  org/dir are n x 3 floats.
+ * prim = 0xFFFFFFFF on miss (t = FLT_MAX, b0 = b1 = 0). */
+void oracle_bvh_intersect(const oracle_bvh* b, const float* org, const float* dir,
+                          uint32_t n, uint32_t* prim, float* t, float* b0, float* b1,
+                          uint32_t nthreads);
+/* Any hit (bvh.cpp:236-241). */
+void oracle_bvh_occluded(const oracle_bvh* b, const float* org, const float* dir,
+                         const float* tfar, uint32_t n, uint8_t* occluded,
+                         uint32_t nthreads);
+
+/* Brute force: every triangle tested in index order with the same
+ * Primitive::intersect; ties keep the first index. */
+void oracle_brute_intersect(const oracle_bvh* b, const float* org, const float* dir,
+                            uint32_t n, uint32_t* prim, float* t, float* b0, float* b1);
+void oracle_brute_occluded(const oracle_bvh* b, const float* org, const float* dir,
+                           const float* tfar, uint32_t n, uint8_t* occluded);
+
+/* One Primitive::intersect (bvh_common.cpp:66-103) of triangle `tri`.
+ * Returns 1 on hit and writes t, b0, b1. */
+int oracle_triangle_intersect(const oracle_bvh* b, uint32_t tri, const float* org,
+                              const float* dir, float* t, float* b0, float* b1);
+
+typedef struct oracle_bvh_stats {
+    uint32_t node_count;     /* flattened LinearNode count */
+    uint32_t leaf_count;
+    uint32_t max_depth;      /* root depth 0 */
+    uint32_t max_leaf_size;
+    double sah_cost;         /* sum_inner SA*1 + sum_leaf SA*N, divided by SA(root) */
+    double sah_cost_leaf;    /* sum_leaf SA*N / SA(root) (the builder's own C_trav=0 model) */
+} oracle_bvh_stats;
+void oracle_bvh_get_stats(const oracle_bvh* b, oracle_bvh_stats* s);
+
+/* Leaf sets in canonical form: for every triangle, the smallest triangle
+ * index in its leaf.  Two trees have the same leaf sets iff these arrays
+ * are equal. */
+void oracle_bvh_leaf_keys(const oracle_bvh* b, uint32_t* key);
+
+/* Triangle record order the flattened tree uses (ordered primitive index ->
+ * original index), and the flattened nodes (bbox min/max, offset, count,
+ * dim), for structural tests. */
+void oracle_bvh_ordered_prims(const oracle_bvh* b, uint32_t* order);
+void oracle_bvh_nodes(const oracle_bvh* b, float* bbox6, uint32_t* offset,
+                      uint16_t* count, uint8_t* dim);
+
+/* findSplit (bvh_common.cpp:185-272) on an explicit list of boxes (n x 6:
+ * min xyz, max xyz).  Returns 1 and the split when one exists. */
+int oracle_find_split(const float* boxes6, uint32_t n, uint32_t* dim, uint32_t* split_idx,
+                      float* cost);
+
+/* Surface area (bounding_box.h:17-21). */
+float oracle_surface_area(const float* box6);
+
+/* Slab test (bounding_box.h:23-56). */
+int oracle_box_intersect(const float* box6, const float* org, const float* dir, float tfar);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

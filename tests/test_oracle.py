@@ -1,0 +1,135 @@
+"""Pins of the CPU oracle (oracle/bvh.c) -- golden vectors, brute force, closed forms."""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+
+U32_MISS = 0xFFFFFFFF
+
+
+def test_oracle_matches_reference_golden(golden):
+    """Closest/any hit, t and barycentrics equal the reference's compiled Bvh bit for bit."""
+    b = oracle.OracleBvh(golden["verts"], golden["tris"], 8)
+    h = b.intersect(golden["org"], golden["dir"])
+    np.testing.assert_array_equal(h.prim, golden["prim"])
+    np.testing.assert_array_equal(h.t.view(np.uint32), golden["t"].view(np.uint32))
+    np.testing.assert_array_equal(h.b0.view(np.uint32), golden["b0"].view(np.uint32))
+    np.testing.assert_array_equal(h.b1.view(np.uint32), golden["b1"].view(np.uint32))
+    np.testing.assert_array_equal(b.occluded(golden["org"], golden["dir"]), golden["occ"])
+    np.testing.assert_array_equal(b.occluded(golden["org"], golden["dir"], golden["tfar"]),
+                                  golden["occ_tfar"])
+
+
+@pytest.mark.parametrize("lt,key,sah,nodes", [(8, "leaf_keys8", "sah8", "nodes8"),
+                                             (16, "leaf_keys16", "sah16", "nodes16")])
+def test_oracle_tree_matches_reference_builder(golden, lt, key, sah, nodes):
+    """Same leaf sets, SAH cost and node count as the reference buildBvh (9 bins)."""
+    b = oracle.OracleBvh(golden["verts"], golden["tris"], lt)
+    np.testing.assert_array_equal(b.leaf_keys(), golden[key])
+    st = b.stats()
+    assert st["sah_cost"] == pytest.approx(float(golden[sah]), rel=1e-12)
+    assert st["node_count"] == int(golden[nodes])
+
+
+def test_oracle_equals_brute_force(small_scene):
+    """Tree traversal returns the brute-force closest hit (ties: same t)."""
+    from alpine_amd import scenes
+    v, f = small_scene
+    lo, hi = scenes.scene_bounds(v)
+    o, d = scenes.incoherent_rays(3000, lo, hi, seed=21)
+    b = oracle.OracleBvh(v, f, 8)
+    h = b.intersect(o, d)
+    bf = b.brute_intersect(o, d)
+    np.testing.assert_array_equal(h.t.view(np.uint32), bf.t.view(np.uint32))
+    same = h.prim == bf.prim
+    # a differing id must be an exact-t tie
+    assert np.all(h.t[~same] == bf.t[~same])
+    assert same.mean() > 0.999
+    np.testing.assert_array_equal(b.occluded(o, d), b.brute_occluded(o, d))
+
+
+def test_moller_trumbore_dyadic_known_answers():
+    """Axis-aligned right triangle, dyadic rays: t, b0, b1 are exact closed forms."""
+    verts = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    b = oracle.OracleBvh(verts, np.array([[0, 1, 2]], np.uint32), 8)
+    for x, y, z in [(0.25, 0.5, 2.0), (0.125, 0.125, -4.0), (0.5, 0.25, 1.0)]:
+        d = np.array([0, 0, -1.0 if z > 0 else 1.0], np.float32)
+        hit, t, b0, b1 = b.triangle(0, np.array([x, y, z], np.float32), d)
+        assert hit and t == abs(z) and b0 == x and b1 == y
+    # outside (b0 + b1 > 1), parallel (det = 0) and behind (t < 0)
+    assert not b.triangle(0, np.float32([0.75, 0.5, 1]), np.float32([0, 0, -1]))[0]
+    assert not b.triangle(0, np.float32([0.2, 0.2, 1]), np.float32([1, 0, 0]))[0]
+    assert not b.triangle(0, np.float32([0.2, 0.2, 1]), np.float32([0, 0, 1]))[0]
+    # edge: b0 = 0 and b0 + b1 = 1 are inside (closed intervals)
+    assert b.triangle(0, np.float32([0.0, 0.5, 1]), np.float32([0, 0, -1]))[0]
+    assert b.triangle(0, np.float32([0.5, 0.5, 1]), np.float32([0, 0, -1]))[0]
+
+
+def test_det_epsilon_rejects_tiny_triangles():
+    """|det| < FLT_EPSILON is rejected even for a head-on ray (absolute threshold)."""
+    s = np.float32(1e-4)
+    verts = np.array([[0, 0, 0], [s, 0, 0], [0, s, 0]], np.float32)
+    b = oracle.OracleBvh(verts, np.array([[0, 1, 2]], np.uint32), 8)
+    assert not b.triangle(0, np.float32([2e-5, 2e-5, 1]), np.float32([0, 0, -1]))[0]
+
+
+def test_surface_area_and_slab_closed_forms():
+    assert oracle.surface_area([0, 0, 0, 1, 2, 3]) == 2 * (2 + 6 + 3)
+    box = [0, 0, 0, 1, 1, 1]
+    assert oracle.box_intersect(box, [0.5, 0.5, -1], [0, 0, 1], 10.0)
+    assert not oracle.box_intersect(box, [0.5, 0.5, -1], [0, 0, -1], 10.0)
+    assert not oracle.box_intersect(box, [0.5, 0.5, -1], [0, 0, 1], 0.5)  # tMin = 1 >= tFar
+    assert not oracle.box_intersect(box, [2, 0.5, 0.5], [0, 1, 0], 10.0)
+
+
+def _brute_split(boxes):
+    """Enumerate the 9-bin split cost directly from the definition."""
+    c = (boxes[:, :3] + boxes[:, 3:]) * 0.5
+    cmin, cmax = c.min(0), c.max(0)
+    best = (np.inf, None, None)
+    for d in range(3):
+        if cmin[d] == cmax[d]:
+            continue
+        bins = np.minimum((9 * (c[:, d] - cmin[d]) / (cmax[d] - cmin[d])).astype(int), 8)
+        for s in range(8):
+            lo, hi = boxes[bins <= s], boxes[bins > s]
+            cost = 0.0
+            for part in (lo, hi):
+                if len(part):
+                    ext = part[:, 3:].max(0) - part[:, :3].min(0)
+                    cost += len(part) * 2 * (ext[0] * ext[1] + ext[1] * ext[2] + ext[2] * ext[0])
+            if cost < best[0] - 1e-9 * max(1.0, cost):
+                best = (cost, d, s)
+    return best
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_find_split_equals_enumeration(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(9, 64))
+    lo = rng.uniform(-1, 1, (n, 3))
+    boxes = np.concatenate([lo, lo + rng.uniform(0, 0.3, (n, 3))], 1).astype(np.float32)
+    ok, d, s, cost = oracle.find_split(boxes)
+    bc, bd, bs = _brute_split(boxes.astype(np.float64))
+    assert ok and (d, s) == (bd, bs)
+    assert cost == pytest.approx(bc, rel=1e-5)
+
+
+def test_tree_invariants(small_scene):
+    v, f = small_scene
+    b = oracle.OracleBvh(v, f, 8)
+    bbox, off, cnt, dim = b.nodes()
+    leaves = cnt > 0
+    # every triangle in exactly one leaf
+    covered = np.zeros(len(f), int)
+    order = b.ordered_prims()
+    for o, c in zip(off[leaves], cnt[leaves]):
+        covered[order[o:o + c]] += 1
+    assert np.all(covered == 1)
+    assert cnt.max() <= 8
+    # children inside parents (depth-first layout: left = i + 1, right = offset)
+    for i in np.where(~leaves)[0]:
+        for ch in (i + 1, off[i]):
+            assert np.all(bbox[ch, :3] >= bbox[i, :3]) and np.all(bbox[ch, 3:] <= bbox[i, 3:])
