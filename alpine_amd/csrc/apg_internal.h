@@ -88,6 +88,7 @@ struct BinaryTree
     uint32_t sah_nodes = 0;
     uint32_t max_depth = 0;
     double sah_cost = 0.0, sah_cost_leaf = 0.0;
+    double build_ms = 0.0;  // device build time (GPU builder), excluding statistics
     std::vector<uint32_t> leaf_keys;
 };
 void build_binary_host(const std::vector<TriPrep>& tris, uint32_t leaf_threshold,

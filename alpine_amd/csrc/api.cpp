@@ -175,7 +175,7 @@ int apg_scene_build(apg_context* ctx, const apg_build_params* params)
     if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
 
     std::vector<apg::TriPrep> tris;
-    apg::prepare_triangles(*ctx, tris);
+    if (builder == APG_BUILDER_HOST) apg::prepare_triangles(*ctx, tris);
     apg::BinaryTree bt;
     std::vector<ApgWideNode> nodes;
     std::vector<float> tri9;
@@ -207,7 +207,9 @@ int apg_scene_build(apg_context* ctx, const apg_build_params* params)
     st.sah_cost = bt.sah_cost;
     st.sah_cost_leaf = bt.sah_cost_leaf;
     st.device_bytes = uint64_t(ctx->scene.node_count) * 64 + uint64_t(ctx->scene.tri_count) * 40;
-    st.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    st.build_ms = bt.build_ms > 0.0
+        ? bt.build_ms
+        : std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     ctx->built = true;
     return APG_OK;
 }

@@ -1,10 +1,1049 @@
-// GPU binned-SAH builder (placeholder until the device builder lands).
+// GPU scene build for gfx950: triangle preparation, binned-SAH binary tree,
+// collapse into 64-byte quantised 8-wide nodes, triangle packing.
+//
+// Binary tree (breadth first, one pass per level over the triangle slots):
+//   bounds   node box and centroid box of every node of the level; min/max
+//            are accumulated as order-preserving integer atomics, first in
+//            LDS for the node that owns the start of a workgroup's slot chunk
+//            (the top levels are one or a few big nodes) and then once per
+//            workgroup into HBM, directly in HBM for the other nodes.
+//   decide   per node: leaf (N <= threshold, zero centroid extent), forced
+//            positional split (an SAH leaf above APG_MAX_LEAF triangles) or SAH
+//            candidate; candidates get a histogram slot.
+//   bin      9 bins x 3 axes of (box, count) per candidate, LDS-privatised
+//            like the bounds pass.
+//   sweep    per candidate the reference's forward/backward cost sweep and
+//            strict-< argmin (bvh_common.cpp:185-272), leaf test
+//            split cost >= N * SA (bvh_common.cpp:274-343); children are
+//            allocated contiguously after the level.
+//   partition stable partition of each node's slot range: a global exclusive
+//            scan of the "below" flags gives every slot its new position.
+// Min/max are exact and order independent and the sweep is sequential per
+// node with the reference's operand order, so with the reference's leaf
+// threshold the leaf sets equal the reference builder's.
+//
+// Wide collapse (breadth first over wide levels): per wide node, the
+// reference's greedy opening of the largest-area inner child up to 8
+// children (wide_bvh.cpp:200-243), octant slot assignment, quantisation;
+// child nodes and triangle ranges are placed with exclusive scans.
 #include "apg_internal.h"
 
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cstring>
+
 namespace apg {
-int build_scene_gpu(Context* ctx, const std::vector<TriPrep>&, uint32_t, BinaryTree&,
-                    std::vector<ApgWideNode>&, std::vector<float>&, std::vector<uint32_t>&)
+
+namespace {
+
+constexpr uint32_t BLOCK = 256;
+constexpr uint32_t ITEMS = 8;               // slots per thread
+constexpr uint32_t CHUNK = BLOCK * ITEMS;   // slots per workgroup
+constexpr uint32_t NBINS = 9;
+constexpr uint32_t NSPLITS = 8;
+constexpr uint32_t BIN_VALS = 7;            // lo xyz, hi xyz (ordered uints), count
+constexpr uint32_t NODE_BIN_WORDS = 3 * NBINS * BIN_VALS;
+constexpr uint32_t RETIRED = 0xFFFFFFFFu;
+
+enum : uint32_t { NF_LEAF = 1u, NF_SAH_LEAF = 2u, NF_FORCED = 4u, NF_SPLIT = 8u };
+
+__host__ __device__ __forceinline__ uint32_t ord_of(float f)
 {
-    return ctx->fail(APG_E_STATE, "GPU builder not available in this build");
+    uint32_t u;
+#if defined(__HIP_DEVICE_COMPILE__)
+    u = __float_as_uint(f);
+#else
+    __builtin_memcpy(&u, &f, 4);
+#endif
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
+
+__host__ __device__ __forceinline__ float float_of(uint32_t o)
+{
+    uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    float f;
+#if defined(__HIP_DEVICE_COMPILE__)
+    f = __uint_as_float(u);
+#else
+    __builtin_memcpy(&f, &u, 4);
+#endif
+    return f;
 }
+
+__device__ __forceinline__ float min_s(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float max_s(float a, float b) { return (a < b) ? b : a; }
+
+__device__ __forceinline__ uint8_t bin_index(float c, float mn, float mx)
+{
+    uint8_t b = static_cast<uint8_t>(float(NBINS) * (c - mn) / (mx - mn));
+    return b < NBINS - 1 ? b : uint8_t(NBINS - 1);
+}
+
+struct BuildState
+{
+    // triangles (original order)
+    float* pbox;        // 6 per triangle: lo xyz, hi xyz
+    float* tri9_src;    // 9 per triangle: v0, e0, e1
+    // slots
+    uint32_t* idx;      // slot -> triangle
+    uint32_t* idx_tmp;
+    uint32_t* snode;    // slot -> node (RETIRED once in a leaf)
+    uint32_t* snode_tmp;
+    uint32_t* flag;     // partition flags / scan input
+    uint32_t* scan;     // exclusive scan of flag (n + 1)
+    // nodes (capacity 2n)
+    uint32_t* nbox;     // 6 ordered uints per node (bounds accumulators)
+    uint32_t* ncbox;    // 6 ordered uints per node (centroid bounds)
+    uint32_t* nfirst;
+    uint32_t* ncount;
+    uint32_t* nleft;    // right = left + 1
+    uint32_t* nflags;
+    uint32_t* nsplit;   // dim | idx << 8
+    float* nsplit_mm;   // mn, mx
+    uint32_t* nbinslot; // histogram slot or RETIRED
+    uint32_t* bins;     // NODE_BIN_WORDS per candidate
+    uint32_t* counters; // [0] node count, [1] bin slots, [2] tmp
+    uint32_t* scan_tmp;
+};
+
+// ---------------------------------------------------------------------------
+// triangle preparation (Primitive::updateVertices, bvh_common.cpp:105-125)
+// ---------------------------------------------------------------------------
+__global__ void k_prep(const float* __restrict__ xyz, const uint32_t* __restrict__ tri,
+                       uint32_t n, float* __restrict__ tri9, float* __restrict__ pbox)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v[3][3];
+    for (int k = 0; k < 3; ++k)
+    {
+        const uint32_t vi = tri[3 * size_t(i) + k];
+        for (int a = 0; a < 3; ++a) v[k][a] = xyz[3 * size_t(vi) + a];
+    }
+    float* t = tri9 + 9 * size_t(i);
+    float* b = pbox + 6 * size_t(i);
+    for (int a = 0; a < 3; ++a)
+    {
+        t[a] = v[0][a];
+        t[3 + a] = v[1][a] - v[0][a];
+        t[6 + a] = v[2][a] - v[0][a];
+        float lo = FLT_MAX, hi = -FLT_MAX;
+        for (int k = 0; k < 3; ++k) { lo = min_s(lo, v[k][a]); hi = max_s(hi, v[k][a]); }
+        b[a] = lo;
+        b[3 + a] = hi;
+    }
+}
+
+__global__ void k_init_slots(uint32_t n, uint32_t* idx, uint32_t* snode)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    idx[i] = i;
+    snode[i] = 0;
+}
+
+__global__ void k_init_nodes(uint32_t begin, uint32_t end, uint32_t* nbox, uint32_t* ncbox,
+                             uint32_t* nbinslot)
+{
+    const uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    for (int a = 0; a < 3; ++a)
+    {
+        nbox[6 * size_t(k) + a] = ord_of(FLT_MAX);
+        nbox[6 * size_t(k) + 3 + a] = ord_of(-FLT_MAX);
+        ncbox[6 * size_t(k) + a] = ord_of(FLT_MAX);
+        ncbox[6 * size_t(k) + 3 + a] = ord_of(-FLT_MAX);
+    }
+    nbinslot[k] = RETIRED;
+}
+
+// Node box and centroid box of every node in [begin, end).
+__global__ __launch_bounds__(BLOCK) void k_bounds(BuildState S, uint32_t n, uint32_t begin,
+                                                  uint32_t end)
+{
+    __shared__ uint32_t acc[12];
+    const uint32_t c0 = blockIdx.x * CHUNK;
+    if (c0 >= n) return;
+    const uint32_t home = S.snode[c0];
+    if (threadIdx.x < 12) acc[threadIdx.x] = (threadIdx.x % 6) < 3 ? ord_of(FLT_MAX) : ord_of(-FLT_MAX);
+    __syncthreads();
+    for (uint32_t k = 0; k < ITEMS; ++k)
+    {
+        const uint32_t i = c0 + k * BLOCK + threadIdx.x;
+        if (i >= n) break;
+        const uint32_t nd = S.snode[i];
+        if (nd == RETIRED || nd < begin || nd >= end) continue;
+        const float* b = S.pbox + 6 * size_t(S.idx[i]);
+        float lo[3], hi[3], c[3];
+        for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; c[a] = (lo[a] + hi[a]) * 0.5f; }
+        if (nd == home)
+        {
+            for (int a = 0; a < 3; ++a)
+            {
+                atomicMin(&acc[a], ord_of(lo[a]));
+                atomicMax(&acc[3 + a], ord_of(hi[a]));
+                atomicMin(&acc[6 + a], ord_of(c[a]));
+                atomicMax(&acc[9 + a], ord_of(c[a]));
+            }
+        }
+        else
+        {
+            for (int a = 0; a < 3; ++a)
+            {
+                atomicMin(&S.nbox[6 * size_t(nd) + a], ord_of(lo[a]));
+                atomicMax(&S.nbox[6 * size_t(nd) + 3 + a], ord_of(hi[a]));
+                atomicMin(&S.ncbox[6 * size_t(nd) + a], ord_of(c[a]));
+                atomicMax(&S.ncbox[6 * size_t(nd) + 3 + a], ord_of(c[a]));
+            }
+        }
+    }
+    __syncthreads();
+    if (home != RETIRED && home >= begin && home < end && threadIdx.x < 12)
+    {
+        const uint32_t v = acc[threadIdx.x];
+        const uint32_t j = threadIdx.x;
+        uint32_t* dst = j < 6 ? &S.nbox[6 * size_t(home) + j] : &S.ncbox[6 * size_t(home) + (j - 6)];
+        if ((j % 6) < 3) atomicMin(dst, v);
+        else atomicMax(dst, v);
+    }
+}
+
+__device__ __forceinline__ float box_area(const float lo[3], const float hi[3])
+{
+    const float x = hi[0] - lo[0], y = hi[1] - lo[1], z = hi[2] - lo[2];
+    return (x * y + y * z + z * x) * 2.0f;
+}
+
+// Classify every node of the level.
+__global__ void k_decide(BuildState S, uint32_t begin, uint32_t end, uint32_t threshold)
+{
+    const uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    const uint32_t cnt = S.ncount[k];
+    uint32_t fl = S.nflags[k];
+    if (fl & NF_FORCED)
+    {
+        if (cnt <= APG_MAX_LEAF) fl |= NF_LEAF;
+        else fl |= NF_SPLIT;
+        S.nflags[k] = fl;
+        return;
+    }
+    bool leaf = cnt <= threshold;
+    if (!leaf)
+    {
+        float clo[3], chi[3];
+        for (int a = 0; a < 3; ++a)
+        {
+            clo[a] = float_of(S.ncbox[6 * size_t(k) + a]);
+            chi[a] = float_of(S.ncbox[6 * size_t(k) + 3 + a]);
+        }
+        const float diag = max_s(chi[0] - clo[0], max_s(chi[1] - clo[1], chi[2] - clo[2]));
+        if (diag == 0.0f) leaf = true;
+    }
+    if (leaf)
+    {
+        fl |= NF_SAH_LEAF;
+        fl |= cnt <= APG_MAX_LEAF ? NF_LEAF : (NF_FORCED | NF_SPLIT);
+        S.nflags[k] = fl;
+        return;
+    }
+    const uint32_t slot = atomicAdd(&S.counters[1], 1u);
+    S.nbinslot[k] = slot;
+    uint32_t* b = S.bins + size_t(slot) * NODE_BIN_WORDS;
+    for (uint32_t j = 0; j < 3 * NBINS; ++j)
+    {
+        uint32_t* e = b + j * BIN_VALS;
+        for (int a = 0; a < 3; ++a) { e[a] = ord_of(FLT_MAX); e[3 + a] = ord_of(-FLT_MAX); }
+        e[6] = 0;
+    }
+}
+
+// 9-bin histograms (box + count) per axis for the SAH candidates.
+__global__ __launch_bounds__(BLOCK) void k_bin(BuildState S, uint32_t n, uint32_t begin,
+                                               uint32_t end)
+{
+    __shared__ uint32_t acc[NODE_BIN_WORDS];
+    __shared__ float cmn[3], cmx[3];
+    const uint32_t c0 = blockIdx.x * CHUNK;
+    if (c0 >= n) return;
+    const uint32_t home = S.snode[c0];
+    const bool home_ok = home != RETIRED && home >= begin && home < end && S.nbinslot[home] != RETIRED;
+    for (uint32_t j = threadIdx.x; j < NODE_BIN_WORDS; j += BLOCK)
+    {
+        const uint32_t v = j % BIN_VALS;
+        acc[j] = v < 3 ? ord_of(FLT_MAX) : (v < 6 ? ord_of(-FLT_MAX) : 0u);
+    }
+    if (home_ok && threadIdx.x < 3)
+    {
+        cmn[threadIdx.x] = float_of(S.ncbox[6 * size_t(home) + threadIdx.x]);
+        cmx[threadIdx.x] = float_of(S.ncbox[6 * size_t(home) + 3 + threadIdx.x]);
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < ITEMS; ++k)
+    {
+        const uint32_t i = c0 + k * BLOCK + threadIdx.x;
+        if (i >= n) break;
+        const uint32_t nd = S.snode[i];
+        if (nd == RETIRED || nd < begin || nd >= end) continue;
+        const uint32_t slot = S.nbinslot[nd];
+        if (slot == RETIRED) continue;
+        const float* b = S.pbox + 6 * size_t(S.idx[i]);
+        float lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; }
+        const bool local = nd == home;
+        uint32_t* dst = local ? acc : S.bins + size_t(slot) * NODE_BIN_WORDS;
+        for (int d = 0; d < 3; ++d)
+        {
+            float mn, mx;
+            if (local) { mn = cmn[d]; mx = cmx[d]; }
+            else
+            {
+                mn = float_of(S.ncbox[6 * size_t(nd) + d]);
+                mx = float_of(S.ncbox[6 * size_t(nd) + 3 + d]);
+            }
+            if (mn == mx) continue;
+            const float c = (lo[d] + hi[d]) * 0.5f;
+            const uint32_t bi = bin_index(c, mn, mx);
+            uint32_t* e = dst + (d * NBINS + bi) * BIN_VALS;
+            for (int a = 0; a < 3; ++a)
+            {
+                atomicMin(&e[a], ord_of(lo[a]));
+                atomicMax(&e[3 + a], ord_of(hi[a]));
+            }
+            atomicAdd(&e[6], 1u);
+        }
+    }
+    __syncthreads();
+    if (!home_ok) return;
+    uint32_t* g = S.bins + size_t(S.nbinslot[home]) * NODE_BIN_WORDS;
+    for (uint32_t j = threadIdx.x; j < NODE_BIN_WORDS; j += BLOCK)
+    {
+        const uint32_t v = j % BIN_VALS;
+        if (v == 6) { if (acc[j]) atomicAdd(&g[j], acc[j]); }
+        else if (v < 3) atomicMin(&g[j], acc[j]);
+        else atomicMax(&g[j], acc[j]);
+    }
+}
+
+// Sweep, split choice and child allocation for the SAH candidates; positional
+// splits for forced nodes.
+__global__ void k_sweep(BuildState S, uint32_t begin, uint32_t end)
+{
+    const uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    uint32_t fl = S.nflags[k];
+    const uint32_t cnt = S.ncount[k];
+    if (fl & NF_SPLIT)
+    {
+        // forced positional split: the first half rounded to whole leaves
+        uint32_t half = ((cnt + 1) / 2 + APG_MAX_LEAF - 1) / APG_MAX_LEAF * APG_MAX_LEAF;
+        half = half < cnt - 1 ? half : cnt - 1;
+        const uint32_t l = atomicAdd(&S.counters[0], 2u);
+        S.nleft[k] = l;
+        S.nsplit[k] = 0xFFu;  // positional
+        S.nsplit_mm[2 * size_t(k)] = __uint_as_float(half);
+        S.ncount[l] = half;
+        S.ncount[l + 1] = cnt - half;
+        S.nflags[l] = NF_FORCED;
+        S.nflags[l + 1] = NF_FORCED;
+        return;
+    }
+    const uint32_t slot = S.nbinslot[k];
+    if (slot == RETIRED) return;  // leaf
+    const uint32_t* b = S.bins + size_t(slot) * NODE_BIN_WORDS;
+    float nlo[3], nhi[3];
+    for (int a = 0; a < 3; ++a)
+    {
+        nlo[a] = float_of(S.nbox[6 * size_t(k) + a]);
+        nhi[a] = float_of(S.nbox[6 * size_t(k) + 3 + a]);
+    }
+    float min_cost = FLT_MAX;
+    uint32_t best_dim = 0, best_idx = 0;
+    float best_mn = 0.0f, best_mx = 0.0f;
+    for (uint32_t d = 0; d < 3; ++d)
+    {
+        const float mn = float_of(S.ncbox[6 * size_t(k) + d]);
+        const float mx = float_of(S.ncbox[6 * size_t(k) + 3 + d]);
+        if (mn == mx) continue;
+        float costs[NSPLITS];
+        for (uint32_t s = 0; s < NSPLITS; ++s) costs[s] = 0.0f;
+        float alo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, ahi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+        uint32_t an = 0;
+        for (uint32_t s = 0; s < NSPLITS; ++s)
+        {
+            const uint32_t* e = b + (d * NBINS + s) * BIN_VALS;
+            for (int a = 0; a < 3; ++a) { alo[a] = min_s(alo[a], float_of(e[a])); ahi[a] = max_s(ahi[a], float_of(e[3 + a])); }
+            an += e[6];
+            costs[s] += float(an) * box_area(alo, ahi);
+        }
+        for (int a = 0; a < 3; ++a) { alo[a] = FLT_MAX; ahi[a] = -FLT_MAX; }
+        an = 0;
+        for (int s = NSPLITS - 1; s >= 0; --s)
+        {
+            const uint32_t* e = b + (d * NBINS + s + 1) * BIN_VALS;
+            for (int a = 0; a < 3; ++a) { alo[a] = min_s(alo[a], float_of(e[a])); ahi[a] = max_s(ahi[a], float_of(e[3 + a])); }
+            an += e[6];
+            costs[s] += float(an) * box_area(alo, ahi);
+        }
+        for (uint32_t s = 0; s < NSPLITS; ++s)
+        {
+            if (costs[s] < min_cost)
+            {
+                min_cost = costs[s];
+                best_dim = d;
+                best_idx = s;
+                best_mn = mn;
+                best_mx = mx;
+            }
+        }
+    }
+    const float leaf_cost = float(cnt) * box_area(nlo, nhi);
+    if (min_cost >= leaf_cost)
+    {
+        fl |= NF_SAH_LEAF;
+        fl |= cnt <= APG_MAX_LEAF ? NF_LEAF : (NF_FORCED | NF_SPLIT);
+        S.nflags[k] = fl;
+        S.nbinslot[k] = RETIRED;
+        if (fl & NF_SPLIT)
+        {
+            uint32_t half = ((cnt + 1) / 2 + APG_MAX_LEAF - 1) / APG_MAX_LEAF * APG_MAX_LEAF;
+            half = half < cnt - 1 ? half : cnt - 1;
+            const uint32_t l = atomicAdd(&S.counters[0], 2u);
+            S.nleft[k] = l;
+            S.nsplit[k] = 0xFFu;
+            S.nsplit_mm[2 * size_t(k)] = __uint_as_float(half);
+            S.ncount[l] = half;
+            S.ncount[l + 1] = cnt - half;
+            S.nflags[l] = NF_FORCED;
+            S.nflags[l + 1] = NF_FORCED;
+        }
+        return;
+    }
+    // child counts from the chosen axis' bins
+    uint32_t nbelow = 0;
+    for (uint32_t s = 0; s <= best_idx; ++s) nbelow += b[(best_dim * NBINS + s) * BIN_VALS + 6];
+    const uint32_t l = atomicAdd(&S.counters[0], 2u);
+    S.nleft[k] = l;
+    S.nsplit[k] = best_dim | (best_idx << 8);
+    S.nsplit_mm[2 * size_t(k)] = best_mn;
+    S.nsplit_mm[2 * size_t(k) + 1] = best_mx;
+    S.nflags[k] = fl | NF_SPLIT;
+    S.ncount[l] = nbelow;
+    S.ncount[l + 1] = cnt - nbelow;
+    S.nflags[l] = 0;
+    S.nflags[l + 1] = 0;
+}
+
+// Partition flags: 1 = goes to the left child.
+__global__ void k_flags(BuildState S, uint32_t n, uint32_t begin, uint32_t end)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t nd = S.snode[i];
+    uint32_t f = 0;
+    if (nd != RETIRED && nd >= begin && nd < end && (S.nflags[nd] & NF_SPLIT))
+    {
+        const uint32_t sp = S.nsplit[nd];
+        if ((sp & 0xFFu) == 0xFFu)
+            f = (i - S.nfirst[nd]) < __float_as_uint(S.nsplit_mm[2 * size_t(nd)]) ? 1u : 0u;
+        else
+        {
+            const uint32_t d = sp & 0xFFu, si = sp >> 8;
+            const float* b = S.pbox + 6 * size_t(S.idx[i]);
+            const float c = (b[d] + b[3 + d]) * 0.5f;
+            f = bin_index(c, S.nsplit_mm[2 * size_t(nd)], S.nsplit_mm[2 * size_t(nd) + 1]) <= si ? 1u : 0u;
+        }
+    }
+    S.flag[i] = f;
+}
+
+__global__ void k_scatter(BuildState S, uint32_t n, uint32_t begin, uint32_t end)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t nd = S.snode[i];
+    if (nd == RETIRED || nd < begin || nd >= end || !(S.nflags[nd] & NF_SPLIT))
+    {
+        S.idx_tmp[i] = S.idx[i];
+        S.snode_tmp[i] = (nd != RETIRED && nd >= begin && nd < end) ? RETIRED : nd;
+        return;
+    }
+    const uint32_t first = S.nfirst[nd], cnt = S.ncount[nd];
+    const uint32_t below_before = S.scan[i] - S.scan[first];
+    const uint32_t nbelow = S.scan[first + cnt] - S.scan[first];
+    const uint32_t l = S.nleft[nd];
+    uint32_t pos, child;
+    if (S.flag[i]) { pos = first + below_before; child = l; }
+    else { pos = first + nbelow + (i - first - below_before); child = l + 1; }
+    S.idx_tmp[pos] = S.idx[i];
+    S.snode_tmp[pos] = child;
+}
+
+// First slot of every child allocated in this level.
+__global__ void k_child_first(BuildState S, uint32_t begin, uint32_t end)
+{
+    const uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    if (!(S.nflags[k] & NF_SPLIT)) return;
+    const uint32_t l = S.nleft[k];
+    S.nfirst[l] = S.nfirst[k];
+    S.nfirst[l + 1] = S.nfirst[k] + S.ncount[l];
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan (3 passes, recursive on the block sums)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_scan_blocks(const uint32_t* in, uint32_t n, uint32_t* out,
+                                                       uint32_t* sums)
+{
+    __shared__ uint32_t sh[BLOCK];
+    const uint32_t base = blockIdx.x * CHUNK + threadIdx.x * ITEMS;
+    uint32_t v[ITEMS], tot = 0;
+    for (uint32_t k = 0; k < ITEMS; ++k)
+    {
+        v[k] = (base + k < n) ? in[base + k] : 0u;
+        tot += v[k];
+    }
+    sh[threadIdx.x] = tot;
+    __syncthreads();
+    for (uint32_t off = 1; off < BLOCK; off <<= 1)
+    {
+        uint32_t x = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += x;
+        __syncthreads();
+    }
+    uint32_t run = sh[threadIdx.x] - tot;
+    for (uint32_t k = 0; k < ITEMS; ++k)
+    {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == BLOCK - 1) sums[blockIdx.x] = sh[threadIdx.x];
+}
+
+__global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* offs)
+{
+    const uint32_t i = blockIdx.x * CHUNK + threadIdx.x;
+    const uint32_t add = offs[blockIdx.x];
+    for (uint32_t k = 0; k < ITEMS; ++k)
+    {
+        const uint32_t j = i + k * BLOCK;
+        if (j < n) out[j] += add;
+    }
+}
+
+__global__ void k_set_total(const uint32_t* in, const uint32_t* out, uint32_t n, uint32_t* total)
+{
+    *total = n ? out[n - 1] + in[n - 1] : 0u;
+}
+
+// out[0..n) = exclusive scan of in; out[n] = total.  tmp needs ceil(n / CHUNK) * 2 + 64 words.
+void exclusive_scan(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* tmp, hipStream_t s)
+{
+    const uint32_t nb = (n + CHUNK - 1) / CHUNK;
+    if (n == 0)
+    {
+        k_set_total<<<1, 1, 0, s>>>(in, out, 0, out);
+        return;
+    }
+    uint32_t* sums = tmp;
+    uint32_t* sums_scan = tmp + nb + 1;
+    k_scan_blocks<<<nb, BLOCK, 0, s>>>(in, n, out, sums);
+    if (nb > 1)
+    {
+        exclusive_scan(sums, nb, sums_scan, sums_scan + nb + 1, s);
+        k_scan_add<<<nb, BLOCK, 0, s>>>(out, n, sums_scan);
+    }
+    k_set_total<<<1, 1, 0, s>>>(in, out, n, out + n);
+}
+
+size_t scan_tmp_words(uint32_t n)
+{
+    size_t words = 64;
+    while (n > 1)
+    {
+        uint32_t nb = (n + CHUNK - 1) / CHUNK;
+        words += 2 * (size_t(nb) + 1);
+        n = nb;
+    }
+    return words;
+}
+
+// ---------------------------------------------------------------------------
+// wide collapse
+// ---------------------------------------------------------------------------
+struct WideState
+{
+    const uint32_t* nbox;   // ordered uints
+    const uint32_t* nfirst;
+    const uint32_t* ncount;
+    const uint32_t* nleft;
+    const uint32_t* nflags;
+    const uint32_t* idx;    // tree order -> triangle
+    const float* tri9_src;
+    uint32_t* wbin;         // wide node -> binary node (per level, indexed by wide id)
+    uint32_t* wchild;       // 8 per wide node at this level (binary ids, slot order, RETIRED = empty)
+    uint32_t* n_inner;      // per wide node of the level
+    uint32_t* n_tris;
+    uint32_t* s_inner;      // scans (n + 1)
+    uint32_t* s_tris;
+    ApgWideNode* nodes;
+    float* tri9;
+    uint32_t* tri_ids;
+    float slack;
+};
+
+__device__ __forceinline__ bool is_leaf_node(const WideState& W, uint32_t b)
+{
+    return (W.nflags[b] & NF_LEAF) != 0;
+}
+
+__device__ __forceinline__ void node_box(const WideState& W, uint32_t b, float lo[3], float hi[3])
+{
+    for (int a = 0; a < 3; ++a)
+    {
+        lo[a] = float_of(W.nbox[6 * size_t(b) + a]);
+        hi[a] = float_of(W.nbox[6 * size_t(b) + 3 + a]);
+    }
+}
+
+__global__ void k_wide_gather(WideState W, uint32_t wbegin, uint32_t wend)
+{
+    const uint32_t w = wbegin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= wend) return;
+    const uint32_t root = W.wbin[w];
+    uint32_t ch[APG_WIDE];
+    uint32_t nch = 0;
+    if (is_leaf_node(W, root)) ch[nch++] = root;
+    else
+    {
+        ch[nch++] = W.nleft[root];
+        ch[nch++] = W.nleft[root] + 1;
+        while (nch < APG_WIDE)
+        {
+            int best = -1;
+            float best_sa = 0.0f;
+            for (uint32_t i = 0; i < nch; ++i)
+            {
+                if (is_leaf_node(W, ch[i])) continue;
+                float lo[3], hi[3];
+                node_box(W, ch[i], lo, hi);
+                const float sa = box_area(lo, hi);
+                if (best < 0 || sa > best_sa) { best = int(i); best_sa = sa; }
+            }
+            if (best < 0) break;
+            const uint32_t c = ch[best];
+            ch[nch++] = W.nleft[c];
+            ch[best] = W.nleft[c] + 1;
+        }
+    }
+    // octant slot assignment (greedy over the 8 x 8 cost table)
+    float plo[3], phi[3];
+    node_box(W, root, plo, phi);
+    float pc[3];
+    for (int a = 0; a < 3; ++a) pc[a] = 0.5f * (plo[a] + phi[a]);
+    float cd[APG_WIDE][3];
+    for (uint32_t c = 0; c < nch; ++c)
+    {
+        float lo[3], hi[3];
+        node_box(W, ch[c], lo, hi);
+        for (int a = 0; a < 3; ++a) cd[c][a] = 0.5f * (lo[a] + hi[a]) - pc[a];
+    }
+    uint32_t slot_child[APG_WIDE];
+    for (uint32_t s = 0; s < APG_WIDE; ++s) slot_child[s] = RETIRED;
+    uint32_t done = 0, used = 0;
+    for (uint32_t r = 0; r < nch; ++r)
+    {
+        float bc = FLT_MAX;
+        int bcc = -1, bs = -1;
+        for (uint32_t c = 0; c < nch; ++c)
+        {
+            if ((done >> c) & 1u) continue;
+            for (uint32_t s = 0; s < APG_WIDE; ++s)
+            {
+                if ((used >> s) & 1u) continue;
+                const float cost = ((s & 1) ? -cd[c][0] : cd[c][0]) + ((s & 2) ? -cd[c][1] : cd[c][1]) +
+                                   ((s & 4) ? -cd[c][2] : cd[c][2]);
+                if (bcc < 0 || cost < bc) { bc = cost; bcc = int(c); bs = int(s); }
+            }
+        }
+        slot_child[bs] = ch[bcc];
+        done |= 1u << bcc;
+        used |= 1u << bs;
+    }
+    uint32_t ni = 0, nt = 0;
+    for (uint32_t s = 0; s < APG_WIDE; ++s)
+    {
+        const uint32_t c = slot_child[s];
+        W.wchild[8 * size_t(w - wbegin) + s] = c;
+        if (c == RETIRED) continue;
+        if (is_leaf_node(W, c)) nt += W.ncount[c];
+        else ni++;
+    }
+    W.n_inner[w - wbegin] = ni;
+    W.n_tris[w - wbegin] = nt;
+}
+
+__device__ uint32_t quantise_axis(float lo_f, float hi_f, float slack, const float* clo,
+                                  const float* chi, uint32_t valid, float& origin,
+                                  uint32_t* qlo, uint32_t* qhi)
+{
+    const float lo = lo_f - slack, hi = hi_f + slack;
+    const float m = fmaxf(fabsf(lo), fabsf(hi));
+    int k = 0;
+    (void)frexpf(m > 0.0f ? m : 1.0f, &k);  // m < 2^k
+    int e = k - 23;
+    const float ext = (hi - lo) * (1.0f + 0x1p-20f);
+    while (ldexpf(62.0f, e) < ext) ++e;
+    if (e < -126) e = -126;
+    for (;;)
+    {
+        const float step = ldexpf(1.0f, e);
+        const float inv = ldexpf(1.0f, -e);
+        const float o = floorf(lo * inv) * step;
+        if (o + 63.0f * step >= hi)
+        {
+            origin = o;
+            for (uint32_t s = 0; s < APG_WIDE; ++s)
+            {
+                if (!((valid >> s) & 1u)) { qlo[s] = APG_QMAX; qhi[s] = 0; continue; }
+                const float a = floorf(((clo[s] - slack) - o) * inv);
+                const float b = ceilf(((chi[s] + slack) - o) * inv);
+                qlo[s] = uint32_t(fminf(fmaxf(a, 0.0f), float(APG_QMAX)));
+                qhi[s] = uint32_t(fminf(fmaxf(b, 0.0f), float(APG_QMAX)));
+            }
+            return uint32_t(e + 127);
+        }
+        ++e;
+    }
+}
+
+__device__ __forceinline__ void put_plane(uint32_t q[9], uint32_t slot, uint32_t plane, uint32_t v)
+{
+    const uint32_t bit = 36u * slot + 6u * plane;
+    const uint32_t w = bit >> 5, s = bit & 31u;
+    q[w] |= (v & 63u) << s;
+    if (s > 26u) q[w + 1] |= (v & 63u) >> (32u - s);
+}
+
+__global__ void k_wide_emit(WideState W, uint32_t wbegin, uint32_t wend, uint32_t child_begin,
+                            uint32_t tri_begin)
+{
+    const uint32_t w = wbegin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= wend) return;
+    const uint32_t lw = w - wbegin;
+    ApgWideNode nd;
+    const uint32_t child_base = child_begin + W.s_inner[lw];
+    const uint32_t tri_base = tri_begin + W.s_tris[lw];
+    uint32_t imask = 0, valid = 0, counts = 0, inner_rank = 0, toff = 0;
+    float clo[3][APG_WIDE], chi[3][APG_WIDE];
+    float nlo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, nhi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+    for (uint32_t s = 0; s < APG_WIDE; ++s)
+    {
+        const uint32_t c = W.wchild[8 * size_t(lw) + s];
+        for (int a = 0; a < 3; ++a) { clo[a][s] = 0.0f; chi[a][s] = 0.0f; }
+        if (c == RETIRED) continue;
+        valid |= 1u << s;
+        float lo[3], hi[3];
+        node_box(W, c, lo, hi);
+        for (int a = 0; a < 3; ++a)
+        {
+            clo[a][s] = lo[a];
+            chi[a][s] = hi[a];
+            nlo[a] = fminf(nlo[a], lo[a]);
+            nhi[a] = fmaxf(nhi[a], hi[a]);
+        }
+        if (is_leaf_node(W, c))
+        {
+            const uint32_t cnt = W.ncount[c], first = W.nfirst[c];
+            counts |= (cnt - 1u) << (2u * s);
+            for (uint32_t k = 0; k < cnt; ++k)
+            {
+                const uint32_t g = W.idx[first + k];
+                const float* src = W.tri9_src + 9 * size_t(g);
+                float* dst = W.tri9 + 9 * size_t(tri_base + toff);
+                for (int j = 0; j < 9; ++j) dst[j] = src[j];
+                W.tri_ids[tri_base + toff] = g;
+                toff++;
+            }
+        }
+        else
+        {
+            imask |= 1u << s;
+            W.wbin[child_base + inner_rank] = c;
+            inner_rank++;
+        }
+    }
+    uint32_t eb[3], qlo[3][APG_WIDE], qhi[3][APG_WIDE];
+    for (int a = 0; a < 3; ++a)
+        eb[a] = quantise_axis(nlo[a], nhi[a], W.slack, clo[a], chi[a], valid, nd.origin[a], qlo[a], qhi[a]);
+    nd.exp_imask = eb[0] | (eb[1] << 8) | (eb[2] << 16) | (imask << 24);
+    nd.child_base = child_base;
+    nd.tri_base = tri_base;
+    nd.meta = counts | (valid << 16);
+    for (int j = 0; j < 9; ++j) nd.q[j] = 0;
+    for (uint32_t s = 0; s < APG_WIDE; ++s)
+        for (uint32_t a = 0; a < 3; ++a)
+        {
+            put_plane(nd.q, s, a, qlo[a][s]);
+            put_plane(nd.q, s, 3 + a, qhi[a][s]);
+        }
+    W.nodes[w] = nd;
+}
+
+__global__ void k_scene_slack(const float* pbox, uint32_t n, uint32_t* out)
+{
+    __shared__ uint32_t sh;
+    if (threadIdx.x == 0) sh = 0;
+    __syncthreads();
+    uint32_t m = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int a = 0; a < 6; ++a) m = max(m, __float_as_uint(fabsf(pbox[6 * size_t(i) + a])));
+    atomicMax(&sh, m);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(out, sh);
+}
+
+template <typename T>
+int dalloc(Context* ctx, T*& p, size_t count)
+{
+    p = nullptr;
+    if (count == 0) count = 1;
+    APG_HIP_CHECK(ctx, hipMalloc(&p, count * sizeof(T)));
+    return APG_OK;
+}
+
+struct Scratch
+{
+    std::vector<void*> ptrs;
+    ~Scratch()
+    {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <typename T>
+    int get(Context* ctx, T*& p, size_t count)
+    {
+        int rc = dalloc(ctx, p, count);
+        if (rc == APG_OK) ptrs.push_back(p);
+        return rc;
+    }
+};
+
+inline uint32_t blocks_for(uint32_t n, uint32_t b) { return n ? (n + b - 1) / b : 1; }
+
+} // namespace
+
+int build_scene_gpu(Context* ctx, const std::vector<TriPrep>& /*host_prep*/, uint32_t leaf_threshold,
+                    BinaryTree& bt, std::vector<ApgWideNode>&, std::vector<float>&,
+                    std::vector<uint32_t>&)
+{
+    const auto t_start = std::chrono::steady_clock::now();
+    const uint32_t n = ctx->total_tris;
+    hipStream_t s = nullptr;
+    // free the previous scene
+    if (ctx->scene.nodes) (void)hipFree(ctx->scene.nodes);
+    if (ctx->scene.tris) (void)hipFree(ctx->scene.tris);
+    if (ctx->scene.tri_ids) (void)hipFree(ctx->scene.tri_ids);
+    ctx->scene = DeviceScene{};
+    bt = BinaryTree{};
+    if (n == 0) return APG_OK;
+
+    Scratch scr;
+    // upload geometry (meshes concatenated, indices made global)
+    size_t nv_total = 0;
+    for (const Shape& sh : ctx->shapes) nv_total += sh.xyz.size() / 3;
+    std::vector<float> xyz;
+    std::vector<uint32_t> tri;
+    xyz.reserve(nv_total * 3);
+    tri.reserve(size_t(n) * 3);
+    for (const Shape& sh : ctx->shapes)
+    {
+        const uint32_t vbase = uint32_t(xyz.size() / 3);
+        xyz.insert(xyz.end(), sh.xyz.begin(), sh.xyz.end());
+        for (uint32_t v : sh.idx) tri.push_back(v + vbase);
+    }
+    float* d_xyz;
+    uint32_t* d_tri;
+    int rc;
+    if ((rc = scr.get(ctx, d_xyz, xyz.size()))) return rc;
+    if ((rc = scr.get(ctx, d_tri, tri.size()))) return rc;
+    APG_HIP_CHECK(ctx, hipMemcpyAsync(d_xyz, xyz.data(), xyz.size() * 4, hipMemcpyHostToDevice, s));
+    APG_HIP_CHECK(ctx, hipMemcpyAsync(d_tri, tri.data(), tri.size() * 4, hipMemcpyHostToDevice, s));
+
+    BuildState S;
+    const size_t ncap = 2 * size_t(n) + 2;
+    const uint32_t thr = std::max<uint32_t>(1, leaf_threshold);
+    const size_t bincap = size_t(n) / (thr + 1) + 2;
+    if ((rc = scr.get(ctx, S.pbox, 6 * size_t(n)))) return rc;
+    if ((rc = scr.get(ctx, S.tri9_src, 9 * size_t(n)))) return rc;
+    if ((rc = scr.get(ctx, S.idx, n))) return rc;
+    if ((rc = scr.get(ctx, S.idx_tmp, n))) return rc;
+    if ((rc = scr.get(ctx, S.snode, n))) return rc;
+    if ((rc = scr.get(ctx, S.snode_tmp, n))) return rc;
+    if ((rc = scr.get(ctx, S.flag, n))) return rc;
+    if ((rc = scr.get(ctx, S.scan, size_t(n) + 1))) return rc;
+    if ((rc = scr.get(ctx, S.nbox, 6 * ncap))) return rc;
+    if ((rc = scr.get(ctx, S.ncbox, 6 * ncap))) return rc;
+    if ((rc = scr.get(ctx, S.nfirst, ncap))) return rc;
+    if ((rc = scr.get(ctx, S.ncount, ncap))) return rc;
+    if ((rc = scr.get(ctx, S.nleft, ncap))) return rc;
+    if ((rc = scr.get(ctx, S.nflags, ncap))) return rc;
+    if ((rc = scr.get(ctx, S.nsplit, ncap))) return rc;
+    if ((rc = scr.get(ctx, S.nsplit_mm, 2 * ncap))) return rc;
+    if ((rc = scr.get(ctx, S.nbinslot, ncap))) return rc;
+    if ((rc = scr.get(ctx, S.bins, bincap * NODE_BIN_WORDS))) return rc;
+    if ((rc = scr.get(ctx, S.counters, 16))) return rc;
+    if ((rc = scr.get(ctx, S.scan_tmp, scan_tmp_words(std::max<uint32_t>(n, 1u) + 1)))) return rc;
+
+    k_prep<<<blocks_for(n, 256), 256, 0, s>>>(d_xyz, d_tri, n, S.tri9_src, S.pbox);
+    k_init_slots<<<blocks_for(n, 256), 256, 0, s>>>(n, S.idx, S.snode);
+    // root
+    const uint32_t root_first = 0, root_flags = 0, two = 1;
+    APG_HIP_CHECK(ctx, hipMemcpyAsync(S.nfirst, &root_first, 4, hipMemcpyHostToDevice, s));
+    APG_HIP_CHECK(ctx, hipMemcpyAsync(S.ncount, &n, 4, hipMemcpyHostToDevice, s));
+    APG_HIP_CHECK(ctx, hipMemcpyAsync(S.nflags, &root_flags, 4, hipMemcpyHostToDevice, s));
+    uint32_t host_counters[2] = { two, 0 };
+    APG_HIP_CHECK(ctx, hipMemcpyAsync(S.counters, host_counters, 8, hipMemcpyHostToDevice, s));
+
+    uint32_t begin = 0, end = 1, levels = 0;
+    const uint32_t nchunks = blocks_for(n, CHUNK);
+    while (begin < end)
+    {
+        const uint32_t width = end - begin;
+        k_init_nodes<<<blocks_for(width, 256), 256, 0, s>>>(begin, end, S.nbox, S.ncbox, S.nbinslot);
+        k_bounds<<<nchunks, BLOCK, 0, s>>>(S, n, begin, end);
+        uint32_t zero = 0;
+        APG_HIP_CHECK(ctx, hipMemcpyAsync(S.counters + 1, &zero, 4, hipMemcpyHostToDevice, s));
+        k_decide<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end, thr);
+        k_bin<<<nchunks, BLOCK, 0, s>>>(S, n, begin, end);
+        k_sweep<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end);
+        k_child_first<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end);
+        k_flags<<<blocks_for(n, 256), 256, 0, s>>>(S, n, begin, end);
+        exclusive_scan(S.flag, n, S.scan, S.scan_tmp, s);
+        k_scatter<<<blocks_for(n, 256), 256, 0, s>>>(S, n, begin, end);
+        std::swap(S.idx, S.idx_tmp);
+        std::swap(S.snode, S.snode_tmp);
+        uint32_t next_end = 0;
+        APG_HIP_CHECK(ctx, hipMemcpyAsync(&next_end, S.counters, 4, hipMemcpyDeviceToHost, s));
+        APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
+        APG_HIP_CHECK(ctx, hipGetLastError());
+        begin = end;
+        end = next_end;
+        if (++levels > 4096) return ctx->fail(APG_E_CAPACITY, "binary tree deeper than 4096 levels");
+    }
+    const uint32_t bin_nodes = end;
+
+    // ---- wide collapse ----
+    float slack = 0.0f;
+    {
+        uint32_t* d_m;
+        if ((rc = scr.get(ctx, d_m, 1))) return rc;
+        APG_HIP_CHECK(ctx, hipMemsetAsync(d_m, 0, 4, s));
+        k_scene_slack<<<256, 256, 0, s>>>(S.pbox, n, d_m);
+        uint32_t m = 0;
+        APG_HIP_CHECK(ctx, hipMemcpyAsync(&m, d_m, 4, hipMemcpyDeviceToHost, s));
+        APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
+        float mf;
+        std::memcpy(&mf, &m, 4);
+        if (mf == 0.0f) mf = 1.0f;
+        slack = std::ldexp(mf, -17);
+    }
+    const size_t wcap = bin_nodes / 2 + 2;  // each wide node holds >= 2 binary nodes or is the root
+    WideState W;
+    W.nbox = S.nbox; W.nfirst = S.nfirst; W.ncount = S.ncount; W.nleft = S.nleft;
+    W.nflags = S.nflags; W.idx = S.idx; W.tri9_src = S.tri9_src; W.slack = slack;
+    if ((rc = scr.get(ctx, W.wbin, 2 * wcap + 8))) return rc;
+    if ((rc = scr.get(ctx, W.wchild, 8 * wcap))) return rc;
+    if ((rc = scr.get(ctx, W.n_inner, wcap))) return rc;
+    if ((rc = scr.get(ctx, W.n_tris, wcap))) return rc;
+    if ((rc = scr.get(ctx, W.s_inner, wcap + 1))) return rc;
+    if ((rc = scr.get(ctx, W.s_tris, wcap + 1))) return rc;
+    uint32_t* wscan_tmp;
+    if ((rc = scr.get(ctx, wscan_tmp, scan_tmp_words(uint32_t(wcap) + 1)))) return rc;
+    ApgWideNode* d_nodes;
+    float* d_tri9;
+    uint32_t* d_ids;
+    if ((rc = dalloc(ctx, d_nodes, wcap))) return rc;
+    if ((rc = dalloc(ctx, d_tri9, 9 * size_t(n)))) { (void)hipFree(d_nodes); return rc; }
+    if ((rc = dalloc(ctx, d_ids, n))) { (void)hipFree(d_nodes); (void)hipFree(d_tri9); return rc; }
+    W.nodes = d_nodes; W.tri9 = d_tri9; W.tri_ids = d_ids;
+    const uint32_t zero = 0;
+    APG_HIP_CHECK(ctx, hipMemcpyAsync(W.wbin, &zero, 4, hipMemcpyHostToDevice, s));
+    uint32_t wbegin = 0, wend = 1, tri_cursor = 0;
+    while (wbegin < wend)
+    {
+        const uint32_t width = wend - wbegin;
+        k_wide_gather<<<blocks_for(width, 128), 128, 0, s>>>(W, wbegin, wend);
+        exclusive_scan(W.n_inner, width, W.s_inner, wscan_tmp, s);
+        exclusive_scan(W.n_tris, width, W.s_tris, wscan_tmp, s);
+        k_wide_emit<<<blocks_for(width, 128), 128, 0, s>>>(W, wbegin, wend, wend, tri_cursor);
+        uint32_t tot[2] = { 0, 0 };
+        APG_HIP_CHECK(ctx, hipMemcpyAsync(&tot[0], W.s_inner + width, 4, hipMemcpyDeviceToHost, s));
+        APG_HIP_CHECK(ctx, hipMemcpyAsync(&tot[1], W.s_tris + width, 4, hipMemcpyDeviceToHost, s));
+        APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
+        APG_HIP_CHECK(ctx, hipGetLastError());
+        tri_cursor += tot[1];
+        wbegin = wend;
+        wend += tot[0];
+        if (wend > wcap) return ctx->fail(APG_E_CAPACITY, "wide node capacity exceeded");
+    }
+    ctx->scene.nodes = d_nodes;
+    ctx->scene.tris = d_tri9;
+    ctx->scene.tri_ids = d_ids;
+    ctx->scene.node_count = wend;
+    ctx->scene.tri_count = tri_cursor;
+    if (tri_cursor != n) return ctx->fail(APG_E_STATE, "GPU build lost triangles");
+    bt.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+
+    // ---- binary-tree statistics and leaf keys (host, from the device tree) ----
+    std::vector<uint32_t> hbox(6 * size_t(bin_nodes)), hfirst(bin_nodes), hcount(bin_nodes),
+        hleft(bin_nodes), hflags(bin_nodes), hidx(n);
+    APG_HIP_CHECK(ctx, hipMemcpy(hbox.data(), S.nbox, hbox.size() * 4, hipMemcpyDeviceToHost));
+    APG_HIP_CHECK(ctx, hipMemcpy(hfirst.data(), S.nfirst, bin_nodes * 4, hipMemcpyDeviceToHost));
+    APG_HIP_CHECK(ctx, hipMemcpy(hcount.data(), S.ncount, bin_nodes * 4, hipMemcpyDeviceToHost));
+    APG_HIP_CHECK(ctx, hipMemcpy(hleft.data(), S.nleft, bin_nodes * 4, hipMemcpyDeviceToHost));
+    APG_HIP_CHECK(ctx, hipMemcpy(hflags.data(), S.nflags, bin_nodes * 4, hipMemcpyDeviceToHost));
+    APG_HIP_CHECK(ctx, hipMemcpy(hidx.data(), S.idx, size_t(n) * 4, hipMemcpyDeviceToHost));
+    auto area = [&](uint32_t k) {
+        float lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) { lo[a] = float_of(hbox[6 * size_t(k) + a]); hi[a] = float_of(hbox[6 * size_t(k) + 3 + a]); }
+        const float x = hi[0] - lo[0], y = hi[1] - lo[1], z = hi[2] - lo[2];
+        return double((x * y + y * z + z * x) * 2.0f);
+    };
+    bt.leaf_keys.assign(n, 0);
+    double inner = 0.0, leaf = 0.0;
+    uint32_t sah_nodes = 0, sah_leaves = 0, maxd = 0;
+    struct Item { uint32_t k, d; };
+    std::vector<Item> stack{ { 0, 0 } };
+    while (!stack.empty())
+    {
+        Item it = stack.back();
+        stack.pop_back();
+        sah_nodes++;
+        maxd = std::max(maxd, it.d);
+        const double sa = area(it.k);
+        if (hflags[it.k] & NF_SAH_LEAF)
+        {
+            sah_leaves++;
+            leaf += sa * double(hcount[it.k]);
+            uint32_t mn = 0xFFFFFFFFu;
+            for (uint32_t j = 0; j < hcount[it.k]; ++j) mn = std::min(mn, hidx[hfirst[it.k] + j]);
+            for (uint32_t j = 0; j < hcount[it.k]; ++j) bt.leaf_keys[hidx[hfirst[it.k] + j]] = mn;
+            continue;
+        }
+        inner += sa;
+        stack.push_back({ hleft[it.k] + 1, it.d + 1 });
+        stack.push_back({ hleft[it.k], it.d + 1 });
+    }
+    const double root_sa = area(0);
+    bt.sah_cost = (inner + leaf) / root_sa;
+    bt.sah_cost_leaf = leaf / root_sa;
+    bt.sah_nodes = sah_nodes;
+    bt.sah_leaves = sah_leaves;
+    bt.max_depth = maxd;
+    return APG_OK;
+}
+
+} // namespace apg

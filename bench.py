@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=262_144)
-    ap.add_argument("--builder", choices=["gpu", "host"], default="host")
+    ap.add_argument("--builder", choices=["gpu", "host"], default="gpu")
     return ap.parse_args()
 
 

@@ -281,3 +281,204 @@ def ref_tree_stats(verts, tris, leaf_threshold: int = 8) -> dict:
                              C.byref(nc), C.byref(lc), C.byref(md), keys)
     return {"sah_cost": sah.value, "sah_cost_leaf": sahl.value, "node_count": nc.value,
             "leaf_count": lc.value, "max_depth": md.value, "leaf_keys": keys}
+
+
+# ---------------------------------------------------------------------------
+# Render path (oracle/render.c) and the reference's render TUs (oracle/_ref)
+# ---------------------------------------------------------------------------
+REF_RENDER_LIB_PATH = os.path.join(HERE, "_ref", "libalpine_ref_render.so")
+
+MAT_DIELECTRIC, MAT_CONDUCTOR = 0, 1
+LIGHT_DISK, LIGHT_POINT, LIGHT_QUAD, LIGHT_TRIANGLE = 0, 1, 2, 3
+LS_UNIFORM, LS_POWER, LS_BVH = 0, 1, 2
+
+
+class Material(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("base_color", C.c_float * 3), ("alpha", C.c_float * 2)]
+
+
+class Light(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("p0", C.c_float * 3), ("p1", C.c_float * 3),
+                ("p2", C.c_float * 3), ("normal", C.c_float * 3), ("color", C.c_float * 3),
+                ("power", C.c_float), ("radius", C.c_float)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3),
+                ("fovy", C.c_float), ("aspect", C.c_float)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32)]
+
+
+def _render_sigs(L):
+    vp = C.c_void_p
+    L.orc_scene_create.restype = vp
+    L.orc_scene_create.argtypes = [_f32p, C.c_uint32, _u32p, C.c_uint32, _u32p, vp, C.c_uint32,
+                                   vp, vp, C.c_uint32, C.c_int, vp, vp]
+    L.orc_scene_destroy.argtypes = [vp]
+    L.orc_render.argtypes = [vp, vp, _u32p, np.ctypeslib.ndpointer(np.float64, flags="C"),
+                             C.c_uint32]
+    L.orc_sampler_reset.argtypes = [_u32p, C.c_uint32]
+    L.orc_sampler_get1d.restype = C.c_float
+    L.orc_sampler_get1d.argtypes = [_u32p]
+    L.orc_camera_ray.argtypes = [vp, C.c_float, C.c_float, _f32p, _f32p]
+    L.orc_mat_sample.restype = C.c_int
+    L.orc_mat_sample.argtypes = [vp, _f32p, _f32p, _f32p, _f32p, C.POINTER(C.c_float)]
+    L.orc_mat_eval.argtypes = [vp, _f32p, _f32p, _f32p, C.POINTER(C.c_float)]
+    L.orc_light_sampler_sample.restype = C.c_int
+    L.orc_light_sampler_sample.argtypes = [vp, C.c_float, _f32p, _f32p, C.POINTER(C.c_float)]
+    L.orc_light_sampler_pdf.restype = C.c_float
+    L.orc_light_sampler_pdf.argtypes = [vp, C.c_int, _f32p, _f32p]
+    L.orc_light_sample.restype = C.c_int
+    L.orc_light_sample.argtypes = [vp, C.c_int, _f32p, _f32p, _f32p, _f32p,
+                                   C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_light_pdf.restype = C.c_float
+    L.orc_light_pdf.argtypes = [vp, C.c_int, _f32p, _f32p, C.POINTER(C.c_float)]
+
+
+_render_ready = False
+
+
+def rlib():
+    global _render_ready
+    L = lib()
+    if not _render_ready:
+        _render_sigs(L)
+        _render_ready = True
+    return L
+
+
+def make_material(kind, base_color, alpha):
+    return Material(kind, (C.c_float * 3)(*base_color), (C.c_float * 2)(*alpha))
+
+
+def make_light(kind, p0=(0, 0, 0), p1=(0, 0, 0), p2=(0, 0, 0), normal=(0, 1, 0),
+               color=(1, 1, 1), power=1.0, radius=1.0):
+    f3 = lambda v: (C.c_float * 3)(*[float(x) for x in v])
+    return Light(kind, f3(p0), f3(p1), f3(p2), f3(normal), f3(color), float(power), float(radius))
+
+
+def make_camera(eye, target, up, fovy, aspect):
+    f3 = lambda v: (C.c_float * 3)(*[float(x) for x in v])
+    return Camera(f3(eye), f3(target), f3(up), float(fovy), float(aspect))
+
+
+class OracleScene:
+    """The reference integrator restated (alpine.cpp:290-432) over the oracle BVH."""
+
+    def __init__(self, verts, tris, tri_mat, materials, lights, light_sampler=LS_BVH,
+                 camera=None, background=(0.0, 0.0, 0.0), normals=None):
+        L = rlib()
+        self.verts = _f32(verts).reshape(-1, 3)
+        self.tris = _u32(tris).reshape(-1, 3)
+        self.tri_mat = _u32(tri_mat)
+        self._mats = (Material * max(1, len(materials)))(*materials)
+        self._lights = (Light * max(1, len(lights)))(*lights)
+        self._cam = camera
+        self._bg = (C.c_float * 3)(*background)
+        self._normals = None if normals is None else _f32(normals).reshape(-1)
+        self._h = L.orc_scene_create(
+            self.verts.reshape(-1), len(self.verts), self.tris.reshape(-1), len(self.tris),
+            self.tri_mat, C.cast(self._mats, C.c_void_p), len(materials),
+            None if self._normals is None else self._normals.ctypes.data_as(C.c_void_p),
+            C.cast(self._lights, C.c_void_p), len(lights), light_sampler,
+            C.cast(C.pointer(camera), C.c_void_p) if camera is not None else None,
+            C.cast(self._bg, C.c_void_p))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            rlib().orc_scene_destroy(h)
+            self._h = None
+
+    def render(self, width, height, spp, max_depth, states=None, accum=None, rr_depth=0,
+               nthreads=8):
+        if states is None:
+            states = sampler_states(width * height)
+        if accum is None:
+            accum = np.zeros((height, width, 3), np.float64)
+        rp = RenderParams(width, height, spp, max_depth, rr_depth)
+        rlib().orc_render(self._h, C.byref(rp), states.reshape(-1), accum.reshape(-1), nthreads)
+        return states, accum
+
+    def camera_ray(self, x, y):
+        o = np.empty(3, np.float32)
+        d = np.empty(3, np.float32)
+        rlib().orc_camera_ray(self._h, x, y, o, d)
+        return o, d
+
+    def light_sampler_sample(self, u, hit, ns):
+        p = C.c_float()
+        i = rlib().orc_light_sampler_sample(self._h, u, _f32(hit), _f32(ns), C.byref(p))
+        return i, p.value
+
+    def light_sampler_pdf(self, light, hit, ns):
+        return rlib().orc_light_sampler_pdf(self._h, light, _f32(hit), _f32(ns))
+
+    def light_sample(self, light, u2, hit):
+        le, wi = np.empty(3, np.float32), np.empty(3, np.float32)
+        d, p = C.c_float(), C.c_float()
+        ok = rlib().orc_light_sample(self._h, light, _f32(u2), _f32(hit), le, wi, C.byref(d),
+                                     C.byref(p))
+        return bool(ok), le, wi, d.value, p.value
+
+    def light_pdf(self, light, hit, wi):
+        d = C.c_float()
+        p = rlib().orc_light_pdf(self._h, light, _f32(hit), _f32(wi), C.byref(d))
+        return p, d.value
+
+
+def sampler_states(n_pixels, seed_offset=0):
+    """Per-pixel sampler states seeded with the pixel index (alpine.cpp:259-265)."""
+    L = rlib()
+    st = np.empty((n_pixels, 4), np.uint32)
+    for i in range(n_pixels):
+        L.orc_sampler_reset(st[i], (i + seed_offset) & 0xFFFFFFFF)
+    return st
+
+
+def sampler_sequence(seed, n):
+    L = rlib()
+    st = np.empty(4, np.uint32)
+    L.orc_sampler_reset(st, seed)
+    return np.array([L.orc_sampler_get1d(st) for _ in range(n)], np.float32)
+
+
+def mat_sample(m, wo, u):
+    est, wi = np.empty(3, np.float32), np.empty(3, np.float32)
+    p = C.c_float()
+    ok = rlib().orc_mat_sample(C.byref(m), _f32(wo), _f32(u), est, wi, C.byref(p))
+    return bool(ok), est, wi, p.value
+
+
+def mat_eval(m, wo, wi):
+    f = np.empty(3, np.float32)
+    p = C.c_float()
+    rlib().orc_mat_eval(C.byref(m), _f32(wo), _f32(wi), f, C.byref(p))
+    return f, p.value
+
+
+_ref_render = None
+
+
+def ref_render_available():
+    return os.path.exists(REF_RENDER_LIB_PATH)
+
+
+def ref_render_lib():
+    global _ref_render
+    if _ref_render is None:
+        L = C.CDLL(REF_RENDER_LIB_PATH)
+        fp, u8, u32, i32 = _f32p, _u8p, C.c_uint32, np.ctypeslib.ndpointer(np.int32, flags="C")
+        L.ref_sampler_sequence.argtypes = [u32, u32, fp]
+        L.ref_camera_rays.argtypes = [fp, fp, fp, C.c_float, C.c_float, fp, u32, fp, fp]
+        L.ref_material_sample.argtypes = [u32, fp, fp, fp, fp, u32, u8, fp, fp, fp]
+        L.ref_material_eval.argtypes = [u32, fp, fp, fp, fp, u32, fp, fp]
+        L.ref_light_sampler.argtypes = [u32, fp, u32, fp, fp, fp, u32, i32, fp]
+        L.ref_light_sample.argtypes = [fp, fp, fp, u32, u8, fp, fp, fp, fp]
+        L.ref_light_pdf.argtypes = [fp, fp, fp, u32, fp, fp]
+        _ref_render = L
+    return _ref_render

@@ -1,0 +1,1046 @@
+/*
+ * ORACLE -- test infrastructure only (never linked into alpine_amd).
+ *
+ * Plain-C restatement of the reference's render path: per-pixel xorshift128
+ * sampler, pinhole camera, BSDFs (Lambertian, GGX microfacet with VNDF
+ * sampling, Dielectric mixture, Conductor), lights (disk, point, and the quad
+ * and triangle area lights the configs add), light samplers (uniform, power
+ * via alias table, light BVH) and the NEE + 2-strategy MIS path tracer, with
+ * optional Russian roulette.  Radiance is accumulated in double.
+ *
+ * Geometry queries go through the oracle binary BVH (bvh.c).
+ * Compile with -ffp-contract=off; transcendental functions are libm's.
+ */
+#include "oracle.h"
+#include "render.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PI_F 3.14159265358979323846f
+static const float ONE_MINUS_EPS = 1.0f - FLT_EPSILON;  /* util.h:8 */
+static const float RAY_OFFSET = 0.001f;                 /* alpine_config.h:7 */
+
+typedef struct { float x, y, z; } v3;
+typedef struct { float x, y; } v2;
+
+static inline v3 V(float x, float y, float z) { v3 r = { x, y, z }; return r; }
+static inline v3 add(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 sub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 mulv(v3 a, v3 b) { return V(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline v3 muls(v3 a, float s) { return V(a.x * s, a.y * s, a.z * s); }
+static inline v3 divs(v3 a, float s) { return V(a.x / s, a.y / s, a.z / s); }
+static inline v3 divv(v3 a, v3 b) { return V(a.x / b.x, a.y / b.y, a.z / b.z); }
+static inline v3 neg(v3 a) { return V(-a.x, -a.y, -a.z); }
+static inline float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline float len(v3 a) { return sqrtf(dot(a, a)); }
+static inline v3 cross(v3 a, v3 b)
+{
+    return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline v3 normalize(v3 a) { return divs(a, len(a)); }
+static inline float fmin_std(float a, float b) { return (b < a) ? b : a; }
+static inline float fmax_std(float a, float b) { return (a < b) ? b : a; }
+static inline float clampf(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
+static inline v3 ld3(const float* p) { return V(p[0], p[1], p[2]); }
+
+/* ---- sampler (sampler.cpp:7-42) ---- */
+void orc_sampler_reset(uint32_t* s, uint32_t seed)
+{
+    for (uint32_t i = 1; i <= 4; ++i) s[i - 1] = seed = 1812433253u * (seed ^ (seed >> 30)) + i;
+}
+static inline uint32_t sampler_next(uint32_t* st)
+{
+    uint32_t t = st[3];
+    uint32_t s = st[0];
+    st[3] = st[2];
+    st[2] = st[1];
+    st[1] = s;
+    t ^= t << 11;
+    t ^= t >> 8;
+    return st[0] = t ^ s ^ (s >> 19);
+}
+float orc_sampler_get1d(uint32_t* st)
+{
+    uint32_t r = sampler_next(st);
+    double v = (double)r / (double)UINT32_MAX;
+    return fmin_std((float)v, ONE_MINUS_EPS);
+}
+static inline v2 get2d(uint32_t* st)
+{
+    v2 r;
+    r.x = orc_sampler_get1d(st);
+    r.y = orc_sampler_get1d(st);
+    return r;
+}
+
+/* ---- util.cpp:6-51, util.h ---- */
+static v2 concentric_disk(v2 u)
+{
+    v2 o = { 2.0f * u.x - 1.0f, 2.0f * u.y - 1.0f };
+    v2 z = { 0.0f, 0.0f };
+    if (o.x == 0 && o.y == 0) return z;
+    float theta, r;
+    if (fabsf(o.x) > fabsf(o.y)) {
+        r = o.x;
+        theta = 0.25f * PI_F * (o.y / o.x);
+    } else {
+        r = o.y;
+        theta = 0.5f * PI_F * (1.0f - 0.5f * (o.x / o.y));
+    }
+    v2 res = { r * cosf(theta), r * sinf(theta) };
+    return res;
+}
+static v3 cosine_hemisphere(v2 u, float* pdf)
+{
+    v2 d = concentric_disk(u);
+    float z = sqrtf(fmax_std(0.0f, 1.0f - d.x * d.x - d.y * d.y));
+    v3 dir = normalize(V(d.x, d.y, z));
+    *pdf = z / PI_F;
+    return dir;
+}
+static inline float cos2t(v3 w) { return w.z * w.z; }
+static inline float sin2t(v3 w) { return fmax_std(1.0f - cos2t(w), 0.0f); }
+static inline float sint(v3 w) { return sqrtf(sin2t(w)); }
+static inline float tan2t(v3 w) { return sin2t(w) / cos2t(w); }
+static inline float cosphi(v3 w) { float st = sint(w); return st == 0.0f ? 1.0f : clampf(w.x / st, -1.0f, 1.0f); }
+static inline float sinphi(v3 w) { float st = sint(w); return st == 0.0f ? 0.0f : clampf(w.y / st, -1.0f, 1.0f); }
+static inline v3 reflect(v3 v, v3 n) { return add(neg(v), muls(muls(n, 2.0f), dot(v, n))); }
+static inline int same_hemi(v3 a, v3 b) { return a.z * b.z > 0.0f; }
+static inline float power_heuristic(float a, float b) { return (a * a) / (a * a + b * b); }
+static inline v3 schlick(v3 f0, v3 wi, v3 wh)
+{
+    float f = powf(fmax_std(0.0f, 1.0f - dot(wi, wh)), 5.0f);
+    return add(V(f, f, f), muls(f0, 1.0f - f));
+}
+/* vector.h:214-222 */
+static void get_basis(v3 v, v3* b1, v3* b2)
+{
+    *b1 = fabsf(v.x) > fabsf(v.y) ? normalize(V(-v.z, 0.0f, v.x)) : normalize(V(0.0f, v.z, -v.y));
+    *b2 = cross(v, *b1);
+}
+static inline v3 to_local(v3 v, v3 s, v3 t, v3 n) { return V(dot(s, v), dot(t, v), dot(n, v)); }
+static inline v3 to_world(v3 v, v3 s, v3 t, v3 n) { return add(add(muls(s, v.x), muls(t, v.y)), muls(n, v.z)); }
+
+/* ---- microfacet.cpp:10-133 ---- */
+typedef struct { float ax, ay, f0; } mf_t;
+static float mf_d(const mf_t* m, v3 wh)
+{
+    float cos4 = cos2t(wh) * cos2t(wh);
+    float c2p = cosphi(wh) * cosphi(wh);
+    float s2p = sinphi(wh) * sinphi(wh);
+    float term = 1.0f + tan2t(wh) * (c2p / (m->ax * m->ax) + s2p / (m->ay * m->ay));
+    float invD = PI_F * m->ax * m->ay * cos4 * term * term;
+    return invD != 0.0f ? 1.0f / invD : 0.0f;
+}
+static float mf_lambda(const mf_t* m, v3 v)
+{
+    float a = sqrtf(cosphi(v) * cosphi(v) * m->ax * m->ax + sinphi(v) * sinphi(v) * m->ay * m->ay);
+    return (-1.0f + sqrtf(1.0f + a * a * tan2t(v))) / 2.0f;
+}
+static float mf_g2(const mf_t* m, v3 wo, v3 wi) { return 1.0f / (1.0f + mf_lambda(m, wo) + mf_lambda(m, wi)); }
+static int mf_sample(const mf_t* m, v3 wo, v2 u, v3 bc, v3* est, v3* wi_out, float* pdf)
+{
+    v3 woh = normalize(V(m->ax * wo.x, m->ay * wo.y, wo.z));
+    v2 t = concentric_disk(u);
+    float s = 0.5f * (1.0f + woh.z);
+    t.y = (1.0f - s) * sqrtf(1.0f - t.x * t.x) + s * t.y;
+    v3 nhh = V(t.x, t.y, sqrtf(fmax_std(0.0f, 1.0f - t.x * t.x - t.y * t.y)));
+    v3 b1, b2;
+    get_basis(woh, &b1, &b2);
+    v3 nh = add(add(muls(b1, nhh.x), muls(b2, nhh.y)), muls(woh, nhh.z));
+    v3 wh = normalize(V(m->ax * nh.x, m->ay * nh.y, fmax_std(0.0f, nh.z)));
+    if (!same_hemi(wo, wh)) wh = neg(wh);
+    v3 wi = reflect(wo, wh);
+    if (!same_hemi(wo, wi)) return 0;
+    v3 f0 = m->f0 < 0.0f ? bc : V(m->f0, m->f0, m->f0);
+    v3 f = schlick(f0, wi, wh);
+    float g1 = 1.0f / (1.0f + mf_lambda(m, wo));
+    float g2 = mf_g2(m, wo, wi);
+    float d = mf_d(m, wh);
+    *est = divs(muls(f, g2), g1);
+    *wi_out = wi;
+    *pdf = d / (4.0f * dot(wo, wh));
+    return 1;
+}
+static v3 mf_bsdf(const mf_t* m, v3 wo, v3 wi, v3 bc)
+{
+    float co = fabsf(wo.z), ci = fabsf(wi.z);
+    if (co == 0.0f || ci == 0.0f) return V(0, 0, 0);
+    v3 wh = add(wo, wi);
+    float l = len(wh);
+    if (l == 0.0f) return V(0, 0, 0);
+    wh = divs(wh, l);
+    v3 f0 = m->f0 < 0.0f ? bc : V(m->f0, m->f0, m->f0);
+    v3 f = schlick(f0, wi, wh);
+    float d = mf_d(m, wh);
+    float g2 = mf_g2(m, wo, wi);
+    return divs(muls(muls(f, d), g2), 4.0f * co * ci);
+}
+static float mf_pdf(const mf_t* m, v3 wo, v3 wi)
+{
+    v3 wh = add(wo, wi);
+    float l = len(wh);
+    if (l == 0.0f) return 0.0f;
+    wh = divs(wh, l);
+    return mf_d(m, wh) / (4.0f * dot(wo, wh));
+}
+
+/* ---- materials (lambertian.cpp, dielectric.cpp, conductor.cpp) ---- */
+static int lam_sample(v3 wo, v2 u, v3 bc, v3* est, v3* wi, float* pdf)
+{
+    (void)wo;
+    *wi = cosine_hemisphere(u, pdf);
+    if (*pdf == 0.0f) return 0;
+    *est = bc;
+    return 1;
+}
+static v3 lam_bsdf(v3 bc) { return divs(bc, PI_F); }
+static float lam_pdf(v3 wi) { return fmax_std(wi.z, 0.0f) / PI_F; }
+
+typedef struct {
+    int kind;      /* ORC_MAT_DIELECTRIC or ORC_MAT_CONDUCTOR */
+    v3 bc;
+    mf_t mf;
+    int nbsdf;     /* dielectric: 1 (Lambert) or 2 (Lambert + coat) */
+} mat_t;
+
+static void mat_init(mat_t* m, const orc_material* src)
+{
+    m->kind = src->kind;
+    m->bc = ld3(src->base_color);
+    /* microfacet.h:9-12: alpha clamped to >= 0.001 */
+    m->mf.ax = fmax_std(0.001f, src->alpha[0]);
+    m->mf.ay = fmax_std(0.001f, src->alpha[1]);
+    if (src->kind == ORC_MAT_CONDUCTOR) {
+        m->mf.f0 = -1.0f;
+        m->nbsdf = 1;
+    } else {
+        /* dielectric.cpp:10-11, 24: F0 = 0.16 * 0.5^2; coat only when dot(alpha, alpha) < 1 */
+        const float refl = 0.5f;
+        m->mf.f0 = 0.16f * refl * refl;
+        m->nbsdf = (src->alpha[0] * src->alpha[0] + src->alpha[1] * src->alpha[1] < 1.0f) ? 2 : 1;
+    }
+}
+
+static int bsdf_i_sample(const mat_t* m, int i, v3 wo, v2 u, v3* est, v3* wi, float* pdf)
+{
+    return i == 0 ? lam_sample(wo, u, m->bc, est, wi, pdf) : mf_sample(&m->mf, wo, u, m->bc, est, wi, pdf);
+}
+static v3 bsdf_i_eval(const mat_t* m, int i, v3 wo, v3 wi)
+{
+    return i == 0 ? lam_bsdf(m->bc) : mf_bsdf(&m->mf, wo, wi, m->bc);
+}
+static float bsdf_i_pdf(const mat_t* m, int i, v3 wo, v3 wi)
+{
+    return i == 0 ? lam_pdf(wi) : mf_pdf(&m->mf, wo, wi);
+}
+
+int orc_mat_sample(const orc_material* src, const float* wo3, const float* u2, float* est3,
+                   float* wi3, float* pdf_out)
+{
+    mat_t m;
+    mat_init(&m, src);
+    v3 wo = ld3(wo3), est, wi;
+    v2 u = { u2[0], u2[1] };
+    float pdf;
+    if (m.kind == ORC_MAT_CONDUCTOR) {
+        if (!mf_sample(&m.mf, wo, u, m.bc, &est, &wi, &pdf)) return 0;
+    } else {
+        /* dielectric.cpp:29-67 */
+        uint8_t idx = (uint8_t)(uint32_t)(u.x * (float)m.nbsdf);
+        u.x = u.x * (float)m.nbsdf - (float)idx;
+        if (!bsdf_i_sample(&m, idx, wo, u, &est, &wi, &pdf)) return 0;
+        float cos_t = wi.z;
+        if (cos_t <= 0.0f) return 0;
+        v3 f = divs(muls(est, pdf), cos_t);
+        for (int i = 0; i < m.nbsdf; ++i) {
+            if (i != idx) {
+                pdf += bsdf_i_pdf(&m, i, wo, wi);
+                f = add(f, bsdf_i_eval(&m, i, wo, wi));
+            }
+        }
+        pdf /= (float)m.nbsdf;
+        est = divs(muls(f, cos_t), pdf);
+    }
+    est3[0] = est.x; est3[1] = est.y; est3[2] = est.z;
+    wi3[0] = wi.x; wi3[1] = wi.y; wi3[2] = wi.z;
+    *pdf_out = pdf;
+    return 1;
+}
+
+void orc_mat_eval(const orc_material* src, const float* wo3, const float* wi3, float* f3,
+                  float* pdf_out)
+{
+    mat_t m;
+    mat_init(&m, src);
+    v3 wo = ld3(wo3), wi = ld3(wi3), f;
+    float pdf;
+    if (m.kind == ORC_MAT_CONDUCTOR) {
+        f = mf_bsdf(&m.mf, wo, wi, m.bc);
+        pdf = mf_pdf(&m.mf, wo, wi);
+    } else {
+        f = V(0, 0, 0);
+        float ps = 0.0f;
+        for (int i = 0; i < m.nbsdf; ++i) {
+            f = add(f, bsdf_i_eval(&m, i, wo, wi));
+            ps += bsdf_i_pdf(&m, i, wo, wi);
+        }
+        pdf = ps / (float)m.nbsdf;
+    }
+    f3[0] = f.x; f3[1] = f.y; f3[2] = f.z;
+    *pdf_out = pdf;
+}
+
+/* ---- lights (disk_light.cpp, point_light.cpp; quad and triangle lights added) ---- */
+typedef struct {
+    int kind;
+    v3 pos, n, b, t;     /* disk: centre, normal, basis; quad/tri: v0 and normal */
+    v3 e0, e1;           /* quad: edges; triangle: v1 - v0, v2 - v0 */
+    float radius, area;
+    v3 power, le;        /* le: emitted radiance (area) or intensity (point) */
+    v3 lo, hi;           /* bounding box */
+} light_t;
+
+static void light_init(light_t* L, const orc_light* s)
+{
+    memset(L, 0, sizeof(*L));
+    L->kind = s->kind;
+    v3 col = ld3(s->color);
+    L->pos = ld3(s->p0);
+    if (s->kind == ORC_LIGHT_DISK) {
+        /* disk_light.cpp:9-28 */
+        L->power = muls(col, s->power);
+        L->n = ld3(s->normal);
+        L->radius = s->radius;
+        get_basis(L->n, &L->b, &L->t);
+        L->area = L->radius * L->radius * PI_F;
+        L->le = divs(L->power, L->area * PI_F);
+        v3 c[4] = { add(L->pos, muls(L->b, L->radius)), sub(L->pos, muls(L->b, L->radius)),
+                    add(L->pos, muls(L->t, L->radius)), sub(L->pos, muls(L->t, L->radius)) };
+        L->lo = V(FLT_MAX, FLT_MAX, FLT_MAX);
+        L->hi = V(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+        for (int i = 0; i < 4; ++i) {
+            L->lo = V(fmin_std(L->lo.x, c[i].x), fmin_std(L->lo.y, c[i].y), fmin_std(L->lo.z, c[i].z));
+            L->hi = V(fmax_std(L->hi.x, c[i].x), fmax_std(L->hi.y, c[i].y), fmax_std(L->hi.z, c[i].z));
+        }
+    } else if (s->kind == ORC_LIGHT_POINT) {
+        /* point_light.cpp:15-20 */
+        L->power = muls(col, s->power);
+        L->le = divs(L->power, 4.0f * PI_F);
+        L->lo = L->pos;
+        L->hi = L->pos;
+    } else {
+        /* quad: p0 + a e0 + b e1, (a, b) in [0,1]^2; triangle: p0 + b1 e0 + b2 e1.
+         * One-sided, emitting along normalize(e0 x e1); radiance = color. */
+        L->e0 = sub(ld3(s->p1), L->pos);
+        L->e1 = sub(ld3(s->p2), L->pos);
+        v3 c = cross(L->e0, L->e1);
+        float cl = len(c);
+        L->n = divs(c, cl);
+        L->area = s->kind == ORC_LIGHT_QUAD ? cl : 0.5f * cl;
+        L->le = col;
+        L->power = muls(col, L->area * PI_F);
+        v3 pts[4] = { L->pos, add(L->pos, L->e0), add(L->pos, L->e1), add(add(L->pos, L->e0), L->e1) };
+        int np = s->kind == ORC_LIGHT_QUAD ? 4 : 3;
+        L->lo = V(FLT_MAX, FLT_MAX, FLT_MAX);
+        L->hi = V(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+        for (int i = 0; i < np; ++i) {
+            L->lo = V(fmin_std(L->lo.x, pts[i].x), fmin_std(L->lo.y, pts[i].y), fmin_std(L->lo.z, pts[i].z));
+            L->hi = V(fmax_std(L->hi.x, pts[i].x), fmax_std(L->hi.y, pts[i].y), fmax_std(L->hi.z, pts[i].z));
+        }
+    }
+}
+
+static int light_is_delta(const light_t* L) { return L->kind == ORC_LIGHT_POINT; }
+
+/* Light::sample: returns emitted radiance, direction, distance, pdf (solid angle). */
+static int light_sample(const light_t* L, v2 u, v3 hit, v3* le, v3* wi, float* dist, float* pdf)
+{
+    if (L->kind == ORC_LIGHT_POINT) {
+        v3 w = sub(L->pos, hit);
+        float d2 = dot(w, w);
+        if (d2 == 0.0f) return 0;
+        float d = sqrtf(d2);
+        *wi = divs(w, d);
+        *le = divs(L->le, d2);
+        *dist = d;
+        *pdf = 1.0f;
+        return 1;
+    }
+    v3 p;
+    if (L->kind == ORC_LIGHT_DISK) {
+        v2 ds = concentric_disk(u);
+        p = add(L->pos, muls(add(muls(L->b, ds.x), muls(L->t, ds.y)), L->radius));
+    } else if (L->kind == ORC_LIGHT_QUAD) {
+        p = add(add(L->pos, muls(L->e0, u.x)), muls(L->e1, u.y));
+    } else {
+        float su = sqrtf(u.x);
+        float b1 = 1.0f - su, b2 = u.y * su;
+        p = add(add(L->pos, muls(L->e0, b1)), muls(L->e1, b2));
+    }
+    v3 w = sub(p, hit);
+    float d = len(w);
+    if (d == 0.0f) return 0;
+    w = divs(w, d);
+    float c = fmax_std(0.0f, dot(neg(w), L->n));
+    if (c == 0.0f) return 0;
+    *pdf = (d * d) / (L->area * c);
+    *le = L->le;
+    *wi = w;
+    *dist = d;
+    return 1;
+}
+
+/* Light::computePdf: solid-angle pdf of direction wi from hit, and the distance. */
+static float light_pdf(const light_t* L, v3 hit, v3 wi, float* dist)
+{
+    *dist = 0.0f;
+    if (L->kind == ORC_LIGHT_POINT) return 0.0f;
+    float c = fmax_std(0.0f, dot(neg(wi), L->n));
+    if (c == 0.0f) return 0.0f;
+    float d = dot(sub(hit, L->pos), L->n) / c;
+    if (d < 0.0f) return 0.0f;
+    v3 p = add(hit, muls(wi, d));
+    int inside;
+    if (L->kind == ORC_LIGHT_DISK) {
+        v3 q = sub(p, L->pos);
+        inside = dot(q, q) < L->radius * L->radius;
+    } else {
+        /* coordinates of p in the (e0, e1) frame */
+        v3 q = sub(p, L->pos);
+        float e00 = dot(L->e0, L->e0), e01 = dot(L->e0, L->e1), e11 = dot(L->e1, L->e1);
+        float q0 = dot(q, L->e0), q1 = dot(q, L->e1);
+        float det = e00 * e11 - e01 * e01;
+        float a = (q0 * e11 - q1 * e01) / det;
+        float b = (q1 * e00 - q0 * e01) / det;
+        inside = L->kind == ORC_LIGHT_QUAD ? (a >= 0.0f && a <= 1.0f && b >= 0.0f && b <= 1.0f)
+                                           : (a >= 0.0f && b >= 0.0f && a + b <= 1.0f);
+    }
+    if (!inside) return 0.0f;
+    *dist = d;
+    return (d * d) / (L->area * c);
+}
+
+/* ---- light samplers ---- */
+typedef struct {
+    float power;
+    v3 lo, hi;
+    int child[2];   /* -1 for a leaf */
+    int light;      /* leaf: light index */
+    int parent;
+} lnode_t;
+
+typedef struct {
+    int kind;
+    uint32_t n;
+    light_t* lights;
+    /* power sampler: alias table (alias_table.cpp:8-81) */
+    float* p;
+    float* q;
+    uint32_t* alias;
+    /* bvh sampler */
+    lnode_t* nodes;
+    int node_count;
+    int* leaf_of_light;
+} lsampler_t;
+
+static void alias_build(lsampler_t* S, const float* w, uint32_t n)
+{
+    S->p = (float*)malloc(n * sizeof(float));
+    S->q = (float*)malloc(n * sizeof(float));
+    S->alias = (uint32_t*)malloc(n * sizeof(uint32_t));
+    float* sp = (float*)malloc(n * sizeof(float));
+    uint32_t* small = (uint32_t*)malloc(n * sizeof(uint32_t));
+    uint32_t* large = (uint32_t*)malloc(n * sizeof(uint32_t));
+    uint32_t ns = 0, nl = 0;
+    double acc = 0.0;
+    for (uint32_t i = 0; i < n; ++i) acc = acc + (double)w[i];
+    float sum = (float)acc;
+    for (uint32_t i = 0; i < n; ++i) {
+        S->p[i] = w[i] / sum;
+        sp[i] = S->p[i] * (float)n;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        if (sp[i] < 1.0f) small[ns++] = i;
+        else large[nl++] = i;
+    }
+    while (ns && nl) {
+        uint32_t si = small[--ns], li = large[--nl];
+        S->q[si] = sp[si];
+        S->alias[si] = li;
+        sp[li] += sp[si] - 1.0f;
+        if (sp[li] < 1.0f) small[ns++] = li;
+        else large[nl++] = li;
+    }
+    while (ns) { uint32_t i = small[--ns]; S->q[i] = 1.0f; S->alias[i] = UINT32_MAX; }
+    while (nl) { uint32_t i = large[--nl]; S->q[i] = 1.0f; S->alias[i] = UINT32_MAX; }
+    free(sp);
+    free(small);
+    free(large);
+}
+
+static inline v3 box_center(v3 lo, v3 hi) { return muls(add(lo, hi), 0.5f); }
+
+/* LightNode::importance (bvh_light_sampler.cpp:24-51) */
+static float node_importance(const lnode_t* nd, v3 p, v3 n)
+{
+    v3 w = sub(box_center(nd->lo, nd->hi), p);
+    float d2 = dot(w, w);
+    float imp = nd->power / fmax_std(d2, 0.5f * len(sub(nd->hi, nd->lo)));
+    float br = 0.5f * len(sub(nd->hi, nd->lo));
+    float br2 = br * br;
+    if (d2 > 0.0f && d2 > br2) {
+        float sinB2 = br2 / d2;
+        float sinB = sqrtf(sinB2);
+        float cosB = sqrtf(1.0f - sinB2);
+        w = normalize(w);
+        float cosI = fabsf(dot(w, n));
+        float sinI = sqrtf(1 - cosI * cosI);
+        float f = cosI <= cosB ? cosI * cosB + sinI * sinB : 1.0f;
+        imp *= f;
+    }
+    return imp;
+}
+
+static inline float maxc(v3 v) { return fmax_std(v.x, fmax_std(v.y, v.z)); }
+static inline float compf(v3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+
+/* createLightNode / splitLightCluster (bvh_light_sampler.cpp:54-240) */
+static int light_node_build(lsampler_t* S, int* ids, uint32_t n, int parent)
+{
+    int me = S->node_count++;
+    lnode_t* nd = &S->nodes[me];
+    nd->power = 0.0f;
+    nd->lo = V(FLT_MAX, FLT_MAX, FLT_MAX);
+    nd->hi = V(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+    nd->child[0] = nd->child[1] = -1;
+    nd->light = -1;
+    nd->parent = parent;
+    for (uint32_t i = 0; i < n; ++i) {
+        const light_t* L = &S->lights[ids[i]];
+        nd->power += len(L->power);
+        /* merge(bbox, node bbox) */
+        nd->lo = V(fmin_std(L->lo.x, nd->lo.x), fmin_std(L->lo.y, nd->lo.y), fmin_std(L->lo.z, nd->lo.z));
+        nd->hi = V(fmax_std(L->hi.x, nd->hi.x), fmax_std(L->hi.y, nd->hi.y), fmax_std(L->hi.z, nd->hi.z));
+    }
+    if (n == 1) {
+        nd->light = ids[0];
+        S->leaf_of_light[ids[0]] = me;
+        return me;
+    }
+    v3 diag = sub(nd->hi, nd->lo);
+    int have_split = 0, sdim = 0, sidx = 0;
+    float smn = 0, smx = 0;
+    if (n > 2) {
+        v3 clo = V(FLT_MAX, FLT_MAX, FLT_MAX), chi = V(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+        for (uint32_t i = 0; i < n; ++i) {
+            const light_t* L = &S->lights[ids[i]];
+            v3 c = box_center(L->lo, L->hi);
+            clo = V(fmin_std(clo.x, c.x), fmin_std(clo.y, c.y), fmin_std(clo.z, c.z));
+            chi = V(fmax_std(chi.x, c.x), fmax_std(chi.y, c.y), fmax_std(chi.z, c.z));
+        }
+        if (maxc(sub(chi, clo)) != 0.0f) {
+            enum { SPL = 4, BC = SPL + 1 };
+            have_split = 1;
+            float min_cost = FLT_MAX;
+            for (int d = 0; d < 3; ++d) {
+                float mn = compf(clo, d), mx = compf(chi, d);
+                if (mn == mx) continue;
+                float kr = maxc(diag) / fmax_std(compf(diag, d), 0.001f);
+                float bp[BC];
+                v3 blo[BC], bhi[BC];
+                for (int b = 0; b < BC; ++b) {
+                    bp[b] = 0.0f;
+                    blo[b] = V(FLT_MAX, FLT_MAX, FLT_MAX);
+                    bhi[b] = V(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+                }
+                for (uint32_t i = 0; i < n; ++i) {
+                    const light_t* L = &S->lights[ids[i]];
+                    float c = compf(box_center(L->lo, L->hi), d);
+                    uint8_t bi = (uint8_t)((float)BC * (c - mn) / (mx - mn));
+                    if (bi > BC - 1) bi = BC - 1;
+                    bp[bi] += len(L->power);
+                    blo[bi] = V(fmin_std(blo[bi].x, L->lo.x), fmin_std(blo[bi].y, L->lo.y), fmin_std(blo[bi].z, L->lo.z));
+                    bhi[bi] = V(fmax_std(bhi[bi].x, L->hi.x), fmax_std(bhi[bi].y, L->hi.y), fmax_std(bhi[bi].z, L->hi.z));
+                }
+                float costs[SPL] = { 0, 0, 0, 0 };
+                float ap = 0.0f;
+                v3 alo = V(FLT_MAX, FLT_MAX, FLT_MAX), ahi = V(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+                for (int s = 0; s < SPL; ++s) {
+                    ap += bp[s];
+                    alo = V(fmin_std(alo.x, blo[s].x), fmin_std(alo.y, blo[s].y), fmin_std(alo.z, blo[s].z));
+                    ahi = V(fmax_std(ahi.x, bhi[s].x), fmax_std(ahi.y, bhi[s].y), fmax_std(ahi.z, bhi[s].z));
+                    v3 l3 = sub(ahi, alo);
+                    float sa = alo.x < FLT_MAX ? (l3.x * l3.y + l3.y * l3.z + l3.z * l3.x) * 2.0f : 0.0f;
+                    costs[s] += ap * sa;
+                }
+                ap = 0.0f;
+                alo = V(FLT_MAX, FLT_MAX, FLT_MAX);
+                ahi = V(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+                for (int s = SPL - 1; s >= 0; --s) {
+                    ap += bp[s + 1];
+                    alo = V(fmin_std(alo.x, blo[s + 1].x), fmin_std(alo.y, blo[s + 1].y), fmin_std(alo.z, blo[s + 1].z));
+                    ahi = V(fmax_std(ahi.x, bhi[s + 1].x), fmax_std(ahi.y, bhi[s + 1].y), fmax_std(ahi.z, bhi[s + 1].z));
+                    v3 l3 = sub(ahi, alo);
+                    float sa = alo.x < FLT_MAX ? (l3.x * l3.y + l3.y * l3.z + l3.z * l3.x) * 2.0f : 0.0f;
+                    costs[s] += ap * sa;
+                }
+                for (int s = 0; s < SPL; ++s) {
+                    costs[s] *= kr;
+                    if (costs[s] < min_cost) {
+                        min_cost = costs[s];
+                        sdim = d;
+                        sidx = s;
+                        smn = mn;
+                        smx = mx;
+                    }
+                }
+            }
+        }
+    }
+    int* sub0 = (int*)malloc(n * sizeof(int));
+    int* sub1 = (int*)malloc(n * sizeof(int));
+    uint32_t n0 = 0, n1 = 0;
+    if (have_split) {
+        /* Split::isBelow with binCount 5 (the default Split has binCount 0) */
+        for (uint32_t i = 0; i < n; ++i) {
+            const light_t* L = &S->lights[ids[i]];
+            float c = compf(box_center(L->lo, L->hi), sdim);
+            uint8_t bi = (uint8_t)((float)5 * (c - smn) / (smx - smn));
+            if (bi > 4) bi = 4;
+            if (bi <= sidx) sub0[n0++] = ids[i];
+            else sub1[n1++] = ids[i];
+        }
+    } else {
+        uint32_t mid = n / 2;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (i < mid) sub0[n0++] = ids[i];
+            else sub1[n1++] = ids[i];
+        }
+    }
+    int c0 = light_node_build(S, sub0, n0, me);
+    int c1 = light_node_build(S, sub1, n1, me);
+    S->nodes[me].child[0] = c0;
+    S->nodes[me].child[1] = c1;
+    free(sub0);
+    free(sub1);
+    return me;
+}
+
+struct orc_scene {
+    oracle_bvh* bvh;
+    uint32_t nt;
+    uint32_t* tri_mat;      /* per triangle material index */
+    mat_t* mats;
+    uint32_t nmats;
+    float* tri_ng;          /* per triangle geometric normal (bvh_common.cpp:116) */
+    float* tri_ns;          /* optional per-vertex-interpolated normals: 9 per triangle */
+    lsampler_t ls;
+    orc_camera cam;
+    float bg[3];
+    orc_render_params rp;
+    v3 view[3];
+    float film_w, film_h;
+};
+
+static void camera_setup(orc_scene* S)
+{
+    /* camera.cpp:13-29, 86-96 */
+    v3 eye = ld3(S->cam.eye), tgt = ld3(S->cam.target), up = ld3(S->cam.up);
+    S->film_h = 1.0f * tanf(0.5f * S->cam.fovy);
+    S->film_w = S->film_h * S->cam.aspect;
+    v3 z = normalize(sub(tgt, eye));
+    v3 x = normalize(cross(z, up));
+    v3 y = cross(x, z);
+    S->view[0] = x;
+    S->view[1] = y;
+    S->view[2] = z;
+}
+
+/* Camera::generateRay (camera.cpp:73-85) */
+static void camera_ray(const orc_scene* S, float x, float y, v3* o, v3* d)
+{
+    float fx = (x - 0.5f) * S->film_w;
+    float fy = (0.5f - y) * S->film_h;
+    v3 v = add(add(muls(S->view[0], fx), muls(S->view[1], fy)), muls(S->view[2], 1.0f));
+    *o = ld3(S->cam.eye);
+    *d = normalize(v);
+}
+
+void orc_camera_ray(const orc_scene* S, float x, float y, float* o3, float* d3)
+{
+    v3 o, d;
+    camera_ray(S, x, y, &o, &d);
+    o3[0] = o.x; o3[1] = o.y; o3[2] = o.z;
+    d3[0] = d.x; d3[1] = d.y; d3[2] = d.z;
+}
+
+orc_scene* orc_scene_create(const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                            const uint32_t* tri_mat, const orc_material* mats, uint32_t nmats,
+                            const float* normals, const orc_light* lights, uint32_t nl,
+                            int light_sampler, const orc_camera* cam, const float* bg3)
+{
+    orc_scene* S = (orc_scene*)calloc(1, sizeof(orc_scene));
+    S->bvh = oracle_bvh_create(xyz, nv, idx, nt, 8);
+    S->nt = nt;
+    S->tri_mat = (uint32_t*)malloc((nt ? nt : 1) * sizeof(uint32_t));
+    memcpy(S->tri_mat, tri_mat, nt * sizeof(uint32_t));
+    S->nmats = nmats;
+    S->mats = (mat_t*)malloc((nmats ? nmats : 1) * sizeof(mat_t));
+    for (uint32_t i = 0; i < nmats; ++i) mat_init(&S->mats[i], &mats[i]);
+    S->tri_ng = (float*)malloc((nt ? nt : 1) * 3 * sizeof(float));
+    for (uint32_t i = 0; i < nt; ++i) {
+        v3 v0 = ld3(xyz + 3 * idx[3 * i]), v1 = ld3(xyz + 3 * idx[3 * i + 1]), v2 = ld3(xyz + 3 * idx[3 * i + 2]);
+        v3 ng = normalize(cross(sub(v1, v0), sub(v2, v0)));
+        S->tri_ng[3 * i] = ng.x; S->tri_ng[3 * i + 1] = ng.y; S->tri_ng[3 * i + 2] = ng.z;
+    }
+    if (normals) {
+        S->tri_ns = (float*)malloc((nt ? nt : 1) * 9 * sizeof(float));
+        for (uint32_t i = 0; i < nt; ++i)
+            for (int k = 0; k < 3; ++k)
+                for (int a = 0; a < 3; ++a) S->tri_ns[9 * i + 3 * k + a] = normals[3 * idx[3 * i + k] + a];
+    }
+    S->ls.kind = light_sampler;
+    S->ls.n = nl;
+    S->ls.lights = (light_t*)malloc((nl ? nl : 1) * sizeof(light_t));
+    for (uint32_t i = 0; i < nl; ++i) light_init(&S->ls.lights[i], &lights[i]);
+    if (nl && light_sampler == ORC_LS_POWER) {
+        float* w = (float*)malloc(nl * sizeof(float));
+        for (uint32_t i = 0; i < nl; ++i) w[i] = len(S->ls.lights[i].power);
+        alias_build(&S->ls, w, nl);
+        free(w);
+    }
+    if (nl && light_sampler == ORC_LS_BVH) {
+        S->ls.nodes = (lnode_t*)malloc(2 * nl * sizeof(lnode_t));
+        S->ls.leaf_of_light = (int*)malloc(nl * sizeof(int));
+        int* ids = (int*)malloc(nl * sizeof(int));
+        for (uint32_t i = 0; i < nl; ++i) ids[i] = (int)i;
+        light_node_build(&S->ls, ids, nl, -1);
+        free(ids);
+    }
+    if (cam) S->cam = *cam;
+    camera_setup(S);
+    S->bg[0] = bg3 ? bg3[0] : 0.0f;
+    S->bg[1] = bg3 ? bg3[1] : 0.0f;
+    S->bg[2] = bg3 ? bg3[2] : 0.0f;
+    return S;
+}
+
+void orc_scene_destroy(orc_scene* S)
+{
+    if (!S) return;
+    oracle_bvh_destroy(S->bvh);
+    free(S->tri_mat);
+    free(S->mats);
+    free(S->tri_ng);
+    free(S->tri_ns);
+    free(S->ls.lights);
+    free(S->ls.p);
+    free(S->ls.q);
+    free(S->ls.alias);
+    free(S->ls.nodes);
+    free(S->ls.leaf_of_light);
+    free(S);
+}
+
+/* LightSampler::sample: returns light index or -1, pdf */
+static int ls_sample(const lsampler_t* L, float u, v3 hit, v3 ns, float* pdf)
+{
+    if (L->n == 0) return -1;
+    if (L->kind == ORC_LS_UNIFORM) {
+        /* uniform_light_sampler.cpp:15-27 */
+        size_t i = (size_t)(u * (float)L->n);
+        if (i > L->n - 1) i = L->n - 1;
+        *pdf = 1.0f / (float)L->n;
+        return (int)i;
+    }
+    if (L->kind == ORC_LS_POWER) {
+        /* alias_table.cpp:65-81 */
+        uint32_t bin = (uint32_t)(u * (float)L->n);
+        if (bin > L->n - 1) bin = L->n - 1;
+        float up = fmin_std(u * (float)L->n - (float)bin, ONE_MINUS_EPS);
+        uint32_t idx = up < L->q[bin] ? bin : L->alias[bin];
+        *pdf = L->p[idx];
+        return (int)idx;
+    }
+    /* BvhLightSampler::sample (bvh_light_sampler.cpp:267-319) */
+    int node = 0;
+    float p = 1.0f;
+    for (;;) {
+        const lnode_t* nd = &L->nodes[node];
+        if (nd->child[0] >= 0 && nd->child[1] >= 0) {
+            float w0 = node_importance(&L->nodes[nd->child[0]], hit, ns);
+            float w1 = node_importance(&L->nodes[nd->child[1]], hit, ns);
+            if (w0 == 0.0f && w1 == 0.0f) return -1;
+            float up = (w0 + w1) * u;
+            int idx = up < w0 ? 0 : 1;
+            node = nd->child[idx];
+            p *= (idx == 0 ? w0 : w1) / (w0 + w1);
+            if (idx == 0) u = fmin_std(up / w0, ONE_MINUS_EPS);
+            else u = fmin_std((up - w0) / w1, ONE_MINUS_EPS);
+        } else {
+            if (nd->light >= 0 && node_importance(nd, hit, ns) > 0.0f) {
+                *pdf = p;
+                return nd->light;
+            }
+            return -1;
+        }
+    }
+}
+
+/* Probability that ls_sample picks light `li` (0 when it cannot). */
+static float ls_pdf(const lsampler_t* L, int li, v3 hit, v3 ns)
+{
+    if (L->n == 0) return 0.0f;
+    if (L->kind == ORC_LS_UNIFORM) return 1.0f / (float)L->n;
+    if (L->kind == ORC_LS_POWER) return L->p[li];
+    int node = L->leaf_of_light[li];
+    if (!(node_importance(&L->nodes[node], hit, ns) > 0.0f)) return 0.0f;
+    float p = 1.0f;
+    while (L->nodes[node].parent >= 0) {
+        const lnode_t* par = &L->nodes[L->nodes[node].parent];
+        float w0 = node_importance(&L->nodes[par->child[0]], hit, ns);
+        float w1 = node_importance(&L->nodes[par->child[1]], hit, ns);
+        if (w0 == 0.0f && w1 == 0.0f) return 0.0f;
+        p *= (par->child[0] == node ? w0 : w1) / (w0 + w1);
+        node = L->nodes[node].parent;
+    }
+    return p;
+}
+
+int orc_light_sampler_sample(const orc_scene* S, float u, const float* hit, const float* ns,
+                             float* pdf)
+{
+    return ls_sample(&S->ls, u, ld3(hit), ld3(ns), pdf);
+}
+
+float orc_light_sampler_pdf(const orc_scene* S, int light, const float* hit, const float* ns)
+{
+    return ls_pdf(&S->ls, light, ld3(hit), ld3(ns));
+}
+
+int orc_light_sample(const orc_scene* S, int light, const float* u2, const float* hit, float* le3,
+                     float* wi3, float* dist, float* pdf)
+{
+    v3 le, wi;
+    v2 u = { u2[0], u2[1] };
+    if (!light_sample(&S->ls.lights[light], u, ld3(hit), &le, &wi, dist, pdf)) return 0;
+    le3[0] = le.x; le3[1] = le.y; le3[2] = le.z;
+    wi3[0] = wi.x; wi3[1] = wi.y; wi3[2] = wi.z;
+    return 1;
+}
+
+float orc_light_pdf(const orc_scene* S, int light, const float* hit, const float* wi3, float* dist)
+{
+    return light_pdf(&S->ls.lights[light], ld3(hit), ld3(wi3), dist);
+}
+
+/* ---- integrator (alpine.cpp:290-432) ---- */
+typedef struct {
+    uint32_t prim;
+    float t, b0, b1;
+} hit_t;
+
+static int closest(const orc_scene* S, v3 o, v3 d, hit_t* h)
+{
+    float org[3] = { o.x, o.y, o.z }, dir[3] = { d.x, d.y, d.z };
+    oracle_bvh_intersect(S->bvh, org, dir, 1, &h->prim, &h->t, &h->b0, &h->b1, 1);
+    return h->prim != 0xFFFFFFFFu;
+}
+
+static int occluded(const orc_scene* S, v3 o, v3 d, float tfar)
+{
+    float org[3] = { o.x, o.y, o.z }, dir[3] = { d.x, d.y, d.z };
+    uint8_t occ = 0;
+    oracle_bvh_occluded(S->bvh, org, dir, &tfar, 1, &occ, 1);
+    return occ;
+}
+
+static int mat_sample_local(const mat_t* m, v3 wo, v2 u, v3* est, v3* wi, float* pdf)
+{
+    if (m->kind == ORC_MAT_CONDUCTOR) return mf_sample(&m->mf, wo, u, m->bc, est, wi, pdf);
+    uint8_t idx = (uint8_t)(uint32_t)(u.x * (float)m->nbsdf);
+    u.x = u.x * (float)m->nbsdf - (float)idx;
+    if (!bsdf_i_sample(m, idx, wo, u, est, wi, pdf)) return 0;
+    float cos_t = wi->z;
+    if (cos_t <= 0.0f) return 0;
+    v3 f = divs(muls(*est, *pdf), cos_t);
+    for (int i = 0; i < m->nbsdf; ++i) {
+        if (i != idx) {
+            *pdf += bsdf_i_pdf(m, i, wo, *wi);
+            f = add(f, bsdf_i_eval(m, i, wo, *wi));
+        }
+    }
+    *pdf /= (float)m->nbsdf;
+    *est = divs(muls(f, cos_t), *pdf);
+    return 1;
+}
+
+static void mat_eval_local(const mat_t* m, v3 wo, v3 wi, v3* f, float* pdf)
+{
+    if (m->kind == ORC_MAT_CONDUCTOR) {
+        *f = mf_bsdf(&m->mf, wo, wi, m->bc);
+        *pdf = mf_pdf(&m->mf, wo, wi);
+        return;
+    }
+    v3 s = V(0, 0, 0);
+    float ps = 0.0f;
+    for (int i = 0; i < m->nbsdf; ++i) {
+        s = add(s, bsdf_i_eval(m, i, wo, wi));
+        ps += bsdf_i_pdf(m, i, wo, wi);
+    }
+    *f = s;
+    *pdf = ps / (float)m->nbsdf;
+}
+
+static float mat_pdf_local(const mat_t* m, v3 wo, v3 wi)
+{
+    v3 f;
+    float p;
+    mat_eval_local(m, wo, wi, &f, &p);
+    return p;
+}
+
+/* estimateDirectIllumination (alpine.cpp:370-432) */
+static v3 direct(const orc_scene* S, uint32_t* st, v3 hit, v3 wo, v3 ns, v3 ss, v3 ts,
+                 const mat_t* m)
+{
+    v3 rad = V(0, 0, 0);
+    if (S->ls.n == 0) return rad;
+    float lpdf_sel;
+    int li = ls_sample(&S->ls, orc_sampler_get1d(st), hit, ns, &lpdf_sel);
+    if (li < 0) return rad;
+    const light_t* L = &S->ls.lights[li];
+    v3 off = muls(ns, RAY_OFFSET);
+    {
+        v3 le, wiw;
+        float dist, pdf;
+        if (light_sample(L, get2d(st), hit, &le, &wiw, &dist, &pdf)) {
+            v3 wi = to_local(wiw, ss, ts, ns);
+            float bpdf = mat_pdf_local(m, wo, wi);
+            if (bpdf > 0.0f && !occluded(S, add(hit, off), wiw, dist)) {
+                v3 f;
+                float dp;
+                mat_eval_local(m, wo, wi, &f, &dp);
+                float c = fmax_std(0.0f, dot(wiw, ns));
+                float w = !light_is_delta(L) ? power_heuristic(pdf, bpdf) : 1.0f;
+                rad = add(rad, divs(muls(mulv(muls(le, w), f), c), pdf));
+            }
+        }
+    }
+    if (!light_is_delta(L)) {
+        v3 est, wi;
+        float pdf;
+        if (mat_sample_local(m, wo, get2d(st), &est, &wi, &pdf)) {
+            v3 ld = to_world(wi, ss, ts, ns);
+            float dist;
+            float lp = light_pdf(L, hit, ld, &dist);
+            if (lp > 0.0f && !occluded(S, add(hit, off), ld, dist)) {
+                float w = power_heuristic(pdf, lp);
+                rad = add(rad, mulv(muls(L->le, w), est));
+            }
+        }
+    }
+    return divs(rad, lpdf_sel);
+}
+
+/* One sample of one pixel (alpine.cpp:305-364), radiance returned in out3. */
+static void trace_sample(const orc_scene* S, uint32_t* st, uint32_t x, uint32_t y, v3* out)
+{
+    const orc_render_params* rp = &S->rp;
+    v2 j = get2d(st);
+    v3 o, d;
+    camera_ray(S, (x + j.x) / (float)rp->width, (y + j.y) / (float)rp->height, &o, &d);
+    v3 T = V(1, 1, 1), L = V(0, 0, 0);
+    for (uint32_t depth = 0; depth < rp->max_depth; ++depth) {
+        hit_t h;
+        if (!closest(S, o, d, &h)) {
+            L = add(L, mulv(T, ld3(S->bg)));
+            break;
+        }
+        v3 hit = add(o, muls(d, h.t));
+        v3 ng = ld3(S->tri_ng + 3 * h.prim);
+        v3 ns = ng;
+        if (S->tri_ns) {
+            /* mesh.cpp:57-102: b0 = 1 - b1 - b2 with (b1, b2) the hit barycentrics */
+            float b1 = h.b0, b2 = h.b1, b0 = 1.0f - b1 - b2;
+            const float* nv = S->tri_ns + 9 * h.prim;
+            ns = add(add(muls(ld3(nv), b0), muls(ld3(nv + 3), b1)), muls(ld3(nv + 6), b2));
+        }
+        v3 ss, ts;
+        get_basis(ns, &ss, &ts);
+        const mat_t* m = &S->mats[S->tri_mat[h.prim]];
+        v3 wo = to_local(neg(d), ss, ts, ns);
+        L = add(L, mulv(T, direct(S, st, hit, wo, ns, ss, ts, m)));
+        v3 est, wi;
+        float pdf;
+        if (!mat_sample_local(m, wo, get2d(st), &est, &wi, &pdf)) break;
+        v3 wiw = to_world(wi, ss, ts, ns);
+        if (!(dot(wiw, ng) > 0.0f)) break;
+        T = mulv(T, est);
+        if (rp->rr_depth && depth + 1 >= rp->rr_depth) {
+            /* Russian roulette (an option the reference lacks): survive with
+             * probability min(max component of T, 0.95). */
+            float q = fmin_std(maxc(T), 0.95f);
+            if (!(orc_sampler_get1d(st) < q)) break;
+            T = divs(T, q);
+        }
+        o = add(hit, muls(ng, RAY_OFFSET));
+        d = wiw;
+    }
+    *out = L;
+}
+
+typedef struct {
+    const orc_scene* S;
+    uint32_t* states;
+    double* accum;
+    uint32_t y0, y1, spp;
+} rjob_t;
+
+static void* render_rows(void* arg)
+{
+    rjob_t* J = (rjob_t*)arg;
+    const orc_render_params* rp = &J->S->rp;
+    for (uint32_t y = J->y0; y < J->y1; ++y)
+        for (uint32_t x = 0; x < rp->width; ++x) {
+            uint32_t i = y * rp->width + x;
+            uint32_t* st = J->states + 4 * (size_t)i;
+            for (uint32_t s = 0; s < J->spp; ++s) {
+                v3 L;
+                trace_sample(J->S, st, x, y, &L);
+                J->accum[3 * (size_t)i] += L.x;
+                J->accum[3 * (size_t)i + 1] += L.y;
+                J->accum[3 * (size_t)i + 2] += L.z;
+            }
+        }
+    return NULL;
+}
+
+/* Render spp samples per pixel into accum (double, width*height*3, added to).
+ * states: 4 uint32 per pixel (the per-pixel Sampler), advanced in place. */
+void orc_render(orc_scene* S, const orc_render_params* rp, uint32_t* states, double* accum,
+                uint32_t nthreads)
+{
+    S->rp = *rp;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    if (nthreads > rp->height) nthreads = rp->height;
+    pthread_t th[256];
+    rjob_t jobs[256];
+    uint32_t rows = (rp->height + nthreads - 1) / nthreads;
+    for (uint32_t k = 0; k < nthreads; ++k) {
+        jobs[k].S = S;
+        jobs[k].states = states;
+        jobs[k].accum = accum;
+        jobs[k].spp = rp->spp;
+        jobs[k].y0 = k * rows < rp->height ? k * rows : rp->height;
+        jobs[k].y1 = (k + 1) * rows < rp->height ? (k + 1) * rows : rp->height;
+        pthread_create(&th[k], NULL, render_rows, &jobs[k]);
+    }
+    for (uint32_t k = 0; k < nthreads; ++k) pthread_join(th[k], NULL);
+}

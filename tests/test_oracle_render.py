@@ -1,0 +1,160 @@
+"""Pins of the render-path oracle (oracle/render.c) against the reference's compiled TUs
+and against closed forms / statistics."""
+import numpy as np
+import pytest
+
+import oracle
+
+TRI = (np.float32([[0, 0, 0], [1, 0, 0], [0, 1, 0]]), np.uint32([[0, 1, 2]]))
+
+
+@pytest.fixture(scope="module")
+def rg():
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    return dict(np.load(os.path.join(here, "golden", "render_small.npz")))
+
+
+def lights_from_spec(spec):
+    out = []
+    for s in spec:
+        if s[0] == 0:
+            out.append(oracle.make_light(oracle.LIGHT_DISK, p0=s[1:4], normal=s[4:7], color=s[7:10],
+                                         power=s[10], radius=s[11]))
+        else:
+            out.append(oracle.make_light(oracle.LIGHT_POINT, p0=s[1:4], color=s[7:10], power=s[10]))
+    return out
+
+
+def scene_with(lights=(), ls=oracle.LS_BVH, cam=None):
+    v, f = TRI
+    mats = [oracle.make_material(0, (0.5, 0.5, 0.5), (1, 1))]
+    return oracle.OracleScene(v, f, np.zeros(1, np.uint32), mats, list(lights), ls, cam)
+
+
+def test_sampler_matches_reference(rg):
+    for s, seq in zip(rg["sampler_seeds"], rg["sampler_seq"]):
+        np.testing.assert_array_equal(oracle.sampler_sequence(int(s), 64), seq)
+
+
+def test_sampler_closed_form():
+    """reset(seed): s_i = 1812433253 * (s ^ s >> 30) + i; get1D = u32 / (2^32 - 1)."""
+    seed = 42
+    s = []
+    x = seed
+    for i in range(1, 5):
+        x = (1812433253 * (x ^ (x >> 30)) + i) & 0xFFFFFFFF
+        s.append(x)
+    t, s0 = s[3], s[0]
+    t ^= (t << 11) & 0xFFFFFFFF
+    t ^= t >> 8
+    r = (t ^ s0 ^ (s0 >> 19)) & 0xFFFFFFFF
+    assert oracle.sampler_sequence(seed, 1)[0] == np.float32(r / 4294967295.0)
+
+
+def test_camera_matches_reference(rg):
+    cam = oracle.make_camera(rg["cam_eye"], rg["cam_target"], rg["cam_up"], float(rg["cam_fovy"]),
+                             float(rg["cam_aspect"]))
+    sc = scene_with(cam=cam)
+    for (x, y), o, d in zip(rg["cam_xy"], rg["cam_org"], rg["cam_dir"]):
+        oo, dd = sc.camera_ray(float(x), float(y))
+        np.testing.assert_array_equal(oo, o)
+        np.testing.assert_array_equal(dd, d)
+
+
+@pytest.mark.parametrize("k", range(5))
+def test_materials_match_reference(rg, k):
+    spec = rg["mat_specs"][k]
+    m = oracle.make_material(int(spec[0]), spec[1:4], spec[4:6])
+    for i in range(len(rg["mat_wo"])):
+        ok, est, wi, pdf = oracle.mat_sample(m, rg["mat_wo"][i], rg["mat_u"][i])
+        assert ok == bool(rg[f"mat{k}_ok"][i])
+        if ok:
+            np.testing.assert_array_equal(est, rg[f"mat{k}_est"][i])
+            np.testing.assert_array_equal(wi, rg[f"mat{k}_wi"][i])
+            assert pdf == rg[f"mat{k}_pdf"][i]
+        f, fp = oracle.mat_eval(m, rg["mat_wo"][i], rg["mat_wi_eval"][i])
+        np.testing.assert_array_equal(f, rg[f"mat{k}_f"][i])
+        assert fp == rg[f"mat{k}_fpdf"][i] or (np.isnan(fp) and np.isnan(rg[f"mat{k}_fpdf"][i]))
+
+
+@pytest.mark.parametrize("kind", [oracle.LS_UNIFORM, oracle.LS_POWER, oracle.LS_BVH])
+def test_light_samplers_match_reference(rg, kind):
+    sc = scene_with(lights_from_spec(rg["ls_spec"]), kind)
+    for i in range(len(rg["ls_u"])):
+        idx, pdf = sc.light_sampler_sample(float(rg["ls_u"][i]), rg["ls_hit"][i], rg["ls_ns"][i])
+        assert idx == rg[f"ls{kind}_idx"][i]
+        if idx >= 0:
+            assert pdf == rg[f"ls{kind}_pdf"][i]
+
+
+def test_disk_light_matches_reference(rg):
+    sc = scene_with(lights_from_spec(rg["disk_spec"]))
+    for i in range(len(rg["disk_u"])):
+        ok, le, wi, dist, pdf = sc.light_sample(0, rg["disk_u"][i], rg["ls_hit"][i])
+        assert ok == bool(rg["disk_ok"][i])
+        if ok:
+            np.testing.assert_array_equal(le, rg["disk_le"][i])
+            np.testing.assert_array_equal(wi, rg["disk_wi"][i])
+            assert dist == rg["disk_dist"][i] and pdf == rg["disk_pdf"][i]
+        p, d = sc.light_pdf(0, rg["ls_hit"][i], rg["disk_q"][i])
+        assert p == rg["disk_qpdf"][i] and d == rg["disk_qdist"][i]
+
+
+def test_bvh_light_sampler_pdf_is_the_sampling_probability(rg):
+    """pdf(light) (the extension guided MIS needs) equals sample()'s pdf and sums to <= 1."""
+    spec = rg["ls_spec"]
+    sc = scene_with(lights_from_spec(spec), oracle.LS_BVH)
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        hit = rng.uniform(-5, 5, 3).astype(np.float32)
+        ns = rng.standard_normal(3).astype(np.float32)
+        ns /= np.linalg.norm(ns)
+        pdfs = np.array([sc.light_sampler_pdf(i, hit, ns) for i in range(len(spec))])
+        assert pdfs.sum() <= 1.0 + 1e-5
+        counts = np.zeros(len(spec))
+        us = (np.arange(4000) + 0.5) / 4000
+        for u in us:
+            i, p = sc.light_sampler_sample(float(u), hit, ns)
+            if i >= 0:
+                counts[i] += 1
+                assert p == pytest.approx(pdfs[i], rel=1e-5)
+        np.testing.assert_allclose(counts / len(us), pdfs, atol=2e-3)
+
+
+@pytest.mark.parametrize("kind", [oracle.LIGHT_QUAD, oracle.LIGHT_TRIANGLE])
+def test_area_light_pdf_integrates_to_one(kind):
+    """Solid-angle pdf of the new quad/triangle lights: E[1/pdf] over light samples = solid
+    angle, and integrating pdf over directions gives 1 (Monte Carlo, uniform sphere)."""
+    L = oracle.make_light(kind, p0=(-0.5, 2.0, -0.5), p1=(0.5, 2.0, -0.5), p2=(-0.5, 2.0, 0.5),
+                          color=(1, 1, 1))
+    sc = scene_with([L])
+    hit = np.float32([0.1, 0.0, 0.2])
+    rng = np.random.default_rng(5)
+    n = 200_000
+    dirs = rng.standard_normal((n, 3)).astype(np.float32)
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    tot = 0.0
+    for d in dirs[:n]:
+        p, _ = sc.light_pdf(0, hit, d)
+        tot += p
+    est = tot / n * 4 * np.pi
+    assert est == pytest.approx(1.0, abs=0.03)
+    # light samples land on the light: pdf(sampled dir) equals the returned pdf
+    for u in rng.random((200, 2)).astype(np.float32):
+        ok, le, wi, dist, pdf = sc.light_sample(0, u, hit)
+        assert ok
+        p2, d2 = sc.light_pdf(0, hit, wi)
+        assert p2 == pytest.approx(pdf, rel=1e-3) and d2 == pytest.approx(dist, rel=1e-4)
+
+
+def test_lambert_white_furnace():
+    """Lambertian estimator is the albedo (lambertian.cpp:18); the Dielectric wrapper's
+    pdf/cos round trip (dielectric.cpp:53-64) keeps it to within a few ulps."""
+    m = oracle.make_material(0, (0.73, 0.5, 0.25), (1, 1))
+    rng = np.random.default_rng(1)
+    for u in rng.random((500, 2)).astype(np.float32):
+        ok, est, wi, pdf = oracle.mat_sample(m, np.float32([0.3, 0.1, 0.9]), u)
+        assert ok
+        np.testing.assert_allclose(est, np.float32([0.73, 0.5, 0.25]), rtol=1e-6)
+        assert pdf == pytest.approx(max(wi[2], 0) / np.pi, rel=1e-6)

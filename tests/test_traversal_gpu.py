@@ -37,7 +37,18 @@ def _accel(verts, tris, builder, lt=4):
 
 
 def _builders():
-    return [BUILDER_HOST]
+    return [BUILDER_HOST, BUILDER_GPU]
+
+
+@pytest.mark.parametrize("lt,key,sah", [(8, "leaf_keys8", "sah8"), (16, "leaf_keys16", "sah16")])
+def test_gpu_builder_reproduces_reference_tree(golden, lt, key, sah):
+    """GPU binned SAH with the reference's leaf threshold: same leaf sets and SAH cost."""
+    print(f"[gpu] gpu builder tree lt={lt}", flush=True)
+    acc = _accel(golden["verts"], golden["tris"], BUILDER_GPU, lt)
+    np.testing.assert_array_equal(acc.leaf_keys(), golden[key])
+    assert acc.stats()["sah_cost"] == pytest.approx(float(golden[sah]), rel=1e-12)
+    tri9, ids = acc.triangles()
+    assert np.array_equal(np.sort(ids), np.arange(len(golden["tris"])))
 
 
 @pytest.mark.parametrize("builder", _builders())
@@ -75,11 +86,12 @@ def test_small_scene_random_rays(small_scene, builder):
     np.testing.assert_array_equal(occ, ob.occluded(o, d, nthreads=8))
 
 
-def test_empty_inputs_and_scene():
+@pytest.mark.parametrize("builder", _builders())
+def test_empty_inputs_and_scene(builder):
     import torch
-    print("[gpu] empty inputs", flush=True)
+    print(f"[gpu] empty inputs builder={builder}", flush=True)
     acc = GpuAccelerator(0)
-    acc.update_scene(builder=BUILDER_HOST)  # no meshes: every ray misses
+    acc.update_scene(builder=builder)  # no meshes: every ray misses
     o = _dev(np.zeros((5, 3)))
     d = _dev(np.tile([0, 0, 1.0], (5, 1)))
     p, t, b0, b1 = _host(acc.intersect(o, d))
@@ -98,7 +110,7 @@ def test_multi_mesh_offsets_and_axis_rays(small_scene):
     s0 = acc.append_mesh(v, f[:half])
     s1 = acc.append_mesh(v, f[half:])
     assert (s0, s1) == (0, 1) and acc.prim_offset(1) == half
-    acc.update_scene(builder=BUILDER_HOST)
+    acc.update_scene(builder=BUILDER_GPU)
     lo, hi = scenes.scene_bounds(v)
     rng = np.random.default_rng(5)
     o = rng.uniform(lo, hi, (4000, 3)).astype(np.float32)
@@ -110,17 +122,18 @@ def test_multi_mesh_offsets_and_axis_rays(small_scene):
     assert c["unexplained"] == 0, c
 
 
-def test_vertex_update_rebuild(small_scene):
+@pytest.mark.parametrize("builder", _builders())
+def test_vertex_update_rebuild(small_scene, builder):
     """getVertexBuffer / updateShape / updateScene: a moved mesh is traced where it moved to."""
-    print("[gpu] vertex update + rebuild", flush=True)
+    print(f"[gpu] vertex update + rebuild builder={builder}", flush=True)
     v, f = small_scene
     acc = GpuAccelerator(0)
     sid = acc.append_mesh(v, f)
-    acc.update_scene(builder=BUILDER_HOST)
+    acc.update_scene(builder=builder)
     vb = acc.get_vertex_buffer(sid)
     vb[:, 0] += np.float32(0.5)
     acc.update_shape(sid)
-    acc.update_scene(builder=BUILDER_HOST)
+    acc.update_scene(builder=builder)
     moved = v.copy()
     moved[:, 0] += np.float32(0.5)
     lo, hi = scenes.scene_bounds(moved)
@@ -136,7 +149,7 @@ def test_counted_run_same_results(small_scene):
     v, f = small_scene
     lo, hi = scenes.scene_bounds(v)
     o, d = scenes.incoherent_rays(30_000, lo, hi, seed=7)
-    acc = _accel(v, f, BUILDER_HOST)
+    acc = _accel(v, f, BUILDER_GPU)
     g = _host(acc.intersect(_dev(o), _dev(d)))
     out, nodes, tris = acc.intersect_counted(_dev(o), _dev(d))
     g2 = _host(out)
@@ -153,7 +166,13 @@ def test_microbench_scene_full_ray_sets(microbench_scene):
     v, f = microbench_scene
     lo, hi = scenes.scene_bounds(v)
     ob = oracle.OracleBvh(v, f, 8)
-    acc = _accel(v, f, BUILDER_HOST)
+    acc8 = _accel(v, f, BUILDER_GPU, 8)
+    np.testing.assert_array_equal(acc8.leaf_keys(), ob.leaf_keys())
+    ratio = acc8.stats()["sah_cost"] / ob.stats()["sah_cost"]
+    print("gpu-built / reference SAH cost", ratio, "build ms", acc8.stats()["build_ms"])
+    assert ratio <= 1.02
+    del acc8
+    acc = _accel(v, f, BUILDER_GPU)
     o, d = scenes.incoherent_rays(4_194_304, lo, hi, seed=8)
     g = _host(acc.intersect(_dev(o), _dev(d)))
     ref = ob.intersect(o, d, nthreads=16)
