@@ -58,6 +58,50 @@ class _SceneStats(C.Structure):
     ]
 
 
+class Material(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("base_color", C.c_float * 3), ("alpha", C.c_float * 2)]
+
+
+class Light(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("p0", C.c_float * 3), ("p1", C.c_float * 3),
+                ("p2", C.c_float * 3), ("normal", C.c_float * 3), ("color", C.c_float * 3),
+                ("power", C.c_float), ("radius", C.c_float)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3),
+                ("fovy", C.c_float), ("aspect", C.c_float)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
+                ("fp64_accum", C.c_uint32), ("background", C.c_float * 3), ("guide", C.c_uint32),
+                ("reserved", C.c_uint32 * 6)]
+
+
+MAT_DIELECTRIC, MAT_CONDUCTOR = 0, 1
+LIGHT_DISK, LIGHT_POINT, LIGHT_QUAD, LIGHT_TRIANGLE = 0, 1, 2, 3
+LS_UNIFORM, LS_POWER, LS_BVH = 0, 1, 2
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+def material(kind, base_color, alpha):
+    return Material(kind, _f3(base_color), (C.c_float * 2)(*[float(a) for a in alpha]))
+
+
+def light(kind, p0=(0, 0, 0), p1=(0, 0, 0), p2=(0, 0, 0), normal=(0, 1, 0), color=(1, 1, 1),
+          power=1.0, radius=1.0):
+    return Light(kind, _f3(p0), _f3(p1), _f3(p2), _f3(normal), _f3(color), float(power),
+                 float(radius))
+
+
+def camera(eye, target, up, fovy, aspect):
+    return Camera(_f3(eye), _f3(target), _f3(up), float(fovy), float(aspect))
+
+
 class TraceConfig(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("fetch_threshold", C.c_uint32), ("reserved", C.c_uint32 * 5)]
@@ -71,10 +115,13 @@ ABI_SYMBOLS = [
     "apg_mesh_update", "apg_mesh_prim_offset", "apg_sphere_append", "apg_scene_build",
     "apg_scene_get_stats", "apg_trace_config_default", "apg_set_trace_config", "apg_intersect",
     "apg_occluded", "apg_intersect_counted", "apg_scene_triangles", "apg_scene_leaf_keys",
-    "apg_scene_wide_nodes",
+    "apg_scene_wide_nodes", "apg_materials_set", "apg_lights_set", "apg_light_sampler_sample",
+    "apg_light_sampler_pdf", "apg_render_reset", "apg_render_set_camera", "apg_render",
+    "apg_render_read", "apg_render_counters",
 ]
 DEBUG_SYMBOLS = ["apg_debug_host_build", "apg_debug_scene_info", "apg_debug_scene_copy",
-                 "apg_debug_host_trace", "apg_debug_scene_free"]
+                 "apg_debug_host_trace", "apg_debug_scene_free", "apg_debug_material",
+                 "apg_debug_light_sampler", "apg_debug_light"]
 
 
 def lib():
@@ -106,6 +153,16 @@ def lib():
         "apg_scene_triangles": (i32, [vp, vp, vp, u32]),
         "apg_scene_leaf_keys": (i32, [vp, vp, u32]),
         "apg_scene_wide_nodes": (i32, [vp, vp, u32]),
+        "apg_materials_set": (i32, [vp, vp, u32, vp, u32]),
+        "apg_lights_set": (i32, [vp, vp, u32, u32]),
+        "apg_light_sampler_sample": (i32, [vp, vp, vp, vp, u32, vp, vp, vp]),
+        "apg_light_sampler_pdf": (i32, [vp, vp, vp, vp, u32, vp, vp]),
+        "apg_render_reset": (i32, [vp, u32, u32, u32]),
+        "apg_render_set_camera": (i32, [vp, C.POINTER(Camera)]),
+        "apg_render": (i32, [vp, C.POINTER(RenderParams), vp]),
+        "apg_render_read": (i32, [vp, vp, vp, C.POINTER(u32)]),
+        "apg_render_counters": (i32, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                      C.POINTER(C.c_uint64)]),
         "apg_debug_host_build": (i32, [vp, u32, vp, u32, u32, C.POINTER(vp)]),
         "apg_debug_scene_info": (i32, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(u32), C.POINTER(u32)]),
@@ -113,6 +170,9 @@ def lib():
         "apg_debug_host_trace": (i32, [vp, vp, vp, vp, u32, i32, vp, vp, vp, vp, vp,
                                        C.POINTER(u32)]),
         "apg_debug_scene_free": (None, [vp]),
+        "apg_debug_material": (i32, [C.POINTER(Material)] + [vp] * 3 + [u32] + [vp] * 6),
+        "apg_debug_light_sampler": (i32, [vp, u32, u32, vp, vp, vp, u32, vp, vp, vp]),
+        "apg_debug_light": (i32, [C.POINTER(Light), vp, vp, vp, u32] + [vp] * 7),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -274,6 +334,61 @@ class GpuAccelerator:
                                                  None, C.byref(nodes), C.byref(tris),
                                                  self._stream(stream)))
         return out, nodes.value, tris.value
+
+    # ---- shading data and the path tracer ----
+    def set_materials(self, materials, tri_material):
+        arr = (Material * len(materials))(*materials)
+        tm = np.ascontiguousarray(tri_material, np.uint32)
+        self._keep_mats = arr
+        self._rc(self._lib.apg_materials_set(self._h, C.cast(arr, C.c_void_p), len(materials),
+                                             _ptr(tm), len(tm)))
+
+    def set_lights(self, lights, sampler=LS_BVH):
+        arr = (Light * max(1, len(lights)))(*lights)
+        self._rc(self._lib.apg_lights_set(self._h, C.cast(arr, C.c_void_p), len(lights), sampler))
+
+    def light_sampler_sample(self, u, hit, ns, stream=None):
+        """LightSampler::sample for device tensors u (n), hit/ns (n, 3); returns (light, pdf)."""
+        import torch
+        n = u.shape[0]
+        light = torch.empty(n, dtype=torch.int32, device=u.device)
+        pdf = torch.empty(n, dtype=torch.float32, device=u.device)
+        self._rc(self._lib.apg_light_sampler_sample(self._h, _ptr(u), _ptr(hit), _ptr(ns), n,
+                                                    _ptr(light), _ptr(pdf), self._stream(stream)))
+        return light, pdf
+
+    def light_sampler_pdf(self, light, hit, ns, stream=None):
+        import torch
+        n = light.shape[0]
+        pdf = torch.empty(n, dtype=torch.float32, device=light.device)
+        self._rc(self._lib.apg_light_sampler_pdf(self._h, _ptr(light), _ptr(hit), _ptr(ns), n,
+                                                 _ptr(pdf), self._stream(stream)))
+        return pdf
+
+    def render_reset(self, width, height, seed=0):
+        self.width, self.height = width, height
+        self._rc(self._lib.apg_render_reset(self._h, width, height, seed))
+
+    def set_camera(self, cam):
+        self._rc(self._lib.apg_render_set_camera(self._h, C.byref(cam)))
+
+    def render(self, spp, max_depth, rr_depth=0, background=(0.0, 0.0, 0.0), fp64=False,
+               stream=None):
+        p = RenderParams(spp, max_depth, rr_depth, 1 if fp64 else 0, _f3(background), 0)
+        self._rc(self._lib.apg_render(self._h, C.byref(p), self._stream(stream)))
+
+    def render_read(self, fp64=False):
+        rgb = np.empty((self.height, self.width, 3), np.float32)
+        r64 = np.empty((self.height, self.width, 3), np.float64) if fp64 else None
+        spp = C.c_uint32()
+        self._rc(self._lib.apg_render_read(self._h, _ptr(rgb), None if r64 is None else _ptr(r64),
+                                           C.byref(spp)))
+        return rgb, r64, spp.value
+
+    def render_counters(self):
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self._rc(self._lib.apg_render_counters(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"closest_rays": a.value, "shadow_rays": b.value, "vertices": c.value}
 
     # ---- introspection ----
     def leaf_keys(self) -> np.ndarray:

@@ -37,6 +37,9 @@ struct DeviceScene
     uint32_t tri_count = 0;
 };
 
+struct RenderState;  // render.hip
+void render_state_free(RenderState* r);
+
 struct Context
 {
     int device = 0;
@@ -59,7 +62,11 @@ struct Context
     uint64_t* spill = nullptr;        // per persistent thread stack spill
     size_t spill_threads = 0;
 
+    // shading data and the path tracer (render.hip)
+    RenderState* render = nullptr;
+
     ~Context();
+    int rebuild_shading();  // per-triangle normals after a scene build (render.hip)
     int fail(int code, const std::string& msg)
     {
         last_error = msg;
@@ -113,3 +120,9 @@ struct apg_context : apg::Context {};
         if (_e != hipSuccess)                                                      \
             return (ctx)->fail(APG_E_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
     } while (0)
+
+namespace apg {
+int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint32_t* n_dev,
+                     const apg_hits* hits, bool any, uint8_t* occ, unsigned long long* counts,
+                     hipStream_t stream);
+}

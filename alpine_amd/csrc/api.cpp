@@ -28,6 +28,7 @@ Context::~Context()
     if (counters) (void)hipFree(counters);
     if (spill) (void)hipFree(spill);
     if (last_launch) (void)hipEventDestroy(last_launch);
+    if (render) render_state_free(render);
 }
 
 static void free_scene(Context* ctx)
@@ -207,6 +208,7 @@ int apg_scene_build(apg_context* ctx, const apg_build_params* params)
     st.sah_cost = bt.sah_cost;
     st.sah_cost_leaf = bt.sah_cost_leaf;
     st.device_bytes = uint64_t(ctx->scene.node_count) * 64 + uint64_t(ctx->scene.tri_count) * 40;
+    if ((rc = ctx->rebuild_shading())) return rc;
     st.build_ms = bt.build_ms > 0.0
         ? bt.build_ms
         : std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -4,6 +4,7 @@
 #include "../../include/apg_debug.h"
 #include "apg_internal.h"
 #include "traverse_core.h"
+#include "render_internal.h"
 
 #include <cstring>
 
@@ -141,5 +142,82 @@ int apg_debug_host_trace(const apg_debug_scene* s, const float* org3, const floa
 }
 
 void apg_debug_scene_free(apg_debug_scene* s) { delete s; }
+
+int apg_debug_material(const apg_material* mat, const float* wo, const float* u,
+                       const float* wi_eval, uint32_t n, uint8_t* ok, float* est, float* wi,
+                       float* pdf, float* f_eval, float* pdf_eval)
+{
+    using namespace apg;
+    if (!mat) return APG_E_INVALID;
+    const DevMaterial m = make_material(mat->kind, mat->base_color, mat->alpha);
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        f3 e{}, w{};
+        float p = 0.0f;
+        const bool s = mat_sample(m, ld3(wo + 3 * i), f2{ u[2 * i], u[2 * i + 1] }, e, w, p);
+        ok[i] = s ? 1 : 0;
+        est[3 * i] = s ? e.x : 0.0f; est[3 * i + 1] = s ? e.y : 0.0f; est[3 * i + 2] = s ? e.z : 0.0f;
+        wi[3 * i] = s ? w.x : 0.0f; wi[3 * i + 1] = s ? w.y : 0.0f; wi[3 * i + 2] = s ? w.z : 0.0f;
+        pdf[i] = s ? p : 0.0f;
+        f3 f;
+        float fp;
+        mat_eval(m, ld3(wo + 3 * i), ld3(wi_eval + 3 * i), f, fp);
+        f_eval[3 * i] = f.x; f_eval[3 * i + 1] = f.y; f_eval[3 * i + 2] = f.z;
+        pdf_eval[i] = fp;
+    }
+    return APG_OK;
+}
+
+int apg_debug_light_sampler(const apg_light* lights, uint32_t nl, uint32_t sampler,
+                            const float* u, const float* hit, const float* ns, uint32_t n,
+                            int32_t* light, float* pdf, float* pdf_ext)
+{
+    using namespace apg;
+    std::vector<DevLight> dl(nl);
+    for (uint32_t i = 0; i < nl; ++i) dl[i] = prepare_light(lights[i]);
+    std::vector<float> ap, aq;
+    std::vector<uint32_t> aa;
+    std::vector<DevLightNode> nodes;
+    std::vector<int32_t> leaf;
+    if (sampler == LS_POWER && nl)
+    {
+        std::vector<float> w(nl);
+        for (uint32_t i = 0; i < nl; ++i) w[i] = length(ld3(dl[i].power));
+        build_alias(w, ap, aq, aa);
+    }
+    if (sampler == LS_BVH && nl) build_light_bvh(dl, nodes, leaf);
+    LightSamplerView v{ sampler, nl, dl.data(), ap.data(), aq.data(), aa.data(), nodes.data(), leaf.data() };
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        float p = 0.0f;
+        light[i] = light_sampler_sample(v, u[i], ld3(hit + 3 * i), ld3(ns + 3 * i), p);
+        pdf[i] = light[i] >= 0 ? p : 0.0f;
+        pdf_ext[i] = light[i] >= 0 ? light_sampler_pdf(v, light[i], ld3(hit + 3 * i), ld3(ns + 3 * i)) : 0.0f;
+    }
+    return APG_OK;
+}
+
+int apg_debug_light(const apg_light* light, const float* u2, const float* hit, const float* q,
+                    uint32_t n, uint8_t* ok, float* le, float* wi, float* dist, float* pdf,
+                    float* qpdf, float* qdist)
+{
+    using namespace apg;
+    const DevLight L = prepare_light(*light);
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        f3 l{}, w{};
+        float d = 0.0f, p = 0.0f;
+        const bool s = light_sample(L, f2{ u2[2 * i], u2[2 * i + 1] }, ld3(hit + 3 * i), l, w, d, p);
+        ok[i] = s ? 1 : 0;
+        le[3 * i] = l.x; le[3 * i + 1] = l.y; le[3 * i + 2] = l.z;
+        wi[3 * i] = w.x; wi[3 * i + 1] = w.y; wi[3 * i + 2] = w.z;
+        dist[i] = s ? d : 0.0f;
+        pdf[i] = s ? p : 0.0f;
+        float qd = 0.0f;
+        qpdf[i] = light_pdf(L, ld3(hit + 3 * i), ld3(q + 3 * i), qd);
+        qdist[i] = qd;
+    }
+    return APG_OK;
+}
 
 } // extern "C"

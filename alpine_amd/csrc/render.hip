@@ -1,0 +1,755 @@
+// Wavefront path tracer for gfx950: the reference's per-pixel loop
+// (Alpine::traceRayBatch / estimateDirectIllumination, alpine.cpp:290-432)
+// restated as stages over structure-of-arrays queues in HBM.
+//
+// One pass = one sample for every pixel.  A pixel's sampler state persists
+// across passes and calls, so the random numbers each path draws are the ones
+// the reference's per-pixel loop would draw, in the same order:
+//   generate  jitter (2), camera ray                    -> queue A
+//   per bounce
+//     extend   closest hit of queue A (traverse.hip, persistent)
+//     shade    light-sampler choice (1), light sample (2), BSDF sample for MIS
+//              (2), continuation BSDF sample (2): appends up to two shadow rays
+//              and the continuation ray (queue B) with wave-aggregated atomics
+//     shadow   any hit of the shadow queue
+//     resolve  NEE contribution of the visible shadow rays, radiance of
+//              finished paths into the per-pixel accumulation
+//   A <- B
+// Emission enters only through next-event estimation (lights are not
+// geometry), as in the reference.  Optional Russian roulette after
+// rr_depth (not in the reference) draws one extra number per surviving vertex.
+#include "render_internal.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace apg {
+
+struct Queue
+{
+    float* o[3];
+    float* d[3];
+    float* tmax;
+    uint32_t* path;
+};
+
+struct RenderState
+{
+    // shading tables
+    DevMaterial* mats = nullptr;
+    uint32_t nmats = 0;
+    uint32_t* tri_mat = nullptr;
+    uint32_t tri_mat_n = 0;
+    float4* tri_ng = nullptr;
+    uint32_t tri_ng_n = 0;
+    DevLight* lights = nullptr;
+    uint32_t nlights = 0;
+    uint32_t ls_kind = LS_UNIFORM;
+    bool ls_set = false;
+    float* alias_p = nullptr;
+    float* alias_q = nullptr;
+    uint32_t* alias_a = nullptr;
+    DevLightNode* lnodes = nullptr;
+    int32_t* leaf_of = nullptr;
+    DevCamera cam{};
+    bool cam_set = false;
+    // pixels
+    uint32_t W = 0, H = 0, P = 0;
+    uint32_t* rng = nullptr;
+    float* accum = nullptr;
+    double* accum64 = nullptr;
+    uint32_t total_spp = 0;
+    // paths
+    float* T = nullptr;
+    float* L = nullptr;
+    uint32_t* state = nullptr;  // depth | flags
+    float* nee = nullptr;       // 10 per path: Tnee xyz, A xyz, B xyz, light pdf
+    uint8_t* vis = nullptr;     // 2 per path
+    Queue qa{}, qb{}, qs{};
+    uint32_t* hprim = nullptr;
+    float *ht = nullptr, *hb0 = nullptr, *hb1 = nullptr;
+    uint8_t* occ = nullptr;
+    uint32_t* counts = nullptr;             // [0] A, [1] B, [2] shadow
+    unsigned long long* stats = nullptr;    // [0] closest rays, [1] shadow rays, [2] vertices
+    std::vector<void*> pixel_allocs;
+};
+
+namespace {
+
+constexpr uint32_t ST_ENDED = 1u << 31;
+constexpr uint32_t ST_NEE = 1u << 30;
+constexpr uint32_t ST_DEPTH = 0xFFFFu;
+
+struct LsView
+{
+    LightSamplerView v;
+};
+
+struct ShadeArgs
+{
+    DevMaterial* mats;
+    uint32_t* tri_mat;
+    float4* tri_ng;
+    LightSamplerView ls;
+    bool has_ls;
+    uint32_t* rng;
+    float* T;
+    float* L;
+    uint32_t* state;
+    float* nee;
+    uint8_t* vis;
+    Queue in, out, shadow;
+    uint32_t* hprim;
+    float* ht;
+    uint32_t* counts;
+    uint32_t max_depth, rr_depth;
+    float bg[3];
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+// Wave-aggregated append: one atomic per wave, slots handed out in lane order.
+// Every lane of the wave must call it.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
+{
+    const uint64_t m = __ballot(pred);
+    if (m == 0ull) return 0u;
+    const uint32_t leader = __ffsll((unsigned long long)m) - 1u;
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(counter, uint32_t(__popcll(m)));
+    base = __shfl(base, leader);
+    return base + uint32_t(__popcll(m & ((1ull << lane_id()) - 1ull)));
+}
+
+__device__ __forceinline__ void store3(float* a, uint32_t i, f3 v)
+{
+    a[3 * size_t(i)] = v.x;
+    a[3 * size_t(i) + 1] = v.y;
+    a[3 * size_t(i) + 2] = v.z;
+}
+__device__ __forceinline__ f3 load3(const float* a, uint32_t i)
+{
+    return mk(a[3 * size_t(i)], a[3 * size_t(i) + 1], a[3 * size_t(i) + 2]);
+}
+
+__global__ void k_reset_pixels(uint32_t P, uint32_t seed, uint32_t* rng, float* accum, double* accum64)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    uint32_t s[4];
+    sampler_reset(s, p + seed);
+    for (int k = 0; k < 4; ++k) rng[4 * size_t(p) + k] = s[k];
+    for (int k = 0; k < 3; ++k)
+    {
+        accum[3 * size_t(p) + k] = 0.0f;
+        accum64[3 * size_t(p) + k] = 0.0;
+    }
+}
+
+__global__ void k_set_counts(uint32_t* counts, uint32_t a)
+{
+    if (threadIdx.x == 0)
+    {
+        counts[0] = a;
+        counts[1] = 0;
+        counts[2] = 0;
+    }
+}
+
+// Camera rays with the per-pixel jitter (alpine.cpp:305-312).
+__global__ void k_generate(DevCamera cam, uint32_t W, uint32_t H, uint32_t* rng, float* T, float* L,
+                           uint32_t* state, Queue q, unsigned long long* stats)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= W * H) return;
+    uint32_t s[4];
+    for (int k = 0; k < 4; ++k) s[k] = rng[4 * size_t(p) + k];
+    const f2 j = get2d(s);
+    const uint32_t x = p % W, y = p / W;
+    f3 o, d;
+    camera_ray(cam, (float(x) + j.x) / float(W), (float(y) + j.y) / float(H), o, d);
+    for (int k = 0; k < 4; ++k) rng[4 * size_t(p) + k] = s[k];
+    store3(T, p, mk(1.0f, 1.0f, 1.0f));
+    store3(L, p, mk(0.0f, 0.0f, 0.0f));
+    state[p] = 0;
+    q.o[0][p] = o.x; q.o[1][p] = o.y; q.o[2][p] = o.z;
+    q.d[0][p] = d.x; q.d[1][p] = d.y; q.d[2][p] = d.z;
+    q.path[p] = p;
+    if (p == 0) atomicAdd(&stats[0], (unsigned long long)(W * H));
+}
+
+__global__ __launch_bounds__(256) void k_shade(ShadeArgs A)
+{
+    const uint32_t count = A.counts[0];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride)
+    {
+        const uint32_t q = base + threadIdx.x;
+        const bool valid = q < count;
+        bool emit_a = false, emit_b = false, emit_c = false;
+        f3 sa_o{}, sa_d{}, sb_d{}, c_o{}, c_d{};
+        float sa_t = 0.0f, sb_t = 0.0f;
+        uint32_t p = 0;
+        if (valid)
+        {
+            p = A.in.path[q];
+            uint32_t s[4];
+            for (int k = 0; k < 4; ++k) s[k] = A.rng[4 * size_t(p) + k];
+            f3 T = load3(A.T, p), L = load3(A.L, p);
+            uint32_t st = A.state[p] & ST_DEPTH;
+            const uint32_t depth = st;
+            const uint32_t prim = A.hprim[q];
+            const f3 ro = mk(A.in.o[0][q], A.in.o[1][q], A.in.o[2][q]);
+            const f3 rd = mk(A.in.d[0][q], A.in.d[1][q], A.in.d[2][q]);
+            A.vis[2 * size_t(p)] = 0;
+            A.vis[2 * size_t(p) + 1] = 0;
+            bool ended = false;
+            if (prim == 0xFFFFFFFFu)
+            {
+                L = L + T * mk(A.bg[0], A.bg[1], A.bg[2]);
+                ended = true;
+            }
+            else
+            {
+                const f3 hit = ro + rd * A.ht[q];
+                const float4 ngm = A.tri_ng[prim];
+                const f3 ng = mk(ngm.x, ngm.y, ngm.z);
+                const DevMaterial m = A.mats[A.tri_mat[prim]];
+                const f3 ns = ng;
+                f3 ss, ts;
+                get_basis(ns, ss, ts);
+                const f3 wo = to_local(-rd, ss, ts, ns);
+                // next-event estimation (alpine.cpp:370-432)
+                if (A.has_ls)
+                {
+                    float lsel = 0.0f;
+                    const int32_t li = light_sampler_sample(A.ls, get1d(s), hit, ns, lsel);
+                    if (li >= 0)
+                    {
+                        const DevLight& Lt = A.ls.lights[li];
+                        const f3 off = ns * kRayOffset;
+                        f3 ca = mk(0, 0, 0), cb = mk(0, 0, 0);
+                        f3 le, wiw;
+                        float dist, pdf;
+                        if (light_sample(Lt, get2d(s), hit, le, wiw, dist, pdf))
+                        {
+                            const f3 wi = to_local(wiw, ss, ts, ns);
+                            f3 f;
+                            float bpdf;
+                            mat_eval(m, wo, wi, f, bpdf);
+                            if (bpdf > 0.0f)
+                            {
+                                const float c = max_s(0.0f, dot(wiw, ns));
+                                const float w = !light_is_delta(Lt) ? power_heuristic(pdf, bpdf) : 1.0f;
+                                ca = le * w * f * c / pdf;
+                                emit_a = true;
+                                sa_o = hit + off;
+                                sa_d = wiw;
+                                sa_t = dist;
+                            }
+                        }
+                        if (!light_is_delta(Lt))
+                        {
+                            f3 est, wi;
+                            float mpdf;
+                            if (mat_sample(m, wo, get2d(s), est, wi, mpdf))
+                            {
+                                const f3 ld = to_world(wi, ss, ts, ns);
+                                float d2;
+                                const float lp = light_pdf(Lt, hit, ld, d2);
+                                if (lp > 0.0f)
+                                {
+                                    const float w = power_heuristic(mpdf, lp);
+                                    cb = ld3(Lt.le) * w * est;
+                                    emit_b = true;
+                                    sb_d = ld;
+                                    sb_t = d2;
+                                }
+                            }
+                        }
+                        sa_o = hit + off;
+                        float* ne = A.nee + 10 * size_t(p);
+                        ne[0] = T.x; ne[1] = T.y; ne[2] = T.z;
+                        ne[3] = ca.x; ne[4] = ca.y; ne[5] = ca.z;
+                        ne[6] = cb.x; ne[7] = cb.y; ne[8] = cb.z;
+                        ne[9] = lsel;
+                        st |= ST_NEE;
+                    }
+                }
+                // continuation (alpine.cpp:342-361)
+                f3 est, wi;
+                float mpdf;
+                if (!mat_sample(m, wo, get2d(s), est, wi, mpdf)) ended = true;
+                else
+                {
+                    const f3 wiw = to_world(wi, ss, ts, ns);
+                    if (!(dot(wiw, ng) > 0.0f)) ended = true;
+                    else
+                    {
+                        T = T * est;
+                        if (A.rr_depth && depth + 1 >= A.rr_depth)
+                        {
+                            const float qs = min_s(maxc(T), 0.95f);
+                            if (!(get1d(s) < qs)) ended = true;
+                            else T = T / qs;
+                        }
+                        if (!ended && depth + 1 < A.max_depth)
+                        {
+                            emit_c = true;
+                            c_o = hit + ng * kRayOffset;
+                            c_d = wiw;
+                            st = (st & ~ST_DEPTH) | (depth + 1);
+                        }
+                        else
+                            ended = true;
+                    }
+                }
+            }
+            for (int k = 0; k < 4; ++k) A.rng[4 * size_t(p) + k] = s[k];
+            store3(A.T, p, T);
+            store3(A.L, p, L);
+            A.state[p] = st | (ended ? ST_ENDED : 0u);
+        }
+        const uint32_t ia = wave_append(&A.counts[2], emit_a);
+        if (emit_a)
+        {
+            A.shadow.o[0][ia] = sa_o.x; A.shadow.o[1][ia] = sa_o.y; A.shadow.o[2][ia] = sa_o.z;
+            A.shadow.d[0][ia] = sa_d.x; A.shadow.d[1][ia] = sa_d.y; A.shadow.d[2][ia] = sa_d.z;
+            A.shadow.tmax[ia] = sa_t;
+            A.shadow.path[ia] = 2u * p;
+        }
+        const uint32_t ib = wave_append(&A.counts[2], emit_b);
+        if (emit_b)
+        {
+            A.shadow.o[0][ib] = sa_o.x; A.shadow.o[1][ib] = sa_o.y; A.shadow.o[2][ib] = sa_o.z;
+            A.shadow.d[0][ib] = sb_d.x; A.shadow.d[1][ib] = sb_d.y; A.shadow.d[2][ib] = sb_d.z;
+            A.shadow.tmax[ib] = sb_t;
+            A.shadow.path[ib] = 2u * p + 1u;
+        }
+        const uint32_t ic = wave_append(&A.counts[1], emit_c);
+        if (emit_c)
+        {
+            A.out.o[0][ic] = c_o.x; A.out.o[1][ic] = c_o.y; A.out.o[2][ic] = c_o.z;
+            A.out.d[0][ic] = c_d.x; A.out.d[1][ic] = c_d.y; A.out.d[2][ic] = c_d.z;
+            A.out.path[ic] = p;
+        }
+    }
+}
+
+__global__ void k_shadow_vis(const uint32_t* counts, const uint32_t* spath, const uint8_t* occ, uint8_t* vis)
+{
+    const uint32_t count = counts[2];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+        if (!occ[i]) vis[spath[i]] = 1;
+}
+
+__global__ void k_resolve(const uint32_t* counts, const uint32_t* qpath, uint32_t* state, const float* nee,
+                          const uint8_t* vis, float* Lbuf, float* accum, double* accum64, int fp64)
+{
+    const uint32_t count = counts[0];
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x)
+    {
+        const uint32_t p = qpath[q];
+        uint32_t st = state[p];
+        f3 L = load3(Lbuf, p);
+        if (st & ST_NEE)
+        {
+            const float* ne = nee + 10 * size_t(p);
+            f3 r = mk(0, 0, 0);
+            if (vis[2 * size_t(p)]) r = r + mk(ne[3], ne[4], ne[5]);
+            if (vis[2 * size_t(p) + 1]) r = r + mk(ne[6], ne[7], ne[8]);
+            L = L + mk(ne[0], ne[1], ne[2]) * (r / ne[9]);
+            st &= ~ST_NEE;
+            store3(Lbuf, p, L);
+        }
+        if (st & ST_ENDED)
+        {
+            accum[3 * size_t(p)] += L.x;
+            accum[3 * size_t(p) + 1] += L.y;
+            accum[3 * size_t(p) + 2] += L.z;
+            if (fp64)
+            {
+                accum64[3 * size_t(p)] += double(L.x);
+                accum64[3 * size_t(p) + 1] += double(L.y);
+                accum64[3 * size_t(p) + 2] += double(L.z);
+            }
+        }
+        state[p] = st;
+    }
+}
+
+__global__ void k_advance(uint32_t* counts, unsigned long long* stats)
+{
+    if (threadIdx.x == 0)
+    {
+        stats[1] += counts[2];
+        stats[2] += counts[0];
+        stats[0] += counts[1];
+        counts[0] = counts[1];
+        counts[1] = 0;
+        counts[2] = 0;
+    }
+}
+
+__global__ void k_tri_ng(const float* tri9, const uint32_t* ids, uint32_t n, float4* ng)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* t = tri9 + 9 * size_t(i);
+    // Primitive::updateVertices: ng = normalize(cross(e0, e1)) (bvh_common.cpp:116)
+    const f3 g = normalize(cross(mk(t[3], t[4], t[5]), mk(t[6], t[7], t[8])));
+    ng[ids[i]] = make_float4(g.x, g.y, g.z, 0.0f);
+}
+
+__global__ void k_ls_sample(LightSamplerView S, const float* u, const float* hit, const float* ns, uint32_t n,
+                            int32_t* light, float* pdf)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float p = 0.0f;
+    light[i] = light_sampler_sample(S, u[i], load3(hit, i), load3(ns, i), p);
+    pdf[i] = light[i] >= 0 ? p : 0.0f;
+}
+
+__global__ void k_ls_pdf(LightSamplerView S, const int32_t* light, const float* hit, const float* ns, uint32_t n,
+                         float* pdf)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pdf[i] = light_sampler_pdf(S, light[i], load3(hit, i), load3(ns, i));
+}
+
+template <typename T>
+int dmalloc(Context* ctx, T*& p, size_t count)
+{
+    p = nullptr;
+    APG_HIP_CHECK(ctx, hipMalloc(&p, (count ? count : 1) * sizeof(T)));
+    return APG_OK;
+}
+
+template <typename T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void free_pixels(RenderState* R)
+{
+    for (void* p : R->pixel_allocs) (void)hipFree(p);
+    R->pixel_allocs.clear();
+    R->P = R->W = R->H = 0;
+}
+
+RenderState* state_of(Context* ctx)
+{
+    if (!ctx->render) ctx->render = new RenderState;
+    return ctx->render;
+}
+
+LightSamplerView view_of(const RenderState* R)
+{
+    LightSamplerView v;
+    v.kind = R->ls_kind;
+    v.n = R->nlights;
+    v.lights = R->lights;
+    v.alias_p = R->alias_p;
+    v.alias_q = R->alias_q;
+    v.alias_a = R->alias_a;
+    v.nodes = R->lnodes;
+    v.leaf_of_light = R->leaf_of;
+    return v;
+}
+
+inline uint32_t nblocks(uint32_t n, uint32_t b) { return n ? (n + b - 1) / b : 1; }
+
+} // namespace
+
+void render_state_free(RenderState* R)
+{
+    if (!R) return;
+    dfree(R->mats);
+    dfree(R->tri_mat);
+    dfree(R->tri_ng);
+    dfree(R->lights);
+    dfree(R->alias_p);
+    dfree(R->alias_q);
+    dfree(R->alias_a);
+    dfree(R->lnodes);
+    dfree(R->leaf_of);
+    free_pixels(R);
+    delete R;
+}
+
+int Context::rebuild_shading()
+{
+    RenderState* R = state_of(this);
+    const uint32_t n = scene.tri_count;
+    if (R->tri_ng_n != total_tris)
+    {
+        dfree(R->tri_ng);
+        int rc = dmalloc(this, R->tri_ng, total_tris);
+        if (rc) return rc;
+        R->tri_ng_n = total_tris;
+    }
+    if (n) k_tri_ng<<<nblocks(n, 256), 256>>>(scene.tris, scene.tri_ids, n, R->tri_ng);
+    APG_HIP_CHECK(this, hipGetLastError());
+    APG_HIP_CHECK(this, hipDeviceSynchronize());
+    return APG_OK;
+}
+
+} // namespace apg
+
+using namespace apg;
+
+extern "C" {
+
+int apg_materials_set(apg_context* ctx, const apg_material* mats, uint32_t nmats,
+                      const uint32_t* tri_material, uint32_t ntris)
+{
+    if (!ctx || (nmats && !mats) || (ntris && !tri_material)) return APG_E_INVALID;
+    if (nmats == 0) return ctx->fail(APG_E_INVALID, "at least one material is required");
+    for (uint32_t i = 0; i < ntris; ++i)
+        if (tri_material[i] >= nmats) return ctx->fail(APG_E_INVALID, "triangle material index out of range");
+    for (uint32_t i = 0; i < nmats; ++i)
+        if (mats[i].kind > APG_MAT_CONDUCTOR) return ctx->fail(APG_E_INVALID, "unknown material kind");
+    APG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    RenderState* R = state_of(ctx);
+    std::vector<DevMaterial> dm(nmats);
+    for (uint32_t i = 0; i < nmats; ++i) dm[i] = make_material(mats[i].kind, mats[i].base_color, mats[i].alpha);
+    dfree(R->mats);
+    dfree(R->tri_mat);
+    int rc;
+    if ((rc = dmalloc(ctx, R->mats, nmats))) return rc;
+    if ((rc = dmalloc(ctx, R->tri_mat, ntris))) return rc;
+    APG_HIP_CHECK(ctx, hipMemcpy(R->mats, dm.data(), nmats * sizeof(DevMaterial), hipMemcpyHostToDevice));
+    if (ntris)
+        APG_HIP_CHECK(ctx, hipMemcpy(R->tri_mat, tri_material, ntris * sizeof(uint32_t), hipMemcpyHostToDevice));
+    R->nmats = nmats;
+    R->tri_mat_n = ntris;
+    return APG_OK;
+}
+
+int apg_lights_set(apg_context* ctx, const apg_light* lights, uint32_t n, uint32_t sampler)
+{
+    if (!ctx || (n && !lights)) return APG_E_INVALID;
+    if (sampler > APG_LS_BVH) return ctx->fail(APG_E_INVALID, "unknown light sampler");
+    for (uint32_t i = 0; i < n; ++i)
+        if (lights[i].kind > APG_LIGHT_TRIANGLE) return ctx->fail(APG_E_INVALID, "unknown light kind");
+    APG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    RenderState* R = state_of(ctx);
+    std::vector<DevLight> dl(n);
+    for (uint32_t i = 0; i < n; ++i) dl[i] = prepare_light(lights[i]);
+    dfree(R->lights);
+    dfree(R->alias_p);
+    dfree(R->alias_q);
+    dfree(R->alias_a);
+    dfree(R->lnodes);
+    dfree(R->leaf_of);
+    int rc;
+    if ((rc = dmalloc(ctx, R->lights, n))) return rc;
+    if (n) APG_HIP_CHECK(ctx, hipMemcpy(R->lights, dl.data(), n * sizeof(DevLight), hipMemcpyHostToDevice));
+    if (sampler == APG_LS_POWER && n)
+    {
+        std::vector<float> w(n), p, q;
+        std::vector<uint32_t> a;
+        for (uint32_t i = 0; i < n; ++i) w[i] = length(ld3(dl[i].power));
+        build_alias(w, p, q, a);
+        if ((rc = dmalloc(ctx, R->alias_p, n)) || (rc = dmalloc(ctx, R->alias_q, n)) || (rc = dmalloc(ctx, R->alias_a, n)))
+            return rc;
+        APG_HIP_CHECK(ctx, hipMemcpy(R->alias_p, p.data(), n * 4, hipMemcpyHostToDevice));
+        APG_HIP_CHECK(ctx, hipMemcpy(R->alias_q, q.data(), n * 4, hipMemcpyHostToDevice));
+        APG_HIP_CHECK(ctx, hipMemcpy(R->alias_a, a.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    if (sampler == APG_LS_BVH && n)
+    {
+        std::vector<DevLightNode> nodes;
+        std::vector<int32_t> leaf;
+        build_light_bvh(dl, nodes, leaf);
+        if ((rc = dmalloc(ctx, R->lnodes, nodes.size())) || (rc = dmalloc(ctx, R->leaf_of, n))) return rc;
+        APG_HIP_CHECK(ctx, hipMemcpy(R->lnodes, nodes.data(), nodes.size() * sizeof(DevLightNode), hipMemcpyHostToDevice));
+        APG_HIP_CHECK(ctx, hipMemcpy(R->leaf_of, leaf.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    R->nlights = n;
+    R->ls_kind = sampler;
+    R->ls_set = true;
+    return APG_OK;
+}
+
+int apg_light_sampler_sample(apg_context* ctx, const float* u, const float* hit, const float* ns,
+                             uint32_t n, int32_t* light, float* pdf, void* stream)
+{
+    if (!ctx || (n && (!u || !hit || !ns || !light || !pdf))) return APG_E_INVALID;
+    RenderState* R = state_of(ctx);
+    if (!R->ls_set) return ctx->fail(APG_E_STATE, "apg_lights_set has not been called");
+    if (n == 0) return APG_OK;
+    k_ls_sample<<<nblocks(n, 256), 256, 0, static_cast<hipStream_t>(stream)>>>(view_of(R), u, hit, ns, n, light, pdf);
+    APG_HIP_CHECK(ctx, hipGetLastError());
+    return APG_OK;
+}
+
+int apg_light_sampler_pdf(apg_context* ctx, const int32_t* light, const float* hit, const float* ns,
+                          uint32_t n, float* pdf, void* stream)
+{
+    if (!ctx || (n && (!light || !hit || !ns || !pdf))) return APG_E_INVALID;
+    RenderState* R = state_of(ctx);
+    if (!R->ls_set) return ctx->fail(APG_E_STATE, "apg_lights_set has not been called");
+    if (n == 0) return APG_OK;
+    k_ls_pdf<<<nblocks(n, 256), 256, 0, static_cast<hipStream_t>(stream)>>>(view_of(R), light, hit, ns, n, pdf);
+    APG_HIP_CHECK(ctx, hipGetLastError());
+    return APG_OK;
+}
+
+int apg_render_set_camera(apg_context* ctx, const apg_camera* cam)
+{
+    if (!ctx || !cam) return APG_E_INVALID;
+    RenderState* R = state_of(ctx);
+    camera_setup(*cam, R->cam);
+    R->cam_set = true;
+    return APG_OK;
+}
+
+int apg_render_reset(apg_context* ctx, uint32_t width, uint32_t height, uint32_t seed)
+{
+    if (!ctx || width == 0 || height == 0) return APG_E_INVALID;
+    if (uint64_t(width) * height > (1u << 26)) return ctx->fail(APG_E_CAPACITY, "more than 2^26 pixels");
+    APG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    RenderState* R = state_of(ctx);
+    const uint32_t P = width * height;
+    if (P != R->P)
+    {
+        if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+        free_pixels(R);
+        auto get = [&](auto*& ptr, size_t count) -> int {
+            int rc = dmalloc(ctx, ptr, count);
+            if (rc == APG_OK) R->pixel_allocs.push_back(ptr);
+            return rc;
+        };
+        int rc = 0;
+        if ((rc = get(R->rng, 4 * size_t(P))) || (rc = get(R->accum, 3 * size_t(P))) ||
+            (rc = get(R->accum64, 3 * size_t(P))) || (rc = get(R->T, 3 * size_t(P))) ||
+            (rc = get(R->L, 3 * size_t(P))) || (rc = get(R->state, P)) || (rc = get(R->nee, 10 * size_t(P))) ||
+            (rc = get(R->vis, 2 * size_t(P))) || (rc = get(R->hprim, P)) || (rc = get(R->ht, P)) ||
+            (rc = get(R->hb0, P)) || (rc = get(R->hb1, P)) || (rc = get(R->occ, 2 * size_t(P))) ||
+            (rc = get(R->counts, 16)) || (rc = get(R->stats, 8)))
+            return rc;
+        for (Queue* q : { &R->qa, &R->qb, &R->qs })
+        {
+            const size_t cap = q == &R->qs ? 2 * size_t(P) : P;
+            for (int a = 0; a < 3; ++a)
+                if ((rc = get(q->o[a], cap)) || (rc = get(q->d[a], cap))) return rc;
+            if ((rc = get(q->path, cap))) return rc;
+            if (q == &R->qs) { if ((rc = get(q->tmax, cap))) return rc; }
+            else q->tmax = nullptr;
+        }
+        R->P = P;
+    }
+    R->W = width;
+    R->H = height;
+    k_reset_pixels<<<nblocks(P, 256), 256>>>(P, seed, R->rng, R->accum, R->accum64);
+    APG_HIP_CHECK(ctx, hipMemset(R->stats, 0, 8 * sizeof(unsigned long long)));
+    APG_HIP_CHECK(ctx, hipDeviceSynchronize());
+    R->total_spp = 0;
+    return APG_OK;
+}
+
+int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
+{
+    if (!ctx || !prm) return APG_E_INVALID;
+    RenderState* R = state_of(ctx);
+    if (!ctx->built) return ctx->fail(APG_E_STATE, "apg_scene_build has not been called");
+    if (!R->P) return ctx->fail(APG_E_STATE, "apg_render_reset has not been called");
+    if (!R->cam_set) return ctx->fail(APG_E_STATE, "apg_render_set_camera has not been called");
+    if (!R->mats || R->tri_mat_n != ctx->total_tris)
+        return ctx->fail(APG_E_STATE, "apg_materials_set must cover every triangle");
+    if (prm->max_depth == 0 || prm->max_depth > ST_DEPTH) return ctx->fail(APG_E_INVALID, "max_depth out of range");
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    const uint32_t P = R->P;
+
+    ShadeArgs S;
+    S.mats = R->mats;
+    S.tri_mat = R->tri_mat;
+    S.tri_ng = R->tri_ng;
+    S.ls = view_of(R);
+    S.has_ls = R->ls_set;
+    S.rng = R->rng;
+    S.T = R->T;
+    S.L = R->L;
+    S.state = R->state;
+    S.nee = R->nee;
+    S.vis = R->vis;
+    S.shadow = R->qs;
+    S.hprim = R->hprim;
+    S.ht = R->ht;
+    S.counts = R->counts;
+    S.max_depth = prm->max_depth;
+    S.rr_depth = prm->rr_depth;
+    for (int k = 0; k < 3; ++k) S.bg[k] = prm->background[k];
+    const uint32_t sgrid = std::min<uint32_t>(nblocks(P, 256), uint32_t(ctx->cu_count) * 8);
+    const uint32_t rgrid = std::min<uint32_t>(nblocks(P, 256), uint32_t(ctx->cu_count) * 4);
+    const uint32_t vgrid = std::min<uint32_t>(nblocks(2 * P, 256), uint32_t(ctx->cu_count) * 4);
+
+    std::lock_guard<std::mutex> lock(ctx->launch_mutex);
+    if (!ctx->last_launch) APG_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->last_launch, hipEventDisableTiming));
+    APG_HIP_CHECK(ctx, hipStreamWaitEvent(stream, ctx->last_launch, 0));
+    for (uint32_t sidx = 0; sidx < prm->spp; ++sidx)
+    {
+        k_set_counts<<<1, 64, 0, stream>>>(R->counts, P);
+        k_generate<<<nblocks(P, 256), 256, 0, stream>>>(R->cam, R->W, R->H, R->rng, R->T, R->L, R->state, R->qa,
+                                                          R->stats);
+        Queue in = R->qa, out = R->qb;
+        for (uint32_t b = 0; b < prm->max_depth; ++b)
+        {
+            apg_rays rays{ in.o[0], in.o[1], in.o[2], in.d[0], in.d[1], in.d[2], nullptr };
+            apg_hits hits{ R->hprim, R->ht, R->hb0, R->hb1 };
+            int rc = launch_trace_raw(ctx, rays, P, R->counts, &hits, false, nullptr, nullptr, stream);
+            if (rc) return rc;
+            S.in = in;
+            S.out = out;
+            k_shade<<<sgrid, 256, 0, stream>>>(S);
+            apg_rays srays{ R->qs.o[0], R->qs.o[1], R->qs.o[2], R->qs.d[0], R->qs.d[1], R->qs.d[2], R->qs.tmax };
+            rc = launch_trace_raw(ctx, srays, 2 * P, R->counts + 2, nullptr, true, R->occ, nullptr, stream);
+            if (rc) return rc;
+            k_shadow_vis<<<vgrid, 256, 0, stream>>>(R->counts, R->qs.path, R->occ, R->vis);
+            k_resolve<<<rgrid, 256, 0, stream>>>(R->counts, in.path, R->state, R->nee, R->vis, R->L, R->accum,
+                                                 R->accum64, prm->fp64_accum ? 1 : 0);
+            k_advance<<<1, 64, 0, stream>>>(R->counts, R->stats);
+            std::swap(in, out);
+        }
+    }
+    APG_HIP_CHECK(ctx, hipGetLastError());
+    APG_HIP_CHECK(ctx, hipEventRecord(ctx->last_launch, stream));
+    R->total_spp += prm->spp;
+    return APG_OK;
+}
+
+int apg_render_read(apg_context* ctx, float* rgb, double* rgb64, uint32_t* total_spp)
+{
+    if (!ctx) return APG_E_INVALID;
+    RenderState* R = state_of(ctx);
+    if (!R->P) return ctx->fail(APG_E_STATE, "apg_render_reset has not been called");
+    if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+    if (rgb) APG_HIP_CHECK(ctx, hipMemcpy(rgb, R->accum, 3 * size_t(R->P) * 4, hipMemcpyDeviceToHost));
+    if (rgb64) APG_HIP_CHECK(ctx, hipMemcpy(rgb64, R->accum64, 3 * size_t(R->P) * 8, hipMemcpyDeviceToHost));
+    if (total_spp) *total_spp = R->total_spp;
+    return APG_OK;
+}
+
+int apg_render_counters(const apg_context* cctx, uint64_t* closest, uint64_t* shadow, uint64_t* vertices)
+{
+    if (!cctx) return APG_E_INVALID;
+    apg_context* ctx = const_cast<apg_context*>(cctx);
+    RenderState* R = state_of(ctx);
+    unsigned long long h[8] = {};
+    if (R->stats)
+    {
+        if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+        APG_HIP_CHECK(ctx, hipMemcpy(h, R->stats, sizeof(h), hipMemcpyDeviceToHost));
+    }
+    if (closest) *closest = h[0];
+    if (shadow) *shadow = h[1];
+    if (vertices) *vertices = h[2];
+    return APG_OK;
+}
+
+} // extern "C"

@@ -37,6 +37,7 @@ struct TraceArgs
     const uint32_t* tri_ids;
     apg_rays rays;
     uint32_t n;
+    const uint32_t* n_dev;            // when set, the ray count is read here (queued work)
     uint32_t has_scene;
     apg_hits hits;
     uint8_t* occluded;
@@ -61,6 +62,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A)
     uint32_t* st_bits = st_base + APG_STACK_LDS * 64u;
     uint64_t* spill = A.spill + size_t(blockIdx.x * blockDim.x + threadIdx.x) * APG_STACK_SPILL;
 
+    const uint32_t nrays = A.n_dev ? *A.n_dev : A.n;
     bool active = false;
     bool exhausted = false;  // wave-uniform
     uint32_t rid = 0;
@@ -86,11 +88,11 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A)
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(&A.counters[0], nidle);
             base = __shfl(base, leader);
-            if (base + nidle >= A.n) exhausted = true;
+            if (base + nidle >= nrays) exhausted = true;
             if (!active)
             {
                 const uint32_t my = base + __popcll(idle & ((1ull << lane) - 1ull));
-                if (my < A.n)
+                if (my < nrays)
                 {
                     rid = my;
                     active = true;
@@ -230,12 +232,63 @@ static int ensure_trace_buffers(Context* ctx, uint32_t threads)
     }
     if (ctx->spill_threads < threads)
     {
-        if (ctx->spill) (void)hipFree(ctx->spill);
+        if (ctx->spill)
+        {
+            // kernels of earlier calls may still read the old area
+            APG_HIP_CHECK(ctx, hipDeviceSynchronize());
+            (void)hipFree(ctx->spill);
+        }
         ctx->spill = nullptr;
         APG_HIP_CHECK(ctx, hipMalloc(&ctx->spill, size_t(threads) * APG_STACK_SPILL * sizeof(uint64_t)));
         ctx->spill_threads = threads;
     }
-    if (!ctx->last_launch) APG_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->last_launch, hipEventDisableTiming));
+    return APG_OK;
+}
+
+// Launch one traversal over `rays` on `stream` without cross-stream ordering.
+// n_dev, when non-null, holds the ray count on the device (n is then the
+// capacity used to size the grid).
+int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint32_t* n_dev,
+                     const apg_hits* hits, bool any, uint8_t* occ, unsigned long long* counts,
+                     hipStream_t stream)
+{
+    const apg_trace_config& cfg = ctx->cfg;
+    const uint32_t block = cfg.block_size;
+    uint32_t grid = uint32_t(ctx->cu_count) * cfg.blocks_per_cu;
+    const uint32_t need = (n + block - 1) / block;
+    if (grid > need) grid = need;
+    if (grid == 0) grid = 1;
+    int rc = ensure_trace_buffers(ctx, grid * block);
+    if (rc) return rc;
+
+    TraceArgs A;
+    A.nodes = ctx->scene.nodes;
+    A.tris = ctx->scene.tris;
+    A.tri_ids = ctx->scene.tri_ids;
+    A.rays = rays;
+    A.n = n;
+    A.n_dev = n_dev;
+    A.has_scene = ctx->scene.node_count > 0 ? 1u : 0u;
+    if (hits) A.hits = *hits; else A.hits = apg_hits{ nullptr, nullptr, nullptr, nullptr };
+    A.occluded = occ;
+    A.counters = ctx->counters;
+    A.spill = ctx->spill;
+    A.fetch_counts = counts;
+    A.fetch_threshold = cfg.fetch_threshold;
+
+    reset_counters_kernel<<<1, 64, 0, stream>>>(ctx->counters);
+    const size_t shmem = size_t(block / 64) * APG_STACK_LDS * 64 * 2 * sizeof(uint32_t);
+    if (any)
+    {
+        if (counts) trace_kernel<true, true><<<grid, block, shmem, stream>>>(A);
+        else trace_kernel<true, false><<<grid, block, shmem, stream>>>(A);
+    }
+    else
+    {
+        if (counts) trace_kernel<false, true><<<grid, block, shmem, stream>>>(A);
+        else trace_kernel<false, false><<<grid, block, shmem, stream>>>(A);
+    }
+    APG_HIP_CHECK(ctx, hipGetLastError());
     return APG_OK;
 }
 
@@ -251,47 +304,14 @@ int launch_trace(Context* ctx, const apg_rays* rays, uint32_t n, const apg_hits*
     if (n == 0) return APG_OK;
     if (n > 0x7fffffffu) return ctx->fail(APG_E_CAPACITY, "more than 2^31 - 1 rays in one call");
 
-    const apg_trace_config& cfg = ctx->cfg;
-    const uint32_t block = cfg.block_size;
-    uint32_t grid = uint32_t(ctx->cu_count) * cfg.blocks_per_cu;
-    const uint32_t need = (n + block - 1) / block;
-    if (grid > need) grid = need;
-    if (grid == 0) grid = 1;
-
     std::lock_guard<std::mutex> lock(ctx->launch_mutex);
-    int rc = ensure_trace_buffers(ctx, grid * block);
-    if (rc) return rc;
-
-    TraceArgs A;
-    A.nodes = ctx->scene.nodes;
-    A.tris = ctx->scene.tris;
-    A.tri_ids = ctx->scene.tri_ids;
-    A.rays = *rays;
-    A.n = n;
-    A.has_scene = ctx->scene.node_count > 0 ? 1u : 0u;
-    if (hits) A.hits = *hits; else A.hits = apg_hits{ nullptr, nullptr, nullptr, nullptr };
-    A.occluded = occ;
-    A.counters = ctx->counters;
-    A.spill = ctx->spill;
-    A.fetch_counts = counts;
-    A.fetch_threshold = cfg.fetch_threshold;
-
+    if (!ctx->last_launch)
+        APG_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->last_launch, hipEventDisableTiming));
     // Kernels from all streams share the ray cursor and the spill area:
     // order this launch after the previous one.
     APG_HIP_CHECK(ctx, hipStreamWaitEvent(stream, ctx->last_launch, 0));
-    reset_counters_kernel<<<1, 64, 0, stream>>>(ctx->counters);
-    const size_t shmem = size_t(block / 64) * APG_STACK_LDS * 64 * 2 * sizeof(uint32_t);
-    if (any)
-    {
-        if (counts) trace_kernel<true, true><<<grid, block, shmem, stream>>>(A);
-        else trace_kernel<true, false><<<grid, block, shmem, stream>>>(A);
-    }
-    else
-    {
-        if (counts) trace_kernel<false, true><<<grid, block, shmem, stream>>>(A);
-        else trace_kernel<false, false><<<grid, block, shmem, stream>>>(A);
-    }
-    APG_HIP_CHECK(ctx, hipGetLastError());
+    int rc = launch_trace_raw(ctx, *rays, n, nullptr, hits, any, occ, counts, stream);
+    if (rc) return rc;
     APG_HIP_CHECK(ctx, hipEventRecord(ctx->last_launch, stream));
     return APG_OK;
 }

@@ -183,3 +183,79 @@ def small_scene(n_random=2000, subdiv=3, seed=7):
     return concat(displaced_icosphere(subdiv, seed), random_triangles(n_random, seed,
                                                                       extent=1.5, lmin=0.02,
                                                                       lmax=0.4))
+
+
+# ---------------------------------------------------------------------------
+# Render scenes
+# ---------------------------------------------------------------------------
+def quad_mesh(origin, u, v, nu, nv):
+    """Tessellated parallelogram; triangle normals point along cross(u, v)."""
+    origin, u, v = (np.asarray(a, np.float64) for a in (origin, u, v))
+    a = np.linspace(0.0, 1.0, nu + 1)
+    b = np.linspace(0.0, 1.0, nv + 1)
+    A, B = np.meshgrid(a, b, indexing="ij")
+    verts = origin + A[..., None] * u + B[..., None] * v
+    idx = np.arange((nu + 1) * (nv + 1)).reshape(nu + 1, nv + 1)
+    v00, v10 = idx[:-1, :-1].ravel(), idx[1:, :-1].ravel()
+    v11, v01 = idx[1:, 1:].ravel(), idx[:-1, 1:].ravel()
+    tris = np.concatenate([np.stack([v00, v10, v11], 1), np.stack([v00, v11, v01], 1)])
+    return verts.reshape(-1, 3).astype(np.float32), tris.astype(np.uint32)
+
+
+def box_mesh(sx, h, sz, angle_deg, pos, n):
+    """Closed box [-sx/2, sx/2] x [0, h] x [-sz/2, sz/2], outward normals, rotated about y."""
+    faces = [
+        ((sx / 2, 0, -sz / 2), (0, h, 0), (0, 0, sz)),
+        ((-sx / 2, 0, -sz / 2), (0, 0, sz), (0, h, 0)),
+        ((-sx / 2, h, -sz / 2), (0, 0, sz), (sx, 0, 0)),
+        ((-sx / 2, 0, -sz / 2), (sx, 0, 0), (0, 0, sz)),
+        ((-sx / 2, 0, sz / 2), (sx, 0, 0), (0, h, 0)),
+        ((-sx / 2, 0, -sz / 2), (0, h, 0), (sx, 0, 0)),
+    ]
+    mesh = concat(*[quad_mesh(o, u, v, n, n) for o, u, v in faces])
+    t = np.radians(angle_deg)
+    R = np.array([[np.cos(t), 0, np.sin(t)], [0, 1, 0], [-np.sin(t), 0, np.cos(t)]])
+    verts = mesh[0].astype(np.float64) @ R.T + np.asarray(pos, np.float64)
+    return verts.astype(np.float32), mesh[1]
+
+
+class RenderScene:
+    """Geometry + per-triangle material + lights + camera for the path tracer."""
+
+    def __init__(self, verts, tris, tri_mat, materials, lights, camera, width, height, spp,
+                 max_depth, light_sampler, background=(0.0, 0.0, 0.0), rr_depth=0, seed=0):
+        self.verts, self.tris, self.tri_mat = verts, tris, np.asarray(tri_mat, np.uint32)
+        self.materials = materials    # list of (kind, base_color, alpha)
+        self.lights = lights          # list of dicts for alpine_amd.light(**d)
+        self.camera = camera          # dict: eye, target, up, fovy (radians), aspect
+        self.width, self.height, self.spp, self.max_depth = width, height, spp, max_depth
+        self.light_sampler = light_sampler
+        self.background = background
+        self.rr_depth = rr_depth
+        self.seed = seed
+
+
+def cornell_scene(width=256, height=256, spp=16, max_depth=5, seed=1, wall_res=32, box_res=8):
+    """Cornell-style room: 5 walls (32 x 32 quads each) + two boxes rotated -17 / +17 degrees
+    (8 x 8 per face) -> 11,776 triangles; one 0.25 x 0.25 quad light of radiance (17, 12, 4)
+    under the ceiling; Lambertian grey 0.73 / red (0.65, 0.05, 0.05) / green (0.12, 0.45, 0.15);
+    pinhole camera, 40 degree vertical field of view; uniform light sampler."""
+    walls = [
+        (quad_mesh((-1, 0, -1), (0, 0, 2), (2, 0, 0), wall_res, wall_res), 0),  # floor
+        (quad_mesh((-1, 2, -1), (2, 0, 0), (0, 0, 2), wall_res, wall_res), 0),  # ceiling
+        (quad_mesh((-1, 0, 1), (0, 2, 0), (2, 0, 0), wall_res, wall_res), 0),   # back
+        (quad_mesh((-1, 0, -1), (0, 2, 0), (0, 0, 2), wall_res, wall_res), 1),  # left, red
+        (quad_mesh((1, 0, -1), (0, 0, 2), (0, 2, 0), wall_res, wall_res), 2),   # right, green
+        (box_mesh(0.6, 0.6, 0.6, -17.0, (0.35, 0.0, -0.3), box_res), 0),
+        (box_mesh(0.6, 1.2, 0.6, 17.0, (-0.35, 0.0, 0.35), box_res), 0),
+    ]
+    verts, tris = concat(*[m for m, _ in walls])
+    tri_mat = np.concatenate([np.full(len(m[1]), k, np.uint32) for m, k in walls])
+    materials = [(0, (0.73, 0.73, 0.73), (1.0, 1.0)), (0, (0.65, 0.05, 0.05), (1.0, 1.0)),
+                 (0, (0.12, 0.45, 0.15), (1.0, 1.0))]
+    lights = [dict(kind=2, p0=(-0.125, 1.999, -0.125), p1=(0.125, 1.999, -0.125),
+                   p2=(-0.125, 1.999, 0.125), color=(17.0, 12.0, 4.0))]
+    cam = dict(eye=(0.0, 1.0, -3.4), target=(0.0, 1.0, 0.0), up=(0.0, 1.0, 0.0),
+               fovy=float(np.radians(40.0)), aspect=width / height)
+    return RenderScene(verts, tris, tri_mat, materials, lights, cam, width, height, spp,
+                       max_depth, light_sampler=0, seed=seed)

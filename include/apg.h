@@ -135,6 +135,78 @@ int apg_scene_leaf_keys(const apg_context* ctx, uint32_t* keys, uint32_t n);
 /* Raw 64 B wide nodes (host out), for structural tests. */
 int apg_scene_wide_nodes(const apg_context* ctx, void* nodes64, uint32_t n);
 
+/* ---- shading data (Material, Light, LightSampler; materials/material.h:12-29,
+ *      lights/light.h:13-46, light_samplers/light_sampler.h:12-25) ---- */
+#define APG_MAT_DIELECTRIC 0u  /* Lambert + GGX coat F0 0.04 when |alpha|^2 < 1 (dielectric.cpp) */
+#define APG_MAT_CONDUCTOR 1u   /* GGX, F0 = base colour (conductor.cpp) */
+typedef struct apg_material {
+    uint32_t kind;
+    float base_color[3];
+    float alpha[2];
+} apg_material;
+
+#define APG_LIGHT_DISK 0u      /* disk_light.cpp */
+#define APG_LIGHT_POINT 1u     /* point_light.cpp */
+#define APG_LIGHT_QUAD 2u      /* one-sided parallelogram p0 + a (p1 - p0) + b (p2 - p0), radiance = color */
+#define APG_LIGHT_TRIANGLE 3u  /* one-sided triangle p0 p1 p2, radiance = color */
+typedef struct apg_light {
+    uint32_t kind;
+    float p0[3], p1[3], p2[3];
+    float normal[3];           /* disk */
+    float color[3];            /* disk/point: colour (x power); quad/triangle: radiance */
+    float power;               /* disk/point */
+    float radius;              /* disk */
+} apg_light;
+
+#define APG_LS_UNIFORM 0u      /* uniform_light_sampler.cpp */
+#define APG_LS_POWER 1u        /* power_light_sampler.cpp + alias_table.cpp */
+#define APG_LS_BVH 2u          /* bvh_light_sampler.cpp (5-bin light BVH) */
+
+/* Material of every global triangle (Mesh per-primitive material, mesh.cpp:76-79). */
+int apg_materials_set(apg_context* ctx, const apg_material* mats, uint32_t nmats,
+                      const uint32_t* tri_material, uint32_t ntris);
+/* Lights and the sampler over them (Alpine::buildLightSampler, alpine.cpp:238-250). */
+int apg_lights_set(apg_context* ctx, const apg_light* lights, uint32_t n, uint32_t sampler);
+/* LightSampler::sample for n queries (device): u[n], hit/ns as n x 3 floats; light = -1 when
+ * none (the reference's empty optional). */
+int apg_light_sampler_sample(apg_context* ctx, const float* u, const float* hit,
+                             const float* ns, uint32_t n, int32_t* light, float* pdf,
+                             void* stream);
+/* The probability sample() returns `light` at (hit, ns): no reference counterpart; MIS with a
+ * third strategy needs it. */
+int apg_light_sampler_pdf(apg_context* ctx, const int32_t* light, const float* hit,
+                          const float* ns, uint32_t n, float* pdf, void* stream);
+
+/* ---- wavefront path tracer (Alpine::render, alpine.cpp:271-432) ---- */
+typedef struct apg_camera {
+    float eye[3], target[3], up[3];
+    float fovy, aspect;        /* Camera::setLookAt, camera.cpp:13-29 */
+} apg_camera;
+
+typedef struct apg_render_params {
+    uint32_t spp;              /* samples per pixel in this call */
+    uint32_t max_depth;        /* path vertices (alpine.cpp:314) */
+    uint32_t rr_depth;         /* 0: no Russian roulette; else from this depth on */
+    uint32_t fp64_accum;       /* also accumulate in double */
+    float background[3];       /* radiance on a miss (alpine.cpp:318-324) */
+    uint32_t guide;            /* 0 = plain; see apg_guide_* */
+    uint32_t reserved[6];
+} apg_render_params;
+
+/* Allocate per-pixel samplers (seeded with pixel index + seed, alpine.cpp:259-269) and clear
+ * the accumulation (Alpine::resetAccumulation). */
+int apg_render_reset(apg_context* ctx, uint32_t width, uint32_t height, uint32_t seed);
+int apg_render_set_camera(apg_context* ctx, const apg_camera* cam);
+/* Trace params->spp samples per pixel into the accumulation (asynchronous on stream). */
+int apg_render(apg_context* ctx, const apg_render_params* params, void* stream);
+/* Copy the accumulated radiance sum (width x height x 3, host) and the sample count.
+ * rgb64 (optional) receives the double accumulation when fp64_accum was set. */
+int apg_render_read(apg_context* ctx, float* rgb, double* rgb64, uint32_t* total_spp);
+/* Ray and path counters of the last apg_render call (host out): closest-hit rays, shadow
+ * rays, path vertices shaded. */
+int apg_render_counters(const apg_context* ctx, uint64_t* closest_rays, uint64_t* shadow_rays,
+                        uint64_t* vertices);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,8 @@
 
 #include <stdint.h>
 
+#include "apg.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -30,6 +32,20 @@ int apg_debug_host_trace(const apg_debug_scene* s, const float* org3, const floa
                          const float* tmax, uint32_t n, int any, uint32_t* prim, float* t,
                          float* b0, float* b1, uint8_t* occluded, uint32_t* max_stack);
 void apg_debug_scene_free(apg_debug_scene* s);
+
+/* Host build of the shading steps (alpine_amd/csrc/shade_core.h), n queries each.
+ * Material: sample (ok, est, wi, pdf) at (wo, u) and eval (f, pdf) at (wo, wi_eval). */
+int apg_debug_material(const apg_material* m, const float* wo, const float* u,
+                       const float* wi_eval, uint32_t n, uint8_t* ok, float* est, float* wi,
+                       float* pdf, float* f_eval, float* pdf_eval);
+/* Light sampler over `lights`: sample (light, pdf) at (u, hit, ns) and the pdf extension. */
+int apg_debug_light_sampler(const apg_light* lights, uint32_t nl, uint32_t sampler,
+                            const float* u, const float* hit, const float* ns, uint32_t n,
+                            int32_t* light, float* pdf, float* pdf_ext);
+/* One light: sample (ok, le, wi, dist, pdf) at (u2, hit) and pdf (qpdf, qdist) at (hit, q). */
+int apg_debug_light(const apg_light* light, const float* u2, const float* hit, const float* q,
+                    uint32_t n, uint8_t* ok, float* le, float* wi, float* dist, float* pdf,
+                    float* qpdf, float* qdist);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,86 @@
+"""The shading steps the GPU kernels run (shade_core.h), host build, vs the reference's
+compiled TUs (golden vectors) and the oracle.  On the host they use the same libm as the
+reference, so agreement is bit for bit."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import alpine_amd as aa
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def rg():
+    return dict(np.load(os.path.join(HERE, "golden", "render_small.npz")))
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+@pytest.mark.parametrize("k", range(5))
+def test_material_host_matches_reference(rg, k):
+    spec = rg["mat_specs"][k]
+    m = aa.material(int(spec[0]), spec[1:4], spec[4:6])
+    wo, u, we = rg["mat_wo"], rg["mat_u"], rg["mat_wi_eval"]
+    n = len(wo)
+    ok = np.empty(n, np.uint8)
+    est, wi, f = (np.empty((n, 3), np.float32) for _ in range(3))
+    pdf, fp = np.empty(n, np.float32), np.empty(n, np.float32)
+    assert aa.lib().apg_debug_material(C.byref(m), _p(wo), _p(u), _p(we), n, _p(ok), _p(est),
+                                       _p(wi), _p(pdf), _p(f), _p(fp)) == 0
+    np.testing.assert_array_equal(ok, rg[f"mat{k}_ok"])
+    s = ok.astype(bool)
+    np.testing.assert_array_equal(est[s], rg[f"mat{k}_est"][s])
+    np.testing.assert_array_equal(wi[s], rg[f"mat{k}_wi"][s])
+    np.testing.assert_array_equal(pdf[s], rg[f"mat{k}_pdf"][s])
+    np.testing.assert_array_equal(f, rg[f"mat{k}_f"])
+    np.testing.assert_array_equal(fp, rg[f"mat{k}_fpdf"])
+
+
+def _lights(spec):
+    out = []
+    for s in spec:
+        if s[0] == 0:
+            out.append(aa.light(0, p0=s[1:4], normal=s[4:7], color=s[7:10], power=s[10], radius=s[11]))
+        else:
+            out.append(aa.light(1, p0=s[1:4], color=s[7:10], power=s[10]))
+    return out
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_light_sampler_host_matches_reference(rg, kind):
+    lights = _lights(rg["ls_spec"])
+    arr = (aa.Light * len(lights))(*lights)
+    u, hit, ns = rg["ls_u"], rg["ls_hit"], rg["ls_ns"]
+    n = len(u)
+    idx = np.empty(n, np.int32)
+    pdf, pext = np.empty(n, np.float32), np.empty(n, np.float32)
+    aa.lib().apg_debug_light_sampler(C.cast(arr, C.c_void_p), len(lights), kind, _p(u), _p(hit),
+                                     _p(ns), n, _p(idx), _p(pdf), _p(pext))
+    np.testing.assert_array_equal(idx, rg[f"ls{kind}_idx"])
+    v = idx >= 0
+    np.testing.assert_array_equal(pdf[v], rg[f"ls{kind}_pdf"][v])
+    np.testing.assert_allclose(pext[v], pdf[v], rtol=2e-6)
+
+
+def test_disk_light_host_matches_reference(rg):
+    L = _lights(rg["disk_spec"])[0]
+    u2, hit, q = rg["disk_u"], rg["ls_hit"], rg["disk_q"]
+    n = len(u2)
+    ok = np.empty(n, np.uint8)
+    le, wi = np.empty((n, 3), np.float32), np.empty((n, 3), np.float32)
+    d, p, qp, qd = (np.empty(n, np.float32) for _ in range(4))
+    aa.lib().apg_debug_light(C.byref(L), _p(u2), _p(hit), _p(q), n, _p(ok), _p(le), _p(wi), _p(d),
+                             _p(p), _p(qp), _p(qd))
+    np.testing.assert_array_equal(ok, rg["disk_ok"])
+    s = ok.astype(bool)
+    np.testing.assert_array_equal(le[s], rg["disk_le"][s])
+    np.testing.assert_array_equal(wi[s], rg["disk_wi"][s])
+    np.testing.assert_array_equal(d[s], rg["disk_dist"][s])
+    np.testing.assert_array_equal(p[s], rg["disk_pdf"][s])
+    np.testing.assert_array_equal(qp, rg["disk_qpdf"])
+    np.testing.assert_array_equal(qd, rg["disk_qdist"])
