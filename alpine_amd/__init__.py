@@ -117,7 +117,8 @@ ABI_SYMBOLS = [
     "apg_occluded", "apg_intersect_counted", "apg_scene_triangles", "apg_scene_leaf_keys",
     "apg_scene_wide_nodes", "apg_materials_set", "apg_lights_set", "apg_light_sampler_sample",
     "apg_light_sampler_pdf", "apg_render_reset", "apg_render_set_camera", "apg_render",
-    "apg_render_read", "apg_render_counters",
+    "apg_render_read", "apg_render_counters", "apg_guide_param_count", "apg_guide_init",
+    "apg_guide_params", "apg_guide_forward", "apg_guide_train_step",
 ]
 DEBUG_SYMBOLS = ["apg_debug_host_build", "apg_debug_scene_info", "apg_debug_scene_copy",
                  "apg_debug_host_trace", "apg_debug_scene_free", "apg_debug_material",
@@ -163,6 +164,11 @@ def lib():
         "apg_render_read": (i32, [vp, vp, vp, C.POINTER(u32)]),
         "apg_render_counters": (i32, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                       C.POINTER(C.c_uint64)]),
+        "apg_guide_param_count": (i32, []),
+        "apg_guide_init": (i32, [vp, u32, C.c_float]),
+        "apg_guide_params": (i32, [vp, vp, i32]),
+        "apg_guide_forward": (i32, [vp, vp, u32, vp, vp]),
+        "apg_guide_train_step": (i32, [vp, vp, vp, vp, u32, i32, C.POINTER(C.c_float), vp, vp]),
         "apg_debug_host_build": (i32, [vp, u32, vp, u32, u32, C.POINTER(vp)]),
         "apg_debug_scene_info": (i32, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(u32), C.POINTER(u32)]),
@@ -389,6 +395,36 @@ class GpuAccelerator:
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
         self._rc(self._lib.apg_render_counters(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return {"closest_rays": a.value, "shadow_rays": b.value, "vertices": c.value}
+
+    # ---- learned path guiding ----
+    def guide_init(self, seed=1, lr=1e-3):
+        self._rc(self._lib.apg_guide_init(self._h, seed, lr))
+
+    def guide_params(self, params=None):
+        n = self._lib.apg_guide_param_count()
+        if params is None:
+            out = np.empty(n, np.float32)
+            self._rc(self._lib.apg_guide_params(self._h, _ptr(out), 0))
+            return out
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.shape == (n,)
+        self._rc(self._lib.apg_guide_params(self._h, _ptr(p), 1))
+
+    def guide_forward(self, feat, stream=None):
+        import torch
+        n = feat.shape[0]
+        out = torch.empty((n, 48), dtype=torch.float32, device=feat.device)
+        self._rc(self._lib.apg_guide_forward(self._h, _ptr(feat), n, _ptr(out), self._stream(stream)))
+        return out
+
+    def guide_train_step(self, feat, dirs, weight, apply=True, want_grad=False, stream=None):
+        n = feat.shape[0]
+        loss = C.c_float()
+        g = np.empty(self._lib.apg_guide_param_count(), np.float32) if want_grad else None
+        self._rc(self._lib.apg_guide_train_step(self._h, _ptr(feat), _ptr(dirs), _ptr(weight), n,
+                                                1 if apply else 0, C.byref(loss),
+                                                None if g is None else _ptr(g), self._stream(stream)))
+        return loss.value, g
 
     # ---- introspection ----
     def leaf_keys(self) -> np.ndarray:

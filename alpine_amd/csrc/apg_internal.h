@@ -39,6 +39,8 @@ struct DeviceScene
 
 struct RenderState;  // render.hip
 void render_state_free(RenderState* r);
+struct GuideNet;     // guide.hip
+void guide_free(GuideNet* g);
 
 struct Context
 {
@@ -64,6 +66,7 @@ struct Context
 
     // shading data and the path tracer (render.hip)
     RenderState* render = nullptr;
+    GuideNet* guide = nullptr;
 
     ~Context();
     int rebuild_shading();  // per-triangle normals after a scene build (render.hip)

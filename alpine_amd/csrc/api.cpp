@@ -29,6 +29,7 @@ Context::~Context()
     if (spill) (void)hipFree(spill);
     if (last_launch) (void)hipEventDestroy(last_launch);
     if (render) render_state_free(render);
+    if (guide) guide_free(guide);
 }
 
 static void free_scene(Context* ctx)

@@ -207,6 +207,26 @@ int apg_render_read(apg_context* ctx, float* rgb, double* rgb64, uint32_t* total
 int apg_render_counters(const apg_context* ctx, uint64_t* closest_rays, uint64_t* shadow_rays,
                         uint64_t* vertices);
 
+/* ---- learned path guiding (no reference counterpart) ----
+ * An MLP (96 -> 4 x 64 -> 48, BF16 MFMA, FP32 accumulation) maps a shading point to an
+ * 8-lobe von Mises-Fisher mixture over directions.  Features: 96 floats per query
+ * (position normalised to the scene bounds with 12-frequency sin/cos encoding, shading
+ * normal, outgoing direction; alpine_amd/csrc/guide_core.h).  Outputs: 48 floats per query
+ * (40 used: per lobe logit, log kappa, mean direction). */
+int apg_guide_param_count(void);
+/* Deterministic initialisation and the Adam learning rate. */
+int apg_guide_init(apg_context* ctx, uint32_t seed, float lr);
+/* FP32 master parameters (host, apg_guide_param_count() floats): write != 0 sets them. */
+int apg_guide_params(apg_context* ctx, float* params, int write);
+/* Forward pass: feat n x 96 (device) -> out n x 48 (device). */
+int apg_guide_forward(apg_context* ctx, const float* feat, uint32_t n, float* out, void* stream);
+/* One training step on n samples (device): minimise sum_i w_i (-log q(dir_i | feat_i)) /
+ * sum_i w_i.  apply = 0 computes the gradient only.  loss (host, optional) receives the
+ * weighted mean negative log-likelihood; grad_host (optional) the gradient vector. */
+int apg_guide_train_step(apg_context* ctx, const float* feat, const float* dir,
+                         const float* weight, uint32_t n, int apply, float* loss,
+                         float* grad_host, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
