@@ -76,7 +76,8 @@ class Camera(C.Structure):
 class RenderParams(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
                 ("fp64_accum", C.c_uint32), ("background", C.c_float * 3), ("guide", C.c_uint32),
-                ("reserved", C.c_uint32 * 6)]
+                ("train_steps", C.c_uint32), ("guide_warmup", C.c_uint32),
+                ("guide_frac", C.c_float), ("reserved", C.c_uint32 * 3)]
 
 
 MAT_DIELECTRIC, MAT_CONDUCTOR = 0, 1
@@ -118,7 +119,7 @@ ABI_SYMBOLS = [
     "apg_scene_wide_nodes", "apg_materials_set", "apg_lights_set", "apg_light_sampler_sample",
     "apg_light_sampler_pdf", "apg_render_reset", "apg_render_set_camera", "apg_render",
     "apg_render_read", "apg_render_counters", "apg_guide_param_count", "apg_guide_init",
-    "apg_guide_params", "apg_guide_forward", "apg_guide_train_step",
+    "apg_guide_params", "apg_guide_forward", "apg_guide_train_step", "apg_render_guide_stats",
 ]
 DEBUG_SYMBOLS = ["apg_debug_host_build", "apg_debug_scene_info", "apg_debug_scene_copy",
                  "apg_debug_host_trace", "apg_debug_scene_free", "apg_debug_material",
@@ -165,6 +166,7 @@ def lib():
         "apg_render_counters": (i32, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                       C.POINTER(C.c_uint64)]),
         "apg_guide_param_count": (i32, []),
+        "apg_render_guide_stats": (i32, [vp, C.POINTER(u32), C.POINTER(C.c_float)]),
         "apg_guide_init": (i32, [vp, u32, C.c_float]),
         "apg_guide_params": (i32, [vp, vp, i32]),
         "apg_guide_forward": (i32, [vp, vp, u32, vp, vp]),
@@ -379,9 +381,15 @@ class GpuAccelerator:
         self._rc(self._lib.apg_render_set_camera(self._h, C.byref(cam)))
 
     def render(self, spp, max_depth, rr_depth=0, background=(0.0, 0.0, 0.0), fp64=False,
-               stream=None):
-        p = RenderParams(spp, max_depth, rr_depth, 1 if fp64 else 0, _f3(background), 0)
+               guide=0, train_steps=4, guide_warmup=1, guide_frac=0.5, stream=None):
+        p = RenderParams(spp, max_depth, rr_depth, 1 if fp64 else 0, _f3(background), guide,
+                         train_steps, guide_warmup, guide_frac)
         self._rc(self._lib.apg_render(self._h, C.byref(p), self._stream(stream)))
+
+    def guide_stats(self):
+        n, loss = C.c_uint32(), C.c_float()
+        self._rc(self._lib.apg_render_guide_stats(self._h, C.byref(n), C.byref(loss)))
+        return n.value, loss.value
 
     def render_read(self, fp64=False):
         rgb = np.empty((self.height, self.width, 3), np.float32)

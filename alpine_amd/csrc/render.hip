@@ -19,6 +19,7 @@
 // geometry), as in the reference.  Optional Russian roulette after
 // rr_depth (not in the reference) draws one extra number per surviving vertex.
 #include "render_internal.h"
+#include "guide_core.h"
 
 #include <algorithm>
 #include <cstring>
@@ -72,6 +73,22 @@ struct RenderState
     uint32_t* counts = nullptr;             // [0] A, [1] B, [2] shadow
     unsigned long long* stats = nullptr;    // [0] closest rays, [1] shadow rays, [2] vertices
     std::vector<void*> pixel_allocs;
+    // path guiding
+    float* gfeat = nullptr;   // P x 96 features of the shading points of a bounce
+    float* gout = nullptr;    // P x 48 proposal parameters
+    float* vrec = nullptr;    // P x vrec_depth x VREC vertex records
+    uint32_t vrec_depth = 0;
+    float* tin = nullptr;     // tcap x 9 training inputs (p01, n, wo)
+    float* tfeat = nullptr;   // tcap x 96
+    float* tdir = nullptr;    // tcap x 3
+    float* tw = nullptr;      // tcap
+    uint32_t tcap = 0;
+    uint32_t* tcount = nullptr;
+    std::vector<void*> guide_allocs;
+    uint32_t last_train_samples = 0;
+    float last_loss = 0.0f;
+    uint32_t guided_passes = 0;
+    float scene_lo[3] = { 0, 0, 0 }, scene_hi[3] = { 1, 1, 1 };
 };
 
 namespace {
@@ -79,6 +96,7 @@ namespace {
 constexpr uint32_t ST_ENDED = 1u << 31;
 constexpr uint32_t ST_NEE = 1u << 30;
 constexpr uint32_t ST_DEPTH = 0xFFFFu;
+constexpr int VREC = 20;  // p01 3, n 3, wo 3, dir 3, w 3, E 3, has_dir, pad
 
 struct LsView
 {
@@ -104,7 +122,41 @@ struct ShadeArgs
     uint32_t* counts;
     uint32_t max_depth, rr_depth;
     float bg[3];
+    // guiding
+    const float* gout;
+    float guide_frac;
+    float* vrec;
+    uint32_t vrec_depth;
+    uint32_t bounce;
+    float lo[3], inv[3];
 };
+
+__device__ __forceinline__ f3 to_unit_box(const ShadeArgs& A, f3 p)
+{
+    return mk(fminf(fmaxf((p.x - A.lo[0]) * A.inv[0], 0.0f), 1.0f),
+              fminf(fmaxf((p.y - A.lo[1]) * A.inv[1], 0.0f), 1.0f),
+              fminf(fmaxf((p.z - A.lo[2]) * A.inv[2], 0.0f), 1.0f));
+}
+
+// Features of every shading point of the bounce (queue A), for the proposal network.
+__global__ void k_guide_features(ShadeArgs A, float* feat)
+{
+    const uint32_t count = A.counts[0];
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x)
+    {
+        float* f = feat + size_t(q) * GUIDE_IN;
+        const uint32_t prim = A.hprim[q];
+        if (prim == 0xFFFFFFFFu)
+        {
+            for (int i = 0; i < GUIDE_IN; ++i) f[i] = 0.0f;
+            continue;
+        }
+        const f3 ro = mk(A.in.o[0][q], A.in.o[1][q], A.in.o[2][q]);
+        const f3 rd = mk(A.in.d[0][q], A.in.d[1][q], A.in.d[2][q]);
+        const float4 ngm = A.tri_ng[prim];
+        guide_encode(to_unit_box(A, ro + rd * A.ht[q]), mk(ngm.x, ngm.y, ngm.z), -rd, f);
+    }
+}
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
@@ -178,6 +230,7 @@ __global__ void k_generate(DevCamera cam, uint32_t W, uint32_t H, uint32_t* rng,
     if (p == 0) atomicAdd(&stats[0], (unsigned long long)(W * H));
 }
 
+template <bool GUIDED>
 __global__ __launch_bounds__(256) void k_shade(ShadeArgs A)
 {
     const uint32_t count = A.counts[0];
@@ -208,6 +261,12 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A)
             {
                 L = L + T * mk(A.bg[0], A.bg[1], A.bg[2]);
                 ended = true;
+                if (GUIDED && A.bounce < A.vrec_depth)
+                {
+                    float* vr = A.vrec + (size_t(p) * A.vrec_depth + A.bounce) * VREC;
+                    vr[15] = A.bg[0]; vr[16] = A.bg[1]; vr[17] = A.bg[2];
+                    vr[18] = 0.0f;
+                }
             }
             else
             {
@@ -277,12 +336,53 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A)
                     }
                 }
                 // continuation (alpine.cpp:342-361)
-                f3 est, wi;
-                float mpdf;
-                if (!mat_sample(m, wo, get2d(s), est, wi, mpdf)) ended = true;
+                f3 est, wi, wiw;
+                bool ok;
+                float* vr = nullptr;
+                if (GUIDED)
+                {
+                    // one-sample MIS of BSDF sampling and the learned proposal (true densities)
+                    vr = A.vrec + (size_t(p) * A.vrec_depth + A.bounce) * VREC;
+                    const VmfMix mix = guide_decode(A.gout + size_t(q) * GUIDE_OUT);
+                    const float c = A.guide_frac;
+                    if (get1d(s) < c)
+                    {
+                        const f2 uu = get2d(s);
+                        wiw = guide_sample(mix, uu.x, uu.y, get1d(s));
+                        wi = to_local(wiw, ss, ts, ns);
+                        ok = true;
+                    }
+                    else
+                    {
+                        float mpdf;
+                        ok = mat_sample(m, wo, get2d(s), est, wi, mpdf);
+                        if (ok) wiw = to_world(wi, ss, ts, ns);
+                    }
+                    if (ok)
+                    {
+                        f3 f;
+                        float dummy;
+                        mat_eval(m, wo, wi, f, dummy);
+                        const float pm = (1.0f - c) * mat_true_pdf(m, wo, wi) + c * guide_pdf(mix, wiw);
+                        ok = wi.z > 0.0f && pm > 0.0f;
+                        if (ok) est = f * wi.z / pm;
+                    }
+                    const f3 p01 = to_unit_box(A, hit);
+                    vr[0] = p01.x; vr[1] = p01.y; vr[2] = p01.z;
+                    vr[3] = ns.x; vr[4] = ns.y; vr[5] = ns.z;
+                    vr[6] = -rd.x; vr[7] = -rd.y; vr[8] = -rd.z;
+                    vr[15] = 0.0f; vr[16] = 0.0f; vr[17] = 0.0f;
+                    vr[18] = 0.0f;
+                }
                 else
                 {
-                    const f3 wiw = to_world(wi, ss, ts, ns);
+                    float mpdf;
+                    ok = mat_sample(m, wo, get2d(s), est, wi, mpdf);
+                    if (ok) wiw = to_world(wi, ss, ts, ns);
+                }
+                if (!ok) ended = true;
+                else
+                {
                     if (!(dot(wiw, ng) > 0.0f)) ended = true;
                     else
                     {
@@ -299,6 +399,12 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A)
                             c_o = hit + ng * kRayOffset;
                             c_d = wiw;
                             st = (st & ~ST_DEPTH) | (depth + 1);
+                            if (GUIDED)
+                            {
+                                vr[9] = wiw.x; vr[10] = wiw.y; vr[11] = wiw.z;
+                                vr[12] = est.x; vr[13] = est.y; vr[14] = est.z;
+                                vr[18] = 1.0f;
+                            }
                         }
                         else
                             ended = true;
@@ -343,8 +449,22 @@ __global__ void k_shadow_vis(const uint32_t* counts, const uint32_t* spath, cons
         if (!occ[i]) vis[spath[i]] = 1;
 }
 
+struct TrainOut
+{
+    float* vrec;
+    uint32_t vrec_depth;
+    uint32_t bounce;
+    float* tin;
+    float* tdir;
+    float* tw;
+    uint32_t tcap;
+    uint32_t* tcount;
+};
+
+__device__ __forceinline__ float luminance(f3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
 __global__ void k_resolve(const uint32_t* counts, const uint32_t* qpath, uint32_t* state, const float* nee,
-                          const uint8_t* vis, float* Lbuf, float* accum, double* accum64, int fp64)
+                          const uint8_t* vis, float* Lbuf, float* accum, double* accum64, int fp64, TrainOut G)
 {
     const uint32_t count = counts[0];
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x)
@@ -358,9 +478,46 @@ __global__ void k_resolve(const uint32_t* counts, const uint32_t* qpath, uint32_
             f3 r = mk(0, 0, 0);
             if (vis[2 * size_t(p)]) r = r + mk(ne[3], ne[4], ne[5]);
             if (vis[2 * size_t(p) + 1]) r = r + mk(ne[6], ne[7], ne[8]);
-            L = L + mk(ne[0], ne[1], ne[2]) * (r / ne[9]);
+            const f3 e = r / ne[9];
+            L = L + mk(ne[0], ne[1], ne[2]) * e;
             st &= ~ST_NEE;
             store3(Lbuf, p, L);
+            if (G.vrec && G.bounce < G.vrec_depth)
+            {
+                float* vr = G.vrec + (size_t(p) * G.vrec_depth + G.bounce) * VREC;
+                vr[15] = e.x; vr[16] = e.y; vr[17] = e.z;
+            }
+        }
+        if (G.vrec && (st & ST_ENDED))
+        {
+            // radiance leaving each vertex towards the previous one, back to front; each
+            // traced continuation becomes a training sample weighted by F / p = w (x) L_in
+            f3 lnext = mk(0.0f, 0.0f, 0.0f);
+            const uint32_t last = G.bounce < G.vrec_depth ? G.bounce : G.vrec_depth - 1;
+            for (int j = int(last); j >= 0; --j)
+            {
+                const float* vr = G.vrec + (size_t(p) * G.vrec_depth + uint32_t(j)) * VREC;
+                const f3 e = mk(vr[15], vr[16], vr[17]);
+                f3 lo = e;
+                if (vr[18] > 0.5f)
+                {
+                    const f3 w = mk(vr[12], vr[13], vr[14]);
+                    const f3 contrib = w * lnext;
+                    lo = e + contrib;
+                    const float wt = luminance(contrib);
+                    if (wt > 0.0f)
+                    {
+                        const uint32_t t = atomicAdd(G.tcount, 1u);
+                        if (t < G.tcap)
+                        {
+                            for (int k = 0; k < 9; ++k) G.tin[9 * size_t(t) + k] = vr[k];
+                            for (int k = 0; k < 3; ++k) G.tdir[3 * size_t(t) + k] = vr[9 + k];
+                            G.tw[t] = wt;
+                        }
+                    }
+                }
+                lnext = lo;
+            }
         }
         if (st & ST_ENDED)
         {
@@ -389,6 +546,14 @@ __global__ void k_advance(uint32_t* counts, unsigned long long* stats)
         counts[1] = 0;
         counts[2] = 0;
     }
+}
+
+__global__ void k_train_encode(const float* tin, uint32_t n, float* feat)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* t = tin + 9 * size_t(i);
+    guide_encode(mk(t[0], t[1], t[2]), mk(t[3], t[4], t[5]), mk(t[6], t[7], t[8]), feat + size_t(i) * GUIDE_IN);
 }
 
 __global__ void k_tri_ng(const float* tri9, const uint32_t* ids, uint32_t n, float4* ng)
@@ -478,6 +643,7 @@ void render_state_free(RenderState* R)
     dfree(R->lnodes);
     dfree(R->leaf_of);
     free_pixels(R);
+    for (void* p : R->guide_allocs) (void)hipFree(p);
     delete R;
 }
 
@@ -493,6 +659,19 @@ int Context::rebuild_shading()
         R->tri_ng_n = total_tris;
     }
     if (n) k_tri_ng<<<nblocks(n, 256), 256>>>(scene.tris, scene.tri_ids, n, R->tri_ng);
+    float lo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, hi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+    for (const Shape& sh : shapes)
+        for (size_t i = 0; i < sh.xyz.size(); i += 3)
+            for (int a = 0; a < 3; ++a)
+            {
+                lo[a] = std::min(lo[a], sh.xyz[i + a]);
+                hi[a] = std::max(hi[a], sh.xyz[i + a]);
+            }
+    for (int a = 0; a < 3; ++a)
+    {
+        R->scene_lo[a] = lo[a] <= hi[a] ? lo[a] : 0.0f;
+        R->scene_hi[a] = lo[a] <= hi[a] ? hi[a] : 1.0f;
+    }
     APG_HIP_CHECK(this, hipGetLastError());
     APG_HIP_CHECK(this, hipDeviceSynchronize());
     return APG_OK;
@@ -653,6 +832,47 @@ int apg_render_reset(apg_context* ctx, uint32_t width, uint32_t height, uint32_t
     return APG_OK;
 }
 
+int apg_render_guide_stats(const apg_context* cctx, uint32_t* samples, float* loss)
+{
+    if (!cctx) return APG_E_INVALID;
+    RenderState* R = state_of(const_cast<apg_context*>(cctx));
+    if (samples) *samples = R->last_train_samples;
+    if (loss) *loss = R->last_loss;
+    return APG_OK;
+}
+
+namespace apg {
+int guide_ensure(Context* ctx, uint32_t cap);
+int guide_forward(Context* ctx, const float* feat, uint32_t M, const uint32_t* M_dev, float* out,
+                  uint16_t* acts, hipStream_t s);
+int guide_train(Context* ctx, const float* feat, const float* dir, const float* weight, uint32_t M,
+                bool adam, float* loss, hipStream_t s);
+}
+
+static int ensure_guide_buffers(apg_context* ctx, RenderState* R, uint32_t depth)
+{
+    const uint32_t P = R->P;
+    const uint32_t tcap = std::min<uint32_t>(P * 2, 1u << 18);
+    if (R->gfeat && R->vrec_depth >= depth && R->tcap == tcap) return APG_OK;
+    if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+    for (void* p : R->guide_allocs) (void)hipFree(p);
+    R->guide_allocs.clear();
+    auto get = [&](auto*& ptr, size_t count) -> int {
+        int rc = dmalloc(ctx, ptr, count);
+        if (rc == APG_OK) R->guide_allocs.push_back(ptr);
+        return rc;
+    };
+    int rc;
+    if ((rc = get(R->gfeat, size_t(P) * GUIDE_IN)) || (rc = get(R->gout, size_t(P) * GUIDE_OUT)) ||
+        (rc = get(R->vrec, size_t(P) * depth * VREC)) || (rc = get(R->tin, size_t(tcap) * 9)) ||
+        (rc = get(R->tfeat, size_t(tcap) * GUIDE_IN)) || (rc = get(R->tdir, size_t(tcap) * 3)) ||
+        (rc = get(R->tw, tcap)) || (rc = get(R->tcount, 4)))
+        return rc;
+    R->vrec_depth = depth;
+    R->tcap = tcap;
+    return guide_ensure(ctx, tcap);
+}
+
 int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
 {
     if (!ctx || !prm) return APG_E_INVALID;
@@ -685,6 +905,24 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
     S.max_depth = prm->max_depth;
     S.rr_depth = prm->rr_depth;
     for (int k = 0; k < 3; ++k) S.bg[k] = prm->background[k];
+    const uint32_t guide = prm->guide;
+    if (guide > 2) return ctx->fail(APG_E_INVALID, "guide must be 0, 1 or 2");
+    if (guide)
+    {
+        int rc = ensure_guide_buffers(ctx, R, prm->max_depth);
+        if (rc) return rc;
+    }
+    S.gout = R->gout;
+    S.vrec = R->vrec;
+    S.vrec_depth = R->vrec_depth;
+    for (int a = 0; a < 3; ++a)
+    {
+        S.lo[a] = R->scene_lo[a];
+        const float ext = R->scene_hi[a] - R->scene_lo[a];
+        S.inv[a] = ext > 0.0f ? 1.0f / ext : 0.0f;
+    }
+    const float frac = prm->guide_frac > 0.0f ? std::min(prm->guide_frac, 0.95f) : 0.5f;
+    const uint32_t train_steps = prm->train_steps ? prm->train_steps : 4u;
     const uint32_t sgrid = std::min<uint32_t>(nblocks(P, 256), uint32_t(ctx->cu_count) * 8);
     const uint32_t rgrid = std::min<uint32_t>(nblocks(P, 256), uint32_t(ctx->cu_count) * 4);
     const uint32_t vgrid = std::min<uint32_t>(nblocks(2 * P, 256), uint32_t(ctx->cu_count) * 4);
@@ -694,6 +932,14 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
     APG_HIP_CHECK(ctx, hipStreamWaitEvent(stream, ctx->last_launch, 0));
     for (uint32_t sidx = 0; sidx < prm->spp; ++sidx)
     {
+        const bool guided = guide != 0 && (guide == 2 || R->guided_passes >= prm->guide_warmup);
+        S.guide_frac = guided ? frac : 0.0f;
+        TrainOut G{ nullptr, 0, 0, nullptr, nullptr, nullptr, 0, nullptr };
+        if (guide == 1)
+        {
+            G = TrainOut{ R->vrec, R->vrec_depth, 0, R->tin, R->tdir, R->tw, R->tcap, R->tcount };
+            APG_HIP_CHECK(ctx, hipMemsetAsync(R->tcount, 0, 4, stream));
+        }
         k_set_counts<<<1, 64, 0, stream>>>(R->counts, P);
         k_generate<<<nblocks(P, 256), 256, 0, stream>>>(R->cam, R->W, R->H, R->rng, R->T, R->L, R->state, R->qa,
                                                           R->stats);
@@ -706,15 +952,45 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
             if (rc) return rc;
             S.in = in;
             S.out = out;
-            k_shade<<<sgrid, 256, 0, stream>>>(S);
+            S.bounce = b;
+            if (guide)
+            {
+                k_guide_features<<<sgrid, 256, 0, stream>>>(S, R->gfeat);
+                rc = guide_forward(ctx, R->gfeat, P, R->counts, R->gout, nullptr, stream);
+                if (rc) return rc;
+                k_shade<true><<<sgrid, 256, 0, stream>>>(S);
+            }
+            else
+                k_shade<false><<<sgrid, 256, 0, stream>>>(S);
             apg_rays srays{ R->qs.o[0], R->qs.o[1], R->qs.o[2], R->qs.d[0], R->qs.d[1], R->qs.d[2], R->qs.tmax };
             rc = launch_trace_raw(ctx, srays, 2 * P, R->counts + 2, nullptr, true, R->occ, nullptr, stream);
             if (rc) return rc;
             k_shadow_vis<<<vgrid, 256, 0, stream>>>(R->counts, R->qs.path, R->occ, R->vis);
+            G.bounce = b;
             k_resolve<<<rgrid, 256, 0, stream>>>(R->counts, in.path, R->state, R->nee, R->vis, R->L, R->accum,
-                                                 R->accum64, prm->fp64_accum ? 1 : 0);
+                                                 R->accum64, prm->fp64_accum ? 1 : 0, G);
             k_advance<<<1, 64, 0, stream>>>(R->counts, R->stats);
             std::swap(in, out);
+        }
+        if (guide == 1)
+        {
+            // online training on this pass's radiance samples
+            uint32_t tn = 0;
+            APG_HIP_CHECK(ctx, hipMemcpyAsync(&tn, R->tcount, 4, hipMemcpyDeviceToHost, stream));
+            APG_HIP_CHECK(ctx, hipStreamSynchronize(stream));
+            tn = std::min(tn, R->tcap);
+            R->last_train_samples = tn;
+            if (tn)
+            {
+                k_train_encode<<<nblocks(tn, 256), 256, 0, stream>>>(R->tin, tn, R->tfeat);
+                for (uint32_t it = 0; it < train_steps; ++it)
+                {
+                    int rc = guide_train(ctx, R->tfeat, R->tdir, R->tw, tn, true,
+                                         it + 1 == train_steps ? &R->last_loss : nullptr, stream);
+                    if (rc) return rc;
+                }
+            }
+            R->guided_passes++;
         }
     }
     APG_HIP_CHECK(ctx, hipGetLastError());

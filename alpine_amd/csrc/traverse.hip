@@ -295,6 +295,7 @@ int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint3
 int launch_trace(Context* ctx, const apg_rays* rays, uint32_t n, const apg_hits* hits, bool any,
                  uint8_t* occ, unsigned long long* counts, hipStream_t stream)
 {
+    if (n == 0) return APG_OK;
     if (!rays || (!any && (!hits || !hits->prim || !hits->t || !hits->b0 || !hits->b1)) ||
         (any && !occ))
         return ctx->fail(APG_E_INVALID, "null ray or result pointer");

@@ -259,3 +259,54 @@ def cornell_scene(width=256, height=256, spp=16, max_depth=5, seed=1, wall_res=3
                fovy=float(np.radians(40.0)), aspect=width / height)
     return RenderScene(verts, tris, tri_mat, materials, lights, cam, width, height, spp,
                        max_depth, light_sampler=0, seed=seed)
+
+
+def doorway_scene(width=128, height=128, spp=16, max_depth=8, seed=1, res=16):
+    """Two rooms joined by a doorway; the only light (a 0.3 x 0.3 quad of radiance 60) is in
+    the far room, so everything the camera sees is lit indirectly through the opening."""
+    g = 0
+    parts = []
+
+    def add(o, u, v, mat=g, n=res):
+        parts.append((quad_mesh(o, u, v, n, n), mat))
+
+    # room A (camera), z in [-1, 1]; inward normals
+    add((-1, 0, -1), (0, 0, 2), (2, 0, 0))          # floor
+    add((-1, 2, -1), (2, 0, 0), (0, 0, 2))          # ceiling
+    add((-1, 0, -1), (0, 2, 0), (0, 0, 2), 1)       # left (red)
+    add((1, 0, -1), (0, 0, 2), (0, 2, 0), 2)        # right (green)
+    add((-1, 0, -1), (2, 0, 0), (0, 2, 0))          # front, behind the camera (+z)
+    # partition: A side at z = 1 (-z normal), B side at z = 1.02 (+z normal), opening
+    # x in [-0.25, 0.25], y in [0, 1]
+    t = 0.02
+    for (x0, x1, y0, y1) in [(-1, -0.25, 0, 2), (0.25, 1, 0, 2), (-0.25, 0.25, 1, 2)]:
+        add((x0, y0, 1), (0, y1 - y0, 0), (x1 - x0, 0, 0), g, 8)
+        add((x0, y0, 1 + t), (x1 - x0, 0, 0), (0, y1 - y0, 0), g, 8)
+    # jambs and lintel of the opening
+    add((-0.25, 0, 1), (0, 0, t), (0, 1, 0), g, 2)   # left jamb faces +x
+    add((0.25, 0, 1), (0, 1, 0), (0, 0, t), g, 2)    # right jamb faces -x
+    add((-0.25, 1, 1), (0.5, 0, 0), (0, 0, t), g, 2)  # lintel faces -y
+    # room B, z in [1.02, 3]
+    zb, lb = 1 + t, 3 - (1 + t)
+    add((-1, 0, zb), (0, 0, lb), (2, 0, 0))         # floor
+    add((-1, 2, zb), (2, 0, 0), (0, 0, lb))         # ceiling
+    add((-1, 0, zb), (0, 2, 0), (0, 0, lb))         # left
+    add((1, 0, zb), (0, 0, lb), (0, 2, 0))          # right
+    add((-1, 0, 3), (0, 2, 0), (2, 0, 0))           # back
+    verts, tris = concat(*[m for m, _ in parts])
+    tri_mat = np.concatenate([np.full(len(m[1]), k, np.uint32) for m, k in parts])
+    materials = [(0, (0.7, 0.7, 0.7), (1.0, 1.0)), (0, (0.65, 0.1, 0.1), (1.0, 1.0)),
+                 (0, (0.12, 0.45, 0.15), (1.0, 1.0))]
+    lights = [dict(kind=2, p0=(-0.15, 1.99, 2.4), p1=(0.15, 1.99, 2.4), p2=(-0.15, 1.99, 2.7),
+                   color=(60.0, 55.0, 45.0))]
+    cam = dict(eye=(0.3, 1.0, -0.9), target=(-0.1, 0.7, 1.0), up=(0.0, 1.0, 0.0),
+               fovy=float(np.radians(60.0)), aspect=width / height)
+    return RenderScene(verts, tris, tri_mat, materials, lights, cam, width, height, spp,
+                       max_depth, light_sampler=0, seed=seed)
+
+
+def rel_mse(img, ref, eps=1e-2):
+    """mean over pixels and channels of (x - r)^2 / (r^2 + eps)."""
+    img = np.asarray(img, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.mean((img - ref) ** 2 / (ref * ref + eps)))

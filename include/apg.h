@@ -189,8 +189,11 @@ typedef struct apg_render_params {
     uint32_t rr_depth;         /* 0: no Russian roulette; else from this depth on */
     uint32_t fp64_accum;       /* also accumulate in double */
     float background[3];       /* radiance on a miss (alpine.cpp:318-324) */
-    uint32_t guide;            /* 0 = plain; see apg_guide_* */
-    uint32_t reserved[6];
+    uint32_t guide;            /* 0: plain; 1: guided + online training; 2: guided, frozen */
+    uint32_t train_steps;      /* Adam steps after each pass (guide = 1); 0 = 4 */
+    uint32_t guide_warmup;     /* passes traced before the proposal is mixed in (guide = 1) */
+    float guide_frac;          /* probability of sampling the proposal; 0 = 0.5 */
+    uint32_t reserved[3];
 } apg_render_params;
 
 /* Allocate per-pixel samplers (seeded with pixel index + seed, alpine.cpp:259-269) and clear
@@ -206,6 +209,8 @@ int apg_render_read(apg_context* ctx, float* rgb, double* rgb64, uint32_t* total
  * rays, path vertices shaded. */
 int apg_render_counters(const apg_context* ctx, uint64_t* closest_rays, uint64_t* shadow_rays,
                         uint64_t* vertices);
+/* Guided rendering: training samples of the last pass and the last training loss (host). */
+int apg_render_guide_stats(const apg_context* ctx, uint32_t* train_samples, float* loss);
 
 /* ---- learned path guiding (no reference counterpart) ----
  * An MLP (96 -> 4 x 64 -> 48, BF16 MFMA, FP32 accumulation) maps a shading point to an

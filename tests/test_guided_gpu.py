@@ -1,0 +1,68 @@
+"""Guided path tracing on the GPU: the learned proposal keeps the estimator unbiased and,
+once trained, lowers the error of the plain path tracer at equal sample count."""
+import numpy as np
+import pytest
+
+import alpine_amd as aa
+from alpine_amd import scenes
+from tests.render_util import gpu_render
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(sc):
+    acc = aa.GpuAccelerator(0)
+    acc.append_mesh(sc.verts, sc.tris)
+    acc.update_scene()
+    acc.set_materials([aa.material(*m) for m in sc.materials], sc.tri_mat)
+    acc.set_lights([aa.light(**l) for l in sc.lights], sc.light_sampler)
+    acc.set_camera(aa.camera(**sc.camera))
+    acc.guide_init(seed=1, lr=1e-3)
+    return acc
+
+
+def test_guided_estimator_is_unbiased():
+    """Image mean of guided PT equals plain PT's within Monte Carlo error."""
+    print("[gpu] guided unbiased", flush=True)
+    sc = scenes.cornell_scene(64, 64)
+    acc = _setup(sc)
+    acc.render_reset(64, 64, 1)
+    acc.render(64, sc.max_depth)
+    plain, _, _ = acc.render_read()
+    acc.render_reset(64, 64, 7)
+    acc.render(64, sc.max_depth, guide=1, guide_warmup=4)
+    guided, _, spp = acc.render_read()
+    n, loss = acc.guide_stats()
+    print("plain mean", plain.mean() / 64, "guided mean", guided.mean() / 64, "train n", n, "loss", loss)
+    assert np.isfinite(guided).all()
+    assert n > 0
+    assert guided.mean() == pytest.approx(plain.mean(), rel=0.03)
+
+
+def test_trained_guide_lowers_error_on_doorway():
+    """Hard indirect lighting: after online training the frozen proposal beats plain PT at
+    equal spp against a high-spp plain reference."""
+    print("[gpu] doorway guided vs plain", flush=True)
+    w = h = 64
+    sc = scenes.doorway_scene(w, h)
+    acc = _setup(sc)
+    acc.render_reset(w, h, 100)
+    acc.render(1024, sc.max_depth, fp64=True)
+    _, ref64, spp_ref = acc.render_read(fp64=True)
+    ref = ref64 / spp_ref
+    # train online for 32 passes
+    acc.render_reset(w, h, 200)
+    acc.render(32, sc.max_depth, guide=1, guide_warmup=2, train_steps=8)
+    # equal spp, fresh seeds
+    spp = 16
+    acc.render_reset(w, h, 300)
+    acc.render(spp, sc.max_depth)
+    plain, _, _ = acc.render_read()
+    acc.render_reset(w, h, 300)
+    acc.render(spp, sc.max_depth, guide=2, guide_frac=0.5)
+    guided, _, _ = acc.render_read()
+    e_plain = scenes.rel_mse(plain / spp, ref)
+    e_guided = scenes.rel_mse(guided / spp, ref)
+    print("relMSE plain", e_plain, "guided", e_guided, "ratio", e_guided / e_plain)
+    assert np.isfinite(guided).all()
+    assert e_guided < e_plain
