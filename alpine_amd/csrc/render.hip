@@ -26,6 +26,12 @@
 
 namespace apg {
 
+int guide_ensure(Context* ctx, uint32_t cap);
+int guide_forward(Context* ctx, const float* feat, uint32_t M, const uint32_t* M_dev, float* out,
+                  uint16_t* acts, hipStream_t s);
+int guide_train(Context* ctx, const float* feat, const float* dir, const float* weight, uint32_t M,
+                bool adam, float* loss, hipStream_t s);
+
 struct Queue
 {
     float* o[3];
@@ -839,14 +845,6 @@ int apg_render_guide_stats(const apg_context* cctx, uint32_t* samples, float* lo
     if (samples) *samples = R->last_train_samples;
     if (loss) *loss = R->last_loss;
     return APG_OK;
-}
-
-namespace apg {
-int guide_ensure(Context* ctx, uint32_t cap);
-int guide_forward(Context* ctx, const float* feat, uint32_t M, const uint32_t* M_dev, float* out,
-                  uint16_t* acts, hipStream_t s);
-int guide_train(Context* ctx, const float* feat, const float* dir, const float* weight, uint32_t M,
-                bool adam, float* loss, hipStream_t s);
 }
 
 static int ensure_guide_buffers(apg_context* ctx, RenderState* R, uint32_t depth)

@@ -310,3 +310,48 @@ def rel_mse(img, ref, eps=1e-2):
     img = np.asarray(img, np.float64)
     ref = np.asarray(ref, np.float64)
     return float(np.mean((img - ref) ** 2 / (ref * ref + eps)))
+
+
+def many_light_scene(width=1920, height=1080, spp=64, max_depth=8, rr_depth=3, grid=64,
+                     face_res=4, n_lights=10_000, seed=5):
+    """64 x 64 grid of boxes (4 x 4 quads per face -> 192 triangles per box, 786,432 in
+    all) on a ground plane, lit by 10 K emissive triangles (lights only) with log-normal
+    power (mu 0, sigma 1.5).  Materials alternate between rough conductors (alpha uniform
+    0.05-0.8, F0 uniform) and coated dielectrics (F0 0.04, alpha uniform 0.05-0.8).
+    Light-BVH sampler, max depth 8, Russian roulette from depth 3."""
+    rng = np.random.default_rng(seed)
+    meshes, mats, tri_mat = [], [], []
+    half = grid / 2.0
+    n_mat = 64
+    for k in range(n_mat):
+        a = float(rng.uniform(0.05, 0.8))
+        if k % 2 == 0:
+            mats.append((1, tuple(rng.uniform(0.2, 1.0, 3)), (a, a)))
+        else:
+            mats.append((0, tuple(rng.uniform(0.1, 0.9, 3)), (a, a)))
+    ground = quad_mesh((-half, 0.0, -half), (0, 0, 2 * half), (2 * half, 0, 0), 64, 64)
+    meshes.append(ground)
+    tri_mat.append(np.zeros(len(ground[1]), np.uint32))
+    for i in range(grid):
+        for j in range(grid):
+            sx, sz = rng.uniform(0.3, 0.8, 2)
+            h = rng.uniform(0.2, 2.0)
+            ang = rng.uniform(0, 90)
+            pos = (-half + i + 0.5, 0.0, -half + j + 0.5)
+            m = box_mesh(sx, h, sz, ang, pos, face_res)
+            meshes.append(m)
+            tri_mat.append(np.full(len(m[1]), rng.integers(0, n_mat), np.uint32))
+    verts, tris = concat(*meshes)
+    tri_mat = np.concatenate(tri_mat)
+    # emissive triangles, facing down
+    c = np.stack([rng.uniform(-half, half, n_lights), rng.uniform(2.5, 4.0, n_lights),
+                  rng.uniform(-half, half, n_lights)], 1)
+    power = np.exp(rng.normal(0.0, 1.5, n_lights))
+    col = rng.uniform(0.5, 1.0, (n_lights, 3)) * power[:, None] * 4.0
+    s = 0.08
+    lights = [dict(kind=3, p0=c[k], p1=c[k] + (s, 0, 0), p2=c[k] + (0, 0, s), color=col[k])
+              for k in range(n_lights)]
+    cam = dict(eye=(0.0, 14.0, -34.0), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0),
+               fovy=float(np.radians(45.0)), aspect=width / height)
+    return RenderScene(verts, tris, tri_mat, mats, lights, cam, width, height, spp, max_depth,
+                       light_sampler=2, rr_depth=rr_depth, seed=seed)

@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=262_144)
     ap.add_argument("--builder", choices=["gpu", "host"], default="gpu")
+    ap.add_argument("--no-render", action="store_true", help="skip the path-tracing configs")
     return ap.parse_args()
 
 
@@ -88,6 +89,93 @@ def hbm_copy_gbs():
     ms = a.elapsed_time(b) / 5
     del x, y
     return 2 * n * 4 / (ms * 1e-3) / 1e9
+
+
+def _render_setup(sc, builder):
+    import alpine_amd as aa
+    acc = aa.GpuAccelerator(0)
+    acc.append_mesh(sc.verts, sc.tris)
+    acc.update_scene(builder=builder)
+    acc.set_materials([aa.material(*m) for m in sc.materials], sc.tri_mat)
+    acc.set_lights([aa.light(**l) for l in sc.lights], sc.light_sampler)
+    acc.set_camera(aa.camera(**sc.camera))
+    return acc
+
+
+def _timed_render(acc, sc, spp, seed, **kw):
+    import torch
+    acc.render_reset(sc.width, sc.height, seed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    acc.render(spp, sc.max_depth, sc.rr_depth, sc.background, **kw)
+    img, img64, total = acc.render_read(fp64=kw.get("fp64", False))
+    return time.perf_counter() - t0, img, img64
+
+
+def render_benchmarks(builder):
+    """Path-tracing configs: Cornell room (16 spp, 256^2), many-light GGX (1920x1080, 64 spp,
+    light BVH, RR from depth 3) and guided vs plain time to equal relMSE (doorway scene)."""
+    from alpine_amd import scenes
+    out = {}
+    # config 1: Cornell room
+    sc = scenes.cornell_scene()
+    acc = _render_setup(sc, builder)
+    _timed_render(acc, sc, 1, 0)
+    dt, img, _ = _timed_render(acc, sc, sc.spp, sc.seed)
+    ref_spp = 1024
+    _, _, ref64 = _timed_render(acc, sc, ref_spp, 1000, fp64=True)
+    ref = ref64 / ref_spp
+    c = acc.render_counters()
+    out["cornell_256_16spp"] = {
+        "seconds": round(dt, 4), "samples_per_s": round(sc.width * sc.height * sc.spp / dt, 1),
+        "rel_mse_vs_1024spp_fp64": round(scenes.rel_mse(img / sc.spp, ref), 6),
+        "rays_per_s": round((c["closest_rays"] + c["shadow_rays"]) / dt, 1) if dt > 0 else None}
+    del acc
+    # config 3: many-light GGX
+    log("[bench] many-light scene")
+    sc = scenes.many_light_scene()
+    acc = _render_setup(sc, builder)
+    _timed_render(acc, sc, 1, 0)
+    dt, img, _ = _timed_render(acc, sc, sc.spp, sc.seed)
+    c = acc.render_counters()
+    out["many_light_1080p_64spp"] = {
+        "triangles": int(len(sc.tris)), "lights": len(sc.lights), "seconds": round(dt, 3),
+        "samples_per_s": round(sc.width * sc.height * sc.spp / dt, 1),
+        "rays_per_s": round((c["closest_rays"] + c["shadow_rays"]) / dt, 1),
+        "finite": bool(np.isfinite(img).all())}
+    del acc
+    # config 5: guided vs plain, time to equal relMSE on the doorway scene
+    log("[bench] guided vs plain")
+    sc = scenes.doorway_scene(256, 256)
+    acc = _render_setup(sc, builder)
+    acc.guide_init(seed=1, lr=1e-3)
+    _, _, ref64 = _timed_render(acc, sc, 2048, 4000, fp64=True)
+    ref = ref64 / 2048
+    spp_plain = 64
+    t_plain, img, _ = _timed_render(acc, sc, spp_plain, 11)
+    e_plain = scenes.rel_mse(img / spp_plain, ref)
+    # guided: train online, then render with the frozen proposal; time includes training
+    t_guided, e_guided, spp_g = None, None, None
+    import torch
+    acc.render_reset(sc.width, sc.height, 21)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    acc.render(16, sc.max_depth, guide=1, guide_warmup=2, train_steps=8)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    for spp in (4, 8, 12, 16, 24, 32, 48, 64):
+        dt, img, _ = _timed_render(acc, sc, spp, 31, guide=2)
+        e = scenes.rel_mse(img / spp, ref)
+        if e <= e_plain:
+            t_guided, e_guided, spp_g = t_train + dt, e, spp
+            break
+    out["guided_doorway_256"] = {
+        "plain_spp": spp_plain, "plain_seconds": round(t_plain, 4), "plain_rel_mse": round(e_plain, 6),
+        "guided_spp": spp_g, "guided_seconds_incl_training": None if t_guided is None else round(t_guided, 4),
+        "guided_rel_mse": None if e_guided is None else round(e_guided, 6),
+        "training_seconds": round(t_train, 4),
+        "time_ratio_guided_over_plain": None if t_guided is None else round(t_guided / t_plain, 4)}
+    return out
 
 
 def cpu_baseline(verts, tris, org, dir, sample):
@@ -200,6 +288,9 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_render:
+        log("[bench] render configs")
+        result["extra"]["render"] = render_benchmarks(builder)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         cb, href, _ = cpu_baseline(verts, tris, org, dir, args.cpu_sample)
