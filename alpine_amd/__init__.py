@@ -105,7 +105,8 @@ def camera(eye, target, up, fovy, aspect):
 
 class TraceConfig(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("fetch_threshold", C.c_uint32), ("reserved", C.c_uint32 * 5)]
+                ("fetch_threshold", C.c_uint32), ("stack_lds", C.c_uint32),
+                ("reserved", C.c_uint32 * 4)]
 
 
 _lib = None
@@ -113,7 +114,7 @@ _lib = None
 # Every symbol the C ABI headers declare (checked by tests/test_abi.py).
 ABI_SYMBOLS = [
     "apg_create", "apg_destroy", "apg_last_error", "apg_mesh_append", "apg_mesh_vertices",
-    "apg_mesh_update", "apg_mesh_prim_offset", "apg_sphere_append", "apg_scene_build",
+    "apg_mesh_update", "apg_mesh_prim_offset", "apg_mesh_transform", "apg_sphere_append", "apg_scene_build",
     "apg_scene_get_stats", "apg_trace_config_default", "apg_set_trace_config", "apg_intersect",
     "apg_occluded", "apg_intersect_counted", "apg_scene_triangles", "apg_scene_leaf_keys",
     "apg_scene_wide_nodes", "apg_materials_set", "apg_lights_set", "apg_light_sampler_sample",
@@ -133,6 +134,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ApgError(-2, f"{LIB_PATH} is missing: run `python -m alpine_amd.build`")
+    # torch ships its own HIP runtime under the same soname (libamdhip64.so.7) as the
+    # ROCm one libapg.so links; load torch first so the process has a single runtime
+    # (the one torch was built against) whichever module the caller touches first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int
     sig = {
@@ -143,6 +151,7 @@ def lib():
         "apg_mesh_vertices": (i32, [vp, u32, C.POINTER(C.POINTER(C.c_float)), C.POINTER(u32)]),
         "apg_mesh_update": (i32, [vp, u32]),
         "apg_mesh_prim_offset": (i32, [vp, u32, C.POINTER(u32)]),
+        "apg_mesh_transform": (i32, [vp, u32, vp, u32, u32, vp]),
         "apg_sphere_append": (i32, [vp, vp, u32]),
         "apg_scene_build": (i32, [vp, C.POINTER(_BuildParams)]),
         "apg_scene_get_stats": (i32, [vp, C.POINTER(_SceneStats)]),
@@ -249,6 +258,20 @@ class GpuAccelerator:
 
     def update_shape(self, shape_id: int):
         self._rc(self._lib.apg_mesh_update(self._h, shape_id))
+
+    def transform_vertices(self, shape_id, m34, first=0, count=None, stream=None):
+        """Rigid 3 x 4 transform of a vertex range, applied on the GPU (no upload)."""
+        m = np.ascontiguousarray(m34, np.float32).reshape(12)
+        if count is None:
+            count = self.vertex_count(shape_id) - first
+        self._rc(self._lib.apg_mesh_transform(self._h, shape_id, _ptr(m), first, count,
+                                              self._stream(stream)))
+
+    def vertex_count(self, shape_id):
+        p = C.POINTER(C.c_float)()
+        nv = C.c_uint32()
+        self._rc(self._lib.apg_mesh_vertices(self._h, shape_id, C.byref(p), C.byref(nv)))
+        return nv.value
 
     def prim_offset(self, shape_id: int) -> int:
         o = C.c_uint32()

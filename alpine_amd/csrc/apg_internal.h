@@ -50,10 +50,22 @@ struct Context
     std::vector<Shape> shapes;
     uint32_t total_tris = 0;
 
+    // device copy of all vertices (meshes concatenated) and global-index triangles;
+    // geom_dirty: host copy changed (upload at next build or transform);
+    // host_stale: device transforms changed the device copy (download before host access)
+    float* d_xyz = nullptr;
+    uint32_t* d_tri = nullptr;
+    size_t d_nv = 0, d_nt = 0;
+    bool geom_dirty = true;
+    bool host_stale = false;
+    int sync_geometry_to_device(hipStream_t s);
+    int sync_geometry_to_host();
+
     // scene
     bool built = false;
     DeviceScene scene;
     apg_scene_stats stats{};
+    float scene_lo[3] = { 0, 0, 0 }, scene_hi[3] = { 1, 1, 1 };
     std::vector<uint32_t> leaf_keys;  // global triangle -> smallest index in its leaf
 
     // traversal launch state
@@ -99,6 +111,7 @@ struct BinaryTree
     uint32_t max_depth = 0;
     double sah_cost = 0.0, sah_cost_leaf = 0.0;
     double build_ms = 0.0;  // device build time (GPU builder), excluding statistics
+    float scene_lo[3] = { 0, 0, 0 }, scene_hi[3] = { 0, 0, 0 };
     std::vector<uint32_t> leaf_keys;
 };
 void build_binary_host(const std::vector<TriPrep>& tris, uint32_t leaf_threshold,

@@ -26,6 +26,8 @@ Context::~Context()
     if (scene.tris) (void)hipFree(scene.tris);
     if (scene.tri_ids) (void)hipFree(scene.tri_ids);
     if (counters) (void)hipFree(counters);
+    if (d_xyz) (void)hipFree(d_xyz);
+    if (d_tri) (void)hipFree(d_tri);
     if (spill) (void)hipFree(spill);
     if (last_launch) (void)hipEventDestroy(last_launch);
     if (render) render_state_free(render);
@@ -54,6 +56,57 @@ static int upload_scene(Context* ctx, const std::vector<ApgWideNode>& nodes,
     APG_HIP_CHECK(ctx, hipMemcpy(s.nodes, nodes.data(), nodes.size() * sizeof(ApgWideNode), hipMemcpyHostToDevice));
     APG_HIP_CHECK(ctx, hipMemcpy(s.tris, tri9.data(), tri9.size() * sizeof(float), hipMemcpyHostToDevice));
     APG_HIP_CHECK(ctx, hipMemcpy(s.tri_ids, ids.data(), ids.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return APG_OK;
+}
+
+int Context::sync_geometry_to_device(hipStream_t s)
+{
+    if (!geom_dirty && d_xyz) return APG_OK;
+    size_t nv = 0;
+    for (const Shape& sh : shapes) nv += sh.xyz.size() / 3;
+    if (nv != d_nv || total_tris != d_nt || !d_xyz)
+    {
+        if (d_xyz) (void)hipFree(d_xyz);
+        if (d_tri) (void)hipFree(d_tri);
+        d_xyz = nullptr;
+        d_tri = nullptr;
+        APG_HIP_CHECK(this, hipMalloc(&d_xyz, std::max<size_t>(1, nv) * 3 * sizeof(float)));
+        APG_HIP_CHECK(this, hipMalloc(&d_tri, std::max<size_t>(1, total_tris) * 3 * sizeof(uint32_t)));
+        d_nv = nv;
+        d_nt = total_tris;
+    }
+    std::vector<uint32_t> tri;
+    tri.reserve(size_t(total_tris) * 3);
+    size_t vbase = 0;
+    for (const Shape& sh : shapes)
+    {
+        if (!sh.xyz.empty())
+            APG_HIP_CHECK(this, hipMemcpyAsync(d_xyz + 3 * vbase, sh.xyz.data(), sh.xyz.size() * sizeof(float),
+                                               hipMemcpyHostToDevice, s));
+        for (uint32_t v : sh.idx) tri.push_back(uint32_t(v + vbase));
+        vbase += sh.xyz.size() / 3;
+    }
+    if (!tri.empty())
+        APG_HIP_CHECK(this, hipMemcpyAsync(d_tri, tri.data(), tri.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    APG_HIP_CHECK(this, hipStreamSynchronize(s));
+    geom_dirty = false;
+    host_stale = false;
+    return APG_OK;
+}
+
+int Context::sync_geometry_to_host()
+{
+    if (!host_stale) return APG_OK;
+    APG_HIP_CHECK(this, hipDeviceSynchronize());
+    size_t vbase = 0;
+    for (Shape& sh : shapes)
+    {
+        if (!sh.xyz.empty())
+            APG_HIP_CHECK(this, hipMemcpy(sh.xyz.data(), d_xyz + 3 * vbase, sh.xyz.size() * sizeof(float),
+                                          hipMemcpyDeviceToHost));
+        vbase += sh.xyz.size() / 3;
+    }
+    host_stale = false;
     return APG_OK;
 }
 
@@ -121,6 +174,7 @@ int apg_mesh_append(apg_context* ctx, const float* xyz, uint32_t nv, const uint3
     if ((nv && !xyz) || (nt && !idx)) return ctx->fail(APG_E_INVALID, "null geometry pointer");
     if (uint64_t(ctx->total_tris) + nt > 0x7fffffffull)
         return ctx->fail(APG_E_CAPACITY, "scene exceeds 2^31 - 1 triangles");
+    if (int rc = ctx->sync_geometry_to_host()) return rc;
     for (uint64_t i = 0; i < uint64_t(nt) * 3; ++i)
         if (idx[i] >= nv) return ctx->fail(APG_E_INVALID, "triangle index out of range");
     apg::Shape sh;
@@ -129,6 +183,7 @@ int apg_mesh_append(apg_context* ctx, const float* xyz, uint32_t nv, const uint3
     sh.prim_offset = ctx->total_tris;
     ctx->total_tris += nt;
     ctx->shapes.push_back(std::move(sh));
+    ctx->geom_dirty = true;
     if (shape_id) *shape_id = uint32_t(ctx->shapes.size() - 1);
     return APG_OK;
 }
@@ -137,6 +192,7 @@ int apg_mesh_vertices(apg_context* ctx, uint32_t shape_id, float** xyz, uint32_t
 {
     if (!ctx || !xyz) return APG_E_INVALID;
     if (shape_id >= ctx->shapes.size()) return ctx->fail(APG_E_INVALID, "unknown shape id");
+    if (int rc = ctx->sync_geometry_to_host()) return rc;
     *xyz = ctx->shapes[shape_id].xyz.data();
     if (nv) *nv = uint32_t(ctx->shapes[shape_id].xyz.size() / 3);
     return APG_OK;
@@ -147,6 +203,7 @@ int apg_mesh_update(apg_context* ctx, uint32_t shape_id)
     if (!ctx) return APG_E_INVALID;
     if (shape_id >= ctx->shapes.size()) return ctx->fail(APG_E_INVALID, "unknown shape id");
     // Triangle data is re-derived from the vertex copy at the next build.
+    ctx->geom_dirty = true;
     return APG_OK;
 }
 
@@ -175,14 +232,18 @@ int apg_scene_build(apg_context* ctx, const apg_build_params* params)
 
     // Wait for kernels still reading the old scene.
     if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+    int rc = APG_OK;
 
     std::vector<apg::TriPrep> tris;
-    if (builder == APG_BUILDER_HOST) apg::prepare_triangles(*ctx, tris);
+    if (builder == APG_BUILDER_HOST)
+    {
+        if ((rc = ctx->sync_geometry_to_host())) return rc;
+        apg::prepare_triangles(*ctx, tris);
+    }
     apg::BinaryTree bt;
     std::vector<ApgWideNode> nodes;
     std::vector<float> tri9;
     std::vector<uint32_t> ids;
-    int rc = APG_OK;
     if (builder == APG_BUILDER_HOST)
     {
         apg::build_binary_host(tris, leaf_threshold, bt);
@@ -198,6 +259,11 @@ int apg_scene_build(apg_context* ctx, const apg_build_params* params)
     if (rc) return rc;
 
     ctx->leaf_keys = std::move(bt.leaf_keys);
+    for (int a = 0; a < 3; ++a)
+    {
+        ctx->scene_lo[a] = bt.scene_lo[a];
+        ctx->scene_hi[a] = bt.scene_hi[a];
+    }
     apg_scene_stats& st = ctx->stats;
     st = apg_scene_stats{};
     st.triangles = ctx->total_tris;
@@ -232,6 +298,7 @@ int apg_trace_config_default(const apg_context* ctx, apg_trace_config* cfg)
     cfg->block_size = 256;
     cfg->blocks_per_cu = 8;
     cfg->fetch_threshold = 16;
+    cfg->stack_lds = 8;
     return APG_OK;
 }
 
@@ -244,6 +311,8 @@ int apg_set_trace_config(apg_context* ctx, const apg_trace_config* cfg)
         return ctx->fail(APG_E_INVALID, "blocks_per_cu must be in 1..64");
     if (cfg->fetch_threshold == 0 || cfg->fetch_threshold > 64)
         return ctx->fail(APG_E_INVALID, "fetch_threshold must be in 1..64");
+    if (cfg->stack_lds != 8 && cfg->stack_lds != 16)
+        return ctx->fail(APG_E_INVALID, "stack_lds must be 8 or 16");
     ctx->cfg = *cfg;
     return APG_OK;
 }

@@ -135,6 +135,18 @@ __global__ void k_prep(const float* __restrict__ xyz, const uint32_t* __restrict
     }
 }
 
+// Rigid transform (3 x 4 row-major) of a vertex range of the device geometry.
+__global__ void k_transform(float* xyz, uint32_t n, float m0, float m1, float m2, float m3, float m4,
+                            float m5, float m6, float m7, float m8, float m9, float m10, float m11)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float x = xyz[3 * size_t(i)], y = xyz[3 * size_t(i) + 1], z = xyz[3 * size_t(i) + 2];
+    xyz[3 * size_t(i)] = m0 * x + m1 * y + m2 * z + m3;
+    xyz[3 * size_t(i) + 1] = m4 * x + m5 * y + m6 * z + m7;
+    xyz[3 * size_t(i) + 2] = m8 * x + m9 * y + m10 * z + m11;
+}
+
 __global__ void k_init_slots(uint32_t n, uint32_t* idx, uint32_t* snode)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -851,26 +863,11 @@ int build_scene_gpu(Context* ctx, const std::vector<TriPrep>& /*host_prep*/, uin
     if (n == 0) return APG_OK;
 
     Scratch scr;
-    // upload geometry (meshes concatenated, indices made global)
-    size_t nv_total = 0;
-    for (const Shape& sh : ctx->shapes) nv_total += sh.xyz.size() / 3;
-    std::vector<float> xyz;
-    std::vector<uint32_t> tri;
-    xyz.reserve(nv_total * 3);
-    tri.reserve(size_t(n) * 3);
-    for (const Shape& sh : ctx->shapes)
-    {
-        const uint32_t vbase = uint32_t(xyz.size() / 3);
-        xyz.insert(xyz.end(), sh.xyz.begin(), sh.xyz.end());
-        for (uint32_t v : sh.idx) tri.push_back(v + vbase);
-    }
-    float* d_xyz;
-    uint32_t* d_tri;
     int rc;
-    if ((rc = scr.get(ctx, d_xyz, xyz.size()))) return rc;
-    if ((rc = scr.get(ctx, d_tri, tri.size()))) return rc;
-    APG_HIP_CHECK(ctx, hipMemcpyAsync(d_xyz, xyz.data(), xyz.size() * 4, hipMemcpyHostToDevice, s));
-    APG_HIP_CHECK(ctx, hipMemcpyAsync(d_tri, tri.data(), tri.size() * 4, hipMemcpyHostToDevice, s));
+    // geometry lives on the device (uploaded when the host copy changed)
+    if ((rc = ctx->sync_geometry_to_device(s))) return rc;
+    const float* d_xyz = ctx->d_xyz;
+    const uint32_t* d_tri = ctx->d_tri;
 
     BuildState S;
     const size_t ncap = 2 * size_t(n) + 2;
@@ -1037,6 +1034,11 @@ int build_scene_gpu(Context* ctx, const std::vector<TriPrep>& /*host_prep*/, uin
         stack.push_back({ hleft[it.k] + 1, it.d + 1 });
         stack.push_back({ hleft[it.k], it.d + 1 });
     }
+    for (int a = 0; a < 3; ++a)
+    {
+        bt.scene_lo[a] = float_of(hbox[a]);
+        bt.scene_hi[a] = float_of(hbox[3 + a]);
+    }
     const double root_sa = area(0);
     bt.sah_cost = (inner + leaf) / root_sa;
     bt.sah_cost_leaf = leaf / root_sa;
@@ -1047,3 +1049,23 @@ int build_scene_gpu(Context* ctx, const std::vector<TriPrep>& /*host_prep*/, uin
 }
 
 } // namespace apg
+
+extern "C" int apg_mesh_transform(apg_context* ctx, uint32_t shape_id, const float* m,
+                                  uint32_t first, uint32_t count, void* stream)
+{
+    using namespace apg;
+    if (!ctx || !m) return APG_E_INVALID;
+    if (shape_id >= ctx->shapes.size()) return ctx->fail(APG_E_INVALID, "unknown shape id");
+    const uint32_t nv = uint32_t(ctx->shapes[shape_id].xyz.size() / 3);
+    if (uint64_t(first) + count > nv) return ctx->fail(APG_E_INVALID, "vertex range out of bounds");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (int rc = ctx->sync_geometry_to_device(s)) return rc;
+    size_t vbase = 0;
+    for (uint32_t k = 0; k < shape_id; ++k) vbase += ctx->shapes[k].xyz.size() / 3;
+    if (count)
+        k_transform<<<(count + 255) / 256, 256, 0, s>>>(ctx->d_xyz + 3 * (vbase + first), count, m[0], m[1], m[2],
+                                                       m[3], m[4], m[5], m[6], m[7], m[8], m[9], m[10], m[11]);
+    APG_HIP_CHECK(ctx, hipGetLastError());
+    ctx->host_stale = true;
+    return APG_OK;
+}

@@ -277,6 +277,7 @@ void build_binary_host(const std::vector<TriPrep>& tris, uint32_t leaf_threshold
         st.push_back({ nd.right, it.depth + 1 });
         st.push_back({ nd.left, it.depth + 1 });
     }
+    for (int a = 0; a < 3; ++a) { out.scene_lo[a] = out.nodes[0].lo[a]; out.scene_hi[a] = out.nodes[0].hi[a]; }
     double root_sa = double(area_of(out.nodes[0]));
     out.sah_cost = (inner + leaf) / root_sa;
     out.sah_cost_leaf = leaf / root_sa;

@@ -665,18 +665,11 @@ int Context::rebuild_shading()
         R->tri_ng_n = total_tris;
     }
     if (n) k_tri_ng<<<nblocks(n, 256), 256>>>(scene.tris, scene.tri_ids, n, R->tri_ng);
-    float lo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, hi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
-    for (const Shape& sh : shapes)
-        for (size_t i = 0; i < sh.xyz.size(); i += 3)
-            for (int a = 0; a < 3; ++a)
-            {
-                lo[a] = std::min(lo[a], sh.xyz[i + a]);
-                hi[a] = std::max(hi[a], sh.xyz[i + a]);
-            }
     for (int a = 0; a < 3; ++a)
     {
-        R->scene_lo[a] = lo[a] <= hi[a] ? lo[a] : 0.0f;
-        R->scene_hi[a] = lo[a] <= hi[a] ? hi[a] : 1.0f;
+        const bool ok = scene_lo[a] <= scene_hi[a];
+        R->scene_lo[a] = ok ? scene_lo[a] : 0.0f;
+        R->scene_hi[a] = ok ? scene_hi[a] : 1.0f;
     }
     APG_HIP_CHECK(this, hipGetLastError());
     APG_HIP_CHECK(this, hipDeviceSynchronize());

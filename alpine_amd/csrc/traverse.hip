@@ -26,9 +26,10 @@
 
 namespace apg {
 
-constexpr uint32_t APG_STACK_LDS = 8;
-constexpr uint32_t APG_STACK_SPILL = 56;
-constexpr uint32_t APG_STACK_MAX = APG_STACK_LDS + APG_STACK_SPILL;
+// Stack entries per lane: STACK in LDS (8 or 16, chosen by the launch
+// configuration) and APG_STACK_MAX - STACK in the per-thread HBM spill area.
+constexpr uint32_t APG_STACK_MAX = 64;
+constexpr uint32_t APG_SPILL_ALLOC = 56;  // spill entries allocated per thread (>= 64 - 8)
 
 struct TraceArgs
 {
@@ -52,15 +53,16 @@ __global__ void reset_counters_kernel(uint32_t* counters)
     if (threadIdx.x < 2) counters[threadIdx.x] = 0u;
 }
 
-template <bool ANY, bool COUNT>
+template <bool ANY, bool COUNT, uint32_t APG_STACK_LDS>
 __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A)
 {
+    constexpr uint32_t APG_STACK_SPILL = APG_STACK_MAX - APG_STACK_LDS;
     extern __shared__ uint32_t lds[];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = threadIdx.x >> 6;
     uint32_t* st_base = lds + wave * (APG_STACK_LDS * 64u * 2u);
     uint32_t* st_bits = st_base + APG_STACK_LDS * 64u;
-    uint64_t* spill = A.spill + size_t(blockIdx.x * blockDim.x + threadIdx.x) * APG_STACK_SPILL;
+    uint64_t* spill = A.spill + size_t(blockIdx.x * blockDim.x + threadIdx.x) * APG_SPILL_ALLOC;
 
     const uint32_t nrays = A.n_dev ? *A.n_dev : A.n;
     bool active = false;
@@ -239,7 +241,7 @@ static int ensure_trace_buffers(Context* ctx, uint32_t threads)
             (void)hipFree(ctx->spill);
         }
         ctx->spill = nullptr;
-        APG_HIP_CHECK(ctx, hipMalloc(&ctx->spill, size_t(threads) * APG_STACK_SPILL * sizeof(uint64_t)));
+        APG_HIP_CHECK(ctx, hipMalloc(&ctx->spill, size_t(threads) * APG_SPILL_ALLOC * sizeof(uint64_t)));
         ctx->spill_threads = threads;
     }
     return APG_OK;
@@ -277,17 +279,22 @@ int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint3
     A.fetch_threshold = cfg.fetch_threshold;
 
     reset_counters_kernel<<<1, 64, 0, stream>>>(ctx->counters);
-    const size_t shmem = size_t(block / 64) * APG_STACK_LDS * 64 * 2 * sizeof(uint32_t);
-    if (any)
-    {
-        if (counts) trace_kernel<true, true><<<grid, block, shmem, stream>>>(A);
-        else trace_kernel<true, false><<<grid, block, shmem, stream>>>(A);
+    const uint32_t stack = cfg.stack_lds == 16 ? 16u : 8u;
+    const size_t shmem = size_t(block / 64) * stack * 64 * 2 * sizeof(uint32_t);
+#define APG_LAUNCH(S)                                                                   \
+    if (any)                                                                            \
+    {                                                                                   \
+        if (counts) trace_kernel<true, true, S><<<grid, block, shmem, stream>>>(A);     \
+        else trace_kernel<true, false, S><<<grid, block, shmem, stream>>>(A);           \
+    }                                                                                   \
+    else                                                                                \
+    {                                                                                   \
+        if (counts) trace_kernel<false, true, S><<<grid, block, shmem, stream>>>(A);    \
+        else trace_kernel<false, false, S><<<grid, block, shmem, stream>>>(A);          \
     }
-    else
-    {
-        if (counts) trace_kernel<false, true><<<grid, block, shmem, stream>>>(A);
-        else trace_kernel<false, false><<<grid, block, shmem, stream>>>(A);
-    }
+    if (stack == 16) { APG_LAUNCH(16) }
+    else { APG_LAUNCH(8) }
+#undef APG_LAUNCH
     APG_HIP_CHECK(ctx, hipGetLastError());
     return APG_OK;
 }

@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=262_144)
     ap.add_argument("--builder", choices=["gpu", "host"], default="gpu")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing configs")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     return ap.parse_args()
 
 
@@ -233,6 +234,12 @@ def main():
     occ = torch.empty(N_RAYS, dtype=torch.uint8, device="cuda")
     torch.cuda.synchronize()
 
+    tuned = None
+    if not args.no_autotune:
+        from alpine_amd import autotune
+        n_t = 1 << 20
+        tuned, _ = autotune.autotune(acc, o_d[:, :n_t].contiguous(), d_d[:, :n_t].contiguous(),
+                                     quick=True, log=log)
     log("[bench] timing incoherent closest-hit")
     ms, wall_ms = timed_passes(lambda: acc.intersect(o_d, d_d, out=out), args.steps, args.warmup)
     log(f"[bench] incoherent closest {ms:.3f} ms/step")
@@ -268,6 +275,7 @@ def main():
                         "random triangles = 1,510,720 fp32 triangles, 8-wide quantised BVH; "
                         "4,194,304 incoherent rays, closest-hit, tmax = FLT_MAX",
             "triangles": int(st["triangles"]), "rays": N_RAYS, "builder": args.builder,
+            "launch": tuned or "default",
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -87,7 +87,8 @@ typedef struct apg_trace_config {
     uint32_t block_size;       /* threads per workgroup (multiple of 64) */
     uint32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu */
     uint32_t fetch_threshold;  /* idle lanes a wave waits for before fetching rays */
-    uint32_t reserved[5];
+    uint32_t stack_lds;        /* traversal stack entries per lane kept in LDS: 8 or 16 */
+    uint32_t reserved[4];
 } apg_trace_config;
 
 /* ---- lifetime (Alpine::resetScene creates the accelerator, alpine.cpp:158-176) ---- */
@@ -105,6 +106,13 @@ int apg_mesh_append(apg_context* ctx, const float* xyz, uint32_t nv, const uint3
 int apg_mesh_vertices(apg_context* ctx, uint32_t shape_id, float** xyz, uint32_t* nv);
 int apg_mesh_update(apg_context* ctx, uint32_t shape_id);           /* updateShape */
 int apg_mesh_prim_offset(const apg_context* ctx, uint32_t shape_id, uint32_t* offset);
+/* Rigid transform of vertices [first, first + count) of a shape, applied on the GPU to the
+ * library's device copy (no upload); m is 3 x 4 row-major (host).  The next
+ * apg_scene_build rebuilds from the device copy; apg_mesh_vertices downloads it first.
+ * No reference counterpart: it replaces writing through getVertexBuffer + updateShape
+ * (mesh.cpp:29-55) for per-frame animation without a host round trip. */
+int apg_mesh_transform(apg_context* ctx, uint32_t shape_id, const float* m, uint32_t first,
+                       uint32_t count, void* stream);
 /* Appending a sphere is rejected like the reference Bvh (bvh.cpp:108-112). */
 int apg_sphere_append(apg_context* ctx, const float* xyzr, uint32_t n);
 int apg_scene_build(apg_context* ctx, const apg_build_params* params); /* updateScene */
