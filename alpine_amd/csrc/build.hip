@@ -732,14 +732,6 @@ __device__ uint32_t quantise_axis(float lo_f, float hi_f, float slack, const flo
     }
 }
 
-__device__ __forceinline__ void put_plane(uint32_t q[9], uint32_t slot, uint32_t plane, uint32_t v)
-{
-    const uint32_t bit = 36u * slot + 6u * plane;
-    const uint32_t w = bit >> 5, s = bit & 31u;
-    q[w] |= (v & 63u) << s;
-    if (s > 26u) q[w + 1] |= (v & 63u) >> (32u - s);
-}
-
 __global__ void k_wide_emit(WideState W, uint32_t wbegin, uint32_t wend, uint32_t child_begin,
                             uint32_t tri_begin)
 {
@@ -749,7 +741,7 @@ __global__ void k_wide_emit(WideState W, uint32_t wbegin, uint32_t wend, uint32_
     ApgWideNode nd;
     const uint32_t child_base = child_begin + W.s_inner[lw];
     const uint32_t tri_base = tri_begin + W.s_tris[lw];
-    uint32_t imask = 0, valid = 0, counts = 0, inner_rank = 0, toff = 0;
+    uint32_t imask = 0, valid = 0, cmask = 0, inner_rank = 0, toff = 0;
     float clo[3][APG_WIDE], chi[3][APG_WIDE];
     float nlo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, nhi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
     for (uint32_t s = 0; s < APG_WIDE; ++s)
@@ -770,7 +762,7 @@ __global__ void k_wide_emit(WideState W, uint32_t wbegin, uint32_t wend, uint32_
         if (is_leaf_node(W, c))
         {
             const uint32_t cnt = W.ncount[c], first = W.nfirst[c];
-            counts |= (cnt - 1u) << (2u * s);
+            cmask |= apg_leaf_nibble(s, cnt);
             for (uint32_t k = 0; k < cnt; ++k)
             {
                 const uint32_t g = W.idx[first + k];
@@ -794,13 +786,13 @@ __global__ void k_wide_emit(WideState W, uint32_t wbegin, uint32_t wend, uint32_
     nd.exp_imask = eb[0] | (eb[1] << 8) | (eb[2] << 16) | (imask << 24);
     nd.child_base = child_base;
     nd.tri_base = tri_base;
-    nd.meta = counts | (valid << 16);
+    nd.cmask = cmask;
     for (int j = 0; j < 9; ++j) nd.q[j] = 0;
     for (uint32_t s = 0; s < APG_WIDE; ++s)
         for (uint32_t a = 0; a < 3; ++a)
         {
-            put_plane(nd.q, s, a, qlo[a][s]);
-            put_plane(nd.q, s, 3 + a, qhi[a][s]);
+            apg_put_plane(nd.q, s, a, qlo[a][s]);
+            apg_put_plane(nd.q, s, 3 + a, qhi[a][s]);
         }
     W.nodes[w] = nd;
 }

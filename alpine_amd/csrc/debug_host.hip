@@ -81,7 +81,7 @@ int apg_debug_host_trace(const apg_debug_scene* s, const float* org3, const floa
         float tmax = tmaxp ? tmaxp[i] : FLT_MAX;
         RayBox rb = make_raybox(ox, oy, oz, dx, dy, dz);
         uint32_t g_base = 0, g_bits = s->nodes.empty() ? 0u : root_group_bits(rb.oct);
-        uint32_t t_base = 0, t_mask = 0, best = 0xFFFFFFFFu;
+        uint32_t t_base = 0, t_mask = 0, t_cmask = 0, best = 0xFFFFFFFFu;
         float hb0 = 0, hb1 = 0;
         bool found = false;
         stack.clear();
@@ -95,14 +95,14 @@ int apg_debug_host_trace(const apg_debug_scene* s, const float* org3, const floa
                 if (stack.size() > deepest) deepest = uint32_t(stack.size());
                 uint32_t w[16];
                 std::memcpy(w, &s->nodes[node], 64);
-                node_visit(w, rb, tmax, g_base, g_bits, t_base, t_mask);
+                node_visit(w, rb, tmax, g_base, g_bits, t_base, t_mask, t_cmask);
             }
             while (t_mask)
             {
-                uint32_t k = bit_ffs(t_mask) - 1u;
+                const uint32_t tri = tri_index(t_base, t_cmask, bit_ffs(t_mask) - 1u);
                 t_mask &= t_mask - 1u;
                 float t, b0, b1;
-                if (!tri_hit(&s->tri9[size_t(t_base + k) * 9], ox, oy, oz, dx, dy, dz, tmax, t, b0, b1))
+                if (!tri_hit(&s->tri9[size_t(tri) * 9], ox, oy, oz, dx, dy, dz, tmax, t, b0, b1))
                     continue;
                 if (any)
                 {
@@ -113,7 +113,7 @@ int apg_debug_host_trace(const apg_debug_scene* s, const float* org3, const floa
                     break;
                 }
                 tmax = t;
-                best = t_base + k;
+                best = tri;
                 hb0 = b0;
                 hb1 = b1;
             }

@@ -420,7 +420,7 @@ void collapse_to_wide(const BinaryTree& bt, const std::vector<TriPrep>& tris,
 
         ApgWideNode nd;
         std::memset(&nd, 0, sizeof(nd));
-        uint32_t imask = 0, valid = 0, counts = 0;
+        uint32_t imask = 0, cmask = 0;
         nd.child_base = uint32_t(nodes.size());
         nd.tri_base = uint32_t(tri_ids.size());
         float clo[3][APG_WIDE], chi[3][APG_WIDE];
@@ -432,7 +432,6 @@ void collapse_to_wide(const BinaryTree& bt, const std::vector<TriPrep>& tris,
         {
             if (slot_child[s] < 0) continue;
             uint32_t c = ch[slot_child[s]];
-            valid |= 1u << s;
             vmask[s] = true;
             for (int a = 0; a < 3; ++a)
             {
@@ -444,7 +443,7 @@ void collapse_to_wide(const BinaryTree& bt, const std::vector<TriPrep>& tris,
             if (is_leaf(c))
             {
                 const BinNode& lf = bn[c];
-                counts |= (lf.count - 1) << (2 * s);
+                cmask |= apg_leaf_nibble(s, lf.count);
                 for (uint32_t k = 0; k < lf.count; ++k)
                 {
                     uint32_t g = bt.order[lf.first + k];
@@ -467,7 +466,7 @@ void collapse_to_wide(const BinaryTree& bt, const std::vector<TriPrep>& tris,
             eb[a] = quantise_axis(nlo[a], nhi[a], slack, clo[a], chi[a], APG_WIDE, vmask,
                                   nd.origin[a], qlo[a], qhi[a]);
         nd.exp_imask = eb[0] | (eb[1] << 8) | (eb[2] << 16) | (imask << 24);
-        nd.meta = counts | (valid << 16);
+        nd.cmask = cmask;
         for (uint32_t s = 0; s < APG_WIDE; ++s)
             for (int a = 0; a < 3; ++a)
             {

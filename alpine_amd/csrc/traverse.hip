@@ -31,37 +31,69 @@ namespace apg {
 constexpr uint32_t APG_STACK_MAX = 64;
 constexpr uint32_t APG_SPILL_ALLOC = 56;  // spill entries allocated per thread (>= 64 - 8)
 
+// LDS words per wave: the stack's top (2 words per entry and lane), 64 keys of
+// 2 words and 64 barycentric pairs.  With an 8-entry top this is 5 KiB, so 32
+// waves fit in a CU's 160 KiB.
+__host__ __device__ constexpr uint32_t trace_wave_lds_words(uint32_t stack)
+{
+    return stack * 128u + 128u + 128u;
+}
+
 struct TraceArgs
 {
     const ApgWideNode* nodes;
     const float* tris;
     const uint32_t* tri_ids;
+    uint32_t tri_count;
     apg_rays rays;
     uint32_t n;
     const uint32_t* n_dev;            // when set, the ray count is read here (queued work)
     uint32_t has_scene;
     apg_hits hits;
     uint8_t* occluded;
-    uint32_t* counters;               // [0] ray cursor, [1] stack overflow flag
+    uint32_t* counters;               // [0] ray cursor, [1] stack overflow flag, [2] waves done
     uint64_t* spill;
     unsigned long long* fetch_counts; // [0] nodes, [1] triangles
     uint32_t fetch_threshold;
 };
 
-__global__ void reset_counters_kernel(uint32_t* counters)
+// Inclusive prefix sum over the 64 lanes of a wave: shift-and-add inside each
+// row of 16 lanes, then carry the row totals with the row broadcasts.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x)
 {
-    if (threadIdx.x < 2) counters[threadIdx.x] = 0u;
+    x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x111, 0xf, 0xf, false));  // row_shr:1
+    x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x112, 0xf, 0xf, false));  // row_shr:2
+    x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x114, 0xf, 0xf, false));  // row_shr:4
+    x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x118, 0xf, 0xf, false));  // row_shr:8
+    x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x142, 0xa, 0xf, false));  // row_bcast:15
+    x += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return x;
+}
+
+__device__ __forceinline__ uint32_t lane_u32(uint32_t v, uint32_t src)
+{
+    return uint32_t(__builtin_amdgcn_ds_bpermute(int(src << 2), int(v)));
+}
+
+__device__ __forceinline__ float lane_f32(float v, uint32_t src)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(int(src << 2), __float_as_int(v)));
 }
 
 template <bool ANY, bool COUNT, uint32_t APG_STACK_LDS>
-__global__ __launch_bounds__(256) void trace_kernel(TraceArgs A)
+__global__ __launch_bounds__(256, 8) void trace_kernel(TraceArgs A)
 {
     constexpr uint32_t APG_STACK_SPILL = APG_STACK_MAX - APG_STACK_LDS;
     extern __shared__ uint32_t lds[];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = threadIdx.x >> 6;
-    uint32_t* st_base = lds + wave * (APG_STACK_LDS * 64u * 2u);
+    uint32_t* wl = lds + wave * trace_wave_lds_words(APG_STACK_LDS);
+    uint32_t* st_base = wl;
     uint32_t* st_bits = st_base + APG_STACK_LDS * 64u;
+    // per owner lane: best key of the current triangle group (|t| bits, then
+    // count-mask bit, then the sign of t) and the winning hit's b0, b1
+    unsigned long long* keys = reinterpret_cast<unsigned long long*>(wl + APG_STACK_LDS * 128u);
+    float* rec = reinterpret_cast<float*>(wl + APG_STACK_LDS * 128u + 128u);
     uint64_t* spill = A.spill + size_t(blockIdx.x * blockDim.x + threadIdx.x) * APG_SPILL_ALLOC;
 
     const uint32_t nrays = A.n_dev ? *A.n_dev : A.n;
@@ -74,7 +106,6 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A)
     uint32_t best = 0xFFFFFFFFu;
     float hb0 = 0.0f, hb1 = 0.0f;
     uint32_t g_base = 0, g_bits = 0;
-    uint32_t t_base = 0, t_mask = 0;
     uint32_t sp = 0;
     bool found = false;
     uint32_t n_node = 0, n_tri = 0;
@@ -106,7 +137,6 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A)
                     hb0 = hb1 = 0.0f;
                     found = false;
                     sp = 0;
-                    t_mask = 0;
                     g_base = 0;
                     // Root as a group holding one inner child in slot 0.
                     g_bits = A.has_scene ? root_group_bits(rb.oct) : 0u;
@@ -118,10 +148,10 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A)
             if (exhausted) break;
             continue;
         }
-        if (!active) continue;
 
-        // ---- one inner node ----
-        if ((g_bits >> 8) != 0u)
+        // ---- one inner node per lane ----
+        uint32_t t_base = 0, t_mask = 0, t_cmask = 0;
+        if (active && (g_bits >> 8) != 0u)
         {
             uint32_t hits = 0;
             const uint32_t node_idx = group_next_node(g_base, g_bits, rb.oct, hits);
@@ -155,35 +185,84 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A)
             const uint32_t w[16] = { w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w,
                                      w2.x, w2.y, w2.z, w2.w, w3.x, w3.y, w3.z, w3.w };
             if (COUNT) n_node++;
-            node_visit(w, rb, tmax, g_base, g_bits, t_base, t_mask);
+            node_visit(w, rb, tmax, g_base, g_bits, t_base, t_mask, t_cmask);
         }
 
-        // ---- triangles of the current group ----
-        while (t_mask)
+        // ---- the wave's triangle tests, spread over all 64 lanes ----
+        // Lane L owns popcount(t_mask) tests; work item j (lane j % 64 of
+        // round j / 64) belongs to the owner whose inclusive count first
+        // exceeds j.  Each owner keeps the smallest (t, slot bit) key, which
+        // is the hit a sequential pass over its group in bit order accepts.
+        const uint32_t cnt = bit_popc(t_mask);
+        const uint32_t incl = wave_incl_scan(cnt);
+        const uint32_t total = uint32_t(__builtin_amdgcn_readlane(int(incl), 63));
+        if (total != 0u)
         {
-            const uint32_t k = __ffs(t_mask) - 1u;
-            t_mask &= t_mask - 1u;
-            const uint32_t tri = t_base + k;
-            const float* tp = A.tris + size_t(tri) * APG_TRI_FLOATS;
-            if (COUNT) n_tri++;
-            float t, b0, b1;
-            if (!tri_hit(tp, ox, oy, oz, dx, dy, dz, tmax, t, b0, b1)) continue;
-            if (ANY)
+            if (cnt) keys[lane] = ~0ull;
+            for (uint32_t j0 = 0; j0 < total; j0 += 64u)
             {
-                found = true;
-                t_mask = 0;
-                g_bits = 0;
-                sp = 0;
-                break;
+                const uint32_t j = j0 + lane;
+                uint32_t o = 0;
+#pragma unroll
+                for (uint32_t step = 32u; step; step >>= 1)
+                    if (lane_u32(incl, o + step - 1u) <= j) o += step;
+                o = o > 63u ? 63u : o;
+                const uint32_t otm = lane_u32(t_mask, o);
+                const uint32_t oexcl = lane_u32(incl, o) - bit_popc(otm);
+                const uint32_t obase = lane_u32(t_base, o), ocm = lane_u32(t_cmask, o);
+                const float rox = lane_f32(ox, o), roy = lane_f32(oy, o), roz = lane_f32(oz, o);
+                const float rdx = lane_f32(dx, o), rdy = lane_f32(dy, o), rdz = lane_f32(dz, o);
+                const float rtmax = lane_f32(tmax, o);
+                bool hit = false;
+                float t = 0.0f, b0 = 0.0f, b1 = 0.0f;
+                unsigned long long key = 0;
+                if (j < total)
+                {
+                    const uint32_t p = select_bit(otm, j - oexcl);
+                    uint32_t tri = tri_index(obase, ocm, p);
+                    tri = tri < A.tri_count ? tri : A.tri_count - 1u;
+                    if (COUNT) n_tri++;
+                    hit = tri_hit(A.tris + size_t(tri) * APG_TRI_FLOATS, rox, roy, roz, rdx, rdy, rdz,
+                                  rtmax, t, b0, b1);
+                    if (hit)
+                    {
+                        const uint32_t tb = __float_as_uint(t);
+                        key = (uint64_t(tb & 0x7fffffffu) << 32) | (p << 1) | (tb >> 31);
+                        atomicMin(&keys[o], key);
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                if (hit && keys[o] == key)
+                {
+                    rec[2u * o] = b0;
+                    rec[2u * o + 1u] = b1;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
-            tmax = t;
-            best = tri;
-            hb0 = b0;
-            hb1 = b1;
+            if (cnt)
+            {
+                const unsigned long long k = keys[lane];
+                if (k != ~0ull)
+                {
+                    if (ANY)
+                    {
+                        found = true;
+                        g_bits = 0;
+                        sp = 0;
+                    }
+                    else
+                    {
+                        tmax = __uint_as_float(uint32_t(k >> 32) | (uint32_t(k) << 31));
+                        hb0 = rec[2u * lane];
+                        hb1 = rec[2u * lane + 1u];
+                        best = tri_index(t_base, t_cmask, (uint32_t(k) >> 1) & 31u);
+                    }
+                }
+            }
         }
 
         // ---- next group ----
-        if ((g_bits >> 8) == 0u)
+        if (active && (g_bits >> 8) == 0u)
         {
             if (sp == 0u)
             {
@@ -218,6 +297,17 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A)
     {
         atomicAdd(&A.fetch_counts[0], (unsigned long long)n_node);
         atomicAdd(&A.fetch_counts[1], (unsigned long long)n_tri);
+    }
+    // The last wave to finish rewinds the ray cursor for the next launch.
+    if (lane == 0u)
+    {
+        const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+        __threadfence();
+        if (atomicAdd(&A.counters[2], 1u) == waves - 1u)
+        {
+            A.counters[0] = 0u;
+            A.counters[2] = 0u;
+        }
     }
 }
 
@@ -267,6 +357,7 @@ int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint3
     A.nodes = ctx->scene.nodes;
     A.tris = ctx->scene.tris;
     A.tri_ids = ctx->scene.tri_ids;
+    A.tri_count = ctx->scene.tri_count;
     A.rays = rays;
     A.n = n;
     A.n_dev = n_dev;
@@ -278,9 +369,8 @@ int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint3
     A.fetch_counts = counts;
     A.fetch_threshold = cfg.fetch_threshold;
 
-    reset_counters_kernel<<<1, 64, 0, stream>>>(ctx->counters);
     const uint32_t stack = cfg.stack_lds == 16 ? 16u : 8u;
-    const size_t shmem = size_t(block / 64) * stack * 64 * 2 * sizeof(uint32_t);
+    const size_t shmem = size_t(block / 64) * trace_wave_lds_words(stack) * sizeof(uint32_t);
 #define APG_LAUNCH(S)                                                                   \
     if (any)                                                                            \
     {                                                                                   \

@@ -53,22 +53,47 @@ APG_HD float fma_f(float a, float b, float c)
 #endif
 }
 
-APG_HD uint32_t plane6(const uint32_t (&q)[9], uint32_t slot, uint32_t j)
+// Select the r-th (0-based) set bit of m; r < popcount(m).
+APG_HD uint32_t select_bit(uint32_t m, uint32_t r)
 {
-    const uint32_t bit = 36u * slot + 6u * j;
-    const uint32_t w = bit >> 5, s = bit & 31u;
-    uint32_t v;
-    if (s > 26u)
+    uint32_t pos = 0, c;
+    c = bit_popc(m & 0xffffu); if (r >= c) { r -= c; pos += 16u; m >>= 16; }
+    c = bit_popc(m & 0xffu);   if (r >= c) { r -= c; pos += 8u;  m >>= 8; }
+    c = bit_popc(m & 0xfu);    if (r >= c) { r -= c; pos += 4u;  m >>= 4; }
+    c = bit_popc(m & 0x3u);    if (r >= c) { r -= c; pos += 2u;  m >>= 2; }
+    if (r >= (m & 1u)) pos += 1u;
+    return pos;
+}
+
+// Index of the triangle at count-mask bit p of a node (wide_node.h, dw6).
+APG_HD uint32_t tri_index(uint32_t tri_base, uint32_t cmask, uint32_t p)
+{
+    return tri_base + bit_popc(cmask & ((1u << p) - 1u));
+}
+
+// Six bits of a 48-bit plane chunk held as a low word and a high half word.
+template <uint32_t S>
+APG_HD uint32_t chunk_field(uint32_t lo, uint32_t hi)
+{
+    if (S < 5u) return (lo >> (6u * S)) & 63u;
+    if (S == 5u)
     {
 #if defined(__HIP_DEVICE_COMPILE__)
-        v = __builtin_amdgcn_alignbit(q[w + 1], q[w], s);
+        return __builtin_amdgcn_alignbit(hi, lo, 30) & 63u;
 #else
-        v = uint32_t(((uint64_t(q[w + 1]) << 32) | q[w]) >> s);
+        return ((lo >> 30) | (hi << 2)) & 63u;
 #endif
     }
-    else
-        v = q[w] >> s;
-    return v & 63u;
+    return (hi >> (6u * S - 32u)) & 63u;
+}
+
+// Spread the 8 bits of x to bit 0 of the 8 nibbles, then fill each nibble.
+APG_HD uint32_t spread_nibbles(uint32_t x)
+{
+    x = (x | (x << 12)) & 0x000f000fu;
+    x = (x | (x << 6)) & 0x03030303u;
+    x = (x | (x << 3)) & 0x11111111u;
+    return x * 15u;
 }
 
 // Map an 8-bit slot mask to traversal order: slot s -> bit s ^ oct.
@@ -117,71 +142,76 @@ APG_HD uint32_t group_next_node(uint32_t g_base, uint32_t g_bits, uint32_t oct, 
     return g_base + bit_popc(gim & ((1u << slot) - 1u));
 }
 
-// Test the 8 children of one node (16 dwords w[0..15]) against the ray with
-// current closest distance tmax.  Produces the new node group and triangle
-// group.
-APG_HD void node_visit(const uint32_t* w, const RayBox& r, float tmax, uint32_t& g_base,
-                       uint32_t& g_bits, uint32_t& t_base, uint32_t& t_mask)
+// Slab test of one slot against near/far plane chunks of the three axes.
+template <uint32_t S>
+APG_HD bool slot_hit(const uint32_t (&nw)[3], const uint32_t (&nh)[3], const uint32_t (&fw)[3],
+                     const uint32_t (&fh)[3], const float (&a)[3], const float (&b)[3], float tmax)
 {
-    float orgx, orgy, orgz;
+    const float tnx = fma_f(float(chunk_field<S>(nw[0], nh[0])), a[0], b[0]);
+    const float tfx = fma_f(float(chunk_field<S>(fw[0], fh[0])), a[0], b[0]);
+    const float tny = fma_f(float(chunk_field<S>(nw[1], nh[1])), a[1], b[1]);
+    const float tfy = fma_f(float(chunk_field<S>(fw[1], fh[1])), a[1], b[1]);
+    const float tnz = fma_f(float(chunk_field<S>(nw[2], nh[2])), a[2], b[2]);
+    const float tfz = fma_f(float(chunk_field<S>(fw[2], fh[2])), a[2], b[2]);
+    const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+    const float tf = fminf(fminf(tfx, tfy), tfz);
+    return tn <= tf && tn < tmax;
+}
+
+// Test the 8 children of one node (16 dwords w[0..15]) against the ray with
+// current closest distance tmax.  Produces the new node group and the
+// triangle group: t_base, the hit triangles as count-mask bits (t_mask) and
+// the node's count mask (t_cmask).
+APG_HD void node_visit(const uint32_t* w, const RayBox& r, float tmax, uint32_t& g_base,
+                       uint32_t& g_bits, uint32_t& t_base, uint32_t& t_mask, uint32_t& t_cmask)
+{
+    float org[3];
 #if defined(__HIP_DEVICE_COMPILE__)
-    orgx = __uint_as_float(w[0]); orgy = __uint_as_float(w[1]); orgz = __uint_as_float(w[2]);
+    org[0] = __uint_as_float(w[0]); org[1] = __uint_as_float(w[1]); org[2] = __uint_as_float(w[2]);
 #else
-    __builtin_memcpy(&orgx, &w[0], 4); __builtin_memcpy(&orgy, &w[1], 4); __builtin_memcpy(&orgz, &w[2], 4);
+    __builtin_memcpy(org, w, 12);
 #endif
     const uint32_t ei = w[3];
-    float stx, sty, stz;
+    const float o[3] = { r.ox, r.oy, r.oz };
+    const float inv[3] = { r.ix, r.iy, r.iz };
+    float a[3], b[3];
+    uint32_t nw[3], nh[3], fw[3], fh[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
     {
-        uint32_t bx_ = (ei & 0xffu) << 23, by_ = ((ei >> 8) & 0xffu) << 23, bz_ = ((ei >> 16) & 0xffu) << 23;
+        const uint32_t sb = ((ei >> (8 * k)) & 0xffu) << 23;
+        float st;
 #if defined(__HIP_DEVICE_COMPILE__)
-        stx = __uint_as_float(bx_); sty = __uint_as_float(by_); stz = __uint_as_float(bz_);
+        st = __uint_as_float(sb);
 #else
-        __builtin_memcpy(&stx, &bx_, 4); __builtin_memcpy(&sty, &by_, 4); __builtin_memcpy(&stz, &bz_, 4);
+        __builtin_memcpy(&st, &sb, 4);
 #endif
+        a[k] = st * inv[k];
+        b[k] = (org[k] - o[k]) * inv[k];
+        // near chunk = lo planes on a positive axis, hi planes on a negative one
+        const uint32_t sh = ((r.oct >> k) & 1u) << 4;
+        const uint32_t lo = w[7 + 2 * k], hi = w[8 + 2 * k], half = w[13 + k];
+        nw[k] = sh ? hi : lo;
+        fw[k] = sh ? lo : hi;
+        nh[k] = half >> sh;
+        fh[k] = half >> (16u - sh);
     }
-    const uint32_t nim = ei >> 24;
-    const uint32_t child_base = w[4], tri_base = w[5], meta = w[6];
-    const uint32_t q[9] = { w[7], w[8], w[9], w[10], w[11], w[12], w[13], w[14], w[15] };
-
-    const float ax = stx * r.ix, ay = sty * r.iy, az = stz * r.iz;
-    const float bx = (orgx - r.ox) * r.ix, by = (orgy - r.oy) * r.iy, bz = (orgz - r.oz) * r.iz;
-    const bool px = r.ix >= 0.0f, py = r.iy >= 0.0f, pz = r.iz >= 0.0f;
-
     uint32_t hitmask = 0;
-#pragma unroll
-    for (uint32_t s = 0; s < APG_WIDE; ++s)
-    {
-        const uint32_t lx = plane6(q, s, 0), ly = plane6(q, s, 1), lz = plane6(q, s, 2);
-        const uint32_t hx = plane6(q, s, 3), hy = plane6(q, s, 4), hz = plane6(q, s, 5);
-        const float tnx = fma_f(float(px ? lx : hx), ax, bx);
-        const float tfx = fma_f(float(px ? hx : lx), ax, bx);
-        const float tny = fma_f(float(py ? ly : hy), ay, by);
-        const float tfy = fma_f(float(py ? hy : ly), ay, by);
-        const float tnz = fma_f(float(pz ? lz : hz), az, bz);
-        const float tfz = fma_f(float(pz ? hz : lz), az, bz);
-        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
-        const float tf = fminf(fminf(tfx, tfy), tfz);
-        if (tn <= tf && tn < tmax) hitmask |= 1u << s;
-    }
+    hitmask |= slot_hit<0>(nw, nh, fw, fh, a, b, tmax) ? 0x01u : 0u;
+    hitmask |= slot_hit<1>(nw, nh, fw, fh, a, b, tmax) ? 0x02u : 0u;
+    hitmask |= slot_hit<2>(nw, nh, fw, fh, a, b, tmax) ? 0x04u : 0u;
+    hitmask |= slot_hit<3>(nw, nh, fw, fh, a, b, tmax) ? 0x08u : 0u;
+    hitmask |= slot_hit<4>(nw, nh, fw, fh, a, b, tmax) ? 0x10u : 0u;
+    hitmask |= slot_hit<5>(nw, nh, fw, fh, a, b, tmax) ? 0x20u : 0u;
+    hitmask |= slot_hit<6>(nw, nh, fw, fh, a, b, tmax) ? 0x40u : 0u;
+    hitmask |= slot_hit<7>(nw, nh, fw, fh, a, b, tmax) ? 0x80u : 0u;
 
-    const uint32_t inner = hitmask & nim;
-    g_base = child_base;
-    g_bits = nim | (octant_permute(inner, r.oct) << 8);
-
-    const uint32_t leafs = (meta >> 16) & ~nim & 0xffu;
-    uint32_t tm = 0, off = 0;
-#pragma unroll
-    for (uint32_t s = 0; s < APG_WIDE; ++s)
-    {
-        if ((leafs >> s) & 1u)
-        {
-            const uint32_t cnt = ((meta >> (2u * s)) & 3u) + 1u;
-            if ((hitmask >> s) & 1u) tm |= ((1u << cnt) - 1u) << off;
-            off += cnt;
-        }
-    }
-    t_base = tri_base;
-    t_mask = tm;
+    const uint32_t nim = ei >> 24;
+    g_base = w[4];
+    g_bits = nim | (octant_permute(hitmask & nim, r.oct) << 8);
+    t_base = w[5];
+    t_cmask = w[6];
+    t_mask = spread_nibbles(hitmask) & t_cmask;
 }
 
 // Moller-Trumbore exactly as the reference (bvh_common.cpp:66-103), combined

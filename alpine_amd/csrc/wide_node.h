@@ -10,11 +10,16 @@
 //           are stored contiguously in slot order.
 //   dw5     tri_base: index of the first triangle of the first leaf slot; the
 //           leaf slots' triangles follow one another in slot order.
-//   dw6     bits 0..15: (count - 1) of each leaf slot, 2 bits per slot
-//           (leaves hold 1..4 triangles); bits 16..23: valid-slot mask.
-//   dw7..15 288-bit stream of 6-bit quantised planes: slot s, plane j
-//           (0..5 = lo x, lo y, lo z, hi x, hi y, hi z) at bit 36 * s + 6 * j.
-//           Empty slots carry lo = 63, hi = 0, which no ray can hit.
+//   dw6     cmask: triangle count mask.  Nibble s holds (1 << count) - 1 for
+//           a leaf slot s (leaves hold 1..4 triangles) and 0 otherwise, so
+//           bit 4 s + i stands for triangle i of slot s and the triangle's
+//           index is tri_base + popcount(cmask & ((1 << (4 s + i)) - 1)).
+//   dw7..15 6-bit quantised planes.  For axis a and side (0 = lo, 1 = hi)
+//           the eight slots' planes form a 48-bit chunk (slot s at bit 6 s);
+//           its low 32 bits are q[2 a + side] and its high 16 bits are
+//           bits 16 side .. 16 side + 15 of q[6 + a].  A ray picks its near
+//           and far chunk of an axis with two word selects and two shifts.
+//           Empty slots carry lo = 63, hi = 0 and a zero cmask nibble.
 //
 // Quantised boxes are conservative: lo = origin + qlo * 2^e <= child min - d
 // and hi = origin + qhi * 2^e >= child max + d for a scene-wide slack
@@ -31,31 +36,52 @@
 #define APG_NODE_BYTES 64
 #define APG_TRI_FLOATS 9
 
+#if defined(__HIPCC__)
+#define APG_NODE_FN __host__ __device__ static inline
+#else
+#define APG_NODE_FN static inline
+#endif
+
 struct alignas(64) ApgWideNode
 {
     float origin[3];
     uint32_t exp_imask;
     uint32_t child_base;
     uint32_t tri_base;
-    uint32_t meta;
+    uint32_t cmask;
     uint32_t q[9];
 };
 static_assert(sizeof(ApgWideNode) == 64, "wide node must be 64 bytes");
 
-// Host-side packing helper for the 288-bit plane stream.
-static inline void apg_put_plane(uint32_t q[9], uint32_t slot, uint32_t plane, uint32_t v)
+// Plane j of slot s: j = 0..2 lo x/y/z, 3..5 hi x/y/z.
+APG_NODE_FN void apg_put_plane(uint32_t q[9], uint32_t slot, uint32_t plane, uint32_t v)
 {
-    uint32_t bit = 36u * slot + 6u * plane;
-    uint32_t w = bit >> 5, s = bit & 31u;
-    q[w] |= (v & 63u) << s;
-    if (s > 26u) q[w + 1] |= (v & 63u) >> (32u - s);
+    const uint32_t a = plane % 3u, side = plane / 3u;
+    const uint32_t bit = 6u * slot;
+    v &= 63u;
+    if (bit + 6u <= 32u)
+        q[2u * a + side] |= v << bit;
+    else if (bit >= 32u)
+        q[6u + a] |= v << (bit - 32u + 16u * side);
+    else
+    {
+        q[2u * a + side] |= (v & 3u) << 30;
+        q[6u + a] |= (v >> 2) << (16u * side);
+    }
 }
 
-static inline uint32_t apg_get_plane(const uint32_t q[9], uint32_t slot, uint32_t plane)
+APG_NODE_FN uint32_t apg_get_plane(const uint32_t q[9], uint32_t slot, uint32_t plane)
 {
-    uint32_t bit = 36u * slot + 6u * plane;
-    uint32_t w = bit >> 5, s = bit & 31u;
-    uint32_t v = q[w] >> s;
-    if (s > 26u) v |= q[w + 1] << (32u - s);
-    return v & 63u;
+    const uint32_t a = plane % 3u, side = plane / 3u;
+    const uint32_t bit = 6u * slot;
+    const uint32_t lo = q[2u * a + side], hi = q[6u + a] >> (16u * side);
+    if (bit + 6u <= 32u) return (lo >> bit) & 63u;
+    if (bit >= 32u) return (hi >> (bit - 32u)) & 63u;
+    return ((lo >> 30) | (hi << 2)) & 63u;
+}
+
+// Count mask nibble of a leaf slot holding `count` (1..4) triangles.
+APG_NODE_FN uint32_t apg_leaf_nibble(uint32_t slot, uint32_t count)
+{
+    return ((1u << count) - 1u) << (4u * slot);
 }

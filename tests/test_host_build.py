@@ -6,6 +6,13 @@ import oracle
 from alpine_amd import DebugScene, scenes
 
 
+def _plane(q, s, j):
+    """Six-bit plane j (lo x/y/z, hi x/y/z) of slot s from the nine plane words."""
+    a, side = j % 3, j // 3
+    chunk = int(q[2 * a + side]) | (((int(q[6 + a]) >> (16 * side)) & 0xFFFF) << 32)
+    return (chunk >> (6 * s)) & 63
+
+
 def _decode(nodes):
     """Decode 64 B wide nodes (layout in alpine_amd/csrc/wide_node.h) into float boxes."""
     out = []
@@ -14,16 +21,21 @@ def _decode(nodes):
         e = [(int(w[3]) >> (8 * a)) & 0xFF for a in range(3)]
         step = np.array([2.0 ** (x - 127) for x in e])
         imask = int(w[3]) >> 24
-        valid = (int(w[6]) >> 16) & 0xFF
-        q = [int(x) for x in w[7:16]]
-        stream = sum(q[i] << (32 * i) for i in range(9))
+        cmask = int(w[6])
+        nib = [(cmask >> (4 * s)) & 0xF for s in range(8)]
+        assert all(n in (0, 1, 3, 7, 15) for n in nib), "count nibbles are low runs"
+        assert all(not ((imask >> s) & 1 and nib[s]) for s in range(8)), "inner slots hold no triangles"
+        valid = imask | sum(1 << s for s in range(8) if nib[s])
+        q = w[7:16]
         boxes = []
         for s in range(8):
-            p = [(stream >> (36 * s + 6 * j)) & 63 for j in range(6)]
+            p = [_plane(q, s, j) for j in range(6)]
+            if not (valid >> s) & 1:
+                assert p == [63, 63, 63, 0, 0, 0], "empty slots cannot be hit"
             lo = origin.astype(np.float64) + np.array(p[:3]) * step
             hi = origin.astype(np.float64) + np.array(p[3:]) * step
             boxes.append((lo, hi))
-        counts = [((int(w[6]) >> (2 * s)) & 3) + 1 for s in range(8)]
+        counts = [bin(n).count("1") for n in nib]
         out.append(dict(boxes=boxes, imask=imask, valid=valid, child=int(w[4]), tri=int(w[5]),
                         counts=counts))
     return out
