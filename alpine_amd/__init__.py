@@ -119,7 +119,8 @@ ABI_SYMBOLS = [
     "apg_occluded", "apg_intersect_counted", "apg_scene_triangles", "apg_scene_leaf_keys",
     "apg_scene_wide_nodes", "apg_materials_set", "apg_lights_set", "apg_light_sampler_sample",
     "apg_light_sampler_pdf", "apg_render_reset", "apg_render_set_camera", "apg_render",
-    "apg_render_read", "apg_render_counters", "apg_guide_param_count", "apg_guide_init",
+    "apg_render_read", "apg_render_postprocess", "apg_render_counters", "apg_guide_param_count",
+    "apg_guide_init",
     "apg_guide_params", "apg_guide_forward", "apg_guide_train_step", "apg_render_guide_stats",
 ]
 DEBUG_SYMBOLS = ["apg_debug_host_build", "apg_debug_scene_info", "apg_debug_scene_copy",
@@ -172,6 +173,7 @@ def lib():
         "apg_render_set_camera": (i32, [vp, C.POINTER(Camera)]),
         "apg_render": (i32, [vp, C.POINTER(RenderParams), vp]),
         "apg_render_read": (i32, [vp, vp, vp, C.POINTER(u32)]),
+        "apg_render_postprocess": (i32, [vp, i32, vp, vp]),
         "apg_render_counters": (i32, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                       C.POINTER(C.c_uint64)]),
         "apg_guide_param_count": (i32, []),
@@ -421,6 +423,14 @@ class GpuAccelerator:
         self._rc(self._lib.apg_render_read(self._h, _ptr(rgb), None if r64 is None else _ptr(r64),
                                            C.byref(spp)))
         return rgb, r64, spp.value
+
+    def postprocess(self, tonemap=False, stream=None):
+        """Display frame (height x width x 3 uint8 CUDA tensor): Alpine::postProcess."""
+        import torch
+        frame = torch.empty((self.height, self.width, 3), dtype=torch.uint8, device="cuda")
+        self._rc(self._lib.apg_render_postprocess(self._h, int(bool(tonemap)), frame.data_ptr(),
+                                                  self._stream(stream)))
+        return frame
 
     def render_counters(self):
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -11,15 +11,15 @@ from __future__ import annotations
 import itertools
 import time
 
-BLOCK_SIZES = (64, 128, 256)
-BLOCKS_PER_CU = (2, 4, 8, 16)
-FETCH_THRESHOLDS = (8, 16, 32)
+BLOCK_SIZES = (128, 256)
+BLOCKS_PER_CU = (2, 3, 4, 5, 6, 8)
+FETCH_THRESHOLDS = (16, 24, 32, 40, 48, 64)
 STACK_LDS = (8, 16)
 
 
 def candidates(quick: bool = False):
     if quick:
-        return [(256, b, f, s) for b, f, s in itertools.product((4, 8, 16), (8, 16, 32), (8, 16))]
+        return [(256, b, f, s) for b, f, s in itertools.product((4, 5, 6, 8), (24, 32, 40), (8, 16))]
     return list(itertools.product(BLOCK_SIZES, BLOCKS_PER_CU, FETCH_THRESHOLDS, STACK_LDS))
 
 

@@ -296,8 +296,8 @@ int apg_trace_config_default(const apg_context* ctx, apg_trace_config* cfg)
     (void)ctx;
     std::memset(cfg, 0, sizeof(*cfg));
     cfg->block_size = 256;
-    cfg->blocks_per_cu = 8;
-    cfg->fetch_threshold = 16;
+    cfg->blocks_per_cu = 6;
+    cfg->fetch_threshold = 32;
     cfg->stack_lds = 8;
     return APG_OK;
 }

@@ -554,6 +554,21 @@ __global__ void k_advance(uint32_t* counts, unsigned long long* stats)
     }
 }
 
+// Alpine::postProcess without denoising (alpine.cpp:446-483): mean colour, optional ACES
+// fit (:434-442), clamp, sRGB encode (:461-465), round to bytes.
+__global__ void k_postprocess(const float* accum, uint32_t n3, float spp, uint32_t tonemap,
+                              uint8_t* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    float c = accum[i] / spp;
+    if (tonemap)
+        c = (c * (c * 2.51f + 0.03f)) / (c * (c * 2.43f + 0.59f) + 0.14f);
+    c = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+    c = c <= 0.0031308f ? 12.92f * c : 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
+    out[i] = uint8_t(c * 255.0f + 0.5f);
+}
+
 __global__ void k_train_encode(const float* tin, uint32_t n, float* feat)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -999,6 +1014,22 @@ int apg_render_read(apg_context* ctx, float* rgb, double* rgb64, uint32_t* total
     if (rgb) APG_HIP_CHECK(ctx, hipMemcpy(rgb, R->accum, 3 * size_t(R->P) * 4, hipMemcpyDeviceToHost));
     if (rgb64) APG_HIP_CHECK(ctx, hipMemcpy(rgb64, R->accum64, 3 * size_t(R->P) * 8, hipMemcpyDeviceToHost));
     if (total_spp) *total_spp = R->total_spp;
+    return APG_OK;
+}
+
+int apg_render_postprocess(apg_context* ctx, int tonemap, uint8_t* frame, void* stream_)
+{
+    if (!ctx || !frame) return APG_E_INVALID;
+    RenderState* R = state_of(ctx);
+    if (!R->P) return ctx->fail(APG_E_STATE, "apg_render_reset has not been called");
+    if (R->total_spp == 0) return ctx->fail(APG_E_STATE, "no samples accumulated");
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    std::lock_guard<std::mutex> lock(ctx->launch_mutex);
+    if (ctx->last_launch) APG_HIP_CHECK(ctx, hipStreamWaitEvent(stream, ctx->last_launch, 0));
+    const uint32_t n3 = 3u * R->P;
+    k_postprocess<<<nblocks(n3, 256), 256, 0, stream>>>(R->accum, n3, float(R->total_spp),
+                                                        tonemap ? 1u : 0u, frame);
+    APG_HIP_CHECK(ctx, hipGetLastError());
     return APG_OK;
 }
 

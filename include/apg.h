@@ -213,6 +213,9 @@ int apg_render(apg_context* ctx, const apg_render_params* params, void* stream);
 /* Copy the accumulated radiance sum (width x height x 3, host) and the sample count.
  * rgb64 (optional) receives the double accumulation when fp64_accum was set. */
 int apg_render_read(apg_context* ctx, float* rgb, double* rgb64, uint32_t* total_spp);
+/* Alpine::postProcess without denoising (alpine.cpp:446-483): mean colour, ACES fit when
+ * tonemap != 0, clamp, sRGB, bytes.  frame: width x height x 3 bytes, device memory. */
+int apg_render_postprocess(apg_context* ctx, int tonemap, uint8_t* frame, void* stream);
 /* Ray and path counters of the last apg_render call (host out): closest-hit rays, shadow
  * rays, path vertices shaded. */
 int apg_render_counters(const apg_context* ctx, uint64_t* closest_rays, uint64_t* shadow_rays,

@@ -337,6 +337,8 @@ def _render_sigs(L):
                                    C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.orc_light_pdf.restype = C.c_float
     L.orc_light_pdf.argtypes = [vp, C.c_int, _f32p, _f32p, C.POINTER(C.c_float)]
+    L.orc_postprocess.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_int,
+                                  np.ctypeslib.ndpointer(np.uint8, flags="C")]
 
 
 _render_ready = False
@@ -438,6 +440,15 @@ def sampler_states(n_pixels, seed_offset=0):
     for i in range(n_pixels):
         L.orc_sampler_reset(st[i], (i + seed_offset) & 0xFFFFFFFF)
     return st
+
+
+def postprocess(accum, total_spp, tonemap=False):
+    """Alpine::postProcess without denoising (alpine.cpp:446-483): accum (..., 3) -> u8."""
+    a = _f32(accum)
+    out = np.empty(a.shape, np.uint8)
+    rlib().orc_postprocess(a.reshape(-1), a.size // 3, int(total_spp), int(bool(tonemap)),
+                           out.reshape(-1))
+    return out
 
 
 def sampler_sequence(seed, n):

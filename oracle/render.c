@@ -1044,3 +1044,29 @@ void orc_render(orc_scene* S, const orc_render_params* rp, uint32_t* states, dou
     }
     for (uint32_t k = 0; k < nthreads; ++k) pthread_join(th[k], NULL);
 }
+
+/* Alpine::postProcess without denoising (alpine.cpp:433-483): divide the colour sum by the
+ * sample count, optional ACES fit (applyAcesTone, :434-442), clamp to [0, 1], sRGB encode
+ * (linearToSrgb, :461-465) and round to bytes (:477-479). */
+static float orc_aces(float c)
+{
+    const float a = 2.51f, b = 0.03f, cc = 2.43f, d = 0.59f, e = 0.14f;
+    return (c * (c * a + b)) / (c * (c * cc + d) + e);
+}
+
+static float orc_srgb(float c)
+{
+    return c <= 0.0031308f ? 12.92f * c : 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
+}
+
+void orc_postprocess(const float* accum, uint32_t n_pixels, uint32_t total_spp, int tonemap,
+                     uint8_t* out)
+{
+    for (uint32_t i = 0; i < 3 * n_pixels; ++i)
+    {
+        float c = accum[i] / (float)total_spp;
+        if (tonemap) c = orc_aces(c);
+        c = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+        out[i] = (uint8_t)(orc_srgb(c) * 255.0f + 0.5f);
+    }
+}

@@ -51,6 +51,8 @@ void orc_render(orc_scene* s, const orc_render_params* rp, uint32_t* states, dou
                 uint32_t nthreads);
 
 void orc_sampler_reset(uint32_t* state4, uint32_t seed);
+void orc_postprocess(const float* accum, uint32_t n_pixels, uint32_t total_spp, int tonemap,
+                     uint8_t* out);
 float orc_sampler_get1d(uint32_t* state4);
 void orc_camera_ray(const orc_scene* s, float x, float y, float* o3, float* d3);
 int orc_mat_sample(const orc_material* m, const float* wo3, const float* u2, float* est3,

@@ -158,3 +158,29 @@ def test_lambert_white_furnace():
         assert ok
         np.testing.assert_allclose(est, np.float32([0.73, 0.5, 0.25]), rtol=1e-6)
         assert pdf == pytest.approx(max(wi[2], 0) / np.pi, rel=1e-6)
+
+
+def test_postprocess_closed_forms():
+    """Alpine::postProcess (alpine.cpp:446-483) at points the formulas fix exactly."""
+    srgb = lambda c: 12.92 * c if c <= 0.0031308 else 1.055 * c ** (1 / 2.4) - 0.055
+    aces = lambda c: (c * (2.51 * c + 0.03)) / (c * (2.43 * c + 0.59) + 0.14)
+    vals = np.array([0.0, 1e-4, 0.0031308, 0.01, 0.18, 0.5, 0.9, 1.0, 2.0, 7.5, -3.0],
+                    np.float32)
+    spp = 4
+    acc = np.repeat((vals * spp)[:, None], 3, axis=1)
+    for tm in (False, True):
+        got = oracle.postprocess(acc, spp, tm)
+        for i, v in enumerate(vals.astype(np.float64)):
+            c = aces(v) if tm else v
+            c = min(max(c, 0.0), 1.0)
+            want = srgb(c) * 255.0 + 0.5
+            # the float32 pipeline may round across an integer only when the double value
+            # sits within 1e-4 of one
+            assert got[i, 0] == int(want) or abs(want - round(want)) < 1e-4, (tm, v, got[i])
+    # fixed points: black stays 0, white 255, anything above 1 clamps to 255 without tone map
+    out = oracle.postprocess(np.array([[0, 0, 0], [4, 4, 4], [40, 40, 40]], np.float32), 4)
+    assert out[0].tolist() == [0, 0, 0] and out[1].tolist() == [255] * 3 == out[2].tolist()
+    # the tone map is monotone and below 1 for every finite input
+    x = np.linspace(0, 50, 2001, dtype=np.float32)
+    t = oracle.postprocess(np.repeat(x[:, None], 3, 1), 1, True)[:, 0].astype(int)
+    assert np.all(np.diff(t) >= 0) and t[-1] <= 255
