@@ -355,3 +355,67 @@ def many_light_scene(width=1920, height=1080, spp=64, max_depth=8, rr_depth=3, g
                fovy=float(np.radians(45.0)), aspect=width / height)
     return RenderScene(verts, tris, tri_mat, mats, lights, cam, width, height, spp, max_depth,
                        light_sampler=2, rr_depth=rr_depth, seed=seed)
+
+
+def guiding_scene(width=256, height=256, spp=16, max_depth=8, grid=64, face_res=4, seed=5,
+                  radiance=(400.0, 380.0, 340.0)):
+    """Learned-guiding workload: the many-light scene's box grid (GGX conductors and coated
+    dielectrics on a ground plane) closed in by walls and a ceiling, lit only by one small
+    quad light in a side chamber behind a 4 x 3 doorway, so every visible surface is lit
+    indirectly through the opening.  The camera stands inside the main room."""
+    rng = np.random.default_rng(seed)
+    half = grid / 2.0
+    H = 6.0
+    meshes, tri_mat, mats = [], [], []
+    n_mat = 16
+    for k in range(n_mat):
+        a = float(rng.uniform(0.05, 0.8))
+        if k % 2 == 0:
+            mats.append((1, tuple(rng.uniform(0.2, 1.0, 3)), (a, a)))
+        else:
+            mats.append((0, tuple(rng.uniform(0.1, 0.9, 3)), (a, a)))
+    wall = len(mats)
+    mats.append((0, (0.75, 0.75, 0.75), (1.0, 1.0)))
+
+    def add(m, k):
+        meshes.append(m)
+        tri_mat.append(np.full(len(m[1]), k, np.uint32))
+
+    # main room: floor, ceiling, walls with inward normals
+    add(quad_mesh((-half, 0, -half), (0, 0, 2 * half), (2 * half, 0, 0), 32, 32), wall)
+    add(quad_mesh((-half, H, -half), (2 * half, 0, 0), (0, 0, 2 * half), 32, 32), wall)
+    add(quad_mesh((-half, 0, -half), (0, H, 0), (0, 0, 2 * half), 8, 32), wall)
+    add(quad_mesh((half, 0, -half), (0, 0, 2 * half), (0, H, 0), 32, 8), wall)
+    add(quad_mesh((-half, 0, -half), (2 * half, 0, 0), (0, H, 0), 32, 8), wall)
+    # front wall at z = +half with a doorway x in [-2, 2], y in [0, 3]; both faces
+    t = 0.2
+    for (x0, x1, y0, y1) in [(-half, -2.0, 0, H), (2.0, half, 0, H), (-2.0, 2.0, 3.0, H)]:
+        add(quad_mesh((x0, y0, half), (0, y1 - y0, 0), (x1 - x0, 0, 0), 8, 4), wall)
+        add(quad_mesh((x0, y0, half + t), (x1 - x0, 0, 0), (0, y1 - y0, 0), 8, 4), wall)
+    add(quad_mesh((-2.0, 0, half), (0, 0, t), (0, 3.0, 0), 1, 2), wall)
+    add(quad_mesh((2.0, 0, half), (0, 3.0, 0), (0, 0, t), 2, 1), wall)
+    add(quad_mesh((-2.0, 3.0, half), (4.0, 0, 0), (0, 0, t), 2, 1), wall)
+    # side chamber z in [half + t, half + 8], x in [-8, 8]
+    z0, z1, cx = half + t, half + 8.0, 8.0
+    add(quad_mesh((-cx, 0, z0), (0, 0, z1 - z0), (2 * cx, 0, 0), 8, 8), wall)
+    add(quad_mesh((-cx, H, z0), (2 * cx, 0, 0), (0, 0, z1 - z0), 8, 8), wall)
+    add(quad_mesh((-cx, 0, z0), (0, H, 0), (0, 0, z1 - z0), 4, 8), wall)
+    add(quad_mesh((cx, 0, z0), (0, 0, z1 - z0), (0, H, 0), 8, 4), wall)
+    add(quad_mesh((-cx, 0, z1), (0, H, 0), (2 * cx, 0, 0), 4, 8), wall)
+    # the box grid
+    for i in range(grid):
+        for j in range(grid):
+            sx, sz = rng.uniform(0.3, 0.8, 2)
+            h = rng.uniform(0.2, 2.0)
+            ang = rng.uniform(0, 90)
+            add(box_mesh(sx, h, sz, ang, (-half + i + 0.5, 0.0, -half + j + 0.5), face_res),
+                int(rng.integers(0, n_mat)))
+    verts, tris = concat(*meshes)
+    tri_mat = np.concatenate(tri_mat)
+    s = 1.0
+    lights = [dict(kind=2, p0=(-s / 2, H - 0.01, z1 - 3.0), p1=(s / 2, H - 0.01, z1 - 3.0),
+                   p2=(-s / 2, H - 0.01, z1 - 3.0 + s), color=radiance)]
+    cam = dict(eye=(6.0, 4.5, half - 14.0), target=(-2.0, 0.5, half - 1.0), up=(0.0, 1.0, 0.0),
+               fovy=float(np.radians(60.0)), aspect=width / height)
+    return RenderScene(verts, tris, tri_mat, mats, lights, cam, width, height, spp, max_depth,
+                       light_sampler=0, seed=seed)

@@ -48,7 +48,10 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=262_144)
     ap.add_argument("--builder", choices=["gpu", "host"], default="gpu")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing configs")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--autotune", action="store_true",
+                    help="sweep the launch shape first (the library default is the sweep's pick)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape "
+                    "(the default behaviour)")
     return ap.parse_args()
 
 
@@ -69,6 +72,20 @@ def timed_passes(fn, steps, warmup):
     wall = (time.perf_counter() - t0) * 1e3 / steps
     ev = [s.elapsed_time(e) for s, e in zip(start, stop)]
     return float(np.mean(ev)), wall
+
+
+PROFILE_JSON = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "trace_kernel_incoherent_closest_1p5Mtri_4Mrays.json")
+
+
+def _profiled_traffic():
+    """HBM-side bytes per launch of this kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (each scaled by the counter's reading for a 4 GiB copy), or None."""
+    try:
+        with open(PROFILE_JSON) as f:
+            return int(json.load(f)["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def hbm_copy_gbs():
@@ -123,10 +140,10 @@ def render_benchmarks(builder):
     acc = _render_setup(sc, builder)
     _timed_render(acc, sc, 1, 0)
     dt, img, _ = _timed_render(acc, sc, sc.spp, sc.seed)
+    c = acc.render_counters()
     ref_spp = 1024
     _, _, ref64 = _timed_render(acc, sc, ref_spp, 1000, fp64=True)
     ref = ref64 / ref_spp
-    c = acc.render_counters()
     out["cornell_256_16spp"] = {
         "seconds": round(dt, 4), "samples_per_s": round(sc.width * sc.height * sc.spp / dt, 1),
         "rel_mse_vs_1024spp_fp64": round(scenes.rel_mse(img / sc.spp, ref), 6),
@@ -235,7 +252,7 @@ def main():
     torch.cuda.synchronize()
 
     tuned = None
-    if not args.no_autotune:
+    if args.autotune and not args.no_autotune:
         from alpine_amd import autotune
         n_t = 1 << 20
         tuned, _ = autotune.autotune(acc, o_d[:, :n_t].contiguous(), d_d[:, :n_t].contiguous(),
@@ -279,7 +296,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _profiled_traffic(),
             "bytes_per_ray": round(b_closest, 1), "node_fetches_per_ray": round(node_pr, 3),
             "tri_fetches_per_ray": round(tri_pr, 3), "hbm_copy_gbs": round(bw, 1),
             "frac_of_copy": round(achieved / bw, 4),
