@@ -118,10 +118,11 @@ ABI_SYMBOLS = [
     "apg_scene_get_stats", "apg_trace_config_default", "apg_set_trace_config", "apg_intersect",
     "apg_occluded", "apg_intersect_counted", "apg_scene_triangles", "apg_scene_leaf_keys",
     "apg_scene_wide_nodes", "apg_materials_set", "apg_lights_set", "apg_light_sampler_sample",
-    "apg_light_sampler_pdf", "apg_render_reset", "apg_render_set_camera", "apg_render",
+    "apg_light_sampler_pdf", "apg_render_reset", "apg_render_reset_slots", "apg_render_set_camera", "apg_render",
     "apg_render_read", "apg_render_postprocess", "apg_render_counters", "apg_guide_param_count",
     "apg_guide_init",
-    "apg_guide_params", "apg_guide_forward", "apg_guide_train_step", "apg_render_guide_stats",
+    "apg_guide_params", "apg_guide_forward", "apg_guide_infer", "apg_guide_train_step",
+    "apg_render_guide_stats",
 ]
 DEBUG_SYMBOLS = ["apg_debug_host_build", "apg_debug_scene_info", "apg_debug_scene_copy",
                  "apg_debug_host_trace", "apg_debug_scene_free", "apg_debug_material",
@@ -170,6 +171,7 @@ def lib():
         "apg_light_sampler_sample": (i32, [vp, vp, vp, vp, u32, vp, vp, vp]),
         "apg_light_sampler_pdf": (i32, [vp, vp, vp, vp, u32, vp, vp]),
         "apg_render_reset": (i32, [vp, u32, u32, u32]),
+        "apg_render_reset_slots": (i32, [vp, u32, u32, u32, u32]),
         "apg_render_set_camera": (i32, [vp, C.POINTER(Camera)]),
         "apg_render": (i32, [vp, C.POINTER(RenderParams), vp]),
         "apg_render_read": (i32, [vp, vp, vp, C.POINTER(u32)]),
@@ -181,6 +183,7 @@ def lib():
         "apg_guide_init": (i32, [vp, u32, C.c_float]),
         "apg_guide_params": (i32, [vp, vp, i32]),
         "apg_guide_forward": (i32, [vp, vp, u32, vp, vp]),
+        "apg_guide_infer": (i32, [vp, vp, u32, vp, vp]),
         "apg_guide_train_step": (i32, [vp, vp, vp, vp, u32, i32, C.POINTER(C.c_float), vp, vp]),
         "apg_debug_host_build": (i32, [vp, u32, vp, u32, u32, C.POINTER(vp)]),
         "apg_debug_scene_info": (i32, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_double),
@@ -398,9 +401,13 @@ class GpuAccelerator:
                                                  _ptr(pdf), self._stream(stream)))
         return pdf
 
-    def render_reset(self, width, height, seed=0):
+    def render_reset(self, width, height, seed=0, slots=1):
+        """Clear the accumulation; `slots` samples per pixel are traced side by side per pass."""
         self.width, self.height = width, height
-        self._rc(self._lib.apg_render_reset(self._h, width, height, seed))
+        if slots == 1:
+            self._rc(self._lib.apg_render_reset(self._h, width, height, seed))
+        else:
+            self._rc(self._lib.apg_render_reset_slots(self._h, width, height, seed, slots))
 
     def set_camera(self, cam):
         self._rc(self._lib.apg_render_set_camera(self._h, C.byref(cam)))
@@ -456,6 +463,14 @@ class GpuAccelerator:
         n = feat.shape[0]
         out = torch.empty((n, 48), dtype=torch.float32, device=feat.device)
         self._rc(self._lib.apg_guide_forward(self._h, _ptr(feat), n, _ptr(out), self._stream(stream)))
+        return out
+
+    def guide_infer(self, q, stream=None):
+        """Proposal parameters (n x 48) from n x 9 queries (p01, n, wo), encoding fused in."""
+        import torch
+        n = q.shape[0]
+        out = torch.empty((n, 48), dtype=torch.float32, device=q.device)
+        self._rc(self._lib.apg_guide_infer(self._h, _ptr(q), n, _ptr(out), self._stream(stream)))
         return out
 
     def guide_train_step(self, feat, dirs, weight, apply=True, want_grad=False, stream=None):

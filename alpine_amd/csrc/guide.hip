@@ -189,6 +189,142 @@ __global__ __launch_bounds__(BLOCK) void k_guide_fwd(FwdArgs A)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Inference with the encoding fused in: queries are (p01, n, wo), 9 floats.
+// Each wave carries 16 queries through the network in the transposed form
+// D[unit][query] = W[unit][:] . act[query][:], so a layer's MFMA result tile
+// (lane (r, kg) holds units 4 kg .. 4 kg + 3 of query r) is exactly the next
+// layer's B operand for one 16-wide k-step: activations never leave
+// registers.  Weights live in LDS with rows padded by 4 BF16 so the 16 rows a
+// wave reads at once fall in different banks.
+// ---------------------------------------------------------------------------
+constexpr int IPAD = 4;
+__host__ __device__ constexpr int infer_row(int l) { return (l == 0 ? GUIDE_IN : GUIDE_HID) + IPAD; }
+__host__ __device__ constexpr int infer_off(int l)
+{
+    return l == 0 ? 0 : GUIDE_HID * infer_row(0) + (l - 1) * GUIDE_HID * infer_row(1);
+}
+constexpr int INFER_WSZ = infer_off(4) + GUIDE_OUT * infer_row(4);
+
+struct InferArgs
+{
+    const uint16_t* w16;
+    const float* w32;
+    const float* q;         // M x 9
+    uint32_t M;
+    const uint32_t* M_dev;
+    float* out;             // M x 48
+};
+
+__device__ __forceinline__ uint32_t pack_bf2(float a, float b)
+{
+    return uint32_t(f2bf(a)) | (uint32_t(f2bf(b)) << 16);
+}
+
+__device__ __forceinline__ s4v as_s4v(uint32_t lo, uint32_t hi)
+{
+    s4v r;
+    r[0] = short(lo & 0xFFFFu);
+    r[1] = short(lo >> 16);
+    r[2] = short(hi & 0xFFFFu);
+    r[3] = short(hi >> 16);
+    return r;
+}
+
+// One layer: act_out tiles (N / 16) from act_in tiles (K / 16).
+template <int L, int K, int N, bool RELU>
+__device__ __forceinline__ void infer_layer(const uint16_t* sw, const float* sb, const s4v (&in)[K / 16],
+                                            f4v (&res)[N / 16], int r, int kg)
+{
+    const uint16_t* W = sw + infer_off(L);
+#pragma unroll
+    for (int nt = 0; nt < N / 16; ++nt)
+    {
+        f4v acc = { 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+        for (int ks = 0; ks < K / 16; ++ks)
+        {
+            const s4v a = load_a_row(W + (nt * 16 + r) * infer_row(L) + ks * 16 + 4 * kg);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, in[ks], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+        {
+            float v = acc[i] + sb[guide_b_off(L) - 21504 + nt * 16 + 4 * kg + i];
+            if (RELU) v = v > 0.0f ? v : 0.0f;
+            acc[i] = v;
+        }
+        res[nt] = acc;
+    }
+}
+
+template <int T>
+__device__ __forceinline__ void to_bf16_tiles(const f4v (&x)[T], s4v (&y)[T])
+{
+#pragma unroll
+    for (int t = 0; t < T; ++t) y[t] = as_s4v(pack_bf2(x[t][0], x[t][1]), pack_bf2(x[t][2], x[t][3]));
+}
+
+__global__ __launch_bounds__(BLOCK) void k_guide_infer(InferArgs A)
+{
+    __shared__ __align__(16) uint16_t sw[INFER_WSZ];
+    __shared__ float sb[GUIDE_PARAMS - 21504];
+    for (int l = 0; l < GUIDE_LAYERS; ++l)
+    {
+        const int K = guide_k(l), N = guide_n(l);
+        for (int i = threadIdx.x; i < N * K / 4; i += BLOCK)
+        {
+            const int row = (4 * i) / K, col = (4 * i) % K;
+            *reinterpret_cast<uint2*>(sw + infer_off(l) + row * infer_row(l) + col) =
+                *reinterpret_cast<const uint2*>(A.w16 + guide_w_off(l) + 4 * i);
+        }
+    }
+    for (int i = threadIdx.x; i < GUIDE_PARAMS - 21504; i += BLOCK) sb[i] = A.w32[21504 + i];
+    __syncthreads();
+    const uint32_t M = A.M_dev ? *A.M_dev : A.M;
+    const int lane = threadIdx.x & 63, r = lane & 15, kg = lane >> 4;
+    const uint32_t nwaves = gridDim.x * WAVES;
+    for (uint32_t tile = blockIdx.x * WAVES + (threadIdx.x >> 6); tile * 16u < M; tile += nwaves)
+    {
+        const uint32_t qi = tile * 16u + uint32_t(r);
+        float p[3] = { 0, 0, 0 }, n[3] = { 0, 0, 0 }, wo[3] = { 0, 0, 0 };
+        if (qi < M)
+        {
+            const float* qq = A.q + size_t(qi) * 9;
+            p[0] = qq[0]; p[1] = qq[1]; p[2] = qq[2];
+            n[0] = qq[3]; n[1] = qq[4]; n[2] = qq[5];
+            wo[0] = qq[6]; wo[1] = qq[7]; wo[2] = qq[8];
+        }
+        s4v h0[GUIDE_IN / 16];
+#pragma unroll
+        for (int ks = 0; ks < GUIDE_IN / 16; ++ks)
+        {
+            const int f = ks * 16 + 4 * kg;
+            h0[ks] = as_s4v(pack_bf2(guide_feature(f, p, n, wo), guide_feature(f + 1, p, n, wo)),
+                            pack_bf2(guide_feature(f + 2, p, n, wo), guide_feature(f + 3, p, n, wo)));
+        }
+        f4v t1[4], t2[4];
+        s4v h1[4], h2[4];
+        infer_layer<0, GUIDE_IN, GUIDE_HID, true>(sw, sb, h0, t1, r, kg);
+        to_bf16_tiles(t1, h1);
+        infer_layer<1, GUIDE_HID, GUIDE_HID, true>(sw, sb, h1, t2, r, kg);
+        to_bf16_tiles(t2, h2);
+        infer_layer<2, GUIDE_HID, GUIDE_HID, true>(sw, sb, h2, t1, r, kg);
+        to_bf16_tiles(t1, h1);
+        infer_layer<3, GUIDE_HID, GUIDE_HID, true>(sw, sb, h1, t2, r, kg);
+        to_bf16_tiles(t2, h2);
+        f4v o[GUIDE_OUT / 16];
+        infer_layer<4, GUIDE_HID, GUIDE_OUT, false>(sw, sb, h2, o, r, kg);
+        if (qi < M)
+        {
+#pragma unroll
+            for (int nt = 0; nt < GUIDE_OUT / 16; ++nt)
+                *reinterpret_cast<float4*>(A.out + size_t(qi) * GUIDE_OUT + nt * 16 + 4 * kg) =
+                    make_float4(o[nt][0], o[nt][1], o[nt][2], o[nt][3]);
+        }
+    }
+}
+
 // Loss head: dZ5 = d(-w log q)/d out, scaled by 1 / sum(w).
 __global__ void k_guide_wsum(const float* weight, uint32_t M, float* scratch)
 {
@@ -457,6 +593,18 @@ int guide_forward(Context* ctx, const float* feat, uint32_t M, const uint32_t* M
 }
 
 // Forward + backward into g->grad (zeroed first); optionally the Adam step.
+int guide_infer(Context* ctx, const float* q9, uint32_t M, const uint32_t* M_dev, float* out, hipStream_t s)
+{
+    int rc = guide_ensure(ctx, 1);
+    if (rc) return rc;
+    GuideNet* g = guide_of(ctx);
+    InferArgs A{ g->w16, g->w32, q9, M, M_dev, out };
+    const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(nb(M, 16 * WAVES), uint32_t(ctx->cu_count) * 3));
+    k_guide_infer<<<grid, BLOCK, 0, s>>>(A);
+    APG_HIP_CHECK(ctx, hipGetLastError());
+    return APG_OK;
+}
+
 int guide_train(Context* ctx, const float* feat, const float* dir, const float* weight, uint32_t M,
                 bool adam, float* loss, hipStream_t s)
 {
@@ -513,6 +661,13 @@ int apg_guide_init(apg_context* ctx, uint32_t seed, float lr)
     g->step = 0;
     g->lr = lr > 0.0f ? lr : 1e-3f;
     return APG_OK;
+}
+
+int apg_guide_infer(apg_context* ctx, const float* q, uint32_t n, float* out, void* stream)
+{
+    if (!ctx || (n && (!q || !out))) return APG_E_INVALID;
+    if (n == 0) return APG_OK;
+    return guide_infer(ctx, q, n, nullptr, out, static_cast<hipStream_t>(stream));
 }
 
 int apg_guide_params(apg_context* ctx, float* params, int write)

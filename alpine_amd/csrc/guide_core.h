@@ -40,25 +40,40 @@ SHD f3 load3c(const float* a, uint32_t i)
     return mk(a[3 * size_t(i)], a[3 * size_t(i) + 1], a[3 * size_t(i) + 2]);
 }
 
+// sin(2 pi x) for x in [0, 1).
+SHD float sin_rev(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sinf(x);
+#else
+    return sinf(6.28318530717958647692f * x);
+#endif
+}
+
+// Feature f (0..95) of a query.  sin and cos of 2^k pi p are evaluated as
+// sin(2 pi frac(2^(k-1) p + c / 4)), c = 0 / 1, so the phase is exact in
+// float at every frequency.
+SHD float guide_feature(int f, const float (&p)[3], const float (&n)[3], const float (&wo)[3])
+{
+    if (f < 3) return f == 0 ? p[0] : (f == 1 ? p[1] : p[2]);
+    if (f < 75)
+    {
+        const int g = f - 3, a = g / 24, k = (g % 24) >> 1;
+        const float pa = a == 0 ? p[0] : (a == 1 ? p[1] : p[2]);
+        float x = ldexpf(pa, k - 1) + ((g & 1) ? 0.25f : 0.0f);
+        x -= floorf(x);
+        return sin_rev(x);
+    }
+    if (f < 78) return f == 75 ? n[0] : (f == 76 ? n[1] : n[2]);
+    if (f < 81) return f == 78 ? wo[0] : (f == 79 ? wo[1] : wo[2]);
+    return 0.0f;
+}
+
 // Encoding of one query into 96 floats.
 SHD void guide_encode(f3 p01, f3 n, f3 wo, float* f)
 {
-    f[0] = p01.x; f[1] = p01.y; f[2] = p01.z;
-    const float pa[3] = { p01.x, p01.y, p01.z };
-    for (int a = 0; a < 3; ++a)
-    {
-        float s = 1.0f;
-        for (int k = 0; k < GUIDE_FREQS; ++k)
-        {
-            const float x = s * kPi * pa[a];
-            f[3 + 24 * a + 2 * k] = sinf(x);
-            f[3 + 24 * a + 2 * k + 1] = cosf(x);
-            s *= 2.0f;
-        }
-    }
-    f[75] = n.x; f[76] = n.y; f[77] = n.z;
-    f[78] = wo.x; f[79] = wo.y; f[80] = wo.z;
-    for (int i = 81; i < GUIDE_IN; ++i) f[i] = 0.0f;
+    const float p[3] = { p01.x, p01.y, p01.z }, nn[3] = { n.x, n.y, n.z }, w[3] = { wo.x, wo.y, wo.z };
+    for (int i = 0; i < GUIDE_IN; ++i) f[i] = guide_feature(i, p, nn, w);
 }
 
 struct VmfMix

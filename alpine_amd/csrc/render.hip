@@ -31,6 +31,7 @@ int guide_forward(Context* ctx, const float* feat, uint32_t M, const uint32_t* M
                   uint16_t* acts, hipStream_t s);
 int guide_train(Context* ctx, const float* feat, const float* dir, const float* weight, uint32_t M,
                 bool adam, float* loss, hipStream_t s);
+int guide_infer(Context* ctx, const float* q9, uint32_t M, const uint32_t* M_dev, float* out, hipStream_t s);
 
 struct Queue
 {
@@ -60,11 +61,13 @@ struct RenderState
     int32_t* leaf_of = nullptr;
     DevCamera cam{};
     bool cam_set = false;
-    // pixels
-    uint32_t W = 0, H = 0, P = 0;
-    uint32_t* rng = nullptr;
-    float* accum = nullptr;
-    double* accum64 = nullptr;
+    // pixels and paths: P = NPIX x S paths per pass, S samples of each pixel in flight
+    uint32_t W = 0, H = 0, P = 0, NPIX = 0, S = 1;
+    uint32_t* rng = nullptr;      // per path slot
+    float* accum = nullptr;       // per path slot
+    double* accum64 = nullptr;    // per path slot
+    float* px = nullptr;          // per pixel: slot sums (reads, postprocess)
+    double* px64 = nullptr;
     uint32_t total_spp = 0;
     // paths
     float* T = nullptr;
@@ -80,7 +83,7 @@ struct RenderState
     unsigned long long* stats = nullptr;    // [0] closest rays, [1] shadow rays, [2] vertices
     std::vector<void*> pixel_allocs;
     // path guiding
-    float* gfeat = nullptr;   // P x 96 features of the shading points of a bounce
+    float* gfeat = nullptr;   // P x 9 network inputs of the shading points of a bounce
     float* gout = nullptr;    // P x 48 proposal parameters
     float* vrec = nullptr;    // P x vrec_depth x VREC vertex records
     uint32_t vrec_depth = 0;
@@ -88,6 +91,8 @@ struct RenderState
     float* tfeat = nullptr;   // tcap x 96
     float* tdir = nullptr;    // tcap x 3
     float* tw = nullptr;      // tcap
+    float* tdir_p = nullptr;  // tcap x 3, minibatch order
+    float* tw_p = nullptr;    // tcap, minibatch order
     uint32_t tcap = 0;
     uint32_t* tcount = nullptr;
     std::vector<void*> guide_allocs;
@@ -145,22 +150,27 @@ __device__ __forceinline__ f3 to_unit_box(const ShadeArgs& A, f3 p)
 }
 
 // Features of every shading point of the bounce (queue A), for the proposal network.
-__global__ void k_guide_features(ShadeArgs A, float* feat)
+// Network inputs of every shading point of the bounce (queue A): position
+// normalised to the scene bounds, shading normal, outgoing direction.
+__global__ void k_guide_features(ShadeArgs A, float* q9)
 {
     const uint32_t count = A.counts[0];
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x)
     {
-        float* f = feat + size_t(q) * GUIDE_IN;
+        float* f = q9 + size_t(q) * 9;
         const uint32_t prim = A.hprim[q];
         if (prim == 0xFFFFFFFFu)
         {
-            for (int i = 0; i < GUIDE_IN; ++i) f[i] = 0.0f;
+            for (int i = 0; i < 9; ++i) f[i] = 0.0f;
             continue;
         }
         const f3 ro = mk(A.in.o[0][q], A.in.o[1][q], A.in.o[2][q]);
         const f3 rd = mk(A.in.d[0][q], A.in.d[1][q], A.in.d[2][q]);
         const float4 ngm = A.tri_ng[prim];
-        guide_encode(to_unit_box(A, ro + rd * A.ht[q]), mk(ngm.x, ngm.y, ngm.z), -rd, f);
+        const f3 p = to_unit_box(A, ro + rd * A.ht[q]);
+        f[0] = p.x; f[1] = p.y; f[2] = p.z;
+        f[3] = ngm.x; f[4] = ngm.y; f[5] = ngm.z;
+        f[6] = -rd.x; f[7] = -rd.y; f[8] = -rd.z;
     }
 }
 
@@ -190,18 +200,44 @@ __device__ __forceinline__ f3 load3(const float* a, uint32_t i)
     return mk(a[3 * size_t(i)], a[3 * size_t(i) + 1], a[3 * size_t(i) + 2]);
 }
 
-__global__ void k_reset_pixels(uint32_t P, uint32_t seed, uint32_t* rng, float* accum, double* accum64)
+// Sampler seeds: slot 0 of pixel i is seeded with i + seed as the reference
+// seeds its per-pixel sampler (alpine.cpp:259-265); further slots of the
+// pixel get hashed seeds of their own.
+__device__ __forceinline__ uint32_t slot_seed(uint32_t pixel, uint32_t slot, uint32_t seed)
+{
+    uint32_t h = pixel + seed;
+    if (slot == 0u) return h;
+    h ^= slot * 0x9E3779B9u;
+    h ^= h >> 16; h *= 0x7feb352du;
+    h ^= h >> 15; h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return h;
+}
+
+__global__ void k_reset_pixels(uint32_t P, uint32_t NPIX, uint32_t seed, uint32_t* rng, float* accum,
+                               double* accum64)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
     uint32_t s[4];
-    sampler_reset(s, p + seed);
+    sampler_reset(s, slot_seed(p % NPIX, p / NPIX, seed));
     for (int k = 0; k < 4; ++k) rng[4 * size_t(p) + k] = s[k];
     for (int k = 0; k < 3; ++k)
     {
         accum[3 * size_t(p) + k] = 0.0f;
         accum64[3 * size_t(p) + k] = 0.0;
     }
+}
+
+// Per-pixel sums over the slots, in slot order.
+template <typename T>
+__global__ void k_sum_slots(const T* acc, uint32_t NPIX, uint32_t S, T* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * NPIX) return;
+    T v = acc[i];
+    for (uint32_t s = 1; s < S; ++s) v += acc[size_t(s) * 3 * NPIX + i];
+    out[i] = v;
 }
 
 __global__ void k_set_counts(uint32_t* counts, uint32_t a)
@@ -215,15 +251,16 @@ __global__ void k_set_counts(uint32_t* counts, uint32_t a)
 }
 
 // Camera rays with the per-pixel jitter (alpine.cpp:305-312).
-__global__ void k_generate(DevCamera cam, uint32_t W, uint32_t H, uint32_t* rng, float* T, float* L,
+__global__ void k_generate(DevCamera cam, uint32_t W, uint32_t H, uint32_t P, uint32_t* rng, float* T, float* L,
                            uint32_t* state, Queue q, unsigned long long* stats)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= W * H) return;
+    if (p >= P) return;
     uint32_t s[4];
     for (int k = 0; k < 4; ++k) s[k] = rng[4 * size_t(p) + k];
     const f2 j = get2d(s);
-    const uint32_t x = p % W, y = p / W;
+    const uint32_t pix = p % (W * H);
+    const uint32_t x = pix % W, y = pix / W;
     f3 o, d;
     camera_ray(cam, (float(x) + j.x) / float(W), (float(y) + j.y) / float(H), o, d);
     for (int k = 0; k < 4; ++k) rng[4 * size_t(p) + k] = s[k];
@@ -233,7 +270,7 @@ __global__ void k_generate(DevCamera cam, uint32_t W, uint32_t H, uint32_t* rng,
     q.o[0][p] = o.x; q.o[1][p] = o.y; q.o[2][p] = o.z;
     q.d[0][p] = d.x; q.d[1][p] = d.y; q.d[2][p] = d.z;
     q.path[p] = p;
-    if (p == 0) atomicAdd(&stats[0], (unsigned long long)(W * H));
+    if (p == 0) atomicAdd(&stats[0], (unsigned long long)P);
 }
 
 template <bool GUIDED>
@@ -569,12 +606,17 @@ __global__ void k_postprocess(const float* accum, uint32_t n3, float spp, uint32
     out[i] = uint8_t(c * 255.0f + 0.5f);
 }
 
-__global__ void k_train_encode(const float* tin, uint32_t n, float* feat)
+// Training inputs in minibatch order: sample i goes to minibatch i % S, so
+// each of the S minibatches spreads over the whole image.
+__global__ void k_train_encode(const float* tin, const float* tdir, const float* tw, uint32_t n, uint32_t S,
+                               uint32_t B, float* feat, float* dir_p, float* w_p)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float* t = tin + 9 * size_t(i);
-    guide_encode(mk(t[0], t[1], t[2]), mk(t[3], t[4], t[5]), mk(t[6], t[7], t[8]), feat + size_t(i) * GUIDE_IN);
+    const uint32_t o = (i % S) * B + i / S;
+    guide_encode(load3(tin, 3 * i), load3(tin, 3 * i + 1), load3(tin, 3 * i + 2), feat + size_t(o) * GUIDE_IN);
+    for (int k = 0; k < 3; ++k) dir_p[3 * size_t(o) + k] = tdir[3 * size_t(i) + k];
+    w_p[o] = tw[i];
 }
 
 __global__ void k_tri_ng(const float* tri9, const uint32_t* ids, uint32_t n, float4* ng)
@@ -624,7 +666,8 @@ void free_pixels(RenderState* R)
 {
     for (void* p : R->pixel_allocs) (void)hipFree(p);
     R->pixel_allocs.clear();
-    R->P = R->W = R->H = 0;
+    R->P = R->W = R->H = R->NPIX = 0;
+    R->S = 1;
 }
 
 RenderState* state_of(Context* ctx)
@@ -802,14 +845,16 @@ int apg_render_set_camera(apg_context* ctx, const apg_camera* cam)
     return APG_OK;
 }
 
-int apg_render_reset(apg_context* ctx, uint32_t width, uint32_t height, uint32_t seed)
+int apg_render_reset_slots(apg_context* ctx, uint32_t width, uint32_t height, uint32_t seed, uint32_t slots)
 {
-    if (!ctx || width == 0 || height == 0) return APG_E_INVALID;
-    if (uint64_t(width) * height > (1u << 26)) return ctx->fail(APG_E_CAPACITY, "more than 2^26 pixels");
+    if (!ctx || width == 0 || height == 0 || slots == 0) return APG_E_INVALID;
+    if (uint64_t(width) * height * slots > (1u << 26))
+        return ctx->fail(APG_E_CAPACITY, "more than 2^26 paths per pass");
     APG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     RenderState* R = state_of(ctx);
-    const uint32_t P = width * height;
-    if (P != R->P)
+    const uint32_t NPIX = width * height;
+    const uint32_t P = NPIX * slots;
+    if (P != R->P || NPIX != R->NPIX)
     {
         if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
         free_pixels(R);
@@ -820,7 +865,8 @@ int apg_render_reset(apg_context* ctx, uint32_t width, uint32_t height, uint32_t
         };
         int rc = 0;
         if ((rc = get(R->rng, 4 * size_t(P))) || (rc = get(R->accum, 3 * size_t(P))) ||
-            (rc = get(R->accum64, 3 * size_t(P))) || (rc = get(R->T, 3 * size_t(P))) ||
+            (rc = get(R->accum64, 3 * size_t(P))) || (rc = get(R->px, 3 * size_t(NPIX))) ||
+            (rc = get(R->px64, 3 * size_t(NPIX))) || (rc = get(R->T, 3 * size_t(P))) ||
             (rc = get(R->L, 3 * size_t(P))) || (rc = get(R->state, P)) || (rc = get(R->nee, 10 * size_t(P))) ||
             (rc = get(R->vis, 2 * size_t(P))) || (rc = get(R->hprim, P)) || (rc = get(R->ht, P)) ||
             (rc = get(R->hb0, P)) || (rc = get(R->hb1, P)) || (rc = get(R->occ, 2 * size_t(P))) ||
@@ -836,14 +882,21 @@ int apg_render_reset(apg_context* ctx, uint32_t width, uint32_t height, uint32_t
             else q->tmax = nullptr;
         }
         R->P = P;
+        R->NPIX = NPIX;
     }
     R->W = width;
     R->H = height;
-    k_reset_pixels<<<nblocks(P, 256), 256>>>(P, seed, R->rng, R->accum, R->accum64);
+    R->S = slots;
+    k_reset_pixels<<<nblocks(P, 256), 256>>>(P, NPIX, seed, R->rng, R->accum, R->accum64);
     APG_HIP_CHECK(ctx, hipMemset(R->stats, 0, 8 * sizeof(unsigned long long)));
     APG_HIP_CHECK(ctx, hipDeviceSynchronize());
     R->total_spp = 0;
     return APG_OK;
+}
+
+int apg_render_reset(apg_context* ctx, uint32_t width, uint32_t height, uint32_t seed)
+{
+    return apg_render_reset_slots(ctx, width, height, seed, 1);
 }
 
 int apg_render_guide_stats(const apg_context* cctx, uint32_t* samples, float* loss)
@@ -869,10 +922,11 @@ static int ensure_guide_buffers(apg_context* ctx, RenderState* R, uint32_t depth
         return rc;
     };
     int rc;
-    if ((rc = get(R->gfeat, size_t(P) * GUIDE_IN)) || (rc = get(R->gout, size_t(P) * GUIDE_OUT)) ||
+    if ((rc = get(R->gfeat, size_t(P) * 9)) || (rc = get(R->gout, size_t(P) * GUIDE_OUT)) ||
         (rc = get(R->vrec, size_t(P) * depth * VREC)) || (rc = get(R->tin, size_t(tcap) * 9)) ||
         (rc = get(R->tfeat, size_t(tcap) * GUIDE_IN)) || (rc = get(R->tdir, size_t(tcap) * 3)) ||
-        (rc = get(R->tw, tcap)) || (rc = get(R->tcount, 4)))
+        (rc = get(R->tw, tcap)) || (rc = get(R->tdir_p, size_t(tcap) * 3)) || (rc = get(R->tw_p, tcap)) ||
+        (rc = get(R->tcount, 4)))
         return rc;
     R->vrec_depth = depth;
     R->tcap = tcap;
@@ -936,7 +990,8 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
     std::lock_guard<std::mutex> lock(ctx->launch_mutex);
     if (!ctx->last_launch) APG_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->last_launch, hipEventDisableTiming));
     APG_HIP_CHECK(ctx, hipStreamWaitEvent(stream, ctx->last_launch, 0));
-    for (uint32_t sidx = 0; sidx < prm->spp; ++sidx)
+    if (prm->spp % R->S) return ctx->fail(APG_E_INVALID, "spp must be a multiple of the slots per pixel");
+    for (uint32_t sidx = 0; sidx < prm->spp / R->S; ++sidx)
     {
         const bool guided = guide != 0 && (guide == 2 || R->guided_passes >= prm->guide_warmup);
         S.guide_frac = guided ? frac : 0.0f;
@@ -947,8 +1002,8 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
             APG_HIP_CHECK(ctx, hipMemsetAsync(R->tcount, 0, 4, stream));
         }
         k_set_counts<<<1, 64, 0, stream>>>(R->counts, P);
-        k_generate<<<nblocks(P, 256), 256, 0, stream>>>(R->cam, R->W, R->H, R->rng, R->T, R->L, R->state, R->qa,
-                                                          R->stats);
+        k_generate<<<nblocks(P, 256), 256, 0, stream>>>(R->cam, R->W, R->H, P, R->rng, R->T, R->L, R->state,
+                                                          R->qa, R->stats);
         Queue in = R->qa, out = R->qb;
         for (uint32_t b = 0; b < prm->max_depth; ++b)
         {
@@ -962,7 +1017,7 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
             if (guide)
             {
                 k_guide_features<<<sgrid, 256, 0, stream>>>(S, R->gfeat);
-                rc = guide_forward(ctx, R->gfeat, P, R->counts, R->gout, nullptr, stream);
+                rc = guide_infer(ctx, R->gfeat, P, R->counts, R->gout, stream);
                 if (rc) return rc;
                 k_shade<true><<<sgrid, 256, 0, stream>>>(S);
             }
@@ -988,11 +1043,17 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
             R->last_train_samples = tn;
             if (tn)
             {
-                k_train_encode<<<nblocks(tn, 256), 256, 0, stream>>>(R->tin, tn, R->tfeat);
-                for (uint32_t it = 0; it < train_steps; ++it)
+                // one epoch over the pass's samples in train_steps minibatches
+                const uint32_t S = std::min(train_steps, tn);
+                const uint32_t B = (tn + S - 1) / S;
+                k_train_encode<<<nblocks(tn, 256), 256, 0, stream>>>(R->tin, R->tdir, R->tw, tn, S, B, R->tfeat,
+                                                                     R->tdir_p, R->tw_p);
+                for (uint32_t j = 0; j < S; ++j)
                 {
-                    int rc = guide_train(ctx, R->tfeat, R->tdir, R->tw, tn, true,
-                                         it + 1 == train_steps ? &R->last_loss : nullptr, stream);
+                    const uint32_t mb = (tn - j + S - 1) / S;
+                    int rc = guide_train(ctx, R->tfeat + size_t(j) * B * GUIDE_IN, R->tdir_p + size_t(j) * B * 3,
+                                         R->tw_p + size_t(j) * B, mb, true,
+                                         j + 1 == S ? &R->last_loss : nullptr, stream);
                     if (rc) return rc;
                 }
             }
@@ -1011,8 +1072,19 @@ int apg_render_read(apg_context* ctx, float* rgb, double* rgb64, uint32_t* total
     RenderState* R = state_of(ctx);
     if (!R->P) return ctx->fail(APG_E_STATE, "apg_render_reset has not been called");
     if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
-    if (rgb) APG_HIP_CHECK(ctx, hipMemcpy(rgb, R->accum, 3 * size_t(R->P) * 4, hipMemcpyDeviceToHost));
-    if (rgb64) APG_HIP_CHECK(ctx, hipMemcpy(rgb64, R->accum64, 3 * size_t(R->P) * 8, hipMemcpyDeviceToHost));
+    const uint32_t n3 = 3u * R->NPIX;
+    if (rgb)
+    {
+        k_sum_slots<<<nblocks(n3, 256), 256>>>(R->accum, R->NPIX, R->S, R->px);
+        APG_HIP_CHECK(ctx, hipGetLastError());
+        APG_HIP_CHECK(ctx, hipMemcpy(rgb, R->px, size_t(n3) * 4, hipMemcpyDeviceToHost));
+    }
+    if (rgb64)
+    {
+        k_sum_slots<<<nblocks(n3, 256), 256>>>(R->accum64, R->NPIX, R->S, R->px64);
+        APG_HIP_CHECK(ctx, hipGetLastError());
+        APG_HIP_CHECK(ctx, hipMemcpy(rgb64, R->px64, size_t(n3) * 8, hipMemcpyDeviceToHost));
+    }
     if (total_spp) *total_spp = R->total_spp;
     return APG_OK;
 }
@@ -1026,8 +1098,9 @@ int apg_render_postprocess(apg_context* ctx, int tonemap, uint8_t* frame, void* 
     hipStream_t stream = static_cast<hipStream_t>(stream_);
     std::lock_guard<std::mutex> lock(ctx->launch_mutex);
     if (ctx->last_launch) APG_HIP_CHECK(ctx, hipStreamWaitEvent(stream, ctx->last_launch, 0));
-    const uint32_t n3 = 3u * R->P;
-    k_postprocess<<<nblocks(n3, 256), 256, 0, stream>>>(R->accum, n3, float(R->total_spp),
+    const uint32_t n3 = 3u * R->NPIX;
+    k_sum_slots<<<nblocks(n3, 256), 256, 0, stream>>>(R->accum, R->NPIX, R->S, R->px);
+    k_postprocess<<<nblocks(n3, 256), 256, 0, stream>>>(R->px, n3, float(R->total_spp),
                                                         tonemap ? 1u : 0u, frame);
     APG_HIP_CHECK(ctx, hipGetLastError());
     return APG_OK;

@@ -198,7 +198,8 @@ typedef struct apg_render_params {
     uint32_t fp64_accum;       /* also accumulate in double */
     float background[3];       /* radiance on a miss (alpine.cpp:318-324) */
     uint32_t guide;            /* 0: plain; 1: guided + online training; 2: guided, frozen */
-    uint32_t train_steps;      /* Adam steps after each pass (guide = 1); 0 = 4 */
+    uint32_t train_steps;      /* guide = 1: after each pass, one epoch over its radiance samples
+                                * in this many Adam minibatch steps; 0 = 4 */
     uint32_t guide_warmup;     /* passes traced before the proposal is mixed in (guide = 1) */
     float guide_frac;          /* probability of sampling the proposal; 0 = 0.5 */
     uint32_t reserved[3];
@@ -207,6 +208,12 @@ typedef struct apg_render_params {
 /* Allocate per-pixel samplers (seeded with pixel index + seed, alpine.cpp:259-269) and clear
  * the accumulation (Alpine::resetAccumulation). */
 int apg_render_reset(apg_context* ctx, uint32_t width, uint32_t height, uint32_t seed);
+/* As apg_render_reset with `slots` samples of every pixel traced side by side in each pass
+ * (width x height x slots paths), which fills the GPU on small frames.  Slot 0 of each pixel
+ * keeps the reference's per-pixel random sequence; the other slots are seeded from a hash.
+ * apg_render's spp must then be a multiple of slots.  No reference counterpart. */
+int apg_render_reset_slots(apg_context* ctx, uint32_t width, uint32_t height, uint32_t seed,
+                           uint32_t slots);
 int apg_render_set_camera(apg_context* ctx, const apg_camera* cam);
 /* Trace params->spp samples per pixel into the accumulation (asynchronous on stream). */
 int apg_render(apg_context* ctx, const apg_render_params* params, void* stream);
@@ -236,6 +243,9 @@ int apg_guide_init(apg_context* ctx, uint32_t seed, float lr);
 int apg_guide_params(apg_context* ctx, float* params, int write);
 /* Forward pass: feat n x 96 (device) -> out n x 48 (device). */
 int apg_guide_forward(apg_context* ctx, const float* feat, uint32_t n, float* out, void* stream);
+/* Forward pass with the encoding fused in: q n x 9 (device; position normalised to the scene
+ * bounds, shading normal, outgoing direction) -> out n x 48 (device). */
+int apg_guide_infer(apg_context* ctx, const float* q, uint32_t n, float* out, void* stream);
 /* One training step on n samples (device): minimise sum_i w_i (-log q(dir_i | feat_i)) /
  * sum_i w_i.  apply = 0 computes the gradient only.  loss (host, optional) receives the
  * weighted mean negative log-likelihood; grad_host (optional) the gradient vector. */

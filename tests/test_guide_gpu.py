@@ -45,6 +45,30 @@ def test_forward_matches_oracle():
     assert err < 2e-2
 
 
+def test_fused_inference_matches_oracle():
+    """Inference with the encoding computed in registers (the renderer's path) vs the oracle."""
+    print("[gpu] guide fused inference", flush=True)
+    acc = _acc()
+    params = acc.guide_params().astype(np.float64)
+    rng = np.random.default_rng(5)
+    m = 1000  # not a multiple of the 16-query tile
+    p01 = rng.random((m, 3))
+    n = rng.standard_normal((m, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    wo = rng.standard_normal((m, 3))
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    q = np.concatenate([p01, n, wo], 1).astype(np.float32)
+    out = acc.guide_infer(_cuda(q)).cpu().numpy()
+    ref, _ = G.forward(params, G.encode(q[:, 0:3].astype(np.float64), q[:, 3:6], q[:, 6:9]), bf=True)
+    err = np.abs(out[:, :40] - ref[:, :40]).max() / np.abs(ref[:, :40]).max()
+    print("fused inference max rel err", err)
+    assert err < 2e-2
+    # same network as the unfused forward pass on the oracle's features
+    feat = G.encode(q[:, 0:3].astype(np.float64), q[:, 3:6], q[:, 6:9]).astype(np.float32)
+    out2 = acc.guide_forward(_cuda(feat)).cpu().numpy()
+    assert np.abs(out - out2)[:, :40].max() / np.abs(out2[:, :40]).max() < 2e-2
+
+
 def test_gradient_matches_oracle():
     print("[gpu] guide gradient", flush=True)
     acc = _acc()

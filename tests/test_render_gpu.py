@@ -118,3 +118,28 @@ def test_postprocess_matches_oracle():
         # powf on the device and in libm may differ in the last bit, which moves a byte only
         # when the scaled value sits at a rounding boundary
         assert diff.max() <= 1 and np.count_nonzero(diff) <= max(3, diff.size // 2000), tm
+
+
+@pytest.mark.gpu
+def test_slots_converge_like_single_slot():
+    """Samples of a pixel traced side by side (slots) estimate the same image, deterministically."""
+    sc = scenes.cornell_scene(64, 64, spp=4)
+    acc, _, ref64, total = gpu_render(sc, spp=512, fp64=True)
+    ref = ref64 / total
+    errs = {}
+    for slots in (1, 8):
+        acc.render_reset(sc.width, sc.height, 7, slots=slots)
+        acc.render(64, sc.max_depth, sc.rr_depth, sc.background)
+        a, _, n = acc.render_read()
+        assert n == 64
+        errs[slots] = scenes.rel_mse(a / n, ref)
+        acc.render_reset(sc.width, sc.height, 7, slots=slots)
+        acc.render(64, sc.max_depth, sc.rr_depth, sc.background)
+        b, _, _ = acc.render_read()
+        np.testing.assert_array_equal(a, b)
+    print("relMSE by slots", errs)
+    assert errs[8] < 1.5 * errs[1] + 1e-4
+    # spp must be a multiple of the slots
+    with pytest.raises(aa.ApgError):
+        acc.render_reset(sc.width, sc.height, 7, slots=4)
+        acc.render(6, sc.max_depth, sc.rr_depth, sc.background)
