@@ -45,8 +45,12 @@ class _Hits(C.Structure):
 
 
 class _BuildParams(C.Structure):
-    _fields_ = [("leaf_threshold", C.c_uint32), ("builder", C.c_uint32),
-                ("reserved", C.c_uint32 * 6)]
+    _fields_ = [("leaf_threshold", C.c_uint32), ("builder", C.c_uint32), ("roots", C.c_uint32),
+                ("flags", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+
+
+ROOTS_SCENE, ROOTS_PER_SHAPE = 0, 1
+BUILD_NO_STATS = 1
 
 
 class _SceneStats(C.Structure):
@@ -77,7 +81,8 @@ class RenderParams(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
                 ("fp64_accum", C.c_uint32), ("background", C.c_float * 3), ("guide", C.c_uint32),
                 ("train_steps", C.c_uint32), ("guide_warmup", C.c_uint32),
-                ("guide_frac", C.c_float), ("reserved", C.c_uint32 * 3)]
+                ("guide_frac", C.c_float), ("graph", C.c_uint32), ("train_samples", C.c_uint32),
+                ("reserved", C.c_uint32 * 1)]
 
 
 MAT_DIELECTRIC, MAT_CONDUCTOR = 0, 1
@@ -116,9 +121,9 @@ ABI_SYMBOLS = [
     "apg_create", "apg_destroy", "apg_last_error", "apg_mesh_append", "apg_mesh_vertices",
     "apg_mesh_update", "apg_mesh_prim_offset", "apg_mesh_transform", "apg_sphere_append", "apg_scene_build",
     "apg_scene_get_stats", "apg_trace_config_default", "apg_set_trace_config", "apg_intersect",
-    "apg_occluded", "apg_intersect_counted", "apg_scene_triangles", "apg_scene_leaf_keys",
+    "apg_occluded", "apg_intersect_roots", "apg_occluded_roots", "apg_intersect_counted", "apg_scene_triangles", "apg_scene_leaf_keys",
     "apg_scene_wide_nodes", "apg_materials_set", "apg_lights_set", "apg_light_sampler_sample",
-    "apg_light_sampler_pdf", "apg_render_reset", "apg_render_reset_slots", "apg_render_set_camera", "apg_render",
+    "apg_light_sampler_pdf", "apg_render_reset", "apg_render_reset_slots", "apg_render_set_camera", "apg_render_set_tiles", "apg_render",
     "apg_render_read", "apg_render_postprocess", "apg_render_counters", "apg_guide_param_count",
     "apg_guide_init",
     "apg_guide_params", "apg_guide_forward", "apg_guide_infer", "apg_guide_train_step",
@@ -161,6 +166,8 @@ def lib():
         "apg_set_trace_config": (i32, [vp, C.POINTER(TraceConfig)]),
         "apg_intersect": (i32, [vp, C.POINTER(_Rays), u32, C.POINTER(_Hits), vp]),
         "apg_occluded": (i32, [vp, C.POINTER(_Rays), u32, vp, vp]),
+        "apg_intersect_roots": (i32, [vp, C.POINTER(_Rays), vp, u32, C.POINTER(_Hits), vp]),
+        "apg_occluded_roots": (i32, [vp, C.POINTER(_Rays), vp, u32, vp, vp]),
         "apg_intersect_counted": (i32, [vp, C.POINTER(_Rays), u32, C.POINTER(_Hits), i32, vp,
                                         C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), vp]),
         "apg_scene_triangles": (i32, [vp, vp, vp, u32]),
@@ -173,6 +180,7 @@ def lib():
         "apg_render_reset": (i32, [vp, u32, u32, u32]),
         "apg_render_reset_slots": (i32, [vp, u32, u32, u32, u32]),
         "apg_render_set_camera": (i32, [vp, C.POINTER(Camera)]),
+        "apg_render_set_tiles": (i32, [vp, vp, vp, u32, u32, u32]),
         "apg_render": (i32, [vp, C.POINTER(RenderParams), vp]),
         "apg_render_read": (i32, [vp, vp, vp, C.POINTER(u32)]),
         "apg_render_postprocess": (i32, [vp, i32, vp, vp]),
@@ -283,8 +291,13 @@ class GpuAccelerator:
         self._rc(self._lib.apg_mesh_prim_offset(self._h, shape_id, C.byref(o)))
         return o.value
 
-    def update_scene(self, leaf_threshold: int = 4, builder: int = BUILDER_GPU):
-        p = _BuildParams(leaf_threshold, builder)
+    def update_scene(self, leaf_threshold: int = 4, builder: int = BUILDER_GPU,
+                     per_shape: bool = False, stats: bool = True):
+        """Rebuild (Accelerator::updateScene).  per_shape: an independent tree per shape
+        (GPU builder), wide node s the root of shape s; stats=False skips the host-side
+        tree statistics (per-frame rebuilds)."""
+        p = _BuildParams(leaf_threshold, builder, ROOTS_PER_SHAPE if per_shape else ROOTS_SCENE,
+                         0 if stats else BUILD_NO_STATS)
         self._rc(self._lib.apg_scene_build(self._h, C.byref(p)))
 
     def stats(self) -> dict:
@@ -334,6 +347,31 @@ class GpuAccelerator:
         hits = _Hits(*(t.data_ptr() for t in out))
         self._rc(self._lib.apg_intersect(self._h, C.byref(rays), n, C.byref(hits),
                                          self._stream(stream)))
+        return out
+
+    def intersect_roots(self, org, dir, roots, stream=None):
+        """Closest hit of ray i in the tree of shape roots[i] (per-shape forest)."""
+        import torch
+        _check_cuda(roots, "roots")
+        n = org.shape[1]
+        rays = self._rays(org, dir)
+        out = (torch.empty(n, dtype=torch.int32, device=org.device),
+               torch.empty(n, dtype=torch.float32, device=org.device),
+               torch.empty(n, dtype=torch.float32, device=org.device),
+               torch.empty(n, dtype=torch.float32, device=org.device))
+        hits = _Hits(*(t.data_ptr() for t in out))
+        self._rc(self._lib.apg_intersect_roots(self._h, C.byref(rays), C.c_void_p(roots.data_ptr()), n,
+                                               C.byref(hits), self._stream(stream)))
+        return out
+
+    def intersect_any_roots(self, org, dir, roots, tmax=None, stream=None):
+        import torch
+        _check_cuda(roots, "roots")
+        n = org.shape[1]
+        rays = self._rays(org, dir, tmax)
+        out = torch.empty(n, dtype=torch.uint8, device=org.device)
+        self._rc(self._lib.apg_occluded_roots(self._h, C.byref(rays), C.c_void_p(roots.data_ptr()), n,
+                                              C.c_void_p(out.data_ptr()), self._stream(stream)))
         return out
 
     def intersect_any(self, org, dir, tmax=None, out=None, stream=None):
@@ -412,10 +450,19 @@ class GpuAccelerator:
     def set_camera(self, cam):
         self._rc(self._lib.apg_render_set_camera(self._h, C.byref(cam)))
 
+    def set_tiles(self, cams, tile_width, tile_height, roots=None):
+        """Tiled frame: tile t (row-major) seen through cams[t], traced in tree roots[t]."""
+        arr = (Camera * max(1, len(cams)))(*cams)
+        r = None if roots is None else np.ascontiguousarray(roots, np.uint32)
+        self._rc(self._lib.apg_render_set_tiles(self._h, C.cast(arr, C.c_void_p),
+                                                None if r is None else _ptr(r), len(cams),
+                                                tile_width, tile_height))
+
     def render(self, spp, max_depth, rr_depth=0, background=(0.0, 0.0, 0.0), fp64=False,
-               guide=0, train_steps=4, guide_warmup=1, guide_frac=0.5, stream=None):
+               guide=0, train_steps=4, guide_warmup=1, guide_frac=0.5, graph=False,
+               train_samples=0, stream=None):
         p = RenderParams(spp, max_depth, rr_depth, 1 if fp64 else 0, _f3(background), guide,
-                         train_steps, guide_warmup, guide_frac)
+                         train_steps, guide_warmup, guide_frac, 1 if graph else 0, train_samples)
         self._rc(self._lib.apg_render(self._h, C.byref(p), self._stream(stream)))
 
     def guide_stats(self):

@@ -35,6 +35,7 @@ struct DeviceScene
     uint32_t* tri_ids = nullptr; // tree order -> global triangle index
     uint32_t node_count = 0;
     uint32_t tri_count = 0;
+    uint32_t roots = 1;          // wide nodes 0 .. roots-1 are tree roots
 };
 
 struct RenderState;  // render.hip
@@ -138,7 +139,8 @@ struct apg_context : apg::Context {};
     } while (0)
 
 namespace apg {
+int ensure_trace_capacity(Context* ctx, uint32_t n);
 int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint32_t* n_dev,
                      const apg_hits* hits, bool any, uint8_t* occ, unsigned long long* counts,
-                     hipStream_t stream);
+                     hipStream_t stream, const uint32_t* roots = nullptr);
 }

@@ -15,9 +15,7 @@
 
 namespace apg {
 
-int build_scene_gpu(Context* ctx, const std::vector<TriPrep>& tris, uint32_t leaf_threshold,
-                    BinaryTree& bt, std::vector<ApgWideNode>& nodes, std::vector<float>& tri9,
-                    std::vector<uint32_t>& ids);
+int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool stats, BinaryTree& bt);
 
 Context::~Context()
 {
@@ -228,7 +226,17 @@ int apg_scene_build(apg_context* ctx, const apg_build_params* params)
     auto t0 = std::chrono::steady_clock::now();
     uint32_t leaf_threshold = params && params->leaf_threshold ? params->leaf_threshold : 4u;
     uint32_t builder = params ? params->builder : APG_BUILDER_GPU;
+    const uint32_t roots = params ? params->roots : APG_ROOTS_SCENE;
+    const bool stats = !(params && (params->flags & APG_BUILD_NO_STATS));
     if (leaf_threshold > 255) return ctx->fail(APG_E_INVALID, "leaf_threshold must be <= 255");
+    if (roots != APG_ROOTS_SCENE && roots != APG_ROOTS_PER_SHAPE)
+        return ctx->fail(APG_E_INVALID, "roots must be APG_ROOTS_SCENE or APG_ROOTS_PER_SHAPE");
+    if (roots == APG_ROOTS_PER_SHAPE)
+    {
+        if (builder != APG_BUILDER_GPU) return ctx->fail(APG_E_INVALID, "per-shape trees need the GPU builder");
+        for (const auto& sh : ctx->shapes)
+            if (sh.idx.empty()) return ctx->fail(APG_E_INVALID, "per-shape trees need every shape non-empty");
+    }
 
     // Wait for kernels still reading the old scene.
     if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
@@ -252,7 +260,7 @@ int apg_scene_build(apg_context* ctx, const apg_build_params* params)
     }
     else if (builder == APG_BUILDER_GPU)
     {
-        rc = apg::build_scene_gpu(ctx, tris, leaf_threshold, bt, nodes, tri9, ids);
+        rc = apg::build_scene_gpu(ctx, leaf_threshold, roots == APG_ROOTS_PER_SHAPE, stats, bt);
     }
     else
         return ctx->fail(APG_E_INVALID, "unknown builder");

@@ -147,12 +147,21 @@ __global__ void k_transform(float* xyz, uint32_t n, float m0, float m1, float m2
     xyz[3 * size_t(i) + 2] = m8 * x + m9 * y + m10 * z + m11;
 }
 
-__global__ void k_init_slots(uint32_t n, uint32_t* idx, uint32_t* snode)
+// Slot i holds triangle i, in the root of its shape's tree (shape_first:
+// first triangle of each shape, R + 1 entries) or in the single root (R = 1).
+__global__ void k_init_slots(uint32_t n, uint32_t* idx, uint32_t* snode, const uint32_t* shape_first, uint32_t R)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     idx[i] = i;
-    snode[i] = 0;
+    uint32_t lo = 0, hi = R;  // largest r with shape_first[r] <= i
+    while (hi - lo > 1)
+    {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (shape_first[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    snode[i] = lo;
 }
 
 __global__ void k_init_nodes(uint32_t begin, uint32_t end, uint32_t* nbox, uint32_t* ncbox,
@@ -171,6 +180,22 @@ __global__ void k_init_nodes(uint32_t begin, uint32_t end, uint32_t* nbox, uint3
 }
 
 // Node box and centroid box of every node in [begin, end).
+// Bounds and centroid bounds of every node of the level.  Slots are sorted by
+// node, so each thread walks ITEMS consecutive slots, keeps a running
+// min / max (as order-preserving uints, so -0 < +0 as with the atomics) and
+// flushes it once per run of equal node ids: to the workgroup's copy for the
+// chunk's first node, to the node's global entry otherwise.
+__device__ __forceinline__ void flush_bounds(uint32_t* dst_box, uint32_t* dst_cbox, const uint32_t (&v)[12])
+{
+    for (int a = 0; a < 3; ++a)
+    {
+        atomicMin(&dst_box[a], v[a]);
+        atomicMax(&dst_box[3 + a], v[3 + a]);
+        atomicMin(&dst_cbox[a], v[6 + a]);
+        atomicMax(&dst_cbox[3 + a], v[9 + a]);
+    }
+}
+
 __global__ __launch_bounds__(BLOCK) void k_bounds(BuildState S, uint32_t n, uint32_t begin,
                                                   uint32_t end)
 {
@@ -180,44 +205,66 @@ __global__ __launch_bounds__(BLOCK) void k_bounds(BuildState S, uint32_t n, uint
     const uint32_t home = S.snode[c0];
     if (threadIdx.x < 12) acc[threadIdx.x] = (threadIdx.x % 6) < 3 ? ord_of(FLT_MAX) : ord_of(-FLT_MAX);
     __syncthreads();
-    for (uint32_t k = 0; k < ITEMS; ++k)
+    const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
+    uint32_t v[12], hv[12];
+    for (int a = 0; a < 3; ++a)
     {
-        const uint32_t i = c0 + k * BLOCK + threadIdx.x;
-        if (i >= n) break;
-        const uint32_t nd = S.snode[i];
-        if (nd == RETIRED || nd < begin || nd >= end) continue;
-        const float* b = S.pbox + 6 * size_t(S.idx[i]);
-        float lo[3], hi[3], c[3];
-        for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; c[a] = (lo[a] + hi[a]) * 0.5f; }
-        if (nd == home)
+        hv[a] = lo_ord; hv[3 + a] = hi_ord; hv[6 + a] = lo_ord; hv[9 + a] = hi_ord;
+    }
+    uint32_t cur = RETIRED;
+    const uint32_t first = c0 + threadIdx.x * ITEMS;
+    for (uint32_t k = 0; k <= ITEMS; ++k)
+    {
+        const uint32_t i = first + k;
+        const uint32_t nd = (k < ITEMS && i < n) ? S.snode[i] : RETIRED;
+        const bool live = nd != RETIRED && nd >= begin && nd < end;
+        if (cur != RETIRED && nd != cur)
         {
-            for (int a = 0; a < 3; ++a)
-            {
-                atomicMin(&acc[a], ord_of(lo[a]));
-                atomicMax(&acc[3 + a], ord_of(hi[a]));
-                atomicMin(&acc[6 + a], ord_of(c[a]));
-                atomicMax(&acc[9 + a], ord_of(c[a]));
-            }
+            flush_bounds(&S.nbox[6 * size_t(cur)], &S.ncbox[6 * size_t(cur)], v);
+            cur = RETIRED;
         }
-        else
+        if (!live) continue;
+        uint32_t* r = hv;
+        if (nd != home)
         {
-            for (int a = 0; a < 3; ++a)
+            if (cur == RETIRED)
             {
-                atomicMin(&S.nbox[6 * size_t(nd) + a], ord_of(lo[a]));
-                atomicMax(&S.nbox[6 * size_t(nd) + 3 + a], ord_of(hi[a]));
-                atomicMin(&S.ncbox[6 * size_t(nd) + a], ord_of(c[a]));
-                atomicMax(&S.ncbox[6 * size_t(nd) + 3 + a], ord_of(c[a]));
+                cur = nd;
+                for (int a = 0; a < 3; ++a)
+                {
+                    v[a] = lo_ord; v[3 + a] = hi_ord; v[6 + a] = lo_ord; v[9 + a] = hi_ord;
+                }
             }
+            r = v;
+        }
+        const float* b = S.pbox + 6 * size_t(S.idx[i]);
+        for (int a = 0; a < 3; ++a)
+        {
+            const float lo = b[a], hi = b[3 + a], c = (lo + hi) * 0.5f;
+            r[a] = min(r[a], ord_of(lo));
+            r[3 + a] = max(r[3 + a], ord_of(hi));
+            r[6 + a] = min(r[6 + a], ord_of(c));
+            r[9 + a] = max(r[9 + a], ord_of(c));
         }
     }
+    // the chunk's first node: reduce over the wave, one LDS update per wave
+    for (int off = 32; off > 0; off >>= 1)
+        for (int a = 0; a < 3; ++a)
+        {
+            hv[a] = min(hv[a], uint32_t(__shfl_xor(int(hv[a]), off)));
+            hv[3 + a] = max(hv[3 + a], uint32_t(__shfl_xor(int(hv[3 + a]), off)));
+            hv[6 + a] = min(hv[6 + a], uint32_t(__shfl_xor(int(hv[6 + a]), off)));
+            hv[9 + a] = max(hv[9 + a], uint32_t(__shfl_xor(int(hv[9 + a]), off)));
+        }
+    if ((threadIdx.x & 63u) == 0u) flush_bounds(acc, acc + 6, hv);
     __syncthreads();
     if (home != RETIRED && home >= begin && home < end && threadIdx.x < 12)
     {
-        const uint32_t v = acc[threadIdx.x];
+        const uint32_t val = acc[threadIdx.x];
         const uint32_t j = threadIdx.x;
         uint32_t* dst = j < 6 ? &S.nbox[6 * size_t(home) + j] : &S.ncbox[6 * size_t(home) + (j - 6)];
-        if ((j % 6) < 3) atomicMin(dst, v);
-        else atomicMax(dst, v);
+        if ((j % 6) < 3) atomicMin(dst, val);
+        else atomicMax(dst, val);
     }
 }
 
@@ -839,9 +886,7 @@ inline uint32_t blocks_for(uint32_t n, uint32_t b) { return n ? (n + b - 1) / b 
 
 } // namespace
 
-int build_scene_gpu(Context* ctx, const std::vector<TriPrep>& /*host_prep*/, uint32_t leaf_threshold,
-                    BinaryTree& bt, std::vector<ApgWideNode>&, std::vector<float>&,
-                    std::vector<uint32_t>&)
+int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool stats, BinaryTree& bt)
 {
     const auto t_start = std::chrono::steady_clock::now();
     const uint32_t n = ctx->total_tris;
@@ -862,7 +907,7 @@ int build_scene_gpu(Context* ctx, const std::vector<TriPrep>& /*host_prep*/, uin
     const uint32_t* d_tri = ctx->d_tri;
 
     BuildState S;
-    const size_t ncap = 2 * size_t(n) + 2;
+    const size_t ncap = 2 * size_t(n) + 2 + (per_shape ? ctx->shapes.size() : 0);
     const uint32_t thr = std::max<uint32_t>(1, leaf_threshold);
     const size_t bincap = size_t(n) / (thr + 1) + 2;
     if ((rc = scr.get(ctx, S.pbox, 6 * size_t(n)))) return rc;
@@ -887,16 +932,31 @@ int build_scene_gpu(Context* ctx, const std::vector<TriPrep>& /*host_prep*/, uin
     if ((rc = scr.get(ctx, S.scan_tmp, scan_tmp_words(std::max<uint32_t>(n, 1u) + 1)))) return rc;
 
     k_prep<<<blocks_for(n, 256), 256, 0, s>>>(d_xyz, d_tri, n, S.tri9_src, S.pbox);
-    k_init_slots<<<blocks_for(n, 256), 256, 0, s>>>(n, S.idx, S.snode);
-    // root
-    const uint32_t root_first = 0, root_flags = 0, two = 1;
-    APG_HIP_CHECK(ctx, hipMemcpyAsync(S.nfirst, &root_first, 4, hipMemcpyHostToDevice, s));
-    APG_HIP_CHECK(ctx, hipMemcpyAsync(S.ncount, &n, 4, hipMemcpyHostToDevice, s));
-    APG_HIP_CHECK(ctx, hipMemcpyAsync(S.nflags, &root_flags, 4, hipMemcpyHostToDevice, s));
-    uint32_t host_counters[2] = { two, 0 };
+    // roots: one per shape (forest) or one for the scene
+    const uint32_t R = per_shape ? uint32_t(ctx->shapes.size()) : 1u;
+    std::vector<uint32_t> rfirst(R + 1), rcount(R), rflags(R, 0u);
+    {
+        uint32_t off = 0;
+        for (uint32_t r = 0; r < R; ++r)
+        {
+            const uint32_t c = per_shape ? uint32_t(ctx->shapes[r].idx.size() / 3) : n;
+            rfirst[r] = off;
+            rcount[r] = c;
+            off += c;
+        }
+        rfirst[R] = off;
+    }
+    uint32_t* d_rfirst;
+    if ((rc = scr.get(ctx, d_rfirst, R + 1))) return rc;
+    APG_HIP_CHECK(ctx, hipMemcpyAsync(d_rfirst, rfirst.data(), 4 * (R + 1), hipMemcpyHostToDevice, s));
+    k_init_slots<<<blocks_for(n, 256), 256, 0, s>>>(n, S.idx, S.snode, d_rfirst, R);
+    APG_HIP_CHECK(ctx, hipMemcpyAsync(S.nfirst, rfirst.data(), 4 * R, hipMemcpyHostToDevice, s));
+    APG_HIP_CHECK(ctx, hipMemcpyAsync(S.ncount, rcount.data(), 4 * R, hipMemcpyHostToDevice, s));
+    APG_HIP_CHECK(ctx, hipMemcpyAsync(S.nflags, rflags.data(), 4 * R, hipMemcpyHostToDevice, s));
+    uint32_t host_counters[2] = { R, 0 };
     APG_HIP_CHECK(ctx, hipMemcpyAsync(S.counters, host_counters, 8, hipMemcpyHostToDevice, s));
 
-    uint32_t begin = 0, end = 1, levels = 0;
+    uint32_t begin = 0, end = R, levels = 0;
     const uint32_t nchunks = blocks_for(n, CHUNK);
     while (begin < end)
     {
@@ -939,7 +999,7 @@ int build_scene_gpu(Context* ctx, const std::vector<TriPrep>& /*host_prep*/, uin
         if (mf == 0.0f) mf = 1.0f;
         slack = std::ldexp(mf, -17);
     }
-    const size_t wcap = bin_nodes / 2 + 2;  // each wide node holds >= 2 binary nodes or is the root
+    const size_t wcap = bin_nodes / 2 + 2 + R;  // each wide node holds >= 2 binary nodes or is a root
     WideState W;
     W.nbox = S.nbox; W.nfirst = S.nfirst; W.ncount = S.ncount; W.nleft = S.nleft;
     W.nflags = S.nflags; W.idx = S.idx; W.tri9_src = S.tri9_src; W.slack = slack;
@@ -958,9 +1018,14 @@ int build_scene_gpu(Context* ctx, const std::vector<TriPrep>& /*host_prep*/, uin
     if ((rc = dalloc(ctx, d_tri9, 9 * size_t(n)))) { (void)hipFree(d_nodes); return rc; }
     if ((rc = dalloc(ctx, d_ids, n))) { (void)hipFree(d_nodes); (void)hipFree(d_tri9); return rc; }
     W.nodes = d_nodes; W.tri9 = d_tri9; W.tri_ids = d_ids;
-    const uint32_t zero = 0;
-    APG_HIP_CHECK(ctx, hipMemcpyAsync(W.wbin, &zero, 4, hipMemcpyHostToDevice, s));
-    uint32_t wbegin = 0, wend = 1, tri_cursor = 0;
+    {
+        // wide nodes 0 .. R-1 are the roots' images
+        std::vector<uint32_t> wroots(R);
+        for (uint32_t r = 0; r < R; ++r) wroots[r] = r;
+        APG_HIP_CHECK(ctx, hipMemcpyAsync(W.wbin, wroots.data(), 4 * R, hipMemcpyHostToDevice, s));
+        APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    }
+    uint32_t wbegin = 0, wend = R, tri_cursor = 0;
     while (wbegin < wend)
     {
         const uint32_t width = wend - wbegin;
@@ -983,8 +1048,26 @@ int build_scene_gpu(Context* ctx, const std::vector<TriPrep>& /*host_prep*/, uin
     ctx->scene.tri_ids = d_ids;
     ctx->scene.node_count = wend;
     ctx->scene.tri_count = tri_cursor;
+    ctx->scene.roots = R;
     if (tri_cursor != n) return ctx->fail(APG_E_STATE, "GPU build lost triangles");
     bt.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+
+    // ---- scene bounds (union of the roots) ----
+    {
+        std::vector<uint32_t> rbox(6 * size_t(R));
+        APG_HIP_CHECK(ctx, hipMemcpy(rbox.data(), S.nbox, rbox.size() * 4, hipMemcpyDeviceToHost));
+        for (int a = 0; a < 3; ++a)
+        {
+            bt.scene_lo[a] = FLT_MAX;
+            bt.scene_hi[a] = -FLT_MAX;
+            for (uint32_t r = 0; r < R; ++r)
+            {
+                bt.scene_lo[a] = std::min(bt.scene_lo[a], float_of(rbox[6 * size_t(r) + a]));
+                bt.scene_hi[a] = std::max(bt.scene_hi[a], float_of(rbox[6 * size_t(r) + 3 + a]));
+            }
+        }
+    }
+    if (!stats) return APG_OK;
 
     // ---- binary-tree statistics and leaf keys (host, from the device tree) ----
     std::vector<uint32_t> hbox(6 * size_t(bin_nodes)), hfirst(bin_nodes), hcount(bin_nodes),
@@ -1002,38 +1085,40 @@ int build_scene_gpu(Context* ctx, const std::vector<TriPrep>& /*host_prep*/, uin
         return double((x * y + y * z + z * x) * 2.0f);
     };
     bt.leaf_keys.assign(n, 0);
-    double inner = 0.0, leaf = 0.0;
     uint32_t sah_nodes = 0, sah_leaves = 0, maxd = 0;
+    double sah = 0.0, sah_leaf = 0.0;
     struct Item { uint32_t k, d; };
-    std::vector<Item> stack{ { 0, 0 } };
-    while (!stack.empty())
+    for (uint32_t r = 0; r < R; ++r)
     {
-        Item it = stack.back();
-        stack.pop_back();
-        sah_nodes++;
-        maxd = std::max(maxd, it.d);
-        const double sa = area(it.k);
-        if (hflags[it.k] & NF_SAH_LEAF)
+        // per tree, normalised by its root's area; summed over the trees
+        double inner = 0.0, leaf = 0.0;
+        std::vector<Item> stack{ { r, 0 } };
+        while (!stack.empty())
         {
-            sah_leaves++;
-            leaf += sa * double(hcount[it.k]);
-            uint32_t mn = 0xFFFFFFFFu;
-            for (uint32_t j = 0; j < hcount[it.k]; ++j) mn = std::min(mn, hidx[hfirst[it.k] + j]);
-            for (uint32_t j = 0; j < hcount[it.k]; ++j) bt.leaf_keys[hidx[hfirst[it.k] + j]] = mn;
-            continue;
+            Item it = stack.back();
+            stack.pop_back();
+            sah_nodes++;
+            maxd = std::max(maxd, it.d);
+            const double sa = area(it.k);
+            if (hflags[it.k] & NF_SAH_LEAF)
+            {
+                sah_leaves++;
+                leaf += sa * double(hcount[it.k]);
+                uint32_t mn = 0xFFFFFFFFu;
+                for (uint32_t j = 0; j < hcount[it.k]; ++j) mn = std::min(mn, hidx[hfirst[it.k] + j]);
+                for (uint32_t j = 0; j < hcount[it.k]; ++j) bt.leaf_keys[hidx[hfirst[it.k] + j]] = mn;
+                continue;
+            }
+            inner += sa;
+            stack.push_back({ hleft[it.k] + 1, it.d + 1 });
+            stack.push_back({ hleft[it.k], it.d + 1 });
         }
-        inner += sa;
-        stack.push_back({ hleft[it.k] + 1, it.d + 1 });
-        stack.push_back({ hleft[it.k], it.d + 1 });
+        const double root_sa = area(r);
+        sah += (inner + leaf) / root_sa;
+        sah_leaf += leaf / root_sa;
     }
-    for (int a = 0; a < 3; ++a)
-    {
-        bt.scene_lo[a] = float_of(hbox[a]);
-        bt.scene_hi[a] = float_of(hbox[3 + a]);
-    }
-    const double root_sa = area(0);
-    bt.sah_cost = (inner + leaf) / root_sa;
-    bt.sah_cost_leaf = leaf / root_sa;
+    bt.sah_cost = sah;
+    bt.sah_cost_leaf = sah_leaf;
     bt.sah_nodes = sah_nodes;
     bt.sah_leaves = sah_leaves;
     bt.max_depth = maxd;

@@ -55,6 +55,7 @@ struct GuideNet
     float* m = nullptr;
     float* v = nullptr;
     float* grad = nullptr;
+    float* part = nullptr;     // WG_MAX x GUIDE_PARAMS partial gradients
     float* scratch = nullptr;  // [0] sum of weights, [1] loss sum
     uint32_t step = 0;
     float lr = 1e-3f;
@@ -334,22 +335,31 @@ __global__ void k_guide_wsum(const float* weight, uint32_t M, float* scratch)
     if ((threadIdx.x & 63) == 0) atomicAdd(&scratch[0], s);
 }
 
+// wsum: the minibatch's weight sum (device); loss_sum, when set, accumulates the
+// weighted mean negative log-likelihood.
 __global__ void k_guide_head(const float* out, const float* dir, const float* weight, uint32_t M,
-                             float* scratch, uint16_t* dz)
+                             const float* wsum_p, float* loss_sum, uint16_t* dz)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     float nll = 0.0f;
     if (i < M)
     {
-        const float wsum = scratch[0];
+        const float wsum = *wsum_p;
         const float scale = wsum > 0.0f ? weight[i] / wsum : 0.0f;
         float g[GUIDE_OUT];
         nll = guide_nll_grad(out + size_t(i) * GUIDE_OUT, load3c(dir, i), scale, g) * scale;
         uint16_t* d = dz + size_t(i) * (GUIDE_OUT + 4 * GUIDE_HID);
         for (int k = 0; k < GUIDE_OUT; ++k) d[k] = f2bf(g[k]);
     }
+    if (!loss_sum) return;
     for (int off = 32; off > 0; off >>= 1) nll += __shfl_down(nll, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&scratch[1], nll);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss_sum, nll);
+}
+
+__global__ void k_guide_zero(float* p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
 }
 
 // Backward through the layers for 16-sample tiles: dA_{l-1} = dZ_l . W_l, dZ_{l-1} = dA (A > 0).
@@ -419,86 +429,123 @@ __global__ __launch_bounds__(BLOCK) void k_guide_bwd_data(const uint16_t* w16, c
 
 // dW_l[n][k] = sum_s dZ_l[s][n] A_{l-1}[s][k], db_l[n] = sum_s dZ_l[s][n] over a chunk of
 // samples per workgroup (blockIdx.y = layer); added to grad with one atomic per value.
-constexpr int WG_CHUNK = 256;
+// Weight and bias gradients of one layer over a sample range, written as
+// this workgroup's partial sums (no atomics; k_guide_reduce adds the partials
+// in a fixed order).  64 samples are staged per round; each wave owns up to
+// six 16 x 16 output tiles and runs four MFMA k-steps (16 samples each) per
+// tile and round.
+constexpr int WG_MAX = 32;  // partial-sum slots
+constexpr int WG_STEP = 64;
+constexpr int SIN_LD = GUIDE_IN + 8, SDZ_LD = GUIDE_HID + 8;
+
+__device__ __forceinline__ s4v gather4_ld(const uint16_t* base, int ld)
+{
+    s4v r;
+    r[0] = short(base[0]);
+    r[1] = short(base[ld]);
+    r[2] = short(base[2 * ld]);
+    r[3] = short(base[3 * ld]);
+    return r;
+}
 
 __global__ __launch_bounds__(BLOCK) void k_guide_bwd_weight(const float* feat, const uint16_t* acts,
-                                                            const uint16_t* dz, uint32_t M, float* grad)
+                                                            const uint16_t* dz, uint32_t M, uint32_t chunk,
+                                                            float* part)
 {
     const int layer = blockIdx.y;
     const int K = guide_k(layer), N = guide_n(layer);
-    __shared__ __align__(16) uint16_t s_in[16][GUIDE_IN];
-    __shared__ __align__(16) uint16_t s_dz[16][GUIDE_HID];
+    __shared__ __align__(16) uint16_t s_in[WG_STEP * SIN_LD];
+    __shared__ __align__(16) uint16_t s_dz[WG_STEP * SDZ_LD];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = lane & 15, kg = lane >> 4;
     constexpr int DZS = GUIDE_OUT + 4 * GUIDE_HID;
-    const int ntile_n = N / 16, ntile_k = K / 16, ntiles = ntile_n * ntile_k;
+    const int ntile_k = K / 16, ntiles = (N / 16) * ntile_k;
     f4v acc[6];
     for (int t = 0; t < 6; ++t) acc[t] = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
     float bacc = 0.0f;
-    const uint32_t s0 = blockIdx.x * WG_CHUNK;
-    const uint32_t s1 = min(M, s0 + WG_CHUNK);
+    const uint32_t s0 = blockIdx.x * chunk;
+    const uint32_t s1 = min(M, s0 + chunk);
     const int dz_off = layer == 4 ? 0 : GUIDE_OUT + layer * GUIDE_HID;
-    for (uint32_t sb = s0; sb < s1; sb += 16)
+    for (uint32_t sb = s0; sb < s1; sb += WG_STEP)
     {
         __syncthreads();
-        for (int i = threadIdx.x; i < 16 * K; i += BLOCK)
+        for (int i = threadIdx.x; i < WG_STEP * K; i += BLOCK)
         {
-            const int rr = i / K, c = i % K;
-            const uint32_t s = sb + rr;
+            const int rr = i / K, c = i - rr * K;
+            const uint32_t smp = sb + rr;
             uint16_t v = 0;
-            if (s < s1)
-                v = layer == 0 ? f2bf(feat[size_t(s) * GUIDE_IN + c])
-                               : acts[size_t(s) * (4 * GUIDE_HID) + (layer - 1) * GUIDE_HID + c];
-            s_in[rr][c] = v;
+            if (smp < s1)
+                v = layer == 0 ? f2bf(feat[size_t(smp) * GUIDE_IN + c])
+                               : acts[size_t(smp) * (4 * GUIDE_HID) + (layer - 1) * GUIDE_HID + c];
+            s_in[rr * SIN_LD + c] = v;
         }
-        for (int i = threadIdx.x; i < 16 * N; i += BLOCK)
+        for (int i = threadIdx.x; i < WG_STEP * N; i += BLOCK)
         {
-            const int rr = i / N, c = i % N;
-            const uint32_t s = sb + rr;
-            s_dz[rr][c] = s < s1 ? dz[size_t(s) * DZS + dz_off + c] : 0;
+            const int rr = i / N, c = i - rr * N;
+            const uint32_t smp = sb + rr;
+            s_dz[rr * SDZ_LD + c] = smp < s1 ? dz[size_t(smp) * DZS + dz_off + c] : 0;
         }
         __syncthreads();
+#pragma unroll
         for (int t = 0; t < 6; ++t)
         {
             const int tile = wave + WAVES * t;
             if (tile >= ntiles) break;
-            const int tn = tile / ntile_k, tk = tile % ntile_k;
+            const int tn = tile / ntile_k, tk = tile - tn * ntile_k;
             // A[m = n unit][kdim = sample] = dZ[s][n]; B[kdim = sample][col = k unit] = in[s][k]
-            const s4v a = gather4(&s_dz[4 * kg][tn * 16 + r], GUIDE_HID);
-            const s4v b = gather4(&s_in[4 * kg][tk * 16 + r], GUIDE_IN);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+#pragma unroll
+            for (int ks = 0; ks < WG_STEP / 16; ++ks)
+            {
+                const s4v a = gather4_ld(&s_dz[(16 * ks + 4 * kg) * SDZ_LD + tn * 16 + r], SDZ_LD);
+                const s4v b = gather4_ld(&s_in[(16 * ks + 4 * kg) * SIN_LD + tk * 16 + r], SIN_LD);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+            }
         }
         if (threadIdx.x < N)
-            for (int rr = 0; rr < 16; ++rr) bacc += bf2f(s_dz[rr][threadIdx.x]);
+            for (int rr = 0; rr < WG_STEP; ++rr) bacc += bf2f(s_dz[rr * SDZ_LD + threadIdx.x]);
     }
+    float* out = part + size_t(blockIdx.x) * GUIDE_PARAMS;
     for (int t = 0; t < 6; ++t)
     {
         const int tile = wave + WAVES * t;
         if (tile >= ntiles) break;
-        const int tn = tile / ntile_k, tk = tile % ntile_k;
+        const int tn = tile / ntile_k, tk = tile - tn * ntile_k;
         for (int i = 0; i < 4; ++i)
         {
             const int n = tn * 16 + 4 * kg + i;
             const int k = tk * 16 + r;
-            atomicAdd(&grad[guide_w_off(layer) + n * K + k], acc[t][i]);
+            out[guide_w_off(layer) + n * K + k] = acc[t][i];
         }
     }
-    if (threadIdx.x < N) atomicAdd(&grad[guide_b_off(layer) + threadIdx.x], bacc);
+    if (threadIdx.x < N) out[guide_b_off(layer) + threadIdx.x] = bacc;
 }
 
-__global__ void k_guide_adam(float* w32, uint16_t* w16, float* m, float* v, const float* grad, uint32_t n,
-                             float lr, float b1, float b2, float eps, float c1, float c2)
+struct AdamArgs
+{
+    float* w32;
+    uint16_t* w16;
+    float* m;
+    float* v;
+    float lr, b1, b2, eps, c1, c2;
+};
+
+// Sum of the partial gradients in slot order; then, when apply is set, the
+// Adam step with FP32 master weights and the BF16 copy the kernels read.
+__global__ void k_guide_reduce(const float* part, uint32_t G, float* grad, int apply, AdamArgs A)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float g = grad[i];
-    const float mm = b1 * m[i] + (1.0f - b1) * g;
-    const float vv = b2 * v[i] + (1.0f - b2) * g * g;
-    m[i] = mm;
-    v[i] = vv;
-    const float p = w32[i] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
-    w32[i] = p;
-    if (i < uint32_t(WSZ)) w16[i] = f2bf(p);
+    if (i >= uint32_t(GUIDE_PARAMS)) return;
+    float g = 0.0f;
+    for (uint32_t k = 0; k < G; ++k) g += part[size_t(k) * GUIDE_PARAMS + i];
+    grad[i] = g;
+    if (!apply) return;
+    const float mm = A.b1 * A.m[i] + (1.0f - A.b1) * g;
+    const float vv = A.b2 * A.v[i] + (1.0f - A.b2) * g * g;
+    A.m[i] = mm;
+    A.v[i] = vv;
+    const float p = A.w32[i] - A.lr * (mm / A.c1) / (sqrtf(vv / A.c2) + A.eps);
+    A.w32[i] = p;
+    if (i < uint32_t(WSZ)) A.w16[i] = f2bf(p);
 }
 
 __global__ void k_guide_sync16(const float* w32, uint16_t* w16)
@@ -548,6 +595,7 @@ int guide_ensure(Context* ctx, uint32_t cap)
         APG_HIP_CHECK(ctx, hipMalloc(&g->m, GUIDE_PARAMS * 4));
         APG_HIP_CHECK(ctx, hipMalloc(&g->v, GUIDE_PARAMS * 4));
         APG_HIP_CHECK(ctx, hipMalloc(&g->grad, GUIDE_PARAMS * 4));
+        APG_HIP_CHECK(ctx, hipMalloc(&g->part, size_t(WG_MAX) * GUIDE_PARAMS * 4));
         APG_HIP_CHECK(ctx, hipMalloc(&g->scratch, 64));
         std::vector<float> p;
         init_params(p, 1);
@@ -575,7 +623,7 @@ int guide_ensure(Context* ctx, uint32_t cap)
 void guide_free(GuideNet* g)
 {
     if (!g) return;
-    for (void* p : { (void*)g->w32, (void*)g->w16, (void*)g->m, (void*)g->v, (void*)g->grad, (void*)g->scratch,
+    for (void* p : { (void*)g->w32, (void*)g->w16, (void*)g->m, (void*)g->v, (void*)g->grad, (void*)g->part, (void*)g->scratch,
                      (void*)g->acts, (void*)g->out, (void*)g->dz })
         if (p) (void)hipFree(p);
     delete g;
@@ -606,27 +654,34 @@ int guide_infer(Context* ctx, const float* q9, uint32_t M, const uint32_t* M_dev
 }
 
 int guide_train(Context* ctx, const float* feat, const float* dir, const float* weight, uint32_t M,
-                bool adam, float* loss, hipStream_t s)
+                bool adam, float* loss, hipStream_t s, const float* wsum_dev)
 {
     int rc = guide_ensure(ctx, M);
     if (rc) return rc;
     GuideNet* g = guide_of(ctx);
-    APG_HIP_CHECK(ctx, hipMemsetAsync(g->grad, 0, GUIDE_PARAMS * 4, s));
-    APG_HIP_CHECK(ctx, hipMemsetAsync(g->scratch, 0, 64, s));
+    // scratch[0]: weight sum when the caller has none, scratch[1]: loss sum
+    if (loss || !wsum_dev) k_guide_zero<<<1, 64, 0, s>>>(g->scratch, 2);
+    if (!wsum_dev)
+    {
+        k_guide_wsum<<<std::min<uint32_t>(nb(M, 256), 1024), 256, 0, s>>>(weight, M, g->scratch);
+        wsum_dev = g->scratch;
+    }
     if ((rc = guide_forward(ctx, feat, M, nullptr, g->out, g->acts, s))) return rc;
-    k_guide_wsum<<<std::min<uint32_t>(nb(M, 256), 1024), 256, 0, s>>>(weight, M, g->scratch);
-    k_guide_head<<<nb(M, 256), 256, 0, s>>>(g->out, dir, weight, M, g->scratch, g->dz);
+    k_guide_head<<<nb(M, 256), 256, 0, s>>>(g->out, dir, weight, M, wsum_dev, loss ? g->scratch + 1 : nullptr,
+                                            g->dz);
     const uint32_t grid = std::min<uint32_t>(nb(M, 16 * WAVES), uint32_t(ctx->cu_count) * 4);
     k_guide_bwd_data<<<grid, BLOCK, 0, s>>>(g->w16, g->acts, g->dz, M);
-    k_guide_bwd_weight<<<dim3(nb(M, WG_CHUNK), GUIDE_LAYERS), BLOCK, 0, s>>>(feat, g->acts, g->dz, M, g->grad);
+    const uint32_t G = std::max<uint32_t>(1u, std::min<uint32_t>(WG_MAX, nb(M, WG_STEP)));
+    const uint32_t chunk = nb(nb(M, G), WG_STEP) * WG_STEP;
+    k_guide_bwd_weight<<<dim3(G, GUIDE_LAYERS), BLOCK, 0, s>>>(feat, g->acts, g->dz, M, chunk, g->part);
+    AdamArgs A{ g->w32, g->w16, g->m, g->v, g->lr, 0.9f, 0.999f, 1e-8f, 1.0f, 1.0f };
     if (adam)
     {
         g->step++;
-        const float b1 = 0.9f, b2 = 0.999f;
-        const float c1 = 1.0f - std::pow(b1, float(g->step)), c2 = 1.0f - std::pow(b2, float(g->step));
-        k_guide_adam<<<nb(GUIDE_PARAMS, 256), 256, 0, s>>>(g->w32, g->w16, g->m, g->v, g->grad, GUIDE_PARAMS,
-                                                          g->lr, b1, b2, 1e-8f, c1, c2);
+        A.c1 = 1.0f - std::pow(A.b1, float(g->step));
+        A.c2 = 1.0f - std::pow(A.b2, float(g->step));
     }
+    k_guide_reduce<<<nb(GUIDE_PARAMS, 256), 256, 0, s>>>(g->part, G, g->grad, adam ? 1 : 0, A);
     APG_HIP_CHECK(ctx, hipGetLastError());
     if (loss)
     {
@@ -701,7 +756,7 @@ int apg_guide_train_step(apg_context* ctx, const float* feat, const float* dir, 
     if (!ctx || (n && (!feat || !dir || !weight))) return APG_E_INVALID;
     if (n == 0) return APG_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    int rc = guide_train(ctx, feat, dir, weight, n, apply != 0, loss, s);
+    int rc = guide_train(ctx, feat, dir, weight, n, apply != 0, loss, s, nullptr);
     if (rc) return rc;
     if (grad_host)
     {

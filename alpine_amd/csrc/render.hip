@@ -30,7 +30,7 @@ int guide_ensure(Context* ctx, uint32_t cap);
 int guide_forward(Context* ctx, const float* feat, uint32_t M, const uint32_t* M_dev, float* out,
                   uint16_t* acts, hipStream_t s);
 int guide_train(Context* ctx, const float* feat, const float* dir, const float* weight, uint32_t M,
-                bool adam, float* loss, hipStream_t s);
+                bool adam, float* loss, hipStream_t s, const float* wsum_dev = nullptr);
 int guide_infer(Context* ctx, const float* q9, uint32_t M, const uint32_t* M_dev, float* out, hipStream_t s);
 
 struct Queue
@@ -39,6 +39,16 @@ struct Queue
     float* d[3];
     float* tmax;
     uint32_t* path;
+    uint32_t* root;   // tree of each ray (per-shape forests)
+};
+
+// Tiled frames: tile t (row-major over tiles_x columns) has its own camera
+// and names the tree its rays are traced in.
+struct Tiles
+{
+    const DevCamera* cams;
+    const uint32_t* roots;
+    uint32_t tw, th, tiles_x;
 };
 
 struct RenderState
@@ -61,6 +71,14 @@ struct RenderState
     int32_t* leaf_of = nullptr;
     DevCamera cam{};
     bool cam_set = false;
+    DevCamera* tile_cams = nullptr;
+    uint32_t* tile_roots = nullptr;
+    uint32_t ntiles = 0, tile_w = 0, tile_h = 0;
+    // graph replay of a pass
+    hipGraphExec_t gexec = nullptr;
+    hipStream_t gstream = nullptr;
+    hipEvent_t gev = nullptr;
+    std::vector<char> gkey;
     // pixels and paths: P = NPIX x S paths per pass, S samples of each pixel in flight
     uint32_t W = 0, H = 0, P = 0, NPIX = 0, S = 1;
     uint32_t* rng = nullptr;      // per path slot
@@ -93,6 +111,8 @@ struct RenderState
     float* tw = nullptr;      // tcap
     float* tdir_p = nullptr;  // tcap x 3, minibatch order
     float* tw_p = nullptr;    // tcap, minibatch order
+    float* twsum = nullptr;   // weight sum of each minibatch
+    float train_keep = 1.0f;  // fraction of candidate samples kept in the last pass
     uint32_t tcap = 0;
     uint32_t* tcount = nullptr;
     std::vector<void*> guide_allocs;
@@ -108,6 +128,7 @@ constexpr uint32_t ST_ENDED = 1u << 31;
 constexpr uint32_t ST_NEE = 1u << 30;
 constexpr uint32_t ST_DEPTH = 0xFFFFu;
 constexpr int VREC = 20;  // p01 3, n 3, wo 3, dir 3, w 3, E 3, has_dir, pad
+constexpr uint32_t TRAIN_MAX_STEPS = 1024;  // minibatches per pass
 
 struct LsView
 {
@@ -251,8 +272,8 @@ __global__ void k_set_counts(uint32_t* counts, uint32_t a)
 }
 
 // Camera rays with the per-pixel jitter (alpine.cpp:305-312).
-__global__ void k_generate(DevCamera cam, uint32_t W, uint32_t H, uint32_t P, uint32_t* rng, float* T, float* L,
-                           uint32_t* state, Queue q, unsigned long long* stats)
+__global__ void k_generate(DevCamera cam, Tiles tl, uint32_t W, uint32_t H, uint32_t P, uint32_t* rng, float* T,
+                           float* L, uint32_t* state, Queue q, unsigned long long* stats)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
@@ -262,7 +283,16 @@ __global__ void k_generate(DevCamera cam, uint32_t W, uint32_t H, uint32_t P, ui
     const uint32_t pix = p % (W * H);
     const uint32_t x = pix % W, y = pix / W;
     f3 o, d;
-    camera_ray(cam, (float(x) + j.x) / float(W), (float(y) + j.y) / float(H), o, d);
+    uint32_t root = 0;
+    if (tl.cams)
+    {
+        const uint32_t t = (y / tl.th) * tl.tiles_x + x / tl.tw;
+        camera_ray(tl.cams[t], (float(x % tl.tw) + j.x) / float(tl.tw), (float(y % tl.th) + j.y) / float(tl.th),
+                   o, d);
+        root = tl.roots ? tl.roots[t] : 0u;
+    }
+    else
+        camera_ray(cam, (float(x) + j.x) / float(W), (float(y) + j.y) / float(H), o, d);
     for (int k = 0; k < 4; ++k) rng[4 * size_t(p) + k] = s[k];
     store3(T, p, mk(1.0f, 1.0f, 1.0f));
     store3(L, p, mk(0.0f, 0.0f, 0.0f));
@@ -270,6 +300,7 @@ __global__ void k_generate(DevCamera cam, uint32_t W, uint32_t H, uint32_t P, ui
     q.o[0][p] = o.x; q.o[1][p] = o.y; q.o[2][p] = o.z;
     q.d[0][p] = d.x; q.d[1][p] = d.y; q.d[2][p] = d.z;
     q.path[p] = p;
+    q.root[p] = root;
     if (p == 0) atomicAdd(&stats[0], (unsigned long long)P);
 }
 
@@ -285,10 +316,11 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A)
         bool emit_a = false, emit_b = false, emit_c = false;
         f3 sa_o{}, sa_d{}, sb_d{}, c_o{}, c_d{};
         float sa_t = 0.0f, sb_t = 0.0f;
-        uint32_t p = 0;
+        uint32_t p = 0, rt = 0;
         if (valid)
         {
             p = A.in.path[q];
+            rt = A.in.root[q];
             uint32_t s[4];
             for (int k = 0; k < 4; ++k) s[k] = A.rng[4 * size_t(p) + k];
             f3 T = load3(A.T, p), L = load3(A.L, p);
@@ -466,6 +498,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A)
             A.shadow.d[0][ia] = sa_d.x; A.shadow.d[1][ia] = sa_d.y; A.shadow.d[2][ia] = sa_d.z;
             A.shadow.tmax[ia] = sa_t;
             A.shadow.path[ia] = 2u * p;
+            A.shadow.root[ia] = rt;
         }
         const uint32_t ib = wave_append(&A.counts[2], emit_b);
         if (emit_b)
@@ -474,6 +507,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A)
             A.shadow.d[0][ib] = sb_d.x; A.shadow.d[1][ib] = sb_d.y; A.shadow.d[2][ib] = sb_d.z;
             A.shadow.tmax[ib] = sb_t;
             A.shadow.path[ib] = 2u * p + 1u;
+            A.shadow.root[ib] = rt;
         }
         const uint32_t ic = wave_append(&A.counts[1], emit_c);
         if (emit_c)
@@ -481,6 +515,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A)
             A.out.o[0][ic] = c_o.x; A.out.o[1][ic] = c_o.y; A.out.o[2][ic] = c_o.z;
             A.out.d[0][ic] = c_d.x; A.out.d[1][ic] = c_d.y; A.out.d[2][ic] = c_d.z;
             A.out.path[ic] = p;
+            A.out.root[ic] = rt;
         }
     }
 }
@@ -502,7 +537,18 @@ struct TrainOut
     float* tw;
     uint32_t tcap;
     uint32_t* tcount;
+    float keep;      // probability of keeping a training sample (spreads tcap over the frame)
+    uint32_t seed;   // per pass
 };
+
+__device__ __forceinline__ float hash01(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u ^ c * 0xC2B2AE3Du;
+    h ^= h >> 16; h *= 0x7feb352du;
+    h ^= h >> 15; h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return float(h >> 8) * (1.0f / 16777216.0f);
+}
 
 __device__ __forceinline__ float luminance(f3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 
@@ -548,7 +594,7 @@ __global__ void k_resolve(const uint32_t* counts, const uint32_t* qpath, uint32_
                     const f3 contrib = w * lnext;
                     lo = e + contrib;
                     const float wt = luminance(contrib);
-                    if (wt > 0.0f)
+                    if (wt > 0.0f && (G.keep >= 1.0f || hash01(p, uint32_t(j), G.seed) < G.keep))
                     {
                         const uint32_t t = atomicAdd(G.tcount, 1u);
                         if (t < G.tcap)
@@ -609,7 +655,7 @@ __global__ void k_postprocess(const float* accum, uint32_t n3, float spp, uint32
 // Training inputs in minibatch order: sample i goes to minibatch i % S, so
 // each of the S minibatches spreads over the whole image.
 __global__ void k_train_encode(const float* tin, const float* tdir, const float* tw, uint32_t n, uint32_t S,
-                               uint32_t B, float* feat, float* dir_p, float* w_p)
+                               uint32_t B, float* feat, float* dir_p, float* w_p, float* wsum)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -617,6 +663,13 @@ __global__ void k_train_encode(const float* tin, const float* tdir, const float*
     guide_encode(load3(tin, 3 * i), load3(tin, 3 * i + 1), load3(tin, 3 * i + 2), feat + size_t(o) * GUIDE_IN);
     for (int k = 0; k < 3; ++k) dir_p[3 * size_t(o) + k] = tdir[3 * size_t(i) + k];
     w_p[o] = tw[i];
+    atomicAdd(&wsum[i % S], tw[i]);
+}
+
+__global__ void k_zero_f32(float* p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
 }
 
 __global__ void k_tri_ng(const float* tri9, const uint32_t* ids, uint32_t n, float4* ng)
@@ -706,6 +759,11 @@ void render_state_free(RenderState* R)
     dfree(R->alias_a);
     dfree(R->lnodes);
     dfree(R->leaf_of);
+    dfree(R->tile_cams);
+    dfree(R->tile_roots);
+    if (R->gexec) (void)hipGraphExecDestroy(R->gexec);
+    if (R->gev) (void)hipEventDestroy(R->gev);
+    if (R->gstream) (void)hipStreamDestroy(R->gstream);
     free_pixels(R);
     for (void* p : R->guide_allocs) (void)hipFree(p);
     delete R;
@@ -845,6 +903,32 @@ int apg_render_set_camera(apg_context* ctx, const apg_camera* cam)
     return APG_OK;
 }
 
+int apg_render_set_tiles(apg_context* ctx, const apg_camera* cams, const uint32_t* roots, uint32_t ntiles,
+                         uint32_t tile_width, uint32_t tile_height)
+{
+    if (!ctx || (ntiles && (!cams || tile_width == 0 || tile_height == 0))) return APG_E_INVALID;
+    RenderState* R = state_of(ctx);
+    if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+    dfree(R->tile_cams);
+    dfree(R->tile_roots);
+    R->ntiles = 0;
+    if (ntiles == 0) return APG_OK;
+    std::vector<DevCamera> dc(ntiles);
+    for (uint32_t t = 0; t < ntiles; ++t) camera_setup(cams[t], dc[t]);
+    int rc;
+    if ((rc = dmalloc(ctx, R->tile_cams, ntiles))) return rc;
+    APG_HIP_CHECK(ctx, hipMemcpy(R->tile_cams, dc.data(), ntiles * sizeof(DevCamera), hipMemcpyHostToDevice));
+    if (roots)
+    {
+        if ((rc = dmalloc(ctx, R->tile_roots, ntiles))) return rc;
+        APG_HIP_CHECK(ctx, hipMemcpy(R->tile_roots, roots, ntiles * 4, hipMemcpyHostToDevice));
+    }
+    R->ntiles = ntiles;
+    R->tile_w = tile_width;
+    R->tile_h = tile_height;
+    return APG_OK;
+}
+
 int apg_render_reset_slots(apg_context* ctx, uint32_t width, uint32_t height, uint32_t seed, uint32_t slots)
 {
     if (!ctx || width == 0 || height == 0 || slots == 0) return APG_E_INVALID;
@@ -877,7 +961,7 @@ int apg_render_reset_slots(apg_context* ctx, uint32_t width, uint32_t height, ui
             const size_t cap = q == &R->qs ? 2 * size_t(P) : P;
             for (int a = 0; a < 3; ++a)
                 if ((rc = get(q->o[a], cap)) || (rc = get(q->d[a], cap))) return rc;
-            if ((rc = get(q->path, cap))) return rc;
+            if ((rc = get(q->path, cap)) || (rc = get(q->root, cap))) return rc;
             if (q == &R->qs) { if ((rc = get(q->tmax, cap))) return rc; }
             else q->tmax = nullptr;
         }
@@ -908,10 +992,10 @@ int apg_render_guide_stats(const apg_context* cctx, uint32_t* samples, float* lo
     return APG_OK;
 }
 
-static int ensure_guide_buffers(apg_context* ctx, RenderState* R, uint32_t depth)
+static int ensure_guide_buffers(apg_context* ctx, RenderState* R, uint32_t depth, uint32_t want)
 {
     const uint32_t P = R->P;
-    const uint32_t tcap = std::min<uint32_t>(P * 2, 1u << 18);
+    const uint32_t tcap = std::min<uint32_t>(P * 2, want ? want : (1u << 18));
     if (R->gfeat && R->vrec_depth >= depth && R->tcap == tcap) return APG_OK;
     if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
     for (void* p : R->guide_allocs) (void)hipFree(p);
@@ -925,7 +1009,7 @@ static int ensure_guide_buffers(apg_context* ctx, RenderState* R, uint32_t depth
     if ((rc = get(R->gfeat, size_t(P) * 9)) || (rc = get(R->gout, size_t(P) * GUIDE_OUT)) ||
         (rc = get(R->vrec, size_t(P) * depth * VREC)) || (rc = get(R->tin, size_t(tcap) * 9)) ||
         (rc = get(R->tfeat, size_t(tcap) * GUIDE_IN)) || (rc = get(R->tdir, size_t(tcap) * 3)) ||
-        (rc = get(R->tw, tcap)) || (rc = get(R->tdir_p, size_t(tcap) * 3)) || (rc = get(R->tw_p, tcap)) ||
+        (rc = get(R->tw, tcap)) || (rc = get(R->tdir_p, size_t(tcap) * 3)) || (rc = get(R->tw_p, tcap)) || (rc = get(R->twsum, TRAIN_MAX_STEPS)) ||
         (rc = get(R->tcount, 4)))
         return rc;
     R->vrec_depth = depth;
@@ -939,7 +1023,7 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
     RenderState* R = state_of(ctx);
     if (!ctx->built) return ctx->fail(APG_E_STATE, "apg_scene_build has not been called");
     if (!R->P) return ctx->fail(APG_E_STATE, "apg_render_reset has not been called");
-    if (!R->cam_set) return ctx->fail(APG_E_STATE, "apg_render_set_camera has not been called");
+    if (!R->cam_set && !R->ntiles) return ctx->fail(APG_E_STATE, "apg_render_set_camera has not been called");
     if (!R->mats || R->tri_mat_n != ctx->total_tris)
         return ctx->fail(APG_E_STATE, "apg_materials_set must cover every triangle");
     if (prm->max_depth == 0 || prm->max_depth > ST_DEPTH) return ctx->fail(APG_E_INVALID, "max_depth out of range");
@@ -947,6 +1031,7 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
     const uint32_t P = R->P;
 
     ShadeArgs S;
+    std::memset(&S, 0, sizeof(S));  // padding included: the graph key compares bytes
     S.mats = R->mats;
     S.tri_mat = R->tri_mat;
     S.tri_ng = R->tri_ng;
@@ -969,7 +1054,7 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
     if (guide > 2) return ctx->fail(APG_E_INVALID, "guide must be 0, 1 or 2");
     if (guide)
     {
-        int rc = ensure_guide_buffers(ctx, R, prm->max_depth);
+        int rc = ensure_guide_buffers(ctx, R, prm->max_depth, prm->train_samples);
         if (rc) return rc;
     }
     S.gout = R->gout;
@@ -991,69 +1076,149 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
     if (!ctx->last_launch) APG_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->last_launch, hipEventDisableTiming));
     APG_HIP_CHECK(ctx, hipStreamWaitEvent(stream, ctx->last_launch, 0));
     if (prm->spp % R->S) return ctx->fail(APG_E_INVALID, "spp must be a multiple of the slots per pixel");
-    for (uint32_t sidx = 0; sidx < prm->spp / R->S; ++sidx)
+    Tiles tl;
+    std::memset(&tl, 0, sizeof(tl));
+    tl.tw = tl.th = tl.tiles_x = 1;
+    if (R->ntiles)
     {
-        const bool guided = guide != 0 && (guide == 2 || R->guided_passes >= prm->guide_warmup);
-        S.guide_frac = guided ? frac : 0.0f;
-        TrainOut G{ nullptr, 0, 0, nullptr, nullptr, nullptr, 0, nullptr };
-        if (guide == 1)
-        {
-            G = TrainOut{ R->vrec, R->vrec_depth, 0, R->tin, R->tdir, R->tw, R->tcap, R->tcount };
-            APG_HIP_CHECK(ctx, hipMemsetAsync(R->tcount, 0, 4, stream));
-        }
-        k_set_counts<<<1, 64, 0, stream>>>(R->counts, P);
-        k_generate<<<nblocks(P, 256), 256, 0, stream>>>(R->cam, R->W, R->H, P, R->rng, R->T, R->L, R->state,
-                                                          R->qa, R->stats);
+        if (R->W % R->tile_w || R->H % R->tile_h || (R->W / R->tile_w) * (R->H / R->tile_h) != R->ntiles)
+            return ctx->fail(APG_E_INVALID, "the frame is not a grid of the configured tiles");
+        tl.cams = R->tile_cams;
+        tl.roots = R->tile_roots;
+        tl.tw = R->tile_w;
+        tl.th = R->tile_h;
+        tl.tiles_x = R->W / R->tile_w;
+    }
+    const bool forest = ctx->scene.roots > 1;
+    // One pass (one sample per path slot) on stream st: generate, then per bounce
+    // extend / shade / shadow / resolve.
+    auto enqueue_pass = [&](hipStream_t st, const TrainOut& G0) -> int {
+        TrainOut G = G0;
+        k_set_counts<<<1, 64, 0, st>>>(R->counts, P);
+        k_generate<<<nblocks(P, 256), 256, 0, st>>>(R->cam, tl, R->W, R->H, P, R->rng, R->T, R->L, R->state,
+                                                      R->qa, R->stats);
         Queue in = R->qa, out = R->qb;
         for (uint32_t b = 0; b < prm->max_depth; ++b)
         {
             apg_rays rays{ in.o[0], in.o[1], in.o[2], in.d[0], in.d[1], in.d[2], nullptr };
             apg_hits hits{ R->hprim, R->ht, R->hb0, R->hb1 };
-            int rc = launch_trace_raw(ctx, rays, P, R->counts, &hits, false, nullptr, nullptr, stream);
+            int rc = launch_trace_raw(ctx, rays, P, R->counts, &hits, false, nullptr, nullptr, st,
+                                      forest ? in.root : nullptr);
             if (rc) return rc;
             S.in = in;
             S.out = out;
             S.bounce = b;
             if (guide)
             {
-                k_guide_features<<<sgrid, 256, 0, stream>>>(S, R->gfeat);
-                rc = guide_infer(ctx, R->gfeat, P, R->counts, R->gout, stream);
+                k_guide_features<<<sgrid, 256, 0, st>>>(S, R->gfeat);
+                rc = guide_infer(ctx, R->gfeat, P, R->counts, R->gout, st);
                 if (rc) return rc;
-                k_shade<true><<<sgrid, 256, 0, stream>>>(S);
+                k_shade<true><<<sgrid, 256, 0, st>>>(S);
             }
             else
-                k_shade<false><<<sgrid, 256, 0, stream>>>(S);
+                k_shade<false><<<sgrid, 256, 0, st>>>(S);
             apg_rays srays{ R->qs.o[0], R->qs.o[1], R->qs.o[2], R->qs.d[0], R->qs.d[1], R->qs.d[2], R->qs.tmax };
-            rc = launch_trace_raw(ctx, srays, 2 * P, R->counts + 2, nullptr, true, R->occ, nullptr, stream);
+            rc = launch_trace_raw(ctx, srays, 2 * P, R->counts + 2, nullptr, true, R->occ, nullptr, st,
+                                  forest ? R->qs.root : nullptr);
             if (rc) return rc;
-            k_shadow_vis<<<vgrid, 256, 0, stream>>>(R->counts, R->qs.path, R->occ, R->vis);
+            k_shadow_vis<<<vgrid, 256, 0, st>>>(R->counts, R->qs.path, R->occ, R->vis);
             G.bounce = b;
-            k_resolve<<<rgrid, 256, 0, stream>>>(R->counts, in.path, R->state, R->nee, R->vis, R->L, R->accum,
-                                                 R->accum64, prm->fp64_accum ? 1 : 0, G);
-            k_advance<<<1, 64, 0, stream>>>(R->counts, R->stats);
+            k_resolve<<<rgrid, 256, 0, st>>>(R->counts, in.path, R->state, R->nee, R->vis, R->L, R->accum,
+                                             R->accum64, prm->fp64_accum ? 1 : 0, G);
+            k_advance<<<1, 64, 0, st>>>(R->counts, R->stats);
             std::swap(in, out);
         }
+        return APG_OK;
+    };
+    const TrainOut no_train{ nullptr, 0, 0, nullptr, nullptr, nullptr, 0, nullptr, 1.0f, 0u };
+
+    if (prm->graph && guide != 1)
+    {
+        // The pass is captured once into a HIP graph on the library's own stream and
+        // replayed; it is captured again when anything it baked in has changed.
+        S.guide_frac = guide ? frac : 0.0f;
+        int rc = ensure_trace_capacity(ctx, 2 * P);
+        if (rc) return rc;
+        std::vector<char> key;
+        auto put = [&key](const void* p, size_t n) {
+            key.insert(key.end(), static_cast<const char*>(p), static_cast<const char*>(p) + n);
+        };
+        put(&S, sizeof(S));
+        put(&tl, sizeof(tl));
+        put(&R->cam, sizeof(R->cam));
+        put(&ctx->scene, sizeof(ctx->scene));
+        put(&ctx->cfg, sizeof(ctx->cfg));
+        const uintptr_t extra[] = { P, R->W, R->H, uintptr_t(R->accum), uintptr_t(R->accum64), prm->fp64_accum,
+                                    uintptr_t(R->gfeat), uintptr_t(R->gout), guide, uintptr_t(ctx->spill) };
+        put(extra, sizeof(extra));
+        if (!R->gexec || key != R->gkey)
+        {
+            if (!R->gstream) APG_HIP_CHECK(ctx, hipStreamCreateWithFlags(&R->gstream, hipStreamNonBlocking));
+            if (!R->gev) APG_HIP_CHECK(ctx, hipEventCreateWithFlags(&R->gev, hipEventDisableTiming));
+            if (R->gexec) { (void)hipGraphExecDestroy(R->gexec); R->gexec = nullptr; }
+            hipGraph_t graph = nullptr;
+            APG_HIP_CHECK(ctx, hipStreamBeginCapture(R->gstream, hipStreamCaptureModeRelaxed));
+            rc = enqueue_pass(R->gstream, no_train);
+            hipError_t e = hipStreamEndCapture(R->gstream, &graph);
+            if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+            if (e != hipSuccess) return ctx->fail(APG_E_HIP, hipGetErrorString(e));
+            e = hipGraphInstantiate(&R->gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) { R->gexec = nullptr; return ctx->fail(APG_E_HIP, hipGetErrorString(e)); }
+            R->gkey = key;
+        }
+        APG_HIP_CHECK(ctx, hipEventRecord(R->gev, stream));
+        APG_HIP_CHECK(ctx, hipStreamWaitEvent(R->gstream, R->gev, 0));
+        for (uint32_t sidx = 0; sidx < prm->spp / R->S; ++sidx)
+            APG_HIP_CHECK(ctx, hipGraphLaunch(R->gexec, R->gstream));
+        APG_HIP_CHECK(ctx, hipEventRecord(R->gev, R->gstream));
+        APG_HIP_CHECK(ctx, hipStreamWaitEvent(stream, R->gev, 0));
+        if (guide == 2) R->guided_passes += prm->spp / R->S;
+    }
+    else
+    for (uint32_t sidx = 0; sidx < prm->spp / R->S; ++sidx)
+    {
+        const bool guided = guide != 0 && (guide == 2 || R->guided_passes >= prm->guide_warmup);
+        S.guide_frac = guided ? frac : 0.0f;
+        TrainOut G = no_train;
+        if (guide == 1)
+        {
+            // keep about tcap of the pass's P x (max_depth - 1) candidate samples, spread
+            // uniformly over the frame instead of the first paths to finish
+            G = TrainOut{ R->vrec, R->vrec_depth, 0, R->tin, R->tdir, R->tw, R->tcap, R->tcount,
+                          R->train_keep, R->guided_passes * 0x632BE5ABu + 17u };
+            APG_HIP_CHECK(ctx, hipMemsetAsync(R->tcount, 0, 4, stream));
+        }
+        int rc = enqueue_pass(stream, G);
+        if (rc) return rc;
         if (guide == 1)
         {
             // online training on this pass's radiance samples
             uint32_t tn = 0;
             APG_HIP_CHECK(ctx, hipMemcpyAsync(&tn, R->tcount, 4, hipMemcpyDeviceToHost, stream));
             APG_HIP_CHECK(ctx, hipStreamSynchronize(stream));
+            // keep about 95 % of the capacity next pass, spread over the whole frame
+            // instead of the paths that happen to finish first
+            {
+                const float cand = float(tn) / R->train_keep;
+                R->train_keep = cand > 0.0f ? std::min(1.0f, 0.95f * float(R->tcap) / cand) : 1.0f;
+            }
             tn = std::min(tn, R->tcap);
             R->last_train_samples = tn;
             if (tn)
             {
                 // one epoch over the pass's samples in train_steps minibatches
-                const uint32_t S = std::min(train_steps, tn);
+                const uint32_t S = std::min(std::min(train_steps, tn), TRAIN_MAX_STEPS);
                 const uint32_t B = (tn + S - 1) / S;
+                k_zero_f32<<<nblocks(S, 256), 256, 0, stream>>>(R->twsum, S);
                 k_train_encode<<<nblocks(tn, 256), 256, 0, stream>>>(R->tin, R->tdir, R->tw, tn, S, B, R->tfeat,
-                                                                     R->tdir_p, R->tw_p);
+                                                                     R->tdir_p, R->tw_p, R->twsum);
                 for (uint32_t j = 0; j < S; ++j)
                 {
                     const uint32_t mb = (tn - j + S - 1) / S;
-                    int rc = guide_train(ctx, R->tfeat + size_t(j) * B * GUIDE_IN, R->tdir_p + size_t(j) * B * 3,
-                                         R->tw_p + size_t(j) * B, mb, true,
-                                         j + 1 == S ? &R->last_loss : nullptr, stream);
+                    rc = guide_train(ctx, R->tfeat + size_t(j) * B * GUIDE_IN, R->tdir_p + size_t(j) * B * 3,
+                                     R->tw_p + size_t(j) * B, mb, true,
+                                     j + 1 == S ? &R->last_loss : nullptr, stream, R->twsum + j);
                     if (rc) return rc;
                 }
             }

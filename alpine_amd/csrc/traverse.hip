@@ -48,6 +48,8 @@ struct TraceArgs
     apg_rays rays;
     uint32_t n;
     const uint32_t* n_dev;            // when set, the ray count is read here (queued work)
+    const uint32_t* roots;            // when set, ray i starts at tree root roots[i]
+    uint32_t root_count;
     uint32_t has_scene;
     apg_hits hits;
     uint8_t* occluded;
@@ -137,8 +139,13 @@ __global__ __launch_bounds__(256, 8) void trace_kernel(TraceArgs A)
                     hb0 = hb1 = 0.0f;
                     found = false;
                     sp = 0;
-                    g_base = 0;
                     // Root as a group holding one inner child in slot 0.
+                    g_base = 0;
+                    if (A.roots)
+                    {
+                        const uint32_t r = A.roots[rid];
+                        g_base = r < A.root_count ? r : A.root_count - 1u;
+                    }
                     g_bits = A.has_scene ? root_group_bits(rb.oct) : 0u;
                 }
             }
@@ -337,12 +344,24 @@ static int ensure_trace_buffers(Context* ctx, uint32_t threads)
     return APG_OK;
 }
 
+// Size the shared traversal buffers for a launch over up to n rays (callers that
+// capture launches into graphs do this first: the capture must not allocate).
+int ensure_trace_capacity(Context* ctx, uint32_t n)
+{
+    const apg_trace_config& cfg = ctx->cfg;
+    uint32_t grid = uint32_t(ctx->cu_count) * cfg.blocks_per_cu;
+    const uint32_t need = (n + cfg.block_size - 1) / cfg.block_size;
+    if (grid > need) grid = need;
+    if (grid == 0) grid = 1;
+    return ensure_trace_buffers(ctx, grid * cfg.block_size);
+}
+
 // Launch one traversal over `rays` on `stream` without cross-stream ordering.
 // n_dev, when non-null, holds the ray count on the device (n is then the
 // capacity used to size the grid).
 int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint32_t* n_dev,
                      const apg_hits* hits, bool any, uint8_t* occ, unsigned long long* counts,
-                     hipStream_t stream)
+                     hipStream_t stream, const uint32_t* roots)
 {
     const apg_trace_config& cfg = ctx->cfg;
     const uint32_t block = cfg.block_size;
@@ -361,6 +380,8 @@ int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint3
     A.rays = rays;
     A.n = n;
     A.n_dev = n_dev;
+    A.roots = roots;
+    A.root_count = ctx->scene.roots;
     A.has_scene = ctx->scene.node_count > 0 ? 1u : 0u;
     if (hits) A.hits = *hits; else A.hits = apg_hits{ nullptr, nullptr, nullptr, nullptr };
     A.occluded = occ;
@@ -390,7 +411,7 @@ int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint3
 }
 
 int launch_trace(Context* ctx, const apg_rays* rays, uint32_t n, const apg_hits* hits, bool any,
-                 uint8_t* occ, unsigned long long* counts, hipStream_t stream)
+                 uint8_t* occ, unsigned long long* counts, hipStream_t stream, const uint32_t* roots = nullptr)
 {
     if (n == 0) return APG_OK;
     if (!rays || (!any && (!hits || !hits->prim || !hits->t || !hits->b0 || !hits->b1)) ||
@@ -408,7 +429,7 @@ int launch_trace(Context* ctx, const apg_rays* rays, uint32_t n, const apg_hits*
     // Kernels from all streams share the ray cursor and the spill area:
     // order this launch after the previous one.
     APG_HIP_CHECK(ctx, hipStreamWaitEvent(stream, ctx->last_launch, 0));
-    int rc = launch_trace_raw(ctx, *rays, n, nullptr, hits, any, occ, counts, stream);
+    int rc = launch_trace_raw(ctx, *rays, n, nullptr, hits, any, occ, counts, stream, roots);
     if (rc) return rc;
     APG_HIP_CHECK(ctx, hipEventRecord(ctx->last_launch, stream));
     return APG_OK;
@@ -432,6 +453,24 @@ int apg_occluded(apg_context* ctx, const apg_rays* rays, uint32_t n, uint8_t* oc
     if (!ctx) return APG_E_INVALID;
     return apg::launch_trace(ctx, rays, n, nullptr, true, occluded, nullptr,
                              static_cast<hipStream_t>(stream));
+}
+
+int apg_intersect_roots(apg_context* ctx, const apg_rays* rays, const uint32_t* roots, uint32_t n,
+                        const apg_hits* hits, void* stream)
+{
+    if (!ctx) return APG_E_INVALID;
+    if (n && !roots) return ctx->fail(APG_E_INVALID, "null roots");
+    return apg::launch_trace(ctx, rays, n, hits, false, nullptr, nullptr,
+                             static_cast<hipStream_t>(stream), roots);
+}
+
+int apg_occluded_roots(apg_context* ctx, const apg_rays* rays, const uint32_t* roots, uint32_t n,
+                       uint8_t* occluded, void* stream)
+{
+    if (!ctx) return APG_E_INVALID;
+    if (n && !roots) return ctx->fail(APG_E_INVALID, "null roots");
+    return apg::launch_trace(ctx, rays, n, nullptr, true, occluded, nullptr,
+                             static_cast<hipStream_t>(stream), roots);
 }
 
 int apg_intersect_counted(apg_context* ctx, const apg_rays* rays, uint32_t n,

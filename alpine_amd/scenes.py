@@ -419,3 +419,65 @@ def guiding_scene(width=256, height=256, spp=16, max_depth=8, grid=64, face_res=
                fovy=float(np.radians(60.0)), aspect=width / height)
     return RenderScene(verts, tris, tri_mat, mats, lights, cam, width, height, spp, max_depth,
                        light_sampler=0, seed=seed)
+
+
+_SPHERES = {}
+
+
+def _unit_icosphere(subdiv):
+    if subdiv not in _SPHERES:
+        v, f = icosahedron()
+        for _ in range(subdiv):
+            v, f = subdivide(v, f)
+        _SPHERES[subdiv] = (v, f)
+    return _SPHERES[subdiv]
+
+
+class BatchedScenes:
+    """Many independent scenes for one tiled frame: scene k is a displaced icosphere plus
+    random triangles (the traversal microbench's construction with seed `seed + k`), sized
+    log-uniformly between `lo` and `hi` triangles.  The last `anim` fraction of every scene's
+    triangles are random triangles with vertices of their own (a contiguous vertex range)
+    that turn rigidly about the y axis from frame to frame.  All scenes share one quad light
+    overhead and a grey Lambertian material; scene k is seen by camera k."""
+
+    def __init__(self, n_scenes=32, lo=10_000, hi=2_000_000, seed=107, anim=0.25, tile=512):
+        rng = np.random.default_rng(seed)
+        self.sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n_scenes)).astype(np.int64)
+        self.meshes, self.anim_range = [], []
+        for k, n in enumerate(self.sizes):
+            s = 3
+            while 20 * 4 ** (s + 1) <= (1.0 - anim) * n:
+                s += 1
+            v, f = _unit_icosphere(s)
+            r = 1.0 + 0.1 * value_noise(v, seed + k)
+            sphere = ((v * r[:, None]).astype(np.float32), f.astype(np.uint32))
+            n_rand = max(int(n) - len(f), 1)
+            rand = random_triangles(n_rand, seed + k)
+            verts, tris = concat(sphere, rand)
+            n_anim = max(1, int(round(anim * len(tris))))
+            # the animated triangles are the last n_anim random ones: 3 vertices each, at the end
+            self.anim_range.append((len(verts) - 3 * n_anim, 3 * n_anim))
+            self.meshes.append((verts, tris))
+        self.tile = tile
+        self.cams = []
+        for k in range(n_scenes):
+            a = 2.0 * np.pi * k / n_scenes
+            eye = (6.5 * np.sin(a), 2.5 + 0.5 * np.cos(3 * a), 6.5 * np.cos(a))
+            self.cams.append(dict(eye=eye, target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0),
+                                  fovy=float(np.radians(50.0)), aspect=1.0))
+        self.materials = [(0, (0.7, 0.7, 0.7), (1.0, 1.0))]
+        self.lights = [dict(kind=2, p0=(-1.0, 5.0, -1.0), p1=(1.0, 5.0, -1.0), p2=(-1.0, 5.0, 1.0),
+                            color=(20.0, 19.0, 17.0))]
+        self.tiles_x = 8 if n_scenes % 8 == 0 else n_scenes
+        self.tiles_y = n_scenes // self.tiles_x
+
+    @property
+    def triangles(self):
+        return int(sum(len(t) for _, t in self.meshes))
+
+    @staticmethod
+    def rotation(angle):
+        """3 x 4 rigid transform: rotation about y by `angle` radians."""
+        c, s = np.cos(angle), np.sin(angle)
+        return np.array([[c, 0, s, 0], [0, 1, 0, 0], [-s, 0, c, 0]], np.float32)

@@ -63,10 +63,19 @@ typedef struct apg_hits {
 typedef struct apg_build_params {
     uint32_t leaf_threshold;  /* binned-SAH leaf threshold (reference Bvh: 8); 0 = default 4 */
     uint32_t builder;         /* APG_BUILDER_GPU (default) or APG_BUILDER_HOST */
-    uint32_t reserved[6];
+    uint32_t roots;           /* APG_ROOTS_SCENE: one tree over all shapes (the reference's
+                               * updateScene); APG_ROOTS_PER_SHAPE: an independent tree per shape,
+                               * built together, wide node s the root of shape s (GPU builder;
+                               * rays then name their tree, apg_intersect_roots) */
+    uint32_t flags;           /* APG_BUILD_NO_STATS: skip the host-side tree statistics and
+                               * leaf keys (per-frame rebuilds) */
+    uint32_t reserved[4];
 } apg_build_params;
 #define APG_BUILDER_GPU 0u
 #define APG_BUILDER_HOST 1u
+#define APG_ROOTS_SCENE 0u
+#define APG_ROOTS_PER_SHAPE 1u
+#define APG_BUILD_NO_STATS 1u
 
 typedef struct apg_scene_stats {
     uint32_t triangles;
@@ -130,6 +139,13 @@ int apg_occluded(apg_context* ctx, const apg_rays* rays, uint32_t n, uint8_t* oc
 /* Instrumented closest/any-hit run: same results, plus total 64 B node
  * fetches and 36 B triangle fetches over all rays (host uint64 out).  Used for
  * the byte model of the roofline. */
+/* As apg_intersect / apg_occluded in a per-shape forest (APG_ROOTS_PER_SHAPE): ray i is traced
+ * in the tree of shape roots[i] (device array).  No reference counterpart: the reference keeps
+ * one scene per accelerator; many independent scenes share one launch this way. */
+int apg_intersect_roots(apg_context* ctx, const apg_rays* rays, const uint32_t* roots, uint32_t n,
+                        const apg_hits* hits, void* stream);
+int apg_occluded_roots(apg_context* ctx, const apg_rays* rays, const uint32_t* roots, uint32_t n,
+                       uint8_t* occluded, void* stream);
 int apg_intersect_counted(apg_context* ctx, const apg_rays* rays, uint32_t n,
                           const apg_hits* hits, int any, uint8_t* occluded,
                           uint64_t* node_fetches, uint64_t* tri_fetches, void* stream);
@@ -202,7 +218,11 @@ typedef struct apg_render_params {
                                 * in this many Adam minibatch steps; 0 = 4 */
     uint32_t guide_warmup;     /* passes traced before the proposal is mixed in (guide = 1) */
     float guide_frac;          /* probability of sampling the proposal; 0 = 0.5 */
-    uint32_t reserved[3];
+    uint32_t graph;            /* 1: capture a pass into a HIP graph once and replay it (plain or
+                                * guide = 2); captured again when the scene, frame or params change */
+    uint32_t train_samples;    /* guide = 1: radiance samples kept per pass for training, spread
+                                * over the frame; 0 = 262144 */
+    uint32_t reserved[1];
 } apg_render_params;
 
 /* Allocate per-pixel samplers (seeded with pixel index + seed, alpine.cpp:259-269) and clear
@@ -215,6 +235,12 @@ int apg_render_reset(apg_context* ctx, uint32_t width, uint32_t height, uint32_t
 int apg_render_reset_slots(apg_context* ctx, uint32_t width, uint32_t height, uint32_t seed,
                            uint32_t slots);
 int apg_render_set_camera(apg_context* ctx, const apg_camera* cam);
+/* Tiled frames (no reference counterpart): the frame is a grid of ntiles tiles of
+ * tile_width x tile_height pixels, row-major; tile t is seen through cams[t] and, with a
+ * per-shape forest, traced in tree roots[t] (host arrays; roots may be NULL = tree 0).
+ * ntiles = 0 returns to the single camera.  Many independent scenes render in one pass. */
+int apg_render_set_tiles(apg_context* ctx, const apg_camera* cams, const uint32_t* roots,
+                         uint32_t ntiles, uint32_t tile_width, uint32_t tile_height);
 /* Trace params->spp samples per pixel into the accumulation (asynchronous on stream). */
 int apg_render(apg_context* ctx, const apg_render_params* params, void* stream);
 /* Copy the accumulated radiance sum (width x height x 3, host) and the sample count.
