@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=262_144)
     ap.add_argument("--builder", choices=["gpu", "host"], default="gpu")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing configs")
+    ap.add_argument("--frames", type=int, default=3,
+                    help="frames of the batched animated config (60 for the full run)")
     ap.add_argument("--autotune", action="store_true",
                     help="sweep the launch shape first (the library default is the sweep's pick)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape "
@@ -130,9 +132,10 @@ def _timed_render(acc, sc, spp, seed, **kw):
     return time.perf_counter() - t0, img, img64
 
 
-def render_benchmarks(builder):
+def render_benchmarks(builder, frames=3):
     """Path-tracing configs: Cornell room (16 spp, 256^2), many-light GGX (1920x1080, 64 spp,
-    light BVH, RR from depth 3) and guided vs plain time to equal relMSE (doorway scene)."""
+    light BVH, RR from depth 3), 32 animated scenes in one tiled frame, and guided vs plain
+    time to equal relMSE (box grid behind a doorway)."""
     from alpine_amd import scenes
     out = {}
     # config 1: Cornell room
@@ -148,6 +151,17 @@ def render_benchmarks(builder):
         "seconds": round(dt, 4), "samples_per_s": round(sc.width * sc.height * sc.spp / dt, 1),
         "rel_mse_vs_1024spp_fp64": round(scenes.rel_mse(img / sc.spp, ref), 6),
         "rays_per_s": round((c["closest_rays"] + c["shadow_rays"]) / dt, 1) if dt > 0 else None}
+    # the same 16 spp traced side by side (16 slots per pixel, one pass)
+    import torch
+    acc.render_reset(sc.width, sc.height, sc.seed, slots=16)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    acc.render(sc.spp, sc.max_depth, sc.rr_depth, sc.background)
+    img, _, _ = acc.render_read()
+    dt16 = time.perf_counter() - t0
+    out["cornell_256_16spp"]["slots16"] = {
+        "seconds": round(dt16, 4), "samples_per_s": round(sc.width * sc.height * sc.spp / dt16, 1),
+        "rel_mse_vs_1024spp_fp64": round(scenes.rel_mse(img / sc.spp, ref), 6)}
     del acc
     # config 3: many-light GGX
     log("[bench] many-light scene")
@@ -162,38 +176,144 @@ def render_benchmarks(builder):
         "rays_per_s": round((c["closest_rays"] + c["shadow_rays"]) / dt, 1),
         "finite": bool(np.isfinite(img).all())}
     del acc
-    # config 5: guided vs plain, time to equal relMSE on the doorway scene
+    # config 4: many independent animated scenes in one tiled frame
+    log("[bench] batched animated scenes")
+    out["batched_32_scenes_512_32spp"] = batched_benchmark(frames=frames)
+    # config 5: guided vs plain, time to equal relMSE
     log("[bench] guided vs plain")
-    sc = scenes.doorway_scene(256, 256)
-    acc = _render_setup(sc, builder)
-    acc.guide_init(seed=1, lr=1e-3)
-    _, _, ref64 = _timed_render(acc, sc, 2048, 4000, fp64=True)
-    ref = ref64 / 2048
-    spp_plain = 64
-    t_plain, img, _ = _timed_render(acc, sc, spp_plain, 11)
-    e_plain = scenes.rel_mse(img / spp_plain, ref)
-    # guided: train online, then render with the frozen proposal; time includes training
-    t_guided, e_guided, spp_g = None, None, None
-    import torch
-    acc.render_reset(sc.width, sc.height, 21)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    acc.render(16, sc.max_depth, guide=1, guide_warmup=2, train_steps=8)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    for spp in (4, 8, 12, 16, 24, 32, 48, 64):
-        dt, img, _ = _timed_render(acc, sc, spp, 31, guide=2)
-        e = scenes.rel_mse(img / spp, ref)
-        if e <= e_plain:
-            t_guided, e_guided, spp_g = t_train + dt, e, spp
-            break
-    out["guided_doorway_256"] = {
-        "plain_spp": spp_plain, "plain_seconds": round(t_plain, 4), "plain_rel_mse": round(e_plain, 6),
-        "guided_spp": spp_g, "guided_seconds_incl_training": None if t_guided is None else round(t_guided, 4),
-        "guided_rel_mse": None if e_guided is None else round(e_guided, 6),
-        "training_seconds": round(t_train, 4),
-        "time_ratio_guided_over_plain": None if t_guided is None else round(t_guided / t_plain, 4)}
+    out["guided_vs_plain"] = guided_benchmark(builder)
     return out
+
+
+# Guided-rendering protocol: plain PT renders PLAIN_SPP samples per pixel progressively; its
+# final relMSE against a REF_SPP FP64 plain reference is the target.  The guided run trains
+# online for TRAIN_SPP samples per pixel (these samples stay in the image) and then renders with
+# the frozen proposal until it reaches the target.  Both run SLOTS samples of each pixel side by
+# side and are timed around the GPU work only (the relMSE checks read the image between chunks).
+GUIDE_PLAIN_SPP, GUIDE_REF_SPP, GUIDE_SLOTS, GUIDE_CHUNK = 1024, 4096, 4, 32
+GUIDE_TRAIN_SPP, GUIDE_STEPS, GUIDE_LR, GUIDE_FRAC, GUIDE_SAMPLES = 64, 64, 3e-3, 0.5, 1 << 19
+
+
+def guided_benchmark(builder):
+    import torch
+    from alpine_amd import scenes
+    sc = scenes.guiding_scene()
+    acc = _render_setup(sc, builder)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    acc.render_reset(sc.width, sc.height, 0, slots=GUIDE_SLOTS)
+    acc.render(GUIDE_SLOTS, sc.max_depth)
+    acc.render_reset(sc.width, sc.height, 4000, slots=GUIDE_SLOTS)
+    t_ref = timed(lambda: acc.render(GUIDE_REF_SPP, sc.max_depth, fp64=True))
+    _, r64, _ = acc.render_read(fp64=True)
+    ref = r64 / GUIDE_REF_SPP
+    # plain, progressive
+    acc.render_reset(sc.width, sc.height, 11, slots=GUIDE_SLOTS)
+    t_plain, spp = 0.0, 0
+    curve_plain = []
+    while spp < GUIDE_PLAIN_SPP:
+        t_plain += timed(lambda: acc.render(GUIDE_CHUNK, sc.max_depth, graph=True))
+        spp += GUIDE_CHUNK
+        img, _, _ = acc.render_read()
+        curve_plain.append((spp, t_plain, scenes.rel_mse(img / spp, ref)))
+    target = curve_plain[-1][2]
+    # guided: online training, then the frozen proposal until the target is reached
+    acc.guide_init(seed=1, lr=GUIDE_LR)
+    acc.render_reset(sc.width, sc.height, 21, slots=GUIDE_SLOTS)
+    t_train = timed(lambda: acc.render(GUIDE_TRAIN_SPP, sc.max_depth, guide=1, guide_warmup=1,
+                                       train_steps=GUIDE_STEPS, guide_frac=GUIDE_FRAC,
+                                       train_samples=GUIDE_SAMPLES))
+    img, _, tot = acc.render_read()
+    t_g = t_train
+    curve_g = [(tot, t_g, scenes.rel_mse(img / tot, ref))]
+    reach = t_g if curve_g[0][2] <= target else None
+    while reach is None and tot < 4 * GUIDE_PLAIN_SPP:
+        t_g += timed(lambda: acc.render(GUIDE_CHUNK, sc.max_depth, guide=2, guide_frac=GUIDE_FRAC,
+                                        graph=True))
+        img, _, tot = acc.render_read()
+        e = scenes.rel_mse(img / tot, ref)
+        curve_g.append((tot, t_g, e))
+        if e <= target:
+            reach = t_g
+    # equal-time comparison: plain's relMSE at the guided run's wall time
+    e_plain_eq = next((e for s_, t_, e in curve_plain if t_ >= t_g), curve_plain[-1][2])
+    return {
+        "scene": "box grid (config-3 geometry) behind a 4 x 3 doorway, one 1 x 1 quad light",
+        "triangles": int(len(sc.tris)), "frame": [sc.width, sc.height], "slots": GUIDE_SLOTS,
+        "reference": f"{GUIDE_REF_SPP} spp plain, FP64 accumulation ({t_ref:.1f} s)",
+        "plain_spp": GUIDE_PLAIN_SPP, "plain_seconds": round(t_plain, 4), "plain_rel_mse": round(target, 6),
+        "guided_train_spp": GUIDE_TRAIN_SPP, "guided_train_seconds": round(t_train, 4),
+        "guided_spp_at_target": int(tot), "guided_seconds_to_target": None if reach is None else round(reach, 4),
+        "guided_rel_mse": round(curve_g[-1][2], 6),
+        "time_ratio_guided_over_plain": None if reach is None else round(reach / t_plain, 4),
+        "plain_rel_mse_at_guided_time": round(e_plain_eq, 6),
+        "train": {"minibatches_per_pass": GUIDE_STEPS, "lr": GUIDE_LR, "proposal_fraction": GUIDE_FRAC,
+                  "samples_per_pass": GUIDE_SAMPLES},
+    }
+    return out
+
+
+def batched_benchmark(frames=3, spp=32, max_depth=5, rr_depth=3, seed=1):
+    """Many independent scenes, animated, one tiled frame: 32 scenes of 10 K - 2 M triangles
+    (log-uniform, about 12 M in all) in a per-shape forest, 512 x 512 tiles, 32 spp per frame.
+    Each frame turns the last 25 % of every scene's triangles about y on the GPU, rebuilds all
+    32 trees on the GPU in one forest build, and renders every tile in the same launches (the
+    pass replayed from a HIP graph)."""
+    import torch
+    import alpine_amd as aa
+    from alpine_amd import scenes
+    t0 = time.perf_counter()
+    B = scenes.BatchedScenes()
+    t_gen = time.perf_counter() - t0
+    acc = aa.GpuAccelerator(0)
+    for v, f in B.meshes:
+        acc.append_mesh(v, f)
+    t0 = time.perf_counter()
+    acc.update_scene(per_shape=True, stats=False)
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    n_tris = B.triangles
+    acc.set_materials([aa.material(*m) for m in B.materials], np.zeros(n_tris, np.uint32))
+    acc.set_lights([aa.light(**l) for l in B.lights], 0)
+    acc.set_tiles([aa.camera(**c) for c in B.cams], B.tile, B.tile, roots=np.arange(len(B.cams)))
+    W, H = B.tile * B.tiles_x, B.tile * B.tiles_y
+    rot = B.rotation(2.0 * np.pi / 120.0)
+    builds, renders = [], []
+    for f in range(frames):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for k, (first, count) in enumerate(B.anim_range):
+            acc.transform_vertices(k, rot, first=first, count=count)
+        acc.update_scene(per_shape=True, stats=False)
+        torch.cuda.synchronize()
+        b = time.perf_counter()
+        acc.render_reset(W, H, seed + f)
+        acc.render(spp, max_depth, rr_depth, graph=True)
+        torch.cuda.synchronize()
+        c = time.perf_counter()
+        builds.append(b - a)
+        renders.append(c - b)
+    cnt = acc.render_counters()
+    img, _, total = acc.render_read()
+    r = float(np.median(renders))
+    return {
+        "scenes": len(B.meshes), "triangles": n_tris, "frame": [W, H], "spp": spp,
+        "frames": frames, "max_depth": max_depth, "rr_depth": rr_depth,
+        "generate_s": round(t_gen, 2), "first_build_upload_ms": round(t_first * 1e3, 1),
+        "rebuild_ms_median": round(float(np.median(builds)) * 1e3, 1),
+        "render_ms_median": round(r * 1e3, 1),
+        "frame_ms_median": round(float(np.median(np.array(builds) + np.array(renders))) * 1e3, 1),
+        "samples_per_s": round(W * H * spp / r, 1),
+        "rays_per_s": round((cnt["closest_rays"] + cnt["shadow_rays"]) / renders[-1], 1),
+        "rebuild_mtris_per_s": round(n_tris / float(np.median(builds)) / 1e6, 2),
+        "finite": bool(np.isfinite(img).all()), "mean_radiance": round(float(img.mean() / total), 5),
+    }
 
 
 def cpu_baseline(verts, tris, org, dir, sample):
@@ -218,6 +338,27 @@ def cpu_baseline(verts, tris, org, dir, sample):
         "sample": f"first {n} of the {len(org)} incoherent rays, closest-hit, binary Bvh "
                   f"(9-bin SAH, leaf 8), median of 3 after a warm-up; build {build_s:.2f} s",
     }, h, bvh
+
+
+def cpu_baseline_wide(verts, tris, org, dir, sample, cores):
+    """The reference's 8-wide WideBvh with the ISPC kernels' arithmetic (restated in the
+    oracle; ISPC itself is not available) on this host's cores."""
+    import oracle
+    t0 = time.perf_counter()
+    w = oracle.OracleWideBvh(verts, tris)
+    build_s = time.perf_counter() - t0
+    n = min(sample, len(org))
+    o, d = org[:n], dir[:n]
+    w.intersect(o[:4096], d[:4096], nthreads=cores)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        w.intersect(o, d, nthreads=cores)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return {"value": round(n / med / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"first {n} incoherent rays, closest-hit, WideBvh restated (8-wide, leaf 16, "
+                      f"ISPC slab/triangle arithmetic, scalar C loops), median of 3; build {build_s:.2f} s"}
 
 
 def main():
@@ -315,11 +456,13 @@ def main():
     }
     if not args.no_render:
         log("[bench] render configs")
-        result["extra"]["render"] = render_benchmarks(builder)
+        result["extra"]["render"] = render_benchmarks(builder, args.frames)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         cb, href, _ = cpu_baseline(verts, tris, org, dir, args.cpu_sample)
         result["cpu_baseline"] = cb
+        result["extra"]["cpu_baseline_wide"] = cpu_baseline_wide(verts, tris, org, dir,
+                                                                 args.cpu_sample, cb["cores"])
         n = len(href.prim)
         torch.cuda.synchronize()
         g = out[0][:n].cpu().numpy().view(np.uint32)

@@ -62,6 +62,16 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.oracle_bvh_create.restype = C.c_void_p
         L.oracle_bvh_create.argtypes = [_f32p, C.c_uint32, _u32p, C.c_uint32, C.c_uint32]
+        L.oracle_wide_create.restype = C.c_void_p
+        L.oracle_wide_create.argtypes = [_f32p, C.c_uint32, _u32p, C.c_uint32]
+        L.oracle_wide_destroy.argtypes = [C.c_void_p]
+        L.oracle_wide_node_count.restype = C.c_uint32
+        L.oracle_wide_node_count.argtypes = [C.c_void_p]
+        L.oracle_wide_intersect.argtypes = [C.c_void_p, _f32p, _f32p, C.c_uint32, _u32p, _f32p,
+                                            _f32p, _f32p, C.c_uint32]
+        L.oracle_wide_occluded.argtypes = [C.c_void_p, _f32p, _f32p, C.c_void_p, C.c_uint32,
+                                           _u8p, C.c_uint32]
+        L.oracle_wide_brute.argtypes = [C.c_void_p, _f32p, _f32p, C.c_uint32, _u32p, _f32p]
         L.oracle_bvh_destroy.argtypes = [C.c_void_p]
         L.oracle_bvh_intersect.argtypes = [C.c_void_p, _f32p, _f32p, C.c_uint32, _u32p, _f32p,
                                            _f32p, _f32p, C.c_uint32]
@@ -137,6 +147,53 @@ class Hits:
         self.t = np.empty(n, np.float32)
         self.b0 = np.empty(n, np.float32)
         self.b1 = np.empty(n, np.float32)
+
+
+class OracleWideBvh:
+    """The reference WideBvh (wide_bvh.cpp) with the ISPC kernels' arithmetic, restated; the
+    8-wide CPU path timed beside the GPU.  Parity against the reference is unpinned (ISPC is
+    absent); brute_intersect uses the same triangle test and pins the traversal."""
+
+    def __init__(self, verts, tris):
+        self.verts = _f32(verts).reshape(-1, 3)
+        self.tris = _u32(tris).reshape(-1, 3)
+        self._h = lib().oracle_wide_create(self.verts.reshape(-1), len(self.verts),
+                                           self.tris.reshape(-1), len(self.tris))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().oracle_wide_destroy(h)
+            self._h = None
+
+    @property
+    def node_count(self):
+        return lib().oracle_wide_node_count(self._h)
+
+    def intersect(self, org, dir, nthreads: int = 1) -> "Hits":
+        org, dir = _f32(org), _f32(dir)
+        n = len(org)
+        h = Hits(n)
+        lib().oracle_wide_intersect(self._h, org.reshape(-1), dir.reshape(-1), n, h.prim, h.t,
+                                    h.b0, h.b1, nthreads)
+        return h
+
+    def occluded(self, org, dir, tfar=None, nthreads: int = 1):
+        org, dir = _f32(org), _f32(dir)
+        n = len(org)
+        out = np.empty(n, np.uint8)
+        keep, ptr = _tfar_arg(tfar, n)
+        lib().oracle_wide_occluded(self._h, org.reshape(-1), dir.reshape(-1), ptr, n, out, nthreads)
+        del keep
+        return out
+
+    def brute_intersect(self, org, dir):
+        org, dir = _f32(org), _f32(dir)
+        n = len(org)
+        prim = np.empty(n, np.uint32)
+        t = np.empty(n, np.float32)
+        lib().oracle_wide_brute(self._h, org.reshape(-1), dir.reshape(-1), n, prim, t)
+        return prim, t
 
 
 class OracleBvh:

@@ -666,3 +666,322 @@ int oracle_box_intersect(const float* b6, const float* org, const float* dir, fl
     f3 inv = mk3(1.0f / dir[0], 1.0f / dir[1], 1.0f / dir[2]);
     return box_intersect(&b, mk3(org[0], org[1], org[2]), tfar, inv);
 }
+
+/* ------------------------------------------------------------------------
+ * WideBvh (wide_bvh.cpp) restated: the binary tree above with leaf threshold
+ * 16 (wide_bvh.cpp:21), collapsed to 8-wide nodes by repeatedly opening the
+ * inner child of largest surface area (collapseNode, wide_bvh.cpp:200-243),
+ * flattened depth first (flattenDepthFirst, :290-356) with leaves packed into
+ * 8-triangle SoA groups (createLeaf, :430-472), traversed with the ISPC
+ * kernels' arithmetic: the slab test of bounding_box.ispc:15-61 (tMax scaled
+ * by 1.0000004f) and the Moller-Trumbore of triangle.ispc:27-87 (|det| >=
+ * 1e-6, one reciprocal multiplied in).  ISPC is not available here, so
+ * whether its build contracts to FMA or approximates the reciprocal is not
+ * pinned; this restatement uses IEEE division and no contraction.
+ * ------------------------------------------------------------------------ */
+#define WIDE_N 8
+#define WIDE_LEAF 16
+#define WIDE_NO_HIT FLT_MAX
+
+typedef struct {
+    float mnx[WIDE_N], mny[WIDE_N], mnz[WIDE_N], mxx[WIDE_N], mxy[WIDE_N], mxz[WIDE_N];
+    uint32_t offset[WIDE_N - 1];
+    uint16_t child_count;
+    uint8_t is_leaf;
+} wnode_t;
+
+typedef struct {
+    float vx[WIDE_N], vy[WIDE_N], vz[WIDE_N];
+    float ex0[WIDE_N], ey0[WIDE_N], ez0[WIDE_N];
+    float ex1[WIDE_N], ey1[WIDE_N], ez1[WIDE_N];
+} trin_t;
+
+struct oracle_wide {
+    oracle_bvh* bin;
+    wnode_t* nodes;
+    uint32_t node_count, node_cap;
+    trin_t* groups;
+    uint32_t group_count, group_cap;
+};
+
+static int bnode_is_leaf(const bnode* n) { return n->count > 0; }
+
+/* collapseNode (wide_bvh.cpp:200-243) */
+static uint32_t wide_collapse(const bnode* node, const bnode** ch)
+{
+    uint32_t n = 0;
+    ch[n++] = node->child[0];
+    ch[n++] = node->child[1];
+    while (n < WIDE_N) {
+        uint32_t candidates = 0, max_idx = 0;
+        float max_sa = 0.0f;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (bnode_is_leaf(ch[i])) continue;
+            float sa = box_sa(ch[i]->bbox);
+            if (sa > max_sa) { max_sa = sa; max_idx = i; }
+            candidates++;
+        }
+        if (candidates == 0) break;
+        /* an inner child of zero area is never chosen by sa > max_sa; the reference then opens
+         * slot 0, which is only defined when slot 0 is an inner node */
+        if (bnode_is_leaf(ch[max_idx])) break;
+        const bnode* c = ch[max_idx];
+        ch[n++] = c->child[0];
+        ch[max_idx] = c->child[1];
+    }
+    return n;
+}
+
+static void wide_leaf(struct oracle_wide* w, const bnode* node, wnode_t* ln)
+{
+    const oracle_bvh* b = w->bin;
+    ln->offset[0] = node->offset;
+    ln->child_count = (uint16_t)node->count;
+    ln->is_leaf = 1;
+    ln->offset[1] = w->group_count;
+    uint32_t groups = (node->count - 1) / WIDE_N + 1;
+    for (uint32_t g = 0; g < groups; ++g) {
+        if (w->group_count == w->group_cap) {
+            w->group_cap = w->group_cap ? 2 * w->group_cap : 1024;
+            w->groups = (trin_t*)realloc(w->groups, (size_t)w->group_cap * sizeof(trin_t));
+        }
+        trin_t* t = &w->groups[w->group_count++];
+        memset(t, 0, sizeof(*t));
+        for (uint32_t j = 0; j < WIDE_N; ++j) {
+            uint32_t idx = g * WIDE_N + j;
+            if (idx >= node->count) break;
+            const prim_t* p = &b->ordered_prims[node->offset + idx];
+            t->vx[j] = p->v0.x; t->vy[j] = p->v0.y; t->vz[j] = p->v0.z;
+            t->ex0[j] = p->e0.x; t->ey0[j] = p->e0.y; t->ez0[j] = p->e0.z;
+            t->ex1[j] = p->e1.x; t->ey1[j] = p->e1.y; t->ez1[j] = p->e1.z;
+        }
+    }
+}
+
+/* flattenDepthFirst (wide_bvh.cpp:290-356) */
+static void wide_flatten(struct oracle_wide* w)
+{
+    typedef struct { const bnode* node; uint32_t node_index_of_parent; int slot; } item_t;
+    uint32_t cap = 1024, sp = 0;
+    item_t* stack = (item_t*)malloc(cap * sizeof(item_t));
+    stack[sp].node = w->bin->root; stack[sp].slot = -1; stack[sp].node_index_of_parent = 0; sp++;
+    while (sp) {
+        item_t it = stack[--sp];
+        if (w->node_count == w->node_cap) {
+            w->node_cap = w->node_cap ? 2 * w->node_cap : 1024;
+            w->nodes = (wnode_t*)realloc(w->nodes, (size_t)w->node_cap * sizeof(wnode_t));
+        }
+        uint32_t me = w->node_count++;
+        if (it.slot > 0) w->nodes[it.node_index_of_parent].offset[it.slot - 1] = me;
+        wnode_t* ln = &w->nodes[me];
+        for (int i = 0; i < WIDE_N; ++i) {
+            ln->mnx[i] = ln->mny[i] = ln->mnz[i] = FLT_MAX;
+            ln->mxx[i] = ln->mxy[i] = ln->mxz[i] = -FLT_MAX;
+        }
+        for (int i = 0; i < WIDE_N - 1; ++i) ln->offset[i] = 0xFFFFFFFFu;
+        if (bnode_is_leaf(it.node)) { wide_leaf(w, it.node, ln); continue; }
+        ln->is_leaf = 0;
+        const bnode* ch[WIDE_N];
+        uint32_t n = wide_collapse(it.node, ch);
+        ln->child_count = (uint16_t)n;
+        for (int i = (int)n - 1; i >= 0; --i) {
+            box3 bb = ch[i]->bbox;
+            ln->mnx[i] = bb.min.x; ln->mny[i] = bb.min.y; ln->mnz[i] = bb.min.z;
+            ln->mxx[i] = bb.max.x; ln->mxy[i] = bb.max.y; ln->mxz[i] = bb.max.z;
+            if (sp == cap) { cap *= 2; stack = (item_t*)realloc(stack, cap * sizeof(item_t)); }
+            stack[sp].node = ch[i]; stack[sp].slot = i; stack[sp].node_index_of_parent = me; sp++;
+        }
+    }
+    free(stack);
+}
+
+oracle_wide* oracle_wide_create(const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt)
+{
+    oracle_wide* w = (oracle_wide*)calloc(1, sizeof(oracle_wide));
+    w->bin = oracle_bvh_create(xyz, nv, idx, nt, WIDE_LEAF);
+    if (nt) wide_flatten(w);
+    return w;
+}
+
+void oracle_wide_destroy(oracle_wide* w)
+{
+    if (!w) return;
+    oracle_bvh_destroy(w->bin);
+    free(w->nodes);
+    free(w->groups);
+    free(w);
+}
+
+uint32_t oracle_wide_node_count(const oracle_wide* w) { return w->node_count; }
+
+static inline float ispc_min(float a, float b) { return a < b ? a : b; }
+static inline float ispc_max(float a, float b) { return a > b ? a : b; }
+
+/* intersectBoundingBoxN (bounding_box.ispc:15-61) */
+static void wide_boxes(const wnode_t* n, f3 org, f3 inv, float tfar, uint8_t* hit)
+{
+    const float corr = 1.0000004f;
+    for (int i = 0; i < WIDE_N; ++i) {
+        float tx0 = (n->mnx[i] - org.x) * inv.x, tx1 = (n->mxx[i] - org.x) * inv.x;
+        float ty0 = (n->mny[i] - org.y) * inv.y, ty1 = (n->mxy[i] - org.y) * inv.y;
+        float tz0 = (n->mnz[i] - org.z) * inv.z, tz1 = (n->mxz[i] - org.z) * inv.z;
+        float tmin = ispc_max(ispc_max(ispc_min(tx0, tx1), ispc_min(ty0, ty1)), ispc_min(tz0, tz1));
+        float tmax = ispc_min(ispc_min(ispc_max(tx0, tx1), ispc_max(ty0, ty1)), ispc_max(tz0, tz1));
+        tmax *= corr;
+        hit[i] = (uint8_t)((tmin <= tmax) & (tmax > 0.0f) & (tmin < tfar));
+    }
+}
+
+/* intersectTriangleN (triangle.ispc:27-87) */
+static void wide_tris(const trin_t* g, f3 o, f3 d, float* t, float* b0, float* b1)
+{
+    for (int i = 0; i < WIDE_N; ++i) {
+        float sx = o.x - g->vx[i], sy = o.y - g->vy[i], sz = o.z - g->vz[i];
+        float s1x = d.y * g->ez1[i] - d.z * g->ey1[i];
+        float s1y = d.z * g->ex1[i] - d.x * g->ez1[i];
+        float s1z = d.x * g->ey1[i] - d.y * g->ex1[i];
+        float s2x = sy * g->ez0[i] - sz * g->ey0[i];
+        float s2y = sz * g->ex0[i] - sx * g->ez0[i];
+        float s2z = sx * g->ey0[i] - sy * g->ex0[i];
+        float det = s1x * g->ex0[i] + s1y * g->ey0[i] + s1z * g->ez0[i];
+        int hit = fabsf(det) >= 1e-6f;
+        float rcp = 1.0f / det;
+        float tt = (s2x * g->ex1[i] + s2y * g->ey1[i] + s2z * g->ez1[i]) * rcp;
+        float u = (s1x * sx + s1y * sy + s1z * sz) * rcp;
+        float v = (d.x * s2x + d.y * s2y + d.z * s2z) * rcp;
+        hit &= tt >= 0.0f;
+        hit &= (u >= 0.0f) & (v >= 0.0f) & (u + v <= 1.0f);
+        t[i] = hit ? tt : WIDE_NO_HIT;
+        b0[i] = hit ? u : WIDE_NO_HIT;
+        b1[i] = hit ? v : WIDE_NO_HIT;
+    }
+}
+
+/* WideBvh::Impl::traverse (wide_bvh.cpp:487-619) */
+static int wide_traverse(const oracle_wide* w, f3 org, f3 dir, float tfar, int any, uint32_t* prim,
+                         float* tout, float* b0out, float* b1out)
+{
+    if (w->node_count == 0) return 0;
+    float tnear = tfar;
+    f3 inv = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+    uint32_t stack[256];
+    uint32_t sp = 0;
+    int found = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        uint32_t cur = stack[--sp];
+        const wnode_t* n = &w->nodes[cur];
+        if (n->is_leaf) {
+            uint32_t groups = (n->child_count - 1u) / WIDE_N + 1u;
+            for (uint32_t g = 0; g < groups; ++g) {
+                float t[WIDE_N], u[WIDE_N], v[WIDE_N];
+                wide_tris(&w->groups[n->offset[1] + g], org, dir, t, u, v);
+                for (uint32_t j = 0; j < WIDE_N; ++j) {
+                    uint32_t idx = g * WIDE_N + j;
+                    if (idx >= n->child_count) break;
+                    if (t[j] == WIDE_NO_HIT || t[j] >= tnear) continue;
+                    if (any) return 1;
+                    found = 1;
+                    tnear = t[j];
+                    *prim = w->bin->ordered[n->offset[0] + idx];
+                    *tout = t[j]; *b0out = u[j]; *b1out = v[j];
+                }
+            }
+        } else {
+            uint8_t hit[WIDE_N];
+            wide_boxes(n, org, inv, tnear, hit);
+            for (int i = (int)n->child_count - 1; i >= 0; --i) {
+                if (!hit[i]) continue;
+                if (sp >= 256) return found;  /* beyond the reference's stack */
+                stack[sp++] = i == 0 ? cur + 1 : n->offset[i - 1];
+            }
+        }
+    }
+    return found;
+}
+
+typedef struct {
+    const oracle_wide* w;
+    const float *org, *dir, *tfar;
+    uint32_t begin, end;
+    int any;
+    uint32_t* prim; float *t, *b0, *b1; uint8_t* occ;
+} wjob_t;
+
+static void* wide_run(void* arg)
+{
+    wjob_t* j = (wjob_t*)arg;
+    for (uint32_t i = j->begin; i < j->end; ++i) {
+        f3 o = mk3(j->org[3 * (size_t)i], j->org[3 * (size_t)i + 1], j->org[3 * (size_t)i + 2]);
+        f3 d = mk3(j->dir[3 * (size_t)i], j->dir[3 * (size_t)i + 1], j->dir[3 * (size_t)i + 2]);
+        if (j->any) {
+            uint32_t p; float t, u, v;
+            j->occ[i] = (uint8_t)wide_traverse(j->w, o, d, j->tfar ? j->tfar[i] : FLT_MAX, 1, &p, &t, &u, &v);
+        } else {
+            uint32_t p = 0xFFFFFFFFu; float t = FLT_MAX, u = 0.0f, v = 0.0f;
+            if (!wide_traverse(j->w, o, d, FLT_MAX, 0, &p, &t, &u, &v)) { p = 0xFFFFFFFFu; t = FLT_MAX; u = v = 0.0f; }
+            j->prim[i] = p; j->t[i] = t; j->b0[i] = u; j->b1[i] = v;
+        }
+    }
+    return NULL;
+}
+
+static void wide_jobs(wjob_t proto, uint32_t n, uint32_t nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    wjob_t jobs[256];
+    uint32_t chunk = (n + nthreads - 1) / nthreads;
+    uint32_t used = 0;
+    for (uint32_t k = 0; k < nthreads; ++k) {
+        jobs[k] = proto;
+        jobs[k].begin = k * chunk < n ? k * chunk : n;
+        jobs[k].end = (k + 1) * chunk < n ? (k + 1) * chunk : n;
+        if (jobs[k].begin >= jobs[k].end) continue;
+        if (nthreads == 1) { wide_run(&jobs[k]); continue; }
+        pthread_create(&th[used], NULL, wide_run, &jobs[k]);
+        used++;
+    }
+    for (uint32_t k = 0; k < used; ++k) pthread_join(th[k], NULL);
+}
+
+void oracle_wide_intersect(const oracle_wide* w, const float* org, const float* dir, uint32_t n,
+                           uint32_t* prim, float* t, float* b0, float* b1, uint32_t nthreads)
+{
+    wjob_t j = { w, org, dir, NULL, 0, 0, 0, prim, t, b0, b1, NULL };
+    wide_jobs(j, n, nthreads);
+}
+
+void oracle_wide_occluded(const oracle_wide* w, const float* org, const float* dir, const float* tfar,
+                          uint32_t n, uint8_t* occ, uint32_t nthreads)
+{
+    wjob_t j = { w, org, dir, tfar, 0, 0, 1, NULL, NULL, NULL, NULL, occ };
+    wide_jobs(j, n, nthreads);
+}
+
+/* Brute force with the ISPC triangle test, for pinning the traversal. */
+void oracle_wide_brute(const oracle_wide* w, const float* org, const float* dir, uint32_t n,
+                       uint32_t* prim, float* tout)
+{
+    const oracle_bvh* b = w->bin;
+    for (uint32_t i = 0; i < n; ++i) {
+        f3 o = mk3(org[3 * (size_t)i], org[3 * (size_t)i + 1], org[3 * (size_t)i + 2]);
+        f3 d = mk3(dir[3 * (size_t)i], dir[3 * (size_t)i + 1], dir[3 * (size_t)i + 2]);
+        float best = FLT_MAX;
+        uint32_t bp = 0xFFFFFFFFu;
+        for (uint32_t k = 0; k < b->nt; ++k) {
+            trin_t g;
+            memset(&g, 0, sizeof(g));
+            const prim_t* p = &b->prims[k];
+            g.vx[0] = p->v0.x; g.vy[0] = p->v0.y; g.vz[0] = p->v0.z;
+            g.ex0[0] = p->e0.x; g.ey0[0] = p->e0.y; g.ez0[0] = p->e0.z;
+            g.ex1[0] = p->e1.x; g.ey1[0] = p->e1.y; g.ez1[0] = p->e1.z;
+            float t[WIDE_N], u[WIDE_N], v[WIDE_N];
+            wide_tris(&g, o, d, t, u, v);
+            if (t[0] != WIDE_NO_HIT && t[0] < best) { best = t[0]; bp = k; }
+        }
+        prim[i] = bp;
+        tout[i] = best;
+    }
+}

@@ -87,6 +87,21 @@ void oracle_bvh_nodes(const oracle_bvh* b, float* bbox6, uint32_t* offset,
 int oracle_find_split(const float* boxes6, uint32_t n, uint32_t* dim, uint32_t* split_idx,
                       float* cost);
 
+/* WideBvh (wide_bvh.cpp) with the ISPC kernels' arithmetic (bounding_box.ispc, triangle.ispc):
+ * binary tree with leaf threshold 16 collapsed to 8-wide nodes, 8-triangle SoA leaf groups.
+ * ISPC is absent here, so the compiled reference cannot pin it ("parity unpinned" against the
+ * reference); tests pin it against brute force with the same triangle test. */
+typedef struct oracle_wide oracle_wide;
+oracle_wide* oracle_wide_create(const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt);
+void oracle_wide_destroy(oracle_wide* w);
+uint32_t oracle_wide_node_count(const oracle_wide* w);
+void oracle_wide_intersect(const oracle_wide* w, const float* org, const float* dir, uint32_t n,
+                           uint32_t* prim, float* t, float* b0, float* b1, uint32_t nthreads);
+void oracle_wide_occluded(const oracle_wide* w, const float* org, const float* dir,
+                          const float* tfar, uint32_t n, uint8_t* occluded, uint32_t nthreads);
+void oracle_wide_brute(const oracle_wide* w, const float* org, const float* dir, uint32_t n,
+                       uint32_t* prim, float* t);
+
 /* Surface area (bounding_box.h:17-21). */
 float oracle_surface_area(const float* box6);
 

@@ -224,3 +224,36 @@ def test_per_shape_forest_traces_each_ray_in_its_own_tree():
         print("shape", s, c)
         assert c["unexplained"] == 0 and c["agree"] + c["tie"] == c["n"]
         np.testing.assert_array_equal(occ[sel], ob.occluded(org[sel], dir[sel]))
+
+
+def test_animated_forest_rebuild_matches_oracle():
+    """GPU rigid transform of a vertex range, then a per-shape rebuild without host round trip:
+    hits equal the oracle's on the transformed geometry read back from the library."""
+    import torch
+    print("[gpu] animated forest rebuild", flush=True)
+    from alpine_amd.scenes import BatchedScenes
+    meshes = [scenes.random_triangles(2000, 21, extent=1.5, lmin=0.1, lmax=0.5),
+              scenes.random_triangles(4000, 22, extent=1.5, lmin=0.1, lmax=0.5)]
+    acc = GpuAccelerator(0)
+    for v, f in meshes:
+        acc.append_mesh(v, f)
+    acc.update_scene(per_shape=True, stats=False)
+    nv1 = len(meshes[1][0])
+    for frame in range(3):
+        acc.transform_vertices(1, BatchedScenes.rotation(0.3), first=nv1 // 2, count=nv1 - nv1 // 2)
+        acc.update_scene(per_shape=True, stats=False)
+    rng = np.random.default_rng(8)
+    n = 4000
+    org = rng.uniform(-2, 2, (n, 3)).astype(np.float32)
+    d = rng.standard_normal((n, 3))
+    dir = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    roots = np.full(n, 1, np.uint32)
+    got = _host(acc.intersect_roots(_dev(org), _dev(dir), torch.from_numpy(roots.view(np.int32)).cuda()))
+    v1 = np.array(acc.get_vertex_buffer(1))
+    assert not np.array_equal(v1, meshes[1][0]) and np.array_equal(v1[: nv1 // 2], meshes[1][0][: nv1 // 2])
+    h = oracle.OracleBvh(v1, meshes[1][1], 8).intersect(org, dir)
+    off = acc.prim_offset(1)
+    ref_p = np.where(h.prim == MISS, MISS, h.prim + off).astype(np.uint32)
+    c = classify(got, (ref_p, h.t, h.b0, h.b1))
+    print(c)
+    assert c["unexplained"] == 0 and c["agree"] + c["tie"] == c["n"] and c["hits"] > 100
