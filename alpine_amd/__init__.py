@@ -127,6 +127,7 @@ ABI_SYMBOLS = [
     "apg_render_read", "apg_render_postprocess", "apg_render_counters", "apg_guide_param_count",
     "apg_guide_init",
     "apg_guide_params", "apg_guide_forward", "apg_guide_infer", "apg_guide_train_step",
+    "apg_guide_train_queries",
     "apg_render_guide_stats",
 ]
 DEBUG_SYMBOLS = ["apg_debug_host_build", "apg_debug_scene_info", "apg_debug_scene_copy",
@@ -193,6 +194,7 @@ def lib():
         "apg_guide_forward": (i32, [vp, vp, u32, vp, vp]),
         "apg_guide_infer": (i32, [vp, vp, u32, vp, vp]),
         "apg_guide_train_step": (i32, [vp, vp, vp, vp, u32, i32, C.POINTER(C.c_float), vp, vp]),
+        "apg_guide_train_queries": (i32, [vp, vp, vp, vp, u32, i32, C.POINTER(C.c_float), vp, vp]),
         "apg_debug_host_build": (i32, [vp, u32, vp, u32, u32, C.POINTER(vp)]),
         "apg_debug_scene_info": (i32, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(u32), C.POINTER(u32)]),
@@ -519,6 +521,16 @@ class GpuAccelerator:
         out = torch.empty((n, 48), dtype=torch.float32, device=q.device)
         self._rc(self._lib.apg_guide_infer(self._h, _ptr(q), n, _ptr(out), self._stream(stream)))
         return out
+
+    def guide_train_queries(self, q, dirs, weight, apply=True, want_grad=False, stream=None):
+        """One training step from n x 9 queries (encoding fused into the training kernel)."""
+        n = q.shape[0]
+        loss = C.c_float()
+        g = np.empty(self._lib.apg_guide_param_count(), np.float32) if want_grad else None
+        self._rc(self._lib.apg_guide_train_queries(self._h, _ptr(q), _ptr(dirs), _ptr(weight), n,
+                                                   1 if apply else 0, C.byref(loss),
+                                                   None if g is None else _ptr(g), self._stream(stream)))
+        return loss.value, g
 
     def guide_train_step(self, feat, dirs, weight, apply=True, want_grad=False, stream=None):
         n = feat.shape[0]

@@ -62,6 +62,7 @@ struct GuideNet
     // training buffers (capacity in samples)
     uint32_t cap = 0;
     uint16_t* acts = nullptr;  // cap x 4 x 64
+    uint16_t* feat16 = nullptr; // cap x 96 (fused training path)
     float* out = nullptr;      // cap x 48
     uint16_t* dz = nullptr;    // cap x (48 + 4 x 64)
 };
@@ -326,6 +327,182 @@ __global__ __launch_bounds__(BLOCK) void k_guide_infer(InferArgs A)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Training step, fused: encoding, forward, loss head and backward through the
+// layers for 16-query tiles, one tile per wave at a time, activations and
+// gradients in registers in the transposed tile layout of k_guide_infer.
+// Writes what the weight-gradient kernel reads: the BF16 features, the hidden
+// activations and the pre-activation gradients of every layer.
+// ---------------------------------------------------------------------------
+struct TrainArgs
+{
+    const uint16_t* w16;
+    const float* w32;
+    const float* q;        // M x 9
+    const float* dir;      // M x 3
+    const float* weight;   // M
+    const float* wsum;     // device: sum of the minibatch's weights
+    uint32_t M;
+    uint16_t* feat16;      // M x 96
+    uint16_t* acts;        // M x 256
+    uint16_t* dz;          // M x (48 + 256)
+    float* loss_sum;       // optional
+};
+
+__global__ __launch_bounds__(BLOCK) void k_guide_train_fused(TrainArgs A)
+{
+    __shared__ __align__(16) uint16_t sw[INFER_WSZ];
+    __shared__ float sb[GUIDE_PARAMS - 21504];
+    __shared__ float sh[WAVES][16 * GUIDE_OUT];
+    for (int l = 0; l < GUIDE_LAYERS; ++l)
+    {
+        const int K = guide_k(l), N = guide_n(l);
+        for (int i = threadIdx.x; i < N * K / 4; i += BLOCK)
+        {
+            const int row = (4 * i) / K, col = (4 * i) % K;
+            *reinterpret_cast<uint2*>(sw + infer_off(l) + row * infer_row(l) + col) =
+                *reinterpret_cast<const uint2*>(A.w16 + guide_w_off(l) + 4 * i);
+        }
+    }
+    for (int i = threadIdx.x; i < GUIDE_PARAMS - 21504; i += BLOCK) sb[i] = A.w32[21504 + i];
+    __syncthreads();
+    const uint32_t M = A.M;
+    const float wsum = *A.wsum;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, kg = lane >> 4;
+    float* hw = sh[wave];
+    constexpr int DZS = GUIDE_OUT + 4 * GUIDE_HID;
+    float loss = 0.0f;
+    const uint32_t nwaves = gridDim.x * WAVES;
+    for (uint32_t tile = blockIdx.x * WAVES + wave; tile * 16u < M; tile += nwaves)
+    {
+        const uint32_t qi = tile * 16u + uint32_t(r);
+        const bool valid = qi < M;
+        float p[3] = { 0, 0, 0 }, n[3] = { 0, 0, 0 }, wo[3] = { 0, 0, 0 };
+        if (valid)
+        {
+            const float* qq = A.q + size_t(qi) * 9;
+            p[0] = qq[0]; p[1] = qq[1]; p[2] = qq[2];
+            n[0] = qq[3]; n[1] = qq[4]; n[2] = qq[5];
+            wo[0] = qq[6]; wo[1] = qq[7]; wo[2] = qq[8];
+        }
+        // ---- forward ----
+        s4v h0[GUIDE_IN / 16];
+#pragma unroll
+        for (int ks = 0; ks < GUIDE_IN / 16; ++ks)
+        {
+            const int f = ks * 16 + 4 * kg;
+            const uint32_t lo = pack_bf2(guide_feature(f, p, n, wo), guide_feature(f + 1, p, n, wo));
+            const uint32_t hi = pack_bf2(guide_feature(f + 2, p, n, wo), guide_feature(f + 3, p, n, wo));
+            h0[ks] = as_s4v(lo, hi);
+            if (valid) *reinterpret_cast<uint2*>(A.feat16 + size_t(qi) * GUIDE_IN + f) = make_uint2(lo, hi);
+        }
+        f4v t[4];
+        s4v h[4][4];  // hidden layer outputs (post-ReLU, BF16), layer-major
+        infer_layer<0, GUIDE_IN, GUIDE_HID, true>(sw, sb, h0, t, r, kg);
+        to_bf16_tiles(t, h[0]);
+        infer_layer<1, GUIDE_HID, GUIDE_HID, true>(sw, sb, h[0], t, r, kg);
+        to_bf16_tiles(t, h[1]);
+        infer_layer<2, GUIDE_HID, GUIDE_HID, true>(sw, sb, h[1], t, r, kg);
+        to_bf16_tiles(t, h[2]);
+        infer_layer<3, GUIDE_HID, GUIDE_HID, true>(sw, sb, h[2], t, r, kg);
+        to_bf16_tiles(t, h[3]);
+        if (valid)
+        {
+#pragma unroll
+            for (int l = 0; l < 4; ++l)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                {
+                    const s4v v = h[l][nt];
+                    const uint32_t lo = uint32_t(uint16_t(v[0])) | (uint32_t(uint16_t(v[1])) << 16);
+                    const uint32_t hi = uint32_t(uint16_t(v[2])) | (uint32_t(uint16_t(v[3])) << 16);
+                    *reinterpret_cast<uint2*>(A.acts + size_t(qi) * (4 * GUIDE_HID) + l * GUIDE_HID + nt * 16 + 4 * kg) =
+                        make_uint2(lo, hi);
+                }
+        }
+        f4v o[GUIDE_OUT / 16];
+        infer_layer<4, GUIDE_HID, GUIDE_OUT, false>(sw, sb, h[3], o, r, kg);
+        // ---- loss head: one lane per query ----
+#pragma unroll
+        for (int nt = 0; nt < GUIDE_OUT / 16; ++nt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) hw[r * GUIDE_OUT + nt * 16 + 4 * kg + i] = o[nt][i];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (kg == 0)
+        {
+            float g[GUIDE_OUT];
+            if (valid)
+            {
+                const float scale = wsum > 0.0f ? A.weight[qi] / wsum : 0.0f;
+                const f3 d = mk(A.dir[3 * size_t(qi)], A.dir[3 * size_t(qi) + 1], A.dir[3 * size_t(qi) + 2]);
+                loss += guide_nll_grad(hw + r * GUIDE_OUT, d, scale, g) * scale;
+            }
+            else
+                for (int k = 0; k < GUIDE_OUT; ++k) g[k] = 0.0f;
+            for (int k = 0; k < GUIDE_OUT; ++k) hw[r * GUIDE_OUT + k] = g[k];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // dZ5 in the tile layout, rounded to BF16 as stored
+        s4v d5[GUIDE_OUT / 16];
+#pragma unroll
+        for (int nt = 0; nt < GUIDE_OUT / 16; ++nt)
+        {
+            const float* gp = hw + r * GUIDE_OUT + nt * 16 + 4 * kg;
+            const uint32_t lo = pack_bf2(gp[0], gp[1]), hi = pack_bf2(gp[2], gp[3]);
+            d5[nt] = as_s4v(lo, hi);
+            if (valid) *reinterpret_cast<uint2*>(A.dz + size_t(qi) * DZS + nt * 16 + 4 * kg) = make_uint2(lo, hi);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // ---- backward through layers 4 .. 1: dA[k][q] = sum_n W[n][k] dZ[n][q] ----
+        s4v dcur[4];
+        s4v dnext[4];
+        auto back = [&](const uint16_t* W, int rowlen, int ksteps, const s4v* din, int hl) {
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt)
+            {
+                f4v acc = { 0.0f, 0.0f, 0.0f, 0.0f };
+                for (int ks = 0; ks < ksteps; ++ks)
+                {
+                    // A[m = input unit k][kdim = output unit n] = W[n][k]
+                    const uint16_t* wp = W + (ks * 16 + 4 * kg) * rowlen + kt * 16 + r;
+                    s4v a;
+                    a[0] = short(wp[0]);
+                    a[1] = short(wp[rowlen]);
+                    a[2] = short(wp[2 * rowlen]);
+                    a[3] = short(wp[3 * rowlen]);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, din[ks], acc, 0, 0, 0);
+                }
+                // lane holds dA for input units 16 kt + 4 kg + i of query r; ReLU mask of the
+                // hidden layer that produced them
+                const s4v act = h[hl][kt];
+                float v[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) v[i] = bf2f(uint16_t(act[i])) > 0.0f ? acc[i] : 0.0f;
+                const uint32_t lo = pack_bf2(v[0], v[1]), hi = pack_bf2(v[2], v[3]);
+                dnext[kt] = as_s4v(lo, hi);
+                if (valid)
+                    *reinterpret_cast<uint2*>(A.dz + size_t(qi) * DZS + GUIDE_OUT + hl * GUIDE_HID + kt * 16 + 4 * kg) =
+                        make_uint2(lo, hi);
+            }
+        };
+        back(sw + infer_off(4), infer_row(4), GUIDE_OUT / 16, d5, 3);
+        for (int k = 0; k < 4; ++k) dcur[k] = dnext[k];
+        back(sw + infer_off(3), infer_row(3), GUIDE_HID / 16, dcur, 2);
+        for (int k = 0; k < 4; ++k) dcur[k] = dnext[k];
+        back(sw + infer_off(2), infer_row(2), GUIDE_HID / 16, dcur, 1);
+        for (int k = 0; k < 4; ++k) dcur[k] = dnext[k];
+        back(sw + infer_off(1), infer_row(1), GUIDE_HID / 16, dcur, 0);
+    }
+    if (A.loss_sum)
+    {
+        for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off);
+        if (lane == 0) atomicAdd(A.loss_sum, loss);
+    }
+}
+
 // Loss head: dZ5 = d(-w log q)/d out, scaled by 1 / sum(w).
 __global__ void k_guide_wsum(const float* weight, uint32_t M, float* scratch)
 {
@@ -434,7 +611,7 @@ __global__ __launch_bounds__(BLOCK) void k_guide_bwd_data(const uint16_t* w16, c
 // in a fixed order).  64 samples are staged per round; each wave owns up to
 // six 16 x 16 output tiles and runs four MFMA k-steps (16 samples each) per
 // tile and round.
-constexpr int WG_MAX = 32;  // partial-sum slots
+constexpr int WG_MAX = 64;  // partial-sum slots
 constexpr int WG_STEP = 64;
 constexpr int SIN_LD = GUIDE_IN + 8, SDZ_LD = GUIDE_HID + 8;
 
@@ -448,9 +625,9 @@ __device__ __forceinline__ s4v gather4_ld(const uint16_t* base, int ld)
     return r;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_guide_bwd_weight(const float* feat, const uint16_t* acts,
-                                                            const uint16_t* dz, uint32_t M, uint32_t chunk,
-                                                            float* part)
+__global__ __launch_bounds__(BLOCK) void k_guide_bwd_weight(const float* feat, const uint16_t* feat16,
+                                                            const uint16_t* acts, const uint16_t* dz,
+                                                            uint32_t M, uint32_t chunk, float* part)
 {
     const int layer = blockIdx.y;
     const int K = guide_k(layer), N = guide_n(layer);
@@ -466,24 +643,40 @@ __global__ __launch_bounds__(BLOCK) void k_guide_bwd_weight(const float* feat, c
     const uint32_t s0 = blockIdx.x * chunk;
     const uint32_t s1 = min(M, s0 + chunk);
     const int dz_off = layer == 4 ? 0 : GUIDE_OUT + layer * GUIDE_HID;
+    // rows of the layer's input and of its output gradient, moved as 8-BF16 vectors
+    const int in_vec = K / 8, dz_vec = N / 8;
     for (uint32_t sb = s0; sb < s1; sb += WG_STEP)
     {
         __syncthreads();
-        for (int i = threadIdx.x; i < WG_STEP * K; i += BLOCK)
+        for (int i = threadIdx.x; i < WG_STEP * in_vec; i += BLOCK)
         {
-            const int rr = i / K, c = i - rr * K;
+            const int rr = i / in_vec, c = (i - rr * in_vec) * 8;
             const uint32_t smp = sb + rr;
-            uint16_t v = 0;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
             if (smp < s1)
-                v = layer == 0 ? f2bf(feat[size_t(smp) * GUIDE_IN + c])
-                               : acts[size_t(smp) * (4 * GUIDE_HID) + (layer - 1) * GUIDE_HID + c];
-            s_in[rr * SIN_LD + c] = v;
+            {
+                if (layer > 0)
+                    v = *reinterpret_cast<const uint4*>(acts + size_t(smp) * (4 * GUIDE_HID) + (layer - 1) * GUIDE_HID + c);
+                else if (feat16)
+                    v = *reinterpret_cast<const uint4*>(feat16 + size_t(smp) * GUIDE_IN + c);
+                else
+                {
+                    const float* f = feat + size_t(smp) * GUIDE_IN + c;
+                    v.x = uint32_t(f2bf(f[0])) | (uint32_t(f2bf(f[1])) << 16);
+                    v.y = uint32_t(f2bf(f[2])) | (uint32_t(f2bf(f[3])) << 16);
+                    v.z = uint32_t(f2bf(f[4])) | (uint32_t(f2bf(f[5])) << 16);
+                    v.w = uint32_t(f2bf(f[6])) | (uint32_t(f2bf(f[7])) << 16);
+                }
+            }
+            *reinterpret_cast<uint4*>(&s_in[rr * SIN_LD + c]) = v;
         }
-        for (int i = threadIdx.x; i < WG_STEP * N; i += BLOCK)
+        for (int i = threadIdx.x; i < WG_STEP * dz_vec; i += BLOCK)
         {
-            const int rr = i / N, c = i - rr * N;
+            const int rr = i / dz_vec, c = (i - rr * dz_vec) * 8;
             const uint32_t smp = sb + rr;
-            s_dz[rr * SDZ_LD + c] = smp < s1 ? dz[size_t(smp) * DZS + dz_off + c] : 0;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (smp < s1) v = *reinterpret_cast<const uint4*>(dz + size_t(smp) * DZS + dz_off + c);
+            *reinterpret_cast<uint4*>(&s_dz[rr * SDZ_LD + c]) = v;
         }
         __syncthreads();
 #pragma unroll
@@ -611,7 +804,9 @@ int guide_ensure(Context* ctx, uint32_t cap)
         if (g->acts) (void)hipFree(g->acts);
         if (g->out) (void)hipFree(g->out);
         if (g->dz) (void)hipFree(g->dz);
-        g->acts = nullptr; g->out = nullptr; g->dz = nullptr;
+        if (g->feat16) (void)hipFree(g->feat16);
+        g->acts = nullptr; g->out = nullptr; g->dz = nullptr; g->feat16 = nullptr;
+        APG_HIP_CHECK(ctx, hipMalloc(&g->feat16, size_t(cap) * GUIDE_IN * 2));
         APG_HIP_CHECK(ctx, hipMalloc(&g->acts, size_t(cap) * 4 * GUIDE_HID * 2));
         APG_HIP_CHECK(ctx, hipMalloc(&g->out, size_t(cap) * GUIDE_OUT * 4));
         APG_HIP_CHECK(ctx, hipMalloc(&g->dz, size_t(cap) * (GUIDE_OUT + 4 * GUIDE_HID) * 2));
@@ -654,7 +849,7 @@ int guide_infer(Context* ctx, const float* q9, uint32_t M, const uint32_t* M_dev
 }
 
 int guide_train(Context* ctx, const float* feat, const float* dir, const float* weight, uint32_t M,
-                bool adam, float* loss, hipStream_t s, const float* wsum_dev)
+                bool adam, float* loss, hipStream_t s, const float* wsum_dev, const float* q9)
 {
     int rc = guide_ensure(ctx, M);
     if (rc) return rc;
@@ -666,14 +861,27 @@ int guide_train(Context* ctx, const float* feat, const float* dir, const float* 
         k_guide_wsum<<<std::min<uint32_t>(nb(M, 256), 1024), 256, 0, s>>>(weight, M, g->scratch);
         wsum_dev = g->scratch;
     }
-    if ((rc = guide_forward(ctx, feat, M, nullptr, g->out, g->acts, s))) return rc;
-    k_guide_head<<<nb(M, 256), 256, 0, s>>>(g->out, dir, weight, M, wsum_dev, loss ? g->scratch + 1 : nullptr,
-                                            g->dz);
-    const uint32_t grid = std::min<uint32_t>(nb(M, 16 * WAVES), uint32_t(ctx->cu_count) * 4);
-    k_guide_bwd_data<<<grid, BLOCK, 0, s>>>(g->w16, g->acts, g->dz, M);
+    const uint16_t* feat16 = nullptr;
+    if (q9)
+    {
+        // queries: encoding, forward, head and backward-data in one kernel
+        TrainArgs A{ g->w16, g->w32, q9, dir, weight, wsum_dev, M, g->feat16, g->acts, g->dz,
+                     loss ? g->scratch + 1 : nullptr };
+        const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(nb(M, 16 * WAVES), uint32_t(ctx->cu_count) * 2));
+        k_guide_train_fused<<<grid, BLOCK, 0, s>>>(A);
+        feat16 = g->feat16;
+    }
+    else
+    {
+        if ((rc = guide_forward(ctx, feat, M, nullptr, g->out, g->acts, s))) return rc;
+        k_guide_head<<<nb(M, 256), 256, 0, s>>>(g->out, dir, weight, M, wsum_dev, loss ? g->scratch + 1 : nullptr,
+                                                g->dz);
+        const uint32_t grid = std::min<uint32_t>(nb(M, 16 * WAVES), uint32_t(ctx->cu_count) * 4);
+        k_guide_bwd_data<<<grid, BLOCK, 0, s>>>(g->w16, g->acts, g->dz, M);
+    }
     const uint32_t G = std::max<uint32_t>(1u, std::min<uint32_t>(WG_MAX, nb(M, WG_STEP)));
     const uint32_t chunk = nb(nb(M, G), WG_STEP) * WG_STEP;
-    k_guide_bwd_weight<<<dim3(G, GUIDE_LAYERS), BLOCK, 0, s>>>(feat, g->acts, g->dz, M, chunk, g->part);
+    k_guide_bwd_weight<<<dim3(G, GUIDE_LAYERS), BLOCK, 0, s>>>(feat, feat16, g->acts, g->dz, M, chunk, g->part);
     AdamArgs A{ g->w32, g->w16, g->m, g->v, g->lr, 0.9f, 0.999f, 1e-8f, 1.0f, 1.0f };
     if (adam)
     {
@@ -756,7 +964,23 @@ int apg_guide_train_step(apg_context* ctx, const float* feat, const float* dir, 
     if (!ctx || (n && (!feat || !dir || !weight))) return APG_E_INVALID;
     if (n == 0) return APG_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    int rc = guide_train(ctx, feat, dir, weight, n, apply != 0, loss, s, nullptr);
+    int rc = guide_train(ctx, feat, dir, weight, n, apply != 0, loss, s, nullptr, nullptr);
+    if (rc) return rc;
+    if (grad_host)
+    {
+        APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
+        APG_HIP_CHECK(ctx, hipMemcpy(grad_host, guide_of(ctx)->grad, GUIDE_PARAMS * 4, hipMemcpyDeviceToHost));
+    }
+    return APG_OK;
+}
+
+int apg_guide_train_queries(apg_context* ctx, const float* q, const float* dir, const float* weight,
+                            uint32_t n, int apply, float* loss, float* grad_host, void* stream)
+{
+    if (!ctx || (n && (!q || !dir || !weight))) return APG_E_INVALID;
+    if (n == 0) return APG_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc = guide_train(ctx, nullptr, dir, weight, n, apply != 0, loss, s, nullptr, q);
     if (rc) return rc;
     if (grad_host)
     {

@@ -30,7 +30,7 @@ int guide_ensure(Context* ctx, uint32_t cap);
 int guide_forward(Context* ctx, const float* feat, uint32_t M, const uint32_t* M_dev, float* out,
                   uint16_t* acts, hipStream_t s);
 int guide_train(Context* ctx, const float* feat, const float* dir, const float* weight, uint32_t M,
-                bool adam, float* loss, hipStream_t s, const float* wsum_dev = nullptr);
+                bool adam, float* loss, hipStream_t s, const float* wsum_dev, const float* q9);
 int guide_infer(Context* ctx, const float* q9, uint32_t M, const uint32_t* M_dev, float* out, hipStream_t s);
 
 struct Queue
@@ -106,7 +106,7 @@ struct RenderState
     float* vrec = nullptr;    // P x vrec_depth x VREC vertex records
     uint32_t vrec_depth = 0;
     float* tin = nullptr;     // tcap x 9 training inputs (p01, n, wo)
-    float* tfeat = nullptr;   // tcap x 96
+    float* tfeat = nullptr;   // tcap x 9, minibatch order
     float* tdir = nullptr;    // tcap x 3
     float* tw = nullptr;      // tcap
     float* tdir_p = nullptr;  // tcap x 3, minibatch order
@@ -655,12 +655,12 @@ __global__ void k_postprocess(const float* accum, uint32_t n3, float spp, uint32
 // Training inputs in minibatch order: sample i goes to minibatch i % S, so
 // each of the S minibatches spreads over the whole image.
 __global__ void k_train_encode(const float* tin, const float* tdir, const float* tw, uint32_t n, uint32_t S,
-                               uint32_t B, float* feat, float* dir_p, float* w_p, float* wsum)
+                               uint32_t B, float* q_p, float* dir_p, float* w_p, float* wsum)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t o = (i % S) * B + i / S;
-    guide_encode(load3(tin, 3 * i), load3(tin, 3 * i + 1), load3(tin, 3 * i + 2), feat + size_t(o) * GUIDE_IN);
+    for (int k = 0; k < 9; ++k) q_p[9 * size_t(o) + k] = tin[9 * size_t(i) + k];
     for (int k = 0; k < 3; ++k) dir_p[3 * size_t(o) + k] = tdir[3 * size_t(i) + k];
     w_p[o] = tw[i];
     atomicAdd(&wsum[i % S], tw[i]);
@@ -1008,7 +1008,7 @@ static int ensure_guide_buffers(apg_context* ctx, RenderState* R, uint32_t depth
     int rc;
     if ((rc = get(R->gfeat, size_t(P) * 9)) || (rc = get(R->gout, size_t(P) * GUIDE_OUT)) ||
         (rc = get(R->vrec, size_t(P) * depth * VREC)) || (rc = get(R->tin, size_t(tcap) * 9)) ||
-        (rc = get(R->tfeat, size_t(tcap) * GUIDE_IN)) || (rc = get(R->tdir, size_t(tcap) * 3)) ||
+        (rc = get(R->tfeat, size_t(tcap) * 9)) || (rc = get(R->tdir, size_t(tcap) * 3)) ||
         (rc = get(R->tw, tcap)) || (rc = get(R->tdir_p, size_t(tcap) * 3)) || (rc = get(R->tw_p, tcap)) || (rc = get(R->twsum, TRAIN_MAX_STEPS)) ||
         (rc = get(R->tcount, 4)))
         return rc;
@@ -1216,9 +1216,9 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
                 for (uint32_t j = 0; j < S; ++j)
                 {
                     const uint32_t mb = (tn - j + S - 1) / S;
-                    rc = guide_train(ctx, R->tfeat + size_t(j) * B * GUIDE_IN, R->tdir_p + size_t(j) * B * 3,
-                                     R->tw_p + size_t(j) * B, mb, true,
-                                     j + 1 == S ? &R->last_loss : nullptr, stream, R->twsum + j);
+                    rc = guide_train(ctx, nullptr, R->tdir_p + size_t(j) * B * 3, R->tw_p + size_t(j) * B, mb, true,
+                                     j + 1 == S ? &R->last_loss : nullptr, stream, R->twsum + j,
+                                     R->tfeat + size_t(j) * B * 9);
                     if (rc) return rc;
                 }
             }

@@ -191,7 +191,7 @@ def render_benchmarks(builder, frames=3):
 # the frozen proposal until it reaches the target.  Both run SLOTS samples of each pixel side by
 # side and are timed around the GPU work only (the relMSE checks read the image between chunks).
 GUIDE_PLAIN_SPP, GUIDE_REF_SPP, GUIDE_SLOTS, GUIDE_CHUNK = 1024, 4096, 4, 32
-GUIDE_TRAIN_SPP, GUIDE_STEPS, GUIDE_LR, GUIDE_FRAC, GUIDE_SAMPLES = 64, 64, 3e-3, 0.5, 1 << 19
+GUIDE_TRAIN_SPP, GUIDE_STEPS, GUIDE_LR, GUIDE_FRAC, GUIDE_SAMPLES = 64, 32, 5e-3, 0.5, 1 << 19
 
 
 def guided_benchmark(builder):

@@ -278,6 +278,12 @@ int apg_guide_infer(apg_context* ctx, const float* q, uint32_t n, float* out, vo
 int apg_guide_train_step(apg_context* ctx, const float* feat, const float* dir,
                          const float* weight, uint32_t n, int apply, float* loss,
                          float* grad_host, void* stream);
+/* The same step from queries (n x 9 device: position normalised to the scene bounds, shading
+ * normal, outgoing direction), with the encoding, forward pass, loss head and backward pass
+ * through the layers fused into one kernel (the renderer's training path). */
+int apg_guide_train_queries(apg_context* ctx, const float* q, const float* dir,
+                            const float* weight, uint32_t n, int apply, float* loss,
+                            float* grad_host, void* stream);
 
 #ifdef __cplusplus
 }

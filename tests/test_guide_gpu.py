@@ -87,6 +87,34 @@ def test_gradient_matches_oracle():
         assert float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b))) > 0.995, l
 
 
+def test_fused_training_step_matches_oracle():
+    """The renderer's training path (encoding, forward, head and backward-data in one kernel)
+    against the FP64 oracle with the same BF16 rounding points."""
+    print("[gpu] guide fused training gradient", flush=True)
+    acc = _acc()
+    params = acc.guide_params().astype(np.float64)
+    rng = np.random.default_rng(9)
+    m = 2999  # not a multiple of the 16-query tile
+    p01 = rng.random((m, 3))
+    n = rng.standard_normal((m, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    wo = rng.standard_normal((m, 3))
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    q = np.concatenate([p01, n, wo], 1).astype(np.float32)
+    d = rng.standard_normal((m, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    w = rng.exponential(1.0, m).astype(np.float32)
+    loss, g = acc.guide_train_queries(_cuda(q), _cuda(d), _cuda(w), apply=False, want_grad=True)
+    feat = G.encode(q[:, 0:3].astype(np.float64), q[:, 3:6], q[:, 6:9])
+    gref = G.grad(params, feat, d.astype(np.float64), w.astype(np.float64), bf=True)
+    lref = G.loss(params, feat, d.astype(np.float64), w.astype(np.float64), bf=True)
+    cos = float(g @ gref / (np.linalg.norm(g) * np.linalg.norm(gref)))
+    rel = float(np.linalg.norm(g - gref) / np.linalg.norm(gref))
+    print("fused grad cos", cos, "rel l2", rel, "loss", loss, lref)
+    assert cos > 0.999 and rel < 3e-2
+    assert loss == pytest.approx(lref, rel=1e-2)
+
+
 def test_adam_step_matches_oracle():
     print("[gpu] guide adam", flush=True)
     acc = _acc()
