@@ -82,7 +82,7 @@ class RenderParams(C.Structure):
                 ("fp64_accum", C.c_uint32), ("background", C.c_float * 3), ("guide", C.c_uint32),
                 ("train_steps", C.c_uint32), ("guide_warmup", C.c_uint32),
                 ("guide_frac", C.c_float), ("graph", C.c_uint32), ("train_samples", C.c_uint32),
-                ("reserved", C.c_uint32 * 1)]
+                ("guide_depth", C.c_uint32)]
 
 
 MAT_DIELECTRIC, MAT_CONDUCTOR = 0, 1
@@ -462,9 +462,10 @@ class GpuAccelerator:
 
     def render(self, spp, max_depth, rr_depth=0, background=(0.0, 0.0, 0.0), fp64=False,
                guide=0, train_steps=4, guide_warmup=1, guide_frac=0.5, graph=False,
-               train_samples=0, stream=None):
+               train_samples=0, guide_depth=0, stream=None):
         p = RenderParams(spp, max_depth, rr_depth, 1 if fp64 else 0, _f3(background), guide,
-                         train_steps, guide_warmup, guide_frac, 1 if graph else 0, train_samples)
+                         train_steps, guide_warmup, guide_frac, 1 if graph else 0, train_samples,
+                         guide_depth)
         self._rc(self._lib.apg_render(self._h, C.byref(p), self._stream(stream)))
 
     def guide_stats(self):

@@ -1108,7 +1108,8 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
             S.in = in;
             S.out = out;
             S.bounce = b;
-            if (guide)
+            // frozen proposals may stop guiding after guide_depth bounces (no inference there)
+            if (guide && (guide == 1 || !prm->guide_depth || b < prm->guide_depth))
             {
                 k_guide_features<<<sgrid, 256, 0, st>>>(S, R->gfeat);
                 rc = guide_infer(ctx, R->gfeat, P, R->counts, R->gout, st);
@@ -1149,7 +1150,8 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
         put(&ctx->scene, sizeof(ctx->scene));
         put(&ctx->cfg, sizeof(ctx->cfg));
         const uintptr_t extra[] = { P, R->W, R->H, uintptr_t(R->accum), uintptr_t(R->accum64), prm->fp64_accum,
-                                    uintptr_t(R->gfeat), uintptr_t(R->gout), guide, uintptr_t(ctx->spill) };
+                                    uintptr_t(R->gfeat), uintptr_t(R->gout), guide, uintptr_t(ctx->spill),
+                                    prm->guide_depth };
         put(extra, sizeof(extra));
         if (!R->gexec || key != R->gkey)
         {

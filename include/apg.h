@@ -222,7 +222,8 @@ typedef struct apg_render_params {
                                 * guide = 2); captured again when the scene, frame or params change */
     uint32_t train_samples;    /* guide = 1: radiance samples kept per pass for training, spread
                                 * over the frame; 0 = 262144 */
-    uint32_t reserved[1];
+    uint32_t guide_depth;      /* guide = 2: path vertices below this depth use the proposal, deeper
+                                * ones BSDF sampling only (no inference); 0 = all */
 } apg_render_params;
 
 /* Allocate per-pixel samplers (seeded with pixel index + seed, alpine.cpp:259-269) and clear

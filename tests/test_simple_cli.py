@@ -1,0 +1,80 @@
+"""The `simple` command-line renderer: OBJ/MTL reading, PPM/PFM writing, end to end on the GPU."""
+import os
+
+import numpy as np
+import pytest
+
+from alpine_amd import simple
+
+OBJ = """# two materials, a quad (fanned), a triangle with negative indices
+mtllib box.mtl
+v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+usemtl red
+f 1/1/1 2/2/2 3/3/3 4/4/4
+usemtl white
+v 0 0 1
+v 1 0 1
+v 0 1 1
+f -3 -2 -1
+"""
+MTL = """newmtl red
+Kd 0.8 0.1 0.1
+newmtl white
+Kd 0.7 0.7 0.7
+"""
+
+
+def _write(tmp_path):
+    (tmp_path / "box.obj").write_text(OBJ)
+    (tmp_path / "box.mtl").write_text(MTL)
+    return str(tmp_path / "box.obj")
+
+
+def test_obj_reader(tmp_path):
+    v, t, m, colours = simple.read_obj(_write(tmp_path))
+    assert v.shape == (7, 3)
+    np.testing.assert_array_equal(t, [[0, 1, 2], [0, 2, 3], [4, 5, 6]])
+    np.testing.assert_array_equal(m, [0, 0, 1])
+    assert colours == [(0.8, 0.1, 0.1), (0.7, 0.7, 0.7)]
+
+
+def test_image_writers(tmp_path):
+    frame = np.arange(2 * 3 * 3, dtype=np.uint8).reshape(2, 3, 3)
+    simple.write_ppm(str(tmp_path / "a.ppm"), frame)
+    lines = (tmp_path / "a.ppm").read_text().split("\n")
+    assert lines[:3] == ["P3", "3 2", "255"] and lines[3] == "0 1 2" and lines[8] == "15 16 17"
+    rgb = np.random.default_rng(0).random((2, 3, 3)).astype(np.float32)
+    simple.write_pfm(str(tmp_path / "a.pfm"), rgb)
+    raw = (tmp_path / "a.pfm").read_bytes()
+    head = b"PF\n3 2\n-1.0\n"
+    assert raw.startswith(head)
+    back = np.frombuffer(raw[len(head):], "<f4").reshape(2, 3, 3)[::-1]
+    np.testing.assert_array_equal(back, rgb)
+
+
+def test_rejects_bad_arguments(tmp_path, capsys):
+    assert simple.main(["-i", "x.glb", "-o", "y.ppm", "--spp", "1", "--accelerator", "gpu",
+                        "--lightSampler", "bvh"]) == 1
+    assert simple.main(["-i", "x.obj", "-o", "y.png", "--spp", "1", "--accelerator", "gpu",
+                        "--lightSampler", "bvh"]) == 1
+
+
+@pytest.mark.gpu
+def test_simple_renders_an_obj(tmp_path):
+    print("[gpu] simple cli", flush=True)
+    obj = _write(tmp_path)
+    out = str(tmp_path / "out.ppm")
+    assert simple.main(["-i", obj, "-o", out, "--spp", "4", "--accelerator", "gpu",
+                        "--lightSampler", "bvh"]) == 0
+    lines = open(out).read().split("\n")
+    assert lines[:3] == ["P3", "256 256", "255"]
+    px = np.array([list(map(int, l.split())) for l in lines[3:3 + 256 * 256]])
+    assert px.shape == (65536, 3) and px.max() == 255  # the white background
+    assert (px < 250).any()  # the red quad
+    out2 = str(tmp_path / "out.pfm")
+    assert simple.main(["-i", obj, "-o", out2, "--spp", "4", "--accelerator", "gpu",
+                        "--lightSampler", "uniform"]) == 0
+    assert os.path.getsize(out2) > 256 * 256 * 12
