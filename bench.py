@@ -234,7 +234,7 @@ def guided_benchmark(builder):
     curve_g = [(tot, t_g, scenes.rel_mse(img / tot, ref))]
     reach = t_g if curve_g[0][2] <= target else None
     while reach is None and tot < 4 * GUIDE_PLAIN_SPP:
-        t_g += timed(lambda: acc.render(GUIDE_CHUNK, sc.max_depth, guide=2, guide_frac=GUIDE_FRAC,
+        t_g += timed(lambda: acc.render(GUIDE_CHUNK // 2, sc.max_depth, guide=2, guide_frac=GUIDE_FRAC,
                                         graph=True))
         img, _, tot = acc.render_read()
         e = scenes.rel_mse(img / tot, ref)
