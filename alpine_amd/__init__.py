@@ -111,7 +111,7 @@ def camera(eye, target, up, fovy, aspect):
 class TraceConfig(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("fetch_threshold", C.c_uint32), ("stack_lds", C.c_uint32),
-                ("reserved", C.c_uint32 * 4)]
+                ("tri_batch", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 _lib = None
@@ -293,7 +293,7 @@ class GpuAccelerator:
         self._rc(self._lib.apg_mesh_prim_offset(self._h, shape_id, C.byref(o)))
         return o.value
 
-    def update_scene(self, leaf_threshold: int = 4, builder: int = BUILDER_GPU,
+    def update_scene(self, leaf_threshold: int = 2, builder: int = BUILDER_GPU,
                      per_shape: bool = False, stats: bool = True):
         """Rebuild (Accelerator::updateScene).  per_shape: an independent tree per shape
         (GPU builder), wide node s the root of shape s; stats=False skips the host-side

@@ -224,7 +224,7 @@ int apg_scene_build(apg_context* ctx, const apg_build_params* params)
     if (!ctx) return APG_E_INVALID;
     APG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     auto t0 = std::chrono::steady_clock::now();
-    uint32_t leaf_threshold = params && params->leaf_threshold ? params->leaf_threshold : 4u;
+    uint32_t leaf_threshold = params && params->leaf_threshold ? params->leaf_threshold : 2u;
     uint32_t builder = params ? params->builder : APG_BUILDER_GPU;
     const uint32_t roots = params ? params->roots : APG_ROOTS_SCENE;
     const bool stats = !(params && (params->flags & APG_BUILD_NO_STATS));
@@ -307,6 +307,7 @@ int apg_trace_config_default(const apg_context* ctx, apg_trace_config* cfg)
     cfg->blocks_per_cu = 6;
     cfg->fetch_threshold = 32;
     cfg->stack_lds = 8;
+    cfg->tri_batch = 48;
     return APG_OK;
 }
 
@@ -321,6 +322,8 @@ int apg_set_trace_config(apg_context* ctx, const apg_trace_config* cfg)
         return ctx->fail(APG_E_INVALID, "fetch_threshold must be in 1..64");
     if (cfg->stack_lds != 8 && cfg->stack_lds != 16)
         return ctx->fail(APG_E_INVALID, "stack_lds must be 8 or 16");
+    if (cfg->tri_batch == 0 || cfg->tri_batch > 64)
+        return ctx->fail(APG_E_INVALID, "tri_batch must be in 1..64");
     ctx->cfg = *cfg;
     return APG_OK;
 }

@@ -31,12 +31,23 @@ namespace apg {
 constexpr uint32_t APG_STACK_MAX = 64;
 constexpr uint32_t APG_SPILL_ALLOC = 56;  // spill entries allocated per thread (>= 64 - 8)
 
+// Triangle-group queue entries per wave: room for 64 new groups on top of the
+// 32 a wave may hold between rounds.
+constexpr uint32_t TRACE_QCAP = 96;
+
 // LDS words per wave: the stack's top (2 words per entry and lane), 64 keys of
-// 2 words and 64 barycentric pairs.  With an 8-entry top this is 5 KiB, so 32
-// waves fit in a CU's 160 KiB.
+// 2 words, 64 barycentric pairs, 64 pending counts and the triangle-group queue
+// (3 words and an owner byte per entry): 6.5 KiB with an 8-entry stack top, so
+// six 256-thread workgroups fit in a CU's 160 KiB.
 __host__ __device__ constexpr uint32_t trace_wave_lds_words(uint32_t stack)
 {
-    return stack * 128u + 128u + 128u;
+    return stack * 128u + 128u + 128u + 64u + 3u * TRACE_QCAP + TRACE_QCAP / 4u;
+}
+
+__device__ __forceinline__ uint32_t queue_slot(uint32_t head, uint32_t k)
+{
+    const uint32_t i = head + k;
+    return i >= TRACE_QCAP ? i - TRACE_QCAP : i;
 }
 
 struct TraceArgs
@@ -57,6 +68,7 @@ struct TraceArgs
     uint64_t* spill;
     unsigned long long* fetch_counts; // [0] nodes, [1] triangles
     uint32_t fetch_threshold;
+    uint32_t tri_batch;               // queued triangle tests that start a round
 };
 
 // Inclusive prefix sum over the 64 lanes of a wave: shift-and-add inside each
@@ -83,7 +95,7 @@ __device__ __forceinline__ float lane_f32(float v, uint32_t src)
 }
 
 template <bool ANY, bool COUNT, uint32_t APG_STACK_LDS>
-__global__ __launch_bounds__(256, 8) void trace_kernel(TraceArgs A)
+__global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
 {
     constexpr uint32_t APG_STACK_SPILL = APG_STACK_MAX - APG_STACK_LDS;
     extern __shared__ uint32_t lds[];
@@ -92,25 +104,31 @@ __global__ __launch_bounds__(256, 8) void trace_kernel(TraceArgs A)
     uint32_t* wl = lds + wave * trace_wave_lds_words(APG_STACK_LDS);
     uint32_t* st_base = wl;
     uint32_t* st_bits = st_base + APG_STACK_LDS * 64u;
-    // per owner lane: best key of the current triangle group (|t| bits, then
-    // count-mask bit, then the sign of t) and the winning hit's b0, b1
+    // per lane (ray): best hit key so far (|t| bits, then triangle index, then the
+    // sign of t), the winning hit's b0, b1 and the number of its queued triangle tests
     unsigned long long* keys = reinterpret_cast<unsigned long long*>(wl + APG_STACK_LDS * 128u);
     float* rec = reinterpret_cast<float*>(wl + APG_STACK_LDS * 128u + 128u);
+    uint32_t* pend = wl + APG_STACK_LDS * 128u + 256u;
+    // the wave's ring of triangle groups: base, count mask and pending mask of each
+    // group, and the lane that owns it
+    uint32_t* queue = wl + APG_STACK_LDS * 128u + 320u;
+    uint8_t* q_owner = reinterpret_cast<uint8_t*>(queue + 3u * TRACE_QCAP);
     uint64_t* spill = A.spill + size_t(blockIdx.x * blockDim.x + threadIdx.x) * APG_SPILL_ALLOC;
 
     const uint32_t nrays = A.n_dev ? *A.n_dev : A.n;
+    const uint32_t batch = A.tri_batch;
     bool active = false;
     bool exhausted = false;  // wave-uniform
     uint32_t rid = 0;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0;
     RayBox rb{};
     float tmax = 0;
-    uint32_t best = 0xFFFFFFFFu;
-    float hb0 = 0.0f, hb1 = 0.0f;
     uint32_t g_base = 0, g_bits = 0;
     uint32_t sp = 0;
-    bool found = false;
     uint32_t n_node = 0, n_tri = 0;
+    uint32_t q_head = 0, q_count = 0, q_items = 0;  // wave-uniform
+    pend[lane] = 0u;
+    keys[lane] = ~0ull;
 
     for (;;)
     {
@@ -135,9 +153,7 @@ __global__ __launch_bounds__(256, 8) void trace_kernel(TraceArgs A)
                     dx = A.rays.dx[rid]; dy = A.rays.dy[rid]; dz = A.rays.dz[rid];
                     tmax = A.rays.tmax ? A.rays.tmax[rid] : FLT_MAX;
                     rb = make_raybox(ox, oy, oz, dx, dy, dz);
-                    best = 0xFFFFFFFFu;
-                    hb0 = hb1 = 0.0f;
-                    found = false;
+                    keys[lane] = ~0ull;
                     sp = 0;
                     // Root as a group holding one inner child in slot 0.
                     g_base = 0;
@@ -195,109 +211,144 @@ __global__ __launch_bounds__(256, 8) void trace_kernel(TraceArgs A)
             node_visit(w, rb, tmax, g_base, g_bits, t_base, t_mask, t_cmask);
         }
 
-        // ---- the wave's triangle tests, spread over all 64 lanes ----
-        // Lane L owns popcount(t_mask) tests; work item j (lane j % 64 of
-        // round j / 64) belongs to the owner whose inclusive count first
-        // exceeds j.  Each owner keeps the smallest (t, slot bit) key, which
-        // is the hit a sequential pass over its group in bit order accepts.
-        const uint32_t cnt = bit_popc(t_mask);
-        const uint32_t incl = wave_incl_scan(cnt);
-        const uint32_t total = uint32_t(__builtin_amdgcn_readlane(int(incl), 63));
-        if (total != 0u)
+        // ---- queue this iteration's triangle groups ----
         {
-            if (cnt) keys[lane] = ~0ull;
-            for (uint32_t j0 = 0; j0 < total; j0 += 64u)
+            const uint32_t cnt = bit_popc(t_mask);
+            const uint64_t m = __ballot(cnt != 0u);
+            if (m)
             {
-                const uint32_t j = j0 + lane;
-                uint32_t o = 0;
-#pragma unroll
-                for (uint32_t step = 32u; step; step >>= 1)
-                    if (lane_u32(incl, o + step - 1u) <= j) o += step;
-                o = o > 63u ? 63u : o;
-                const uint32_t otm = lane_u32(t_mask, o);
-                const uint32_t oexcl = lane_u32(incl, o) - bit_popc(otm);
-                const uint32_t obase = lane_u32(t_base, o), ocm = lane_u32(t_cmask, o);
-                const float rox = lane_f32(ox, o), roy = lane_f32(oy, o), roz = lane_f32(oz, o);
-                const float rdx = lane_f32(dx, o), rdy = lane_f32(dy, o), rdz = lane_f32(dz, o);
-                const float rtmax = lane_f32(tmax, o);
-                bool hit = false;
-                float t = 0.0f, b0 = 0.0f, b1 = 0.0f;
-                unsigned long long key = 0;
-                if (j < total)
+                if (cnt)
                 {
-                    const uint32_t p = select_bit(otm, j - oexcl);
-                    uint32_t tri = tri_index(obase, ocm, p);
-                    tri = tri < A.tri_count ? tri : A.tri_count - 1u;
-                    if (COUNT) n_tri++;
-                    hit = tri_hit(A.tris + size_t(tri) * APG_TRI_FLOATS, rox, roy, roz, rdx, rdy, rdz,
-                                  rtmax, t, b0, b1);
-                    if (hit)
-                    {
-                        const uint32_t tb = __float_as_uint(t);
-                        key = (uint64_t(tb & 0x7fffffffu) << 32) | (p << 1) | (tb >> 31);
-                        atomicMin(&keys[o], key);
-                    }
+                    const uint32_t pos = __popcll(m & ((1ull << lane) - 1ull));
+                    const uint32_t slot = queue_slot(q_head, q_count + pos);
+                    queue[3u * slot] = t_base;
+                    queue[3u * slot + 1u] = t_cmask;
+                    queue[3u * slot + 2u] = t_mask;
+                    q_owner[slot] = uint8_t(lane);
+                    pend[lane] += cnt;
                 }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                if (hit && keys[o] == key)
-                {
-                    rec[2u * o] = b0;
-                    rec[2u * o + 1u] = b1;
-                }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            }
-            if (cnt)
-            {
-                const unsigned long long k = keys[lane];
-                if (k != ~0ull)
-                {
-                    if (ANY)
-                    {
-                        found = true;
-                        g_bits = 0;
-                        sp = 0;
-                    }
-                    else
-                    {
-                        tmax = __uint_as_float(uint32_t(k >> 32) | (uint32_t(k) << 31));
-                        hb0 = rec[2u * lane];
-                        hb1 = rec[2u * lane + 1u];
-                        best = tri_index(t_base, t_cmask, (uint32_t(k) >> 1) & 31u);
-                    }
-                }
+                q_count += uint32_t(__popcll(m));
+                q_items += uint32_t(__builtin_amdgcn_readlane(int(wave_incl_scan(cnt)), 63));
             }
         }
 
-        // ---- next group ----
-        if (active && (g_bits >> 8) == 0u)
+        // ---- next node group from the stack ----
+        if (active && (g_bits >> 8) == 0u && sp != 0u)
         {
-            if (sp == 0u)
+            sp--;
+            const uint32_t ring = (sp % APG_STACK_LDS) * 64u + lane;
+            g_base = st_base[ring];
+            g_bits = st_bits[ring];
+            if (sp >= APG_STACK_LDS)
+            {
+                const uint64_t e = spill[sp - APG_STACK_LDS];
+                st_base[ring] = uint32_t(e);
+                st_bits[ring] = uint32_t(e >> 32);
+            }
+        }
+
+        // ---- triangle rounds: 64 queued tests at a time over all lanes ----
+        // A round runs once `batch` tests are queued, when no lane has node work
+        // left, or to keep room in the queue.  Item j belongs to the queued group
+        // whose inclusive count first exceeds j; every owner keeps the smallest
+        // key, i.e. the closest hit (exact-t ties go to the lower triangle index).
+        const bool any_work = __ballot(active && (g_bits >> 8) != 0u) != 0ull;
+        while (q_items >= batch || (q_items != 0u && !any_work) || q_count > TRACE_QCAP - 64u)
+        {
+            const uint32_t slot = queue_slot(q_head, lane);
+            uint32_t gb = 0, gcm = 0, gm = 0, gown = 0;
+            if (lane < q_count)
+            {
+                gb = queue[3u * slot];
+                gcm = queue[3u * slot + 1u];
+                gm = queue[3u * slot + 2u];
+                gown = q_owner[slot];
+            }
+            const uint32_t c = bit_popc(gm);
+            const uint32_t incl = wave_incl_scan(c);
+            const uint32_t total = uint32_t(__builtin_amdgcn_readlane(int(incl), 63));
+            const uint32_t take = total < 64u ? total : 64u;
+            uint32_t o = 0;
+#pragma unroll
+            for (uint32_t step = 32u; step; step >>= 1)
+                if (lane_u32(incl, o + step - 1u) <= lane) o += step;
+            o = o > 63u ? 63u : o;
+            const uint32_t om = lane_u32(gm, o);
+            const uint32_t oexcl = lane_u32(incl, o) - bit_popc(om);
+            const uint32_t ob = lane_u32(gb, o), ocm = lane_u32(gcm, o), own = lane_u32(gown, o);
+            const float rox = lane_f32(ox, own), roy = lane_f32(oy, own), roz = lane_f32(oz, own);
+            const float rdx = lane_f32(dx, own), rdy = lane_f32(dy, own), rdz = lane_f32(dz, own);
+            const float rtmax = lane_f32(tmax, own);
+            bool hit = false;
+            float t = 0.0f, b0 = 0.0f, b1 = 0.0f;
+            unsigned long long key = 0;
+            if (lane < take)
+            {
+                const uint32_t p = select_bit(om, lane - oexcl);
+                uint32_t tri = tri_index(ob, ocm, p);
+                tri = tri < A.tri_count ? tri : A.tri_count - 1u;
+                if (COUNT) n_tri++;
+                hit = tri_hit(A.tris + size_t(tri) * APG_TRI_FLOATS, rox, roy, roz, rdx, rdy, rdz, rtmax, t, b0, b1);
+                if (hit)
+                {
+                    const uint32_t tb = __float_as_uint(t);
+                    key = (uint64_t(tb & 0x7fffffffu) << 32) | (tri << 1) | (tb >> 31);
+                    atomicMin(&keys[own], key);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            if (hit && keys[own] == key)
+            {
+                rec[2u * own] = b0;
+                rec[2u * own + 1u] = b1;
+            }
+            // retire what this round consumed: whole groups from the head, and the
+            // lowest set bits of the group the round ended in
+            const uint32_t excl = incl - c;
+            const uint32_t cons = take > excl ? (take - excl < c ? take - excl : c) : 0u;
+            if (cons) atomicSub(&pend[gown], cons);
+            const bool full = c != 0u && cons == c;
+            if (cons && !full)
+                queue[3u * slot + 2u] = gm & ~((1u << select_bit(gm, cons)) - 1u);
+            const uint32_t popped = uint32_t(__popcll(__ballot(full)));
+            q_head = queue_slot(q_head, popped);
+            q_count -= popped;
+            q_items -= take;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+
+        // ---- hits found so far tighten tmax (any hit: stop) ----
+        if (active)
+        {
+            const unsigned long long k = keys[lane];
+            if (k != ~0ull)
             {
                 if (ANY)
-                    A.occluded[rid] = found ? 1u : 0u;
-                else
                 {
-                    const bool hit = best != 0xFFFFFFFFu;
-                    A.hits.prim[rid] = hit ? A.tri_ids[best] : 0xFFFFFFFFu;
-                    A.hits.t[rid] = hit ? tmax : FLT_MAX;
-                    A.hits.b0[rid] = hit ? hb0 : 0.0f;
-                    A.hits.b1[rid] = hit ? hb1 : 0.0f;
+                    g_bits = 0;
+                    sp = 0;
                 }
-                active = false;
+                else
+                    tmax = __uint_as_float(uint32_t(k >> 32));
             }
+        }
+
+        // ---- finish: no node work and no queued tests left ----
+        if (active && (g_bits >> 8) == 0u && sp == 0u && pend[lane] == 0u)
+        {
+            const unsigned long long k = keys[lane];
+            const bool hit = k != ~0ull;
+            if (ANY)
+                A.occluded[rid] = hit ? 1u : 0u;
             else
             {
-                sp--;
-                const uint32_t ring = (sp % APG_STACK_LDS) * 64u + lane;
-                g_base = st_base[ring];
-                g_bits = st_bits[ring];
-                if (sp >= APG_STACK_LDS)
-                {
-                    const uint64_t e = spill[sp - APG_STACK_LDS];
-                    st_base[ring] = uint32_t(e);
-                    st_bits[ring] = uint32_t(e >> 32);
-                }
+                const uint32_t tri = (uint32_t(k) >> 1);
+                A.hits.prim[rid] = hit ? A.tri_ids[tri] : 0xFFFFFFFFu;
+                A.hits.t[rid] = hit ? __uint_as_float(uint32_t(k >> 32) | (uint32_t(k) << 31)) : FLT_MAX;
+                A.hits.b0[rid] = hit ? rec[2u * lane] : 0.0f;
+                A.hits.b1[rid] = hit ? rec[2u * lane + 1u] : 0.0f;
             }
+            active = false;
         }
     }
     if (COUNT)
@@ -389,6 +440,7 @@ int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint3
     A.spill = ctx->spill;
     A.fetch_counts = counts;
     A.fetch_threshold = cfg.fetch_threshold;
+    A.tri_batch = cfg.tri_batch >= 1u && cfg.tri_batch <= 64u ? cfg.tri_batch : 48u;
 
     const uint32_t stack = cfg.stack_lds == 16 ? 16u : 8u;
     const size_t shmem = size_t(block / 64) * trace_wave_lds_words(stack) * sizeof(uint32_t);

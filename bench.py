@@ -381,7 +381,7 @@ def main():
     acc = GpuAccelerator(0)
     acc.append_mesh(verts, tris)
     builder = BUILDER_GPU if args.builder == "gpu" else BUILDER_HOST
-    acc.update_scene(leaf_threshold=4, builder=builder)
+    acc.update_scene(leaf_threshold=2, builder=builder)
     st = acc.stats()
 
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a.T)).cuda()

@@ -61,7 +61,7 @@ typedef struct apg_hits {
 
 /* Which builder and which tree shape scene_build produces. */
 typedef struct apg_build_params {
-    uint32_t leaf_threshold;  /* binned-SAH leaf threshold (reference Bvh: 8); 0 = default 4 */
+    uint32_t leaf_threshold;  /* binned-SAH leaf threshold (reference Bvh: 8); 0 = default 2 */
     uint32_t builder;         /* APG_BUILDER_GPU (default) or APG_BUILDER_HOST */
     uint32_t roots;           /* APG_ROOTS_SCENE: one tree over all shapes (the reference's
                                * updateScene); APG_ROOTS_PER_SHAPE: an independent tree per shape,
@@ -97,7 +97,8 @@ typedef struct apg_trace_config {
     uint32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu */
     uint32_t fetch_threshold;  /* idle lanes a wave waits for before fetching rays */
     uint32_t stack_lds;        /* traversal stack entries per lane kept in LDS: 8 or 16 */
-    uint32_t reserved[4];
+    uint32_t tri_batch;        /* queued triangle tests per wave that start a round: 1..64 */
+    uint32_t reserved[3];
 } apg_trace_config;
 
 /* ---- lifetime (Alpine::resetScene creates the accelerator, alpine.cpp:158-176) ---- */
