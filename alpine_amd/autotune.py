@@ -14,7 +14,7 @@ import time
 
 BLOCK_SIZES = (128, 256)
 BLOCKS_PER_CU = (2, 3, 4, 5, 6, 8)
-FETCH_THRESHOLDS = (16, 24, 32, 40, 48, 64)
+FETCH_THRESHOLDS = (4, 8, 16, 24, 32, 48)
 STACK_LDS = (8, 16)
 TRI_BATCH = (32, 48, 64)
 KNOBS = ("block_size", "blocks_per_cu", "fetch_threshold", "stack_lds", "tri_batch")
@@ -23,7 +23,7 @@ KNOBS = ("block_size", "blocks_per_cu", "fetch_threshold", "stack_lds", "tri_bat
 def candidates(quick: bool = False):
     if quick:
         return [(256, b, f, s, t) for b, f, s, t in
-                itertools.product((4, 5, 6), (24, 32, 40), (8, 16), (32, 48, 64))]
+                itertools.product((4, 5, 6), (8, 16, 24), (8, 16), (32, 48, 64))]
     return list(itertools.product(BLOCK_SIZES, BLOCKS_PER_CU, FETCH_THRESHOLDS, STACK_LDS, TRI_BATCH))
 
 

@@ -305,7 +305,7 @@ int apg_trace_config_default(const apg_context* ctx, apg_trace_config* cfg)
     std::memset(cfg, 0, sizeof(*cfg));
     cfg->block_size = 256;
     cfg->blocks_per_cu = 6;
-    cfg->fetch_threshold = 32;
+    cfg->fetch_threshold = 16;
     cfg->stack_lds = 8;
     cfg->tri_batch = 48;
     return APG_OK;

@@ -105,7 +105,8 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
     uint32_t* st_base = wl;
     uint32_t* st_bits = st_base + APG_STACK_LDS * 64u;
     // per lane (ray): best hit key so far (|t| bits, then triangle index, then the
-    // sign of t), the winning hit's b0, b1 and the number of its queued triangle tests
+    // sign of t), the winning hit's b0, b1, and a pending word: queued triangle
+    // tests in the low half, lanes helping with the ray in the high half
     unsigned long long* keys = reinterpret_cast<unsigned long long*>(wl + APG_STACK_LDS * 128u);
     float* rec = reinterpret_cast<float*>(wl + APG_STACK_LDS * 128u + 128u);
     uint32_t* pend = wl + APG_STACK_LDS * 128u + 256u;
@@ -118,7 +119,9 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
     const uint32_t nrays = A.n_dev ? *A.n_dev : A.n;
     const uint32_t batch = A.tri_batch;
     bool active = false;
-    bool exhausted = false;  // wave-uniform
+    bool exhausted = false;  // wave-uniform: the cursor has passed the last ray
+    uint32_t pool_next = 0, pool_end = 0, seen = 0;  // wave-uniform: claimed indices
+    const uint32_t total_waves = gridDim.x * (blockDim.x >> 6);
     uint32_t rid = 0;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0;
     RayBox rb{};
@@ -127,48 +130,65 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
     uint32_t sp = 0;
     uint32_t n_node = 0, n_tri = 0;
     uint32_t q_head = 0, q_count = 0, q_items = 0;  // wave-uniform
+    // The lane whose ray this lane traces: itself, or, once the ray cursor is
+    // exhausted, another lane whose stack it took a node group from.
+    uint32_t owner = lane;
     pend[lane] = 0u;
     keys[lane] = ~0ull;
 
     for (;;)
     {
-        // ---- ray fetch: idle lanes take the next indices in lane order ----
+        // ---- ray fetch: idle lanes take the next indices of the wave's pool ----
+        // A wave claims a run of ray indices from the global cursor with one
+        // atomic and hands them out locally: same-address atomics serialise
+        // device-wide, so claims are sized to the rays left (guided
+        // self-scheduling: a 1/(2 x waves) share, 64..512 rays), which also
+        // keeps the last claims small and the waves' ends close together.
         const uint64_t idle = __ballot(!active);
         const uint32_t nidle = __popcll(idle);
-        if (!exhausted && (nidle >= A.fetch_threshold || nidle == 64u))
+        if (nidle >= A.fetch_threshold || nidle == 64u)
         {
-            const uint32_t leader = __ffsll((unsigned long long)idle) - 1u;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&A.counters[0], nidle);
-            base = __shfl(base, leader);
-            if (base + nidle >= nrays) exhausted = true;
-            if (!active)
+            if (pool_next == pool_end && !exhausted)
             {
-                const uint32_t my = base + __popcll(idle & ((1ull << lane) - 1ull));
-                if (my < nrays)
-                {
-                    rid = my;
-                    active = true;
-                    ox = A.rays.ox[rid]; oy = A.rays.oy[rid]; oz = A.rays.oz[rid];
-                    dx = A.rays.dx[rid]; dy = A.rays.dy[rid]; dz = A.rays.dz[rid];
-                    tmax = A.rays.tmax ? A.rays.tmax[rid] : FLT_MAX;
-                    rb = make_raybox(ox, oy, oz, dx, dy, dz);
-                    keys[lane] = ~0ull;
-                    sp = 0;
-                    // Root as a group holding one inner child in slot 0.
-                    g_base = 0;
-                    if (A.roots)
-                    {
-                        const uint32_t r = A.roots[rid];
-                        g_base = r < A.root_count ? r : A.root_count - 1u;
-                    }
-                    g_bits = A.has_scene ? root_group_bits(rb.oct) : 0u;
-                }
+                const uint32_t left = nrays > seen ? nrays - seen : 0u;
+                uint32_t chunk = (left / (2u * total_waves)) & ~63u;
+                chunk = chunk < 64u ? 64u : (chunk > 512u ? 512u : chunk);
+                uint32_t base = 0;
+                if (lane == 0u) base = atomicAdd(&A.counters[0], chunk);
+                base = uint32_t(__builtin_amdgcn_readlane(int(base), 0));
+                seen = base + chunk;
+                pool_next = base < nrays ? base : nrays;
+                pool_end = seen < nrays ? seen : nrays;
+                if (seen >= nrays) exhausted = true;
             }
+            const uint32_t avail = pool_end - pool_next;
+            const uint32_t rank = __popcll(idle & ((1ull << lane) - 1ull));
+            if (!active && rank < avail)
+            {
+                rid = pool_next + rank;
+                active = true;
+                owner = lane;
+                ox = A.rays.ox[rid]; oy = A.rays.oy[rid]; oz = A.rays.oz[rid];
+                dx = A.rays.dx[rid]; dy = A.rays.dy[rid]; dz = A.rays.dz[rid];
+                tmax = A.rays.tmax ? A.rays.tmax[rid] : FLT_MAX;
+                rb = make_raybox(ox, oy, oz, dx, dy, dz);
+                keys[lane] = ~0ull;
+                sp = 0;
+                // Root as a group holding one inner child in slot 0.
+                g_base = 0;
+                if (A.roots)
+                {
+                    const uint32_t r = A.roots[rid];
+                    g_base = r < A.root_count ? r : A.root_count - 1u;
+                }
+                g_bits = A.has_scene ? root_group_bits(rb.oct) : 0u;
+            }
+            pool_next += avail < nidle ? avail : nidle;
         }
+        const bool drained = exhausted && pool_next == pool_end;  // wave-uniform
         if (__ballot(active) == 0ull)
         {
-            if (exhausted) break;
+            if (drained) break;
             continue;
         }
 
@@ -224,8 +244,8 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
                     queue[3u * slot] = t_base;
                     queue[3u * slot + 1u] = t_cmask;
                     queue[3u * slot + 2u] = t_mask;
-                    q_owner[slot] = uint8_t(lane);
-                    pend[lane] += cnt;
+                    q_owner[slot] = uint8_t(owner);
+                    atomicAdd(&pend[owner], cnt);
                 }
                 q_count += uint32_t(__popcll(m));
                 q_items += uint32_t(__builtin_amdgcn_readlane(int(wave_incl_scan(cnt)), 63));
@@ -233,17 +253,63 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
         }
 
         // ---- next node group from the stack ----
-        if (active && (g_bits >> 8) == 0u && sp != 0u)
-        {
+        auto pop = [&](uint32_t& b, uint32_t& bits) {
             sp--;
             const uint32_t ring = (sp % APG_STACK_LDS) * 64u + lane;
-            g_base = st_base[ring];
-            g_bits = st_bits[ring];
+            b = st_base[ring];
+            bits = st_bits[ring];
             if (sp >= APG_STACK_LDS)
             {
                 const uint64_t e = spill[sp - APG_STACK_LDS];
                 st_base[ring] = uint32_t(e);
                 st_bits[ring] = uint32_t(e >> 32);
+            }
+        };
+        if (active && (g_bits >> 8) == 0u && sp != 0u) pop(g_base, g_bits);
+
+        // ---- idle lanes take node groups from other lanes' stacks ----
+        // Once no rays are left to fetch, the k-th idle lane takes the top stack
+        // entry of the k-th lane with a non-empty stack and traces that subtree
+        // for the entry's owner: its triangle groups are queued under the owner,
+        // and the owner finishes only after its helpers (counted in the high
+        // half of its pending word) are done.
+        if (drained)
+        {
+            const uint64_t idle = __ballot(!active);
+            const uint64_t donors = __ballot(active && sp != 0u);
+            if (idle != 0ull && donors != 0ull)
+            {
+                const uint32_t ni = __popcll(idle), nd = __popcll(donors);
+                const uint32_t npair = ni < nd ? ni : nd;
+                const uint64_t below = (1ull << lane) - 1ull;
+                uint32_t xb = 0, xbits = 0, src = lane;
+                if (active && sp != 0u && uint32_t(__popcll(donors & below)) < npair) pop(xb, xbits);
+                const uint32_t irank = __popcll(idle & below);
+                const bool take = !active && irank < npair;
+                if (take)
+                {
+                    const uint32_t lo = __popc(uint32_t(donors));
+                    src = irank < lo ? select_bit(uint32_t(donors), irank)
+                                     : 32u + select_bit(uint32_t(donors >> 32), irank - lo);
+                }
+                const uint32_t sb = lane_u32(xb, src), sbits = lane_u32(xbits, src);
+                const uint32_t sown = lane_u32(owner, src);
+                const float sox = lane_f32(ox, src), soy = lane_f32(oy, src), soz = lane_f32(oz, src);
+                const float sdx = lane_f32(dx, src), sdy = lane_f32(dy, src), sdz = lane_f32(dz, src);
+                const float stm = lane_f32(tmax, src);
+                if (take)
+                {
+                    active = true;
+                    owner = sown;
+                    g_base = sb;
+                    g_bits = sbits;
+                    sp = 0;
+                    ox = sox; oy = soy; oz = soz;
+                    dx = sdx; dy = sdy; dz = sdz;
+                    tmax = stm;
+                    rb = make_raybox(ox, oy, oz, dx, dy, dz);
+                    atomicAdd(&pend[owner], 1u << 16);
+                }
             }
         }
 
@@ -320,7 +386,7 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
         // ---- hits found so far tighten tmax (any hit: stop) ----
         if (active)
         {
-            const unsigned long long k = keys[lane];
+            const unsigned long long k = keys[owner];
             if (k != ~0ull)
             {
                 if (ANY)
@@ -333,8 +399,15 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
             }
         }
 
-        // ---- finish: no node work and no queued tests left ----
-        if (active && (g_bits >> 8) == 0u && sp == 0u && pend[lane] == 0u)
+        // ---- a helper is done with its subtree ----
+        if (active && owner != lane && (g_bits >> 8) == 0u && sp == 0u)
+        {
+            atomicSub(&pend[owner], 1u << 16);
+            active = false;
+        }
+
+        // ---- finish: no node work, no queued tests and no helpers left ----
+        if (active && owner == lane && (g_bits >> 8) == 0u && sp == 0u && pend[lane] == 0u)
         {
             const unsigned long long k = keys[lane];
             const bool hit = k != ~0ull;

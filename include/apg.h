@@ -95,7 +95,7 @@ typedef struct apg_scene_stats {
 typedef struct apg_trace_config {
     uint32_t block_size;       /* threads per workgroup (multiple of 64) */
     uint32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu */
-    uint32_t fetch_threshold;  /* idle lanes a wave waits for before fetching rays */
+    uint32_t fetch_threshold;  /* idle lanes a wave waits for before handing out rays */
     uint32_t stack_lds;        /* traversal stack entries per lane kept in LDS: 8 or 16 */
     uint32_t tri_batch;        /* queued triangle tests per wave that start a round: 1..64 */
     uint32_t reserved[3];
