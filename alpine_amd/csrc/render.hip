@@ -197,17 +197,57 @@ __global__ void k_guide_features(ShadeArgs A, float* q9)
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
-// Wave-aggregated append: one atomic per wave, slots handed out in lane order.
-// Every lane of the wave must call it.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
+// Inclusive prefix sum over the wave's lanes.
+__device__ __forceinline__ uint32_t wave_scan(uint32_t x)
 {
-    const uint64_t m = __ballot(pred);
-    if (m == 0ull) return 0u;
-    const uint32_t leader = __ffsll((unsigned long long)m) - 1u;
-    uint32_t base = 0;
-    if (lane_id() == leader) base = atomicAdd(counter, uint32_t(__popcll(m)));
-    base = __shfl(base, leader);
-    return base + uint32_t(__popcll(m & ((1ull << lane_id()) - 1ull)));
+#pragma unroll
+    for (uint32_t d = 1; d < 64u; d <<= 1)
+    {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane_id() >= d) x += y;
+    }
+    return x;
+}
+
+// Block-aggregated appends for k_shade: the shadow queue takes both NEE rays of
+// a path (a then b), the continuation queue one ray.  Device-scope atomics on
+// one address serialise across the chip (~11 ns each), so the whole 256-thread
+// workgroup reserves its slots with one atomic per queue.  Every thread of the
+// block must call it; `buf` alternates between calls so that a wave still
+// reading one call's totals never sees the next call's.
+struct ShadeSlots { uint32_t a, b, c; };
+__device__ __forceinline__ ShadeSlots block_append3(uint32_t* shadow_count, uint32_t* cont_count, bool ea,
+                                                    bool eb, bool ec, uint32_t buf)
+{
+    __shared__ uint32_t s_cnt[2][3][4];
+    __shared__ uint32_t s_base[2][2];
+    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint64_t ma = __ballot(ea), mb = __ballot(eb), mc = __ballot(ec);
+    if (lane == 0u)
+    {
+        s_cnt[buf][0][w] = uint32_t(__popcll(ma));
+        s_cnt[buf][1][w] = uint32_t(__popcll(mb));
+        s_cnt[buf][2][w] = uint32_t(__popcll(mc));
+    }
+    __syncthreads();
+    const uint32_t nw = blockDim.x >> 6;
+    uint32_t ta = 0, tb = 0, tc = 0, pa = 0, pb = 0, pc = 0;
+    for (uint32_t k = 0; k < nw; ++k)
+    {
+        const uint32_t ca = s_cnt[buf][0][k], cb = s_cnt[buf][1][k], cc = s_cnt[buf][2][k];
+        if (k < w) { pa += ca; pb += cb; pc += cc; }
+        ta += ca; tb += cb; tc += cc;
+    }
+    if (threadIdx.x == 0u)
+    {
+        s_base[buf][0] = ta + tb ? atomicAdd(shadow_count, ta + tb) : 0u;
+        s_base[buf][1] = tc ? atomicAdd(cont_count, tc) : 0u;
+    }
+    __syncthreads();
+    const uint32_t sb = s_base[buf][0], cb = s_base[buf][1];
+    return { sb + pa + uint32_t(__popcll(ma & below)), sb + ta + pb + uint32_t(__popcll(mb & below)),
+             cb + pc + uint32_t(__popcll(mc & below)) };
 }
 
 __device__ __forceinline__ void store3(float* a, uint32_t i, f3 v)
@@ -309,6 +349,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A)
 {
     const uint32_t count = A.counts[0];
     const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t buf = 0;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride)
     {
         const uint32_t q = base + threadIdx.x;
@@ -491,7 +532,9 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A)
             store3(A.L, p, L);
             A.state[p] = st | (ended ? ST_ENDED : 0u);
         }
-        const uint32_t ia = wave_append(&A.counts[2], emit_a);
+        const ShadeSlots sl = block_append3(&A.counts[2], &A.counts[1], emit_a, emit_b, emit_c, buf);
+        buf ^= 1u;
+        const uint32_t ia = sl.a, ib = sl.b, ic = sl.c;
         if (emit_a)
         {
             A.shadow.o[0][ia] = sa_o.x; A.shadow.o[1][ia] = sa_o.y; A.shadow.o[2][ia] = sa_o.z;
@@ -500,7 +543,6 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A)
             A.shadow.path[ia] = 2u * p;
             A.shadow.root[ia] = rt;
         }
-        const uint32_t ib = wave_append(&A.counts[2], emit_b);
         if (emit_b)
         {
             A.shadow.o[0][ib] = sa_o.x; A.shadow.o[1][ib] = sa_o.y; A.shadow.o[2][ib] = sa_o.z;
@@ -509,7 +551,6 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A)
             A.shadow.path[ib] = 2u * p + 1u;
             A.shadow.root[ib] = rt;
         }
-        const uint32_t ic = wave_append(&A.counts[1], emit_c);
         if (emit_c)
         {
             A.out.o[0][ic] = c_o.x; A.out.o[1][ic] = c_o.y; A.out.o[2][ic] = c_o.z;
@@ -556,12 +597,15 @@ __global__ void k_resolve(const uint32_t* counts, const uint32_t* qpath, uint32_
                           const uint8_t* vis, float* Lbuf, float* accum, double* accum64, int fp64, TrainOut G)
 {
     const uint32_t count = counts[0];
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x)
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride)
     {
-        const uint32_t p = qpath[q];
-        uint32_t st = state[p];
-        f3 L = load3(Lbuf, p);
-        if (st & ST_NEE)
+        const uint32_t q = base + threadIdx.x;
+        const bool valid = q < count;
+        const uint32_t p = valid ? qpath[q] : 0u;
+        uint32_t st = valid ? state[p] : 0u;
+        f3 L = valid ? load3(Lbuf, p) : mk(0.0f, 0.0f, 0.0f);
+        if (valid && (st & ST_NEE))
         {
             const float* ne = nee + 10 * size_t(p);
             f3 r = mk(0, 0, 0);
@@ -577,38 +621,55 @@ __global__ void k_resolve(const uint32_t* counts, const uint32_t* qpath, uint32_
                 vr[15] = e.x; vr[16] = e.y; vr[17] = e.z;
             }
         }
-        if (G.vrec && (st & ST_ENDED))
+        if (G.vrec)
         {
             // radiance leaving each vertex towards the previous one, back to front; each
-            // traced continuation becomes a training sample weighted by F / p = w (x) L_in
-            f3 lnext = mk(0.0f, 0.0f, 0.0f);
+            // traced continuation becomes a training sample weighted by F / p = w (x) L_in.
+            // Pass 0 counts a path's samples, one atomic reserves the wave's, pass 1 writes.
+            const bool ended = valid && (st & ST_ENDED);
             const uint32_t last = G.bounce < G.vrec_depth ? G.bounce : G.vrec_depth - 1;
-            for (int j = int(last); j >= 0; --j)
+            uint32_t t = 0;
+            for (int pass = 0; pass < 2; ++pass)
             {
-                const float* vr = G.vrec + (size_t(p) * G.vrec_depth + uint32_t(j)) * VREC;
-                const f3 e = mk(vr[15], vr[16], vr[17]);
-                f3 lo = e;
-                if (vr[18] > 0.5f)
+                uint32_t n = 0;
+                f3 lnext = mk(0.0f, 0.0f, 0.0f);
+                for (int j = ended ? int(last) : -1; j >= 0; --j)
                 {
-                    const f3 w = mk(vr[12], vr[13], vr[14]);
-                    const f3 contrib = w * lnext;
-                    lo = e + contrib;
-                    const float wt = luminance(contrib);
-                    if (wt > 0.0f && (G.keep >= 1.0f || hash01(p, uint32_t(j), G.seed) < G.keep))
+                    const float* vr = G.vrec + (size_t(p) * G.vrec_depth + uint32_t(j)) * VREC;
+                    const f3 e = mk(vr[15], vr[16], vr[17]);
+                    f3 lo = e;
+                    if (vr[18] > 0.5f)
                     {
-                        const uint32_t t = atomicAdd(G.tcount, 1u);
-                        if (t < G.tcap)
+                        const f3 w = mk(vr[12], vr[13], vr[14]);
+                        const f3 contrib = w * lnext;
+                        lo = e + contrib;
+                        const float wt = luminance(contrib);
+                        if (wt > 0.0f && (G.keep >= 1.0f || hash01(p, uint32_t(j), G.seed) < G.keep))
                         {
-                            for (int k = 0; k < 9; ++k) G.tin[9 * size_t(t) + k] = vr[k];
-                            for (int k = 0; k < 3; ++k) G.tdir[3 * size_t(t) + k] = vr[9 + k];
-                            G.tw[t] = wt;
+                            if (pass == 1 && t < G.tcap)
+                            {
+                                for (int k = 0; k < 9; ++k) G.tin[9 * size_t(t) + k] = vr[k];
+                                for (int k = 0; k < 3; ++k) G.tdir[3 * size_t(t) + k] = vr[9 + k];
+                                G.tw[t] = wt;
+                            }
+                            ++t;
+                            ++n;
                         }
                     }
+                    lnext = lo;
                 }
-                lnext = lo;
+                if (pass == 0)
+                {
+                    const uint32_t incl = wave_scan(n);
+                    const uint32_t total = uint32_t(__builtin_amdgcn_readlane(int(incl), 63));
+                    uint32_t wbase = 0;
+                    if (lane_id() == 0u && total) wbase = atomicAdd(G.tcount, total);
+                    wbase = uint32_t(__builtin_amdgcn_readlane(int(wbase), 0));
+                    t = wbase + incl - n;
+                }
             }
         }
-        if (st & ST_ENDED)
+        if (valid && (st & ST_ENDED))
         {
             accum[3 * size_t(p)] += L.x;
             accum[3 * size_t(p) + 1] += L.y;
@@ -620,7 +681,7 @@ __global__ void k_resolve(const uint32_t* counts, const uint32_t* qpath, uint32_
                 accum64[3 * size_t(p) + 2] += double(L.z);
             }
         }
-        state[p] = st;
+        if (valid) state[p] = st;
     }
 }
 
