@@ -45,6 +45,9 @@ constexpr uint32_t NSPLITS = 8;
 constexpr uint32_t BIN_VALS = 7;            // lo xyz, hi xyz (ordered uints), count
 constexpr uint32_t NODE_BIN_WORDS = 3 * NBINS * BIN_VALS;
 constexpr uint32_t RETIRED = 0xFFFFFFFFu;
+// Nodes of at most SMALL slots get one wave each for bounds and bins (no global
+// atomics); larger nodes are accumulated by chunks of CHUNK slots.
+constexpr uint32_t SMALL = CHUNK;
 
 enum : uint32_t { NF_LEAF = 1u, NF_SAH_LEAF = 2u, NF_FORCED = 4u, NF_SPLIT = 8u };
 
@@ -217,7 +220,7 @@ __global__ __launch_bounds__(BLOCK) void k_bounds(BuildState S, uint32_t n, uint
     {
         const uint32_t i = first + k;
         const uint32_t nd = (k < ITEMS && i < n) ? S.snode[i] : RETIRED;
-        const bool live = nd != RETIRED && nd >= begin && nd < end;
+        const bool live = nd != RETIRED && nd >= begin && nd < end && S.ncount[nd] > SMALL;
         if (cur != RETIRED && nd != cur)
         {
             flush_bounds(&S.nbox[6 * size_t(cur)], &S.ncbox[6 * size_t(cur)], v);
@@ -278,37 +281,54 @@ __device__ __forceinline__ float box_area(const float lo[3], const float hi[3])
 __global__ void k_decide(BuildState S, uint32_t begin, uint32_t end, uint32_t threshold)
 {
     const uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= end) return;
-    const uint32_t cnt = S.ncount[k];
-    uint32_t fl = S.nflags[k];
-    if (fl & NF_FORCED)
+    bool cand = false;
+    uint32_t cnt = 0;
+    if (k < end)
     {
-        if (cnt <= APG_MAX_LEAF) fl |= NF_LEAF;
-        else fl |= NF_SPLIT;
-        S.nflags[k] = fl;
-        return;
-    }
-    bool leaf = cnt <= threshold;
-    if (!leaf)
-    {
-        float clo[3], chi[3];
-        for (int a = 0; a < 3; ++a)
+        cnt = S.ncount[k];
+        uint32_t fl = S.nflags[k];
+        if (fl & NF_FORCED)
         {
-            clo[a] = float_of(S.ncbox[6 * size_t(k) + a]);
-            chi[a] = float_of(S.ncbox[6 * size_t(k) + 3 + a]);
+            if (cnt <= APG_MAX_LEAF) fl |= NF_LEAF;
+            else fl |= NF_SPLIT;
+            S.nflags[k] = fl;
         }
-        const float diag = max_s(chi[0] - clo[0], max_s(chi[1] - clo[1], chi[2] - clo[2]));
-        if (diag == 0.0f) leaf = true;
+        else
+        {
+            bool leaf = cnt <= threshold;
+            if (!leaf)
+            {
+                float clo[3], chi[3];
+                for (int a = 0; a < 3; ++a)
+                {
+                    clo[a] = float_of(S.ncbox[6 * size_t(k) + a]);
+                    chi[a] = float_of(S.ncbox[6 * size_t(k) + 3 + a]);
+                }
+                const float diag = max_s(chi[0] - clo[0], max_s(chi[1] - clo[1], chi[2] - clo[2]));
+                if (diag == 0.0f) leaf = true;
+            }
+            if (leaf)
+            {
+                fl |= NF_SAH_LEAF;
+                fl |= cnt <= APG_MAX_LEAF ? NF_LEAF : (NF_FORCED | NF_SPLIT);
+                S.nflags[k] = fl;
+            }
+            else
+                cand = true;
+        }
     }
-    if (leaf)
-    {
-        fl |= NF_SAH_LEAF;
-        fl |= cnt <= APG_MAX_LEAF ? NF_LEAF : (NF_FORCED | NF_SPLIT);
-        S.nflags[k] = fl;
-        return;
-    }
-    const uint32_t slot = atomicAdd(&S.counters[1], 1u);
+    // histogram slots for the SAH candidates: one counter atomic per wave
+    const uint64_t m = __ballot(cand);
+    if (m == 0ull) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t leader = __ffsll((unsigned long long)m) - 1u;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&S.counters[1], uint32_t(__popcll(m)));
+    base = uint32_t(__shfl(int(base), int(leader)));
+    if (!cand) return;
+    const uint32_t slot = base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
     S.nbinslot[k] = slot;
+    if (cnt <= SMALL) return;  // k_bin_small writes the whole histogram
     uint32_t* b = S.bins + size_t(slot) * NODE_BIN_WORDS;
     for (uint32_t j = 0; j < 3 * NBINS; ++j)
     {
@@ -327,7 +347,8 @@ __global__ __launch_bounds__(BLOCK) void k_bin(BuildState S, uint32_t n, uint32_
     const uint32_t c0 = blockIdx.x * CHUNK;
     if (c0 >= n) return;
     const uint32_t home = S.snode[c0];
-    const bool home_ok = home != RETIRED && home >= begin && home < end && S.nbinslot[home] != RETIRED;
+    const bool home_ok = home != RETIRED && home >= begin && home < end && S.ncount[home] > SMALL &&
+                         S.nbinslot[home] != RETIRED;
     for (uint32_t j = threadIdx.x; j < NODE_BIN_WORDS; j += BLOCK)
     {
         const uint32_t v = j % BIN_VALS;
@@ -344,7 +365,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin(BuildState S, uint32_t n, uint32_
         const uint32_t i = c0 + k * BLOCK + threadIdx.x;
         if (i >= n) break;
         const uint32_t nd = S.snode[i];
-        if (nd == RETIRED || nd < begin || nd >= end) continue;
+        if (nd == RETIRED || nd < begin || nd >= end || S.ncount[nd] <= SMALL) continue;
         const uint32_t slot = S.nbinslot[nd];
         if (slot == RETIRED) continue;
         const float* b = S.pbox + 6 * size_t(S.idx[i]);
@@ -383,6 +404,101 @@ __global__ __launch_bounds__(BLOCK) void k_bin(BuildState S, uint32_t n, uint32_
         else if (v < 3) atomicMin(&g[j], acc[j]);
         else atomicMax(&g[j], acc[j]);
     }
+}
+
+// Bounds of a node of at most SMALL slots: one wave walks the node's slot range
+// and reduces across lanes; the result is stored, not accumulated.
+__global__ __launch_bounds__(BLOCK) void k_bounds_small(BuildState S, uint32_t begin, uint32_t end)
+{
+    const uint32_t k = begin + (blockIdx.x * BLOCK + threadIdx.x) / 64u;
+    const uint32_t lane = threadIdx.x & 63u;
+    if (k >= end) return;
+    const uint32_t cnt = S.ncount[k];
+    if (cnt == 0u || cnt > SMALL) return;
+    const uint32_t first = S.nfirst[k];
+    const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
+    uint32_t r[12];
+    for (int a = 0; a < 3; ++a)
+    {
+        r[a] = lo_ord; r[3 + a] = hi_ord; r[6 + a] = lo_ord; r[9 + a] = hi_ord;
+    }
+    for (uint32_t i = first + lane; i < first + cnt; i += 64u)
+    {
+        const float* b = S.pbox + 6 * size_t(S.idx[i]);
+        for (int a = 0; a < 3; ++a)
+        {
+            const float lo = b[a], hi = b[3 + a], c = (lo + hi) * 0.5f;
+            r[a] = min(r[a], ord_of(lo));
+            r[3 + a] = max(r[3 + a], ord_of(hi));
+            r[6 + a] = min(r[6 + a], ord_of(c));
+            r[9 + a] = max(r[9 + a], ord_of(c));
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int a = 0; a < 3; ++a)
+        {
+            r[a] = min(r[a], uint32_t(__shfl_xor(int(r[a]), off)));
+            r[3 + a] = max(r[3 + a], uint32_t(__shfl_xor(int(r[3 + a]), off)));
+            r[6 + a] = min(r[6 + a], uint32_t(__shfl_xor(int(r[6 + a]), off)));
+            r[9 + a] = max(r[9 + a], uint32_t(__shfl_xor(int(r[9 + a]), off)));
+        }
+    if (lane == 0u)
+        for (int j = 0; j < 6; ++j)
+        {
+            S.nbox[6 * size_t(k) + j] = r[j];
+            S.ncbox[6 * size_t(k) + j] = r[6 + j];
+        }
+}
+
+// Bins of a node of at most SMALL slots: one wave, a histogram of its own in LDS,
+// written out whole.
+__global__ __launch_bounds__(BLOCK) void k_bin_small(BuildState S, uint32_t begin, uint32_t end)
+{
+    __shared__ uint32_t hist[BLOCK / 64u][NODE_BIN_WORDS];
+    const uint32_t k = begin + (blockIdx.x * BLOCK + threadIdx.x) / 64u;
+    const uint32_t lane = threadIdx.x & 63u;
+    if (k >= end) return;
+    const uint32_t cnt = S.ncount[k];
+    if (cnt == 0u || cnt > SMALL) return;
+    const uint32_t slot = S.nbinslot[k];
+    if (slot == RETIRED) return;
+    uint32_t* acc = hist[threadIdx.x / 64u];
+    for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u)
+    {
+        const uint32_t v = j % BIN_VALS;
+        acc[j] = v < 3 ? ord_of(FLT_MAX) : (v < 6 ? ord_of(-FLT_MAX) : 0u);
+    }
+    float mn[3], mx[3];
+    for (int d = 0; d < 3; ++d)
+    {
+        mn[d] = float_of(S.ncbox[6 * size_t(k) + d]);
+        mx[d] = float_of(S.ncbox[6 * size_t(k) + 3 + d]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t first = S.nfirst[k];
+    for (uint32_t i = first + lane; i < first + cnt; i += 64u)
+    {
+        const float* b = S.pbox + 6 * size_t(S.idx[i]);
+        float lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; }
+        for (int d = 0; d < 3; ++d)
+        {
+            if (mn[d] == mx[d]) continue;
+            const float c = (lo[d] + hi[d]) * 0.5f;
+            uint32_t* e = acc + (d * NBINS + bin_index(c, mn[d], mx[d])) * BIN_VALS;
+            for (int a = 0; a < 3; ++a)
+            {
+                atomicMin(&e[a], ord_of(lo[a]));
+                atomicMax(&e[3 + a], ord_of(hi[a]));
+            }
+            atomicAdd(&e[6], 1u);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    uint32_t* g = S.bins + size_t(slot) * NODE_BIN_WORDS;
+    for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u) g[j] = acc[j];
 }
 
 // Sweep, split choice and child allocation for the SAH candidates; positional
@@ -963,10 +1079,12 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
         const uint32_t width = end - begin;
         k_init_nodes<<<blocks_for(width, 256), 256, 0, s>>>(begin, end, S.nbox, S.ncbox, S.nbinslot);
         k_bounds<<<nchunks, BLOCK, 0, s>>>(S, n, begin, end);
+        k_bounds_small<<<blocks_for(width, BLOCK / 64u), BLOCK, 0, s>>>(S, begin, end);
         uint32_t zero = 0;
         APG_HIP_CHECK(ctx, hipMemcpyAsync(S.counters + 1, &zero, 4, hipMemcpyHostToDevice, s));
         k_decide<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end, thr);
         k_bin<<<nchunks, BLOCK, 0, s>>>(S, n, begin, end);
+        k_bin_small<<<blocks_for(width, BLOCK / 64u), BLOCK, 0, s>>>(S, begin, end);
         k_sweep<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end);
         k_child_first<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end);
         k_flags<<<blocks_for(n, 256), 256, 0, s>>>(S, n, begin, end);
