@@ -182,64 +182,54 @@ __global__ void k_init_nodes(uint32_t begin, uint32_t end, uint32_t* nbox, uint3
     nbinslot[k] = RETIRED;
 }
 
-// Node box and centroid box of every node in [begin, end).
-// Bounds and centroid bounds of every node of the level.  Slots are sorted by
-// node, so each thread walks ITEMS consecutive slots, keeps a running
-// min / max (as order-preserving uints, so -0 < +0 as with the atomics) and
-// flushes it once per run of equal node ids: to the workgroup's copy for the
-// chunk's first node, to the node's global entry otherwise.
-__device__ __forceinline__ void flush_bounds(uint32_t* dst_box, uint32_t* dst_cbox, const uint32_t (&v)[12])
+// Per level, every node of more than SMALL slots is cut into chunks of CHUNK
+// slots of that node alone; chunk_off (exclusive scan of the chunk counts, the
+// total at [width]) maps a workgroup to its node and chunk.
+__global__ void k_large_chunks(BuildState S, uint32_t begin, uint32_t end, uint32_t* chunk_cnt)
 {
-    for (int a = 0; a < 3; ++a)
-    {
-        atomicMin(&dst_box[a], v[a]);
-        atomicMax(&dst_box[3 + a], v[3 + a]);
-        atomicMin(&dst_cbox[a], v[6 + a]);
-        atomicMax(&dst_cbox[3 + a], v[9 + a]);
-    }
+    const uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    const uint32_t cnt = S.ncount[k];
+    chunk_cnt[k - begin] = cnt > SMALL ? (cnt + CHUNK - 1) / CHUNK : 0u;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_bounds(BuildState S, uint32_t n, uint32_t begin,
-                                                  uint32_t end)
+// The node and slot range of chunk c (false past the last chunk).
+__device__ __forceinline__ bool chunk_range(const BuildState& S, uint32_t begin, uint32_t width,
+                                            const uint32_t* chunk_off, uint32_t c, uint32_t& node,
+                                            uint32_t& lo, uint32_t& hi)
 {
-    __shared__ uint32_t acc[12];
-    const uint32_t c0 = blockIdx.x * CHUNK;
-    if (c0 >= n) return;
-    const uint32_t home = S.snode[c0];
-    if (threadIdx.x < 12) acc[threadIdx.x] = (threadIdx.x % 6) < 3 ? ord_of(FLT_MAX) : ord_of(-FLT_MAX);
-    __syncthreads();
+    if (c >= chunk_off[width]) return false;
+    uint32_t a = 0, b = width;  // largest j with chunk_off[j] <= c
+    while (b - a > 1)
+    {
+        const uint32_t m = (a + b) >> 1;
+        if (chunk_off[m] <= c) a = m;
+        else b = m;
+    }
+    node = begin + a;
+    const uint32_t first = S.nfirst[node], cnt = S.ncount[node];
+    lo = first + (c - chunk_off[a]) * CHUNK;
+    hi = lo + CHUNK < first + cnt ? lo + CHUNK : first + cnt;
+    return true;
+}
+
+// Bounds and centroid bounds of a large node's chunk: per-thread running
+// min / max as order-preserving uints (-0 < +0), reduced over the wave and the
+// workgroup, then one atomic per value into the node's entry.
+__global__ __launch_bounds__(BLOCK) void k_bounds_large(BuildState S, uint32_t begin, uint32_t width,
+                                                        const uint32_t* chunk_off)
+{
+    __shared__ uint32_t part[BLOCK / 64u][12];
+    uint32_t node, lo_i, hi_i;
+    if (!chunk_range(S, begin, width, chunk_off, blockIdx.x, node, lo_i, hi_i)) return;
     const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
-    uint32_t v[12], hv[12];
+    uint32_t r[12];
     for (int a = 0; a < 3; ++a)
     {
-        hv[a] = lo_ord; hv[3 + a] = hi_ord; hv[6 + a] = lo_ord; hv[9 + a] = hi_ord;
+        r[a] = lo_ord; r[3 + a] = hi_ord; r[6 + a] = lo_ord; r[9 + a] = hi_ord;
     }
-    uint32_t cur = RETIRED;
-    const uint32_t first = c0 + threadIdx.x * ITEMS;
-    for (uint32_t k = 0; k <= ITEMS; ++k)
+    for (uint32_t i = lo_i + threadIdx.x; i < hi_i; i += BLOCK)
     {
-        const uint32_t i = first + k;
-        const uint32_t nd = (k < ITEMS && i < n) ? S.snode[i] : RETIRED;
-        const bool live = nd != RETIRED && nd >= begin && nd < end && S.ncount[nd] > SMALL;
-        if (cur != RETIRED && nd != cur)
-        {
-            flush_bounds(&S.nbox[6 * size_t(cur)], &S.ncbox[6 * size_t(cur)], v);
-            cur = RETIRED;
-        }
-        if (!live) continue;
-        uint32_t* r = hv;
-        if (nd != home)
-        {
-            if (cur == RETIRED)
-            {
-                cur = nd;
-                for (int a = 0; a < 3; ++a)
-                {
-                    v[a] = lo_ord; v[3 + a] = hi_ord; v[6 + a] = lo_ord; v[9 + a] = hi_ord;
-                }
-            }
-            r = v;
-        }
         const float* b = S.pbox + 6 * size_t(S.idx[i]);
         for (int a = 0; a < 3; ++a)
         {
@@ -250,24 +240,25 @@ __global__ __launch_bounds__(BLOCK) void k_bounds(BuildState S, uint32_t n, uint
             r[9 + a] = max(r[9 + a], ord_of(c));
         }
     }
-    // the chunk's first node: reduce over the wave, one LDS update per wave
     for (int off = 32; off > 0; off >>= 1)
         for (int a = 0; a < 3; ++a)
         {
-            hv[a] = min(hv[a], uint32_t(__shfl_xor(int(hv[a]), off)));
-            hv[3 + a] = max(hv[3 + a], uint32_t(__shfl_xor(int(hv[3 + a]), off)));
-            hv[6 + a] = min(hv[6 + a], uint32_t(__shfl_xor(int(hv[6 + a]), off)));
-            hv[9 + a] = max(hv[9 + a], uint32_t(__shfl_xor(int(hv[9 + a]), off)));
+            r[a] = min(r[a], uint32_t(__shfl_xor(int(r[a]), off)));
+            r[3 + a] = max(r[3 + a], uint32_t(__shfl_xor(int(r[3 + a]), off)));
+            r[6 + a] = min(r[6 + a], uint32_t(__shfl_xor(int(r[6 + a]), off)));
+            r[9 + a] = max(r[9 + a], uint32_t(__shfl_xor(int(r[9 + a]), off)));
         }
-    if ((threadIdx.x & 63u) == 0u) flush_bounds(acc, acc + 6, hv);
+    if ((threadIdx.x & 63u) == 0u)
+        for (int j = 0; j < 12; ++j) part[threadIdx.x / 64u][j] = r[j];
     __syncthreads();
-    if (home != RETIRED && home >= begin && home < end && threadIdx.x < 12)
+    if (threadIdx.x < 12)
     {
-        const uint32_t val = acc[threadIdx.x];
         const uint32_t j = threadIdx.x;
-        uint32_t* dst = j < 6 ? &S.nbox[6 * size_t(home) + j] : &S.ncbox[6 * size_t(home) + (j - 6)];
-        if ((j % 6) < 3) atomicMin(dst, val);
-        else atomicMax(dst, val);
+        uint32_t v = part[0][j];
+        for (uint32_t w = 1; w < BLOCK / 64u; ++w) v = (j % 6) < 3 ? min(v, part[w][j]) : max(v, part[w][j]);
+        uint32_t* dst = j < 6 ? &S.nbox[6 * size_t(node) + j] : &S.ncbox[6 * size_t(node) + (j - 6)];
+        if ((j % 6) < 3) atomicMin(dst, v);
+        else atomicMax(dst, v);
     }
 }
 
@@ -338,54 +329,39 @@ __global__ void k_decide(BuildState S, uint32_t begin, uint32_t end, uint32_t th
     }
 }
 
-// 9-bin histograms (box + count) per axis for the SAH candidates.
-__global__ __launch_bounds__(BLOCK) void k_bin(BuildState S, uint32_t n, uint32_t begin,
-                                               uint32_t end)
+// 9-bin histograms (box + count) per axis of a large SAH candidate's chunk: one
+// LDS histogram per wave, merged into the node's global histogram.
+__global__ __launch_bounds__(BLOCK) void k_bin_large(BuildState S, uint32_t begin, uint32_t width,
+                                                     const uint32_t* chunk_off)
 {
-    __shared__ uint32_t acc[NODE_BIN_WORDS];
-    __shared__ float cmn[3], cmx[3];
-    const uint32_t c0 = blockIdx.x * CHUNK;
-    if (c0 >= n) return;
-    const uint32_t home = S.snode[c0];
-    const bool home_ok = home != RETIRED && home >= begin && home < end && S.ncount[home] > SMALL &&
-                         S.nbinslot[home] != RETIRED;
-    for (uint32_t j = threadIdx.x; j < NODE_BIN_WORDS; j += BLOCK)
+    __shared__ uint32_t hist[BLOCK / 64u][NODE_BIN_WORDS];
+    uint32_t node, lo_i, hi_i;
+    if (!chunk_range(S, begin, width, chunk_off, blockIdx.x, node, lo_i, hi_i)) return;
+    const uint32_t slot = S.nbinslot[node];
+    if (slot == RETIRED) return;
+    for (uint32_t j = threadIdx.x; j < (BLOCK / 64u) * NODE_BIN_WORDS; j += BLOCK)
     {
-        const uint32_t v = j % BIN_VALS;
-        acc[j] = v < 3 ? ord_of(FLT_MAX) : (v < 6 ? ord_of(-FLT_MAX) : 0u);
+        const uint32_t v = (j % NODE_BIN_WORDS) % BIN_VALS;
+        (&hist[0][0])[j] = v < 3 ? ord_of(FLT_MAX) : (v < 6 ? ord_of(-FLT_MAX) : 0u);
     }
-    if (home_ok && threadIdx.x < 3)
+    float mn[3], mx[3];
+    for (int d = 0; d < 3; ++d)
     {
-        cmn[threadIdx.x] = float_of(S.ncbox[6 * size_t(home) + threadIdx.x]);
-        cmx[threadIdx.x] = float_of(S.ncbox[6 * size_t(home) + 3 + threadIdx.x]);
+        mn[d] = float_of(S.ncbox[6 * size_t(node) + d]);
+        mx[d] = float_of(S.ncbox[6 * size_t(node) + 3 + d]);
     }
     __syncthreads();
-    for (uint32_t k = 0; k < ITEMS; ++k)
+    uint32_t* acc = hist[threadIdx.x / 64u];
+    for (uint32_t i = lo_i + threadIdx.x; i < hi_i; i += BLOCK)
     {
-        const uint32_t i = c0 + k * BLOCK + threadIdx.x;
-        if (i >= n) break;
-        const uint32_t nd = S.snode[i];
-        if (nd == RETIRED || nd < begin || nd >= end || S.ncount[nd] <= SMALL) continue;
-        const uint32_t slot = S.nbinslot[nd];
-        if (slot == RETIRED) continue;
         const float* b = S.pbox + 6 * size_t(S.idx[i]);
         float lo[3], hi[3];
         for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; }
-        const bool local = nd == home;
-        uint32_t* dst = local ? acc : S.bins + size_t(slot) * NODE_BIN_WORDS;
         for (int d = 0; d < 3; ++d)
         {
-            float mn, mx;
-            if (local) { mn = cmn[d]; mx = cmx[d]; }
-            else
-            {
-                mn = float_of(S.ncbox[6 * size_t(nd) + d]);
-                mx = float_of(S.ncbox[6 * size_t(nd) + 3 + d]);
-            }
-            if (mn == mx) continue;
+            if (mn[d] == mx[d]) continue;
             const float c = (lo[d] + hi[d]) * 0.5f;
-            const uint32_t bi = bin_index(c, mn, mx);
-            uint32_t* e = dst + (d * NBINS + bi) * BIN_VALS;
+            uint32_t* e = acc + (d * NBINS + bin_index(c, mn[d], mx[d])) * BIN_VALS;
             for (int a = 0; a < 3; ++a)
             {
                 atomicMin(&e[a], ord_of(lo[a]));
@@ -395,14 +371,16 @@ __global__ __launch_bounds__(BLOCK) void k_bin(BuildState S, uint32_t n, uint32_
         }
     }
     __syncthreads();
-    if (!home_ok) return;
-    uint32_t* g = S.bins + size_t(S.nbinslot[home]) * NODE_BIN_WORDS;
+    uint32_t* g = S.bins + size_t(slot) * NODE_BIN_WORDS;
     for (uint32_t j = threadIdx.x; j < NODE_BIN_WORDS; j += BLOCK)
     {
         const uint32_t v = j % BIN_VALS;
-        if (v == 6) { if (acc[j]) atomicAdd(&g[j], acc[j]); }
-        else if (v < 3) atomicMin(&g[j], acc[j]);
-        else atomicMax(&g[j], acc[j]);
+        uint32_t x = hist[0][j];
+        for (uint32_t w = 1; w < BLOCK / 64u; ++w)
+            x = v == 6 ? x + hist[w][j] : (v < 3 ? min(x, hist[w][j]) : max(x, hist[w][j]));
+        if (v == 6) { if (x) atomicAdd(&g[j], x); }
+        else if (v < 3) atomicMin(&g[j], x);
+        else atomicMax(&g[j], x);
     }
 }
 
@@ -1078,12 +1056,15 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
     {
         const uint32_t width = end - begin;
         k_init_nodes<<<blocks_for(width, 256), 256, 0, s>>>(begin, end, S.nbox, S.ncbox, S.nbinslot);
-        k_bounds<<<nchunks, BLOCK, 0, s>>>(S, n, begin, end);
+        // large nodes in chunks of their own: S.flag and S.scan are free until k_flags
+        k_large_chunks<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end, S.flag);
+        exclusive_scan(S.flag, width, S.scan, S.scan_tmp, s);
+        k_bounds_large<<<2 * nchunks + 1, BLOCK, 0, s>>>(S, begin, width, S.scan);
         k_bounds_small<<<blocks_for(width, BLOCK / 64u), BLOCK, 0, s>>>(S, begin, end);
         uint32_t zero = 0;
         APG_HIP_CHECK(ctx, hipMemcpyAsync(S.counters + 1, &zero, 4, hipMemcpyHostToDevice, s));
         k_decide<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end, thr);
-        k_bin<<<nchunks, BLOCK, 0, s>>>(S, n, begin, end);
+        k_bin_large<<<2 * nchunks + 1, BLOCK, 0, s>>>(S, begin, width, S.scan);
         k_bin_small<<<blocks_for(width, BLOCK / 64u), BLOCK, 0, s>>>(S, begin, end);
         k_sweep<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end);
         k_child_first<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end);
