@@ -48,6 +48,10 @@ constexpr uint32_t RETIRED = 0xFFFFFFFFu;
 // Nodes of at most SMALL slots get one wave each for bounds and bins (no global
 // atomics); larger nodes are accumulated by chunks of CHUNK slots.
 constexpr uint32_t SMALL = CHUNK;
+// Nodes of at most TINY slots get one thread each: bounds by a serial loop, and
+// the SAH sweep evaluates the splits from the slots directly (no histogram).
+constexpr uint32_t TINY = 16;
+constexpr uint32_t TINY_SLOT = 0xFFFFFFFEu;  // nbinslot of a tiny SAH candidate
 
 enum : uint32_t { NF_LEAF = 1u, NF_SAH_LEAF = 2u, NF_FORCED = 4u, NF_SPLIT = 8u };
 
@@ -304,6 +308,8 @@ __global__ void k_decide(BuildState S, uint32_t begin, uint32_t end, uint32_t th
                 fl |= cnt <= APG_MAX_LEAF ? NF_LEAF : (NF_FORCED | NF_SPLIT);
                 S.nflags[k] = fl;
             }
+            else if (cnt <= TINY)
+                S.nbinslot[k] = TINY_SLOT;
             else
                 cand = true;
         }
@@ -392,7 +398,7 @@ __global__ __launch_bounds__(BLOCK) void k_bounds_small(BuildState S, uint32_t b
     const uint32_t lane = threadIdx.x & 63u;
     if (k >= end) return;
     const uint32_t cnt = S.ncount[k];
-    if (cnt == 0u || cnt > SMALL) return;
+    if (cnt <= TINY || cnt > SMALL) return;
     const uint32_t first = S.nfirst[k];
     const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
     uint32_t r[12];
@@ -428,6 +434,39 @@ __global__ __launch_bounds__(BLOCK) void k_bounds_small(BuildState S, uint32_t b
         }
 }
 
+// Bounds of a node of at most TINY slots: one thread, a serial loop.
+__global__ void k_bounds_tiny(BuildState S, uint32_t begin, uint32_t end)
+{
+    const uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    const uint32_t cnt = S.ncount[k];
+    if (cnt == 0u || cnt > TINY) return;
+    const uint32_t first = S.nfirst[k];
+    const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
+    uint32_t r[12];
+    for (int a = 0; a < 3; ++a)
+    {
+        r[a] = lo_ord; r[3 + a] = hi_ord; r[6 + a] = lo_ord; r[9 + a] = hi_ord;
+    }
+    for (uint32_t i = first; i < first + cnt; ++i)
+    {
+        const float* b = S.pbox + 6 * size_t(S.idx[i]);
+        for (int a = 0; a < 3; ++a)
+        {
+            const float lo = b[a], hi = b[3 + a], c = (lo + hi) * 0.5f;
+            r[a] = min(r[a], ord_of(lo));
+            r[3 + a] = max(r[3 + a], ord_of(hi));
+            r[6 + a] = min(r[6 + a], ord_of(c));
+            r[9 + a] = max(r[9 + a], ord_of(c));
+        }
+    }
+    for (int j = 0; j < 6; ++j)
+    {
+        S.nbox[6 * size_t(k) + j] = r[j];
+        S.ncbox[6 * size_t(k) + j] = r[6 + j];
+    }
+}
+
 // Bins of a node of at most SMALL slots: one wave, a histogram of its own in LDS,
 // written out whole.
 __global__ __launch_bounds__(BLOCK) void k_bin_small(BuildState S, uint32_t begin, uint32_t end)
@@ -439,7 +478,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin_small(BuildState S, uint32_t begi
     const uint32_t cnt = S.ncount[k];
     if (cnt == 0u || cnt > SMALL) return;
     const uint32_t slot = S.nbinslot[k];
-    if (slot == RETIRED) return;
+    if (slot == RETIRED || slot == TINY_SLOT) return;
     uint32_t* acc = hist[threadIdx.x / 64u];
     for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u)
     {
@@ -479,6 +518,124 @@ __global__ __launch_bounds__(BLOCK) void k_bin_small(BuildState S, uint32_t begi
     for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u) g[j] = acc[j];
 }
 
+// An SAH leaf; one of more than APG_MAX_LEAF triangles is split positionally
+// into halves rounded to whole leaves.
+__device__ __forceinline__ void sah_leaf(BuildState& S, uint32_t k, uint32_t fl, uint32_t cnt)
+{
+    fl |= NF_SAH_LEAF;
+    fl |= cnt <= APG_MAX_LEAF ? NF_LEAF : (NF_FORCED | NF_SPLIT);
+    S.nflags[k] = fl;
+    S.nbinslot[k] = RETIRED;
+    if (fl & NF_SPLIT)
+    {
+        uint32_t half = ((cnt + 1) / 2 + APG_MAX_LEAF - 1) / APG_MAX_LEAF * APG_MAX_LEAF;
+        half = half < cnt - 1 ? half : cnt - 1;
+        const uint32_t l = atomicAdd(&S.counters[0], 2u);
+        S.nleft[k] = l;
+        S.nsplit[k] = 0xFFu;
+        S.nsplit_mm[2 * size_t(k)] = __uint_as_float(half);
+        S.ncount[l] = half;
+        S.ncount[l + 1] = cnt - half;
+        S.nflags[l] = NF_FORCED;
+        S.nflags[l + 1] = NF_FORCED;
+    }
+}
+
+// Split a node at bin `idx` of axis `dim` into two children.
+__device__ __forceinline__ void split_node(BuildState& S, uint32_t k, uint32_t fl, uint32_t cnt, uint32_t nbelow,
+                                           uint32_t dim, uint32_t idx, float mn, float mx)
+{
+    const uint32_t l = atomicAdd(&S.counters[0], 2u);
+    S.nleft[k] = l;
+    S.nsplit[k] = dim | (idx << 8);
+    S.nsplit_mm[2 * size_t(k)] = mn;
+    S.nsplit_mm[2 * size_t(k) + 1] = mx;
+    S.nflags[k] = fl | NF_SPLIT;
+    S.ncount[l] = nbelow;
+    S.ncount[l + 1] = cnt - nbelow;
+    S.nflags[l] = 0;
+    S.nflags[l + 1] = 0;
+}
+
+// The SAH sweep of a node of at most TINY slots, with each split's below and
+// above boxes and counts taken from the slots directly.  Min / max are exact, so
+// the boxes, and with the same operation order the costs, equal those the bin
+// histograms give.
+__device__ void sweep_tiny(BuildState& S, uint32_t k, uint32_t fl, uint32_t cnt)
+{
+    const uint32_t first = S.nfirst[k];
+    float nlo[3], nhi[3];
+    for (int a = 0; a < 3; ++a)
+    {
+        nlo[a] = float_of(S.nbox[6 * size_t(k) + a]);
+        nhi[a] = float_of(S.nbox[6 * size_t(k) + 3 + a]);
+    }
+    float min_cost = FLT_MAX;
+    uint32_t best_dim = 0, best_idx = 0, best_below = 0;
+    float best_mn = 0.0f, best_mx = 0.0f;
+    for (uint32_t d = 0; d < 3; ++d)
+    {
+        const float mn = float_of(S.ncbox[6 * size_t(k) + d]);
+        const float mx = float_of(S.ncbox[6 * size_t(k) + 3 + d]);
+        if (mn == mx) continue;
+        float blo[NSPLITS][3], bhi[NSPLITS][3], alo[NSPLITS][3], ahi[NSPLITS][3];
+        uint32_t bn[NSPLITS], an[NSPLITS];
+#pragma unroll
+        for (uint32_t s = 0; s < NSPLITS; ++s)
+        {
+            bn[s] = 0; an[s] = 0;
+            for (int a = 0; a < 3; ++a)
+            {
+                blo[s][a] = FLT_MAX; bhi[s][a] = -FLT_MAX; alo[s][a] = FLT_MAX; ahi[s][a] = -FLT_MAX;
+            }
+        }
+        for (uint32_t i = first; i < first + cnt; ++i)
+        {
+            const float* b = S.pbox + 6 * size_t(S.idx[i]);
+            float lo[3], hi[3];
+            for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; }
+            const uint32_t bi = bin_index((lo[d] + hi[d]) * 0.5f, mn, mx);
+#pragma unroll
+            for (uint32_t s = 0; s < NSPLITS; ++s)
+            {
+                if (bi <= s)
+                {
+                    bn[s]++;
+                    for (int a = 0; a < 3; ++a) { blo[s][a] = min_s(blo[s][a], lo[a]); bhi[s][a] = max_s(bhi[s][a], hi[a]); }
+                }
+                else
+                {
+                    an[s]++;
+                    for (int a = 0; a < 3; ++a) { alo[s][a] = min_s(alo[s][a], lo[a]); ahi[s][a] = max_s(ahi[s][a], hi[a]); }
+                }
+            }
+        }
+#pragma unroll
+        for (uint32_t s = 0; s < NSPLITS; ++s)
+        {
+            float cost = 0.0f;
+            cost += float(bn[s]) * box_area(blo[s], bhi[s]);
+            cost += float(an[s]) * box_area(alo[s], ahi[s]);
+            if (cost < min_cost)
+            {
+                min_cost = cost;
+                best_dim = d;
+                best_idx = s;
+                best_below = bn[s];
+                best_mn = mn;
+                best_mx = mx;
+            }
+        }
+    }
+    const float leaf_cost = float(cnt) * box_area(nlo, nhi);
+    if (min_cost >= leaf_cost)
+    {
+        sah_leaf(S, k, fl, cnt);
+        return;
+    }
+    split_node(S, k, fl, cnt, best_below, best_dim, best_idx, best_mn, best_mx);
+}
+
 // Sweep, split choice and child allocation for the SAH candidates; positional
 // splits for forced nodes.
 __global__ void k_sweep(BuildState S, uint32_t begin, uint32_t end)
@@ -504,6 +661,11 @@ __global__ void k_sweep(BuildState S, uint32_t begin, uint32_t end)
     }
     const uint32_t slot = S.nbinslot[k];
     if (slot == RETIRED) return;  // leaf
+    if (slot == TINY_SLOT)
+    {
+        sweep_tiny(S, k, fl, cnt);
+        return;
+    }
     const uint32_t* b = S.bins + size_t(slot) * NODE_BIN_WORDS;
     float nlo[3], nhi[3];
     for (int a = 0; a < 3; ++a)
@@ -554,38 +716,13 @@ __global__ void k_sweep(BuildState S, uint32_t begin, uint32_t end)
     const float leaf_cost = float(cnt) * box_area(nlo, nhi);
     if (min_cost >= leaf_cost)
     {
-        fl |= NF_SAH_LEAF;
-        fl |= cnt <= APG_MAX_LEAF ? NF_LEAF : (NF_FORCED | NF_SPLIT);
-        S.nflags[k] = fl;
-        S.nbinslot[k] = RETIRED;
-        if (fl & NF_SPLIT)
-        {
-            uint32_t half = ((cnt + 1) / 2 + APG_MAX_LEAF - 1) / APG_MAX_LEAF * APG_MAX_LEAF;
-            half = half < cnt - 1 ? half : cnt - 1;
-            const uint32_t l = atomicAdd(&S.counters[0], 2u);
-            S.nleft[k] = l;
-            S.nsplit[k] = 0xFFu;
-            S.nsplit_mm[2 * size_t(k)] = __uint_as_float(half);
-            S.ncount[l] = half;
-            S.ncount[l + 1] = cnt - half;
-            S.nflags[l] = NF_FORCED;
-            S.nflags[l + 1] = NF_FORCED;
-        }
+        sah_leaf(S, k, fl, cnt);
         return;
     }
     // child counts from the chosen axis' bins
     uint32_t nbelow = 0;
     for (uint32_t s = 0; s <= best_idx; ++s) nbelow += b[(best_dim * NBINS + s) * BIN_VALS + 6];
-    const uint32_t l = atomicAdd(&S.counters[0], 2u);
-    S.nleft[k] = l;
-    S.nsplit[k] = best_dim | (best_idx << 8);
-    S.nsplit_mm[2 * size_t(k)] = best_mn;
-    S.nsplit_mm[2 * size_t(k) + 1] = best_mx;
-    S.nflags[k] = fl | NF_SPLIT;
-    S.ncount[l] = nbelow;
-    S.ncount[l + 1] = cnt - nbelow;
-    S.nflags[l] = 0;
-    S.nflags[l + 1] = 0;
+    split_node(S, k, fl, cnt, nbelow, best_dim, best_idx, best_mn, best_mx);
 }
 
 // Partition flags: 1 = goes to the left child.
@@ -1061,6 +1198,7 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
         exclusive_scan(S.flag, width, S.scan, S.scan_tmp, s);
         k_bounds_large<<<2 * nchunks + 1, BLOCK, 0, s>>>(S, begin, width, S.scan);
         k_bounds_small<<<blocks_for(width, BLOCK / 64u), BLOCK, 0, s>>>(S, begin, end);
+        k_bounds_tiny<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end);
         uint32_t zero = 0;
         APG_HIP_CHECK(ctx, hipMemcpyAsync(S.counters + 1, &zero, 4, hipMemcpyHostToDevice, s));
         k_decide<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end, thr);
