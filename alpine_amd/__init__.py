@@ -63,7 +63,14 @@ class _SceneStats(C.Structure):
 
 
 class Material(C.Structure):
-    _fields_ = [("kind", C.c_uint32), ("base_color", C.c_float * 3), ("alpha", C.c_float * 2)]
+    _fields_ = [("kind", C.c_uint32), ("base_color", C.c_float * 3), ("alpha", C.c_float * 2),
+                ("base_color_tex", C.c_uint32), ("normal_tex", C.c_uint32)]
+
+
+class MeshShading(C.Structure):
+    _fields_ = [("normals", C.c_void_p), ("nn", C.c_uint32), ("normal_idx", C.c_void_p),
+                ("tangents", C.c_void_p), ("uvs", C.c_void_p), ("nuv", C.c_uint32),
+                ("uv_idx", C.c_void_p)]
 
 
 class Light(C.Structure):
@@ -82,7 +89,7 @@ class RenderParams(C.Structure):
                 ("fp64_accum", C.c_uint32), ("background", C.c_float * 3), ("guide", C.c_uint32),
                 ("train_steps", C.c_uint32), ("guide_warmup", C.c_uint32),
                 ("guide_frac", C.c_float), ("graph", C.c_uint32), ("train_samples", C.c_uint32),
-                ("guide_depth", C.c_uint32)]
+                ("guide_depth", C.c_uint32), ("aovs", C.c_uint32)]
 
 
 MAT_DIELECTRIC, MAT_CONDUCTOR = 0, 1
@@ -94,8 +101,11 @@ def _f3(v):
     return (C.c_float * 3)(*[float(x) for x in v])
 
 
-def material(kind, base_color, alpha):
-    return Material(kind, _f3(base_color), (C.c_float * 2)(*[float(a) for a in alpha]))
+def material(kind, base_color, alpha, base_color_tex=None, normal_tex=None):
+    """A material; base_color_tex / normal_tex name a texture index of set_textures (None: none)."""
+    return Material(kind, _f3(base_color), (C.c_float * 2)(*[float(a) for a in alpha]),
+                    0 if base_color_tex is None else int(base_color_tex) + 1,
+                    0 if normal_tex is None else int(normal_tex) + 1)
 
 
 def light(kind, p0=(0, 0, 0), p1=(0, 0, 0), p2=(0, 0, 0), normal=(0, 1, 0), color=(1, 1, 1),
@@ -128,11 +138,13 @@ ABI_SYMBOLS = [
     "apg_guide_init",
     "apg_guide_params", "apg_guide_forward", "apg_guide_infer", "apg_guide_train_step",
     "apg_guide_train_queries",
-    "apg_render_guide_stats",
+    "apg_render_guide_stats", "apg_mesh_set_shading", "apg_mesh_set_targets", "apg_mesh_morph",
+    "apg_textures_set", "apg_render_read_aovs", "apg_render_set_environment",
 ]
 DEBUG_SYMBOLS = ["apg_debug_host_build", "apg_debug_scene_info", "apg_debug_scene_copy",
                  "apg_debug_host_trace", "apg_debug_scene_free", "apg_debug_material",
-                 "apg_debug_light_sampler", "apg_debug_light"]
+                 "apg_debug_light_sampler", "apg_debug_light", "apg_debug_shading",
+                 "apg_debug_environment"]
 
 
 def lib():
@@ -184,6 +196,12 @@ def lib():
         "apg_render_set_tiles": (i32, [vp, vp, vp, u32, u32, u32]),
         "apg_render": (i32, [vp, C.POINTER(RenderParams), vp]),
         "apg_render_read": (i32, [vp, vp, vp, C.POINTER(u32)]),
+        "apg_render_read_aovs": (i32, [vp, vp, vp]),
+        "apg_render_set_environment": (i32, [vp, vp, u32, u32]),
+        "apg_textures_set": (i32, [vp, vp, vp, u32]),
+        "apg_mesh_set_shading": (i32, [vp, u32, C.POINTER(MeshShading)]),
+        "apg_mesh_set_targets": (i32, [vp, u32, u32, vp, vp, vp]),
+        "apg_mesh_morph": (i32, [vp, u32, vp, vp]),
         "apg_render_postprocess": (i32, [vp, i32, vp, vp]),
         "apg_render_counters": (i32, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                       C.POINTER(C.c_uint64)]),
@@ -205,6 +223,9 @@ def lib():
         "apg_debug_material": (i32, [C.POINTER(Material)] + [vp] * 3 + [u32] + [vp] * 6),
         "apg_debug_light_sampler": (i32, [vp, u32, u32, vp, vp, vp, u32, vp, vp, vp]),
         "apg_debug_light": (i32, [C.POINTER(Light), vp, vp, vp, u32] + [vp] * 7),
+        "apg_debug_shading": (i32, [vp, u32, u32, C.POINTER(MeshShading), u32, vp, vp, vp,
+                                    C.POINTER(Material), vp, vp, u32, vp, vp, vp, u32] + [vp] * 5),
+        "apg_debug_environment": (i32, [vp, u32, u32, vp, u32, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -281,6 +302,45 @@ class GpuAccelerator:
             count = self.vertex_count(shape_id) - first
         self._rc(self._lib.apg_mesh_transform(self._h, shape_id, _ptr(m), first, count,
                                               self._stream(stream)))
+
+    def set_shading(self, shape_id, normals=None, normal_idx=None, tangents=None, uvs=None,
+                    uv_idx=None):
+        """Shading attributes of a mesh (Mesh::Data normals / tangents / uvs and their index
+        buffers, shapes/mesh.h:16-33); None leaves an attribute out."""
+        keep = []
+
+        def arr(a, dtype, cols):
+            if a is None:
+                return None, 0
+            a = np.ascontiguousarray(a, dtype).reshape(-1, cols)
+            keep.append(a)
+            return _ptr(a), len(a)
+
+        n, nn = arr(normals, np.float32, 3)
+        ni, _ = arr(normal_idx, np.uint32, 3)
+        t, nt = arr(tangents, np.float32, 4)
+        if tangents is not None and nt != nn:
+            raise ApgError(-1, "tangents must have one row per normal")
+        u, nuv = arr(uvs, np.float32, 2)
+        ui, _ = arr(uv_idx, np.uint32, 3)
+        s = MeshShading(n, nn, ni, t, u, nuv, ui)
+        self._rc(self._lib.apg_mesh_set_shading(self._h, shape_id, C.byref(s)))
+
+    def set_targets(self, shape_id, dpos, dnormals=None, dtangents=None):
+        """Morph targets (Mesh::Data::Target): dpos (ntargets, nv, 3), optional normal and
+        tangent deltas (ntargets, nn, 3).  The current vertices become the blend base."""
+        d = np.ascontiguousarray(dpos, np.float32)
+        nt = 0 if d.size == 0 else d.shape[0]
+        dn = None if dnormals is None else np.ascontiguousarray(dnormals, np.float32)
+        dt = None if dtangents is None else np.ascontiguousarray(dtangents, np.float32)
+        self._rc(self._lib.apg_mesh_set_targets(self._h, shape_id, nt, _ptr(d) if nt else None,
+                                                None if dn is None else _ptr(dn),
+                                                None if dt is None else _ptr(dt)))
+
+    def morph(self, shape_id, weights, stream=None):
+        """Mesh::update on the GPU: vertices = base + sum_j dpos_j * weights[j]."""
+        w = np.ascontiguousarray(weights, np.float32).reshape(-1)
+        self._rc(self._lib.apg_mesh_morph(self._h, shape_id, _ptr(w), self._stream(stream)))
 
     def vertex_count(self, shape_id):
         p = C.POINTER(C.c_float)()
@@ -462,11 +522,36 @@ class GpuAccelerator:
 
     def render(self, spp, max_depth, rr_depth=0, background=(0.0, 0.0, 0.0), fp64=False,
                guide=0, train_steps=4, guide_warmup=1, guide_frac=0.5, graph=False,
-               train_samples=0, guide_depth=0, stream=None):
+               train_samples=0, guide_depth=0, aovs=False, stream=None):
         p = RenderParams(spp, max_depth, rr_depth, 1 if fp64 else 0, _f3(background), guide,
                          train_steps, guide_warmup, guide_frac, 1 if graph else 0, train_samples,
-                         guide_depth)
+                         guide_depth, 1 if aovs else 0)
         self._rc(self._lib.apg_render(self._h, C.byref(p), self._stream(stream)))
+
+    def render_read_aovs(self):
+        """Albedo and normal render-target sums (height, width, 3) of the aovs=True passes."""
+        alb = np.empty((self.height, self.width, 3), np.float32)
+        nrm = np.empty((self.height, self.width, 3), np.float32)
+        self._rc(self._lib.apg_render_read_aovs(self._h, _ptr(alb), _ptr(nrm)))
+        return alb, nrm
+
+    def set_textures(self, textures):
+        """Textures (Texture4f): a list of (h, w, 4) float arrays of linear texel values."""
+        if not textures:
+            self._rc(self._lib.apg_textures_set(self._h, None, None, 0))
+            return
+        ts = [np.ascontiguousarray(t, np.float32).reshape(t.shape[0], t.shape[1], 4) for t in textures]
+        texels = np.ascontiguousarray(np.concatenate([t.reshape(-1) for t in ts]))
+        sizes = np.array([(t.shape[1], t.shape[0]) for t in ts], np.uint32).reshape(-1)
+        self._rc(self._lib.apg_textures_set(self._h, _ptr(texels), _ptr(sizes), len(ts)))
+
+    def set_environment(self, rgb):
+        """Environment map on a miss: (h, w, 3) float array, or None for the background."""
+        if rgb is None:
+            self._rc(self._lib.apg_render_set_environment(self._h, None, 0, 0))
+            return
+        e = np.ascontiguousarray(rgb, np.float32)
+        self._rc(self._lib.apg_render_set_environment(self._h, _ptr(e), e.shape[1], e.shape[0]))
 
     def guide_stats(self):
         n, loss = C.c_uint32(), C.c_float()
@@ -621,3 +706,56 @@ class DebugScene:
                                    None if tm is None else _ptr(tm), n, 0, _ptr(prim), _ptr(t),
                                    _ptr(b0), _ptr(b1), None, C.byref(depth))
         return (prim, t, b0, b1), depth.value
+
+
+def debug_shading(idx, nv, queries_prim, queries_b, ng, mat, normals=None, normal_idx=None,
+                  tangents=None, uvs=None, uv_idx=None, dnormals=None, dtangents=None,
+                  weights=None, textures=()):
+    """Host build of the device shading-attribute steps (test hook, apg_debug.h): per query
+    (prim, (b1, b2), ng) the shading normal, its basis, uv and the base colour."""
+    keep = []
+
+    def arr(a, dtype, cols):
+        if a is None:
+            return None, 0
+        a = np.ascontiguousarray(a, dtype).reshape(-1, cols)
+        keep.append(a)
+        return _ptr(a), len(a)
+
+    n, nn = arr(normals, np.float32, 3)
+    ni, _ = arr(normal_idx, np.uint32, 3)
+    t, _ = arr(tangents, np.float32, 4)
+    u, nuv = arr(uvs, np.float32, 2)
+    ui, _ = arr(uv_idx, np.uint32, 3)
+    sh = MeshShading(n, nn, ni, t, u, nuv, ui)
+    tri = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32).reshape(-1)
+    nt_ = 0 if w is None else len(w)
+    dn = None if dnormals is None else np.ascontiguousarray(dnormals, np.float32)
+    dt = None if dtangents is None else np.ascontiguousarray(dtangents, np.float32)
+    ts = [np.ascontiguousarray(x, np.float32) for x in textures]
+    texels = np.ascontiguousarray(np.concatenate([x.reshape(-1) for x in ts])) if ts else np.zeros(4, np.float32)
+    sizes = np.array([(x.shape[1], x.shape[0]) for x in ts], np.uint32).reshape(-1) if ts else np.zeros(2, np.uint32)
+    qp = np.ascontiguousarray(queries_prim, np.uint32)
+    qb = np.ascontiguousarray(queries_b, np.float32).reshape(-1, 2)
+    g = np.ascontiguousarray(np.broadcast_to(np.asarray(ng, np.float32), (len(qp), 3)))
+    m = len(qp)
+    outs = [np.empty((m, 3), np.float32) for _ in range(3)] + [np.empty((m, 2), np.float32),
+                                                              np.empty((m, 3), np.float32)]
+    rc = lib().apg_debug_shading(_ptr(tri), len(tri), nv, C.byref(sh), nt_,
+                                 None if dn is None else _ptr(dn), None if dt is None else _ptr(dt),
+                                 None if w is None else _ptr(w), C.byref(mat), _ptr(texels), _ptr(sizes),
+                                 len(ts), _ptr(qp), _ptr(qb), _ptr(g), m, *[_ptr(o) for o in outs])
+    if rc != APG_OK:
+        raise ApgError(rc, "apg_debug_shading failed")
+    return outs
+
+
+def debug_environment(rgb, dirs):
+    e = np.ascontiguousarray(rgb, np.float32)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    out = np.empty_like(d)
+    rc = lib().apg_debug_environment(_ptr(e), e.shape[1], e.shape[0], _ptr(d), len(d), _ptr(out))
+    if rc != APG_OK:
+        raise ApgError(rc, "apg_debug_environment failed")
+    return out

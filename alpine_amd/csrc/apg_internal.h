@@ -18,6 +18,19 @@ struct Shape
     std::vector<float> xyz;      // the library's copy (Accelerator::getVertexBuffer)
     std::vector<uint32_t> idx;   // 3 per triangle
     uint32_t prim_offset = 0;
+    // shading attributes (Mesh::Data, shapes/mesh.h:16-33); empty = absent
+    std::vector<float> normals;      // 3 per normal vertex
+    std::vector<uint32_t> nidx;      // 3 per triangle (empty: idx)
+    std::vector<float> tangents;     // 4 per normal vertex (xyz, handedness)
+    std::vector<float> uvs;          // 2 per uv vertex
+    std::vector<uint32_t> uvidx;     // 3 per triangle (empty: idx)
+    // morph targets (Mesh::Data::Target): per target nv x 3 position deltas and
+    // nn x 3 normal / tangent deltas; the base the position deltas are added to
+    uint32_t ntargets = 0;
+    std::vector<float> tpos, tnrm, ttan;
+    std::vector<float> weights;      // current morph weights (ntargets)
+    float* d_base = nullptr;         // device: Mesh::Data::vertices
+    float* d_tpos = nullptr;         // device: position deltas
 };
 
 // Binary tree produced by either builder (host mirror for stats and collapse).
@@ -80,6 +93,8 @@ struct Context
     // shading data and the path tracer (render.hip)
     RenderState* render = nullptr;
     GuideNet* guide = nullptr;
+    uint64_t attr_version = 0;   // bumped when a shape's shading attributes or targets change
+    bool weights_dirty = false;  // morph weights changed since the render pools were filled
 
     ~Context();
     int rebuild_shading();  // per-triangle normals after a scene build (render.hip)

@@ -27,6 +27,11 @@ Context::~Context()
     if (d_xyz) (void)hipFree(d_xyz);
     if (d_tri) (void)hipFree(d_tri);
     if (spill) (void)hipFree(spill);
+    for (Shape& sh : shapes)
+    {
+        if (sh.d_base) (void)hipFree(sh.d_base);
+        if (sh.d_tpos) (void)hipFree(sh.d_tpos);
+    }
     if (last_launch) (void)hipEventDestroy(last_launch);
     if (render) render_state_free(render);
     if (guide) guide_free(guide);

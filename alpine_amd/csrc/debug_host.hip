@@ -197,6 +197,99 @@ int apg_debug_light_sampler(const apg_light* lights, uint32_t nl, uint32_t sampl
     return APG_OK;
 }
 
+int apg_debug_shading(const uint32_t* idx, uint32_t nt, uint32_t nv, const apg_mesh_shading* sh,
+                      uint32_t ntargets, const float* dnormals, const float* dtangents,
+                      const float* weights, const apg_material* mat, const float* texels,
+                      const uint32_t* sizes, uint32_t ntex, const uint32_t* prim, const float* b,
+                      const float* ng, uint32_t n, float* ns, float* ss, float* ts, float* uv,
+                      float* base_color)
+{
+    using namespace apg;
+    if (!idx || !sh || !mat || (ntargets && !weights)) return APG_E_INVALID;
+    Shape s;
+    s.idx.assign(idx, idx + size_t(nt) * 3);
+    s.xyz.assign(size_t(nv) * 3, 0.0f);
+    const uint32_t nn = sh->normals ? sh->nn : 0u;
+    if (nn)
+    {
+        s.normals.assign(sh->normals, sh->normals + size_t(nn) * 3);
+        if (sh->normal_idx) s.nidx.assign(sh->normal_idx, sh->normal_idx + size_t(nt) * 3);
+        if (sh->tangents) s.tangents.assign(sh->tangents, sh->tangents + size_t(nn) * 4);
+    }
+    if (sh->uvs)
+    {
+        s.uvs.assign(sh->uvs, sh->uvs + size_t(sh->nuv) * 2);
+        if (sh->uv_idx) s.uvidx.assign(sh->uv_idx, sh->uv_idx + size_t(nt) * 3);
+    }
+    s.ntargets = ntargets;
+    s.tnrm.assign(size_t(ntargets) * nn * 3, 0.0f);
+    s.ttan.assign(size_t(ntargets) * nn * 3, 0.0f);
+    if (dnormals) std::memcpy(s.tnrm.data(), dnormals, s.tnrm.size() * sizeof(float));
+    if (dtangents) std::memcpy(s.ttan.data(), dtangents, s.ttan.size() * sizeof(float));
+    s.weights.assign(weights, weights + ntargets);
+    std::vector<Shape> shapes(1, s);
+    AttrPools P;
+    gather_attributes(shapes, nt, P);
+    std::vector<DevTexture> tx(ntex);
+    size_t total = 0;
+    for (uint32_t k = 0; k < ntex; ++k)
+    {
+        tx[k] = DevTexture{ uint32_t(total), sizes[2 * k], sizes[2 * k + 1], 0u };
+        total += size_t(sizes[2 * k]) * sizes[2 * k + 1];
+    }
+    std::vector<float4> tl(total);
+    for (size_t i = 0; i < total; ++i)
+        tl[i] = make_float4(texels[4 * i], texels[4 * i + 1], texels[4 * i + 2], texels[4 * i + 3]);
+    AttrView V{};
+    if (P.any)
+    {
+        V.tri = P.tri.data();
+        V.normals = P.normals.data();
+        V.tangents = P.any_tangents ? P.tangents.data() : nullptr;
+        V.uvs = P.uvs.data();
+        V.morphs = P.morphs.data();
+        V.weights = P.weights.data();
+        V.tnormals = P.tnormals.data();
+        V.ttangents = P.ttangents.data();
+    }
+    V.texs = tx.data();
+    V.texels = tl.data();
+    DevMaterial m = make_material(mat->kind, mat->base_color, mat->alpha);
+    m.tex_bc = mat->base_color_tex;
+    m.tex_n = mat->normal_tex;
+    if (m.tex_bc > ntex || m.tex_n > ntex) return APG_E_INVALID;
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        if (prim[i] >= nt) return APG_E_INVALID;
+        f3 nsv, ssv, tsv;
+        f2 u;
+        shading_attributes(V, prim[i], b[2 * i], b[2 * i + 1], ld3(ng + 3 * i), m, nsv, u);
+        get_basis(nsv, ssv, tsv);
+        const f3 c = material_base_color(V, m, u);
+        ns[3 * i] = nsv.x; ns[3 * i + 1] = nsv.y; ns[3 * i + 2] = nsv.z;
+        ss[3 * i] = ssv.x; ss[3 * i + 1] = ssv.y; ss[3 * i + 2] = ssv.z;
+        ts[3 * i] = tsv.x; ts[3 * i + 1] = tsv.y; ts[3 * i + 2] = tsv.z;
+        uv[2 * i] = u.x; uv[2 * i + 1] = u.y;
+        base_color[3 * i] = c.x; base_color[3 * i + 1] = c.y; base_color[3 * i + 2] = c.z;
+    }
+    return APG_OK;
+}
+
+int apg_debug_environment(const float* rgb, uint32_t w, uint32_t h, const float* dirs, uint32_t n,
+                          float* out)
+{
+    using namespace apg;
+    if (!rgb || w == 0 || h == 0) return APG_E_INVALID;
+    std::vector<float4> t(size_t(w) * h);
+    for (size_t i = 0; i < t.size(); ++i) t[i] = make_float4(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2], 0.0f);
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        const f3 e = environment_sample(t.data(), w, h, ld3(dirs + 3 * i));
+        out[3 * i] = e.x; out[3 * i + 1] = e.y; out[3 * i + 2] = e.z;
+    }
+    return APG_OK;
+}
+
 int apg_debug_light(const apg_light* light, const float* u2, const float* hit, const float* q,
                     uint32_t n, uint8_t* ok, float* le, float* wi, float* dist, float* pdf,
                     float* qpdf, float* qdist)

@@ -120,6 +120,24 @@ struct RenderState
     float last_loss = 0.0f;
     uint32_t guided_passes = 0;
     float scene_lo[3] = { 0, 0, 0 }, scene_hi[3] = { 1, 1, 1 };
+    // shading attributes of the shapes (pools over all shapes, global triangle order)
+    AttrView attr{};
+    std::vector<void*> attr_allocs;
+    uint64_t attr_version = ~0ull;  // Context::attr_version the pools were built from
+    uint32_t attr_tris = 0;         // Context::total_tris the pools were built for
+    uint32_t nweights = 0;
+    float* weights_dev = nullptr;   // == attr.weights
+    uint32_t max_tex_ref = 0;       // largest texture number the materials name
+    // textures, environment map
+    DevTexture* texs = nullptr;
+    float4* texels = nullptr;
+    uint32_t ntex = 0;
+    float4* env = nullptr;
+    uint32_t env_w = 0, env_h = 0;
+    // albedo / normal AOVs per path slot (allocated on first use)
+    float* aov_albedo = nullptr;
+    float* aov_normal = nullptr;
+    uint32_t aov_P = 0;
 };
 
 namespace {
@@ -151,9 +169,16 @@ struct ShadeArgs
     Queue in, out, shadow;
     uint32_t* hprim;
     float* ht;
+    float* hb0;
+    float* hb1;
     uint32_t* counts;
     uint32_t max_depth, rr_depth;
     float bg[3];
+    AttrView attr;
+    const float4* env;       // environment map on a miss (nullptr: bg)
+    uint32_t env_w, env_h;
+    float* aov_albedo;       // depth-0 base colour and shading normal sums (nullptr: off)
+    float* aov_normal;
     // guiding
     const float* gout;
     float guide_frac;
@@ -188,9 +213,13 @@ __global__ void k_guide_features(ShadeArgs A, float* q9)
         const f3 ro = mk(A.in.o[0][q], A.in.o[1][q], A.in.o[2][q]);
         const f3 rd = mk(A.in.d[0][q], A.in.d[1][q], A.in.d[2][q]);
         const float4 ngm = A.tri_ng[prim];
+        f3 ns;
+        f2 uv;
+        shading_attributes(A.attr, prim, A.hb0[q], A.hb1[q], mk(ngm.x, ngm.y, ngm.z), A.mats[A.tri_mat[prim]],
+                           ns, uv);
         const f3 p = to_unit_box(A, ro + rd * A.ht[q]);
         f[0] = p.x; f[1] = p.y; f[2] = p.z;
-        f[3] = ngm.x; f[4] = ngm.y; f[5] = ngm.z;
+        f[3] = ns.x; f[4] = ns.y; f[5] = ns.z;
         f[6] = -rd.x; f[7] = -rd.y; f[8] = -rd.z;
     }
 }
@@ -375,12 +404,14 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A)
             bool ended = false;
             if (prim == 0xFFFFFFFFu)
             {
-                L = L + T * mk(A.bg[0], A.bg[1], A.bg[2]);
+                // background colour or environment map (alpine.cpp:318-324)
+                const f3 bg = A.env ? environment_sample(A.env, A.env_w, A.env_h, rd) : mk(A.bg[0], A.bg[1], A.bg[2]);
+                L = L + T * bg;
                 ended = true;
                 if (GUIDED && A.bounce < A.vrec_depth)
                 {
                     float* vr = A.vrec + (size_t(p) * A.vrec_depth + A.bounce) * VREC;
-                    vr[15] = A.bg[0]; vr[16] = A.bg[1]; vr[17] = A.bg[2];
+                    vr[15] = bg.x; vr[16] = bg.y; vr[17] = bg.z;
                     vr[18] = 0.0f;
                 }
             }
@@ -389,8 +420,21 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs A)
                 const f3 hit = ro + rd * A.ht[q];
                 const float4 ngm = A.tri_ng[prim];
                 const f3 ng = mk(ngm.x, ngm.y, ngm.z);
-                const DevMaterial m = A.mats[A.tri_mat[prim]];
-                const f3 ns = ng;
+                DevMaterial m = A.mats[A.tri_mat[prim]];
+                f3 ns;
+                f2 uv;
+                shading_attributes(A.attr, prim, A.hb0[q], A.hb1[q], ng, m, ns, uv);
+                if (m.tex_bc)
+                {
+                    const f3 bc = material_base_color(A.attr, m, uv);
+                    m.bc[0] = bc.x; m.bc[1] = bc.y; m.bc[2] = bc.z;
+                }
+                if (A.aov_albedo && depth == 0u)
+                {
+                    // albedo and normal AOVs of the first hit (alpine.cpp:333-337)
+                    store3(A.aov_albedo, p, load3(A.aov_albedo, p) + ld3(m.bc));
+                    store3(A.aov_normal, p, load3(A.aov_normal, p) + ns);
+                }
                 f3 ss, ts;
                 get_basis(ns, ss, ts);
                 const f3 wo = to_local(-rd, ss, ts, ns);
@@ -822,6 +866,12 @@ void render_state_free(RenderState* R)
     dfree(R->leaf_of);
     dfree(R->tile_cams);
     dfree(R->tile_roots);
+    for (void* p : R->attr_allocs) (void)hipFree(p);
+    dfree(R->texs);
+    dfree(R->texels);
+    dfree(R->env);
+    dfree(R->aov_albedo);
+    dfree(R->aov_normal);
     if (R->gexec) (void)hipGraphExecDestroy(R->gexec);
     if (R->gev) (void)hipEventDestroy(R->gev);
     if (R->gstream) (void)hipStreamDestroy(R->gstream);
@@ -853,6 +903,77 @@ int Context::rebuild_shading()
     return APG_OK;
 }
 
+namespace {
+
+// Pools of the shapes' shading attributes in global triangle order (rebuilt when a shape's
+// attributes or targets change, or shapes are appended); morph weights are refreshed alone.
+int ensure_attributes(Context* ctx, RenderState* R)
+{
+    if (R->attr_version == ctx->attr_version && R->attr_tris == ctx->total_tris)
+    {
+        if (ctx->weights_dirty && R->nweights)
+        {
+            std::vector<float> w;
+            gather_weights(ctx->shapes, w);
+            APG_HIP_CHECK(ctx, hipMemcpy(R->weights_dev, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
+        ctx->weights_dirty = false;
+        return APG_OK;
+    }
+    if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+    for (void* p : R->attr_allocs) (void)hipFree(p);
+    R->attr_allocs.clear();
+    const DevTexture* texs = R->attr.texs;
+    const float4* texels = R->attr.texels;
+    R->attr = AttrView{};
+    R->attr.texs = texs;
+    R->attr.texels = texels;
+    R->weights_dev = nullptr;
+    R->nweights = 0;
+    AttrPools P;
+    gather_attributes(ctx->shapes, ctx->total_tris, P);
+    if (P.any)
+    {
+        auto up = [&](auto*& dst, const auto& src) -> int {
+            using T = typename std::decay_t<decltype(src)>::value_type;
+            T* p = nullptr;
+            int rc = dmalloc(ctx, p, src.size());
+            if (rc) return rc;
+            R->attr_allocs.push_back(p);
+            if (!src.empty())
+                APG_HIP_CHECK(ctx, hipMemcpy(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+            dst = p;
+            return APG_OK;
+        };
+        const DevTriAttr* dta = nullptr;
+        const float4 *dn = nullptr, *dt = nullptr, *dtn = nullptr, *dtt = nullptr;
+        const float* duv = nullptr;
+        const DevMorph* dm = nullptr;
+        float* dw = nullptr;
+        int rc;
+        if ((rc = up(dta, P.tri)) || (rc = up(dn, P.normals)) || (P.any_tangents && (rc = up(dt, P.tangents))) ||
+            (rc = up(duv, P.uvs)) || (rc = up(dm, P.morphs)) || (rc = up(dw, P.weights)) ||
+            (rc = up(dtn, P.tnormals)) || (rc = up(dtt, P.ttangents)))
+            return rc;
+        R->attr.tri = dta;
+        R->attr.normals = dn;
+        R->attr.tangents = dt;
+        R->attr.uvs = duv;
+        R->attr.morphs = dm;
+        R->attr.weights = dw;
+        R->attr.tnormals = dtn;
+        R->attr.ttangents = dtt;
+        R->weights_dev = dw;
+        R->nweights = uint32_t(P.weights.size());
+    }
+    R->attr_version = ctx->attr_version;
+    R->attr_tris = ctx->total_tris;
+    ctx->weights_dirty = false;
+    return APG_OK;
+}
+
+} // namespace
+
 } // namespace apg
 
 using namespace apg;
@@ -871,7 +992,15 @@ int apg_materials_set(apg_context* ctx, const apg_material* mats, uint32_t nmats
     APG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     RenderState* R = state_of(ctx);
     std::vector<DevMaterial> dm(nmats);
-    for (uint32_t i = 0; i < nmats; ++i) dm[i] = make_material(mats[i].kind, mats[i].base_color, mats[i].alpha);
+    uint32_t max_tex = 0;
+    for (uint32_t i = 0; i < nmats; ++i)
+    {
+        dm[i] = make_material(mats[i].kind, mats[i].base_color, mats[i].alpha);
+        dm[i].tex_bc = mats[i].base_color_tex;
+        dm[i].tex_n = mats[i].normal_tex;
+        max_tex = std::max(max_tex, std::max(mats[i].base_color_tex, mats[i].normal_tex));
+    }
+    R->max_tex_ref = max_tex;
     dfree(R->mats);
     dfree(R->tri_mat);
     int rc;
@@ -955,6 +1084,58 @@ int apg_light_sampler_pdf(apg_context* ctx, const int32_t* light, const float* h
     return APG_OK;
 }
 
+int apg_textures_set(apg_context* ctx, const float* texels, const uint32_t* sizes, uint32_t ntex)
+{
+    if (!ctx || (ntex && (!texels || !sizes))) return APG_E_INVALID;
+    std::vector<DevTexture> dt(ntex);
+    size_t total = 0;
+    for (uint32_t k = 0; k < ntex; ++k)
+    {
+        const uint64_t n = uint64_t(sizes[2 * k]) * sizes[2 * k + 1];
+        if (n == 0) return ctx->fail(APG_E_INVALID, "empty texture");
+        if (total + n > 0xffffffffull) return ctx->fail(APG_E_CAPACITY, "more than 2^32 texels");
+        dt[k] = DevTexture{ uint32_t(total), sizes[2 * k], sizes[2 * k + 1], 0u };
+        total += n;
+    }
+    APG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    RenderState* R = state_of(ctx);
+    if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+    dfree(R->texs);
+    dfree(R->texels);
+    R->ntex = 0;
+    R->attr.texs = nullptr;
+    R->attr.texels = nullptr;
+    if (ntex == 0) return APG_OK;
+    int rc;
+    if ((rc = dmalloc(ctx, R->texs, ntex)) || (rc = dmalloc(ctx, R->texels, total))) return rc;
+    APG_HIP_CHECK(ctx, hipMemcpy(R->texs, dt.data(), ntex * sizeof(DevTexture), hipMemcpyHostToDevice));
+    APG_HIP_CHECK(ctx, hipMemcpy(R->texels, texels, total * sizeof(float4), hipMemcpyHostToDevice));
+    R->attr.texs = R->texs;
+    R->attr.texels = R->texels;
+    R->ntex = ntex;
+    return APG_OK;
+}
+
+int apg_render_set_environment(apg_context* ctx, const float* rgb, uint32_t width, uint32_t height)
+{
+    if (!ctx || (rgb && (width == 0 || height == 0))) return APG_E_INVALID;
+    APG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    RenderState* R = state_of(ctx);
+    if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+    dfree(R->env);
+    R->env_w = R->env_h = 0;
+    if (!rgb) return APG_OK;
+    const size_t n = size_t(width) * height;
+    std::vector<float4> t(n);
+    for (size_t i = 0; i < n; ++i) t[i] = make_float4(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2], 0.0f);
+    int rc;
+    if ((rc = dmalloc(ctx, R->env, n))) return rc;
+    APG_HIP_CHECK(ctx, hipMemcpy(R->env, t.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    R->env_w = width;
+    R->env_h = height;
+    return APG_OK;
+}
+
 int apg_render_set_camera(apg_context* ctx, const apg_camera* cam)
 {
     if (!ctx || !cam) return APG_E_INVALID;
@@ -1034,6 +1215,17 @@ int apg_render_reset_slots(apg_context* ctx, uint32_t width, uint32_t height, ui
     R->S = slots;
     k_reset_pixels<<<nblocks(P, 256), 256>>>(P, NPIX, seed, R->rng, R->accum, R->accum64);
     APG_HIP_CHECK(ctx, hipMemset(R->stats, 0, 8 * sizeof(unsigned long long)));
+    if (R->aov_albedo && R->aov_P != P)
+    {
+        dfree(R->aov_albedo);
+        dfree(R->aov_normal);
+        R->aov_P = 0;
+    }
+    if (R->aov_albedo)
+    {
+        APG_HIP_CHECK(ctx, hipMemset(R->aov_albedo, 0, 3 * size_t(P) * sizeof(float)));
+        APG_HIP_CHECK(ctx, hipMemset(R->aov_normal, 0, 3 * size_t(P) * sizeof(float)));
+    }
     APG_HIP_CHECK(ctx, hipDeviceSynchronize());
     R->total_spp = 0;
     return APG_OK;
@@ -1088,8 +1280,20 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
     if (!R->mats || R->tri_mat_n != ctx->total_tris)
         return ctx->fail(APG_E_STATE, "apg_materials_set must cover every triangle");
     if (prm->max_depth == 0 || prm->max_depth > ST_DEPTH) return ctx->fail(APG_E_INVALID, "max_depth out of range");
+    if (R->max_tex_ref > R->ntex) return ctx->fail(APG_E_STATE, "a material names a texture that is not set");
     hipStream_t stream = static_cast<hipStream_t>(stream_);
     const uint32_t P = R->P;
+    if (int rc = ensure_attributes(ctx, R)) return rc;
+    if (prm->aovs && !R->aov_albedo)
+    {
+        if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+        int rc;
+        if ((rc = dmalloc(ctx, R->aov_albedo, 3 * size_t(P))) || (rc = dmalloc(ctx, R->aov_normal, 3 * size_t(P))))
+            return rc;
+        APG_HIP_CHECK(ctx, hipMemset(R->aov_albedo, 0, 3 * size_t(P) * sizeof(float)));
+        APG_HIP_CHECK(ctx, hipMemset(R->aov_normal, 0, 3 * size_t(P) * sizeof(float)));
+        R->aov_P = P;
+    }
 
     ShadeArgs S;
     std::memset(&S, 0, sizeof(S));  // padding included: the graph key compares bytes
@@ -1107,10 +1311,18 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
     S.shadow = R->qs;
     S.hprim = R->hprim;
     S.ht = R->ht;
+    S.hb0 = R->hb0;
+    S.hb1 = R->hb1;
     S.counts = R->counts;
     S.max_depth = prm->max_depth;
     S.rr_depth = prm->rr_depth;
     for (int k = 0; k < 3; ++k) S.bg[k] = prm->background[k];
+    S.attr = R->attr;
+    S.env = R->env;
+    S.env_w = R->env_w;
+    S.env_h = R->env_h;
+    S.aov_albedo = prm->aovs ? R->aov_albedo : nullptr;
+    S.aov_normal = prm->aovs ? R->aov_normal : nullptr;
     const uint32_t guide = prm->guide;
     if (guide > 2) return ctx->fail(APG_E_INVALID, "guide must be 0, 1 or 2");
     if (guide)
@@ -1314,6 +1526,30 @@ int apg_render_read(apg_context* ctx, float* rgb, double* rgb64, uint32_t* total
         APG_HIP_CHECK(ctx, hipMemcpy(rgb64, R->px64, size_t(n3) * 8, hipMemcpyDeviceToHost));
     }
     if (total_spp) *total_spp = R->total_spp;
+    return APG_OK;
+}
+
+int apg_render_read_aovs(apg_context* ctx, float* albedo, float* normal)
+{
+    if (!ctx) return APG_E_INVALID;
+    RenderState* R = state_of(ctx);
+    if (!R->P) return ctx->fail(APG_E_STATE, "apg_render_reset has not been called");
+    if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+    const uint32_t n3 = 3u * R->NPIX;
+    for (int k = 0; k < 2; ++k)
+    {
+        float* out = k == 0 ? albedo : normal;
+        if (!out) continue;
+        const float* src = k == 0 ? R->aov_albedo : R->aov_normal;
+        if (!src)
+        {
+            std::memset(out, 0, size_t(n3) * sizeof(float));
+            continue;
+        }
+        k_sum_slots<<<nblocks(n3, 256), 256>>>(src, R->NPIX, R->S, R->px);
+        APG_HIP_CHECK(ctx, hipGetLastError());
+        APG_HIP_CHECK(ctx, hipMemcpy(out, R->px, size_t(n3) * 4, hipMemcpyDeviceToHost));
+    }
     return APG_OK;
 }
 

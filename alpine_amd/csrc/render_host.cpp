@@ -286,4 +286,78 @@ void camera_setup(const apg_camera& c, DevCamera& out)
     put(out.vz, z);
 }
 
+void gather_attributes(const std::vector<Shape>& shapes, uint32_t total_tris, AttrPools& P)
+{
+    P = AttrPools{};
+    for (const Shape& sh : shapes) P.any = P.any || !sh.normals.empty() || !sh.uvs.empty();
+    if (!P.any) return;
+    P.tri.assign(total_tris, DevTriAttr{});
+    for (const Shape& sh : shapes)
+    {
+        const uint32_t nt = uint32_t(sh.idx.size() / 3);
+        const uint32_t n_base = uint32_t(P.normals.size()), uv_base = uint32_t(P.uvs.size() / 2);
+        const uint32_t nn = uint32_t(sh.normals.size() / 3);
+        uint32_t flags = 0, morph = 0;
+        if (nn)
+        {
+            flags |= TA_NORMALS;
+            for (uint32_t k = 0; k < nn; ++k)
+                P.normals.push_back(make_float4(sh.normals[3 * k], sh.normals[3 * k + 1], sh.normals[3 * k + 2], 0.0f));
+            P.tangents.resize(P.normals.size(), make_float4(0, 0, 0, 0));
+            if (!sh.tangents.empty())
+            {
+                flags |= TA_TANGENTS;
+                P.any_tangents = true;
+                for (uint32_t k = 0; k < nn; ++k)
+                    P.tangents[n_base + k] = make_float4(sh.tangents[4 * k], sh.tangents[4 * k + 1],
+                                                         sh.tangents[4 * k + 2], sh.tangents[4 * k + 3]);
+            }
+            if (sh.ntargets)
+            {
+                flags |= TA_MORPH;
+                morph = uint32_t(P.morphs.size());
+                DevMorph M{};
+                M.ntargets = sh.ntargets;
+                M.w_off = uint32_t(P.weights.size());
+                M.n_base = n_base;
+                M.nn = nn;
+                M.d_off = uint32_t(P.tnormals.size());
+                P.morphs.push_back(M);
+                P.weights.insert(P.weights.end(), sh.weights.begin(), sh.weights.end());
+                for (size_t k = 0; k < size_t(sh.ntargets) * nn; ++k)
+                {
+                    P.tnormals.push_back(make_float4(sh.tnrm[3 * k], sh.tnrm[3 * k + 1], sh.tnrm[3 * k + 2], 0.0f));
+                    P.ttangents.push_back(make_float4(sh.ttan[3 * k], sh.ttan[3 * k + 1], sh.ttan[3 * k + 2], 0.0f));
+                }
+            }
+        }
+        if (!sh.uvs.empty())
+        {
+            flags |= TA_UVS;
+            P.uvs.insert(P.uvs.end(), sh.uvs.begin(), sh.uvs.end());
+        }
+        const std::vector<uint32_t>& ni = sh.nidx.empty() ? sh.idx : sh.nidx;
+        const std::vector<uint32_t>& ui = sh.uvidx.empty() ? sh.idx : sh.uvidx;
+        for (uint32_t t = 0; t < nt; ++t)
+        {
+            DevTriAttr& a = P.tri[sh.prim_offset + t];
+            a.flags = flags;
+            a.morph = morph;
+            for (int k = 0; k < 3; ++k)
+            {
+                a.n[k] = nn ? n_base + ni[3 * t + k] : 0u;
+                a.uv[k] = (flags & TA_UVS) ? uv_base + ui[3 * t + k] : 0u;
+            }
+        }
+    }
+    if (!P.any_tangents) P.tangents.clear();
+}
+
+void gather_weights(const std::vector<Shape>& shapes, std::vector<float>& out)
+{
+    out.clear();
+    for (const Shape& sh : shapes)
+        if (sh.ntargets && !sh.normals.empty()) out.insert(out.end(), sh.weights.begin(), sh.weights.end());
+}
+
 } // namespace apg

@@ -134,12 +134,15 @@ struct DevMaterial
     float f0;       // < 0: conductor (F0 = base colour)
     uint32_t kind;
     uint32_t nbsdf; // dielectric: 1 = Lambert only, 2 = Lambert + GGX coat
+    uint32_t tex_bc, tex_n;  // base colour / normal texture: 0 = none, else texture index + 1
 };
 
 SHD DevMaterial make_material(uint32_t kind, const float bc[3], const float alpha[2])
 {
     DevMaterial m;
     m.bc[0] = bc[0]; m.bc[1] = bc[1]; m.bc[2] = bc[2];
+    m.tex_bc = 0;
+    m.tex_n = 0;
     m.kind = kind;
     m.ax = max_s(0.001f, alpha[0]);
     m.ay = max_s(0.001f, alpha[1]);
@@ -277,6 +280,144 @@ SHD void mat_eval(const DevMaterial& m, f3 wo, f3 wi, f3& f, float& pdf)
     }
     f = s;
     pdf = ps / float(m.nbsdf);
+}
+
+// ---- textures (texture.h:11-42), the environment map (environment_map.cpp:15-37) and
+//      shading attributes (Mesh::getIntersectionAttributes, shapes/mesh.cpp:57-154) ----
+struct DevTexture
+{
+    uint32_t offset, w, h, pad;  // texels [offset, offset + w * h) of the texel pool, row-major
+};
+
+// Texture::sample: the nearest texel of uv wrapped to [0, 1)^2 (texture.h:19-35).
+SHD float4 texture_sample(const DevTexture& t, const float4* texels, f2 uv)
+{
+    const float u = uv.x - floorf(uv.x);
+    const float v = uv.y - floorf(uv.y);
+    uint32_t x = static_cast<uint32_t>(u * float(t.w));
+    uint32_t y = static_cast<uint32_t>(v * float(t.h));
+    x = t.w - 1u < x ? t.w - 1u : x;
+    y = t.h - 1u < y ? t.h - 1u : y;
+    return texels[t.offset + x + y * t.w];
+}
+
+// EnvironmentMap::sample: latitude-longitude lookup of the nearest texel.
+SHD f3 environment_sample(const float4* texels, uint32_t w, uint32_t h, f3 dir)
+{
+    float u = atan2f(dir.z, dir.x) / (2.0f * kPi) + 0.5f;
+    float v = acosf(clamp_s(dir.y, -1.0f, 1.0f)) / kPi;
+    u = u - floorf(u);
+    v = clamp_s(v, 0.0f, 1.0f);
+    uint32_t x = static_cast<uint32_t>(u * float(w));
+    uint32_t y = static_cast<uint32_t>(v * float(h));
+    x = w - 1u < x ? w - 1u : x;
+    y = h - 1u < y ? h - 1u : y;
+    const float4 e = texels[y * w + x];
+    return mk(e.x, e.y, e.z);
+}
+
+enum : uint32_t { TA_NORMALS = 1u, TA_TANGENTS = 2u, TA_UVS = 4u, TA_MORPH = 8u };
+
+// Per global triangle: indices into the global normal / tangent and uv pools.
+struct DevTriAttr
+{
+    uint32_t n[3];
+    uint32_t flags;   // TA_*
+    uint32_t uv[3];
+    uint32_t morph;   // TA_MORPH: index of the shape's DevMorph
+};
+
+// Morph targets of one shape (Mesh::Data::Target, shapes/mesh.h:27-32): target j's normal and
+// tangent deltas of the shape's normal vertex k are tnormals / ttangents[d_off + j * nn + k].
+struct DevMorph
+{
+    uint32_t ntargets, w_off, n_base, nn, d_off, pad[3];
+};
+
+struct AttrView
+{
+    const DevTriAttr* tri;     // nullptr: every triangle shades with ns = ng, uv = (0, 0)
+    const float4* normals;     // xyz
+    const float4* tangents;    // xyz, w = handedness
+    const float* uvs;          // 2 per uv vertex
+    const DevMorph* morphs;
+    const float* weights;
+    const float4* tnormals;
+    const float4* ttangents;
+    const DevTexture* texs;
+    const float4* texels;
+};
+
+SHD f3 xyz4(float4 v) { return mk(v.x, v.y, v.z); }
+SHD f3 interp3(f3 a, f3 b, f3 c, float b0, float b1, float b2) { return a * b0 + b * b1 + c * b2; }
+
+// Material::getBaseColor (dielectric.cpp:97-101, conductor.cpp:45-49).
+SHD f3 material_base_color(const AttrView& V, const DevMaterial& m, f2 uv)
+{
+    if (m.tex_bc == 0u) return ld3(m.bc);
+    return xyz4(texture_sample(V.texs[m.tex_bc - 1u], V.texels, uv));
+}
+
+// Mesh::getIntersectionAttributes: the shading normal ns and uv of a hit on global triangle
+// `prim` with barycentrics (b1, b2) weighting vertices 1 and 2.  The basis is getBasis(ns).
+SHD void shading_attributes(const AttrView& V, uint32_t prim, float b1, float b2, f3 ng,
+                            const DevMaterial& m, f3& ns, f2& uv)
+{
+    ns = ng;
+    uv = f2{ 0.0f, 0.0f };
+    if (!V.tri) return;
+    const DevTriAttr a = V.tri[prim];
+    const float b0 = 1.0f - b1 - b2;
+    if (a.flags & TA_UVS)
+    {
+        const float* u0 = V.uvs + 2 * size_t(a.uv[0]);
+        const float* u1 = V.uvs + 2 * size_t(a.uv[1]);
+        const float* u2 = V.uvs + 2 * size_t(a.uv[2]);
+        uv.x = u0[0] * b0 + u1[0] * b1 + u2[0] * b2;
+        uv.y = u0[1] * b0 + u1[1] * b1 + u2[1] * b2;
+    }
+    if (!(a.flags & TA_NORMALS)) return;
+    const f3 n0 = xyz4(V.normals[a.n[0]]), n1 = xyz4(V.normals[a.n[1]]), n2 = xyz4(V.normals[a.n[2]]);
+    ns = interp3(n0, n1, n2, b0, b1, b2);
+    DevMorph M{};
+    uint32_t k[3] = { 0, 0, 0 };
+    if (a.flags & TA_MORPH)
+    {
+        // mesh.cpp:88-98: add each target's interpolated normal delta times its weight
+        M = V.morphs[a.morph];
+        for (int i = 0; i < 3; ++i) k[i] = a.n[i] - M.n_base;
+        for (uint32_t j = 0; j < M.ntargets; ++j)
+        {
+            const float4* d = V.tnormals + M.d_off + size_t(j) * M.nn;
+            ns = ns + interp3(xyz4(d[k[0]]), xyz4(d[k[1]]), xyz4(d[k[2]]), b0, b1, b2) * V.weights[M.w_off + j];
+        }
+        ns = normalize(ns);
+    }
+    if (!(a.flags & TA_TANGENTS)) return;
+    // tangent space (mesh.cpp:104-144)
+    f3 t[3], bt[3];
+    float sgn[3];
+    const f3 nb[3] = { n0, n1, n2 };
+    for (int i = 0; i < 3; ++i)
+    {
+        const float4 tv = V.tangents[a.n[i]];
+        t[i] = mk(tv.x, tv.y, tv.z);
+        sgn[i] = tv.w;
+    }
+    if (a.flags & TA_MORPH)
+    {
+        for (int i = 0; i < 3; ++i)
+        {
+            for (uint32_t j = 0; j < M.ntargets; ++j)
+                t[i] = t[i] + xyz4(V.ttangents[M.d_off + size_t(j) * M.nn + k[i]]) * V.weights[M.w_off + j];
+            t[i] = normalize(t[i]);
+        }
+    }
+    for (int i = 0; i < 3; ++i) bt[i] = normalize(cross(nb[i], t[i]) * sgn[i]);
+    const f3 tan = interp3(t[0], t[1], t[2], b0, b1, b2);
+    const f3 bitan = interp3(bt[0], bt[1], bt[2], b0, b1, b2);
+    const f3 v = m.tex_n ? xyz4(texture_sample(V.texs[m.tex_n - 1u], V.texels, uv)) : mk(0.0f, 0.0f, 1.0f);
+    ns = normalize(bitan * v.x + tan * v.y + ns * v.z);
 }
 
 // ---- lights (disk_light.cpp, point_light.cpp; quad and triangle area lights) ----

@@ -123,6 +123,33 @@ int apg_mesh_prim_offset(const apg_context* ctx, uint32_t shape_id, uint32_t* of
  * (mesh.cpp:29-55) for per-frame animation without a host round trip. */
 int apg_mesh_transform(apg_context* ctx, uint32_t shape_id, const float* m, uint32_t first,
                        uint32_t count, void* stream);
+/* Shading attributes of a mesh: the Mesh::Data fields normals, tangents, uvs and their index
+ * buffers (shapes/mesh.h:16-33) that Mesh::getIntersectionAttributes interpolates at a hit
+ * (shapes/mesh.cpp:57-154).  The render path shades with them; traversal ignores them.
+ * All arrays are host memory and are copied. */
+typedef struct apg_mesh_shading {
+    const float* normals;        /* nn x 3, or NULL: the geometric normal shades (mesh.cpp:147-150) */
+    uint32_t nn;
+    const uint32_t* normal_idx;  /* nt x 3 indices into normals, or NULL: the position indices */
+    const float* tangents;       /* nn x 4 (xyz, handedness w), or NULL: no tangent space */
+    const float* uvs;            /* nuv x 2, or NULL: uv = (0, 0) */
+    uint32_t nuv;
+    const uint32_t* uv_idx;      /* nt x 3 indices into uvs, or NULL: the position indices */
+} apg_mesh_shading;
+int apg_mesh_set_shading(apg_context* ctx, uint32_t shape_id, const apg_mesh_shading* shading);
+/* Morph targets of a mesh (Mesh::Data::Target, shapes/mesh.h:27-32): ntargets x nv x 3 position
+ * deltas, and ntargets x nn x 3 normal and tangent deltas (NULL: zero).  The shape's current
+ * vertices become the base the position deltas are added to (Mesh::Data::vertices).
+ * ntargets = 0 removes them.  At most APG_MAX_MORPH_TARGETS targets. */
+#define APG_MAX_MORPH_TARGETS 64u
+int apg_mesh_set_targets(apg_context* ctx, uint32_t shape_id, uint32_t ntargets, const float* dpos,
+                         const float* dnormals, const float* dtangents);
+/* Mesh::update (shapes/mesh.cpp:29-55) on the GPU: vertex i = base_i + sum_j dpos_j,i * w_j
+ * written into the library's device copy (no upload), and the shading normals and tangents use
+ * the same weights (mesh.cpp:88-124).  weights: ntargets floats (host), the caller's
+ * std::lerp of the two keyframes (Animation::update, animation.cpp:10-39).  The next
+ * apg_scene_build rebuilds from the morphed vertices. */
+int apg_mesh_morph(apg_context* ctx, uint32_t shape_id, const float* weights, void* stream);
 /* Appending a sphere is rejected like the reference Bvh (bvh.cpp:108-112). */
 int apg_sphere_append(apg_context* ctx, const float* xyzr, uint32_t n);
 int apg_scene_build(apg_context* ctx, const apg_build_params* params); /* updateScene */
@@ -168,7 +195,15 @@ typedef struct apg_material {
     uint32_t kind;
     float base_color[3];
     float alpha[2];
+    uint32_t base_color_tex;   /* 0: base_color; k: texture k - 1 (Material::getBaseColor) */
+    uint32_t normal_tex;       /* 0: none; k: tangent-space normals of texture k - 1 (getNormal) */
 } apg_material;
+
+/* Textures (Texture4f, texture.h:11-42): texture k is sizes[2k] x sizes[2k+1] float4 texels,
+ * row-major, linear values (the loader's conversions already applied, gltf_loader.cpp:417-460);
+ * texels holds all textures back to back (host).  Sampling takes the nearest texel of uv
+ * wrapped to [0, 1)^2.  ntex = 0 removes them. */
+int apg_textures_set(apg_context* ctx, const float* texels, const uint32_t* sizes, uint32_t ntex);
 
 #define APG_LIGHT_DISK 0u      /* disk_light.cpp */
 #define APG_LIGHT_POINT 1u     /* point_light.cpp */
@@ -225,6 +260,8 @@ typedef struct apg_render_params {
                                 * over the frame; 0 = 262144 */
     uint32_t guide_depth;      /* guide = 2: path vertices below this depth use the proposal, deeper
                                 * ones BSDF sampling only (no inference); 0 = all */
+    uint32_t aovs;             /* 1: also accumulate the first hit's base colour and shading normal
+                                * (the albedo / normal render targets, alpine.cpp:333-337) */
 } apg_render_params;
 
 /* Allocate per-pixel samplers (seeded with pixel index + seed, alpine.cpp:259-269) and clear
@@ -248,6 +285,14 @@ int apg_render(apg_context* ctx, const apg_render_params* params, void* stream);
 /* Copy the accumulated radiance sum (width x height x 3, host) and the sample count.
  * rgb64 (optional) receives the double accumulation when fp64_accum was set. */
 int apg_render_read(apg_context* ctx, float* rgb, double* rgb64, uint32_t* total_spp);
+/* The albedo and normal render-target sums (width x height x 3 each, host; either may be NULL)
+ * of the passes traced with params.aovs = 1 since the last reset (denoiser::RenderTarget,
+ * alpine.cpp:259-269, 333-337). */
+int apg_render_read_aovs(apg_context* ctx, float* albedo, float* normal);
+/* Environment map seen by rays that leave the scene (Alpine::loadEnvironmentMap,
+ * alpine.cpp:252-257; EnvironmentMap::sample, environment_map.cpp:15-37): width x height RGB
+ * floats, row-major (host, copied).  rgb = NULL returns to the background colour. */
+int apg_render_set_environment(apg_context* ctx, const float* rgb, uint32_t width, uint32_t height);
 /* Alpine::postProcess without denoising (alpine.cpp:446-483): mean colour, ACES fit when
  * tonemap != 0, clamp, sRGB, bytes.  frame: width x height x 3 bytes, device memory. */
 int apg_render_postprocess(apg_context* ctx, int tonemap, uint8_t* frame, void* stream);

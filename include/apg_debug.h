@@ -42,6 +42,19 @@ int apg_debug_material(const apg_material* m, const float* wo, const float* u,
 int apg_debug_light_sampler(const apg_light* lights, uint32_t nl, uint32_t sampler,
                             const float* u, const float* hit, const float* ns, uint32_t n,
                             int32_t* light, float* pdf, float* pdf_ext);
+/* Shading attributes of one mesh of nt triangles (attributes as apg_mesh_set_shading takes them;
+ * ntargets normal / tangent deltas, nn x 3 per target, NULL = zero, with their weights), one
+ * material for every triangle and its textures as apg_textures_set takes them.  Per query
+ * (prim, b = (b1, b2), geometric normal ng): ns, ss, ts, uv and the base colour there. */
+int apg_debug_shading(const uint32_t* idx, uint32_t nt, uint32_t nv, const apg_mesh_shading* sh,
+                      uint32_t ntargets, const float* dnormals, const float* dtangents,
+                      const float* weights, const apg_material* mat, const float* texels,
+                      const uint32_t* sizes, uint32_t ntex, const uint32_t* prim, const float* b,
+                      const float* ng, uint32_t n, float* ns, float* ss, float* ts, float* uv,
+                      float* base_color);
+/* Environment map lookups over a width x height RGB map for n directions. */
+int apg_debug_environment(const float* rgb, uint32_t w, uint32_t h, const float* dirs, uint32_t n,
+                          float* out);
 /* One light: sample (ok, le, wi, dist, pdf) at (u2, hit) and pdf (qpdf, qdist) at (hit, q). */
 int apg_debug_light(const apg_light* light, const float* u2, const float* hit, const float* q,
                     uint32_t n, uint8_t* ok, float* le, float* wi, float* dist, float* pdf,

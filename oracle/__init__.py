@@ -351,7 +351,14 @@ LS_UNIFORM, LS_POWER, LS_BVH = 0, 1, 2
 
 
 class Material(C.Structure):
-    _fields_ = [("kind", C.c_uint32), ("base_color", C.c_float * 3), ("alpha", C.c_float * 2)]
+    _fields_ = [("kind", C.c_uint32), ("base_color", C.c_float * 3), ("alpha", C.c_float * 2),
+                ("base_color_tex", C.c_uint32), ("normal_tex", C.c_uint32)]
+
+
+class Shading(C.Structure):
+    _fields_ = [("tri_normals", C.c_void_p), ("tri_tangents", C.c_void_p), ("tri_uvs", C.c_void_p),
+                ("ntargets", C.c_uint32), ("tri_dnormals", C.c_void_p),
+                ("tri_dtangents", C.c_void_p), ("weights", C.c_void_p)]
 
 
 class Light(C.Structure):
@@ -377,7 +384,12 @@ def _render_sigs(L):
                                    vp, vp, C.c_uint32, C.c_int, vp, vp]
     L.orc_scene_destroy.argtypes = [vp]
     L.orc_render.argtypes = [vp, vp, _u32p, np.ctypeslib.ndpointer(np.float64, flags="C"),
-                             C.c_uint32]
+                             vp, C.c_uint32]
+    L.orc_scene_set_shading.argtypes = [vp, C.POINTER(Shading)]
+    L.orc_scene_set_textures.argtypes = [vp, vp, vp, C.c_uint32]
+    L.orc_scene_set_environment.argtypes = [vp, vp, C.c_uint32, C.c_uint32]
+    L.orc_shading_attributes.argtypes = [vp, C.c_uint32, C.c_float, C.c_float] + [_f32p] * 5
+    L.orc_environment_sample.argtypes = [vp, _f32p, _f32p]
     L.orc_sampler_reset.argtypes = [_u32p, C.c_uint32]
     L.orc_sampler_get1d.restype = C.c_float
     L.orc_sampler_get1d.argtypes = [_u32p]
@@ -410,8 +422,10 @@ def rlib():
     return L
 
 
-def make_material(kind, base_color, alpha):
-    return Material(kind, (C.c_float * 3)(*base_color), (C.c_float * 2)(*alpha))
+def make_material(kind, base_color, alpha, base_color_tex=None, normal_tex=None):
+    return Material(kind, (C.c_float * 3)(*base_color), (C.c_float * 2)(*alpha),
+                    0 if base_color_tex is None else int(base_color_tex) + 1,
+                    0 if normal_tex is None else int(normal_tex) + 1)
 
 
 def make_light(kind, p0=(0, 0, 0), p1=(0, 0, 0), p2=(0, 0, 0), normal=(0, 1, 0),
@@ -454,14 +468,95 @@ class OracleScene:
             self._h = None
 
     def render(self, width, height, spp, max_depth, states=None, accum=None, rr_depth=0,
-               nthreads=8):
+               nthreads=8, aov=None):
+        """aov: optional (height, width, 6) float64 array receiving the first hit's base colour
+        and shading normal sums."""
         if states is None:
             states = sampler_states(width * height)
         if accum is None:
             accum = np.zeros((height, width, 3), np.float64)
+        if aov is not None:
+            assert aov.dtype == np.float64 and aov.flags.c_contiguous and aov.size == width * height * 6
         rp = RenderParams(width, height, spp, max_depth, rr_depth)
-        rlib().orc_render(self._h, C.byref(rp), states.reshape(-1), accum.reshape(-1), nthreads)
+        rlib().orc_render(self._h, C.byref(rp), states.reshape(-1), accum.reshape(-1),
+                          None if aov is None else aov.ctypes.data_as(C.c_void_p), nthreads)
         return states, accum
+
+    def set_shading(self, normals=None, normal_idx=None, tangents=None, uvs=None, uv_idx=None,
+                    dnormals=None, dtangents=None, weights=None):
+        """Per-vertex attributes (indexed like the GPU API: normal_idx / uv_idx default to the
+        position indices), expanded per triangle for the restatement.  dnormals / dtangents:
+        (ntargets, nn, 3) morph deltas of the normals / tangents, weights (ntargets)."""
+        t = self.tris
+        ni = t if normal_idx is None else _u32(normal_idx).reshape(-1, 3)
+        ui = t if uv_idx is None else _u32(uv_idx).reshape(-1, 3)
+        keep = []
+
+        def exp(a, idx, cols):
+            if a is None:
+                return None
+            e = np.ascontiguousarray(_f32(a).reshape(-1, cols)[idx].reshape(-1))
+            keep.append(e)
+            return e.ctypes.data_as(C.c_void_p)
+
+        nt = 0
+        dn = dt = w = None
+        if normals is not None and weights is not None:
+            w = _f32(weights).reshape(-1)
+            nt = len(w)
+            nn = len(_f32(normals).reshape(-1, 3))
+            z = np.zeros((nt, nn, 3), np.float32)
+            dn_a = z if dnormals is None else _f32(dnormals).reshape(nt, nn, 3)
+            dt_a = z if dtangents is None else _f32(dtangents).reshape(nt, nn, 3)
+            dn_e = np.ascontiguousarray(dn_a[:, ni].reshape(-1))
+            dt_e = np.ascontiguousarray(dt_a[:, ni].reshape(-1))
+            keep += [w, dn_e, dt_e]
+            dn, dt = dn_e.ctypes.data_as(C.c_void_p), dt_e.ctypes.data_as(C.c_void_p)
+        s = Shading(exp(normals, ni, 3), exp(tangents, ni, 4) if normals is not None else None,
+                    exp(uvs, ui, 2), nt, dn, dt, None if w is None else w.ctypes.data_as(C.c_void_p))
+        rlib().orc_scene_set_shading(self._h, C.byref(s))
+
+    def set_textures(self, textures):
+        """A list of (h, w, 4) float texel arrays (Texture4f)."""
+        if not textures:
+            rlib().orc_scene_set_textures(self._h, None, None, 0)
+            return
+        ts = [_f32(t).reshape(t.shape[0], t.shape[1], 4) for t in textures]
+        self._texels = np.ascontiguousarray(np.concatenate([t.reshape(-1) for t in ts]))
+        sizes = np.array([(t.shape[1], t.shape[0]) for t in ts], np.uint32).reshape(-1)
+        rlib().orc_scene_set_textures(self._h, self._texels.ctypes.data_as(C.c_void_p),
+                                      sizes.ctypes.data_as(C.c_void_p), len(ts))
+
+    def set_environment(self, rgb):
+        if rgb is None:
+            rlib().orc_scene_set_environment(self._h, None, 0, 0)
+            return
+        e = _f32(rgb)
+        rlib().orc_scene_set_environment(self._h, e.ctypes.data_as(C.c_void_p), e.shape[1], e.shape[0])
+
+    def shading_attributes(self, prims, b1, b2):
+        """Mesh::getIntersectionAttributes per query: ns, ss, ts (n, 3), uv (n, 2), base colour."""
+        n = len(prims)
+        out = [np.empty((n, 3), np.float32) for _ in range(3)] + [np.empty((n, 2), np.float32),
+                                                                  np.empty((n, 3), np.float32)]
+        L = rlib()
+        for i in range(n):
+            r = [np.empty(3, np.float32), np.empty(3, np.float32), np.empty(3, np.float32),
+                 np.empty(2, np.float32), np.empty(3, np.float32)]
+            L.orc_shading_attributes(self._h, int(prims[i]), float(b1[i]), float(b2[i]), *r)
+            for k in range(5):
+                out[k][i] = r[k]
+        return out
+
+    def environment_sample(self, dirs):
+        d = _f32(dirs).reshape(-1, 3)
+        out = np.empty_like(d)
+        L = rlib()
+        for i in range(len(d)):
+            o = np.empty(3, np.float32)
+            L.orc_environment_sample(self._h, np.ascontiguousarray(d[i]), o)
+            out[i] = o
+        return out
 
     def camera_ray(self, x, y):
         o = np.empty(3, np.float32)
@@ -548,5 +643,10 @@ def ref_render_lib():
         L.ref_light_sampler.argtypes = [u32, fp, u32, fp, fp, fp, u32, i32, fp]
         L.ref_light_sample.argtypes = [fp, fp, fp, u32, u8, fp, fp, fp, fp]
         L.ref_light_pdf.argtypes = [fp, fp, fp, u32, fp, fp]
+        vp, f = C.c_void_p, C.c_float
+        L.ref_mesh_attributes.argtypes = [vp, u32, vp, u32, vp, u32, vp, vp, vp, u32, vp, u32, vp,
+                                          vp, vp, u32, vp, vp, f, u32, vp, vp, vp, u32, u32, vp,
+                                          u32, u32, vp, vp, u32, vp, vp, vp, vp, vp, vp]
+        L.ref_environment_sample.argtypes = [vp, u32, u32, vp, u32, vp]
         _ref_render = L
     return _ref_render

@@ -636,9 +636,21 @@ struct orc_scene {
     uint32_t nt;
     uint32_t* tri_mat;      /* per triangle material index */
     mat_t* mats;
+    orc_material* srcmats;  /* as given (texture numbers) */
     uint32_t nmats;
     float* tri_ng;          /* per triangle geometric normal (bvh_common.cpp:116) */
     float* tri_ns;          /* optional per-vertex-interpolated normals: 9 per triangle */
+    float* tri_ts;          /* optional tangents: 12 per triangle */
+    float* tri_uv;          /* optional uvs: 6 per triangle */
+    uint32_t ntargets;      /* morph targets of the normals / tangents */
+    float* tri_dn;          /* ntargets x nt x 9 */
+    float* tri_dt;          /* ntargets x nt x 9 */
+    float* weights;
+    float* texels;          /* float4 texels of all textures */
+    uint32_t* tex_off;      /* 3 per texture: offset, width, height */
+    uint32_t ntex;
+    float* env;             /* environment map, RGB */
+    uint32_t env_w, env_h;
     lsampler_t ls;
     orc_camera cam;
     float bg[3];
@@ -692,6 +704,8 @@ orc_scene* orc_scene_create(const float* xyz, uint32_t nv, const uint32_t* idx, 
     S->nmats = nmats;
     S->mats = (mat_t*)malloc((nmats ? nmats : 1) * sizeof(mat_t));
     for (uint32_t i = 0; i < nmats; ++i) mat_init(&S->mats[i], &mats[i]);
+    S->srcmats = (orc_material*)malloc((nmats ? nmats : 1) * sizeof(orc_material));
+    memcpy(S->srcmats, mats, nmats * sizeof(orc_material));
     S->tri_ng = (float*)malloc((nt ? nt : 1) * 3 * sizeof(float));
     for (uint32_t i = 0; i < nt; ++i) {
         v3 v0 = ld3(xyz + 3 * idx[3 * i]), v1 = ld3(xyz + 3 * idx[3 * i + 1]), v2 = ld3(xyz + 3 * idx[3 * i + 2]);
@@ -737,7 +751,16 @@ void orc_scene_destroy(orc_scene* S)
     free(S->tri_mat);
     free(S->mats);
     free(S->tri_ng);
+    free(S->srcmats);
     free(S->tri_ns);
+    free(S->tri_ts);
+    free(S->tri_uv);
+    free(S->tri_dn);
+    free(S->tri_dt);
+    free(S->weights);
+    free(S->texels);
+    free(S->tex_off);
+    free(S->env);
     free(S->ls.lights);
     free(S->ls.p);
     free(S->ls.q);
@@ -745,6 +768,148 @@ void orc_scene_destroy(orc_scene* S)
     free(S->ls.nodes);
     free(S->ls.leaf_of_light);
     free(S);
+}
+
+static float* dup_floats(const float* src, size_t n)
+{
+    if (!src || n == 0) return NULL;
+    float* p = (float*)malloc(n * sizeof(float));
+    memcpy(p, src, n * sizeof(float));
+    return p;
+}
+
+void orc_scene_set_shading(orc_scene* S, const orc_shading* sh)
+{
+    free(S->tri_ns); free(S->tri_ts); free(S->tri_uv); free(S->tri_dn); free(S->tri_dt); free(S->weights);
+    size_t nt = S->nt;
+    S->tri_ns = dup_floats(sh->tri_normals, nt * 9);
+    S->tri_ts = sh->tri_normals ? dup_floats(sh->tri_tangents, nt * 12) : NULL;
+    S->tri_uv = dup_floats(sh->tri_uvs, nt * 6);
+    S->ntargets = sh->tri_normals ? sh->ntargets : 0;
+    S->tri_dn = S->ntargets ? dup_floats(sh->tri_dnormals, S->ntargets * nt * 9) : NULL;
+    S->tri_dt = S->ntargets ? dup_floats(sh->tri_dtangents, S->ntargets * nt * 9) : NULL;
+    S->weights = S->ntargets ? dup_floats(sh->weights, S->ntargets) : NULL;
+}
+
+void orc_scene_set_textures(orc_scene* S, const float* texels, const uint32_t* sizes, uint32_t ntex)
+{
+    free(S->texels);
+    free(S->tex_off);
+    S->texels = NULL;
+    S->tex_off = NULL;
+    S->ntex = ntex;
+    if (!ntex) return;
+    size_t total = 0;
+    S->tex_off = (uint32_t*)malloc(3 * ntex * sizeof(uint32_t));
+    for (uint32_t k = 0; k < ntex; ++k) {
+        S->tex_off[3 * k] = (uint32_t)total;
+        S->tex_off[3 * k + 1] = sizes[2 * k];
+        S->tex_off[3 * k + 2] = sizes[2 * k + 1];
+        total += (size_t)sizes[2 * k] * sizes[2 * k + 1];
+    }
+    S->texels = dup_floats(texels, total * 4);
+}
+
+void orc_scene_set_environment(orc_scene* S, const float* rgb, uint32_t w, uint32_t h)
+{
+    free(S->env);
+    S->env = rgb ? dup_floats(rgb, (size_t)w * h * 3) : NULL;
+    S->env_w = rgb ? w : 0;
+    S->env_h = rgb ? h : 0;
+}
+
+/* Texture<float4>::sample (texture.h:19-35): nearest texel, uv wrapped into [0, 1). */
+static const float* tex_sample(const orc_scene* S, uint32_t tex, v2 uv)
+{
+    const uint32_t* t = S->tex_off + 3 * tex;
+    float u = uv.x - floorf(uv.x), v = uv.y - floorf(uv.y);
+    uint32_t x = (uint32_t)(u * (float)t[1]);
+    uint32_t y = (uint32_t)(v * (float)t[2]);
+    if (x > t[1] - 1) x = t[1] - 1;
+    if (y > t[2] - 1) y = t[2] - 1;
+    return S->texels + 4 * ((size_t)t[0] + x + (size_t)y * t[1]);
+}
+
+/* EnvironmentMap::sample (environment_map.cpp:15-37): equirectangular, nearest texel. */
+static v3 env_sample(const orc_scene* S, v3 d)
+{
+    float u = atan2f(d.z, d.x) / (2.0f * PI_F) + 0.5f;
+    float v = acosf(clampf(d.y, -1.0f, 1.0f)) / PI_F;
+    u = u - floorf(u);
+    v = clampf(v, 0.0f, 1.0f);
+    uint32_t x = (uint32_t)(u * (float)S->env_w), y = (uint32_t)(v * (float)S->env_h);
+    if (x > S->env_w - 1) x = S->env_w - 1;
+    if (y > S->env_h - 1) y = S->env_h - 1;
+    return ld3(S->env + 3 * ((size_t)y * S->env_w + x));
+}
+
+void orc_environment_sample(const orc_scene* S, const float* dir3, float* rgb3)
+{
+    v3 e = env_sample(S, ld3(dir3));
+    rgb3[0] = e.x; rgb3[1] = e.y; rgb3[2] = e.z;
+}
+
+/* values[0] * b0 + values[1] * b1 + values[2] * b2, left to right (mesh.cpp:65-67) */
+static v3 interp(v3 a, v3 b, v3 c, float b0, float b1, float b2)
+{
+    return add(add(muls(a, b0), muls(b, b1)), muls(c, b2));
+}
+
+/* Material::getBaseColor / getNormal (dielectric.cpp:97-107, conductor.cpp:45-55) */
+static v3 base_color_at(const orc_scene* S, const orc_material* src, v3 bc, v2 uv)
+{
+    if (!src->base_color_tex) return bc;
+    return ld3(tex_sample(S, src->base_color_tex - 1, uv));
+}
+
+/* Mesh::getIntersectionAttributes (mesh.cpp:57-154) on the per-triangle expanded attributes:
+ * uv and shading normal interpolated with b0 = 1 - b1 - b2; with morph targets the weighted
+ * normal deltas are added and the sum normalised; with tangents the tangent-space normal of
+ * the material (normal texture, or +z) is applied.  ns = ng without normals. */
+static void shading_attrs(const orc_scene* S, uint32_t prim, float b1, float b2, v3 ng,
+                          const orc_material* src, v3* ns_out, v2* uv_out)
+{
+    float b0 = 1.0f - b1 - b2;
+    v2 uv = { 0.0f, 0.0f };
+    if (S->tri_uv) {
+        const float* u = S->tri_uv + 6 * (size_t)prim;
+        uv.x = u[0] * b0 + u[2] * b1 + u[4] * b2;
+        uv.y = u[1] * b0 + u[3] * b1 + u[5] * b2;
+    }
+    v3 ns = ng;
+    if (S->tri_ns) {
+        const float* nv = S->tri_ns + 9 * (size_t)prim;
+        v3 n[3] = { ld3(nv), ld3(nv + 3), ld3(nv + 6) };
+        ns = interp(n[0], n[1], n[2], b0, b1, b2);
+        if (S->ntargets) {
+            for (uint32_t j = 0; j < S->ntargets; ++j) {
+                const float* d = S->tri_dn + 9 * ((size_t)j * S->nt + prim);
+                ns = add(ns, muls(interp(ld3(d), ld3(d + 3), ld3(d + 6), b0, b1, b2), S->weights[j]));
+            }
+            ns = normalize(ns);
+        }
+        if (S->tri_ts) {
+            const float* tv = S->tri_ts + 12 * (size_t)prim;
+            v3 t[3], bt[3];
+            for (int i = 0; i < 3; ++i) {
+                t[i] = ld3(tv + 4 * i);
+                if (S->ntargets) {
+                    for (uint32_t j = 0; j < S->ntargets; ++j) {
+                        const float* d = S->tri_dt + 9 * ((size_t)j * S->nt + prim) + 3 * i;
+                        t[i] = add(t[i], muls(ld3(d), S->weights[j]));
+                    }
+                    t[i] = normalize(t[i]);
+                }
+            }
+            for (int i = 0; i < 3; ++i) bt[i] = normalize(muls(cross(n[i], t[i]), tv[4 * i + 3]));
+            v3 tan = interp(t[0], t[1], t[2], b0, b1, b2);
+            v3 bitan = interp(bt[0], bt[1], bt[2], b0, b1, b2);
+            v3 v = src->normal_tex ? ld3(tex_sample(S, src->normal_tex - 1, uv)) : V(0.0f, 0.0f, 1.0f);
+            ns = normalize(add(add(muls(bitan, v.x), muls(tan, v.y)), muls(ns, v.z)));
+        }
+    }
+    *ns_out = ns;
+    *uv_out = uv;
 }
 
 /* LightSampler::sample: returns light index or -1, pdf */
@@ -948,8 +1113,26 @@ static v3 direct(const orc_scene* S, uint32_t* st, v3 hit, v3 wo, v3 ns, v3 ss, 
     return divs(rad, lpdf_sel);
 }
 
-/* One sample of one pixel (alpine.cpp:305-364), radiance returned in out3. */
-static void trace_sample(const orc_scene* S, uint32_t* st, uint32_t x, uint32_t y, v3* out)
+void orc_shading_attributes(const orc_scene* S, uint32_t prim, float b1, float b2, float* ns3,
+                            float* ss3, float* ts3, float* uv2, float* base_color3)
+{
+    v3 ns, ss, ts;
+    v2 uv;
+    const uint32_t mi = S->tri_mat[prim];
+    shading_attrs(S, prim, b1, b2, ld3(S->tri_ng + 3 * (size_t)prim), &S->srcmats[mi], &ns, &uv);
+    get_basis(ns, &ss, &ts);
+    v3 bc = base_color_at(S, &S->srcmats[mi], S->mats[mi].bc, uv);
+    ns3[0] = ns.x; ns3[1] = ns.y; ns3[2] = ns.z;
+    ss3[0] = ss.x; ss3[1] = ss.y; ss3[2] = ss.z;
+    ts3[0] = ts.x; ts3[1] = ts.y; ts3[2] = ts.z;
+    uv2[0] = uv.x; uv2[1] = uv.y;
+    base_color3[0] = bc.x; base_color3[1] = bc.y; base_color3[2] = bc.z;
+}
+
+/* One sample of one pixel (alpine.cpp:305-364); aov6 (optional) receives the first hit's base
+ * colour and shading normal (alpine.cpp:333-337). */
+static void trace_sample(const orc_scene* S, uint32_t* st, uint32_t x, uint32_t y, v3* out,
+                         double* aov6)
 {
     const orc_render_params* rp = &S->rp;
     v2 j = get2d(st);
@@ -959,21 +1142,26 @@ static void trace_sample(const orc_scene* S, uint32_t* st, uint32_t x, uint32_t 
     for (uint32_t depth = 0; depth < rp->max_depth; ++depth) {
         hit_t h;
         if (!closest(S, o, d, &h)) {
-            L = add(L, mulv(T, ld3(S->bg)));
+            /* background colour or environment map (alpine.cpp:318-324) */
+            L = add(L, mulv(T, S->env ? env_sample(S, d) : ld3(S->bg)));
             break;
         }
         v3 hit = add(o, muls(d, h.t));
         v3 ng = ld3(S->tri_ng + 3 * h.prim);
-        v3 ns = ng;
-        if (S->tri_ns) {
-            /* mesh.cpp:57-102: b0 = 1 - b1 - b2 with (b1, b2) the hit barycentrics */
-            float b1 = h.b0, b2 = h.b1, b0 = 1.0f - b1 - b2;
-            const float* nv = S->tri_ns + 9 * h.prim;
-            ns = add(add(muls(ld3(nv), b0), muls(ld3(nv + 3), b1)), muls(ld3(nv + 6), b2));
-        }
+        const uint32_t mi = S->tri_mat[h.prim];
+        v3 ns;
+        v2 uv;
+        /* (b1, b2) = the hit barycentrics (mesh.cpp:61-63) */
+        shading_attrs(S, h.prim, h.b0, h.b1, ng, &S->srcmats[mi], &ns, &uv);
         v3 ss, ts;
         get_basis(ns, &ss, &ts);
-        const mat_t* m = &S->mats[S->tri_mat[h.prim]];
+        mat_t mtex = S->mats[mi];
+        mtex.bc = base_color_at(S, &S->srcmats[mi], mtex.bc, uv);
+        const mat_t* m = &mtex;
+        if (depth == 0 && aov6) {
+            aov6[0] += m->bc.x; aov6[1] += m->bc.y; aov6[2] += m->bc.z;
+            aov6[3] += ns.x; aov6[4] += ns.y; aov6[5] += ns.z;
+        }
         v3 wo = to_local(neg(d), ss, ts, ns);
         L = add(L, mulv(T, direct(S, st, hit, wo, ns, ss, ts, m)));
         v3 est, wi;
@@ -999,6 +1187,7 @@ typedef struct {
     const orc_scene* S;
     uint32_t* states;
     double* accum;
+    double* aov;
     uint32_t y0, y1, spp;
 } rjob_t;
 
@@ -1012,7 +1201,7 @@ static void* render_rows(void* arg)
             uint32_t* st = J->states + 4 * (size_t)i;
             for (uint32_t s = 0; s < J->spp; ++s) {
                 v3 L;
-                trace_sample(J->S, st, x, y, &L);
+                trace_sample(J->S, st, x, y, &L, J->aov ? J->aov + 6 * (size_t)i : NULL);
                 J->accum[3 * (size_t)i] += L.x;
                 J->accum[3 * (size_t)i + 1] += L.y;
                 J->accum[3 * (size_t)i + 2] += L.z;
@@ -1024,7 +1213,7 @@ static void* render_rows(void* arg)
 /* Render spp samples per pixel into accum (double, width*height*3, added to).
  * states: 4 uint32 per pixel (the per-pixel Sampler), advanced in place. */
 void orc_render(orc_scene* S, const orc_render_params* rp, uint32_t* states, double* accum,
-                uint32_t nthreads)
+                double* aov, uint32_t nthreads)
 {
     S->rp = *rp;
     if (nthreads < 1) nthreads = 1;
@@ -1037,6 +1226,7 @@ void orc_render(orc_scene* S, const orc_render_params* rp, uint32_t* states, dou
         jobs[k].S = S;
         jobs[k].states = states;
         jobs[k].accum = accum;
+        jobs[k].aov = aov;
         jobs[k].spp = rp->spp;
         jobs[k].y0 = k * rows < rp->height ? k * rows : rp->height;
         jobs[k].y1 = (k + 1) * rows < rp->height ? (k + 1) * rows : rp->height;

@@ -18,7 +18,20 @@ typedef struct orc_material {
     uint32_t kind;        /* ORC_MAT_* */
     float base_color[3];
     float alpha[2];       /* GGX roughness; Dielectric alpha (1,1) = Lambert only */
+    uint32_t base_color_tex;  /* 0: base_color; k: texture k - 1 */
+    uint32_t normal_tex;      /* 0: none; k: texture k - 1 */
 } orc_material;
+
+/* Shading attributes expanded per triangle (the three corners' values, in corner order). */
+typedef struct orc_shading {
+    const float* tri_normals;    /* nt x 3 x 3, or NULL: ns = ng */
+    const float* tri_tangents;   /* nt x 3 x 4 (xyz, handedness), or NULL */
+    const float* tri_uvs;        /* nt x 3 x 2, or NULL: uv = (0, 0) */
+    uint32_t ntargets;           /* morph targets */
+    const float* tri_dnormals;   /* ntargets x nt x 3 x 3 normal deltas */
+    const float* tri_dtangents;  /* ntargets x nt x 3 x 3 tangent deltas */
+    const float* weights;        /* ntargets */
+} orc_shading;
 
 typedef struct orc_light {
     uint32_t kind;        /* ORC_LIGHT_* */
@@ -47,8 +60,19 @@ orc_scene* orc_scene_create(const float* xyz, uint32_t nv, const uint32_t* idx, 
                             const float* normals, const orc_light* lights, uint32_t nl,
                             int light_sampler, const orc_camera* cam, const float* bg3);
 void orc_scene_destroy(orc_scene* s);
+/* aov (optional): width x height x 6 doubles, first-hit base colour and shading normal sums. */
 void orc_render(orc_scene* s, const orc_render_params* rp, uint32_t* states, double* accum,
-                uint32_t nthreads);
+                double* aov, uint32_t nthreads);
+/* Replace the scene's shading attributes (overrides orc_scene_create's normals). */
+void orc_scene_set_shading(orc_scene* s, const orc_shading* sh);
+/* texels: all textures' float4 texels back to back; sizes: width, height per texture. */
+void orc_scene_set_textures(orc_scene* s, const float* texels, const uint32_t* sizes, uint32_t ntex);
+void orc_scene_set_environment(orc_scene* s, const float* rgb, uint32_t width, uint32_t height);
+/* Mesh::getIntersectionAttributes of triangle prim at barycentrics (b1, b2): ns, ss, ts, uv,
+ * and the material's base colour there. */
+void orc_shading_attributes(const orc_scene* s, uint32_t prim, float b1, float b2, float* ns3,
+                            float* ss3, float* ts3, float* uv2, float* base_color3);
+void orc_environment_sample(const orc_scene* s, const float* dir3, float* rgb3);
 
 void orc_sampler_reset(uint32_t* state4, uint32_t seed);
 void orc_postprocess(const float* accum, uint32_t n_pixels, uint32_t total_spp, int tonemap,
