@@ -51,6 +51,18 @@ struct DeviceScene
     uint32_t roots = 1;          // wide nodes 0 .. roots-1 are tree roots
 };
 
+// A stream with device and pinned host buffers for the host-memory entry points
+// (per-ray callers); the context keeps a pool, one per concurrently calling thread.
+struct HostStaging
+{
+    hipStream_t stream = nullptr;
+    size_t cap = 0;           // items
+    float* d_in = nullptr;    // 8 floats per item
+    float* d_out = nullptr;   // 4 floats per item
+    float* h_in = nullptr;    // pinned
+    float* h_out = nullptr;   // pinned
+};
+
 struct RenderState;  // render.hip
 void render_state_free(RenderState* r);
 struct GuideNet;     // guide.hip
@@ -95,6 +107,12 @@ struct Context
     GuideNet* guide = nullptr;
     uint64_t attr_version = 0;   // bumped when a shape's shading attributes or targets change
     bool weights_dirty = false;  // morph weights changed since the render pools were filled
+
+    // host-memory entry points (host_calls.cpp)
+    std::mutex staging_mutex;
+    std::vector<HostStaging*> staging_free, staging_all;
+    HostStaging* take_staging(size_t n, int& rc);
+    void give_staging(HostStaging* s);
 
     ~Context();
     int rebuild_shading();  // per-triangle normals after a scene build (render.hip)

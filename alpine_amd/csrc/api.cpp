@@ -32,6 +32,15 @@ Context::~Context()
         if (sh.d_base) (void)hipFree(sh.d_base);
         if (sh.d_tpos) (void)hipFree(sh.d_tpos);
     }
+    for (HostStaging* s : staging_all)
+    {
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+        if (s->d_in) (void)hipFree(s->d_in);
+        if (s->d_out) (void)hipFree(s->d_out);
+        if (s->h_in) (void)hipHostFree(s->h_in);
+        if (s->h_out) (void)hipHostFree(s->h_out);
+        delete s;
+    }
     if (last_launch) (void)hipEventDestroy(last_launch);
     if (render) render_state_free(render);
     if (guide) guide_free(guide);

@@ -974,6 +974,9 @@ int ensure_attributes(Context* ctx, RenderState* R)
 
 } // namespace
 
+// The lights and the sampler over them, shared by apg_lights_set and apg_light_sampler_set.
+int install_lights(Context* ctx, const std::vector<DevLight>& dl, uint32_t sampler);
+
 } // namespace apg
 
 using namespace apg;
@@ -1021,9 +1024,87 @@ int apg_lights_set(apg_context* ctx, const apg_light* lights, uint32_t n, uint32
     for (uint32_t i = 0; i < n; ++i)
         if (lights[i].kind > APG_LIGHT_TRIANGLE) return ctx->fail(APG_E_INVALID, "unknown light kind");
     APG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-    RenderState* R = state_of(ctx);
     std::vector<DevLight> dl(n);
     for (uint32_t i = 0; i < n; ++i) dl[i] = prepare_light(lights[i]);
+    return install_lights(ctx, dl, sampler);
+}
+
+int apg_light_sampler_set(apg_context* ctx, const float* power, const float* bounds, uint32_t n,
+                          uint32_t sampler)
+{
+    if (!ctx || (n && (!power || !bounds))) return APG_E_INVALID;
+    if (sampler > APG_LS_BVH) return ctx->fail(APG_E_INVALID, "unknown light sampler");
+    APG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    // the samplers read a light's power and bounds only (uniform_light_sampler.cpp,
+    // power_light_sampler.cpp:6-20, bvh_light_sampler.cpp:167-240)
+    std::vector<DevLight> dl(n);
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        DevLight L{};
+        L.kind = LIGHT_POINT;
+        for (int a = 0; a < 3; ++a)
+        {
+            L.power[a] = power[3 * size_t(i) + a];
+            L.lo[a] = bounds[6 * size_t(i) + a];
+            L.hi[a] = bounds[6 * size_t(i) + 3 + a];
+        }
+        dl[i] = L;
+    }
+    return install_lights(ctx, dl, sampler);
+}
+
+int apg_light_sampler_sample_host(apg_context* ctx, const float* u, const float* hit, const float* ns,
+                                  uint32_t n, int32_t* light, float* pdf)
+{
+    if (!ctx || (n && (!u || !hit || !ns || !light || !pdf))) return APG_E_INVALID;
+    RenderState* R = state_of(ctx);
+    if (!R->ls_set) return ctx->fail(APG_E_STATE, "no lights set");
+    if (n == 0) return APG_OK;
+    int rc;
+    HostStaging* s = ctx->take_staging(n, rc);
+    if (!s) return rc;
+    const size_t N = s->cap;
+    float* h = s->h_in;
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        h[i] = u[i];
+        for (int a = 0; a < 3; ++a)
+        {
+            h[N + 3 * size_t(i) + a] = hit[3 * size_t(i) + a];
+            h[4 * N + 3 * size_t(i) + a] = ns[3 * size_t(i) + a];
+        }
+    }
+    float* d = s->d_in;
+    hipError_t e = hipMemcpyAsync(d, h, 7 * N * sizeof(float), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+    {
+        k_ls_sample<<<nblocks(n, 256), 256, 0, s->stream>>>(view_of(R), d, d + N, d + 4 * N, n,
+                                                             reinterpret_cast<int32_t*>(s->d_out), s->d_out + N);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(s->h_out, s->d_out, 2 * N * sizeof(float), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess)
+    {
+        (void)hipStreamSynchronize(s->stream);
+        ctx->give_staging(s);
+        return ctx->fail(APG_E_HIP, std::string("staged light sampling: ") + hipGetErrorString(e));
+    }
+    std::memcpy(light, s->h_out, n * sizeof(int32_t));
+    std::memcpy(pdf, s->h_out + N, n * sizeof(float));
+    ctx->give_staging(s);
+    return APG_OK;
+}
+
+} // extern "C"
+
+namespace apg {
+
+int install_lights(Context* ctx, const std::vector<DevLight>& dl, uint32_t sampler)
+{
+    RenderState* R = state_of(ctx);
+    const uint32_t n = uint32_t(dl.size());
+    if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
     dfree(R->lights);
     dfree(R->alias_p);
     dfree(R->alias_q);
@@ -1059,6 +1140,10 @@ int apg_lights_set(apg_context* ctx, const apg_light* lights, uint32_t n, uint32
     R->ls_set = true;
     return APG_OK;
 }
+
+} // namespace apg
+
+extern "C" {
 
 int apg_light_sampler_sample(apg_context* ctx, const float* u, const float* hit, const float* ns,
                              uint32_t n, int32_t* light, float* pdf, void* stream)

@@ -164,6 +164,15 @@ int apg_intersect(apg_context* ctx, const apg_rays* rays, uint32_t n, const apg_
 /* Any hit with t < tmax[i] (FLT_MAX when tmax is NULL); occluded[i] = 0 or 1. */
 int apg_occluded(apg_context* ctx, const apg_rays* rays, uint32_t n, uint8_t* occluded,
                  void* stream);
+/* Host-memory, synchronous variants for per-ray callers: Accelerator::intersect(const Ray&) and
+ * intersectAny(const Ray&, float tFar) (accelerator.h:33-35), called one ray at a time from up to
+ * 256 tile threads (alpine.cpp:276-285).  rays6: n rays in alpine's Ray layout (org xyz, dir xyz;
+ * ray.h:6-10), host memory; results in host memory as apg_intersect / apg_occluded write them.
+ * Thread-safe: each concurrently calling thread stages through its own stream and pinned buffers. */
+int apg_intersect_host(apg_context* ctx, const float* rays6, uint32_t n, uint32_t* prim, float* t,
+                       float* b0, float* b1);
+int apg_occluded_host(apg_context* ctx, const float* rays6, const float* tmax, uint32_t n,
+                      uint8_t* occluded);
 /* Instrumented closest/any-hit run: same results, plus total 64 B node
  * fetches and 36 B triangle fetches over all rays (host uint64 out).  Used for
  * the byte model of the roofline. */
@@ -232,6 +241,16 @@ int apg_lights_set(apg_context* ctx, const apg_light* lights, uint32_t n, uint32
 int apg_light_sampler_sample(apg_context* ctx, const float* u, const float* hit,
                              const float* ns, uint32_t n, int32_t* light, float* pdf,
                              void* stream);
+/* The LightSampler constructors (uniform_light_sampler.cpp, power_light_sampler.cpp:6-20,
+ * bvh_light_sampler.cpp:167-240) over lights given by what they read of each Light: its power
+ * (Light::getPower, 3 floats) and bounds (Light::getBoundingBox, lo xyz then hi xyz), host arrays.
+ * Enough for sampling; a render needs the lights' geometry (apg_lights_set), which this replaces. */
+int apg_light_sampler_set(apg_context* ctx, const float* power, const float* bounds, uint32_t n,
+                          uint32_t sampler);
+/* LightSampler::sample (light_sampler.h:24) for n queries in host memory, synchronous and
+ * thread-safe like apg_intersect_host; light = -1 for the empty optional. */
+int apg_light_sampler_sample_host(apg_context* ctx, const float* u, const float* hit,
+                                  const float* ns, uint32_t n, int32_t* light, float* pdf);
 /* The probability sample() returns `light` at (hit, ns): no reference counterpart; MIS with a
  * third strategy needs it. */
 int apg_light_sampler_pdf(apg_context* ctx, const int32_t* light, const float* hit,

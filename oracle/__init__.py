@@ -37,6 +37,9 @@ def build(ref: bool | None = None) -> None:
         ref = os.path.isdir(os.path.join(REF_SRC, "src"))
     if ref:
         subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+        # the drop-in adapter beside the reference's Bvh / samplers (needs alpine_amd/libapg.so)
+        if os.path.exists(os.path.join(HERE, "..", "alpine_amd", "libapg.so")):
+            subprocess.run(["make", "-s", "-C", HERE, "adapter"], check=True)
 
 
 class _Stats(C.Structure):
@@ -650,3 +653,30 @@ def ref_render_lib():
         L.ref_environment_sample.argtypes = [vp, u32, u32, vp, u32, vp]
         _ref_render = L
     return _ref_render
+
+
+ADAPTER_LIB_PATH = os.path.join(HERE, "_ref", "libadapter_check.so")
+_adapter = None
+
+
+def adapter_available():
+    return os.path.exists(ADAPTER_LIB_PATH)
+
+
+def adapter_lib():
+    """The drop-in adapter (alpine_amd/adapter) beside the reference's Bvh and light samplers
+    (oracle/adapter_check.cpp).  Loads alpine_amd's library first so both share one copy."""
+    global _adapter
+    if _adapter is None:
+        import alpine_amd
+        alpine_amd.lib()
+        L = C.CDLL(ADAPTER_LIB_PATH)
+        vp, u32 = C.c_void_p, C.c_uint32
+        L.adapter_check_trace.restype = C.c_int
+        L.adapter_check_trace.argtypes = [vp, u32, vp, vp, u32, vp, vp, u32, u32] + [vp] * 12
+        L.adapter_check_light_sampler.restype = C.c_int
+        L.adapter_check_light_sampler.argtypes = [u32, vp, u32, vp, vp, vp, u32, vp, vp, vp, vp]
+        L.adapter_check_no_device.restype = C.c_int
+        L.adapter_check_no_device.argtypes = []
+        _adapter = L
+    return _adapter
