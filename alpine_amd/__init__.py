@@ -93,7 +93,7 @@ class RenderParams(C.Structure):
 
 
 MAT_DIELECTRIC, MAT_CONDUCTOR = 0, 1
-LIGHT_DISK, LIGHT_POINT, LIGHT_QUAD, LIGHT_TRIANGLE = 0, 1, 2, 3
+LIGHT_DISK, LIGHT_POINT, LIGHT_QUAD, LIGHT_TRIANGLE, LIGHT_POINT_INTENSITY = 0, 1, 2, 3, 4
 LS_UNIFORM, LS_POWER, LS_BVH = 0, 1, 2
 
 
@@ -140,6 +140,8 @@ ABI_SYMBOLS = [
     "apg_guide_train_queries",
     "apg_render_guide_stats", "apg_mesh_set_shading", "apg_mesh_set_targets", "apg_mesh_morph",
     "apg_textures_set", "apg_render_read_aovs", "apg_render_set_environment",
+    "apg_intersect_host", "apg_occluded_host", "apg_light_sampler_set",
+    "apg_light_sampler_sample_host",
 ]
 DEBUG_SYMBOLS = ["apg_debug_host_build", "apg_debug_scene_info", "apg_debug_scene_copy",
                  "apg_debug_host_trace", "apg_debug_scene_free", "apg_debug_material",

@@ -1022,7 +1022,7 @@ int apg_lights_set(apg_context* ctx, const apg_light* lights, uint32_t n, uint32
     if (!ctx || (n && !lights)) return APG_E_INVALID;
     if (sampler > APG_LS_BVH) return ctx->fail(APG_E_INVALID, "unknown light sampler");
     for (uint32_t i = 0; i < n; ++i)
-        if (lights[i].kind > APG_LIGHT_TRIANGLE) return ctx->fail(APG_E_INVALID, "unknown light kind");
+        if (lights[i].kind > APG_LIGHT_POINT_INTENSITY) return ctx->fail(APG_E_INVALID, "unknown light kind");
     APG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     std::vector<DevLight> dl(n);
     for (uint32_t i = 0; i < n; ++i) dl[i] = prepare_light(lights[i]);

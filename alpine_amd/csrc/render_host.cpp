@@ -200,6 +200,15 @@ DevLight prepare_light(const apg_light& s)
         put(L.lo, pos);
         put(L.hi, pos);
     }
+    else if (s.kind == APG_LIGHT_POINT_INTENSITY)
+    {
+        // point_light.cpp:8-13: given its intensity (glTF punctual lights)
+        L.kind = LIGHT_POINT;
+        put(L.le, col);
+        put(L.power, col * 4.0f * kPi);
+        put(L.lo, pos);
+        put(L.hi, pos);
+    }
     else
     {
         const f3 e0 = ld3(s.p1) - pos, e1 = ld3(s.p2) - pos;

@@ -170,11 +170,17 @@ def render_benchmarks(builder, frames=3):
     _timed_render(acc, sc, 1, 0)
     dt, img, _ = _timed_render(acc, sc, sc.spp, sc.seed)
     c = acc.render_counters()
+    # convergence: the same estimator at 16 spp and a 1024-spp FP64 reference (other seeds)
+    _, img16, _ = _timed_render(acc, sc, 16, sc.seed + 7)
+    _, _, ref64 = _timed_render(acc, sc, ML_REF_SPP, 1000, fp64=True, graph=True)
+    ref = ref64 / ML_REF_SPP
     out["many_light_1080p_64spp"] = {
         "triangles": int(len(sc.tris)), "lights": len(sc.lights), "seconds": round(dt, 3),
         "samples_per_s": round(sc.width * sc.height * sc.spp / dt, 1),
         "rays_per_s": round((c["closest_rays"] + c["shadow_rays"]) / dt, 1),
-        "finite": bool(np.isfinite(img).all())}
+        "finite": bool(np.isfinite(img).all()),
+        "rel_mse_vs_1024spp_fp64": round(scenes.rel_mse(img / sc.spp, ref), 6),
+        "rel_mse_16spp_vs_1024spp_fp64": round(scenes.rel_mse(img16 / 16, ref), 6)}
     del acc
     # config 4: many independent animated scenes in one tiled frame
     log("[bench] batched animated scenes")
@@ -191,6 +197,8 @@ def render_benchmarks(builder, frames=3):
 # the frozen proposal until it reaches the target.  Both run SLOTS samples of each pixel side by
 # side and are timed around the GPU work only (the relMSE checks read the image between chunks).
 GUIDE_PLAIN_SPP, GUIDE_REF_SPP, GUIDE_SLOTS, GUIDE_CHUNK = 1024, 4096, 4, 32
+# FP64 references of the many-light frame and of the batched frame
+ML_REF_SPP, BATCH_REF_SPP = 1024, 256
 GUIDE_TRAIN_SPP, GUIDE_STEPS, GUIDE_LR, GUIDE_FRAC, GUIDE_SAMPLES = 64, 32, 5e-3, 0.5, 1 << 19
 
 
@@ -301,6 +309,11 @@ def batched_benchmark(frames=3, spp=32, max_depth=5, rr_depth=3, seed=1):
         renders.append(c - b)
     cnt = acc.render_counters()
     img, _, total = acc.render_read()
+    # convergence of the last frame: a BATCH_REF_SPP FP64 reference of the same geometry
+    acc.render_reset(W, H, 5000)
+    acc.render(BATCH_REF_SPP, max_depth, rr_depth, fp64=True, graph=True)
+    _, ref64, _ = acc.render_read(fp64=True)
+    rel = scenes.rel_mse(img / total, ref64 / BATCH_REF_SPP)
     r = float(np.median(renders))
     return {
         "scenes": len(B.meshes), "triangles": n_tris, "frame": [W, H], "spp": spp,
@@ -313,6 +326,7 @@ def batched_benchmark(frames=3, spp=32, max_depth=5, rr_depth=3, seed=1):
         "rays_per_s": round((cnt["closest_rays"] + cnt["shadow_rays"]) / renders[-1], 1),
         "rebuild_mtris_per_s": round(n_tris / float(np.median(builds)) / 1e6, 2),
         "finite": bool(np.isfinite(img).all()), "mean_radiance": round(float(img.mean() / total), 5),
+        f"rel_mse_vs_{BATCH_REF_SPP}spp_fp64": round(rel, 6),
     }
 
 

@@ -218,6 +218,8 @@ int apg_textures_set(apg_context* ctx, const float* texels, const uint32_t* size
 #define APG_LIGHT_POINT 1u     /* point_light.cpp */
 #define APG_LIGHT_QUAD 2u      /* one-sided parallelogram p0 + a (p1 - p0) + b (p2 - p0), radiance = color */
 #define APG_LIGHT_TRIANGLE 3u  /* one-sided triangle p0 p1 p2, radiance = color */
+#define APG_LIGHT_POINT_INTENSITY 4u /* point light of intensity = color (point_light.cpp:8-13, the
+                                      * glTF punctual lights, gltf_loader.cpp:460-472) */
 typedef struct apg_light {
     uint32_t kind;
     float p0[3], p1[3], p2[3];

@@ -349,7 +349,7 @@ def ref_tree_stats(verts, tris, leaf_threshold: int = 8) -> dict:
 REF_RENDER_LIB_PATH = os.path.join(HERE, "_ref", "libalpine_ref_render.so")
 
 MAT_DIELECTRIC, MAT_CONDUCTOR = 0, 1
-LIGHT_DISK, LIGHT_POINT, LIGHT_QUAD, LIGHT_TRIANGLE = 0, 1, 2, 3
+LIGHT_DISK, LIGHT_POINT, LIGHT_QUAD, LIGHT_TRIANGLE, LIGHT_POINT_INTENSITY = 0, 1, 2, 3, 4
 LS_UNIFORM, LS_POWER, LS_BVH = 0, 1, 2
 
 
@@ -651,6 +651,7 @@ def ref_render_lib():
                                           vp, vp, u32, vp, vp, f, u32, vp, vp, vp, u32, u32, vp,
                                           u32, u32, vp, vp, u32, vp, vp, vp, vp, vp, vp]
         L.ref_environment_sample.argtypes = [vp, u32, u32, vp, u32, vp]
+        L.ref_mat4.argtypes = [vp, vp, vp, u32, vp, vp]
         _ref_render = L
     return _ref_render
 

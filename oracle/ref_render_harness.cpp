@@ -21,6 +21,7 @@
 #include <light_samplers/uniform_light_sampler.h>
 #include <materials/conductor.h>
 #include <materials/dielectric.h>
+#include <math/matrix.h>
 #include <ray.h>
 #include <sampler.h>
 #include <shapes/mesh.h>
@@ -264,6 +265,24 @@ void ref_mesh_attributes(const float* verts, uint32_t nv, const uint32_t* prims,
         }
         uv[2 * i] = at.uv.x;
         uv[2 * i + 1] = at.uv.y;
+    }
+}
+
+// math/matrix.h float4x4: prod = mul(m0, m1) (16 floats, column-major), mv[i] = mul(m0, v[i]).
+void ref_mat4(const float* m0, const float* m1, const float* v, uint32_t n, float* prod, float* mv)
+{
+    float4x4 a, b;
+    for (uint32_t i = 0; i < 16; ++i)
+    {
+        a[i] = m0[i];
+        b[i] = m1[i];
+    }
+    float4x4 p = mul(a, b);
+    for (uint32_t i = 0; i < 16; ++i) prod[i] = p[i];
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        float4 r = mul(a, float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]));
+        for (int k = 0; k < 4; ++k) mv[4 * i + k] = r[k];
     }
 }
 

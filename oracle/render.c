@@ -333,6 +333,13 @@ static void light_init(light_t* L, const orc_light* s)
         L->le = divs(L->power, 4.0f * PI_F);
         L->lo = L->pos;
         L->hi = L->pos;
+    } else if (s->kind == ORC_LIGHT_POINT_INTENSITY) {
+        /* point_light.cpp:8-13: given its intensity */
+        L->kind = ORC_LIGHT_POINT;
+        L->le = col;
+        L->power = muls(muls(col, 4.0f), PI_F);
+        L->lo = L->pos;
+        L->hi = L->pos;
     } else {
         /* quad: p0 + a e0 + b e1, (a, b) in [0,1]^2; triangle: p0 + b1 e0 + b2 e1.
          * One-sided, emitting along normalize(e0 x e1); radiance = color. */

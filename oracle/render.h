@@ -11,7 +11,8 @@ extern "C" {
 #endif
 
 enum { ORC_MAT_DIELECTRIC = 0, ORC_MAT_CONDUCTOR = 1 };
-enum { ORC_LIGHT_DISK = 0, ORC_LIGHT_POINT = 1, ORC_LIGHT_QUAD = 2, ORC_LIGHT_TRIANGLE = 3 };
+enum { ORC_LIGHT_DISK = 0, ORC_LIGHT_POINT = 1, ORC_LIGHT_QUAD = 2, ORC_LIGHT_TRIANGLE = 3,
+       ORC_LIGHT_POINT_INTENSITY = 4 };
 enum { ORC_LS_UNIFORM = 0, ORC_LS_POWER = 1, ORC_LS_BVH = 2 };
 
 typedef struct orc_material {
