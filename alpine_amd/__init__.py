@@ -121,7 +121,10 @@ def camera(eye, target, up, fovy, aspect):
 class TraceConfig(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("fetch_threshold", C.c_uint32), ("stack_lds", C.c_uint32),
-                ("tri_batch", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("tri_batch", C.c_uint32), ("triangle_test", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+
+
+TRI_REFERENCE, TRI_WATERTIGHT = 0, 1
 
 
 _lib = None
@@ -146,7 +149,7 @@ ABI_SYMBOLS = [
 DEBUG_SYMBOLS = ["apg_debug_host_build", "apg_debug_scene_info", "apg_debug_scene_copy",
                  "apg_debug_host_trace", "apg_debug_scene_free", "apg_debug_material",
                  "apg_debug_light_sampler", "apg_debug_light", "apg_debug_shading",
-                 "apg_debug_environment"]
+                 "apg_debug_environment", "apg_debug_triangles"]
 
 
 def lib():
@@ -228,6 +231,7 @@ def lib():
         "apg_debug_shading": (i32, [vp, u32, u32, C.POINTER(MeshShading), u32, vp, vp, vp,
                                     C.POINTER(Material), vp, vp, u32, vp, vp, vp, u32] + [vp] * 5),
         "apg_debug_environment": (i32, [vp, u32, u32, vp, u32, vp]),
+        "apg_debug_triangles": (i32, [vp, u32, vp, vp, vp, u32, i32, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -370,12 +374,23 @@ class GpuAccelerator:
         return {k: getattr(s, k) for k, _ in _SceneStats._fields_}
 
     def trace_config(self) -> TraceConfig:
+        """The launch shape in use (the library default until set_trace_config)."""
+        cur = getattr(self, "_cfg", None)
+        if cur is not None:
+            return TraceConfig.from_buffer_copy(cur)
         c = TraceConfig()
         self._rc(self._lib.apg_trace_config_default(self._h, C.byref(c)))
         return c
 
     def set_trace_config(self, cfg: TraceConfig):
         self._rc(self._lib.apg_set_trace_config(self._h, C.byref(cfg)))
+        self._cfg = TraceConfig.from_buffer_copy(cfg)
+
+    def set_triangle_test(self, kind: int):
+        """TRI_REFERENCE (the reference's Moller-Trumbore, bit-exact hits) or TRI_WATERTIGHT."""
+        c = self.trace_config()
+        c.triangle_test = kind
+        self.set_trace_config(c)
 
     # ---- traversal ----
     @staticmethod
@@ -751,6 +766,23 @@ def debug_shading(idx, nv, queries_prim, queries_b, ng, mat, normals=None, norma
     if rc != APG_OK:
         raise ApgError(rc, "apg_debug_shading failed")
     return outs
+
+
+def debug_triangles(v9, org, dir, tmax=None, watertight=False):
+    """Host build of the kernels' triangle tests, brute force over all triangles (test hook):
+    (prim, t, b0, b1) of the closest hit per ray."""
+    v = np.ascontiguousarray(v9, np.float32).reshape(-1, 9)
+    o = np.ascontiguousarray(org, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(dir, np.float32).reshape(-1, 3)
+    n = len(o)
+    tm = None if tmax is None else np.ascontiguousarray(np.broadcast_to(np.asarray(tmax, np.float32), (n,)))
+    prim = np.empty(n, np.uint32)
+    t, b0, b1 = (np.empty(n, np.float32) for _ in range(3))
+    rc = lib().apg_debug_triangles(_ptr(v), len(v), _ptr(o), _ptr(d), None if tm is None else _ptr(tm), n,
+                                   1 if watertight else 0, _ptr(prim), _ptr(t), _ptr(b0), _ptr(b1))
+    if rc != APG_OK:
+        raise ApgError(rc, "apg_debug_triangles failed")
+    return prim, t, b0, b1
 
 
 def debug_environment(rgb, dirs):

@@ -49,6 +49,8 @@ struct DeviceScene
     uint32_t node_count = 0;
     uint32_t tri_count = 0;
     uint32_t roots = 1;          // wide nodes 0 .. roots-1 are tree roots
+    float* tris_v = nullptr;     // 9 floats per triangle, tree order: the input vertices v0, v1, v2
+                                 // (watertight test; gathered on first use after a build)
 };
 
 // A stream with device and pinned host buffers for the host-memory entry points

@@ -16,6 +16,7 @@
 namespace apg {
 
 int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool stats, BinaryTree& bt);
+int gather_vertex_triangles(Context* ctx);
 
 Context::~Context()
 {
@@ -23,6 +24,7 @@ Context::~Context()
     if (scene.nodes) (void)hipFree(scene.nodes);
     if (scene.tris) (void)hipFree(scene.tris);
     if (scene.tri_ids) (void)hipFree(scene.tri_ids);
+    if (scene.tris_v) (void)hipFree(scene.tris_v);
     if (counters) (void)hipFree(counters);
     if (d_xyz) (void)hipFree(d_xyz);
     if (d_tri) (void)hipFree(d_tri);
@@ -51,6 +53,7 @@ static void free_scene(Context* ctx)
     if (ctx->scene.nodes) (void)hipFree(ctx->scene.nodes);
     if (ctx->scene.tris) (void)hipFree(ctx->scene.tris);
     if (ctx->scene.tri_ids) (void)hipFree(ctx->scene.tri_ids);
+    if (ctx->scene.tris_v) (void)hipFree(ctx->scene.tris_v);
     ctx->scene = DeviceScene{};
 }
 
@@ -296,8 +299,10 @@ int apg_scene_build(apg_context* ctx, const apg_build_params* params)
     st.max_depth_binary = bt.max_depth;
     st.sah_cost = bt.sah_cost;
     st.sah_cost_leaf = bt.sah_cost_leaf;
-    st.device_bytes = uint64_t(ctx->scene.node_count) * 64 + uint64_t(ctx->scene.tri_count) * 40;
+    // nodes; reference-form triangles and ids (40 B); vertex-form triangles (36 B)
+    st.device_bytes = uint64_t(ctx->scene.node_count) * 64 + uint64_t(ctx->scene.tri_count) * 76;
     if ((rc = ctx->rebuild_shading())) return rc;
+    if ((rc = apg::gather_vertex_triangles(ctx))) return rc;
     st.build_ms = bt.build_ms > 0.0
         ? bt.build_ms
         : std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -338,6 +343,8 @@ int apg_set_trace_config(apg_context* ctx, const apg_trace_config* cfg)
         return ctx->fail(APG_E_INVALID, "stack_lds must be 8 or 16");
     if (cfg->tri_batch == 0 || cfg->tri_batch > 64)
         return ctx->fail(APG_E_INVALID, "tri_batch must be in 1..64");
+    if (cfg->triangle_test > APG_TRI_WATERTIGHT)
+        return ctx->fail(APG_E_INVALID, "triangle_test must be APG_TRI_REFERENCE or APG_TRI_WATERTIGHT");
     ctx->cfg = *cfg;
     return APG_OK;
 }

@@ -154,6 +154,24 @@ __global__ void k_transform(float* xyz, uint32_t n, float m0, float m1, float m2
     xyz[3 * size_t(i) + 2] = m8 * x + m9 * y + m10 * z + m11;
 }
 
+// The input vertices v0, v1, v2 of every triangle in tree order, for the watertight test
+// (the shared vertices of neighbouring triangles must be bit-identical).
+__global__ void k_gather_vertices(const uint32_t* tri_ids, uint32_t n, const uint32_t* tri, const float* xyz,
+                                  float* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t g = tri_ids[i];
+    float* o = out + 9 * size_t(i);
+    for (int k = 0; k < 3; ++k)
+    {
+        const float* v = xyz + 3 * size_t(tri[3 * size_t(g) + k]);
+        o[3 * k] = v[0];
+        o[3 * k + 1] = v[1];
+        o[3 * k + 2] = v[2];
+    }
+}
+
 // Slot i holds triangle i, in the root of its shape's tree (shape_first:
 // first triangle of each shape, R + 1 entries) or in the single root (R = 1).
 __global__ void k_init_slots(uint32_t n, uint32_t* idx, uint32_t* snode, const uint32_t* shape_first, uint32_t R)
@@ -1126,6 +1144,7 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
     if (ctx->scene.nodes) (void)hipFree(ctx->scene.nodes);
     if (ctx->scene.tris) (void)hipFree(ctx->scene.tris);
     if (ctx->scene.tri_ids) (void)hipFree(ctx->scene.tri_ids);
+    if (ctx->scene.tris_v) (void)hipFree(ctx->scene.tris_v);
     ctx->scene = DeviceScene{};
     bt = BinaryTree{};
     if (n == 0) return APG_OK;
@@ -1359,6 +1378,21 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
     bt.sah_nodes = sah_nodes;
     bt.sah_leaves = sah_leaves;
     bt.max_depth = maxd;
+    return APG_OK;
+}
+
+// After a build: the vertex-form triangles from the vertices the tree was built from.
+int gather_vertex_triangles(Context* ctx)
+{
+    DeviceScene& S = ctx->scene;
+    if (S.tris_v) (void)hipFree(S.tris_v);
+    S.tris_v = nullptr;
+    if (S.tri_count == 0) return APG_OK;
+    if (int rc = ctx->sync_geometry_to_device(nullptr)) return rc;
+    APG_HIP_CHECK(ctx, hipMalloc(&S.tris_v, size_t(S.tri_count) * 9 * sizeof(float)));
+    k_gather_vertices<<<(S.tri_count + 255) / 256, 256>>>(S.tri_ids, S.tri_count, ctx->d_tri, ctx->d_xyz, S.tris_v);
+    APG_HIP_CHECK(ctx, hipGetLastError());
+    APG_HIP_CHECK(ctx, hipDeviceSynchronize());
     return APG_OK;
 }
 

@@ -197,6 +197,50 @@ int apg_debug_light_sampler(const apg_light* lights, uint32_t nl, uint32_t sampl
     return APG_OK;
 }
 
+int apg_debug_triangles(const float* v9, uint32_t ntri, const float* org3, const float* dir3,
+                        const float* tmax, uint32_t n, int watertight, uint32_t* prim, float* t,
+                        float* b0, float* b1)
+{
+    using namespace apg;
+    if ((ntri && !v9) || (n && (!org3 || !dir3 || !prim || !t || !b0 || !b1))) return APG_E_INVALID;
+    // reference form (v0, e0 = v1 - v0, e1 = v2 - v0), bvh_common.cpp:105-125
+    std::vector<float> ref(size_t(ntri) * 9);
+    for (uint32_t i = 0; i < ntri; ++i)
+        for (int a = 0; a < 3; ++a)
+        {
+            const float* v = v9 + 9 * size_t(i);
+            ref[9 * size_t(i) + a] = v[a];
+            ref[9 * size_t(i) + 3 + a] = v[3 + a] - v[a];
+            ref[9 * size_t(i) + 6 + a] = v[6 + a] - v[a];
+        }
+    for (uint32_t r = 0; r < n; ++r)
+    {
+        const float* o = org3 + 3 * size_t(r);
+        const float* d = dir3 + 3 * size_t(r);
+        float best = tmax ? tmax[r] : FLT_MAX;
+        prim[r] = 0xFFFFFFFFu;
+        t[r] = FLT_MAX;
+        b0[r] = b1[r] = 0.0f;
+        const WtRay w = make_wtray(o[0], o[1], o[2], d[0], d[1], d[2]);
+        for (uint32_t i = 0; i < ntri; ++i)
+        {
+            float th, u, v;
+            const bool h = watertight ? tri_hit_watertight(v9 + 9 * size_t(i), w, best, th, u, v)
+                                      : tri_hit(ref.data() + 9 * size_t(i), o[0], o[1], o[2], d[0], d[1], d[2],
+                                                best, th, u, v);
+            if (h)
+            {
+                best = th;
+                prim[r] = i;
+                t[r] = th;
+                b0[r] = u;
+                b1[r] = v;
+            }
+        }
+    }
+    return APG_OK;
+}
+
 int apg_debug_shading(const uint32_t* idx, uint32_t nt, uint32_t nv, const apg_mesh_shading* sh,
                       uint32_t ntargets, const float* dnormals, const float* dtangents,
                       const float* weights, const apg_material* mat, const float* texels,

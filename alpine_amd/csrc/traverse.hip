@@ -94,7 +94,9 @@ __device__ __forceinline__ float lane_f32(float v, uint32_t src)
     return __int_as_float(__builtin_amdgcn_ds_bpermute(int(src << 2), __float_as_int(v)));
 }
 
-template <bool ANY, bool COUNT, uint32_t APG_STACK_LDS>
+// WT: the watertight triangle test over the vertex-form triangles instead of the reference's
+// Moller-Trumbore over (v0, e0, e1).
+template <bool ANY, bool COUNT, uint32_t APG_STACK_LDS, bool WT>
 __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
 {
     constexpr uint32_t APG_STACK_SPILL = APG_STACK_MAX - APG_STACK_LDS;
@@ -125,6 +127,7 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
     uint32_t rid = 0;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0;
     RayBox rb{};
+    WtRay wr{};
     float tmax = 0;
     uint32_t g_base = 0, g_bits = 0;
     uint32_t sp = 0;
@@ -172,6 +175,7 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
                 dx = A.rays.dx[rid]; dy = A.rays.dy[rid]; dz = A.rays.dz[rid];
                 tmax = A.rays.tmax ? A.rays.tmax[rid] : FLT_MAX;
                 rb = make_raybox(ox, oy, oz, dx, dy, dz);
+                if constexpr (WT) wr = make_wtray(ox, oy, oz, dx, dy, dz);
                 keys[lane] = ~0ull;
                 sp = 0;
                 // Root as a group holding one inner child in slot 0.
@@ -308,6 +312,7 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
                     dx = sdx; dy = sdy; dz = sdz;
                     tmax = stm;
                     rb = make_raybox(ox, oy, oz, dx, dy, dz);
+                    if constexpr (WT) wr = make_wtray(ox, oy, oz, dx, dy, dz);
                     atomicAdd(&pend[owner], 1u << 16);
                 }
             }
@@ -343,8 +348,19 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
             const uint32_t oexcl = lane_u32(incl, o) - bit_popc(om);
             const uint32_t ob = lane_u32(gb, o), ocm = lane_u32(gcm, o), own = lane_u32(gown, o);
             const float rox = lane_f32(ox, own), roy = lane_f32(oy, own), roz = lane_f32(oz, own);
-            const float rdx = lane_f32(dx, own), rdy = lane_f32(dy, own), rdz = lane_f32(dz, own);
             const float rtmax = lane_f32(tmax, own);
+            WtRay rw{};
+            float rdx = 0.0f, rdy = 0.0f, rdz = 0.0f;
+            if constexpr (WT)
+            {
+                rw.ox = rox; rw.oy = roy; rw.oz = roz;
+                rw.sx = lane_f32(wr.sx, own); rw.sy = lane_f32(wr.sy, own); rw.sz = lane_f32(wr.sz, own);
+                rw.k = lane_u32(wr.k, own);
+            }
+            else
+            {
+                rdx = lane_f32(dx, own); rdy = lane_f32(dy, own); rdz = lane_f32(dz, own);
+            }
             bool hit = false;
             float t = 0.0f, b0 = 0.0f, b1 = 0.0f;
             unsigned long long key = 0;
@@ -354,7 +370,11 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
                 uint32_t tri = tri_index(ob, ocm, p);
                 tri = tri < A.tri_count ? tri : A.tri_count - 1u;
                 if (COUNT) n_tri++;
-                hit = tri_hit(A.tris + size_t(tri) * APG_TRI_FLOATS, rox, roy, roz, rdx, rdy, rdz, rtmax, t, b0, b1);
+                if constexpr (WT)
+                    hit = tri_hit_watertight(A.tris + size_t(tri) * APG_TRI_FLOATS, rw, rtmax, t, b0, b1);
+                else
+                    hit = tri_hit(A.tris + size_t(tri) * APG_TRI_FLOATS, rox, roy, roz, rdx, rdy, rdz, rtmax, t,
+                                  b0, b1);
                 if (hit)
                 {
                     const uint32_t tb = __float_as_uint(t);
@@ -515,21 +535,30 @@ int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint3
     A.fetch_threshold = cfg.fetch_threshold;
     A.tri_batch = cfg.tri_batch >= 1u && cfg.tri_batch <= 64u ? cfg.tri_batch : 48u;
 
-    const uint32_t stack = cfg.stack_lds == 16 ? 16u : 8u;
-    const size_t shmem = size_t(block / 64) * trace_wave_lds_words(stack) * sizeof(uint32_t);
-#define APG_LAUNCH(S)                                                                   \
-    if (any)                                                                            \
-    {                                                                                   \
-        if (counts) trace_kernel<true, true, S><<<grid, block, shmem, stream>>>(A);     \
-        else trace_kernel<true, false, S><<<grid, block, shmem, stream>>>(A);           \
-    }                                                                                   \
-    else                                                                                \
-    {                                                                                   \
-        if (counts) trace_kernel<false, true, S><<<grid, block, shmem, stream>>>(A);    \
-        else trace_kernel<false, false, S><<<grid, block, shmem, stream>>>(A);          \
+    // the watertight test reads the vertex-form triangles (8-entry LDS stack only)
+    const bool wt = cfg.triangle_test == APG_TRI_WATERTIGHT;
+    if (wt)
+    {
+        if (!ctx->scene.tris_v && ctx->scene.tri_count)
+            return ctx->fail(APG_E_STATE, "no vertex-form triangles for the watertight test");
+        A.tris = ctx->scene.tris_v;
     }
-    if (stack == 16) { APG_LAUNCH(16) }
-    else { APG_LAUNCH(8) }
+    const uint32_t stack = (cfg.stack_lds == 16 && !wt) ? 16u : 8u;
+    const size_t shmem = size_t(block / 64) * trace_wave_lds_words(stack) * sizeof(uint32_t);
+#define APG_LAUNCH(S, W)                                                                   \
+    if (any)                                                                               \
+    {                                                                                      \
+        if (counts) trace_kernel<true, true, S, W><<<grid, block, shmem, stream>>>(A);     \
+        else trace_kernel<true, false, S, W><<<grid, block, shmem, stream>>>(A);           \
+    }                                                                                      \
+    else                                                                                   \
+    {                                                                                      \
+        if (counts) trace_kernel<false, true, S, W><<<grid, block, shmem, stream>>>(A);    \
+        else trace_kernel<false, false, S, W><<<grid, block, shmem, stream>>>(A);          \
+    }
+    if (wt) { APG_LAUNCH(8, true) }
+    else if (stack == 16) { APG_LAUNCH(16, false) }
+    else { APG_LAUNCH(8, false) }
 #undef APG_LAUNCH
     APG_HIP_CHECK(ctx, hipGetLastError());
     return APG_OK;

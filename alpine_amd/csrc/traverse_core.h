@@ -214,6 +214,86 @@ APG_HD void node_visit(const uint32_t* w, const RayBox& r, float tmax, uint32_t&
     t_mask = spread_nibbles(hitmask) & t_cmask;
 }
 
+// ---- watertight triangle test (Woop, Benthin and Wald, "Watertight Ray/Triangle
+//      Intersection", JCGT 2(1), 2013) ----
+// The ray is sheared so that it runs along +z from the origin; the three 2-D
+// edge functions of a triangle then decide the hit.  An edge shared by two
+// triangles yields the same edge-function magnitude with opposite signs in
+// both (the products are rounded identically and IEEE subtraction is
+// sign-symmetric), and a zero is re-evaluated in double precision, so a ray
+// through a shared edge or vertex hits at least one of the triangles.  Edges
+// are inclusive and both faces are hit, as in the reference's test.
+struct WtRay
+{
+    float ox, oy, oz;
+    float sx, sy, sz;  // shear: d[kx] / d[kz], d[ky] / d[kz], 1 / d[kz]
+    uint32_t k;        // kx | ky << 2 | kz << 4
+};
+
+APG_HD WtRay make_wtray(float ox, float oy, float oz, float dx, float dy, float dz)
+{
+    const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    const uint32_t kz = ax > ay ? (ax > az ? 0u : 2u) : (ay > az ? 1u : 2u);
+    uint32_t kx = kz == 2u ? 0u : kz + 1u;
+    uint32_t ky = kx == 2u ? 0u : kx + 1u;
+    const float dk = kz == 0u ? dx : (kz == 1u ? dy : dz);
+    if (dk < 0.0f)
+    {
+        const uint32_t s = kx;
+        kx = ky;
+        ky = s;
+    }
+    const float dkx = kx == 0u ? dx : (kx == 1u ? dy : dz);
+    const float dky = ky == 0u ? dx : (ky == 1u ? dy : dz);
+    WtRay r;
+    r.ox = ox; r.oy = oy; r.oz = oz;
+    r.sx = dkx / dk;
+    r.sy = dky / dk;
+    r.sz = 1.0f / dk;
+    r.k = kx | (ky << 2) | (kz << 4);
+    return r;
+}
+
+APG_HD float pick3(float a, float b, float c, uint32_t k) { return k == 0u ? a : (k == 1u ? b : c); }
+
+// v: the triangle's vertices v0, v1, v2 (9 floats).  On a hit with 0 <= t < tmax returns
+// t and the barycentrics weighting v1 and v2 (the reference's convention).
+APG_HD bool tri_hit_watertight(const float* v, const WtRay& r, float tmax, float& t_out, float& b0_out,
+                               float& b1_out)
+{
+    const uint32_t kx = r.k & 3u, ky = (r.k >> 2) & 3u, kz = r.k >> 4;
+    const float ax = v[0] - r.ox, ay = v[1] - r.oy, az = v[2] - r.oz;
+    const float bx = v[3] - r.ox, by = v[4] - r.oy, bz = v[5] - r.oz;
+    const float cx = v[6] - r.ox, cy = v[7] - r.oy, cz = v[8] - r.oz;
+    const float Akz = pick3(ax, ay, az, kz), Bkz = pick3(bx, by, bz, kz), Ckz = pick3(cx, cy, cz, kz);
+    const float Ax = pick3(ax, ay, az, kx) - r.sx * Akz;
+    const float Ay = pick3(ax, ay, az, ky) - r.sy * Akz;
+    const float Bx = pick3(bx, by, bz, kx) - r.sx * Bkz;
+    const float By = pick3(bx, by, bz, ky) - r.sy * Bkz;
+    const float Cx = pick3(cx, cy, cz, kx) - r.sx * Ckz;
+    const float Cy = pick3(cx, cy, cz, ky) - r.sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f)
+    {
+        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float T = U * (r.sz * Akz) + V * (r.sz * Bkz) + W * (r.sz * Ckz);
+    const float rcp = 1.0f / det;
+    const float t = T * rcp;
+    if (!(t >= 0.0f) || !(t < tmax)) return false;
+    t_out = t;
+    b0_out = V * rcp;
+    b1_out = W * rcp;
+    return true;
+}
+
 // Moller-Trumbore exactly as the reference (bvh_common.cpp:66-103), combined
 // with the closest/any acceptance t < tmax (bvh.cpp:287-307).  The caller's
 // translation unit is compiled with -ffp-contract=off.

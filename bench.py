@@ -382,7 +382,8 @@ def main():
                                    "described in DESIGN.md and not launched"}))
         return 2
     import torch
-    from alpine_amd import BUILDER_GPU, BUILDER_HOST, GpuAccelerator, scenes
+    from alpine_amd import (BUILDER_GPU, BUILDER_HOST, TRI_REFERENCE, TRI_WATERTIGHT, GpuAccelerator,
+                            scenes)
 
     log("[bench] generating scene")
     verts, tris = scenes.microbench_scene()
@@ -418,6 +419,9 @@ def main():
     ms_coh, _ = timed_passes(lambda: acc.intersect(co_d, cd_d, out=out), args.steps, 2)
     ms_any, _ = timed_passes(lambda: acc.intersect_any(o_d, d_d, out=occ), args.steps, 2)
     ms_any_t, _ = timed_passes(lambda: acc.intersect_any(o_d, d_d, tf_d, out=occ), args.steps, 2)
+    acc.set_triangle_test(TRI_WATERTIGHT)
+    ms_wt, _ = timed_passes(lambda: acc.intersect(o_d, d_d, out=out), args.steps, 2)
+    acc.set_triangle_test(TRI_REFERENCE)
 
     log("[bench] instrumented run (byte model)")
     _, n_node, n_tri = acc.intersect_counted(o_d, d_d)
@@ -462,6 +466,7 @@ def main():
             "coherent_bytes_per_ray": round(B_RAY_IN + B_HIT_OUT + (B_NODE * cn_node + B_TRI * cn_tri) / N_RAYS, 1),
             "incoherent_any_mrays": round(N_RAYS / (ms_any * 1e-3) / 1e6, 2),
             "incoherent_any_tfar_mrays": round(N_RAYS / (ms_any_t * 1e-3) / 1e6, 2),
+            "incoherent_closest_watertight_mrays": round(N_RAYS / (ms_wt * 1e-3) / 1e6, 2),
             "any_bytes_per_ray": round(B_RAY_IN + B_OCC_OUT + (B_NODE * an_node + B_TRI * an_tri) / N_RAYS, 1),
             "build_ms": round(st["build_ms"], 1), "wide_nodes": int(st["wide_nodes"]),
             "sah_cost": round(st["sah_cost"], 3), "device_mb": round(st["device_bytes"] / 2**20, 1),

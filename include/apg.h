@@ -98,8 +98,14 @@ typedef struct apg_trace_config {
     uint32_t fetch_threshold;  /* idle lanes a wave waits for before handing out rays */
     uint32_t stack_lds;        /* traversal stack entries per lane kept in LDS: 8 or 16 */
     uint32_t tri_batch;        /* queued triangle tests per wave that start a round: 1..64 */
-    uint32_t reserved[3];
+    uint32_t triangle_test;    /* APG_TRI_REFERENCE (default): the reference's Moller-Trumbore,
+                                * bit-exact hits (bvh_common.cpp:66-103); APG_TRI_WATERTIGHT: the
+                                * watertight test of Woop, Benthin and Wald (JCGT 2013) on the input
+                                * vertices, no ray passes between triangles sharing an edge */
+    uint32_t reserved[2];
 } apg_trace_config;
+#define APG_TRI_REFERENCE 0u
+#define APG_TRI_WATERTIGHT 1u
 
 /* ---- lifetime (Alpine::resetScene creates the accelerator, alpine.cpp:158-176) ---- */
 int apg_create(apg_context** ctx, int device);

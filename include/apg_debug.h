@@ -52,6 +52,12 @@ int apg_debug_shading(const uint32_t* idx, uint32_t nt, uint32_t nv, const apg_m
                       const uint32_t* sizes, uint32_t ntex, const uint32_t* prim, const float* b,
                       const float* ng, uint32_t n, float* ns, float* ss, float* ts, float* uv,
                       float* base_color);
+/* Brute-force closest hits over ntri triangles given by their vertices (v0, v1, v2; 9 floats
+ * each) with the kernels' triangle tests: watertight = 0 the reference's Moller-Trumbore on
+ * (v0, v1 - v0, v2 - v0), 1 the watertight test.  tmax may be NULL (FLT_MAX). */
+int apg_debug_triangles(const float* v9, uint32_t ntri, const float* org3, const float* dir3,
+                        const float* tmax, uint32_t n, int watertight, uint32_t* prim, float* t,
+                        float* b0, float* b1);
 /* Environment map lookups over a width x height RGB map for n directions. */
 int apg_debug_environment(const float* rgb, uint32_t w, uint32_t h, const float* dirs, uint32_t n,
                           float* out);
