@@ -54,6 +54,9 @@ def parse():
                     help="sweep the launch shape first (the library default is the sweep's pick)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape "
                     "(the default behaviour)")
+    ap.add_argument("--render-config", choices=sorted(RENDER_CONFIGS),
+                    help="run only this path-tracing config (or the MLP kernels) and print its "
+                         "figures as one JSON line; for profiling, not the headline measurement")
     return ap.parse_args()
 
 
@@ -134,11 +137,25 @@ def _timed_render(acc, sc, spp, seed, **kw):
 
 def render_benchmarks(builder, frames=3):
     """Path-tracing configs: Cornell room (16 spp, 256^2), many-light GGX (1920x1080, 64 spp,
-    light BVH, RR from depth 3), 32 animated scenes in one tiled frame, and guided vs plain
-    time to equal relMSE (box grid behind a doorway)."""
+    light BVH, RR from depth 3), 32 animated scenes in one tiled frame, guided vs plain
+    time to equal relMSE (box grid behind a doorway) and the guide network's kernels."""
+    out = {}
+    out.update(cornell_benchmark(builder))
+    log("[bench] many-light scene")
+    out.update(many_light_benchmark(builder))
+    log("[bench] batched animated scenes")
+    out["batched_32_scenes_512_32spp"] = batched_benchmark(frames=frames)
+    log("[bench] guided vs plain")
+    out["guided_vs_plain"] = guided_benchmark(builder)
+    log("[bench] guide network kernels")
+    out["guide_mlp"] = mlp_benchmark()
+    return out
+
+
+def cornell_benchmark(builder):
+    """Config 1: the Cornell room, 256 x 256, 16 spp, against a 1024-spp FP64 reference."""
     from alpine_amd import scenes
     out = {}
-    # config 1: Cornell room
     sc = scenes.cornell_scene()
     acc = _render_setup(sc, builder)
     _timed_render(acc, sc, 1, 0)
@@ -162,9 +179,14 @@ def render_benchmarks(builder, frames=3):
     out["cornell_256_16spp"]["slots16"] = {
         "seconds": round(dt16, 4), "samples_per_s": round(sc.width * sc.height * sc.spp / dt16, 1),
         "rel_mse_vs_1024spp_fp64": round(scenes.rel_mse(img / sc.spp, ref), 6)}
-    del acc
-    # config 3: many-light GGX
-    log("[bench] many-light scene")
+    return out
+
+
+def many_light_benchmark(builder):
+    """Config 3: 1920 x 1080, 64 spp, GGX materials, 10,000 emissive triangles behind the
+    light BVH, RR from depth 3; relMSE against a 1024-spp FP64 reference."""
+    from alpine_amd import scenes
+    out = {}
     sc = scenes.many_light_scene()
     acc = _render_setup(sc, builder)
     _timed_render(acc, sc, 1, 0)
@@ -181,14 +203,57 @@ def render_benchmarks(builder, frames=3):
         "finite": bool(np.isfinite(img).all()),
         "rel_mse_vs_1024spp_fp64": round(scenes.rel_mse(img / sc.spp, ref), 6),
         "rel_mse_16spp_vs_1024spp_fp64": round(scenes.rel_mse(img16 / 16, ref), 6)}
-    del acc
-    # config 4: many independent animated scenes in one tiled frame
-    log("[bench] batched animated scenes")
-    out["batched_32_scenes_512_32spp"] = batched_benchmark(frames=frames)
-    # config 5: guided vs plain, time to equal relMSE
-    log("[bench] guided vs plain")
-    out["guided_vs_plain"] = guided_benchmark(builder)
     return out
+
+
+# The guide network: 96 -> 64 -> 64 -> 64 -> 64 -> 48 (bias adds not counted).  Forward FLOPs
+# per query are 2 * sum(fan_in * fan_out); a training sample adds the backward pass (input and
+# weight gradients, twice the forward), 3x in all.
+MLP_LAYERS = [(96, 64), (64, 64), (64, 64), (64, 64), (64, 48)]
+MLP_FWD_FLOP = 2 * sum(a * b for a, b in MLP_LAYERS)
+MLP_TRAIN_FLOP = 3 * MLP_FWD_FLOP
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+
+
+def mlp_benchmark(n_infer=1 << 20, n_train=1 << 19, reps=10):
+    """The guide network's fused kernels on their own: inference (encoding fused in) on
+    n_infer queries and training steps (encoding, forward, loss head, backward, Adam) on
+    n_train samples, as in a training pass; TFLOP/s against the dense BF16 MFMA peak."""
+    import torch
+    import alpine_amd as aa
+    acc = aa.GpuAccelerator(0)
+    acc.guide_init(seed=1, lr=1e-3)
+    g = torch.Generator(device="cpu").manual_seed(3)
+
+    def queries(n):
+        p = torch.rand(n, 3, generator=g)
+        nn = torch.nn.functional.normalize(torch.randn(n, 3, generator=g), dim=1)
+        wo = torch.nn.functional.normalize(torch.randn(n, 3, generator=g), dim=1)
+        return torch.cat([p, nn, wo], 1).contiguous().cuda()
+
+    q = queries(n_infer)
+    ms_inf, _ = timed_passes(lambda: acc.guide_infer(q), reps, 2)
+    qt = queries(n_train)
+    dirs = torch.nn.functional.normalize(torch.randn(n_train, 3, generator=g), dim=1).cuda()
+    w = torch.rand(n_train, generator=g).cuda()
+    acc.guide_train_queries(qt, dirs, w)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        acc.guide_train_queries(qt, dirs, w)   # returns the loss to the host: synchronous
+    ms_tr = (time.perf_counter() - t0) * 1e3 / reps
+    tf_inf = n_infer * MLP_FWD_FLOP / (ms_inf * 1e-3) / 1e12
+    tf_tr = n_train * MLP_TRAIN_FLOP / (ms_tr * 1e-3) / 1e12
+    return {
+        "layers": MLP_LAYERS, "forward_flop_per_query": MLP_FWD_FLOP,
+        "train_flop_per_sample": MLP_TRAIN_FLOP,
+        "infer_queries": n_infer, "infer_ms": round(ms_inf, 4), "infer_tflops": round(tf_inf, 2),
+        "infer_frac_bf16_peak": round(tf_inf / BF16_MFMA_PEAK_TFLOPS, 4),
+        "train_samples": n_train, "train_step_ms": round(ms_tr, 4), "train_tflops": round(tf_tr, 2),
+        "train_frac_bf16_peak": round(tf_tr / BF16_MFMA_PEAK_TFLOPS, 4),
+        "note": "train step timed on the host clock (it returns the loss), so it includes one "
+                "device-to-host copy and synchronisation",
+    }
 
 
 # Guided-rendering protocol: plain PT renders PLAIN_SPP samples per pixel progressively; its
@@ -264,7 +329,6 @@ def guided_benchmark(builder):
         "train": {"minibatches_per_pass": GUIDE_STEPS, "lr": GUIDE_LR, "proposal_fraction": GUIDE_FRAC,
                   "samples_per_pass": GUIDE_SAMPLES},
     }
-    return out
 
 
 def batched_benchmark(frames=3, spp=32, max_depth=5, rr_depth=3, seed=1):
@@ -325,9 +389,67 @@ def batched_benchmark(frames=3, spp=32, max_depth=5, rr_depth=3, seed=1):
         "samples_per_s": round(W * H * spp / r, 1),
         "rays_per_s": round((cnt["closest_rays"] + cnt["shadow_rays"]) / renders[-1], 1),
         "rebuild_mtris_per_s": round(n_tris / float(np.median(builds)) / 1e6, 2),
+        "rebuild_frac_hbm": round(build_bytes(n_tris, acc.stats()["wide_nodes"])
+                                  / float(np.median(builds)) / 1e9 / HBM_PEAK_GBS, 5),
         "finite": bool(np.isfinite(img).all()), "mean_radiance": round(float(img.mean() / total), 5),
         f"rel_mse_vs_{BATCH_REF_SPP}spp_fp64": round(rel, 6),
     }
+
+
+def build_bytes(n_tris, n_nodes):
+    """Algorithmic bytes of one build: per triangle its 12 B of indices and 36 B of vertices
+    read and its 36 B reordered triangle written, plus 64 B per wide node written."""
+    return n_tris * (12 + 36 + 36) + 64 * n_nodes
+
+
+def _builder_id(name):
+    from alpine_amd import BUILDER_GPU, BUILDER_HOST
+    return BUILDER_GPU if name == "gpu" else BUILDER_HOST
+
+
+def render_pass(which, builder):
+    """One timed pass of a render config with stream launches only (no HIP graph, no
+    reference render): the workload a per-kernel profile of the path-tracing stages needs."""
+    import torch
+    from alpine_amd import scenes
+    sc = scenes.many_light_scene() if which == "many_light" else scenes.guiding_scene()
+    acc = _render_setup(sc, builder)
+    out = {"config": which, "frame": [sc.width, sc.height]}
+    if which == "many_light":
+        _timed_render(acc, sc, 1, 0)
+        dt, img, _ = _timed_render(acc, sc, sc.spp, sc.seed)
+        out.update(spp=sc.spp, seconds=round(dt, 4), samples_per_s=round(sc.width * sc.height * sc.spp / dt, 1))
+        return out
+    acc.render_reset(sc.width, sc.height, 11, slots=GUIDE_SLOTS)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    acc.render(GUIDE_CHUNK, sc.max_depth)
+    torch.cuda.synchronize()
+    out["plain_chunk_s"] = round(time.perf_counter() - t0, 4)
+    acc.guide_init(seed=1, lr=GUIDE_LR)
+    acc.render_reset(sc.width, sc.height, 21, slots=GUIDE_SLOTS)
+    t0 = time.perf_counter()
+    acc.render(GUIDE_TRAIN_SPP, sc.max_depth, guide=1, guide_warmup=1, train_steps=GUIDE_STEPS,
+               guide_frac=GUIDE_FRAC, train_samples=GUIDE_SAMPLES)
+    torch.cuda.synchronize()
+    out["train_s"] = round(time.perf_counter() - t0, 4)
+    t0 = time.perf_counter()
+    acc.render(GUIDE_CHUNK, sc.max_depth, guide=2, guide_frac=GUIDE_FRAC)
+    torch.cuda.synchronize()
+    out["guided_chunk_s"] = round(time.perf_counter() - t0, 4)
+    out.update(chunk_spp=GUIDE_CHUNK, train_spp=GUIDE_TRAIN_SPP)
+    return out
+
+
+RENDER_CONFIGS = {
+    "many_light_pass": lambda a: render_pass("many_light", _builder_id(a.builder)),
+    "guided_pass": lambda a: render_pass("guided", _builder_id(a.builder)),
+    "cornell": lambda a: cornell_benchmark(_builder_id(a.builder)),
+    "many_light": lambda a: many_light_benchmark(_builder_id(a.builder)),
+    "batched": lambda a: {"batched_32_scenes_512_32spp": batched_benchmark(frames=a.frames)},
+    "guided": lambda a: {"guided_vs_plain": guided_benchmark(_builder_id(a.builder))},
+    "mlp": lambda a: {"guide_mlp": mlp_benchmark()},
+}
 
 
 def cpu_baseline(verts, tris, org, dir, sample):
@@ -384,6 +506,9 @@ def main():
     import torch
     from alpine_amd import (BUILDER_GPU, BUILDER_HOST, TRI_REFERENCE, TRI_WATERTIGHT, GpuAccelerator,
                             scenes)
+    if args.render_config:
+        print(json.dumps(RENDER_CONFIGS[args.render_config](args)), flush=True)
+        return 0
 
     log("[bench] generating scene")
     verts, tris = scenes.microbench_scene()
@@ -395,9 +520,19 @@ def main():
     log("[bench] building BVH")
     acc = GpuAccelerator(0)
     acc.append_mesh(verts, tris)
-    builder = BUILDER_GPU if args.builder == "gpu" else BUILDER_HOST
+    builder = _builder_id(args.builder)
     acc.update_scene(leaf_threshold=2, builder=builder)
     st = acc.stats()
+    # steady-state rebuild of the same geometry (the first build also loads the kernels)
+    rebuild = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        acc.update_scene(leaf_threshold=2, builder=builder, stats=False)
+        torch.cuda.synchronize()
+        rebuild.append(time.perf_counter() - t0)
+    rebuild_s = float(np.median(rebuild))
+    b_build = build_bytes(int(st["triangles"]), int(st["wide_nodes"]))
 
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a.T)).cuda()
     o_d, d_d = dev(org), dev(dir)
@@ -428,6 +563,8 @@ def main():
     _, cn_node, cn_tri = acc.intersect_counted(co_d, cd_d)
     _, an_node, an_tri = acc.intersect_counted(o_d, d_d, any_hit=True)
     node_pr, tri_pr = n_node / N_RAYS, n_tri / N_RAYS
+    b_coh = B_RAY_IN + B_HIT_OUT + (B_NODE * cn_node + B_TRI * cn_tri) / N_RAYS
+    b_any = B_RAY_IN + B_OCC_OUT + (B_NODE * an_node + B_TRI * an_tri) / N_RAYS
     b_closest = B_RAY_IN + B_HIT_OUT + B_NODE * node_pr + B_TRI * tri_pr
     achieved = b_closest * N_RAYS / (ms * 1e-3) / 1e9
     bw = hbm_copy_gbs()
@@ -463,12 +600,18 @@ def main():
         "extra": {
             "wall_ms_per_step": round(wall_ms, 4),
             "coherent_closest_mrays": round(N_RAYS / (ms_coh * 1e-3) / 1e6, 2),
-            "coherent_bytes_per_ray": round(B_RAY_IN + B_HIT_OUT + (B_NODE * cn_node + B_TRI * cn_tri) / N_RAYS, 1),
+            "coherent_bytes_per_ray": round(b_coh, 1),
+            "coherent_frac_hbm": round(b_coh * N_RAYS / (ms_coh * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "incoherent_any_mrays": round(N_RAYS / (ms_any * 1e-3) / 1e6, 2),
             "incoherent_any_tfar_mrays": round(N_RAYS / (ms_any_t * 1e-3) / 1e6, 2),
             "incoherent_closest_watertight_mrays": round(N_RAYS / (ms_wt * 1e-3) / 1e6, 2),
-            "any_bytes_per_ray": round(B_RAY_IN + B_OCC_OUT + (B_NODE * an_node + B_TRI * an_tri) / N_RAYS, 1),
-            "build_ms": round(st["build_ms"], 1), "wide_nodes": int(st["wide_nodes"]),
+            "any_bytes_per_ray": round(b_any, 1),
+            "any_frac_hbm": round(b_any * N_RAYS / (ms_any * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "build_ms_first": round(st["build_ms"], 1), "rebuild_ms": round(rebuild_s * 1e3, 2),
+            "rebuild_mtris_per_s": round(st["triangles"] / rebuild_s / 1e6, 1),
+            "build_bytes": b_build,
+            "rebuild_frac_hbm": round(b_build / rebuild_s / 1e9 / HBM_PEAK_GBS, 5),
+            "wide_nodes": int(st["wide_nodes"]),
             "sah_cost": round(st["sah_cost"], 3), "device_mb": round(st["device_bytes"] / 2**20, 1),
         },
         "cpu_baseline": None,
