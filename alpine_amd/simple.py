@@ -147,6 +147,8 @@ def main(argv=None):
     ap.add_argument("--accelerator", required=True, choices=["gpu"])
     ap.add_argument("--lightSampler", required=True, choices=["uniform", "power", "bvh"])
     ap.add_argument("--denoiser", action="store_true", help="not available (no OIDN)")
+    ap.add_argument("--environment", help="Radiance .hdr environment map seen on a miss "
+                    "(Alpine::loadEnvironmentMap, alpine.cpp:253-257); default: white background")
     a = ap.parse_args(argv)
     ext = os.path.splitext(a.input)[1]
     if ext not in (".obj", ".glb"):
@@ -168,7 +170,17 @@ def main(argv=None):
     except (OSError, ValueError, KeyError, IndexError, gltf.GltfError) as e:
         print(f"ERROR: failed to load {a.input}: {e}")
         return 1
+    env = None
+    if a.environment:
+        from alpine_amd import hdr
+        try:
+            env = hdr.read_hdr(a.environment)
+        except (OSError, ValueError) as e:
+            print(f"ERROR: failed to load {a.environment}: {e}")
+            return 1
     acc = aa.GpuAccelerator(0)
+    if env is not None:
+        acc.set_environment(env)
     if ext == ".obj":
         verts, tris, tri_mat, colours = scene
         acc.append_mesh(verts, tris)
