@@ -185,8 +185,16 @@ int apg_debug_light_sampler(const apg_light* lights, uint32_t nl, uint32_t sampl
         for (uint32_t i = 0; i < nl; ++i) w[i] = length(ld3(dl[i].power));
         build_alias(w, ap, aq, aa);
     }
-    if (sampler == LS_BVH && nl) build_light_bvh(dl, nodes, leaf);
-    LightSamplerView v{ sampler, nl, dl.data(), ap.data(), aq.data(), aa.data(), nodes.data(), leaf.data() };
+    std::vector<DevLightPair> pairs;
+    std::vector<int32_t> parent;
+    int32_t root_light = -1;
+    if (sampler == LS_BVH && nl)
+    {
+        build_light_bvh(dl, nodes, leaf);
+        pack_light_bvh(nodes, pairs, parent, root_light);
+    }
+    LightSamplerView v{ sampler, nl, dl.data(), ap.data(), aq.data(), aa.data(), pairs.data(), parent.data(),
+                        leaf.data(), root_light };
     for (uint32_t i = 0; i < n; ++i)
     {
         float p = 0.0f;

@@ -67,8 +67,10 @@ struct RenderState
     float* alias_p = nullptr;
     float* alias_q = nullptr;
     uint32_t* alias_a = nullptr;
-    DevLightNode* lnodes = nullptr;
+    DevLightPair* lpairs = nullptr;
+    int32_t* lparent = nullptr;
     int32_t* leaf_of = nullptr;
+    int32_t root_light = -1;
     DevCamera cam{};
     bool cam_set = false;
     DevCamera* tile_cams = nullptr;
@@ -843,8 +845,10 @@ LightSamplerView view_of(const RenderState* R)
     v.alias_p = R->alias_p;
     v.alias_q = R->alias_q;
     v.alias_a = R->alias_a;
-    v.nodes = R->lnodes;
+    v.pairs = R->lpairs;
+    v.parent_slot = R->lparent;
     v.leaf_of_light = R->leaf_of;
+    v.root_light = R->root_light;
     return v;
 }
 
@@ -862,7 +866,8 @@ void render_state_free(RenderState* R)
     dfree(R->alias_p);
     dfree(R->alias_q);
     dfree(R->alias_a);
-    dfree(R->lnodes);
+    dfree(R->lpairs);
+    dfree(R->lparent);
     dfree(R->leaf_of);
     dfree(R->tile_cams);
     dfree(R->tile_roots);
@@ -1109,8 +1114,10 @@ int install_lights(Context* ctx, const std::vector<DevLight>& dl, uint32_t sampl
     dfree(R->alias_p);
     dfree(R->alias_q);
     dfree(R->alias_a);
-    dfree(R->lnodes);
+    dfree(R->lpairs);
+    dfree(R->lparent);
     dfree(R->leaf_of);
+    R->root_light = -1;
     int rc;
     if ((rc = dmalloc(ctx, R->lights, n))) return rc;
     if (n) APG_HIP_CHECK(ctx, hipMemcpy(R->lights, dl.data(), n * sizeof(DevLight), hipMemcpyHostToDevice));
@@ -1131,8 +1138,14 @@ int install_lights(Context* ctx, const std::vector<DevLight>& dl, uint32_t sampl
         std::vector<DevLightNode> nodes;
         std::vector<int32_t> leaf;
         build_light_bvh(dl, nodes, leaf);
-        if ((rc = dmalloc(ctx, R->lnodes, nodes.size())) || (rc = dmalloc(ctx, R->leaf_of, n))) return rc;
-        APG_HIP_CHECK(ctx, hipMemcpy(R->lnodes, nodes.data(), nodes.size() * sizeof(DevLightNode), hipMemcpyHostToDevice));
+        std::vector<DevLightPair> pairs;
+        std::vector<int32_t> parent;
+        pack_light_bvh(nodes, pairs, parent, R->root_light);
+        if ((rc = dmalloc(ctx, R->lpairs, pairs.size())) || (rc = dmalloc(ctx, R->lparent, parent.size())) ||
+            (rc = dmalloc(ctx, R->leaf_of, n)))
+            return rc;
+        APG_HIP_CHECK(ctx, hipMemcpy(R->lpairs, pairs.data(), pairs.size() * sizeof(DevLightPair), hipMemcpyHostToDevice));
+        APG_HIP_CHECK(ctx, hipMemcpy(R->lparent, parent.data(), parent.size() * 4, hipMemcpyHostToDevice));
         APG_HIP_CHECK(ctx, hipMemcpy(R->leaf_of, leaf.data(), n * 4, hipMemcpyHostToDevice));
     }
     R->nlights = n;

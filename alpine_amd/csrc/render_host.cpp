@@ -243,6 +243,47 @@ void build_light_bvh(const std::vector<DevLight>& lights, std::vector<DevLightNo
     b.build(ids, -1);
 }
 
+// The device form of a built light BVH: one pair record per inner node with both children's
+// centre, half diagonal and power, a parent-slot array for the pdf walk, and the light of a
+// root that is itself a leaf.
+void pack_light_bvh(const std::vector<DevLightNode>& nodes, std::vector<DevLightPair>& pairs,
+                    std::vector<int32_t>& parent_slot, int32_t& root_light)
+{
+    pairs.assign(nodes.empty() ? 0 : nodes.size(), DevLightPair{});
+    parent_slot.assign(nodes.size(), -1);
+    root_light = -1;
+    auto child = [&](int32_t id) {
+        const DevLightNode& nd = nodes[size_t(id)];
+        const f3 lo = ld3(nd.lo), hi = ld3(nd.hi);
+        const f3 c = (lo + hi) * 0.5f;
+        DevLightChild r{};
+        r.c[0] = c.x; r.c[1] = c.y; r.c[2] = c.z;
+        r.br = 0.5f * length(hi - lo);
+        r.power = nd.power;
+        r.light = nd.light;
+        const bool inner = nd.child0 >= 0 || nd.child1 >= 0;
+        r.node = nd.light >= 0 && !inner ? -1 : (nd.child0 >= 0 && nd.child1 >= 0 ? id : -2);
+        if (r.node == -2) r.light = -1;
+        return r;
+    };
+    if (nodes.empty()) return;
+    if (nodes[0].child0 < 0 && nodes[0].child1 < 0)
+    {
+        pairs[0].ch[0] = child(0);
+        root_light = nodes[0].light;
+        return;
+    }
+    for (size_t i = 0; i < nodes.size(); ++i)
+    {
+        const DevLightNode& nd = nodes[i];
+        if (nd.child0 < 0 || nd.child1 < 0) continue;
+        pairs[i].ch[0] = child(nd.child0);
+        pairs[i].ch[1] = child(nd.child1);
+        parent_slot[size_t(nd.child0)] = int32_t(2 * i);
+        parent_slot[size_t(nd.child1)] = int32_t(2 * i + 1);
+    }
+}
+
 // AliasTable (alias_table.cpp:8-63)
 void build_alias(const std::vector<float>& w, std::vector<float>& p, std::vector<float>& q,
                  std::vector<uint32_t>& alias)

@@ -11,6 +11,8 @@ namespace apg {
 DevLight prepare_light(const apg_light& s);
 void build_light_bvh(const std::vector<DevLight>& lights, std::vector<DevLightNode>& nodes,
                      std::vector<int32_t>& leaf_of);
+void pack_light_bvh(const std::vector<DevLightNode>& nodes, std::vector<DevLightPair>& pairs,
+                    std::vector<int32_t>& parent_slot, int32_t& root_light);
 void build_alias(const std::vector<float>& w, std::vector<float>& p, std::vector<float>& q,
                  std::vector<uint32_t>& alias);
 void camera_setup(const apg_camera& c, DevCamera& out);

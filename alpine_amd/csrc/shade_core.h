@@ -513,6 +513,7 @@ SHD float light_pdf(const DevLight& L, f3 hit, f3 wi, float& dist)
 //      bvh_light_sampler.cpp) ----
 enum : uint32_t { LS_UNIFORM = 0, LS_POWER = 1, LS_BVH = 2 };
 
+// Light BVH node as the host builder produces it (bvh_light_sampler.cpp:167-240).
 struct DevLightNode
 {
     float lo[3], hi[3];
@@ -523,6 +524,27 @@ struct DevLightNode
     int32_t pad;
 };
 
+// What LightNode::importance reads of one node, with the box reduced ahead of time to
+// its centre (lo + hi) * 0.5 and half diagonal 0.5 * |hi - lo| (the same float
+// expressions importance() evaluates, so the values are identical).
+struct DevLightChild
+{
+    float c[3];
+    float br;
+    float power;
+    int32_t node;   // >= 0: an inner node (index of its pair); -1: a leaf; -2: an inner node
+                    // without two children (sampling ends there without a light)
+    int32_t light;  // leaf: light index, else -1
+};
+
+// The two children of inner node i, read together in one 64-byte record at every step of
+// the descent (the importance of both children decides the branch).
+struct __attribute__((aligned(64))) DevLightPair
+{
+    DevLightChild ch[2];
+    int32_t pad[2];
+};
+
 struct LightSamplerView
 {
     uint32_t kind;
@@ -531,18 +553,19 @@ struct LightSamplerView
     const float* alias_p;
     const float* alias_q;
     const uint32_t* alias_a;
-    const DevLightNode* nodes;
+    const DevLightPair* pairs;     // by inner node; a single-light tree keeps its leaf in pairs[0].ch[0]
+    const int32_t* parent_slot;    // by node: 2 * parent + side, -1 for the root
     const int32_t* leaf_of_light;
+    int32_t root_light;            // the light when the root is a leaf, else -1
 };
 
 // LightNode::importance (bvh_light_sampler.cpp:24-51)
-SHD float node_importance(const DevLightNode& nd, f3 p, f3 n)
+SHD float light_importance(const DevLightChild& nd, f3 p, f3 n)
 {
-    const f3 lo = ld3(nd.lo), hi = ld3(nd.hi);
-    f3 w = (lo + hi) * 0.5f - p;
+    f3 w = ld3(nd.c) - p;
     const float d2 = dot(w, w);
-    float imp = nd.power / max_s(d2, 0.5f * length(hi - lo));
-    const float br = 0.5f * length(hi - lo);
+    float imp = nd.power / max_s(d2, nd.br);
+    const float br = nd.br;
     const float br2 = br * br;
     if (d2 > 0.0f && d2 > br2)
     {
@@ -557,7 +580,7 @@ SHD float node_importance(const DevLightNode& nd, f3 p, f3 n)
     return imp;
 }
 
-// LightSampler::sample; returns the light index or -1.
+// BvhLightSampler::sample (bvh_light_sampler.cpp:267-319); returns the light index or -1.
 SHD int32_t light_sampler_sample(const LightSamplerView& S, float u, f3 hit, f3 ns, float& pdf)
 {
     if (S.n == 0) return -1;
@@ -577,31 +600,38 @@ SHD int32_t light_sampler_sample(const LightSamplerView& S, float u, f3 hit, f3 
         pdf = S.alias_p[idx];
         return int32_t(idx);
     }
+    if (S.root_light >= 0)
+    {
+        if (!(light_importance(S.pairs[0].ch[0], hit, ns) > 0.0f)) return -1;
+        pdf = 1.0f;
+        return S.root_light;
+    }
     int32_t node = 0;
     float p = 1.0f;
     for (;;)
     {
-        const DevLightNode& nd = S.nodes[node];
-        if (nd.child0 >= 0 && nd.child1 >= 0)
+        const DevLightPair& P = S.pairs[node];
+        const float w0 = light_importance(P.ch[0], hit, ns);
+        const float w1 = light_importance(P.ch[1], hit, ns);
+        if (w0 == 0.0f && w1 == 0.0f) return -1;
+        const float up = (w0 + w1) * u;
+        const int idx = up < w0 ? 0 : 1;
+        const float wc = idx == 0 ? w0 : w1;
+        p *= wc / (w0 + w1);
+        u = idx == 0 ? min_s(up / w0, kOneMinusEps) : min_s((up - w0) / w1, kOneMinusEps);
+        const DevLightChild& C = P.ch[idx];
+        if (C.node >= 0)
         {
-            const float w0 = node_importance(S.nodes[nd.child0], hit, ns);
-            const float w1 = node_importance(S.nodes[nd.child1], hit, ns);
-            if (w0 == 0.0f && w1 == 0.0f) return -1;
-            const float up = (w0 + w1) * u;
-            const int idx = up < w0 ? 0 : 1;
-            node = idx == 0 ? nd.child0 : nd.child1;
-            p *= (idx == 0 ? w0 : w1) / (w0 + w1);
-            u = idx == 0 ? min_s(up / w0, kOneMinusEps) : min_s((up - w0) / w1, kOneMinusEps);
+            node = C.node;
+            continue;
         }
-        else
+        // a leaf is returned when its importance (wc, the same evaluation) is positive
+        if (C.light >= 0 && wc > 0.0f)
         {
-            if (nd.light >= 0 && node_importance(nd, hit, ns) > 0.0f)
-            {
-                pdf = p;
-                return nd.light;
-            }
-            return -1;
+            pdf = p;
+            return C.light;
         }
+        return -1;
     }
 }
 
@@ -611,17 +641,19 @@ SHD float light_sampler_pdf(const LightSamplerView& S, int32_t light, f3 hit, f3
     if (S.n == 0 || light < 0 || uint32_t(light) >= S.n) return 0.0f;
     if (S.kind == LS_UNIFORM) return 1.0f / float(S.n);
     if (S.kind == LS_POWER) return S.alias_p[light];
-    int32_t node = S.leaf_of_light[light];
-    if (!(node_importance(S.nodes[node], hit, ns) > 0.0f)) return 0.0f;
+    if (S.root_light >= 0)
+        return light == S.root_light && light_importance(S.pairs[0].ch[0], hit, ns) > 0.0f ? 1.0f : 0.0f;
+    int32_t ps = S.parent_slot[S.leaf_of_light[light]];
+    if (!(light_importance(S.pairs[ps >> 1].ch[ps & 1], hit, ns) > 0.0f)) return 0.0f;
     float p = 1.0f;
-    while (S.nodes[node].parent >= 0)
+    while (ps >= 0)
     {
-        const DevLightNode& par = S.nodes[S.nodes[node].parent];
-        const float w0 = node_importance(S.nodes[par.child0], hit, ns);
-        const float w1 = node_importance(S.nodes[par.child1], hit, ns);
+        const DevLightPair& P = S.pairs[ps >> 1];
+        const float w0 = light_importance(P.ch[0], hit, ns);
+        const float w1 = light_importance(P.ch[1], hit, ns);
         if (w0 == 0.0f && w1 == 0.0f) return 0.0f;
-        p *= (par.child0 == node ? w0 : w1) / (w0 + w1);
-        node = S.nodes[node].parent;
+        p *= ((ps & 1) == 0 ? w0 : w1) / (w0 + w1);
+        ps = S.parent_slot[ps >> 1];
     }
     return p;
 }

@@ -124,7 +124,7 @@ def test_simple_renders_with_an_environment_map(tmp_path):
     print("[gpu] simple cli with environment map", flush=True)
     (tmp_path / "a.obj").write_text("v -0.2 -0.2 0\nv 0.2 -0.2 0\nv 0 0.2 0\nf 1 2 3\n")
     rgbe = np.zeros((8, 16, 4), np.uint8)
-    rgbe[...] = (16, 32, 255, 128)    # (0.125, 0.25, 1.99) in linear radiance
+    rgbe[...] = (16, 32, 255, 128)    # mantissa x 2^-8: (0.0625, 0.125, 0.996)
     env = str(tmp_path / "env.hdr")
     write_hdr(env, rgbe, rle=True)
     out = str(tmp_path / "o.pfm")
@@ -135,4 +135,4 @@ def test_simple_renders_with_an_environment_map(tmp_path):
         raw = open(out, "rb").read()
         img = np.frombuffer(raw[-256 * 256 * 12:], np.float32).reshape(256, 256, 3)
     corner = img[0, 0]
-    np.testing.assert_allclose(corner, [16 / 128, 32 / 128, 255 / 128], rtol=1e-6)
+    np.testing.assert_allclose(corner, [16 / 256, 32 / 256, 255 / 256], rtol=1e-6)
