@@ -265,6 +265,7 @@ GUIDE_PLAIN_SPP, GUIDE_REF_SPP, GUIDE_SLOTS, GUIDE_CHUNK = 1024, 4096, 4, 32
 # FP64 references of the many-light frame and of the batched frame
 ML_REF_SPP, BATCH_REF_SPP = 1024, 256
 GUIDE_TRAIN_SPP, GUIDE_STEPS, GUIDE_LR, GUIDE_FRAC, GUIDE_SAMPLES = 64, 32, 5e-3, 0.5, 1 << 19
+GUIDE_DEPTH = 0   # path vertices below this depth sample the proposal (0 = every vertex)
 
 
 def guided_benchmark(builder):
@@ -308,7 +309,7 @@ def guided_benchmark(builder):
     reach = t_g if curve_g[0][2] <= target else None
     while reach is None and tot < 4 * GUIDE_PLAIN_SPP:
         t_g += timed(lambda: acc.render(GUIDE_CHUNK // 2, sc.max_depth, guide=2, guide_frac=GUIDE_FRAC,
-                                        graph=True))
+                                        guide_depth=GUIDE_DEPTH, graph=True))
         img, _, tot = acc.render_read()
         e = scenes.rel_mse(img / tot, ref)
         curve_g.append((tot, t_g, e))
