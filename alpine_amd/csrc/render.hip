@@ -1118,6 +1118,9 @@ int install_lights(Context* ctx, const std::vector<DevLight>& dl, uint32_t sampl
     dfree(R->lparent);
     dfree(R->leaf_of);
     R->root_light = -1;
+    // no light sampler until every buffer below is in place (a failed allocation leaves none)
+    R->ls_set = false;
+    R->nlights = 0;
     int rc;
     if ((rc = dmalloc(ctx, R->lights, n))) return rc;
     if (n) APG_HIP_CHECK(ctx, hipMemcpy(R->lights, dl.data(), n * sizeof(DevLight), hipMemcpyHostToDevice));
