@@ -89,7 +89,7 @@ class RenderParams(C.Structure):
                 ("fp64_accum", C.c_uint32), ("background", C.c_float * 3), ("guide", C.c_uint32),
                 ("train_steps", C.c_uint32), ("guide_warmup", C.c_uint32),
                 ("guide_frac", C.c_float), ("graph", C.c_uint32), ("train_samples", C.c_uint32),
-                ("guide_depth", C.c_uint32), ("aovs", C.c_uint32)]
+                ("guide_depth", C.c_uint32), ("aovs", C.c_uint32), ("count_fetches", C.c_uint32)]
 
 
 MAT_DIELECTRIC, MAT_CONDUCTOR = 0, 1
@@ -137,7 +137,8 @@ ABI_SYMBOLS = [
     "apg_occluded", "apg_intersect_roots", "apg_occluded_roots", "apg_intersect_counted", "apg_scene_triangles", "apg_scene_leaf_keys",
     "apg_scene_wide_nodes", "apg_materials_set", "apg_lights_set", "apg_light_sampler_sample",
     "apg_light_sampler_pdf", "apg_render_reset", "apg_render_reset_slots", "apg_render_set_camera", "apg_render_set_tiles", "apg_render",
-    "apg_render_read", "apg_render_postprocess", "apg_render_counters", "apg_guide_param_count",
+    "apg_render_read", "apg_render_postprocess", "apg_render_counters", "apg_render_fetch_counts",
+    "apg_guide_param_count",
     "apg_guide_init",
     "apg_guide_params", "apg_guide_forward", "apg_guide_infer", "apg_guide_train_step",
     "apg_guide_train_queries",
@@ -208,6 +209,7 @@ def lib():
         "apg_mesh_set_targets": (i32, [vp, u32, u32, vp, vp, vp]),
         "apg_mesh_morph": (i32, [vp, u32, vp, vp]),
         "apg_render_postprocess": (i32, [vp, i32, vp, vp]),
+        "apg_render_fetch_counts": (i32, [vp] + [C.POINTER(C.c_uint64)] * 4),
         "apg_render_counters": (i32, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                       C.POINTER(C.c_uint64)]),
         "apg_guide_param_count": (i32, []),
@@ -539,10 +541,10 @@ class GpuAccelerator:
 
     def render(self, spp, max_depth, rr_depth=0, background=(0.0, 0.0, 0.0), fp64=False,
                guide=0, train_steps=4, guide_warmup=1, guide_frac=0.5, graph=False,
-               train_samples=0, guide_depth=0, aovs=False, stream=None):
+               train_samples=0, guide_depth=0, aovs=False, count_fetches=False, stream=None):
         p = RenderParams(spp, max_depth, rr_depth, 1 if fp64 else 0, _f3(background), guide,
                          train_steps, guide_warmup, guide_frac, 1 if graph else 0, train_samples,
-                         guide_depth, 1 if aovs else 0)
+                         guide_depth, 1 if aovs else 0, 1 if count_fetches else 0)
         self._rc(self._lib.apg_render(self._h, C.byref(p), self._stream(stream)))
 
     def render_read_aovs(self):
@@ -590,6 +592,12 @@ class GpuAccelerator:
         self._rc(self._lib.apg_render_postprocess(self._h, int(bool(tonemap)), frame.data_ptr(),
                                                   self._stream(stream)))
         return frame
+
+    def render_fetch_counts(self):
+        """Node and triangle fetches of the count_fetches passes since render_reset."""
+        v = [C.c_uint64() for _ in range(4)]
+        self._rc(self._lib.apg_render_fetch_counts(self._h, *[C.byref(x) for x in v]))
+        return dict(zip(["closest_nodes", "closest_tris", "shadow_nodes", "shadow_tris"], [x.value for x in v]))
 
     def render_counters(self):
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()

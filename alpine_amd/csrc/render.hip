@@ -1476,7 +1476,8 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
         {
             apg_rays rays{ in.o[0], in.o[1], in.o[2], in.d[0], in.d[1], in.d[2], nullptr };
             apg_hits hits{ R->hprim, R->ht, R->hb0, R->hb1 };
-            int rc = launch_trace_raw(ctx, rays, P, R->counts, &hits, false, nullptr, nullptr, st,
+            int rc = launch_trace_raw(ctx, rays, P, R->counts, &hits, false, nullptr,
+                                      prm->count_fetches ? R->stats + 3 : nullptr, st,
                                       forest ? in.root : nullptr);
             if (rc) return rc;
             S.in = in;
@@ -1493,7 +1494,8 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
             else
                 k_shade<false><<<sgrid, 256, 0, st>>>(S);
             apg_rays srays{ R->qs.o[0], R->qs.o[1], R->qs.o[2], R->qs.d[0], R->qs.d[1], R->qs.d[2], R->qs.tmax };
-            rc = launch_trace_raw(ctx, srays, 2 * P, R->counts + 2, nullptr, true, R->occ, nullptr, st,
+            rc = launch_trace_raw(ctx, srays, 2 * P, R->counts + 2, nullptr, true, R->occ,
+                                  prm->count_fetches ? R->stats + 5 : nullptr, st,
                                   forest ? R->qs.root : nullptr);
             if (rc) return rc;
             k_shadow_vis<<<vgrid, 256, 0, st>>>(R->counts, R->qs.path, R->occ, R->vis);
@@ -1525,7 +1527,7 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
         put(&ctx->cfg, sizeof(ctx->cfg));
         const uintptr_t extra[] = { P, R->W, R->H, uintptr_t(R->accum), uintptr_t(R->accum64), prm->fp64_accum,
                                     uintptr_t(R->gfeat), uintptr_t(R->gout), guide, uintptr_t(ctx->spill),
-                                    prm->guide_depth };
+                                    prm->guide_depth, prm->count_fetches };
         put(extra, sizeof(extra));
         if (!R->gexec || key != R->gkey)
         {
@@ -1668,6 +1670,25 @@ int apg_render_postprocess(apg_context* ctx, int tonemap, uint8_t* frame, void* 
     k_postprocess<<<nblocks(n3, 256), 256, 0, stream>>>(R->px, n3, float(R->total_spp),
                                                         tonemap ? 1u : 0u, frame);
     APG_HIP_CHECK(ctx, hipGetLastError());
+    return APG_OK;
+}
+
+int apg_render_fetch_counts(const apg_context* cctx, uint64_t* closest_nodes, uint64_t* closest_tris,
+                            uint64_t* shadow_nodes, uint64_t* shadow_tris)
+{
+    if (!cctx) return APG_E_INVALID;
+    apg_context* ctx = const_cast<apg_context*>(cctx);
+    RenderState* R = state_of(ctx);
+    unsigned long long h[8] = {};
+    if (R->stats)
+    {
+        if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+        APG_HIP_CHECK(ctx, hipMemcpy(h, R->stats, sizeof(h), hipMemcpyDeviceToHost));
+    }
+    if (closest_nodes) *closest_nodes = h[3];
+    if (closest_tris) *closest_tris = h[4];
+    if (shadow_nodes) *shadow_nodes = h[5];
+    if (shadow_tris) *shadow_tris = h[6];
     return APG_OK;
 }
 

@@ -196,14 +196,42 @@ def many_light_benchmark(builder):
     _, img16, _ = _timed_render(acc, sc, 16, sc.seed + 7)
     _, _, ref64 = _timed_render(acc, sc, ML_REF_SPP, 1000, fp64=True, graph=True)
     ref = ref64 / ML_REF_SPP
+    bps = render_bytes_per_sample(acc, sc)
+    sps = sc.width * sc.height * sc.spp / dt
     out["many_light_1080p_64spp"] = {
         "triangles": int(len(sc.tris)), "lights": len(sc.lights), "seconds": round(dt, 3),
         "samples_per_s": round(sc.width * sc.height * sc.spp / dt, 1),
         "rays_per_s": round((c["closest_rays"] + c["shadow_rays"]) / dt, 1),
         "finite": bool(np.isfinite(img).all()),
         "rel_mse_vs_1024spp_fp64": round(scenes.rel_mse(img / sc.spp, ref), 6),
-        "rel_mse_16spp_vs_1024spp_fp64": round(scenes.rel_mse(img16 / 16, ref), 6)}
+        "rel_mse_16spp_vs_1024spp_fp64": round(scenes.rel_mse(img16 / 16, ref), 6),
+        "bytes_per_sample": bps, "samples_frac_hbm": round(bps["total"] * sps / 1e9 / HBM_PEAK_GBS, 4)}
     return out
+
+
+# Algorithmic queue and path-state bytes per shaded path vertex: the shade stage reads the ray
+# (origin, direction, path index: 28 B), its hit (16 B) and the path state (sampler 16 B,
+# throughput 12 B, radiance 12 B) and writes the continuation ray (28 B), the path state (40 B)
+# and two shadow rays (origin, direction, tmax, path index: 2 x 32 B); the two NEE terms
+# (2 x 12 B) are written by shade and read by resolve, with the two occlusion bytes.
+B_PATH_VERTEX = (28 + 16 + 40) + (28 + 40 + 2 * 32) + 2 * (2 * 12) + 2
+
+
+def render_bytes_per_sample(acc, sc, spp=4, seed=77):
+    """Algorithmic HBM bytes per sample of a render config, from one pass of the instrumented
+    traversal: closest rays 28 B in + 16 B hit, shadow rays 28 B + 4 B result, 64 B per node and
+    36 B per triangle fetched, plus B_PATH_VERTEX per shaded vertex."""
+    acc.render_reset(sc.width, sc.height, seed)
+    acc.render(spp, sc.max_depth, sc.rr_depth, sc.background, count_fetches=True)
+    c, f = acc.render_counters(), acc.render_fetch_counts()
+    n = sc.width * sc.height * spp
+    trav = (c["closest_rays"] * (B_RAY_IN + B_HIT_OUT) + c["shadow_rays"] * (B_RAY_IN + B_OCC_OUT)
+            + B_NODE * (f["closest_nodes"] + f["shadow_nodes"]) + B_TRI * (f["closest_tris"] + f["shadow_tris"]))
+    queue = c["vertices"] * B_PATH_VERTEX
+    return {"total": round((trav + queue) / n, 1), "traversal": round(trav / n, 1), "queues": round(queue / n, 1),
+            "closest_rays": round(c["closest_rays"] / n, 3), "shadow_rays": round(c["shadow_rays"] / n, 3),
+            "nodes_per_closest_ray": round(f["closest_nodes"] / max(c["closest_rays"], 1), 2),
+            "nodes_per_shadow_ray": round(f["shadow_nodes"] / max(c["shadow_rays"], 1), 2)}
 
 
 # The guide network: 96 -> 64 -> 64 -> 64 -> 64 -> 48 (bias adds not counted).  Forward FLOPs

@@ -289,6 +289,8 @@ typedef struct apg_render_params {
                                 * ones BSDF sampling only (no inference); 0 = all */
     uint32_t aovs;             /* 1: also accumulate the first hit's base colour and shading normal
                                 * (the albedo / normal render targets, alpine.cpp:333-337) */
+    uint32_t count_fetches;    /* 1: trace with the instrumented traversal and accumulate its node /
+                                * triangle fetch counts (apg_render_fetch_counts); no counterpart */
 } apg_render_params;
 
 /* Allocate per-pixel samplers (seeded with pixel index + seed, alpine.cpp:259-269) and clear
@@ -327,6 +329,11 @@ int apg_render_postprocess(apg_context* ctx, int tonemap, uint8_t* frame, void* 
  * rays, path vertices shaded. */
 int apg_render_counters(const apg_context* ctx, uint64_t* closest_rays, uint64_t* shadow_rays,
                         uint64_t* vertices);
+/* Traversal fetch counts accumulated since apg_render_reset by passes with count_fetches set
+ * (host out): 64-byte node records and triangles fetched by closest-hit and by shadow rays.
+ * No reference counterpart (the byte model of the render's roofline). */
+int apg_render_fetch_counts(const apg_context* ctx, uint64_t* closest_nodes, uint64_t* closest_tris,
+                            uint64_t* shadow_nodes, uint64_t* shadow_tris);
 /* Guided rendering: training samples of the last pass and the last training loss (host). */
 int apg_render_guide_stats(const apg_context* ctx, uint32_t* train_samples, float* loss);
 

@@ -203,3 +203,25 @@ def test_graph_replay_matches_stream_launches():
         outs[graph], _, n = acc.render_read()
         assert n == 16
     np.testing.assert_array_equal(outs[False], outs[True])
+
+
+@pytest.mark.gpu
+def test_counted_render_is_identical_and_counts_fetches():
+    """The instrumented traversal (count_fetches) renders the same image bit for bit and
+    reports plausible fetch counts: every traced ray fetches at least its root node."""
+    print("[gpu] counted render", flush=True)
+    sc = scenes.cornell_scene(64, 64, spp=4)
+    acc, _, _, _ = gpu_render(sc, spp=2)
+    outs = {}
+    for counted in (False, True):
+        acc.render_reset(sc.width, sc.height, 5)
+        acc.render(4, sc.max_depth, sc.rr_depth, sc.background, count_fetches=counted)
+        outs[counted], _, _ = acc.render_read()
+        f, c = acc.render_fetch_counts(), acc.render_counters()
+        if counted:
+            assert f["closest_nodes"] >= c["closest_rays"] > 0
+            assert f["shadow_nodes"] >= c["shadow_rays"] > 0
+            assert f["closest_tris"] > 0 and f["shadow_tris"] > 0
+        else:
+            assert f == {"closest_nodes": 0, "closest_tris": 0, "shadow_nodes": 0, "shadow_tris": 0}
+    np.testing.assert_array_equal(outs[False], outs[True])
