@@ -325,8 +325,8 @@ int apg_render_set_environment(apg_context* ctx, const float* rgb, uint32_t widt
 /* Alpine::postProcess without denoising (alpine.cpp:446-483): mean colour, ACES fit when
  * tonemap != 0, clamp, sRGB, bytes.  frame: width x height x 3 bytes, device memory. */
 int apg_render_postprocess(apg_context* ctx, int tonemap, uint8_t* frame, void* stream);
-/* Ray and path counters of the last apg_render call (host out): closest-hit rays, shadow
- * rays, path vertices shaded. */
+/* Ray and path counters accumulated since apg_render_reset (host out): closest-hit rays,
+ * shadow rays, path vertices shaded. */
 int apg_render_counters(const apg_context* ctx, uint64_t* closest_rays, uint64_t* shadow_rays,
                         uint64_t* vertices);
 /* Traversal fetch counts accumulated since apg_render_reset by passes with count_fetches set
