@@ -103,6 +103,7 @@ struct Context
     uint32_t* counters = nullptr;     // [0] ray cursor, [1] overflow flag, [2..5] fetch counts
     uint64_t* spill = nullptr;        // per persistent thread stack spill
     size_t spill_threads = 0;
+    uint32_t* readback = nullptr;     // pinned host words the GPU builder reads level sizes into
 
     // shading data and the path tracer (render.hip)
     RenderState* render = nullptr;

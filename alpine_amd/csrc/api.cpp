@@ -29,6 +29,7 @@ Context::~Context()
     if (d_xyz) (void)hipFree(d_xyz);
     if (d_tri) (void)hipFree(d_tri);
     if (spill) (void)hipFree(spill);
+    if (readback) (void)hipHostFree(readback);
     for (Shape& sh : shapes)
     {
         if (sh.d_base) (void)hipFree(sh.d_base);

@@ -1206,6 +1206,9 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
     uint32_t host_counters[2] = { R, 0 };
     APG_HIP_CHECK(ctx, hipMemcpyAsync(S.counters, host_counters, 8, hipMemcpyHostToDevice, s));
 
+    // level sizes come back through pinned words (a pageable read-back is staged and slower)
+    if (!ctx->readback) APG_HIP_CHECK(ctx, hipHostMalloc(&ctx->readback, 4 * sizeof(uint32_t)));
+    uint32_t* rb = ctx->readback;
     uint32_t begin = 0, end = R, levels = 0;
     const uint32_t nchunks = blocks_for(n, CHUNK);
     while (begin < end)
@@ -1218,8 +1221,7 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
         k_bounds_large<<<2 * nchunks + 1, BLOCK, 0, s>>>(S, begin, width, S.scan);
         k_bounds_small<<<blocks_for(width, BLOCK / 64u), BLOCK, 0, s>>>(S, begin, end);
         k_bounds_tiny<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end);
-        uint32_t zero = 0;
-        APG_HIP_CHECK(ctx, hipMemcpyAsync(S.counters + 1, &zero, 4, hipMemcpyHostToDevice, s));
+        APG_HIP_CHECK(ctx, hipMemsetAsync(S.counters + 1, 0, 4, s));
         k_decide<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end, thr);
         k_bin_large<<<2 * nchunks + 1, BLOCK, 0, s>>>(S, begin, width, S.scan);
         k_bin_small<<<blocks_for(width, BLOCK / 64u), BLOCK, 0, s>>>(S, begin, end);
@@ -1230,10 +1232,10 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
         k_scatter<<<blocks_for(n, 256), 256, 0, s>>>(S, n, begin, end);
         std::swap(S.idx, S.idx_tmp);
         std::swap(S.snode, S.snode_tmp);
-        uint32_t next_end = 0;
-        APG_HIP_CHECK(ctx, hipMemcpyAsync(&next_end, S.counters, 4, hipMemcpyDeviceToHost, s));
+        APG_HIP_CHECK(ctx, hipMemcpyAsync(rb, S.counters, 4, hipMemcpyDeviceToHost, s));
         APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
         APG_HIP_CHECK(ctx, hipGetLastError());
+        const uint32_t next_end = rb[0];
         begin = end;
         end = next_end;
         if (++levels > 4096) return ctx->fail(APG_E_CAPACITY, "binary tree deeper than 4096 levels");
@@ -1247,9 +1249,9 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
         if ((rc = scr.get(ctx, d_m, 1))) return rc;
         APG_HIP_CHECK(ctx, hipMemsetAsync(d_m, 0, 4, s));
         k_scene_slack<<<256, 256, 0, s>>>(S.pbox, n, d_m);
-        uint32_t m = 0;
-        APG_HIP_CHECK(ctx, hipMemcpyAsync(&m, d_m, 4, hipMemcpyDeviceToHost, s));
+        APG_HIP_CHECK(ctx, hipMemcpyAsync(rb, d_m, 4, hipMemcpyDeviceToHost, s));
         APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
+        const uint32_t m = rb[0];
         float mf;
         std::memcpy(&mf, &m, 4);
         if (mf == 0.0f) mf = 1.0f;
@@ -1289,11 +1291,11 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
         exclusive_scan(W.n_inner, width, W.s_inner, wscan_tmp, s);
         exclusive_scan(W.n_tris, width, W.s_tris, wscan_tmp, s);
         k_wide_emit<<<blocks_for(width, 128), 128, 0, s>>>(W, wbegin, wend, wend, tri_cursor);
-        uint32_t tot[2] = { 0, 0 };
-        APG_HIP_CHECK(ctx, hipMemcpyAsync(&tot[0], W.s_inner + width, 4, hipMemcpyDeviceToHost, s));
-        APG_HIP_CHECK(ctx, hipMemcpyAsync(&tot[1], W.s_tris + width, 4, hipMemcpyDeviceToHost, s));
+        APG_HIP_CHECK(ctx, hipMemcpyAsync(rb, W.s_inner + width, 4, hipMemcpyDeviceToHost, s));
+        APG_HIP_CHECK(ctx, hipMemcpyAsync(rb + 1, W.s_tris + width, 4, hipMemcpyDeviceToHost, s));
         APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
         APG_HIP_CHECK(ctx, hipGetLastError());
+        const uint32_t tot[2] = { rb[0], rb[1] };
         tri_cursor += tot[1];
         wbegin = wend;
         wend += tot[0];
