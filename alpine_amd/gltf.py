@@ -15,8 +15,11 @@ What is read, as the reference reads it:
     alpha = (roughnessFactor, roughnessFactor), the base colour factor's rgb, the base-colour
     texture (sRGB decoded to linear) and the normal texture (mapped to [-1, 1]) (:384-460);
   * animations keep the "weights" channels of LINEAR samplers on nodes with meshes (:474-530).
-Images: PNG (8-bit RGB / RGBA / palette) is decoded here; other images (JPEG, 16-bit, grey) are
-reported and left out, as the reference does for images of other channel counts.
+Images: PNG (8-bit RGB / RGBA / palette) is decoded here; baseline and progressive JPEG through
+Pillow (libjpeg) where it is importable (stb_image's own JPEG decoder, which the reference uses,
+may differ from libjpeg by a few code values: its IDCT and chroma upsampling are its own); other
+images (16-bit, grey, JPEG without Pillow) are reported and left out, as the reference does for
+images of other channel counts.
 """
 from __future__ import annotations
 
@@ -163,6 +166,28 @@ def decode_png(data):
             return np.concatenate([rgb, a[idx][..., None]], -1)
         return rgb
     return img
+
+
+def decode_image(data):
+    """(height, width, channels) uint8 of an embedded PNG or JPEG image."""
+    if data[:8] == b"\x89PNG\r\n\x1a\n":
+        return decode_png(data)
+    if data[:3] == b"\xff\xd8\xff":
+        try:
+            import io
+            from PIL import Image
+        except ImportError:
+            raise GltfError("JPEG images need Pillow") from None
+        try:
+            with Image.open(io.BytesIO(data)) as im:
+                im.load()
+                if im.mode not in ("L", "RGB"):
+                    im = im.convert("RGB")   # CMYK / YCCK as stb_image returns them: 3 channels
+                a = np.asarray(im, np.uint8)
+        except (OSError, ValueError, SyntaxError) as e:
+            raise GltfError(f"JPEG not decoded ({e})") from None
+        return a[..., None] if a.ndim == 2 else a
+    raise GltfError("unknown image format")
 
 
 def texture_texels(img, signed, srgb):
@@ -351,7 +376,7 @@ class _Loader:
                 raise GltfError("external image")
             v = self._view(im["bufferView"])
             data = self.bin[v.get("byteOffset", 0): v.get("byteOffset", 0) + v["byteLength"]]
-            img = decode_png(bytes(data))
+            img = decode_image(bytes(data))
             if img.shape[2] in (3, 4):
                 self.out.textures.append(texture_texels(img, signed, srgb))
                 result = len(self.out.textures) - 1

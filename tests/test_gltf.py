@@ -146,3 +146,37 @@ def test_simple_reads_glb_before_touching_the_gpu(tmp_path, capsys):
     assert simple.main(["-i", str(bad), "-o", str(tmp_path / "o.ppm"), "--spp", "1", "--accelerator", "gpu",
                         "--lightSampler", "bvh"]) == 1
     assert "ERROR: failed to load" in capsys.readouterr().out
+
+
+def test_jpeg_textures_through_pillow(tmp_path):
+    """A JPEG base-colour texture is decoded (Pillow / libjpeg) within JPEG's loss of the
+    encoded pixels and converted like a PNG one; a grey JPEG is reported as the reference
+    reports images of other channel counts."""
+    PIL = pytest.importorskip("PIL.Image")
+    import io
+    rng = np.random.default_rng(4)
+    img = np.repeat(np.repeat(rng.integers(0, 256, (4, 6, 3), dtype=np.uint8), 8, 0), 8, 1)
+
+    def jpeg(a, mode):
+        b = io.BytesIO()
+        PIL.fromarray(a, mode).save(b, format="JPEG", quality=98, subsampling=0)
+        return b.getvalue()
+
+    w = GlbWriter()
+    for data, name in ((jpeg(img, "RGB"), "colour"), (jpeg(img[..., 0], "L"), "grey")):
+        w.gl.setdefault("images", []).append({"bufferView": w._view(data), "mimeType": "image/jpeg",
+                                              "name": name})
+        w.gl.setdefault("textures", []).append({"source": len(w.gl["images"]) - 1})
+    m0 = w.material(metallic=0.0, base_tex=0)
+    m1 = w.material(metallic=0.0, base_tex=1)
+    tri = dict(pos=[[0, 0, 0], [1, 0, 0], [0, 1, 0]], idx=np.uint16([0, 1, 2]), material=m0)
+    w.node(mesh=w.mesh([tri, dict(tri, material=m1)]))
+    p = tmp_path / "jpeg.glb"
+    p.write_bytes(w.bytes())
+    sc = gltf.read_glb(str(p))
+    t0 = sc.materials[0][3]
+    assert t0 is not None and sc.materials[1][3] is None
+    assert any("grey" in m and "channel count" in m for m in sc.messages)
+    dec = gltf.decode_image(jpeg(img, "RGB"))
+    assert dec.shape == img.shape and np.abs(dec.astype(int) - img).mean() < 3
+    np.testing.assert_array_equal(sc.textures[t0], gltf.texture_texels(dec, signed=False, srgb=True))
