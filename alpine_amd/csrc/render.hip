@@ -1359,12 +1359,15 @@ static int ensure_guide_buffers(apg_context* ctx, RenderState* R, uint32_t depth
         if (rc == APG_OK) R->guide_allocs.push_back(ptr);
         return rc;
     };
+    // Minibatch order places S minibatches of B = ceil(n / S) slots each: S * B <= n + S - 1,
+    // so the minibatch-order arrays hold tcap + TRAIN_MAX_STEPS entries.
+    const size_t tpad = size_t(tcap) + TRAIN_MAX_STEPS;
     int rc;
     if ((rc = get(R->gfeat, size_t(P) * 9)) || (rc = get(R->gout, size_t(P) * GUIDE_OUT)) ||
         (rc = get(R->vrec, size_t(P) * depth * VREC)) || (rc = get(R->tin, size_t(tcap) * 9)) ||
-        (rc = get(R->tfeat, size_t(tcap) * 9)) || (rc = get(R->tdir, size_t(tcap) * 3)) ||
-        (rc = get(R->tw, tcap)) || (rc = get(R->tdir_p, size_t(tcap) * 3)) || (rc = get(R->tw_p, tcap)) || (rc = get(R->twsum, TRAIN_MAX_STEPS)) ||
-        (rc = get(R->tcount, 4)))
+        (rc = get(R->tfeat, tpad * 9)) || (rc = get(R->tdir, size_t(tcap) * 3)) ||
+        (rc = get(R->tw, tcap)) || (rc = get(R->tdir_p, tpad * 3)) || (rc = get(R->tw_p, tpad)) ||
+        (rc = get(R->twsum, TRAIN_MAX_STEPS)) || (rc = get(R->tcount, 4)))
         return rc;
     R->vrec_depth = depth;
     R->tcap = tcap;
@@ -1588,6 +1591,8 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
                 // one epoch over the pass's samples in train_steps minibatches
                 const uint32_t S = std::min(std::min(train_steps, tn), TRAIN_MAX_STEPS);
                 const uint32_t B = (tn + S - 1) / S;
+                if (size_t(S) * B > size_t(R->tcap) + TRAIN_MAX_STEPS)
+                    return ctx->fail(APG_E_CAPACITY, "training minibatches exceed their buffers");
                 k_zero_f32<<<nblocks(S, 256), 256, 0, stream>>>(R->twsum, S);
                 k_train_encode<<<nblocks(tn, 256), 256, 0, stream>>>(R->tin, R->tdir, R->tw, tn, S, B, R->tfeat,
                                                                      R->tdir_p, R->tw_p, R->twsum);

@@ -66,3 +66,20 @@ def test_trained_guide_lowers_error_on_doorway():
     print("relMSE plain", e_plain, "guided", e_guided, "ratio", e_guided / e_plain)
     assert np.isfinite(guided).all()
     assert e_guided < e_plain
+
+
+def test_training_minibatches_that_do_not_divide_the_samples():
+    """train_steps that does not divide the kept sample count (48 minibatches over a full
+    2 x P-sample capacity, P = 64 x 64 x 4 paths): the last minibatches reach past the
+    capacity in minibatch order, which the buffers are sized for; training runs and the
+    image stays finite."""
+    print("[gpu] guided training with 48 minibatches", flush=True)
+    sc = scenes.cornell_scene(64, 64)
+    acc = _setup(sc)
+    acc.render_reset(64, 64, 3, slots=4)
+    P = 64 * 64 * 4
+    acc.render(16, sc.max_depth, guide=1, guide_warmup=1, train_steps=48, train_samples=2 * P)
+    img, _, spp = acc.render_read()
+    n, loss = acc.guide_stats()
+    print("train n", n, "loss", loss)
+    assert spp == 16 and np.isfinite(img).all() and n > 0 and np.isfinite(loss)
