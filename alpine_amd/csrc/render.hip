@@ -107,6 +107,7 @@ struct RenderState
     float* gout = nullptr;    // P x 48 proposal parameters
     float* vrec = nullptr;    // P x vrec_depth x VREC vertex records
     uint32_t vrec_depth = 0;
+    uint32_t guide_P = 0;     // paths per pass the guide's per-path buffers are sized for
     float* tin = nullptr;     // tcap x 9 training inputs (p01, n, wo)
     float* tfeat = nullptr;   // tcap x 9, minibatch order
     float* tdir = nullptr;    // tcap x 3
@@ -1350,7 +1351,7 @@ static int ensure_guide_buffers(apg_context* ctx, RenderState* R, uint32_t depth
 {
     const uint32_t P = R->P;
     const uint32_t tcap = std::min<uint32_t>(P * 2, want ? want : (1u << 18));
-    if (R->gfeat && R->vrec_depth >= depth && R->tcap == tcap) return APG_OK;
+    if (R->gfeat && R->guide_P == P && R->vrec_depth >= depth && R->tcap == tcap) return APG_OK;
     if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
     for (void* p : R->guide_allocs) (void)hipFree(p);
     R->guide_allocs.clear();
@@ -1371,6 +1372,7 @@ static int ensure_guide_buffers(apg_context* ctx, RenderState* R, uint32_t depth
         return rc;
     R->vrec_depth = depth;
     R->tcap = tcap;
+    R->guide_P = P;
     return guide_ensure(ctx, tcap);
 }
 

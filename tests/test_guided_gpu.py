@@ -83,3 +83,18 @@ def test_training_minibatches_that_do_not_divide_the_samples():
     n, loss = acc.guide_stats()
     print("train n", n, "loss", loss)
     assert spp == 16 and np.isfinite(img).all() and n > 0 and np.isfinite(loss)
+
+
+def test_guide_buffers_follow_the_frame_size():
+    """Guided passes on a 64 x 64 frame, then on a 128 x 128 frame with the same training
+    capacity: the per-path guide buffers are sized for the new frame, not reused."""
+    print("[gpu] guided render after a frame resize", flush=True)
+    sc = scenes.cornell_scene(64, 64)
+    acc = _setup(sc)
+    for w in (64, 128):
+        acc.render_reset(w, w, 5)
+        acc.render(4, sc.max_depth, guide=1, guide_warmup=1, train_steps=4, train_samples=4096)
+        acc.render(4, sc.max_depth, guide=2)
+        img, _, spp = acc.render_read()
+        n, loss = acc.guide_stats()
+        assert img.shape == (w, w, 3) and spp == 8 and np.isfinite(img).all() and n > 0
