@@ -802,8 +802,9 @@ __global__ void k_child_first(BuildState S, uint32_t begin, uint32_t end)
 // ---------------------------------------------------------------------------
 // exclusive scan (3 passes, recursive on the block sums)
 // ---------------------------------------------------------------------------
+// A single-workgroup scan (total != nullptr) also writes the total.
 __global__ __launch_bounds__(BLOCK) void k_scan_blocks(const uint32_t* in, uint32_t n, uint32_t* out,
-                                                       uint32_t* sums)
+                                                       uint32_t* sums, uint32_t* total)
 {
     __shared__ uint32_t sh[BLOCK];
     const uint32_t base = blockIdx.x * CHUNK + threadIdx.x * ITEMS;
@@ -828,13 +829,19 @@ __global__ __launch_bounds__(BLOCK) void k_scan_blocks(const uint32_t* in, uint3
         if (base + k < n) out[base + k] = run;
         run += v[k];
     }
-    if (threadIdx.x == BLOCK - 1) sums[blockIdx.x] = sh[threadIdx.x];
+    if (threadIdx.x == BLOCK - 1)
+    {
+        sums[blockIdx.x] = sh[threadIdx.x];
+        if (total) *total = sh[threadIdx.x];
+    }
 }
 
+// Adds each workgroup's offset; offs[gridDim.x] (the scan's total) goes to out[n].
 __global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* offs)
 {
     const uint32_t i = blockIdx.x * CHUNK + threadIdx.x;
     const uint32_t add = offs[blockIdx.x];
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = offs[gridDim.x];
     for (uint32_t k = 0; k < ITEMS; ++k)
     {
         const uint32_t j = i + k * BLOCK;
@@ -858,13 +865,12 @@ void exclusive_scan(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* tmp
     }
     uint32_t* sums = tmp;
     uint32_t* sums_scan = tmp + nb + 1;
-    k_scan_blocks<<<nb, BLOCK, 0, s>>>(in, n, out, sums);
+    k_scan_blocks<<<nb, BLOCK, 0, s>>>(in, n, out, sums, nb == 1 ? out + n : nullptr);
     if (nb > 1)
     {
-        exclusive_scan(sums, nb, sums_scan, sums_scan + nb + 1, s);
+        exclusive_scan(sums, nb, sums_scan, sums_scan + nb + 1, s);   // sums_scan[nb] = total
         k_scan_add<<<nb, BLOCK, 0, s>>>(out, n, sums_scan);
     }
-    k_set_total<<<1, 1, 0, s>>>(in, out, n, out + n);
 }
 
 size_t scan_tmp_words(uint32_t n)
