@@ -40,6 +40,8 @@ def build(ref: bool | None = None) -> None:
         # the drop-in adapter beside the reference's Bvh / samplers (needs alpine_amd/libapg.so)
         if os.path.exists(os.path.join(HERE, "..", "alpine_amd", "libapg.so")):
             subprocess.run(["make", "-s", "-C", HERE, "adapter"], check=True)
+        # and its dummy form (no GPU library), the build without ALPINE_ENABLE_HIP
+        subprocess.run(["make", "-s", "-C", HERE, "dummy"], check=True)
 
 
 class _Stats(C.Structure):

@@ -29,6 +29,17 @@ def test_adapter_without_gpu_behaves_like_the_dummy_accelerators():
     assert oracle.adapter_lib().adapter_check_no_device() == 31
 
 
+def test_dummy_adapter_returns_the_interface_sentinels():
+    """gpu_dummy.cpp (the build without ALPINE_ENABLE_HIP) compiled against alpine's headers:
+    every call returns the sentinel alpine's own dummy accelerators return, GPU or not."""
+    import ctypes
+    import os
+    path = os.path.join(os.path.dirname(oracle.__file__), "_ref", "libdummy_check.so")
+    if not os.path.exists(path):
+        pytest.skip("dummy adapter not built (the reference tree is needed to compile it)")
+    assert ctypes.CDLL(path).adapter_check_no_device() == 31
+
+
 @needs_lib
 @pytest.mark.gpu
 def test_adapter_per_ray_queries_match_reference_bvh():
