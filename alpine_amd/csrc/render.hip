@@ -376,8 +376,10 @@ __global__ void k_generate(DevCamera cam, Tiles tl, uint32_t W, uint32_t H, uint
     if (p == 0) atomicAdd(&stats[0], (unsigned long long)P);
 }
 
+// Plain shading is held to 128 VGPRs (4 waves per SIMD, a few spills) to hide the light-BVH
+// descent's load latency; the guided form keeps its registers (vMF mixture evaluation).
 template <bool GUIDED>
-__global__ __launch_bounds__(256) void k_shade(ShadeArgs A)
+__global__ __launch_bounds__(256, GUIDED ? 1 : 4) void k_shade(ShadeArgs A)
 {
     const uint32_t count = A.counts[0];
     const uint32_t stride = gridDim.x * blockDim.x;
