@@ -59,6 +59,7 @@ class _SceneStats(C.Structure):
         ("binary_leaves", C.c_uint32), ("wide_nodes", C.c_uint32),
         ("max_depth_binary", C.c_uint32), ("sah_cost", C.c_double),
         ("sah_cost_leaf", C.c_double), ("build_ms", C.c_double), ("device_bytes", C.c_uint64),
+        ("wide_depth", C.c_uint32), ("reserved", C.c_uint32),
     ]
 
 
@@ -121,7 +122,8 @@ def camera(eye, target, up, fovy, aspect):
 class TraceConfig(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("fetch_threshold", C.c_uint32), ("stack_lds", C.c_uint32),
-                ("tri_batch", C.c_uint32), ("triangle_test", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+                ("tri_batch", C.c_uint32), ("triangle_test", C.c_uint32), ("stack_limit", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 TRI_REFERENCE, TRI_WATERTIGHT = 0, 1
@@ -145,12 +147,13 @@ ABI_SYMBOLS = [
     "apg_render_guide_stats", "apg_mesh_set_shading", "apg_mesh_set_targets", "apg_mesh_morph",
     "apg_textures_set", "apg_render_read_aovs", "apg_render_set_environment",
     "apg_intersect_host", "apg_occluded_host", "apg_light_sampler_set",
-    "apg_light_sampler_sample_host",
+    "apg_light_sampler_sample_host", "apg_trace_status",
 ]
 DEBUG_SYMBOLS = ["apg_debug_host_build", "apg_debug_scene_info", "apg_debug_scene_copy",
                  "apg_debug_host_trace", "apg_debug_scene_free", "apg_debug_material",
                  "apg_debug_light_sampler", "apg_debug_light", "apg_debug_shading",
-                 "apg_debug_environment", "apg_debug_triangles"]
+                 "apg_debug_environment", "apg_debug_triangles", "apg_debug_host_trace_limited",
+                 "apg_debug_scene_depth", "apg_debug_fail_allocations"]
 
 
 def lib():
@@ -187,6 +190,7 @@ def lib():
         "apg_occluded": (i32, [vp, C.POINTER(_Rays), u32, vp, vp]),
         "apg_intersect_roots": (i32, [vp, C.POINTER(_Rays), vp, u32, C.POINTER(_Hits), vp]),
         "apg_occluded_roots": (i32, [vp, C.POINTER(_Rays), vp, u32, vp, vp]),
+        "apg_trace_status": (i32, [vp]),
         "apg_intersect_counted": (i32, [vp, C.POINTER(_Rays), u32, C.POINTER(_Hits), i32, vp,
                                         C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), vp]),
         "apg_scene_triangles": (i32, [vp, vp, vp, u32]),
@@ -227,6 +231,10 @@ def lib():
         "apg_debug_host_trace": (i32, [vp, vp, vp, vp, u32, i32, vp, vp, vp, vp, vp,
                                        C.POINTER(u32)]),
         "apg_debug_scene_free": (None, [vp]),
+        "apg_debug_host_trace_limited": (i32, [vp, vp, vp, vp, u32, i32, u32, vp, vp, vp, vp, vp,
+                                               C.POINTER(u32)]),
+        "apg_debug_scene_depth": (i32, [vp, C.POINTER(u32)]),
+        "apg_debug_fail_allocations": (i32, [vp, i32]),
         "apg_debug_material": (i32, [C.POINTER(Material)] + [vp] * 3 + [u32] + [vp] * 6),
         "apg_debug_light_sampler": (i32, [vp, u32, u32, vp, vp, vp, u32, vp, vp, vp]),
         "apg_debug_light": (i32, [C.POINTER(Light), vp, vp, vp, u32] + [vp] * 7),
@@ -468,6 +476,10 @@ class GpuAccelerator:
                                         self._stream(stream)))
         return out
 
+    def trace_status(self):
+        """Wait for the traversals issued so far; raise ApgError(-6) if one overflowed its stack."""
+        self._rc(self._lib.apg_trace_status(self._h))
+
     def intersect_counted(self, org, dir, tmax=None, any_hit=False, stream=None):
         """Instrumented run: results plus total node and triangle fetches."""
         import torch
@@ -700,6 +712,32 @@ class DebugScene:
                                    C.byref(sahl), C.byref(bn), C.byref(md))
         return {"wide_nodes": nn.value, "triangles": nt.value, "sah_cost": sah.value,
                 "sah_cost_leaf": sahl.value, "binary_nodes": bn.value, "max_depth": md.value}
+
+    def wide_depth(self):
+        d = C.c_uint32()
+        lib().apg_debug_scene_depth(self._h, C.byref(d))
+        return d.value
+
+    def trace_limited(self, org, dir, stack_limit, any_hit=False):
+        """Host trace with the kernel's bounded stack: (results, rays that overflowed)."""
+        org = np.ascontiguousarray(org, np.float32)
+        dir = np.ascontiguousarray(dir, np.float32)
+        n = len(org)
+        over = C.c_uint32()
+        if any_hit:
+            occ = np.empty(n, np.uint8)
+            rc = lib().apg_debug_host_trace_limited(self._h, _ptr(org), _ptr(dir), None, n, 1, stack_limit,
+                                                    None, None, None, None, _ptr(occ), C.byref(over))
+            if rc != APG_OK:
+                raise ApgError(rc, "apg_debug_host_trace_limited failed")
+            return occ, over.value
+        prim = np.empty(n, np.uint32)
+        t, b0, b1 = (np.empty(n, np.float32) for _ in range(3))
+        rc = lib().apg_debug_host_trace_limited(self._h, _ptr(org), _ptr(dir), None, n, 0, stack_limit, _ptr(prim),
+                                                _ptr(t), _ptr(b0), _ptr(b1), None, C.byref(over))
+        if rc != APG_OK:
+            raise ApgError(rc, "apg_debug_host_trace_limited failed")
+        return (prim, t, b0, b1), over.value
 
     def arrays(self):
         i = self.info()

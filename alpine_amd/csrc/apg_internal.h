@@ -49,6 +49,7 @@ struct DeviceScene
     uint32_t node_count = 0;
     uint32_t tri_count = 0;
     uint32_t roots = 1;          // wide nodes 0 .. roots-1 are tree roots
+    uint32_t wide_depth = 0;     // wide-node levels of the deepest root-to-leaf path
     float* tris_v = nullptr;     // 9 floats per triangle, tree order: the input vertices v0, v1, v2
                                  // (watertight test; gathered on first use after a build)
 };
@@ -74,6 +75,8 @@ struct Context
 {
     int device = 0;
     int cu_count = 0;
+    // the most recent failure; per-ray entry points fail from many threads at once
+    mutable std::mutex error_mutex;
     std::string last_error;
     std::vector<Shape> shapes;
     uint32_t total_tris = 0;
@@ -100,16 +103,22 @@ struct Context
     apg_trace_config cfg{};
     std::mutex launch_mutex;
     hipEvent_t last_launch = nullptr;
-    uint32_t* counters = nullptr;     // [0] ray cursor, [1] overflow flag, [2..5] fetch counts
+    uint32_t* counters = nullptr;     // [0] ray cursor, [1] overflow flag, [2] waves done
     uint64_t* spill = nullptr;        // per persistent thread stack spill
     size_t spill_threads = 0;
     uint32_t* readback = nullptr;     // pinned host words the GPU builder reads level sizes into
+    uint32_t* overflow_host = nullptr; // pinned host word the overflow flag is read back into
+    // After the context's traversals so far have finished (the caller waited for them):
+    // APG_E_OVERFLOW if one of them overflowed its stack since the last check, and the
+    // flag is cleared; else APG_OK.
+    int take_overflow();
 
     // shading data and the path tracer (render.hip)
     RenderState* render = nullptr;
     GuideNet* guide = nullptr;
     uint64_t attr_version = 0;   // bumped when a shape's shading attributes or targets change
     bool weights_dirty = false;  // morph weights changed since the render pools were filled
+    int fail_allocs_after = -1;  // test hook: render-state allocations fail after this many more
 
     // host-memory entry points (host_calls.cpp)
     std::mutex staging_mutex;
@@ -121,6 +130,7 @@ struct Context
     int rebuild_shading();  // per-triangle normals after a scene build (render.hip)
     int fail(int code, const std::string& msg)
     {
+        std::lock_guard<std::mutex> lock(error_mutex);
         last_error = msg;
         return code;
     }
@@ -156,7 +166,8 @@ void build_binary_host(const std::vector<TriPrep>& tris, uint32_t leaf_threshold
 
 // Collapse a binary tree into 64 B wide nodes (reference collapse policy,
 // wide_bvh.cpp:200-243) with octant-ordered slots and quantised boxes.
-void collapse_to_wide(const BinaryTree& bt, const std::vector<TriPrep>& tris,
+// Returns the number of wide-node levels (0 for an empty tree).
+uint32_t collapse_to_wide(const BinaryTree& bt, const std::vector<TriPrep>& tris,
                       std::vector<ApgWideNode>& nodes, std::vector<float>& tri9,
                       std::vector<uint32_t>& tri_ids);
 

@@ -11,6 +11,7 @@
 #include "apg_internal.h"
 
 #include <chrono>
+#include <cmath>
 #include <cstring>
 
 namespace apg {
@@ -30,6 +31,7 @@ Context::~Context()
     if (d_tri) (void)hipFree(d_tri);
     if (spill) (void)hipFree(spill);
     if (readback) (void)hipHostFree(readback);
+    if (overflow_host) (void)hipHostFree(overflow_host);
     for (Shape& sh : shapes)
     {
         if (sh.d_base) (void)hipFree(sh.d_base);
@@ -130,6 +132,16 @@ int Context::sync_geometry_to_host()
 
 namespace {
 thread_local std::string g_create_error;
+thread_local std::string g_error_copy;  // what apg_last_error hands out on this thread
+
+// Accelerator::appendMesh has no check (bvh.cpp:89,101 assert capacity only): a NaN or
+// infinite coordinate would give the builder NaN boxes and bins.
+bool all_finite(const float* v, size_t n)
+{
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
 }
 
 extern "C" {
@@ -179,7 +191,10 @@ int apg_destroy(apg_context* ctx)
 
 const char* apg_last_error(const apg_context* ctx)
 {
-    return ctx ? ctx->last_error.c_str() : g_create_error.c_str();
+    if (!ctx) return g_create_error.c_str();
+    std::lock_guard<std::mutex> lock(ctx->error_mutex);
+    g_error_copy = ctx->last_error;
+    return g_error_copy.c_str();
 }
 
 int apg_mesh_append(apg_context* ctx, const float* xyz, uint32_t nv, const uint32_t* idx,
@@ -193,6 +208,7 @@ int apg_mesh_append(apg_context* ctx, const float* xyz, uint32_t nv, const uint3
     if (int rc = ctx->sync_geometry_to_host()) return rc;
     for (uint64_t i = 0; i < uint64_t(nt) * 3; ++i)
         if (idx[i] >= nv) return ctx->fail(APG_E_INVALID, "triangle index out of range");
+    if (!all_finite(xyz, size_t(nv) * 3)) return ctx->fail(APG_E_INVALID, "vertex coordinate is NaN or infinite");
     apg::Shape sh;
     sh.xyz.assign(xyz, xyz + size_t(nv) * 3);
     sh.idx.assign(idx, idx + size_t(nt) * 3);
@@ -218,6 +234,9 @@ int apg_mesh_update(apg_context* ctx, uint32_t shape_id)
 {
     if (!ctx) return APG_E_INVALID;
     if (shape_id >= ctx->shapes.size()) return ctx->fail(APG_E_INVALID, "unknown shape id");
+    if (int rc = ctx->sync_geometry_to_host()) return rc;
+    const std::vector<float>& v = ctx->shapes[shape_id].xyz;
+    if (!all_finite(v.data(), v.size())) return ctx->fail(APG_E_INVALID, "vertex coordinate is NaN or infinite");
     // Triangle data is re-derived from the vertex copy at the next build.
     ctx->geom_dirty = true;
     return APG_OK;
@@ -259,6 +278,7 @@ int apg_scene_build(apg_context* ctx, const apg_build_params* params)
     // Wait for kernels still reading the old scene.
     if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
     int rc = APG_OK;
+    ctx->built = false;  // until this build has succeeded
 
     std::vector<apg::TriPrep> tris;
     if (builder == APG_BUILDER_HOST)
@@ -273,8 +293,9 @@ int apg_scene_build(apg_context* ctx, const apg_build_params* params)
     if (builder == APG_BUILDER_HOST)
     {
         apg::build_binary_host(tris, leaf_threshold, bt);
-        apg::collapse_to_wide(bt, tris, nodes, tri9, ids);
+        const uint32_t depth = apg::collapse_to_wide(bt, tris, nodes, tri9, ids);
         rc = apg::upload_scene(ctx, nodes, tri9, ids);
+        ctx->scene.wide_depth = depth;
     }
     else if (builder == APG_BUILDER_GPU)
     {
@@ -283,6 +304,15 @@ int apg_scene_build(apg_context* ctx, const apg_build_params* params)
     else
         return ctx->fail(APG_E_INVALID, "unknown builder");
     if (rc) return rc;
+    // One stack entry per wide level at most (a lane pushes the rest of a node's hit children
+    // once per descent), so a tree no deeper than the stack can never overflow it.
+    if (const uint32_t depth = ctx->scene.wide_depth; depth > APG_TRACE_STACK_MAX)
+    {
+        apg::free_scene(ctx);
+        return ctx->fail(APG_E_CAPACITY, "the 8-wide tree has " + std::to_string(depth) +
+                                             " levels, more than the traversal stack's " +
+                                             std::to_string(APG_TRACE_STACK_MAX) + " entries");
+    }
 
     ctx->leaf_keys = std::move(bt.leaf_keys);
     for (int a = 0; a < 3; ++a)
@@ -298,6 +328,7 @@ int apg_scene_build(apg_context* ctx, const apg_build_params* params)
     st.binary_leaves = bt.sah_leaves;
     st.wide_nodes = ctx->scene.node_count;
     st.max_depth_binary = bt.max_depth;
+    st.wide_depth = ctx->scene.wide_depth;
     st.sah_cost = bt.sah_cost;
     st.sah_cost_leaf = bt.sah_cost_leaf;
     // nodes; reference-form triangles and ids (40 B); vertex-form triangles (36 B)
@@ -346,6 +377,8 @@ int apg_set_trace_config(apg_context* ctx, const apg_trace_config* cfg)
         return ctx->fail(APG_E_INVALID, "tri_batch must be in 1..64");
     if (cfg->triangle_test > APG_TRI_WATERTIGHT)
         return ctx->fail(APG_E_INVALID, "triangle_test must be APG_TRI_REFERENCE or APG_TRI_WATERTIGHT");
+    if (cfg->stack_limit > APG_TRACE_STACK_MAX)
+        return ctx->fail(APG_E_INVALID, "stack_limit must be at most APG_TRACE_STACK_MAX");
     ctx->cfg = *cfg;
     return APG_OK;
 }

@@ -591,6 +591,7 @@ __device__ void sweep_tiny(BuildState& S, uint32_t k, uint32_t fl, uint32_t cnt)
     float min_cost = FLT_MAX;
     uint32_t best_dim = 0, best_idx = 0, best_below = 0;
     float best_mn = 0.0f, best_mx = 0.0f;
+    bool found = false;
     for (uint32_t d = 0; d < 3; ++d)
     {
         const float mn = float_of(S.ncbox[6 * size_t(k) + d]);
@@ -642,11 +643,14 @@ __device__ void sweep_tiny(BuildState& S, uint32_t k, uint32_t fl, uint32_t cnt)
                 best_below = bn[s];
                 best_mn = mn;
                 best_mx = mx;
+                found = true;
             }
         }
     }
     const float leaf_cost = float(cnt) * box_area(nlo, nhi);
-    if (min_cost >= leaf_cost)
+    // no plane below FLT_MAX (surface areas overflowed): a leaf, where the reference's
+    // default split would send every primitive to one side and never terminate
+    if (!found || min_cost >= leaf_cost)
     {
         sah_leaf(S, k, fl, cnt);
         return;
@@ -694,6 +698,7 @@ __global__ void k_sweep(BuildState S, uint32_t begin, uint32_t end)
     float min_cost = FLT_MAX;
     uint32_t best_dim = 0, best_idx = 0;
     float best_mn = 0.0f, best_mx = 0.0f;
+    bool found = false;
     for (uint32_t d = 0; d < 3; ++d)
     {
         const float mn = float_of(S.ncbox[6 * size_t(k) + d]);
@@ -728,11 +733,12 @@ __global__ void k_sweep(BuildState S, uint32_t begin, uint32_t end)
                 best_idx = s;
                 best_mn = mn;
                 best_mx = mx;
+                found = true;
             }
         }
     }
     const float leaf_cost = float(cnt) * box_area(nlo, nhi);
-    if (min_cost >= leaf_cost)
+    if (!found || min_cost >= leaf_cost)  // as in sweep_tiny
     {
         sah_leaf(S, k, fl, cnt);
         return;
@@ -1135,6 +1141,12 @@ struct Scratch
         if (rc == APG_OK) ptrs.push_back(p);
         return rc;
     }
+    // p now belongs to the caller: it is not freed with the scratch
+    void release(const void* p)
+    {
+        for (void*& q : ptrs)
+            if (q == p) q = nullptr;
+    }
 };
 
 inline uint32_t blocks_for(uint32_t n, uint32_t b) { return n ? (n + b - 1) / b : 1; }
@@ -1275,12 +1287,14 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
     if ((rc = scr.get(ctx, W.s_tris, wcap + 1))) return rc;
     uint32_t* wscan_tmp;
     if ((rc = scr.get(ctx, wscan_tmp, scan_tmp_words(uint32_t(wcap) + 1)))) return rc;
+    // The scene's buffers are scratch (freed on every error return) until they are handed
+    // to the context below.
     ApgWideNode* d_nodes;
     float* d_tri9;
     uint32_t* d_ids;
-    if ((rc = dalloc(ctx, d_nodes, wcap))) return rc;
-    if ((rc = dalloc(ctx, d_tri9, 9 * size_t(n)))) { (void)hipFree(d_nodes); return rc; }
-    if ((rc = dalloc(ctx, d_ids, n))) { (void)hipFree(d_nodes); (void)hipFree(d_tri9); return rc; }
+    if ((rc = scr.get(ctx, d_nodes, wcap))) return rc;
+    if ((rc = scr.get(ctx, d_tri9, 9 * size_t(n)))) return rc;
+    if ((rc = scr.get(ctx, d_ids, n))) return rc;
     W.nodes = d_nodes; W.tri9 = d_tri9; W.tri_ids = d_ids;
     {
         // wide nodes 0 .. R-1 are the roots' images
@@ -1289,7 +1303,7 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
         APG_HIP_CHECK(ctx, hipMemcpyAsync(W.wbin, wroots.data(), 4 * R, hipMemcpyHostToDevice, s));
         APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
     }
-    uint32_t wbegin = 0, wend = R, tri_cursor = 0;
+    uint32_t wbegin = 0, wend = R, tri_cursor = 0, wlevels = 0;
     while (wbegin < wend)
     {
         const uint32_t width = wend - wbegin;
@@ -1305,15 +1319,20 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
         tri_cursor += tot[1];
         wbegin = wend;
         wend += tot[0];
+        wlevels++;
         if (wend > wcap) return ctx->fail(APG_E_CAPACITY, "wide node capacity exceeded");
     }
+    if (tri_cursor != n) return ctx->fail(APG_E_STATE, "GPU build lost triangles");
+    scr.release(d_nodes);
+    scr.release(d_tri9);
+    scr.release(d_ids);
     ctx->scene.nodes = d_nodes;
     ctx->scene.tris = d_tri9;
     ctx->scene.tri_ids = d_ids;
     ctx->scene.node_count = wend;
     ctx->scene.tri_count = tri_cursor;
     ctx->scene.roots = R;
-    if (tri_cursor != n) return ctx->fail(APG_E_STATE, "GPU build lost triangles");
+    ctx->scene.wide_depth = wlevels;
     bt.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
 
     // ---- scene bounds (union of the roots) ----

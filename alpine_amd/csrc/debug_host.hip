@@ -6,6 +6,7 @@
 #include "traverse_core.h"
 #include "render_internal.h"
 
+#include <cmath>
 #include <cstring>
 
 struct apg_debug_scene
@@ -14,6 +15,7 @@ struct apg_debug_scene
     std::vector<float> tri9;
     std::vector<uint32_t> ids;
     apg::BinaryTree bt;
+    uint32_t wide_depth = 0;
 };
 
 extern "C" {
@@ -24,6 +26,8 @@ int apg_debug_host_build(const float* xyz, uint32_t nv, const uint32_t* idx, uin
     if (!out || (nv && !xyz) || (nt && !idx)) return APG_E_INVALID;
     for (uint64_t i = 0; i < uint64_t(nt) * 3; ++i)
         if (idx[i] >= nv) return APG_E_INVALID;
+    for (uint64_t i = 0; i < uint64_t(nv) * 3; ++i)
+        if (!std::isfinite(xyz[i])) return APG_E_INVALID;  // as apg_mesh_append
     apg::Context tmp;
     apg::Shape sh;
     sh.xyz.assign(xyz, xyz + size_t(nv) * 3);
@@ -34,7 +38,7 @@ int apg_debug_host_build(const float* xyz, uint32_t nv, const uint32_t* idx, uin
     apg::prepare_triangles(tmp, tris);
     auto* s = new apg_debug_scene;
     apg::build_binary_host(tris, leaf_threshold ? leaf_threshold : 4u, s->bt);
-    apg::collapse_to_wide(s->bt, tris, s->nodes, s->tri9, s->ids);
+    s->wide_depth = apg::collapse_to_wide(s->bt, tris, s->nodes, s->tri9, s->ids);
     *out = s;
     return APG_OK;
 }
@@ -65,14 +69,27 @@ int apg_debug_scene_copy(const apg_debug_scene* s, void* nodes64, float* tri9, u
     return APG_OK;
 }
 
-int apg_debug_host_trace(const apg_debug_scene* s, const float* org3, const float* dir3,
-                         const float* tmaxp, uint32_t n, int any, uint32_t* prim, float* tout,
-                         float* b0out, float* b1out, uint8_t* occ, uint32_t* max_stack)
+int apg_debug_scene_depth(const apg_debug_scene* s, uint32_t* wide_depth)
+{
+    if (!s || !wide_depth) return APG_E_INVALID;
+    *wide_depth = s->wide_depth;
+    return APG_OK;
+}
+
+} // extern "C"
+
+namespace {
+
+// The per-lane loop of trace_kernel.  stack_limit = 0: unbounded; otherwise a push onto a
+// full stack is dropped as the kernel drops it, and the ray is counted in *overflowed.
+int host_trace(const apg_debug_scene* s, const float* org3, const float* dir3, const float* tmaxp, uint32_t n,
+               int any, uint32_t stack_limit, uint32_t* prim, float* tout, float* b0out, float* b1out, uint8_t* occ,
+               uint32_t* max_stack, uint32_t* overflowed)
 {
     using namespace apg;
     if (!s || !org3 || !dir3) return APG_E_INVALID;
     if (any ? !occ : (!prim || !tout || !b0out || !b1out)) return APG_E_INVALID;
-    uint32_t deepest = 0;
+    uint32_t deepest = 0, dropped_rays = 0;
     std::vector<uint64_t> stack;
     for (uint32_t i = 0; i < n; ++i)
     {
@@ -83,7 +100,7 @@ int apg_debug_host_trace(const apg_debug_scene* s, const float* org3, const floa
         uint32_t g_base = 0, g_bits = s->nodes.empty() ? 0u : root_group_bits(rb.oct);
         uint32_t t_base = 0, t_mask = 0, t_cmask = 0, best = 0xFFFFFFFFu;
         float hb0 = 0, hb1 = 0;
-        bool found = false;
+        bool found = false, dropped = false;
         stack.clear();
         for (;;)
         {
@@ -91,6 +108,11 @@ int apg_debug_host_trace(const apg_debug_scene* s, const float* org3, const floa
             {
                 uint32_t hits = 0;
                 uint32_t node = group_next_node(g_base, g_bits, rb.oct, hits);
+                if (hits && stack_limit && stack.size() >= stack_limit)
+                {
+                    dropped = true;
+                    hits = 0;
+                }
                 if (hits) stack.push_back((uint64_t((hits << 8) | (g_bits & 0xffu)) << 32) | g_base);
                 if (stack.size() > deepest) deepest = uint32_t(stack.size());
                 uint32_t w[16];
@@ -136,12 +158,42 @@ int apg_debug_host_trace(const apg_debug_scene* s, const float* org3, const floa
             b0out[i] = hit ? hb0 : 0.0f;
             b1out[i] = hit ? hb1 : 0.0f;
         }
+        dropped_rays += dropped ? 1u : 0u;
     }
     if (max_stack) *max_stack = deepest;
+    if (overflowed) *overflowed = dropped_rays;
     return APG_OK;
 }
 
+} // namespace
+
+extern "C" {
+
+int apg_debug_host_trace(const apg_debug_scene* s, const float* org3, const float* dir3,
+                         const float* tmaxp, uint32_t n, int any, uint32_t* prim, float* tout,
+                         float* b0out, float* b1out, uint8_t* occ, uint32_t* max_stack)
+{
+    return host_trace(s, org3, dir3, tmaxp, n, any, 0u, prim, tout, b0out, b1out, occ, max_stack, nullptr);
+}
+
+int apg_debug_host_trace_limited(const apg_debug_scene* s, const float* org3, const float* dir3,
+                                 const float* tmaxp, uint32_t n, int any, uint32_t stack_limit,
+                                 uint32_t* prim, float* tout, float* b0out, float* b1out, uint8_t* occ,
+                                 uint32_t* overflowed)
+{
+    if (stack_limit == 0 || stack_limit > APG_TRACE_STACK_MAX) return APG_E_INVALID;
+    return host_trace(s, org3, dir3, tmaxp, n, any, stack_limit, prim, tout, b0out, b1out, occ, nullptr,
+                      overflowed);
+}
+
 void apg_debug_scene_free(apg_debug_scene* s) { delete s; }
+
+int apg_debug_fail_allocations(apg_context* ctx, int after)
+{
+    if (!ctx) return APG_E_INVALID;
+    ctx->fail_allocs_after = after < 0 ? -1 : after;
+    return APG_OK;
+}
 
 int apg_debug_material(const apg_material* mat, const float* wo, const float* u,
                        const float* wi_eval, uint32_t n, uint8_t* ok, float* est, float* wi,

@@ -57,6 +57,7 @@ struct GuideNet
     float* grad = nullptr;
     float* part = nullptr;     // WG_MAX x GUIDE_PARAMS partial gradients
     float* scratch = nullptr;  // [0] sum of weights, [1] loss sum
+    bool ready = false;        // every buffer above is allocated and initialised
     uint32_t step = 0;
     float lr = 1e-3f;
     // training buffers (capacity in samples)
@@ -781,22 +782,43 @@ GuideNet* guide_of(Context* ctx)
 int guide_ensure(Context* ctx, uint32_t cap)
 {
     GuideNet* g = guide_of(ctx);
-    if (!g->w32)
+    if (!g->ready)
     {
-        APG_HIP_CHECK(ctx, hipMalloc(&g->w32, GUIDE_PARAMS * 4));
-        APG_HIP_CHECK(ctx, hipMalloc(&g->w16, WSZ * 2));
-        APG_HIP_CHECK(ctx, hipMalloc(&g->m, GUIDE_PARAMS * 4));
-        APG_HIP_CHECK(ctx, hipMalloc(&g->v, GUIDE_PARAMS * 4));
-        APG_HIP_CHECK(ctx, hipMalloc(&g->grad, GUIDE_PARAMS * 4));
-        APG_HIP_CHECK(ctx, hipMalloc(&g->part, size_t(WG_MAX) * GUIDE_PARAMS * 4));
-        APG_HIP_CHECK(ctx, hipMalloc(&g->scratch, 64));
+        // allocated in full or not at all (a failure part-way frees what it got, so the next
+        // call starts again instead of using a half-allocated network)
+        auto reset = [g]() {
+            for (void** p : { (void**)&g->w32, (void**)&g->w16, (void**)&g->m, (void**)&g->v, (void**)&g->grad,
+                              (void**)&g->part, (void**)&g->scratch })
+            {
+                if (*p) (void)hipFree(*p);
+                *p = nullptr;
+            }
+        };
+        hipError_t e = hipSuccess;
+        if (e == hipSuccess) e = hipMalloc(&g->w32, GUIDE_PARAMS * 4);
+        if (e == hipSuccess) e = hipMalloc(&g->w16, WSZ * 2);
+        if (e == hipSuccess) e = hipMalloc(&g->m, GUIDE_PARAMS * 4);
+        if (e == hipSuccess) e = hipMalloc(&g->v, GUIDE_PARAMS * 4);
+        if (e == hipSuccess) e = hipMalloc(&g->grad, GUIDE_PARAMS * 4);
+        if (e == hipSuccess) e = hipMalloc(&g->part, size_t(WG_MAX) * GUIDE_PARAMS * 4);
+        if (e == hipSuccess) e = hipMalloc(&g->scratch, 64);
         std::vector<float> p;
         init_params(p, 1);
-        APG_HIP_CHECK(ctx, hipMemcpy(g->w32, p.data(), GUIDE_PARAMS * 4, hipMemcpyHostToDevice));
-        APG_HIP_CHECK(ctx, hipMemset(g->m, 0, GUIDE_PARAMS * 4));
-        APG_HIP_CHECK(ctx, hipMemset(g->v, 0, GUIDE_PARAMS * 4));
-        k_guide_sync16<<<nb(WSZ, 256), 256>>>(g->w32, g->w16);
-        APG_HIP_CHECK(ctx, hipDeviceSynchronize());
+        if (e == hipSuccess) e = hipMemcpy(g->w32, p.data(), GUIDE_PARAMS * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(g->m, 0, GUIDE_PARAMS * 4);
+        if (e == hipSuccess) e = hipMemset(g->v, 0, GUIDE_PARAMS * 4);
+        if (e == hipSuccess)
+        {
+            k_guide_sync16<<<nb(WSZ, 256), 256>>>(g->w32, g->w16);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e != hipSuccess)
+        {
+            reset();
+            return ctx->fail(APG_E_HIP, std::string("guide network allocation: ") + hipGetErrorString(e));
+        }
+        g->ready = true;
     }
     if (cap > g->cap)
     {
@@ -806,6 +828,7 @@ int guide_ensure(Context* ctx, uint32_t cap)
         if (g->dz) (void)hipFree(g->dz);
         if (g->feat16) (void)hipFree(g->feat16);
         g->acts = nullptr; g->out = nullptr; g->dz = nullptr; g->feat16 = nullptr;
+        g->cap = 0;  // set again once all four are allocated
         APG_HIP_CHECK(ctx, hipMalloc(&g->feat16, size_t(cap) * GUIDE_IN * 2));
         APG_HIP_CHECK(ctx, hipMalloc(&g->acts, size_t(cap) * 4 * GUIDE_HID * 2));
         APG_HIP_CHECK(ctx, hipMalloc(&g->out, size_t(cap) * GUIDE_OUT * 4));
@@ -819,7 +842,7 @@ void guide_free(GuideNet* g)
 {
     if (!g) return;
     for (void* p : { (void*)g->w32, (void*)g->w16, (void*)g->m, (void*)g->v, (void*)g->grad, (void*)g->part, (void*)g->scratch,
-                     (void*)g->acts, (void*)g->out, (void*)g->dz })
+                     (void*)g->acts, (void*)g->out, (void*)g->dz, (void*)g->feat16 })
         if (p) (void)hipFree(p);
     delete g;
 }

@@ -85,7 +85,6 @@ Split find_split(const BPrim* p, uint32_t n, float& cost_out)
         cost_out = 0.0f;
         return best;
     }
-    best.valid = true;
     float min_cost = FLT_MAX;
     for (uint8_t d = 0; d < 3; ++d)
     {
@@ -120,6 +119,10 @@ Split find_split(const BPrim* p, uint32_t n, float& cost_out)
         {
             if (costs[s] < min_cost)
             {
+                // valid only once a plane has won: when every cost is >= FLT_MAX (areas
+                // overflowed) the reference keeps its default Split, sends every primitive
+                // below and recurses without end; here the node becomes a leaf
+                best.valid = true;
                 min_cost = costs[s];
                 best.dim = d;
                 best.idx = uint8_t(s);
@@ -335,14 +338,14 @@ uint32_t quantise_axis(float lo_f, float hi_f, float slack, const float* clo, co
 }
 } // namespace
 
-void collapse_to_wide(const BinaryTree& bt, const std::vector<TriPrep>& tris,
-                      std::vector<ApgWideNode>& nodes, std::vector<float>& tri9,
-                      std::vector<uint32_t>& tri_ids)
+uint32_t collapse_to_wide(const BinaryTree& bt, const std::vector<TriPrep>& tris,
+                          std::vector<ApgWideNode>& nodes, std::vector<float>& tri9,
+                          std::vector<uint32_t>& tri_ids)
 {
     nodes.clear();
     tri9.clear();
     tri_ids.clear();
-    if (tris.empty()) return;
+    if (tris.empty()) return 0;
     const float slack = quant_slack(tris);
     const auto& bn = bt.nodes;
     auto is_leaf = [&](uint32_t i) { return bn[i].count > 0; };
@@ -351,14 +354,16 @@ void collapse_to_wide(const BinaryTree& bt, const std::vector<TriPrep>& tris,
     tri9.reserve(tris.size() * 9);
     tri_ids.reserve(tris.size());
 
-    struct Item { uint32_t bin, wide; };
+    struct Item { uint32_t bin, wide, level; };
     std::vector<Item> queue;
     nodes.push_back(ApgWideNode{});
-    queue.push_back({ 0, 0 });
+    queue.push_back({ 0, 0, 1 });
     size_t head = 0;
+    uint32_t levels = 0;
     while (head < queue.size())
     {
         Item it = queue[head++];
+        levels = std::max(levels, it.level);
         // Gather up to 8 children by repeatedly opening the inner child with
         // the largest surface area (wide_bvh.cpp:200-243).
         uint32_t ch[APG_WIDE];
@@ -456,7 +461,7 @@ void collapse_to_wide(const BinaryTree& bt, const std::vector<TriPrep>& tris,
             else
             {
                 imask |= 1u << s;
-                pending.push_back({ c, nd.child_base + inner_rank++ });
+                pending.push_back({ c, nd.child_base + inner_rank++, it.level + 1 });
             }
         }
         for (size_t i = 0; i < pending.size(); ++i) nodes.push_back(ApgWideNode{});
@@ -476,6 +481,7 @@ void collapse_to_wide(const BinaryTree& bt, const std::vector<TriPrep>& tris,
         nodes[it.wide] = nd;
         for (const Item& p : pending) queue.push_back(p);
     }
+    return levels;
 }
 
 void prepare_triangles(const Context& ctx, std::vector<TriPrep>& out)

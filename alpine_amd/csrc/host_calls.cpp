@@ -37,9 +37,8 @@ HostStaging* Context::take_staging(size_t n, int& rc)
     }
     auto fail = [&](hipError_t e, const char* what) {
         give_staging(s);
-        rc = APG_E_HIP;
-        last_error = std::string(what) + ": " + hipGetErrorString(e);
-        return nullptr;
+        rc = this->fail(APG_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+        return static_cast<HostStaging*>(nullptr);
     };
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) return fail(e, "hipSetDevice");
@@ -58,7 +57,8 @@ HostStaging* Context::take_staging(size_t n, int& rc)
         if ((e = hipMalloc(&s->d_in, cap * 8 * sizeof(float))) != hipSuccess) return fail(e, "hipMalloc");
         if ((e = hipMalloc(&s->d_out, cap * 4 * sizeof(float))) != hipSuccess) return fail(e, "hipMalloc");
         if ((e = hipHostMalloc(&s->h_in, cap * 8 * sizeof(float))) != hipSuccess) return fail(e, "hipHostMalloc");
-        if ((e = hipHostMalloc(&s->h_out, cap * 4 * sizeof(float))) != hipSuccess) return fail(e, "hipHostMalloc");
+        // 4 result words per ray, then the overflow flag of the launch
+        if ((e = hipHostMalloc(&s->h_out, (cap * 4 + 1) * sizeof(float))) != hipSuccess) return fail(e, "hipHostMalloc");
         s->cap = cap;
     }
     return s;
@@ -106,6 +106,8 @@ int trace_host(Context* ctx, const float* rays6, const float* tmax, uint32_t n, 
                 for (int k = 0; k < 4 && e == hipSuccess; ++k)
                     e = hipMemcpyAsync(s->h_out + k * N, s->d_out + k * N, n * sizeof(float),
                                        hipMemcpyDeviceToHost, s->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(s->h_out + 4 * N, ctx->counters + 1, 4, hipMemcpyDeviceToHost, s->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
         }
     }
@@ -115,6 +117,10 @@ int trace_host(Context* ctx, const float* rays6, const float* tmax, uint32_t n, 
         ctx->give_staging(s);
         return ctx->fail(APG_E_HIP, std::string("staged trace: ") + hipGetErrorString(e));
     }
+    // the overflow flag read behind the results: set, it is reported (and cleared) here
+    uint32_t overflow = 0;
+    std::memcpy(&overflow, s->h_out + 4 * N, 4);
+    if (rc == APG_OK && overflow) rc = ctx->take_overflow();
     if (rc == APG_OK)
     {
         if (any)

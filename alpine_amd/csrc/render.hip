@@ -814,6 +814,9 @@ template <typename T>
 int dmalloc(Context* ctx, T*& p, size_t count)
 {
     p = nullptr;
+    if (ctx->fail_allocs_after == 0)
+        return ctx->fail(APG_E_HIP, "hipMalloc: out of memory (forced by apg_debug_fail_allocations)");
+    if (ctx->fail_allocs_after > 0) ctx->fail_allocs_after--;
     APG_HIP_CHECK(ctx, hipMalloc(&p, (count ? count : 1) * sizeof(T)));
     return APG_OK;
 }
@@ -862,6 +865,8 @@ inline uint32_t nblocks(uint32_t n, uint32_t b) { return n ? (n + b - 1) / b : 1
 void render_state_free(RenderState* R)
 {
     if (!R) return;
+    // graph replays still queued read every buffer below and the exec itself
+    if (R->gstream) (void)hipStreamSynchronize(R->gstream);
     dfree(R->mats);
     dfree(R->tri_mat);
     dfree(R->tri_ng);
@@ -1357,6 +1362,15 @@ static int ensure_guide_buffers(apg_context* ctx, RenderState* R, uint32_t depth
     if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
     for (void* p : R->guide_allocs) (void)hipFree(p);
     R->guide_allocs.clear();
+    // Nothing refers to the freed buffers from here on, and the sizes are set only once every
+    // allocation has succeeded: a failure part-way leaves the state "not allocated", so the
+    // next guided render allocates again instead of reusing freed memory.
+    R->gfeat = R->gout = R->vrec = nullptr;
+    R->tin = R->tfeat = R->tdir = R->tw = R->tdir_p = R->tw_p = R->twsum = nullptr;
+    R->tcount = nullptr;
+    R->vrec_depth = 0;
+    R->tcap = 0;
+    R->guide_P = 0;
     auto get = [&](auto*& ptr, size_t count) -> int {
         int rc = dmalloc(ctx, ptr, count);
         if (rc == APG_OK) R->guide_allocs.push_back(ptr);
@@ -1371,11 +1385,19 @@ static int ensure_guide_buffers(apg_context* ctx, RenderState* R, uint32_t depth
         (rc = get(R->tfeat, tpad * 9)) || (rc = get(R->tdir, size_t(tcap) * 3)) ||
         (rc = get(R->tw, tcap)) || (rc = get(R->tdir_p, tpad * 3)) || (rc = get(R->tw_p, tpad)) ||
         (rc = get(R->twsum, TRAIN_MAX_STEPS)) || (rc = get(R->tcount, 4)))
+    {
+        R->gfeat = nullptr;  // the check at the top then allocates again
         return rc;
+    }
+    if ((rc = guide_ensure(ctx, tcap)))
+    {
+        R->gfeat = nullptr;
+        return rc;
+    }
     R->vrec_depth = depth;
     R->tcap = tcap;
     R->guide_P = P;
-    return guide_ensure(ctx, tcap);
+    return APG_OK;
 }
 
 int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
@@ -1540,7 +1562,14 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
         {
             if (!R->gstream) APG_HIP_CHECK(ctx, hipStreamCreateWithFlags(&R->gstream, hipStreamNonBlocking));
             if (!R->gev) APG_HIP_CHECK(ctx, hipEventCreateWithFlags(&R->gev, hipEventDisableTiming));
-            if (R->gexec) { (void)hipGraphExecDestroy(R->gexec); R->gexec = nullptr; }
+            if (R->gexec)
+            {
+                // replays of the old exec may still be queued on gstream: destroying it under
+                // them frees what the runtime's completion handling still reads
+                APG_HIP_CHECK(ctx, hipStreamSynchronize(R->gstream));
+                (void)hipGraphExecDestroy(R->gexec);
+                R->gexec = nullptr;
+            }
             hipGraph_t graph = nullptr;
             APG_HIP_CHECK(ctx, hipStreamBeginCapture(R->gstream, hipStreamCaptureModeRelaxed));
             rc = enqueue_pass(R->gstream, no_train);
@@ -1624,6 +1653,7 @@ int apg_render_read(apg_context* ctx, float* rgb, double* rgb64, uint32_t* total
     RenderState* R = state_of(ctx);
     if (!R->P) return ctx->fail(APG_E_STATE, "apg_render_reset has not been called");
     if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+    if (int rc = ctx->take_overflow()) return rc;
     const uint32_t n3 = 3u * R->NPIX;
     if (rgb)
     {
@@ -1647,6 +1677,7 @@ int apg_render_read_aovs(apg_context* ctx, float* albedo, float* normal)
     RenderState* R = state_of(ctx);
     if (!R->P) return ctx->fail(APG_E_STATE, "apg_render_reset has not been called");
     if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+    if (int rc = ctx->take_overflow()) return rc;
     const uint32_t n3 = 3u * R->NPIX;
     for (int k = 0; k < 2; ++k)
     {

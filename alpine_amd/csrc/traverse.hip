@@ -28,7 +28,10 @@ namespace apg {
 
 // Stack entries per lane: STACK in LDS (8 or 16, chosen by the launch
 // configuration) and APG_STACK_MAX - STACK in the per-thread HBM spill area.
-constexpr uint32_t APG_STACK_MAX = 64;
+// A lane pushes at most one entry per wide level it descends, so the builders
+// reject trees deeper than APG_STACK_MAX levels and the push below can only
+// overflow under a lowered stack_limit.
+constexpr uint32_t APG_STACK_MAX = APG_TRACE_STACK_MAX;
 constexpr uint32_t APG_SPILL_ALLOC = 56;  // spill entries allocated per thread (>= 64 - 8)
 
 // Triangle-group queue entries per wave: room for 64 new groups on top of the
@@ -69,6 +72,7 @@ struct TraceArgs
     unsigned long long* fetch_counts; // [0] nodes, [1] triangles
     uint32_t fetch_threshold;
     uint32_t tri_batch;               // queued triangle tests that start a round
+    uint32_t stack_max;               // stack entries a lane may hold (<= APG_STACK_MAX)
 };
 
 // Inclusive prefix sum over the 64 lanes of a wave: shift-and-add inside each
@@ -100,6 +104,7 @@ template <bool ANY, bool COUNT, uint32_t APG_STACK_LDS, bool WT>
 __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
 {
     constexpr uint32_t APG_STACK_SPILL = APG_STACK_MAX - APG_STACK_LDS;
+    static_assert(APG_STACK_SPILL <= APG_SPILL_ALLOC, "spill area smaller than the stack");
     extern __shared__ uint32_t lds[];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = threadIdx.x >> 6;
@@ -205,19 +210,17 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
             if (hits)
             {
                 // push the remaining siblings
-                if (sp >= APG_STACK_LDS)
+                if (sp >= A.stack_max)
                 {
-                    const uint32_t deep = sp - APG_STACK_LDS;
-                    if (deep >= APG_STACK_SPILL)
-                    {
-                        A.counters[1] = 1u;
-                        hits = 0;  // dropped: the overflow flag marks the results unreliable
-                    }
-                    else
-                    {
-                        const uint32_t ring = (sp % APG_STACK_LDS) * 64u + lane;
-                        spill[deep] = (uint64_t(st_bits[ring]) << 32) | st_base[ring];
-                    }
+                    // dropped: the host reports APG_E_OVERFLOW for this launch's results
+                    A.counters[1] = 1u;
+                    hits = 0;
+                }
+                else if (sp >= APG_STACK_LDS)
+                {
+                    const uint32_t deep = sp - APG_STACK_LDS;  // < APG_STACK_SPILL
+                    const uint32_t ring = (sp % APG_STACK_LDS) * 64u + lane;
+                    spill[deep] = (uint64_t(st_bits[ring]) << 32) | st_base[ring];
                 }
                 if (hits)
                 {
@@ -473,6 +476,11 @@ static int ensure_trace_buffers(Context* ctx, uint32_t threads)
         APG_HIP_CHECK(ctx, hipMalloc(&ctx->counters, 64));
         APG_HIP_CHECK(ctx, hipMemset(ctx->counters, 0, 64));
     }
+    if (!ctx->overflow_host)
+    {
+        APG_HIP_CHECK(ctx, hipHostMalloc(&ctx->overflow_host, 64));
+        *ctx->overflow_host = 0u;
+    }
     if (ctx->spill_threads < threads)
     {
         if (ctx->spill)
@@ -534,6 +542,7 @@ int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint3
     A.fetch_counts = counts;
     A.fetch_threshold = cfg.fetch_threshold;
     A.tri_batch = cfg.tri_batch >= 1u && cfg.tri_batch <= 64u ? cfg.tri_batch : 48u;
+    A.stack_max = cfg.stack_limit >= 1u && cfg.stack_limit <= APG_STACK_MAX ? cfg.stack_limit : APG_STACK_MAX;
 
     // the watertight test reads the vertex-form triangles (8-entry LDS stack only)
     const bool wt = cfg.triangle_test == APG_TRI_WATERTIGHT;
@@ -562,6 +571,17 @@ int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint3
 #undef APG_LAUNCH
     APG_HIP_CHECK(ctx, hipGetLastError());
     return APG_OK;
+}
+
+int Context::take_overflow()
+{
+    if (!counters || !overflow_host) return APG_OK;
+    APG_HIP_CHECK(this, hipMemcpy(overflow_host, counters + 1, 4, hipMemcpyDeviceToHost));
+    if (*overflow_host == 0u) return APG_OK;
+    APG_HIP_CHECK(this, hipMemset(counters + 1, 0, 4));
+    *overflow_host = 0u;
+    return fail(APG_E_OVERFLOW, "a traversal needed more stack entries than the launch allows "
+                                "(stack_limit); its results are not reliable");
 }
 
 int launch_trace(Context* ctx, const apg_rays* rays, uint32_t n, const apg_hits* hits, bool any,
@@ -647,7 +667,15 @@ int apg_intersect_counted(apg_context* ctx, const apg_rays* rays, uint32_t n,
     (void)hipFree(counts);
     *node_fetches = host[0];
     *tri_fetches = host[1];
+    if (rc == APG_OK) rc = ctx->take_overflow();
     return rc;
+}
+
+int apg_trace_status(apg_context* ctx)
+{
+    if (!ctx) return APG_E_INVALID;
+    if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+    return ctx->take_overflow();
 }
 
 } // extern "C"

@@ -185,6 +185,34 @@ def small_scene(n_random=2000, subdiv=3, seed=7):
                                                                       lmax=0.4))
 
 
+def geometric_chain(n=852, ratio=1.05, size=1e-6):
+    """A deliberately deep tree: n tiny right triangles at x = ratio^i along the x axis
+    (triangle i spans x_i .. x_i (1 + size) and 0 .. size x_i in y and z).  Binned SAH peels a
+    few triangles off the far end per level, so the binary tree is about 30 levels deep and
+    the 8-wide tree 24 (at the defaults: x up to 2^60, where surface areas stay finite)."""
+    x = ratio ** np.arange(n, dtype=np.float64)
+    v = np.zeros((3 * n, 3), np.float64)
+    v[0::3, 0] = x
+    v[1::3, 0] = x * (1.0 + size)
+    v[1::3, 1] = size * x
+    v[2::3, 0] = x
+    v[2::3, 2] = size * x
+    return v.astype(np.float32), np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+
+
+def chain_rays(n, verts, seed=9):
+    """Rays through the chain's boxes: origins on the chain's axis below its first triangle,
+    directions within 0.01 rad of +x, offsets in y and z up to the first triangles' size."""
+    rng = np.random.default_rng(seed)
+    x0 = float(verts[:, 0].min())
+    s = float(verts[1, 1])
+    org = np.stack([np.full(n, 0.5 * x0), rng.uniform(0.0, s, n), rng.uniform(0.0, s, n)], -1)
+    a = rng.uniform(-0.01, 0.01, (n, 2))
+    d = np.stack([np.ones(n), a[:, 0], a[:, 1]], -1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return org.astype(np.float32), d.astype(np.float32)
+
+
 # ---------------------------------------------------------------------------
 # Render scenes
 # ---------------------------------------------------------------------------

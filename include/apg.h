@@ -37,7 +37,9 @@ extern "C" {
 #define APG_E_HIP (-3)         /* HIP runtime error */
 #define APG_E_STATE (-4)       /* call out of order (e.g. intersect before build) */
 #define APG_E_CAPACITY (-5)    /* size beyond a documented limit */
-#define APG_E_OVERFLOW (-6)    /* traversal stack overflow reported by a kernel */
+#define APG_E_OVERFLOW (-6)    /* traversal stack overflow reported by a kernel: a ray needed more
+                                * stack entries than the launch allowed and its result is not
+                                * reliable (the reference leaves this unchecked, bvh.cpp:230-231) */
 
 #define APG_INVALID_ID 0xFFFFFFFFu /* Accelerator::INVALID_SHAPE_ID, accelerator.h:38 */
 
@@ -88,7 +90,14 @@ typedef struct apg_scene_stats {
     double sah_cost_leaf;      /* binary tree, leaf term only */
     double build_ms;           /* last scene_build, wall clock */
     uint64_t device_bytes;     /* nodes + triangles + index maps */
+    uint32_t wide_depth;       /* levels of 8-wide nodes on the longest root-to-leaf path; a
+                                * traversal holds at most one stack entry per level, so builds
+                                * deeper than APG_TRACE_STACK_MAX fail with APG_E_CAPACITY */
+    uint32_t reserved;
 } apg_scene_stats;
+
+/* Traversal stack entries per ray (the reference's STACK_SIZE, bvh_common.h:15). */
+#define APG_TRACE_STACK_MAX 64u
 
 /* Traversal launch shape (filled with defaults by apg_trace_config_default;
  * the autotuner in alpine_amd/autotune.py writes these). */
@@ -102,7 +111,10 @@ typedef struct apg_trace_config {
                                 * bit-exact hits (bvh_common.cpp:66-103); APG_TRI_WATERTIGHT: the
                                 * watertight test of Woop, Benthin and Wald (JCGT 2013) on the input
                                 * vertices, no ray passes between triangles sharing an edge */
-    uint32_t reserved[2];
+    uint32_t stack_limit;      /* 0: APG_TRACE_STACK_MAX entries; 1..63 caps the stack (test hook:
+                                * a scene deeper than the cap makes traversals report
+                                * APG_E_OVERFLOW, as a tree deeper than the stack would) */
+    uint32_t reserved;
 } apg_trace_config;
 #define APG_TRI_REFERENCE 0u
 #define APG_TRI_WATERTIGHT 1u
@@ -114,13 +126,16 @@ const char* apg_last_error(const apg_context* ctx);
 
 /* ---- geometry (Accelerator::appendMesh / getVertexBuffer / updateShape / updateScene,
  *      accelerator.h:19-31) ---- */
-/* Copies nv vertices (xyz, host) and nt triangles (3 x uint32, host). */
+/* Copies nv vertices (xyz, host) and nt triangles (3 x uint32, host).  A NaN or infinite
+ * coordinate, or an index >= nv, is rejected with APG_E_INVALID (the reference only asserts
+ * its capacity, bvh.cpp:89,101, and would build boxes of NaN). */
 int apg_mesh_append(apg_context* ctx, const float* xyz, uint32_t nv, const uint32_t* idx,
                     uint32_t nt, uint32_t* shape_id);
 /* Mutable host view of the library's copy of the shape's vertices
  * (Accelerator::getVertexBuffer).  Write it, then call apg_mesh_update. */
 int apg_mesh_vertices(apg_context* ctx, uint32_t shape_id, float** xyz, uint32_t* nv);
-int apg_mesh_update(apg_context* ctx, uint32_t shape_id);           /* updateShape */
+/* updateShape: the vertex copy is checked for NaN / infinite coordinates (APG_E_INVALID). */
+int apg_mesh_update(apg_context* ctx, uint32_t shape_id);
 int apg_mesh_prim_offset(const apg_context* ctx, uint32_t shape_id, uint32_t* offset);
 /* Rigid transform of vertices [first, first + count) of a shape, applied on the GPU to the
  * library's device copy (no upload); m is 3 x 4 row-major (host).  The next
@@ -170,6 +185,12 @@ int apg_intersect(apg_context* ctx, const apg_rays* rays, uint32_t n, const apg_
 /* Any hit with t < tmax[i] (FLT_MAX when tmax is NULL); occluded[i] = 0 or 1. */
 int apg_occluded(apg_context* ctx, const apg_rays* rays, uint32_t n, uint8_t* occluded,
                  void* stream);
+/* Wait for the context's traversals issued so far and report whether any of them overflowed its
+ * stack since the last report: APG_OK, or APG_E_OVERFLOW (the flag is then cleared).  The calls
+ * that wait for the GPU themselves (apg_intersect_host, apg_occluded_host, apg_intersect_counted,
+ * apg_render_read, apg_render_read_aovs) report an overflow the same way.  No reference
+ * counterpart: the reference's traversal asserts in debug builds only (bvh.cpp:230-231). */
+int apg_trace_status(apg_context* ctx);
 /* Host-memory, synchronous variants for per-ray callers: Accelerator::intersect(const Ray&) and
  * intersectAny(const Ray&, float tFar) (accelerator.h:33-35), called one ray at a time from up to
  * 256 tile threads (alpine.cpp:276-285).  rays6: n rays in alpine's Ray layout (org xyz, dir xyz;

@@ -31,6 +31,18 @@ int apg_debug_scene_copy(const apg_debug_scene* s, void* nodes64, float* tri9, u
 int apg_debug_host_trace(const apg_debug_scene* s, const float* org3, const float* dir3,
                          const float* tmax, uint32_t n, int any, uint32_t* prim, float* t,
                          float* b0, float* b1, uint8_t* occluded, uint32_t* max_stack);
+/* As apg_debug_host_trace with the traversal stack capped at stack_limit entries (1 ..
+ * APG_TRACE_STACK_MAX), pushes onto a full stack dropped as trace_kernel drops them;
+ * overflowed receives the number of rays that dropped one. */
+int apg_debug_host_trace_limited(const apg_debug_scene* s, const float* org3, const float* dir3,
+                                 const float* tmax, uint32_t n, int any, uint32_t stack_limit,
+                                 uint32_t* prim, float* t, float* b0, float* b1, uint8_t* occluded,
+                                 uint32_t* overflowed);
+/* Render-state device allocations of ctx (render.hip) fail with APG_E_HIP after `after` more
+ * have succeeded; after < 0 restores normal allocation.  For tests of the failure paths. */
+int apg_debug_fail_allocations(apg_context* ctx, int after);
+/* Levels of 8-wide nodes of the scene's deepest root-to-leaf path. */
+int apg_debug_scene_depth(const apg_debug_scene* s, uint32_t* wide_depth);
 void apg_debug_scene_free(apg_debug_scene* s);
 
 /* Host build of the shading steps (alpine_amd/csrc/shade_core.h), n queries each.

@@ -98,3 +98,38 @@ def test_guide_buffers_follow_the_frame_size():
         img, _, spp = acc.render_read()
         n, loss = acc.guide_stats()
         assert img.shape == (w, w, 3) and spp == 8 and np.isfinite(img).all() and n > 0
+
+
+def test_guide_buffers_recover_from_a_failed_allocation():
+    """A guided render whose guide buffers fail to allocate part-way (forced by the test hook)
+    returns an error and leaves nothing behind: the next render allocates them again and its
+    image agrees with a context that never failed within Monte Carlo error (online training
+    appends its samples with atomics, so guided images are not bit-reproducible), and with
+    plain path tracing."""
+    print("[gpu] guided render after a failed guide-buffer allocation", flush=True)
+    sc = scenes.cornell_scene(64, 64)
+    imgs = []
+    for fail in (False, True):
+        acc = _setup(sc)
+        acc.render_reset(64, 64, 5)
+        acc.render(4, 4, guide=1, guide_warmup=1, train_steps=4, train_samples=4096)
+        if fail:
+            # a deeper path needs larger per-path records: the buffers are freed and
+            # re-allocated, and the fourth allocation fails
+            assert aa.lib().apg_debug_fail_allocations(acc._h, 3) == 0
+            with pytest.raises(aa.ApgError):
+                acc.render(4, 8, guide=1, guide_warmup=1, train_steps=4, train_samples=4096)
+            assert aa.lib().apg_debug_fail_allocations(acc._h, -1) == 0
+        acc.render(60, 8, guide=1, guide_warmup=1, train_steps=4, train_samples=4096)
+        img, _, spp = acc.render_read()
+        assert spp == 64 and np.isfinite(img).all()
+        imgs.append(img / spp)
+        acc.close()
+    acc = _setup(sc)
+    acc.render_reset(64, 64, 11)
+    acc.render(64, 8)
+    plain, _, _ = acc.render_read()
+    acc.close()
+    print("means", imgs[0].mean(), imgs[1].mean(), plain.mean() / 64)
+    assert imgs[1].mean() == pytest.approx(imgs[0].mean(), rel=0.03)
+    assert imgs[1].mean() == pytest.approx(plain.mean() / 64, rel=0.03)
