@@ -147,7 +147,7 @@ ABI_SYMBOLS = [
     "apg_render_guide_stats", "apg_mesh_set_shading", "apg_mesh_set_targets", "apg_mesh_morph",
     "apg_textures_set", "apg_render_read_aovs", "apg_render_set_environment",
     "apg_intersect_host", "apg_occluded_host", "apg_light_sampler_set",
-    "apg_light_sampler_sample_host", "apg_trace_status",
+    "apg_light_sampler_sample_host", "apg_trace_status", "apg_intersect_noalloc",
 ]
 DEBUG_SYMBOLS = ["apg_debug_host_build", "apg_debug_scene_info", "apg_debug_scene_copy",
                  "apg_debug_host_trace", "apg_debug_scene_free", "apg_debug_material",
@@ -191,6 +191,7 @@ def lib():
         "apg_intersect_roots": (i32, [vp, C.POINTER(_Rays), vp, u32, C.POINTER(_Hits), vp]),
         "apg_occluded_roots": (i32, [vp, C.POINTER(_Rays), vp, u32, vp, vp]),
         "apg_trace_status": (i32, [vp]),
+        "apg_intersect_noalloc": (i32, [vp, C.POINTER(_Rays), u32, C.POINTER(_Hits), vp]),
         "apg_intersect_counted": (i32, [vp, C.POINTER(_Rays), u32, C.POINTER(_Hits), i32, vp,
                                         C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), vp]),
         "apg_scene_triangles": (i32, [vp, vp, vp, u32]),

@@ -103,6 +103,7 @@ struct Context
     apg_trace_config cfg{};
     std::mutex launch_mutex;
     hipEvent_t last_launch = nullptr;
+    hipEvent_t noalloc_launch = nullptr;  // the last apg_intersect_noalloc outside graph capture
     uint32_t* counters = nullptr;     // [0] ray cursor, [1] overflow flag, [2] waves done
     uint64_t* spill = nullptr;        // per persistent thread stack spill
     size_t spill_threads = 0;
@@ -186,6 +187,9 @@ struct apg_context : apg::Context {};
     } while (0)
 
 namespace apg {
+// The device address of apg_intersect_noalloc's scratch on `device` (current), looked up once
+// per device (traverse.hip).
+int noalloc_scratch_init(int device, std::string& err);
 int ensure_trace_capacity(Context* ctx, uint32_t n);
 int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint32_t* n_dev,
                      const apg_hits* hits, bool any, uint8_t* occ, unsigned long long* counts,

@@ -47,6 +47,7 @@ Context::~Context()
         delete s;
     }
     if (last_launch) (void)hipEventDestroy(last_launch);
+    if (noalloc_launch) (void)hipEventDestroy(noalloc_launch);
     if (render) render_state_free(render);
     if (guide) guide_free(guide);
 }
@@ -175,9 +176,23 @@ int apg_create(apg_context** out, int device)
         g_create_error = std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
         return APG_E_HIP;
     }
+    std::string err;
+    if (int rc = apg::noalloc_scratch_init(device, err))
+    {
+        g_create_error = err;
+        return rc;
+    }
+    hipEvent_t ev = nullptr;
+    e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e != hipSuccess)
+    {
+        g_create_error = std::string("hipEventCreateWithFlags: ") + hipGetErrorString(e);
+        return APG_E_HIP;
+    }
     auto* ctx = new apg_context;
     ctx->device = device;
     ctx->cu_count = prop.multiProcessorCount;
+    ctx->noalloc_launch = ev;
     apg_trace_config_default(ctx, &ctx->cfg);
     *out = ctx;
     return APG_OK;
@@ -277,6 +292,7 @@ int apg_scene_build(apg_context* ctx, const apg_build_params* params)
 
     // Wait for kernels still reading the old scene.
     if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+    if (ctx->noalloc_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->noalloc_launch));
     int rc = APG_OK;
     ctx->built = false;  // until this build has succeeded
 

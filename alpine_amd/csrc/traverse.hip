@@ -22,7 +22,9 @@
 #include "apg_internal.h"
 #include "traverse_core.h"
 
+#include <algorithm>
 #include <cfloat>
+#include <string>
 
 namespace apg {
 
@@ -68,7 +70,8 @@ struct TraceArgs
     apg_hits hits;
     uint8_t* occluded;
     uint32_t* counters;               // [0] ray cursor, [1] stack overflow flag, [2] waves done
-    uint64_t* spill;
+    uint64_t* spill;                  // spill_stride entries per persistent thread
+    uint32_t spill_stride;
     unsigned long long* fetch_counts; // [0] nodes, [1] triangles
     uint32_t fetch_threshold;
     uint32_t tri_batch;               // queued triangle tests that start a round
@@ -121,7 +124,7 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
     // group, and the lane that owns it
     uint32_t* queue = wl + APG_STACK_LDS * 128u + 320u;
     uint8_t* q_owner = reinterpret_cast<uint8_t*>(queue + 3u * TRACE_QCAP);
-    uint64_t* spill = A.spill + size_t(blockIdx.x * blockDim.x + threadIdx.x) * APG_SPILL_ALLOC;
+    uint64_t* spill = A.spill + size_t(blockIdx.x * blockDim.x + threadIdx.x) * A.spill_stride;
 
     const uint32_t nrays = A.n_dev ? *A.n_dev : A.n;
     const uint32_t batch = A.tri_batch;
@@ -508,6 +511,83 @@ int ensure_trace_capacity(Context* ctx, uint32_t n)
     return ensure_trace_buffers(ctx, grid * cfg.block_size);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free closest hit (apg_intersect_noalloc)
+// ---------------------------------------------------------------------------
+// The ray cursor / flags and the stack spill live in this file-scope device array instead of
+// context buffers, so the entry point never allocates.  Its stack is the LDS top plus
+// NOALLOC_SPILL spilled entries; scenes whose wide tree needs more are refused up front.
+constexpr uint32_t NOALLOC_THREADS = 256u * 6u * 256u;  // the default persistent grid on 256 CUs
+constexpr uint32_t NOALLOC_SPILL = 8u;
+constexpr size_t NOALLOC_COUNTER_WORDS = 32u;         // 256 B: 16 counters, then padding
+constexpr size_t NOALLOC_WORDS = NOALLOC_COUNTER_WORDS + size_t(NOALLOC_THREADS) * NOALLOC_SPILL;
+static __device__ __align__(256) uint64_t scratch[NOALLOC_WORDS];  // 24 MiB + 256 B per GPU
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mutex;
+void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};  // device address of `scratch` on each GPU
+
+// Look up the device address of `scratch` on `device` (current) once; apg_create calls this.
+int noalloc_scratch_init(int device, std::string& err)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+    {
+        err = "device index beyond the allocation-free scratch table";
+        return APG_E_CAPACITY;
+    }
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (g_noalloc_ptr[device]) return APG_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+    {
+        err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return APG_E_HIP;
+    }
+    g_noalloc_ptr[device] = p;
+    return APG_OK;
+}
+
+namespace {
+__global__ void k_clear_words(uint32_t* w, uint32_t n)
+{
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) w[i] = 0u;
+}
+} // namespace
+
+// Where a launch keeps its ray cursor / flags and its stack spill, and what that allows.
+struct TraceScratch
+{
+    uint32_t* counters;     // 16 words, [0] and [2] zero between launches
+    uint64_t* spill;        // spill_stride entries per persistent thread
+    uint32_t spill_stride;
+    uint32_t max_threads;   // persistent threads the spill has room for (0: any)
+    uint32_t stack_max;     // stack entries per lane
+};
+
+namespace {
+
+uint32_t trace_grid(const Context* ctx, uint32_t n, uint32_t max_threads)
+{
+    const apg_trace_config& cfg = ctx->cfg;
+    uint32_t grid = uint32_t(ctx->cu_count) * cfg.blocks_per_cu;
+    const uint32_t need = (n + cfg.block_size - 1) / cfg.block_size;
+    if (grid > need) grid = need;
+    if (max_threads && grid > max_threads / cfg.block_size) grid = max_threads / cfg.block_size;
+    return grid ? grid : 1u;
+}
+
+uint32_t config_stack_max(const apg_trace_config& cfg)
+{
+    return cfg.stack_limit >= 1u && cfg.stack_limit <= APG_STACK_MAX ? cfg.stack_limit : APG_STACK_MAX;
+}
+
+int launch_trace_kernel(Context* ctx, const apg_rays& rays, uint32_t n, const uint32_t* n_dev,
+                        const apg_hits* hits, bool any, uint8_t* occ, unsigned long long* counts,
+                        hipStream_t stream, const uint32_t* roots, const TraceScratch& scr);
+
+} // namespace
+
 // Launch one traversal over `rays` on `stream` without cross-stream ordering.
 // n_dev, when non-null, holds the ray count on the device (n is then the
 // capacity used to size the grid).
@@ -515,14 +595,22 @@ int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint3
                      const apg_hits* hits, bool any, uint8_t* occ, unsigned long long* counts,
                      hipStream_t stream, const uint32_t* roots)
 {
+    const uint32_t grid = trace_grid(ctx, n, 0u);
+    int rc = ensure_trace_buffers(ctx, grid * ctx->cfg.block_size);
+    if (rc) return rc;
+    const TraceScratch scr{ ctx->counters, ctx->spill, APG_SPILL_ALLOC, 0u, config_stack_max(ctx->cfg) };
+    return launch_trace_kernel(ctx, rays, n, n_dev, hits, any, occ, counts, stream, roots, scr);
+}
+
+namespace {
+
+int launch_trace_kernel(Context* ctx, const apg_rays& rays, uint32_t n, const uint32_t* n_dev,
+                        const apg_hits* hits, bool any, uint8_t* occ, unsigned long long* counts,
+                        hipStream_t stream, const uint32_t* roots, const TraceScratch& scr)
+{
     const apg_trace_config& cfg = ctx->cfg;
     const uint32_t block = cfg.block_size;
-    uint32_t grid = uint32_t(ctx->cu_count) * cfg.blocks_per_cu;
-    const uint32_t need = (n + block - 1) / block;
-    if (grid > need) grid = need;
-    if (grid == 0) grid = 1;
-    int rc = ensure_trace_buffers(ctx, grid * block);
-    if (rc) return rc;
+    const uint32_t grid = trace_grid(ctx, n, scr.max_threads);
 
     TraceArgs A;
     A.nodes = ctx->scene.nodes;
@@ -537,12 +625,13 @@ int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint3
     A.has_scene = ctx->scene.node_count > 0 ? 1u : 0u;
     if (hits) A.hits = *hits; else A.hits = apg_hits{ nullptr, nullptr, nullptr, nullptr };
     A.occluded = occ;
-    A.counters = ctx->counters;
-    A.spill = ctx->spill;
+    A.counters = scr.counters;
+    A.spill = scr.spill;
+    A.spill_stride = scr.spill_stride;
     A.fetch_counts = counts;
     A.fetch_threshold = cfg.fetch_threshold;
     A.tri_batch = cfg.tri_batch >= 1u && cfg.tri_batch <= 64u ? cfg.tri_batch : 48u;
-    A.stack_max = cfg.stack_limit >= 1u && cfg.stack_limit <= APG_STACK_MAX ? cfg.stack_limit : APG_STACK_MAX;
+    A.stack_max = scr.stack_max;
 
     // the watertight test reads the vertex-form triangles (8-entry LDS stack only)
     const bool wt = cfg.triangle_test == APG_TRI_WATERTIGHT;
@@ -572,6 +661,8 @@ int launch_trace_raw(Context* ctx, const apg_rays& rays, uint32_t n, const uint3
     APG_HIP_CHECK(ctx, hipGetLastError());
     return APG_OK;
 }
+
+} // namespace
 
 int Context::take_overflow()
 {
@@ -619,6 +710,48 @@ int apg_intersect(apg_context* ctx, const apg_rays* rays, uint32_t n, const apg_
     if (!ctx) return APG_E_INVALID;
     return apg::launch_trace(ctx, rays, n, hits, false, nullptr, nullptr,
                              static_cast<hipStream_t>(stream));
+}
+
+int apg_intersect_noalloc(apg_context* ctx, const apg_rays* rays, uint32_t n, const apg_hits* hits,
+                          void* stream_)
+{
+    using namespace apg;
+    if (!ctx) return APG_E_INVALID;
+    if (n == 0) return APG_OK;
+    if (!rays || !hits || !hits->prim || !hits->t || !hits->b0 || !hits->b1)
+        return ctx->fail(APG_E_INVALID, "null ray or result pointer");
+    if (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz)
+        return ctx->fail(APG_E_INVALID, "null ray component pointer");
+    if (!ctx->built) return ctx->fail(APG_E_STATE, "apg_scene_build has not been called");
+    if (n > 0x7fffffffu) return ctx->fail(APG_E_CAPACITY, "more than 2^31 - 1 rays in one call");
+    void* base = ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES ? g_noalloc_ptr[ctx->device] : nullptr;
+    if (!base) return ctx->fail(APG_E_STATE, "allocation-free scratch not set up for this device");
+    const apg_trace_config& cfg = ctx->cfg;
+    const bool wt = cfg.triangle_test == APG_TRI_WATERTIGHT;
+    if (wt && !ctx->scene.tris_v && ctx->scene.tri_count)
+        return ctx->fail(APG_E_STATE, "no vertex-form triangles for the watertight test");
+    const uint32_t lds = (cfg.stack_lds == 16 && !wt) ? 16u : 8u;
+    const uint32_t stack_max = std::min<uint32_t>(lds + NOALLOC_SPILL, APG_STACK_MAX);
+    // a lane holds at most wide_depth - 1 entries: refuse scenes the stack cannot hold
+    if (ctx->scene.wide_depth > stack_max + 1u)
+        return ctx->fail(APG_E_CAPACITY, "the scene's 8-wide tree has " + std::to_string(ctx->scene.wide_depth) +
+                                             " levels; apg_intersect_noalloc holds " +
+                                             std::to_string(stack_max + 1u) + " (use apg_intersect)");
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    uint64_t* words = static_cast<uint64_t*>(base);
+    const TraceScratch scr{ reinterpret_cast<uint32_t*>(words), words + NOALLOC_COUNTER_WORDS, NOALLOC_SPILL,
+                            NOALLOC_THREADS, stack_max };
+    // the counters keep the last call's values: start from zero (a kernel, not a memset node,
+    // so the call can be captured into a graph)
+    k_clear_words<<<1, 64, 0, stream>>>(scr.counters, 16u);
+    APG_HIP_CHECK(ctx, hipGetLastError());
+    int rc = launch_trace_kernel(ctx, *rays, n, nullptr, hits, false, nullptr, nullptr, stream, nullptr, scr);
+    if (rc) return rc;
+    // scene builds wait for this launch (not under capture: the replays are the caller's)
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    APG_HIP_CHECK(ctx, hipStreamIsCapturing(stream, &cap));
+    if (cap == hipStreamCaptureStatusNone) APG_HIP_CHECK(ctx, hipEventRecord(ctx->noalloc_launch, stream));
+    return APG_OK;
 }
 
 int apg_occluded(apg_context* ctx, const apg_rays* rays, uint32_t n, uint8_t* occluded,

@@ -182,6 +182,18 @@ int apg_set_trace_config(apg_context* ctx, const apg_trace_config* cfg);
 /* Closest hit for n rays (device SoA in, device SoA out). */
 int apg_intersect(apg_context* ctx, const apg_rays* rays, uint32_t n, const apg_hits* hits,
                   void* stream);
+/* apg_intersect without any allocation: no reference counterpart.  For callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same kernels and results as apg_intersect, but the ray cursor and the traversal stack's spill
+ * live in one file-scope device array per GPU (24 MiB + 256 B, set up by apg_create) instead of
+ * context buffers, and the stack holds the LDS top (cfg.stack_lds, 8 for the watertight test)
+ * plus 8 spilled entries: a scene whose 8-wide tree has more levels than that plus one is
+ * refused with APG_E_CAPACITY (apg_scene_stats.wide_depth; 17 levels with the default
+ * configuration).  Every call on a GPU shares that GPU's array, so calls that overlap on one
+ * GPU (other streams, other contexts, other threads) race: the caller serialises them.  It does
+ * not wait for other traversals and may be captured into a graph; stack_limit is ignored. */
+int apg_intersect_noalloc(apg_context* ctx, const apg_rays* rays, uint32_t n, const apg_hits* hits,
+                          void* stream);
 /* Any hit with t < tmax[i] (FLT_MAX when tmax is NULL); occluded[i] = 0 or 1. */
 int apg_occluded(apg_context* ctx, const apg_rays* rays, uint32_t n, uint8_t* occluded,
                  void* stream);
