@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(HERE, "libapg.so")
 
 APG_OK = 0
 APG_INVALID_ID = 0xFFFFFFFF
+APG_TRACE_STACK_MAX = 64
 BUILDER_GPU = 0
 BUILDER_HOST = 1
 

@@ -379,7 +379,15 @@ class Camera(C.Structure):
 
 class RenderParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
-                ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32)]
+                ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32), ("mis_mode", C.c_uint32),
+                ("plant", C.c_uint32), ("crop_x", C.c_uint32), ("crop_y", C.c_uint32),
+                ("crop_w", C.c_uint32), ("crop_h", C.c_uint32)]
+
+
+# Test switches of the integrator (oracle/render.h): one direct-light strategy only, and
+# planted mistakes that the closed-form tests must catch.
+MIS_BOTH, MIS_LIGHT_ONLY, MIS_BSDF_ONLY = 0, 1, 2
+PLANT_MIS_WEIGHT, PLANT_NO_NG_KILL, PLANT_SHADOW_NG, PLANT_NO_THROUGHPUT = 1, 2, 4, 8
 
 
 def _render_sigs(L):
@@ -473,16 +481,24 @@ class OracleScene:
             self._h = None
 
     def render(self, width, height, spp, max_depth, states=None, accum=None, rr_depth=0,
-               nthreads=8, aov=None):
+               nthreads=8, aov=None, mis_mode=MIS_BOTH, plant=0, crop=None):
         """aov: optional (height, width, 6) float64 array receiving the first hit's base colour
-        and shading normal sums."""
+        and shading normal sums.  crop = (x0, y0, w, h): trace only that window of the frame;
+        states and accum then cover the window (states seeded by frame pixel index, see
+        crop_states)."""
+        cx, cy, cw, ch = crop if crop is not None else (0, 0, 0, 0)
+        pw, ph = (cw, ch) if crop is not None else (width, height)
+        if crop is not None:
+            assert cw > 0 and ch > 0 and cx + cw <= width and cy + ch <= height
         if states is None:
-            states = sampler_states(width * height)
+            states = sampler_states(pw * ph) if crop is None else crop_states(width, crop)
+        assert states.shape == (pw * ph, 4)
         if accum is None:
-            accum = np.zeros((height, width, 3), np.float64)
+            accum = np.zeros((ph, pw, 3), np.float64)
+        assert accum.size == pw * ph * 3
         if aov is not None:
-            assert aov.dtype == np.float64 and aov.flags.c_contiguous and aov.size == width * height * 6
-        rp = RenderParams(width, height, spp, max_depth, rr_depth)
+            assert aov.dtype == np.float64 and aov.flags.c_contiguous and aov.size == pw * ph * 6
+        rp = RenderParams(width, height, spp, max_depth, rr_depth, mis_mode, plant, cx, cy, cw, ch)
         rlib().orc_render(self._h, C.byref(rp), states.reshape(-1), accum.reshape(-1),
                           None if aov is None else aov.ctypes.data_as(C.c_void_p), nthreads)
         return states, accum
@@ -588,6 +604,18 @@ class OracleScene:
         d = C.c_float()
         p = rlib().orc_light_pdf(self._h, light, _f32(hit), _f32(wi), C.byref(d))
         return p, d.value
+
+
+def crop_states(width, crop, seed_offset=0):
+    """Sampler states of the pixels of window crop = (x0, y0, w, h) of a frame `width` wide,
+    each seeded with its frame pixel index (alpine.cpp:259-265)."""
+    x0, y0, w, h = crop
+    L = rlib()
+    st = np.empty((w * h, 4), np.uint32)
+    for r in range(h):
+        for c in range(w):
+            L.orc_sampler_reset(st[r * w + c], ((y0 + r) * width + x0 + c + seed_offset) & 0xFFFFFFFF)
+    return st
 
 
 def sampler_states(n_pixels, seed_offset=0):

@@ -1078,20 +1078,23 @@ static float mat_pdf_local(const mat_t* m, v3 wo, v3 wi)
 }
 
 /* estimateDirectIllumination (alpine.cpp:370-432) */
-static v3 direct(const orc_scene* S, uint32_t* st, v3 hit, v3 wo, v3 ns, v3 ss, v3 ts,
+static v3 direct(const orc_scene* S, uint32_t* st, v3 hit, v3 wo, v3 ns, v3 ss, v3 ts, v3 ng,
                  const mat_t* m)
 {
+    const uint32_t mis = S->rp.mis_mode, plant = S->rp.plant;
     v3 rad = V(0, 0, 0);
     if (S->ls.n == 0) return rad;
     float lpdf_sel;
     int li = ls_sample(&S->ls, orc_sampler_get1d(st), hit, ns, &lpdf_sel);
     if (li < 0) return rad;
     const light_t* L = &S->ls.lights[li];
-    v3 off = muls(ns, RAY_OFFSET);
+    v3 off = muls((plant & ORC_PLANT_SHADOW_NG) ? ng : ns, RAY_OFFSET);
     {
         v3 le, wiw;
         float dist, pdf;
-        if (light_sample(L, get2d(st), hit, &le, &wiw, &dist, &pdf)) {
+        /* the random numbers are drawn in every mode, so the modes see the same sequence */
+        v2 u = get2d(st);
+        if (mis != ORC_MIS_BSDF_ONLY && light_sample(L, u, hit, &le, &wiw, &dist, &pdf)) {
             v3 wi = to_local(wiw, ss, ts, ns);
             float bpdf = mat_pdf_local(m, wo, wi);
             if (bpdf > 0.0f && !occluded(S, add(hit, off), wiw, dist)) {
@@ -1099,7 +1102,7 @@ static v3 direct(const orc_scene* S, uint32_t* st, v3 hit, v3 wo, v3 ns, v3 ss, 
                 float dp;
                 mat_eval_local(m, wo, wi, &f, &dp);
                 float c = fmax_std(0.0f, dot(wiw, ns));
-                float w = !light_is_delta(L) ? power_heuristic(pdf, bpdf) : 1.0f;
+                float w = !light_is_delta(L) && mis == ORC_MIS_BOTH ? power_heuristic(pdf, bpdf) : 1.0f;
                 rad = add(rad, divs(muls(mulv(muls(le, w), f), c), pdf));
             }
         }
@@ -1107,12 +1110,13 @@ static v3 direct(const orc_scene* S, uint32_t* st, v3 hit, v3 wo, v3 ns, v3 ss, 
     if (!light_is_delta(L)) {
         v3 est, wi;
         float pdf;
-        if (mat_sample_local(m, wo, get2d(st), &est, &wi, &pdf)) {
+        v2 u = get2d(st);
+        if (mis != ORC_MIS_LIGHT_ONLY && mat_sample_local(m, wo, u, &est, &wi, &pdf)) {
             v3 ld = to_world(wi, ss, ts, ns);
             float dist;
             float lp = light_pdf(L, hit, ld, &dist);
             if (lp > 0.0f && !occluded(S, add(hit, off), ld, dist)) {
-                float w = power_heuristic(pdf, lp);
+                float w = mis == ORC_MIS_BOTH && !(plant & ORC_PLANT_MIS_WEIGHT) ? power_heuristic(pdf, lp) : 1.0f;
                 rad = add(rad, mulv(muls(L->le, w), est));
             }
         }
@@ -1170,13 +1174,13 @@ static void trace_sample(const orc_scene* S, uint32_t* st, uint32_t x, uint32_t 
             aov6[3] += ns.x; aov6[4] += ns.y; aov6[5] += ns.z;
         }
         v3 wo = to_local(neg(d), ss, ts, ns);
-        L = add(L, mulv(T, direct(S, st, hit, wo, ns, ss, ts, m)));
+        L = add(L, mulv(T, direct(S, st, hit, wo, ns, ss, ts, ng, m)));
         v3 est, wi;
         float pdf;
         if (!mat_sample_local(m, wo, get2d(st), &est, &wi, &pdf)) break;
         v3 wiw = to_world(wi, ss, ts, ns);
-        if (!(dot(wiw, ng) > 0.0f)) break;
-        T = mulv(T, est);
+        if (!(dot(wiw, ng) > 0.0f) && !(rp->plant & ORC_PLANT_NO_NG_KILL)) break;
+        if (!(rp->plant & ORC_PLANT_NO_THROUGHPUT)) T = mulv(T, est);
         if (rp->rr_depth && depth + 1 >= rp->rr_depth) {
             /* Russian roulette (an option the reference lacks): survive with
              * probability min(max component of T, 0.95). */
@@ -1202,9 +1206,12 @@ static void* render_rows(void* arg)
 {
     rjob_t* J = (rjob_t*)arg;
     const orc_render_params* rp = &J->S->rp;
+    const int crop = rp->crop_w && rp->crop_h;
+    const uint32_t x0 = crop ? rp->crop_x : 0, w = crop ? rp->crop_w : rp->width;
+    const uint32_t y0 = crop ? rp->crop_y : 0;
     for (uint32_t y = J->y0; y < J->y1; ++y)
-        for (uint32_t x = 0; x < rp->width; ++x) {
-            uint32_t i = y * rp->width + x;
+        for (uint32_t x = x0; x < x0 + w; ++x) {
+            uint32_t i = (y - y0) * w + (x - x0);
             uint32_t* st = J->states + 4 * (size_t)i;
             for (uint32_t s = 0; s < J->spp; ++s) {
                 v3 L;
@@ -1223,20 +1230,22 @@ void orc_render(orc_scene* S, const orc_render_params* rp, uint32_t* states, dou
                 double* aov, uint32_t nthreads)
 {
     S->rp = *rp;
+    const int crop = rp->crop_w && rp->crop_h;
+    const uint32_t ybeg = crop ? rp->crop_y : 0, nrows = crop ? rp->crop_h : rp->height;
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
-    if (nthreads > rp->height) nthreads = rp->height;
+    if (nthreads > nrows) nthreads = nrows;
     pthread_t th[256];
     rjob_t jobs[256];
-    uint32_t rows = (rp->height + nthreads - 1) / nthreads;
+    uint32_t rows = (nrows + nthreads - 1) / nthreads;
     for (uint32_t k = 0; k < nthreads; ++k) {
         jobs[k].S = S;
         jobs[k].states = states;
         jobs[k].accum = accum;
         jobs[k].aov = aov;
         jobs[k].spp = rp->spp;
-        jobs[k].y0 = k * rows < rp->height ? k * rows : rp->height;
-        jobs[k].y1 = (k + 1) * rows < rp->height ? (k + 1) * rows : rp->height;
+        jobs[k].y0 = ybeg + (k * rows < nrows ? k * rows : nrows);
+        jobs[k].y1 = ybeg + ((k + 1) * rows < nrows ? (k + 1) * rows : nrows);
         pthread_create(&th[k], NULL, render_rows, &jobs[k]);
     }
     for (uint32_t k = 0; k < nthreads; ++k) pthread_join(th[k], NULL);

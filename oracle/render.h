@@ -52,7 +52,24 @@ typedef struct orc_camera {
 typedef struct orc_render_params {
     uint32_t width, height, spp, max_depth;
     uint32_t rr_depth;    /* 0 = no Russian roulette */
+    /* Checks of the integrator's own tests; 0 for the reference's behaviour.
+     * mis_mode: ORC_MIS_LIGHT_ONLY / ORC_MIS_BSDF_ONLY keep one of the two direct-light
+     * strategies of estimateDirectIllumination (alpine.cpp:396-428) with weight 1.
+     * plant: ORC_PLANT_* mistakes, so tests can show their closed forms catch them. */
+    uint32_t mis_mode;
+    uint32_t plant;
+    /* crop_w, crop_h > 0: only the pixels [crop_x, crop_x + crop_w) x [crop_y, crop_y + crop_h)
+     * of the width x height frame are traced; states and accum then hold those pixels only,
+     * row-major within the crop (each still seeded by the caller with its frame index). */
+    uint32_t crop_x, crop_y, crop_w, crop_h;
 } orc_render_params;
+#define ORC_MIS_BOTH 0u
+#define ORC_MIS_LIGHT_ONLY 1u
+#define ORC_MIS_BSDF_ONLY 2u
+#define ORC_PLANT_MIS_WEIGHT 1u     /* BSDF strategy weighted 1 instead of the power heuristic */
+#define ORC_PLANT_NO_NG_KILL 2u     /* continuation not killed when dot(wi, ng) <= 0 */
+#define ORC_PLANT_SHADOW_NG 4u      /* shadow rays offset along ng instead of ns */
+#define ORC_PLANT_NO_THROUGHPUT 8u  /* path throughput not multiplied by the BSDF estimator */
 
 typedef struct orc_scene orc_scene;
 
