@@ -154,7 +154,7 @@ DEBUG_SYMBOLS = ["apg_debug_host_build", "apg_debug_scene_info", "apg_debug_scen
                  "apg_debug_host_trace", "apg_debug_scene_free", "apg_debug_material",
                  "apg_debug_light_sampler", "apg_debug_light", "apg_debug_shading",
                  "apg_debug_environment", "apg_debug_triangles", "apg_debug_host_trace_limited",
-                 "apg_debug_scene_depth", "apg_debug_fail_allocations"]
+                 "apg_debug_scene_depth", "apg_debug_fail_allocations", "apg_debug_stream_read"]
 
 
 def lib():
@@ -237,6 +237,7 @@ def lib():
                                                C.POINTER(u32)]),
         "apg_debug_scene_depth": (i32, [vp, C.POINTER(u32)]),
         "apg_debug_fail_allocations": (i32, [vp, i32]),
+        "apg_debug_stream_read": (i32, [vp, C.c_uint64, vp, u32, vp]),
         "apg_debug_material": (i32, [C.POINTER(Material)] + [vp] * 3 + [u32] + [vp] * 6),
         "apg_debug_light_sampler": (i32, [vp, u32, u32, vp, vp, vp, u32, vp, vp, vp]),
         "apg_debug_light": (i32, [C.POINTER(Light), vp, vp, vp, u32] + [vp] * 7),

@@ -52,7 +52,12 @@ struct DeviceScene
     uint32_t wide_depth = 0;     // wide-node levels of the deepest root-to-leaf path
     float* tris_v = nullptr;     // 9 floats per triangle, tree order: the input vertices v0, v1, v2
                                  // (watertight test; gathered on first use after a build)
+    // allocated sizes (the GPU builder reuses the buffers while they are large enough)
+    uint32_t node_cap = 0, tri_cap = 0, tris_v_cap = 0;
 };
+
+struct BuildCache;  // build.hip: the GPU builder's device scratch, kept across builds
+void build_cache_free(BuildCache* c);
 
 // A stream with device and pinned host buffers for the host-memory entry points
 // (per-ray callers); the context keeps a pool, one per concurrently calling thread.
@@ -108,6 +113,7 @@ struct Context
     uint64_t* spill = nullptr;        // per persistent thread stack spill
     size_t spill_threads = 0;
     uint32_t* readback = nullptr;     // pinned host words the GPU builder reads level sizes into
+    BuildCache* build_cache = nullptr;
     uint32_t* overflow_host = nullptr; // pinned host word the overflow flag is read back into
     // After the context's traversals so far have finished (the caller waited for them):
     // APG_E_OVERFLOW if one of them overflowed its stack since the last check, and the

@@ -50,6 +50,7 @@ Context::~Context()
     if (noalloc_launch) (void)hipEventDestroy(noalloc_launch);
     if (render) render_state_free(render);
     if (guide) guide_free(guide);
+    if (build_cache) build_cache_free(build_cache);
 }
 
 static void free_scene(Context* ctx)

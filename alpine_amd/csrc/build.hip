@@ -112,6 +112,10 @@ struct BuildState
     uint32_t* bins;     // NODE_BIN_WORDS per candidate
     uint32_t* counters; // [0] node count, [1] bin slots, [2] tmp
     uint32_t* scan_tmp;
+    // the level being built: [0] first node, [1] end, [2] levels done, [3] listed large nodes
+    uint32_t* lvl;
+    uint32_t* large;      // the level's nodes of more than SMALL slots
+    uint32_t* large_off;  // exclusive scan of their chunk counts (total at [count])
 };
 
 // ---------------------------------------------------------------------------
@@ -189,48 +193,82 @@ __global__ void k_init_slots(uint32_t n, uint32_t* idx, uint32_t* snode, const u
     snode[i] = lo;
 }
 
-__global__ void k_init_nodes(uint32_t begin, uint32_t end, uint32_t* nbox, uint32_t* ncbox,
-                             uint32_t* nbinslot)
+
+// ---------------------------------------------------------------------------
+// binary levels
+// ---------------------------------------------------------------------------
+// Every kernel of a level reads the level's node range [lvl[0], lvl[1]) from device memory
+// and walks it with a grid-stride loop over a grid fixed for the build, so the host enqueues
+// level after level without knowing their widths; k_next_level moves on to the children.
+
+// Per node of the level: clear the bounds accumulators and list the nodes of more than SMALL
+// slots (their order in the list is arbitrary: what they accumulate is order independent).
+__global__ void k_level_init(BuildState S)
 {
-    const uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= end) return;
-    for (int a = 0; a < 3; ++a)
+    const uint32_t begin = S.lvl[0], end = S.lvl[1];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x; k < end; k += stride)
     {
-        nbox[6 * size_t(k) + a] = ord_of(FLT_MAX);
-        nbox[6 * size_t(k) + 3 + a] = ord_of(-FLT_MAX);
-        ncbox[6 * size_t(k) + a] = ord_of(FLT_MAX);
-        ncbox[6 * size_t(k) + 3 + a] = ord_of(-FLT_MAX);
+        for (int a = 0; a < 3; ++a)
+        {
+            S.nbox[6 * size_t(k) + a] = ord_of(FLT_MAX);
+            S.nbox[6 * size_t(k) + 3 + a] = ord_of(-FLT_MAX);
+            S.ncbox[6 * size_t(k) + a] = ord_of(FLT_MAX);
+            S.ncbox[6 * size_t(k) + 3 + a] = ord_of(-FLT_MAX);
+        }
+        S.nbinslot[k] = RETIRED;
+        if (S.ncount[k] > SMALL) S.large[atomicAdd(&S.lvl[3], 1u)] = k;
     }
-    nbinslot[k] = RETIRED;
 }
 
-// Per level, every node of more than SMALL slots is cut into chunks of CHUNK
-// slots of that node alone; chunk_off (exclusive scan of the chunk counts, the
-// total at [width]) maps a workgroup to its node and chunk.
-__global__ void k_large_chunks(BuildState S, uint32_t begin, uint32_t end, uint32_t* chunk_cnt)
+// One workgroup: large_off = exclusive scan of the listed nodes' chunk counts (total at
+// [count]); also clears the level's histogram-slot counter.
+constexpr uint32_t SCAN1 = 1024;
+__global__ __launch_bounds__(SCAN1) void k_large_scan(BuildState S)
 {
-    const uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= end) return;
-    const uint32_t cnt = S.ncount[k];
-    chunk_cnt[k - begin] = cnt > SMALL ? (cnt + CHUNK - 1) / CHUNK : 0u;
+    __shared__ uint32_t sh[SCAN1];
+    const uint32_t m = S.lvl[3];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < m; base += SCAN1)
+    {
+        const uint32_t j = base + threadIdx.x;
+        const uint32_t c = j < m ? (S.ncount[S.large[j]] + CHUNK - 1) / CHUNK : 0u;
+        sh[threadIdx.x] = c;
+        __syncthreads();
+        for (uint32_t off = 1; off < SCAN1; off <<= 1)
+        {
+            const uint32_t x = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
+            __syncthreads();
+            sh[threadIdx.x] += x;
+            __syncthreads();
+        }
+        if (j < m) S.large_off[j] = carry + sh[threadIdx.x] - c;
+        carry += sh[SCAN1 - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        S.large_off[m] = carry;
+        S.counters[1] = 0u;
+    }
 }
 
-// The node and slot range of chunk c (false past the last chunk).
-__device__ __forceinline__ bool chunk_range(const BuildState& S, uint32_t begin, uint32_t width,
-                                            const uint32_t* chunk_off, uint32_t c, uint32_t& node,
-                                            uint32_t& lo, uint32_t& hi)
+// The node and slot range of chunk c of the listed large nodes (false past the last chunk).
+__device__ __forceinline__ bool chunk_range(const BuildState& S, uint32_t c, uint32_t& node, uint32_t& lo,
+                                            uint32_t& hi)
 {
-    if (c >= chunk_off[width]) return false;
-    uint32_t a = 0, b = width;  // largest j with chunk_off[j] <= c
+    const uint32_t m = S.lvl[3];
+    if (c >= S.large_off[m]) return false;
+    uint32_t a = 0, b = m;  // largest j with large_off[j] <= c
     while (b - a > 1)
     {
-        const uint32_t m = (a + b) >> 1;
-        if (chunk_off[m] <= c) a = m;
-        else b = m;
+        const uint32_t mid = (a + b) >> 1;
+        if (S.large_off[mid] <= c) a = mid;
+        else b = mid;
     }
-    node = begin + a;
+    node = S.large[a];
     const uint32_t first = S.nfirst[node], cnt = S.ncount[node];
-    lo = first + (c - chunk_off[a]) * CHUNK;
+    lo = first + (c - S.large_off[a]) * CHUNK;
     hi = lo + CHUNK < first + cnt ? lo + CHUNK : first + cnt;
     return true;
 }
@@ -238,12 +276,11 @@ __device__ __forceinline__ bool chunk_range(const BuildState& S, uint32_t begin,
 // Bounds and centroid bounds of a large node's chunk: per-thread running
 // min / max as order-preserving uints (-0 < +0), reduced over the wave and the
 // workgroup, then one atomic per value into the node's entry.
-__global__ __launch_bounds__(BLOCK) void k_bounds_large(BuildState S, uint32_t begin, uint32_t width,
-                                                        const uint32_t* chunk_off)
+__global__ __launch_bounds__(BLOCK) void k_bounds_large(BuildState S)
 {
     __shared__ uint32_t part[BLOCK / 64u][12];
     uint32_t node, lo_i, hi_i;
-    if (!chunk_range(S, begin, width, chunk_off, blockIdx.x, node, lo_i, hi_i)) return;
+    if (!chunk_range(S, blockIdx.x, node, lo_i, hi_i)) return;
     const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
     uint32_t r[12];
     for (int a = 0; a < 3; ++a)
@@ -290,77 +327,82 @@ __device__ __forceinline__ float box_area(const float lo[3], const float hi[3])
     return (x * y + y * z + z * x) * 2.0f;
 }
 
-// Classify every node of the level.
-__global__ void k_decide(BuildState S, uint32_t begin, uint32_t end, uint32_t threshold)
+// Classify every node of the level.  The loop is wave-uniform (each wave takes 64
+// consecutive nodes per step) for the histogram-slot ballot.
+__global__ void k_decide(BuildState S, uint32_t threshold)
 {
-    const uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    bool cand = false;
-    uint32_t cnt = 0;
-    if (k < end)
+    const uint32_t begin = S.lvl[0], end = S.lvl[1];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k0 = begin + blockIdx.x * blockDim.x + (threadIdx.x & ~63u); k0 < end; k0 += stride)
     {
-        cnt = S.ncount[k];
-        uint32_t fl = S.nflags[k];
-        if (fl & NF_FORCED)
+        const uint32_t k = k0 + lane;
+        bool cand = false;
+        uint32_t cnt = 0;
+        if (k < end)
         {
-            if (cnt <= APG_MAX_LEAF) fl |= NF_LEAF;
-            else fl |= NF_SPLIT;
-            S.nflags[k] = fl;
-        }
-        else
-        {
-            bool leaf = cnt <= threshold;
-            if (!leaf)
+            cnt = S.ncount[k];
+            uint32_t fl = S.nflags[k];
+            if (fl & NF_FORCED)
             {
-                float clo[3], chi[3];
-                for (int a = 0; a < 3; ++a)
-                {
-                    clo[a] = float_of(S.ncbox[6 * size_t(k) + a]);
-                    chi[a] = float_of(S.ncbox[6 * size_t(k) + 3 + a]);
-                }
-                const float diag = max_s(chi[0] - clo[0], max_s(chi[1] - clo[1], chi[2] - clo[2]));
-                if (diag == 0.0f) leaf = true;
-            }
-            if (leaf)
-            {
-                fl |= NF_SAH_LEAF;
-                fl |= cnt <= APG_MAX_LEAF ? NF_LEAF : (NF_FORCED | NF_SPLIT);
+                if (cnt <= APG_MAX_LEAF) fl |= NF_LEAF;
+                else fl |= NF_SPLIT;
                 S.nflags[k] = fl;
             }
-            else if (cnt <= TINY)
-                S.nbinslot[k] = TINY_SLOT;
             else
-                cand = true;
+            {
+                bool leaf = cnt <= threshold;
+                if (!leaf)
+                {
+                    float clo[3], chi[3];
+                    for (int a = 0; a < 3; ++a)
+                    {
+                        clo[a] = float_of(S.ncbox[6 * size_t(k) + a]);
+                        chi[a] = float_of(S.ncbox[6 * size_t(k) + 3 + a]);
+                    }
+                    const float diag = max_s(chi[0] - clo[0], max_s(chi[1] - clo[1], chi[2] - clo[2]));
+                    if (diag == 0.0f) leaf = true;
+                }
+                if (leaf)
+                {
+                    fl |= NF_SAH_LEAF;
+                    fl |= cnt <= APG_MAX_LEAF ? NF_LEAF : (NF_FORCED | NF_SPLIT);
+                    S.nflags[k] = fl;
+                }
+                else if (cnt <= TINY)
+                    S.nbinslot[k] = TINY_SLOT;
+                else
+                    cand = true;
+            }
         }
-    }
-    // histogram slots for the SAH candidates: one counter atomic per wave
-    const uint64_t m = __ballot(cand);
-    if (m == 0ull) return;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t leader = __ffsll((unsigned long long)m) - 1u;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&S.counters[1], uint32_t(__popcll(m)));
-    base = uint32_t(__shfl(int(base), int(leader)));
-    if (!cand) return;
-    const uint32_t slot = base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
-    S.nbinslot[k] = slot;
-    if (cnt <= SMALL) return;  // k_bin_small writes the whole histogram
-    uint32_t* b = S.bins + size_t(slot) * NODE_BIN_WORDS;
-    for (uint32_t j = 0; j < 3 * NBINS; ++j)
-    {
-        uint32_t* e = b + j * BIN_VALS;
-        for (int a = 0; a < 3; ++a) { e[a] = ord_of(FLT_MAX); e[3 + a] = ord_of(-FLT_MAX); }
-        e[6] = 0;
+        // histogram slots for the SAH candidates: one counter atomic per wave
+        const uint64_t m = __ballot(cand);
+        if (m == 0ull) continue;
+        const uint32_t leader = __ffsll((unsigned long long)m) - 1u;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&S.counters[1], uint32_t(__popcll(m)));
+        base = uint32_t(__shfl(int(base), int(leader)));
+        if (!cand) continue;
+        const uint32_t slot = base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+        S.nbinslot[k] = slot;
+        if (cnt <= SMALL) continue;  // k_bin_small writes the whole histogram
+        uint32_t* b = S.bins + size_t(slot) * NODE_BIN_WORDS;
+        for (uint32_t j = 0; j < 3 * NBINS; ++j)
+        {
+            uint32_t* e = b + j * BIN_VALS;
+            for (int a = 0; a < 3; ++a) { e[a] = ord_of(FLT_MAX); e[3 + a] = ord_of(-FLT_MAX); }
+            e[6] = 0;
+        }
     }
 }
 
 // 9-bin histograms (box + count) per axis of a large SAH candidate's chunk: one
 // LDS histogram per wave, merged into the node's global histogram.
-__global__ __launch_bounds__(BLOCK) void k_bin_large(BuildState S, uint32_t begin, uint32_t width,
-                                                     const uint32_t* chunk_off)
+__global__ __launch_bounds__(BLOCK) void k_bin_large(BuildState S)
 {
     __shared__ uint32_t hist[BLOCK / 64u][NODE_BIN_WORDS];
     uint32_t node, lo_i, hi_i;
-    if (!chunk_range(S, begin, width, chunk_off, blockIdx.x, node, lo_i, hi_i)) return;
+    if (!chunk_range(S, blockIdx.x, node, lo_i, hi_i)) return;
     const uint32_t slot = S.nbinslot[node];
     if (slot == RETIRED) return;
     for (uint32_t j = threadIdx.x; j < (BLOCK / 64u) * NODE_BIN_WORDS; j += BLOCK)
@@ -410,131 +452,145 @@ __global__ __launch_bounds__(BLOCK) void k_bin_large(BuildState S, uint32_t begi
 
 // Bounds of a node of at most SMALL slots: one wave walks the node's slot range
 // and reduces across lanes; the result is stored, not accumulated.
-__global__ __launch_bounds__(BLOCK) void k_bounds_small(BuildState S, uint32_t begin, uint32_t end)
+__global__ __launch_bounds__(BLOCK) void k_bounds_small(BuildState S)
 {
-    const uint32_t k = begin + (blockIdx.x * BLOCK + threadIdx.x) / 64u;
+    const uint32_t begin = S.lvl[0], end = S.lvl[1];
     const uint32_t lane = threadIdx.x & 63u;
-    if (k >= end) return;
-    const uint32_t cnt = S.ncount[k];
-    if (cnt <= TINY || cnt > SMALL) return;
-    const uint32_t first = S.nfirst[k];
-    const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
-    uint32_t r[12];
-    for (int a = 0; a < 3; ++a)
+    const uint32_t waves = gridDim.x * (BLOCK / 64u);
+    for (uint32_t k = begin + (blockIdx.x * BLOCK + threadIdx.x) / 64u; k < end; k += waves)
     {
-        r[a] = lo_ord; r[3 + a] = hi_ord; r[6 + a] = lo_ord; r[9 + a] = hi_ord;
-    }
-    for (uint32_t i = first + lane; i < first + cnt; i += 64u)
-    {
-        const float* b = S.pbox + 6 * size_t(S.idx[i]);
+        const uint32_t cnt = S.ncount[k];
+        if (cnt <= TINY || cnt > SMALL) continue;
+        const uint32_t first = S.nfirst[k];
+        const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
+        uint32_t r[12];
         for (int a = 0; a < 3; ++a)
         {
-            const float lo = b[a], hi = b[3 + a], c = (lo + hi) * 0.5f;
-            r[a] = min(r[a], ord_of(lo));
-            r[3 + a] = max(r[3 + a], ord_of(hi));
-            r[6 + a] = min(r[6 + a], ord_of(c));
-            r[9 + a] = max(r[9 + a], ord_of(c));
+            r[a] = lo_ord; r[3 + a] = hi_ord; r[6 + a] = lo_ord; r[9 + a] = hi_ord;
         }
+        for (uint32_t i = first + lane; i < first + cnt; i += 64u)
+        {
+            const float* b = S.pbox + 6 * size_t(S.idx[i]);
+            for (int a = 0; a < 3; ++a)
+            {
+                const float lo = b[a], hi = b[3 + a], c = (lo + hi) * 0.5f;
+                r[a] = min(r[a], ord_of(lo));
+                r[3 + a] = max(r[3 + a], ord_of(hi));
+                r[6 + a] = min(r[6 + a], ord_of(c));
+                r[9 + a] = max(r[9 + a], ord_of(c));
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1)
+            for (int a = 0; a < 3; ++a)
+            {
+                r[a] = min(r[a], uint32_t(__shfl_xor(int(r[a]), off)));
+                r[3 + a] = max(r[3 + a], uint32_t(__shfl_xor(int(r[3 + a]), off)));
+                r[6 + a] = min(r[6 + a], uint32_t(__shfl_xor(int(r[6 + a]), off)));
+                r[9 + a] = max(r[9 + a], uint32_t(__shfl_xor(int(r[9 + a]), off)));
+            }
+        if (lane == 0u)
+            for (int j = 0; j < 6; ++j)
+            {
+                S.nbox[6 * size_t(k) + j] = r[j];
+                S.ncbox[6 * size_t(k) + j] = r[6 + j];
+            }
     }
-    for (int off = 32; off > 0; off >>= 1)
+}
+
+// Bounds of a node of at most TINY slots: one thread, a serial loop.
+__global__ void k_bounds_tiny(BuildState S)
+{
+    const uint32_t begin = S.lvl[0], end = S.lvl[1];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x; k < end; k += stride)
+    {
+        const uint32_t cnt = S.ncount[k];
+        if (cnt == 0u || cnt > TINY) continue;
+        const uint32_t first = S.nfirst[k];
+        const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
+        uint32_t r[12];
         for (int a = 0; a < 3; ++a)
         {
-            r[a] = min(r[a], uint32_t(__shfl_xor(int(r[a]), off)));
-            r[3 + a] = max(r[3 + a], uint32_t(__shfl_xor(int(r[3 + a]), off)));
-            r[6 + a] = min(r[6 + a], uint32_t(__shfl_xor(int(r[6 + a]), off)));
-            r[9 + a] = max(r[9 + a], uint32_t(__shfl_xor(int(r[9 + a]), off)));
+            r[a] = lo_ord; r[3 + a] = hi_ord; r[6 + a] = lo_ord; r[9 + a] = hi_ord;
         }
-    if (lane == 0u)
+        for (uint32_t i = first; i < first + cnt; ++i)
+        {
+            const float* b = S.pbox + 6 * size_t(S.idx[i]);
+            for (int a = 0; a < 3; ++a)
+            {
+                const float lo = b[a], hi = b[3 + a], c = (lo + hi) * 0.5f;
+                r[a] = min(r[a], ord_of(lo));
+                r[3 + a] = max(r[3 + a], ord_of(hi));
+                r[6 + a] = min(r[6 + a], ord_of(c));
+                r[9 + a] = max(r[9 + a], ord_of(c));
+            }
+        }
         for (int j = 0; j < 6; ++j)
         {
             S.nbox[6 * size_t(k) + j] = r[j];
             S.ncbox[6 * size_t(k) + j] = r[6 + j];
         }
-}
-
-// Bounds of a node of at most TINY slots: one thread, a serial loop.
-__global__ void k_bounds_tiny(BuildState S, uint32_t begin, uint32_t end)
-{
-    const uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= end) return;
-    const uint32_t cnt = S.ncount[k];
-    if (cnt == 0u || cnt > TINY) return;
-    const uint32_t first = S.nfirst[k];
-    const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
-    uint32_t r[12];
-    for (int a = 0; a < 3; ++a)
-    {
-        r[a] = lo_ord; r[3 + a] = hi_ord; r[6 + a] = lo_ord; r[9 + a] = hi_ord;
-    }
-    for (uint32_t i = first; i < first + cnt; ++i)
-    {
-        const float* b = S.pbox + 6 * size_t(S.idx[i]);
-        for (int a = 0; a < 3; ++a)
-        {
-            const float lo = b[a], hi = b[3 + a], c = (lo + hi) * 0.5f;
-            r[a] = min(r[a], ord_of(lo));
-            r[3 + a] = max(r[3 + a], ord_of(hi));
-            r[6 + a] = min(r[6 + a], ord_of(c));
-            r[9 + a] = max(r[9 + a], ord_of(c));
-        }
-    }
-    for (int j = 0; j < 6; ++j)
-    {
-        S.nbox[6 * size_t(k) + j] = r[j];
-        S.ncbox[6 * size_t(k) + j] = r[6 + j];
     }
 }
 
 // Bins of a node of at most SMALL slots: one wave, a histogram of its own in LDS,
 // written out whole.
-__global__ __launch_bounds__(BLOCK) void k_bin_small(BuildState S, uint32_t begin, uint32_t end)
+__global__ __launch_bounds__(BLOCK) void k_bin_small(BuildState S)
 {
     __shared__ uint32_t hist[BLOCK / 64u][NODE_BIN_WORDS];
-    const uint32_t k = begin + (blockIdx.x * BLOCK + threadIdx.x) / 64u;
+    const uint32_t begin = S.lvl[0], end = S.lvl[1];
     const uint32_t lane = threadIdx.x & 63u;
-    if (k >= end) return;
-    const uint32_t cnt = S.ncount[k];
-    if (cnt == 0u || cnt > SMALL) return;
-    const uint32_t slot = S.nbinslot[k];
-    if (slot == RETIRED || slot == TINY_SLOT) return;
+    const uint32_t waves = gridDim.x * (BLOCK / 64u);
     uint32_t* acc = hist[threadIdx.x / 64u];
-    for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u)
+    for (uint32_t k = begin + (blockIdx.x * BLOCK + threadIdx.x) / 64u; k < end; k += waves)
     {
-        const uint32_t v = j % BIN_VALS;
-        acc[j] = v < 3 ? ord_of(FLT_MAX) : (v < 6 ? ord_of(-FLT_MAX) : 0u);
-    }
-    float mn[3], mx[3];
-    for (int d = 0; d < 3; ++d)
-    {
-        mn[d] = float_of(S.ncbox[6 * size_t(k) + d]);
-        mx[d] = float_of(S.ncbox[6 * size_t(k) + 3 + d]);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    const uint32_t first = S.nfirst[k];
-    for (uint32_t i = first + lane; i < first + cnt; i += 64u)
-    {
-        const float* b = S.pbox + 6 * size_t(S.idx[i]);
-        float lo[3], hi[3];
-        for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; }
+        const uint32_t cnt = S.ncount[k];
+        if (cnt == 0u || cnt > SMALL) continue;
+        const uint32_t slot = S.nbinslot[k];
+        if (slot == RETIRED || slot == TINY_SLOT) continue;
+        for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u)
+        {
+            const uint32_t v = j % BIN_VALS;
+            acc[j] = v < 3 ? ord_of(FLT_MAX) : (v < 6 ? ord_of(-FLT_MAX) : 0u);
+        }
+        float mn[3], mx[3];
         for (int d = 0; d < 3; ++d)
         {
-            if (mn[d] == mx[d]) continue;
-            const float c = (lo[d] + hi[d]) * 0.5f;
-            uint32_t* e = acc + (d * NBINS + bin_index(c, mn[d], mx[d])) * BIN_VALS;
-            for (int a = 0; a < 3; ++a)
-            {
-                atomicMin(&e[a], ord_of(lo[a]));
-                atomicMax(&e[3 + a], ord_of(hi[a]));
-            }
-            atomicAdd(&e[6], 1u);
+            mn[d] = float_of(S.ncbox[6 * size_t(k) + d]);
+            mx[d] = float_of(S.ncbox[6 * size_t(k) + 3 + d]);
         }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t first = S.nfirst[k];
+        for (uint32_t i = first + lane; i < first + cnt; i += 64u)
+        {
+            const float* b = S.pbox + 6 * size_t(S.idx[i]);
+            float lo[3], hi[3];
+            for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; }
+            for (int d = 0; d < 3; ++d)
+            {
+                if (mn[d] == mx[d]) continue;
+                const float c = (lo[d] + hi[d]) * 0.5f;
+                uint32_t* e = acc + (d * NBINS + bin_index(c, mn[d], mx[d])) * BIN_VALS;
+                for (int a = 0; a < 3; ++a)
+                {
+                    atomicMin(&e[a], ord_of(lo[a]));
+                    atomicMax(&e[3 + a], ord_of(hi[a]));
+                }
+                atomicAdd(&e[6], 1u);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        uint32_t* g = S.bins + size_t(slot) * NODE_BIN_WORDS;
+        for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u) g[j] = acc[j];
+        // the next node's clear of acc must follow every lane's read above
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    uint32_t* g = S.bins + size_t(slot) * NODE_BIN_WORDS;
-    for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u) g[j] = acc[j];
 }
+
+
 
 // An SAH leaf; one of more than APG_MAX_LEAF triangles is split positionally
 // into halves rounded to whole leaves.
@@ -658,12 +714,10 @@ __device__ void sweep_tiny(BuildState& S, uint32_t k, uint32_t fl, uint32_t cnt)
     split_node(S, k, fl, cnt, best_below, best_dim, best_idx, best_mn, best_mx);
 }
 
-// Sweep, split choice and child allocation for the SAH candidates; positional
-// splits for forced nodes.
-__global__ void k_sweep(BuildState S, uint32_t begin, uint32_t end)
+// Sweep, split choice and child allocation of one node (SAH candidates; positional
+// splits for forced nodes).
+__device__ void sweep_node(BuildState& S, uint32_t k)
 {
-    const uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= end) return;
     uint32_t fl = S.nflags[k];
     const uint32_t cnt = S.ncount[k];
     if (fl & NF_SPLIT)
@@ -749,11 +803,19 @@ __global__ void k_sweep(BuildState S, uint32_t begin, uint32_t end)
     split_node(S, k, fl, cnt, nbelow, best_dim, best_idx, best_mn, best_mx);
 }
 
+__global__ void k_sweep(BuildState S)
+{
+    const uint32_t begin = S.lvl[0], end = S.lvl[1];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x; k < end; k += stride) sweep_node(S, k);
+}
+
 // Partition flags: 1 = goes to the left child.
-__global__ void k_flags(BuildState S, uint32_t n, uint32_t begin, uint32_t end)
+__global__ void k_flags(BuildState S, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const uint32_t begin = S.lvl[0], end = S.lvl[1];
     const uint32_t nd = S.snode[i];
     uint32_t f = 0;
     if (nd != RETIRED && nd >= begin && nd < end && (S.nflags[nd] & NF_SPLIT))
@@ -772,10 +834,11 @@ __global__ void k_flags(BuildState S, uint32_t n, uint32_t begin, uint32_t end)
     S.flag[i] = f;
 }
 
-__global__ void k_scatter(BuildState S, uint32_t n, uint32_t begin, uint32_t end)
+__global__ void k_scatter(BuildState S, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const uint32_t begin = S.lvl[0], end = S.lvl[1];
     const uint32_t nd = S.snode[i];
     if (nd == RETIRED || nd < begin || nd >= end || !(S.nflags[nd] & NF_SPLIT))
     {
@@ -795,15 +858,49 @@ __global__ void k_scatter(BuildState S, uint32_t n, uint32_t begin, uint32_t end
 }
 
 // First slot of every child allocated in this level.
-__global__ void k_child_first(BuildState S, uint32_t begin, uint32_t end)
+__global__ void k_child_first(BuildState S)
 {
-    const uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= end) return;
-    if (!(S.nflags[k] & NF_SPLIT)) return;
-    const uint32_t l = S.nleft[k];
-    S.nfirst[l] = S.nfirst[k];
-    S.nfirst[l + 1] = S.nfirst[k] + S.ncount[l];
+    const uint32_t begin = S.lvl[0], end = S.lvl[1];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x; k < end; k += stride)
+    {
+        if (!(S.nflags[k] & NF_SPLIT)) continue;
+        const uint32_t l = S.nleft[k];
+        S.nfirst[l] = S.nfirst[k];
+        S.nfirst[l + 1] = S.nfirst[k] + S.ncount[l];
+    }
 }
+
+// The next level is the children allocated by this one: [end, node count).
+__global__ void k_next_level(BuildState S)
+{
+    S.lvl[0] = S.lvl[1];
+    S.lvl[1] = S.counters[0];
+    S.lvl[2] += 1u;
+    S.lvl[3] = 0u;
+}
+
+// Roots: the first slot and slot count of each tree, the level control words and counters.
+__global__ void k_init_roots(BuildState S, const uint32_t* rfirst, uint32_t R)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R)
+    {
+        S.nfirst[r] = rfirst[r];
+        S.ncount[r] = rfirst[r + 1] - rfirst[r];
+        S.nflags[r] = 0u;
+    }
+    if (r == 0)
+    {
+        S.lvl[0] = 0u;
+        S.lvl[1] = R;
+        S.lvl[2] = 0u;
+        S.lvl[3] = 0u;
+        S.counters[0] = R;
+        S.counters[1] = 0u;
+    }
+}
+
 
 // ---------------------------------------------------------------------------
 // exclusive scan (3 passes, recursive on the block sums)
@@ -894,6 +991,11 @@ size_t scan_tmp_words(uint32_t n)
 // ---------------------------------------------------------------------------
 // wide collapse
 // ---------------------------------------------------------------------------
+// Breadth first over wide levels, like the binary levels: the level's wide nodes
+// [wl[0], wl[1]) are read from device memory.  Per-node counts are kept by absolute wide id
+// and are zero outside the level, so exclusive scans over the whole capacity give the level's
+// child and triangle offsets in wide-node order (deterministic) without the host knowing the
+// level's width.
 struct WideState
 {
     const uint32_t* nbox;   // ordered uints
@@ -903,16 +1005,18 @@ struct WideState
     const uint32_t* nflags;
     const uint32_t* idx;    // tree order -> triangle
     const float* tri9_src;
-    uint32_t* wbin;         // wide node -> binary node (per level, indexed by wide id)
-    uint32_t* wchild;       // 8 per wide node at this level (binary ids, slot order, RETIRED = empty)
-    uint32_t* n_inner;      // per wide node of the level
-    uint32_t* n_tris;
-    uint32_t* s_inner;      // scans (n + 1)
+    uint32_t* wbin;         // wide node -> binary node
+    uint32_t* wchild;       // 8 per wide node (binary ids, slot order, RETIRED = empty)
+    uint32_t* n_inner;      // per wide node: inner children (0 outside the level)
+    uint32_t* n_tris;       // per wide node: triangles of its leaf slots (0 outside the level)
+    uint32_t* s_inner;      // exclusive scans over the capacity (wcap + 1)
     uint32_t* s_tris;
     ApgWideNode* nodes;
     float* tri9;
     uint32_t* tri_ids;
-    float slack;
+    const float* slack;     // device: the quantisation slack
+    uint32_t* wl;           // [0] level begin, [1] level end, [2] triangles placed, [3] levels, [4] error
+    uint32_t wcap;
 };
 
 __device__ __forceinline__ bool is_leaf_node(const WideState& W, uint32_t b)
@@ -929,10 +1033,8 @@ __device__ __forceinline__ void node_box(const WideState& W, uint32_t b, float l
     }
 }
 
-__global__ void k_wide_gather(WideState W, uint32_t wbegin, uint32_t wend)
+__device__ void wide_gather(const WideState& W, uint32_t w)
 {
-    const uint32_t w = wbegin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= wend) return;
     const uint32_t root = W.wbin[w];
     uint32_t ch[APG_WIDE];
     uint32_t nch = 0;
@@ -997,13 +1099,20 @@ __global__ void k_wide_gather(WideState W, uint32_t wbegin, uint32_t wend)
     for (uint32_t s = 0; s < APG_WIDE; ++s)
     {
         const uint32_t c = slot_child[s];
-        W.wchild[8 * size_t(w - wbegin) + s] = c;
+        W.wchild[8 * size_t(w) + s] = c;
         if (c == RETIRED) continue;
         if (is_leaf_node(W, c)) nt += W.ncount[c];
         else ni++;
     }
-    W.n_inner[w - wbegin] = ni;
-    W.n_tris[w - wbegin] = nt;
+    W.n_inner[w] = ni;
+    W.n_tris[w] = nt;
+}
+
+__global__ void k_wide_gather(WideState W)
+{
+    const uint32_t begin = W.wl[0], end = W.wl[1];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t w = begin + blockIdx.x * blockDim.x + threadIdx.x; w < end; w += stride) wide_gather(W, w);
 }
 
 __device__ uint32_t quantise_axis(float lo_f, float hi_f, float slack, const float* clo,
@@ -1040,21 +1149,18 @@ __device__ uint32_t quantise_axis(float lo_f, float hi_f, float slack, const flo
     }
 }
 
-__global__ void k_wide_emit(WideState W, uint32_t wbegin, uint32_t wend, uint32_t child_begin,
-                            uint32_t tri_begin)
+
+__device__ void wide_emit(const WideState& W, uint32_t w, uint32_t level_end, uint32_t tri_begin, float slack)
 {
-    const uint32_t w = wbegin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= wend) return;
-    const uint32_t lw = w - wbegin;
     ApgWideNode nd;
-    const uint32_t child_base = child_begin + W.s_inner[lw];
-    const uint32_t tri_base = tri_begin + W.s_tris[lw];
+    const uint32_t child_base = level_end + W.s_inner[w];
+    const uint32_t tri_base = tri_begin + W.s_tris[w];
     uint32_t imask = 0, valid = 0, cmask = 0, inner_rank = 0, toff = 0;
     float clo[3][APG_WIDE], chi[3][APG_WIDE];
     float nlo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, nhi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
     for (uint32_t s = 0; s < APG_WIDE; ++s)
     {
-        const uint32_t c = W.wchild[8 * size_t(lw) + s];
+        const uint32_t c = W.wchild[8 * size_t(w) + s];
         for (int a = 0; a < 3; ++a) { clo[a][s] = 0.0f; chi[a][s] = 0.0f; }
         if (c == RETIRED) continue;
         valid |= 1u << s;
@@ -1084,13 +1190,15 @@ __global__ void k_wide_emit(WideState W, uint32_t wbegin, uint32_t wend, uint32_
         else
         {
             imask |= 1u << s;
-            W.wbin[child_base + inner_rank] = c;
+            const uint32_t cw = child_base + inner_rank;
+            if (cw < W.wcap) W.wbin[cw] = c;
+            else W.wl[4] = 1u;  // capacity exceeded: reported, nothing written
             inner_rank++;
         }
     }
     uint32_t eb[3], qlo[3][APG_WIDE], qhi[3][APG_WIDE];
     for (int a = 0; a < 3; ++a)
-        eb[a] = quantise_axis(nlo[a], nhi[a], W.slack, clo[a], chi[a], valid, nd.origin[a], qlo[a], qhi[a]);
+        eb[a] = quantise_axis(nlo[a], nhi[a], slack, clo[a], chi[a], valid, nd.origin[a], qlo[a], qhi[a]);
     nd.exp_imask = eb[0] | (eb[1] << 8) | (eb[2] << 16) | (imask << 24);
     nd.child_base = child_base;
     nd.tri_base = tri_base;
@@ -1103,6 +1211,52 @@ __global__ void k_wide_emit(WideState W, uint32_t wbegin, uint32_t wend, uint32_
             apg_put_plane(nd.q, s, 3 + a, qhi[a][s]);
         }
     W.nodes[w] = nd;
+    // the scans of the next level must see zero outside that level
+    W.n_inner[w] = 0u;
+    W.n_tris[w] = 0u;
+}
+
+__global__ void k_wide_emit(WideState W)
+{
+    const uint32_t begin = W.wl[0], end = W.wl[1], tri_begin = W.wl[2];
+    const float slack = *W.slack;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t w = begin + blockIdx.x * blockDim.x + threadIdx.x; w < end; w += stride)
+        wide_emit(W, w, end, tri_begin, slack);
+}
+
+// The next wide level is the inner children placed by this one.
+__global__ void k_next_wide(WideState W)
+{
+    const uint32_t begin = W.wl[0], end = W.wl[1];
+    if (begin == end) return;  // past the last level: nothing placed, not a level
+    const uint32_t inner = W.s_inner[W.wcap], tris = W.s_tris[W.wcap];
+    const bool fits = end + inner <= W.wcap;
+    W.wl[0] = end;
+    W.wl[1] = fits ? end + inner : end;  // beyond the capacity: stop, with the error set
+    if (!fits) W.wl[4] = 1u;
+    W.wl[2] += tris;
+    W.wl[3] += 1u;
+}
+
+// Wide level 0: the roots' images (wide node r is the root of tree r).
+__global__ void k_init_wide(WideState W, uint32_t R)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R) W.wbin[r] = r;
+    if (r == 0)
+    {
+        W.wl[0] = 0u;
+        W.wl[1] = R;
+        W.wl[2] = 0u;
+        W.wl[3] = 0u;
+        W.wl[4] = 0u;
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
+{
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
 }
 
 __global__ void k_scene_slack(const float* pbox, uint32_t n, uint32_t* out)
@@ -1118,6 +1272,15 @@ __global__ void k_scene_slack(const float* pbox, uint32_t n, uint32_t* out)
     if (threadIdx.x == 0) atomicMax(out, sh);
 }
 
+// The quantisation slack: 2^-17 x the largest absolute coordinate (1 for an all-zero scene).
+__global__ void k_slack_final(const uint32_t* m_bits, float* slack)
+{
+    float mf = __uint_as_float(*m_bits);
+    if (mf == 0.0f) mf = 1.0f;
+    *slack = ldexpf(mf, -17);
+}
+
+
 template <typename T>
 int dalloc(Context* ctx, T*& p, size_t count)
 {
@@ -1127,13 +1290,36 @@ int dalloc(Context* ctx, T*& p, size_t count)
     return APG_OK;
 }
 
-struct Scratch
+inline uint32_t blocks_for(uint32_t n, uint32_t b) { return n ? (n + b - 1) / b : 1; }
+
+} // namespace
+
+// Device scratch of the GPU builder, kept by the context across builds and reallocated only
+// when a build needs more (per-frame rebuilds allocate nothing).
+struct BuildCache
 {
+    uint32_t n_cap = 0, r_cap = 0;
     std::vector<void*> ptrs;
-    ~Scratch()
+    BuildState S{};
+    uint32_t* rfirst = nullptr;      // R + 1: first triangle of each tree
+    uint32_t* slack_bits = nullptr;  // largest |coordinate| as float bits
+    float* slack = nullptr;
+    // wide collapse, capacity n + R + 2 wide nodes
+    uint32_t wide_cap = 0;
+    uint32_t *wbin = nullptr, *wchild = nullptr, *n_inner = nullptr, *n_tris = nullptr;
+    uint32_t *s_inner = nullptr, *s_tris = nullptr, *wscan_tmp = nullptr, *wl = nullptr;
+    uint32_t* rb = nullptr;          // pinned read-back words
+
+    void release()
     {
         for (void* p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+        if (rb) (void)hipHostFree(rb);
+        rb = nullptr;
+        n_cap = r_cap = wide_cap = 0;
     }
+    ~BuildCache() { release(); }
+
     template <typename T>
     int get(Context* ctx, T*& p, size_t count)
     {
@@ -1141,15 +1327,85 @@ struct Scratch
         if (rc == APG_OK) ptrs.push_back(p);
         return rc;
     }
-    // p now belongs to the caller: it is not freed with the scratch
-    void release(const void* p)
+
+    int ensure(Context* ctx, uint32_t n, uint32_t R)
     {
-        for (void*& q : ptrs)
-            if (q == p) q = nullptr;
+        if (n <= n_cap && R <= r_cap) return APG_OK;
+        release();
+        int rc = ensure_sized(ctx, n, R);
+        if (rc) release();
+        return rc;
+    }
+
+    int ensure_sized(Context* ctx, uint32_t n, uint32_t R)
+    {
+        const size_t ncap = 2 * size_t(n) + 2 + R;
+        // histogram slots: only nodes of more than TINY slots take one, and a level's nodes
+        // own disjoint slots
+        const size_t bincap = size_t(n) / (TINY + 1) + 2;
+        const size_t lcap = size_t(n) / SMALL + 2;  // nodes of more than SMALL slots per level
+        const size_t wcap = size_t(n) + R + 2;      // >= binary nodes / 2 + 2 + R
+        int rc;
+        if ((rc = get(ctx, S.pbox, 6 * size_t(n))) || (rc = get(ctx, S.tri9_src, 9 * size_t(n))) ||
+            (rc = get(ctx, S.idx, n)) || (rc = get(ctx, S.idx_tmp, n)) || (rc = get(ctx, S.snode, n)) ||
+            (rc = get(ctx, S.snode_tmp, n)) || (rc = get(ctx, S.flag, n)) || (rc = get(ctx, S.scan, size_t(n) + 1)) ||
+            (rc = get(ctx, S.nbox, 6 * ncap)) || (rc = get(ctx, S.ncbox, 6 * ncap)) || (rc = get(ctx, S.nfirst, ncap)) ||
+            (rc = get(ctx, S.ncount, ncap)) || (rc = get(ctx, S.nleft, ncap)) || (rc = get(ctx, S.nflags, ncap)) ||
+            (rc = get(ctx, S.nsplit, ncap)) || (rc = get(ctx, S.nsplit_mm, 2 * ncap)) ||
+            (rc = get(ctx, S.nbinslot, ncap)) || (rc = get(ctx, S.bins, bincap * NODE_BIN_WORDS)) ||
+            (rc = get(ctx, S.counters, 16)) || (rc = get(ctx, S.scan_tmp, scan_tmp_words(n + 1))) ||
+            (rc = get(ctx, S.lvl, 16)) || (rc = get(ctx, S.large, lcap)) || (rc = get(ctx, S.large_off, lcap + 1)) ||
+            (rc = get(ctx, rfirst, size_t(R) + 1)) || (rc = get(ctx, slack_bits, 1)) || (rc = get(ctx, slack, 1)) ||
+            (rc = get(ctx, wbin, 2 * wcap + 8)) || (rc = get(ctx, wchild, 8 * wcap)) || (rc = get(ctx, n_inner, wcap)) ||
+            (rc = get(ctx, n_tris, wcap)) || (rc = get(ctx, s_inner, wcap + 1)) || (rc = get(ctx, s_tris, wcap + 1)) ||
+            (rc = get(ctx, wscan_tmp, scan_tmp_words(uint32_t(wcap) + 1))) || (rc = get(ctx, wl, 16)))
+            return rc;
+        APG_HIP_CHECK(ctx, hipHostMalloc(&rb, 64 * sizeof(uint32_t)));
+        n_cap = n;
+        r_cap = R;
+        wide_cap = uint32_t(wcap);
+        return APG_OK;
     }
 };
 
-inline uint32_t blocks_for(uint32_t n, uint32_t b) { return n ? (n + b - 1) / b : 1; }
+void build_cache_free(BuildCache* c) { delete c; }
+
+namespace {
+
+// The scene's node and triangle buffers, reused across builds while they are large enough.
+int ensure_scene_buffers(Context* ctx, uint32_t nodes, uint32_t tris)
+{
+    DeviceScene& sc = ctx->scene;
+    if (sc.nodes && sc.node_cap >= nodes && sc.tri_cap >= tris) return APG_OK;
+    if (sc.nodes) (void)hipFree(sc.nodes);
+    if (sc.tris) (void)hipFree(sc.tris);
+    if (sc.tri_ids) (void)hipFree(sc.tri_ids);
+    sc.nodes = nullptr;
+    sc.tris = nullptr;
+    sc.tri_ids = nullptr;
+    sc.node_cap = sc.tri_cap = 0;
+    int rc;
+    if ((rc = dalloc(ctx, sc.nodes, nodes)) || (rc = dalloc(ctx, sc.tris, 9 * size_t(tris))) ||
+        (rc = dalloc(ctx, sc.tri_ids, tris)))
+    {
+        if (sc.nodes) (void)hipFree(sc.nodes);
+        if (sc.tris) (void)hipFree(sc.tris);
+        sc.nodes = nullptr;
+        sc.tris = nullptr;
+        sc.tri_ids = nullptr;
+        return rc;
+    }
+    sc.node_cap = nodes;
+    sc.tri_cap = tris;
+    return APG_OK;
+}
+
+uint32_t ceil_log2(uint32_t n)
+{
+    uint32_t l = 0;
+    while ((1ull << l) < n) ++l;
+    return l;
+}
 
 } // namespace
 
@@ -1158,181 +1414,127 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
     const auto t_start = std::chrono::steady_clock::now();
     const uint32_t n = ctx->total_tris;
     hipStream_t s = nullptr;
-    // free the previous scene
-    if (ctx->scene.nodes) (void)hipFree(ctx->scene.nodes);
-    if (ctx->scene.tris) (void)hipFree(ctx->scene.tris);
-    if (ctx->scene.tri_ids) (void)hipFree(ctx->scene.tri_ids);
-    if (ctx->scene.tris_v) (void)hipFree(ctx->scene.tris_v);
-    ctx->scene = DeviceScene{};
+    // the previous scene's buffers stay allocated for this build to reuse
+    DeviceScene& sc = ctx->scene;
+    if (sc.tris_v) (void)hipFree(sc.tris_v);
+    sc.tris_v = nullptr;
+    sc.node_count = sc.tri_count = sc.wide_depth = 0;
+    sc.roots = 1;
     bt = BinaryTree{};
     if (n == 0) return APG_OK;
 
-    Scratch scr;
     int rc;
     // geometry lives on the device (uploaded when the host copy changed)
     if ((rc = ctx->sync_geometry_to_device(s))) return rc;
     const float* d_xyz = ctx->d_xyz;
     const uint32_t* d_tri = ctx->d_tri;
-
-    BuildState S;
-    const size_t ncap = 2 * size_t(n) + 2 + (per_shape ? ctx->shapes.size() : 0);
+    const uint32_t R = per_shape ? uint32_t(ctx->shapes.size()) : 1u;
+    if (!ctx->build_cache) ctx->build_cache = new BuildCache;
+    BuildCache& C = *ctx->build_cache;
+    if ((rc = C.ensure(ctx, n, R))) return rc;
+    BuildState S = C.S;
     const uint32_t thr = std::max<uint32_t>(1, leaf_threshold);
-    const size_t bincap = size_t(n) / (thr + 1) + 2;
-    if ((rc = scr.get(ctx, S.pbox, 6 * size_t(n)))) return rc;
-    if ((rc = scr.get(ctx, S.tri9_src, 9 * size_t(n)))) return rc;
-    if ((rc = scr.get(ctx, S.idx, n))) return rc;
-    if ((rc = scr.get(ctx, S.idx_tmp, n))) return rc;
-    if ((rc = scr.get(ctx, S.snode, n))) return rc;
-    if ((rc = scr.get(ctx, S.snode_tmp, n))) return rc;
-    if ((rc = scr.get(ctx, S.flag, n))) return rc;
-    if ((rc = scr.get(ctx, S.scan, size_t(n) + 1))) return rc;
-    if ((rc = scr.get(ctx, S.nbox, 6 * ncap))) return rc;
-    if ((rc = scr.get(ctx, S.ncbox, 6 * ncap))) return rc;
-    if ((rc = scr.get(ctx, S.nfirst, ncap))) return rc;
-    if ((rc = scr.get(ctx, S.ncount, ncap))) return rc;
-    if ((rc = scr.get(ctx, S.nleft, ncap))) return rc;
-    if ((rc = scr.get(ctx, S.nflags, ncap))) return rc;
-    if ((rc = scr.get(ctx, S.nsplit, ncap))) return rc;
-    if ((rc = scr.get(ctx, S.nsplit_mm, 2 * ncap))) return rc;
-    if ((rc = scr.get(ctx, S.nbinslot, ncap))) return rc;
-    if ((rc = scr.get(ctx, S.bins, bincap * NODE_BIN_WORDS))) return rc;
-    if ((rc = scr.get(ctx, S.counters, 16))) return rc;
-    if ((rc = scr.get(ctx, S.scan_tmp, scan_tmp_words(std::max<uint32_t>(n, 1u) + 1)))) return rc;
 
     k_prep<<<blocks_for(n, 256), 256, 0, s>>>(d_xyz, d_tri, n, S.tri9_src, S.pbox);
     // roots: one per shape (forest) or one for the scene
-    const uint32_t R = per_shape ? uint32_t(ctx->shapes.size()) : 1u;
-    std::vector<uint32_t> rfirst(R + 1), rcount(R), rflags(R, 0u);
+    std::vector<uint32_t> rfirst(R + 1);
     {
         uint32_t off = 0;
         for (uint32_t r = 0; r < R; ++r)
         {
-            const uint32_t c = per_shape ? uint32_t(ctx->shapes[r].idx.size() / 3) : n;
             rfirst[r] = off;
-            rcount[r] = c;
-            off += c;
+            off += per_shape ? uint32_t(ctx->shapes[r].idx.size() / 3) : n;
         }
         rfirst[R] = off;
     }
-    uint32_t* d_rfirst;
-    if ((rc = scr.get(ctx, d_rfirst, R + 1))) return rc;
-    APG_HIP_CHECK(ctx, hipMemcpyAsync(d_rfirst, rfirst.data(), 4 * (R + 1), hipMemcpyHostToDevice, s));
-    k_init_slots<<<blocks_for(n, 256), 256, 0, s>>>(n, S.idx, S.snode, d_rfirst, R);
-    APG_HIP_CHECK(ctx, hipMemcpyAsync(S.nfirst, rfirst.data(), 4 * R, hipMemcpyHostToDevice, s));
-    APG_HIP_CHECK(ctx, hipMemcpyAsync(S.ncount, rcount.data(), 4 * R, hipMemcpyHostToDevice, s));
-    APG_HIP_CHECK(ctx, hipMemcpyAsync(S.nflags, rflags.data(), 4 * R, hipMemcpyHostToDevice, s));
-    uint32_t host_counters[2] = { R, 0 };
-    APG_HIP_CHECK(ctx, hipMemcpyAsync(S.counters, host_counters, 8, hipMemcpyHostToDevice, s));
+    APG_HIP_CHECK(ctx, hipMemcpyAsync(C.rfirst, rfirst.data(), 4 * (size_t(R) + 1), hipMemcpyHostToDevice, s));
+    k_init_slots<<<blocks_for(n, 256), 256, 0, s>>>(n, S.idx, S.snode, C.rfirst, R);
+    k_init_roots<<<blocks_for(R, 256), 256, 0, s>>>(S, C.rfirst, R);
 
-    // level sizes come back through pinned words (a pageable read-back is staged and slower)
-    if (!ctx->readback) APG_HIP_CHECK(ctx, hipHostMalloc(&ctx->readback, 4 * sizeof(uint32_t)));
-    uint32_t* rb = ctx->readback;
-    uint32_t begin = 0, end = R, levels = 0;
-    const uint32_t nchunks = blocks_for(n, CHUNK);
-    while (begin < end)
-    {
-        const uint32_t width = end - begin;
-        k_init_nodes<<<blocks_for(width, 256), 256, 0, s>>>(begin, end, S.nbox, S.ncbox, S.nbinslot);
-        // large nodes in chunks of their own: S.flag and S.scan are free until k_flags
-        k_large_chunks<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end, S.flag);
-        exclusive_scan(S.flag, width, S.scan, S.scan_tmp, s);
-        k_bounds_large<<<2 * nchunks + 1, BLOCK, 0, s>>>(S, begin, width, S.scan);
-        k_bounds_small<<<blocks_for(width, BLOCK / 64u), BLOCK, 0, s>>>(S, begin, end);
-        k_bounds_tiny<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end);
-        APG_HIP_CHECK(ctx, hipMemsetAsync(S.counters + 1, 0, 4, s));
-        k_decide<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end, thr);
-        k_bin_large<<<2 * nchunks + 1, BLOCK, 0, s>>>(S, begin, width, S.scan);
-        k_bin_small<<<blocks_for(width, BLOCK / 64u), BLOCK, 0, s>>>(S, begin, end);
-        k_sweep<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end);
-        k_child_first<<<blocks_for(width, 256), 256, 0, s>>>(S, begin, end);
-        k_flags<<<blocks_for(n, 256), 256, 0, s>>>(S, n, begin, end);
+    // ---- binary levels: enqueued in batches, the level range read back between batches ----
+    const uint32_t g_node = std::min<uint32_t>(blocks_for(n, 256), 2048u);
+    const uint32_t g_wave = std::min<uint32_t>(blocks_for(n / (TINY + 1) + 1, BLOCK / 64u), 1024u);
+    const uint32_t g_chunk = 2 * blocks_for(n, CHUNK) + 1;
+    const uint32_t g_slot = blocks_for(n, 256);
+    uint32_t* rb = C.rb;
+    auto enqueue_level = [&]() {
+        k_level_init<<<g_node, 256, 0, s>>>(S);
+        k_large_scan<<<1, SCAN1, 0, s>>>(S);
+        k_bounds_large<<<g_chunk, BLOCK, 0, s>>>(S);
+        k_bounds_small<<<g_wave, BLOCK, 0, s>>>(S);
+        k_bounds_tiny<<<g_node, 256, 0, s>>>(S);
+        k_decide<<<g_node, 256, 0, s>>>(S, thr);
+        k_bin_large<<<g_chunk, BLOCK, 0, s>>>(S);
+        k_bin_small<<<g_wave, BLOCK, 0, s>>>(S);
+        k_sweep<<<g_node, 256, 0, s>>>(S);
+        k_child_first<<<g_node, 256, 0, s>>>(S);
+        k_flags<<<g_slot, 256, 0, s>>>(S, n);
         exclusive_scan(S.flag, n, S.scan, S.scan_tmp, s);
-        k_scatter<<<blocks_for(n, 256), 256, 0, s>>>(S, n, begin, end);
+        k_scatter<<<g_slot, 256, 0, s>>>(S, n);
+        k_next_level<<<1, 1, 0, s>>>(S);
         std::swap(S.idx, S.idx_tmp);
         std::swap(S.snode, S.snode_tmp);
-        APG_HIP_CHECK(ctx, hipMemcpyAsync(rb, S.counters, 4, hipMemcpyDeviceToHost, s));
+    };
+    // most trees are a little deeper than log2(n): the first batch rarely runs empty levels
+    uint32_t enqueued = 0;
+    for (;;)
+    {
+        const uint32_t batch = enqueued == 0 ? ceil_log2(n) + 2u : 4u;
+        for (uint32_t b = 0; b < batch; ++b) enqueue_level();
+        enqueued += batch;
+        APG_HIP_CHECK(ctx, hipMemcpyAsync(rb, S.lvl, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
         APG_HIP_CHECK(ctx, hipGetLastError());
-        const uint32_t next_end = rb[0];
-        begin = end;
-        end = next_end;
-        if (++levels > 4096) return ctx->fail(APG_E_CAPACITY, "binary tree deeper than 4096 levels");
+        if (rb[0] == rb[1]) break;
+        if (enqueued > 4096) return ctx->fail(APG_E_CAPACITY, "binary tree deeper than 4096 levels");
     }
-    const uint32_t bin_nodes = end;
+    const uint32_t bin_nodes = rb[1];
 
     // ---- wide collapse ----
-    float slack = 0.0f;
-    {
-        uint32_t* d_m;
-        if ((rc = scr.get(ctx, d_m, 1))) return rc;
-        APG_HIP_CHECK(ctx, hipMemsetAsync(d_m, 0, 4, s));
-        k_scene_slack<<<256, 256, 0, s>>>(S.pbox, n, d_m);
-        APG_HIP_CHECK(ctx, hipMemcpyAsync(rb, d_m, 4, hipMemcpyDeviceToHost, s));
-        APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
-        const uint32_t m = rb[0];
-        float mf;
-        std::memcpy(&mf, &m, 4);
-        if (mf == 0.0f) mf = 1.0f;
-        slack = std::ldexp(mf, -17);
-    }
-    const size_t wcap = bin_nodes / 2 + 2 + R;  // each wide node holds >= 2 binary nodes or is a root
+    k_fill_u32<<<1, 64, 0, s>>>(C.slack_bits, 1, 0u);
+    k_scene_slack<<<256, 256, 0, s>>>(S.pbox, n, C.slack_bits);
+    k_slack_final<<<1, 1, 0, s>>>(C.slack_bits, C.slack);
+    const uint32_t wcap = bin_nodes / 2 + 2 + R;  // each wide node holds >= 2 binary nodes or is a root
+    if (wcap > C.wide_cap) return ctx->fail(APG_E_CAPACITY, "wide node capacity exceeded");
+    if ((rc = ensure_scene_buffers(ctx, wcap, n))) return rc;
     WideState W;
     W.nbox = S.nbox; W.nfirst = S.nfirst; W.ncount = S.ncount; W.nleft = S.nleft;
-    W.nflags = S.nflags; W.idx = S.idx; W.tri9_src = S.tri9_src; W.slack = slack;
-    if ((rc = scr.get(ctx, W.wbin, 2 * wcap + 8))) return rc;
-    if ((rc = scr.get(ctx, W.wchild, 8 * wcap))) return rc;
-    if ((rc = scr.get(ctx, W.n_inner, wcap))) return rc;
-    if ((rc = scr.get(ctx, W.n_tris, wcap))) return rc;
-    if ((rc = scr.get(ctx, W.s_inner, wcap + 1))) return rc;
-    if ((rc = scr.get(ctx, W.s_tris, wcap + 1))) return rc;
-    uint32_t* wscan_tmp;
-    if ((rc = scr.get(ctx, wscan_tmp, scan_tmp_words(uint32_t(wcap) + 1)))) return rc;
-    // The scene's buffers are scratch (freed on every error return) until they are handed
-    // to the context below.
-    ApgWideNode* d_nodes;
-    float* d_tri9;
-    uint32_t* d_ids;
-    if ((rc = scr.get(ctx, d_nodes, wcap))) return rc;
-    if ((rc = scr.get(ctx, d_tri9, 9 * size_t(n)))) return rc;
-    if ((rc = scr.get(ctx, d_ids, n))) return rc;
-    W.nodes = d_nodes; W.tri9 = d_tri9; W.tri_ids = d_ids;
+    W.nflags = S.nflags; W.idx = S.idx; W.tri9_src = S.tri9_src;
+    W.wbin = C.wbin; W.wchild = C.wchild; W.n_inner = C.n_inner; W.n_tris = C.n_tris;
+    W.s_inner = C.s_inner; W.s_tris = C.s_tris;
+    W.nodes = sc.nodes; W.tri9 = sc.tris; W.tri_ids = sc.tri_ids;
+    W.slack = C.slack; W.wl = C.wl; W.wcap = wcap;
+    const uint32_t g_fill = std::min<uint32_t>(blocks_for(wcap, 256), 1024u);
+    k_fill_u32<<<g_fill, 256, 0, s>>>(W.n_inner, wcap, 0u);
+    k_fill_u32<<<g_fill, 256, 0, s>>>(W.n_tris, wcap, 0u);
+    k_init_wide<<<blocks_for(R, 256), 256, 0, s>>>(W, R);
+    const uint32_t g_wide = std::min<uint32_t>(blocks_for(wcap, 128), 2048u);
+    auto enqueue_wide = [&]() {
+        k_wide_gather<<<g_wide, 128, 0, s>>>(W);
+        exclusive_scan(W.n_inner, wcap, W.s_inner, C.wscan_tmp, s);
+        exclusive_scan(W.n_tris, wcap, W.s_tris, C.wscan_tmp, s);
+        k_wide_emit<<<g_wide, 128, 0, s>>>(W);
+        k_next_wide<<<1, 1, 0, s>>>(W);
+    };
+    enqueued = 0;
+    for (;;)
     {
-        // wide nodes 0 .. R-1 are the roots' images
-        std::vector<uint32_t> wroots(R);
-        for (uint32_t r = 0; r < R; ++r) wroots[r] = r;
-        APG_HIP_CHECK(ctx, hipMemcpyAsync(W.wbin, wroots.data(), 4 * R, hipMemcpyHostToDevice, s));
-        APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
-    }
-    uint32_t wbegin = 0, wend = R, tri_cursor = 0, wlevels = 0;
-    while (wbegin < wend)
-    {
-        const uint32_t width = wend - wbegin;
-        k_wide_gather<<<blocks_for(width, 128), 128, 0, s>>>(W, wbegin, wend);
-        exclusive_scan(W.n_inner, width, W.s_inner, wscan_tmp, s);
-        exclusive_scan(W.n_tris, width, W.s_tris, wscan_tmp, s);
-        k_wide_emit<<<blocks_for(width, 128), 128, 0, s>>>(W, wbegin, wend, wend, tri_cursor);
-        APG_HIP_CHECK(ctx, hipMemcpyAsync(rb, W.s_inner + width, 4, hipMemcpyDeviceToHost, s));
-        APG_HIP_CHECK(ctx, hipMemcpyAsync(rb + 1, W.s_tris + width, 4, hipMemcpyDeviceToHost, s));
+        const uint32_t batch = enqueued == 0 ? std::max<uint32_t>(2u, (ceil_log2(n) + 2u) / 3u) : 2u;
+        for (uint32_t b = 0; b < batch; ++b) enqueue_wide();
+        enqueued += batch;
+        APG_HIP_CHECK(ctx, hipMemcpyAsync(rb, W.wl, 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
         APG_HIP_CHECK(ctx, hipGetLastError());
-        const uint32_t tot[2] = { rb[0], rb[1] };
-        tri_cursor += tot[1];
-        wbegin = wend;
-        wend += tot[0];
-        wlevels++;
-        if (wend > wcap) return ctx->fail(APG_E_CAPACITY, "wide node capacity exceeded");
+        if (rb[4]) return ctx->fail(APG_E_CAPACITY, "wide node capacity exceeded");
+        if (rb[0] == rb[1]) break;
+        if (enqueued > 4096) return ctx->fail(APG_E_CAPACITY, "wide tree deeper than 4096 levels");
     }
+    const uint32_t wide_nodes = rb[1], tri_cursor = rb[2], wlevels = rb[3];
     if (tri_cursor != n) return ctx->fail(APG_E_STATE, "GPU build lost triangles");
-    scr.release(d_nodes);
-    scr.release(d_tri9);
-    scr.release(d_ids);
-    ctx->scene.nodes = d_nodes;
-    ctx->scene.tris = d_tri9;
-    ctx->scene.tri_ids = d_ids;
-    ctx->scene.node_count = wend;
-    ctx->scene.tri_count = tri_cursor;
-    ctx->scene.roots = R;
-    ctx->scene.wide_depth = wlevels;
+    sc.node_count = wide_nodes;
+    sc.tri_count = tri_cursor;
+    sc.roots = R;
+    sc.wide_depth = wlevels;
     bt.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
 
     // ---- scene bounds (union of the roots) ----
@@ -1409,14 +1611,23 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
 }
 
 // After a build: the vertex-form triangles from the vertices the tree was built from.
+
+
+// After a build: the vertex-form triangles from the vertices the tree was built from (the
+// buffer is kept across builds while it is large enough).
 int gather_vertex_triangles(Context* ctx)
 {
     DeviceScene& S = ctx->scene;
-    if (S.tris_v) (void)hipFree(S.tris_v);
-    S.tris_v = nullptr;
     if (S.tri_count == 0) return APG_OK;
     if (int rc = ctx->sync_geometry_to_device(nullptr)) return rc;
-    APG_HIP_CHECK(ctx, hipMalloc(&S.tris_v, size_t(S.tri_count) * 9 * sizeof(float)));
+    if (!S.tris_v || S.tris_v_cap < S.tri_count)
+    {
+        if (S.tris_v) (void)hipFree(S.tris_v);
+        S.tris_v = nullptr;
+        S.tris_v_cap = 0;
+        APG_HIP_CHECK(ctx, hipMalloc(&S.tris_v, size_t(S.tri_count) * 9 * sizeof(float)));
+        S.tris_v_cap = S.tri_count;
+    }
     k_gather_vertices<<<(S.tri_count + 255) / 256, 256>>>(S.tri_ids, S.tri_count, ctx->d_tri, ctx->d_xyz, S.tris_v);
     APG_HIP_CHECK(ctx, hipGetLastError());
     APG_HIP_CHECK(ctx, hipDeviceSynchronize());

@@ -93,8 +93,35 @@ def _profiled_traffic():
         return None
 
 
+def hbm_read_gbs(gib=4, reps=5):
+    """Achievable HBM read bandwidth: the library's streaming-read kernel (16-byte
+    non-temporal loads, 8 workgroups of 256 per CU) over `gib` GiB, HIP events on its stream."""
+    import ctypes as C
+    import torch
+    import alpine_amd as aa
+    n = gib << 30
+    x = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    sink = torch.zeros(1, dtype=torch.int32, device="cuda")
+    blocks = torch.cuda.get_device_properties(0).multi_processor_count * 8
+    s = torch.cuda.current_stream()
+    call = lambda: aa.lib().apg_debug_stream_read(C.c_void_p(x.data_ptr()), n, C.c_void_p(sink.data_ptr()),
+                                                  blocks, C.c_void_p(s.cuda_stream))
+    for _ in range(2):
+        assert call() == 0
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        call()
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del x
+    return n / (ms * 1e-3) / 1e9
+
+
 def hbm_copy_gbs():
-    """Achievable HBM bandwidth: device-to-device copy of 4 GiB (read + write bytes)."""
+    """Device-to-device copy of 4 GiB (read + write bytes), for comparison."""
     import torch
     n = 1 << 30
     x = torch.empty(n, dtype=torch.float32, device="cuda")
@@ -482,27 +509,54 @@ RENDER_CONFIGS = {
 
 
 def cpu_baseline(verts, tris, org, dir, sample):
-    """The reference's binary Bvh (compiled from its sources, oracle/_ref) on this host's cores."""
+    """The reference's binary Bvh (compiled from its sources, oracle/_ref) on every host core
+    this process may use (one thread per core, the reference's tile tasks, alpine.cpp:276-285),
+    on the whole incoherent ray set; and on one thread, on the first `sample` rays."""
     import oracle
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores = len(os.sched_getaffinity(0))
     kind = "reference" if oracle.ref_available() else "port"
     t0 = time.perf_counter()
     bvh = oracle.RefBvh(verts, tris) if kind == "reference" else oracle.OracleBvh(verts, tris, 8)
     build_s = time.perf_counter() - t0
-    n = min(sample, len(org))
-    o, d = org[:n], dir[:n]
-    bvh.intersect(o[:4096], d[:4096], nthreads=cores)  # warm
+    bvh.intersect(org[:4096], dir[:4096], nthreads=cores)  # warm
     times = []
     for _ in range(3):
         t0 = time.perf_counter()
-        h = bvh.intersect(o, d, nthreads=cores)
+        h = bvh.intersect(org, dir, nthreads=cores)
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
+    n1 = min(sample, len(org))
+    t0 = time.perf_counter()
+    bvh.intersect(org[:n1], dir[:n1], nthreads=1)
+    one = time.perf_counter() - t0
     return {
-        "value": round(n / med / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": kind,
-        "sample": f"first {n} of the {len(org)} incoherent rays, closest-hit, binary Bvh "
-                  f"(9-bin SAH, leaf 8), median of 3 after a warm-up; build {build_s:.2f} s",
+        "value": round(len(org) / med / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": kind,
+        "sample": f"all {len(org)} incoherent rays, closest-hit, the reference's binary Bvh "
+                  f"(9-bin SAH, leaf 8) compiled from its sources, {cores} threads, median of 3 "
+                  f"after a warm-up ({med:.2f} s each); build {build_s:.2f} s",
+        "one_thread": {"value": round(n1 / one / 1e6, 4), "unit": "Mrays/s", "cores": 1,
+                       "sample": f"first {n1} incoherent rays, one pass ({one:.2f} s)"},
     }, h, bvh
+
+
+def cpu_render_baseline(cores):
+    """The reference integrator (alpine.cpp:290-432, restated in the oracle over the reference
+    binary Bvh's restatement) on config 1, the Cornell room at 256 x 256 x 16 spp, on `cores`
+    threads: the CPU samples/s beside the GPU's."""
+    import oracle
+    from alpine_amd import scenes
+    sc = scenes.cornell_scene()
+    ors = oracle.OracleScene(sc.verts, sc.tris, sc.tri_mat, [oracle.make_material(*m) for m in sc.materials],
+                             [oracle.make_light(**l) for l in sc.lights], sc.light_sampler,
+                             oracle.make_camera(**sc.camera), sc.background)
+    ors.render(sc.width, sc.height, 1, sc.max_depth, None, None, sc.rr_depth, cores)  # warm
+    states = oracle.sampler_states(sc.width * sc.height, sc.seed)
+    t0 = time.perf_counter()
+    ors.render(sc.width, sc.height, sc.spp, sc.max_depth, states, None, sc.rr_depth, cores)
+    dt = time.perf_counter() - t0
+    return {"value": round(sc.width * sc.height * sc.spp / dt, 1), "unit": "samples/s", "cores": cores,
+            "kind": "port", "sample": f"Cornell room {sc.width} x {sc.height} x {sc.spp} spp, depth "
+                                      f"{sc.max_depth}, {len(sc.tris)} triangles, one pass ({dt:.2f} s)"}
 
 
 def cpu_baseline_wide(verts, tris, org, dir, sample, cores):
@@ -596,8 +650,10 @@ def main():
     b_any = B_RAY_IN + B_OCC_OUT + (B_NODE * an_node + B_TRI * an_tri) / N_RAYS
     b_closest = B_RAY_IN + B_HIT_OUT + B_NODE * node_pr + B_TRI * tri_pr
     achieved = b_closest * N_RAYS / (ms * 1e-3) / 1e9
+    bw_read = hbm_read_gbs()
     bw = hbm_copy_gbs()
     mrays = N_RAYS / (ms * 1e-3) / 1e6
+    traffic = _profiled_traffic()
 
     result = {
         "metric": METRIC,
@@ -620,11 +676,21 @@ def main():
             "launch": tuned or "default",
         },
         "roofline": {
+            # achieved / frac: the algorithmic bytes of the byte model (64 B per node fetch,
+            # 36 B per triangle fetch, ray and hit records) per mean kernel time.  The profile
+            # shows the kernel limited by VALU issue, and most node fetches hit L2 / the
+            # Infinity Cache: the HBM-side bytes it does move are `traffic` (rocprofv3
+            # FETCH_SIZE + WRITE_SIZE of the same launch, profiles/), hbm_measured_* below.
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _profiled_traffic(),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": "profiles/trace_kernel_incoherent_closest_1p5Mtri_4Mrays.json",
+            "limiter": "valu issue (rocprofv3 SQ counters, profiles/)",
+            "algorithmic_frac": round(achieved / HBM_PEAK_GBS, 4),
+            "hbm_measured_gbs": round(traffic / (ms * 1e-3) / 1e9, 1) if traffic else None,
+            "hbm_measured_frac": round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
             "bytes_per_ray": round(b_closest, 1), "node_fetches_per_ray": round(node_pr, 3),
-            "tri_fetches_per_ray": round(tri_pr, 3), "hbm_copy_gbs": round(bw, 1),
-            "frac_of_copy": round(achieved / bw, 4),
+            "tri_fetches_per_ray": round(tri_pr, 3), "hbm_stream_read_gbs": round(bw_read, 1),
+            "frac_of_stream_read": round(achieved / bw_read, 4), "hbm_copy_gbs": round(bw, 1),
         },
         "extra": {
             "wall_ms_per_step": round(wall_ms, 4),
@@ -654,10 +720,9 @@ def main():
         result["cpu_baseline"] = cb
         result["extra"]["cpu_baseline_wide"] = cpu_baseline_wide(verts, tris, org, dir,
                                                                  args.cpu_sample, cb["cores"])
+        result["extra"]["cpu_baseline_render"] = cpu_render_baseline(cb["cores"])
         n = len(href.prim)
-        torch.cuda.synchronize()
-        g = out[0][:n].cpu().numpy().view(np.uint32)
-        # the last timed pass was coherent; recompute incoherent hits for the parity summary
+        # the last timed pass was watertight; recompute incoherent hits for the parity summary
         acc.intersect(o_d, d_d, out=out)
         torch.cuda.synchronize()
         g = out[0][:n].cpu().numpy().view(np.uint32)

@@ -38,6 +38,11 @@ int apg_debug_host_trace_limited(const apg_debug_scene* s, const float* org3, co
                                  const float* tmax, uint32_t n, int any, uint32_t stack_limit,
                                  uint32_t* prim, float* t, float* b0, float* b1, uint8_t* occluded,
                                  uint32_t* overflowed);
+/* Streaming read of `bytes` (a multiple of 16) of device memory at buf by `blocks` workgroups of
+ * 256 threads (16-byte non-temporal loads), asynchronous on stream; sink (device, 4 bytes) is
+ * written only if the data's XOR is a fixed constant.  The achievable-bandwidth measurement of
+ * the roofline. */
+int apg_debug_stream_read(const void* buf, uint64_t bytes, uint32_t* sink, uint32_t blocks, void* stream);
 /* Render-state device allocations of ctx (render.hip) fail with APG_E_HIP after `after` more
  * have succeeded; after < 0 restores normal allocation.  For tests of the failure paths. */
 int apg_debug_fail_allocations(apg_context* ctx, int after);
