@@ -52,6 +52,7 @@ struct DeviceScene
     uint32_t wide_depth = 0;     // wide-node levels of the deepest root-to-leaf path
     float* tris_v = nullptr;     // 9 floats per triangle, tree order: the input vertices v0, v1, v2
                                  // (watertight test; gathered on first use after a build)
+    bool tris_v_valid = false;   // tris_v holds this build's triangles
     // allocated sizes (the GPU builder reuses the buffers while they are large enough)
     uint32_t node_cap = 0, tri_cap = 0, tris_v_cap = 0;
 };

@@ -351,7 +351,7 @@ int apg_scene_build(apg_context* ctx, const apg_build_params* params)
     // nodes; reference-form triangles and ids (40 B); vertex-form triangles (36 B)
     st.device_bytes = uint64_t(ctx->scene.node_count) * 64 + uint64_t(ctx->scene.tri_count) * 76;
     if ((rc = ctx->rebuild_shading())) return rc;
-    if ((rc = apg::gather_vertex_triangles(ctx))) return rc;
+    if (ctx->cfg.triangle_test == APG_TRI_WATERTIGHT && (rc = apg::gather_vertex_triangles(ctx))) return rc;
     st.build_ms = bt.build_ms > 0.0
         ? bt.build_ms
         : std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -396,6 +396,12 @@ int apg_set_trace_config(apg_context* ctx, const apg_trace_config* cfg)
         return ctx->fail(APG_E_INVALID, "triangle_test must be APG_TRI_REFERENCE or APG_TRI_WATERTIGHT");
     if (cfg->stack_limit > APG_TRACE_STACK_MAX)
         return ctx->fail(APG_E_INVALID, "stack_limit must be at most APG_TRACE_STACK_MAX");
+    // the watertight test reads vertex-form triangles, gathered once per build when first needed
+    if (cfg->triangle_test == APG_TRI_WATERTIGHT && ctx->built && !ctx->scene.tris_v_valid)
+    {
+        if (ctx->last_launch) APG_HIP_CHECK(ctx, hipEventSynchronize(ctx->last_launch));
+        if (int rc = apg::gather_vertex_triangles(ctx)) return rc;
+    }
     ctx->cfg = *cfg;
     return APG_OK;
 }

@@ -203,7 +203,7 @@ __global__ void k_init_slots(uint32_t n, uint32_t* idx, uint32_t* snode, const u
 
 // Per node of the level: clear the bounds accumulators and list the nodes of more than SMALL
 // slots (their order in the list is arbitrary: what they accumulate is order independent).
-__global__ void k_level_init(BuildState S)
+__global__ __launch_bounds__(256) void k_level_init(BuildState S)
 {
     const uint32_t begin = S.lvl[0], end = S.lvl[1];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -329,7 +329,7 @@ __device__ __forceinline__ float box_area(const float lo[3], const float hi[3])
 
 // Classify every node of the level.  The loop is wave-uniform (each wave takes 64
 // consecutive nodes per step) for the histogram-slot ballot.
-__global__ void k_decide(BuildState S, uint32_t threshold)
+__global__ __launch_bounds__(256) void k_decide(BuildState S, uint32_t threshold)
 {
     const uint32_t begin = S.lvl[0], end = S.lvl[1];
     const uint32_t lane = threadIdx.x & 63u;
@@ -498,7 +498,7 @@ __global__ __launch_bounds__(BLOCK) void k_bounds_small(BuildState S)
 }
 
 // Bounds of a node of at most TINY slots: one thread, a serial loop.
-__global__ void k_bounds_tiny(BuildState S)
+__global__ __launch_bounds__(256) void k_bounds_tiny(BuildState S)
 {
     const uint32_t begin = S.lvl[0], end = S.lvl[1];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -803,7 +803,7 @@ __device__ void sweep_node(BuildState& S, uint32_t k)
     split_node(S, k, fl, cnt, nbelow, best_dim, best_idx, best_mn, best_mx);
 }
 
-__global__ void k_sweep(BuildState S)
+__global__ __launch_bounds__(256) void k_sweep(BuildState S)
 {
     const uint32_t begin = S.lvl[0], end = S.lvl[1];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -858,7 +858,7 @@ __global__ void k_scatter(BuildState S, uint32_t n)
 }
 
 // First slot of every child allocated in this level.
-__global__ void k_child_first(BuildState S)
+__global__ __launch_bounds__(256) void k_child_first(BuildState S)
 {
     const uint32_t begin = S.lvl[0], end = S.lvl[1];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -1108,7 +1108,7 @@ __device__ void wide_gather(const WideState& W, uint32_t w)
     W.n_tris[w] = nt;
 }
 
-__global__ void k_wide_gather(WideState W)
+__global__ __launch_bounds__(128) void k_wide_gather(WideState W)
 {
     const uint32_t begin = W.wl[0], end = W.wl[1];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -1216,7 +1216,7 @@ __device__ void wide_emit(const WideState& W, uint32_t w, uint32_t level_end, ui
     W.n_tris[w] = 0u;
 }
 
-__global__ void k_wide_emit(WideState W)
+__global__ __launch_bounds__(128) void k_wide_emit(WideState W)
 {
     const uint32_t begin = W.wl[0], end = W.wl[1], tri_begin = W.wl[2];
     const float slack = *W.slack;
@@ -1416,8 +1416,7 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
     hipStream_t s = nullptr;
     // the previous scene's buffers stay allocated for this build to reuse
     DeviceScene& sc = ctx->scene;
-    if (sc.tris_v) (void)hipFree(sc.tris_v);
-    sc.tris_v = nullptr;
+    sc.tris_v_valid = false;
     sc.node_count = sc.tri_count = sc.wide_depth = 0;
     sc.roots = 1;
     bt = BinaryTree{};
@@ -1613,12 +1612,13 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
 // After a build: the vertex-form triangles from the vertices the tree was built from.
 
 
-// After a build: the vertex-form triangles from the vertices the tree was built from (the
-// buffer is kept across builds while it is large enough).
+// The vertex-form triangles from the vertices the tree was built from, which the watertight
+// test reads: gathered after a build when that test is selected, or when it is selected later
+// (the buffer is kept across builds while it is large enough).
 int gather_vertex_triangles(Context* ctx)
 {
     DeviceScene& S = ctx->scene;
-    if (S.tri_count == 0) return APG_OK;
+    if (S.tri_count == 0) { S.tris_v_valid = true; return APG_OK; }
     if (int rc = ctx->sync_geometry_to_device(nullptr)) return rc;
     if (!S.tris_v || S.tris_v_cap < S.tri_count)
     {
@@ -1631,6 +1631,7 @@ int gather_vertex_triangles(Context* ctx)
     k_gather_vertices<<<(S.tri_count + 255) / 256, 256>>>(S.tri_ids, S.tri_count, ctx->d_tri, ctx->d_xyz, S.tris_v);
     APG_HIP_CHECK(ctx, hipGetLastError());
     APG_HIP_CHECK(ctx, hipDeviceSynchronize());
+    S.tris_v_valid = true;
     return APG_OK;
 }
 

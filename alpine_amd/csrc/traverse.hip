@@ -637,7 +637,7 @@ int launch_trace_kernel(Context* ctx, const apg_rays& rays, uint32_t n, const ui
     const bool wt = cfg.triangle_test == APG_TRI_WATERTIGHT;
     if (wt)
     {
-        if (!ctx->scene.tris_v && ctx->scene.tri_count)
+        if (!ctx->scene.tris_v_valid && ctx->scene.tri_count)
             return ctx->fail(APG_E_STATE, "no vertex-form triangles for the watertight test");
         A.tris = ctx->scene.tris_v;
     }
@@ -728,7 +728,7 @@ int apg_intersect_noalloc(apg_context* ctx, const apg_rays* rays, uint32_t n, co
     if (!base) return ctx->fail(APG_E_STATE, "allocation-free scratch not set up for this device");
     const apg_trace_config& cfg = ctx->cfg;
     const bool wt = cfg.triangle_test == APG_TRI_WATERTIGHT;
-    if (wt && !ctx->scene.tris_v && ctx->scene.tri_count)
+    if (wt && !ctx->scene.tris_v_valid && ctx->scene.tri_count)
         return ctx->fail(APG_E_STATE, "no vertex-form triangles for the watertight test");
     const uint32_t lds = (cfg.stack_lds == 16 && !wt) ? 16u : 8u;
     const uint32_t stack_max = std::min<uint32_t>(lds + NOALLOC_SPILL, APG_STACK_MAX);
