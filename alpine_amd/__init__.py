@@ -648,24 +648,23 @@ class GpuAccelerator:
         self._rc(self._lib.apg_guide_infer(self._h, _ptr(q), n, _ptr(out), self._stream(stream)))
         return out
 
-    def guide_train_queries(self, q, dirs, weight, apply=True, want_grad=False, stream=None):
-        """One training step from n x 9 queries (encoding fused into the training kernel)."""
-        n = q.shape[0]
-        loss = C.c_float()
-        g = np.empty(self._lib.apg_guide_param_count(), np.float32) if want_grad else None
-        self._rc(self._lib.apg_guide_train_queries(self._h, _ptr(q), _ptr(dirs), _ptr(weight), n,
-                                                   1 if apply else 0, C.byref(loss),
-                                                   None if g is None else _ptr(g), self._stream(stream)))
-        return loss.value, g
+    def guide_train_queries(self, q, dirs, weight, apply=True, want_grad=False, want_loss=True, stream=None):
+        """One training step from n x 9 queries (encoding fused into the training kernel).
+        Returns (loss, grad); with want_loss=False and want_grad=False the step is only
+        enqueued on the stream and (None, None) comes back at once."""
+        return self._train(self._lib.apg_guide_train_queries, q, dirs, weight, apply, want_grad, want_loss, stream)
 
-    def guide_train_step(self, feat, dirs, weight, apply=True, want_grad=False, stream=None):
-        n = feat.shape[0]
-        loss = C.c_float()
+    def guide_train_step(self, feat, dirs, weight, apply=True, want_grad=False, want_loss=True, stream=None):
+        """One training step from n x 96 features; see guide_train_queries."""
+        return self._train(self._lib.apg_guide_train_step, feat, dirs, weight, apply, want_grad, want_loss, stream)
+
+    def _train(self, fn, x, dirs, weight, apply, want_grad, want_loss, stream):
+        loss = C.c_float() if want_loss else None
         g = np.empty(self._lib.apg_guide_param_count(), np.float32) if want_grad else None
-        self._rc(self._lib.apg_guide_train_step(self._h, _ptr(feat), _ptr(dirs), _ptr(weight), n,
-                                                1 if apply else 0, C.byref(loss),
-                                                None if g is None else _ptr(g), self._stream(stream)))
-        return loss.value, g
+        self._rc(fn(self._h, _ptr(x), _ptr(dirs), _ptr(weight), x.shape[0], 1 if apply else 0,
+                    None if loss is None else C.byref(loss), None if g is None else _ptr(g),
+                    self._stream(stream)))
+        return (None if loss is None else loss.value), g
 
     # ---- introspection ----
     def leaf_keys(self) -> np.ndarray:

@@ -1,29 +1,46 @@
-// Online-trained path-guiding MLP on gfx950: fused BF16 MFMA forward,
-// backward (data and weight gradients) and Adam.  No counterpart in the
-// reference; the proposal is mixed with BSDF sampling by one-sample MIS in the
-// path tracer (render.hip), so plain path tracing stays the fallback.
+// Online-trained path-guiding MLP on gfx950.  No counterpart in the reference; the proposal
+// is mixed with BSDF sampling by one-sample MIS in the path tracer (render.hip), so plain
+// path tracing stays the fallback.
 //
-// Layers 96 -> 64 -> 64 -> 64 -> 64 -> 48 (guide_core.h).  All weights (21.5 K
-// BF16 = 43 KB) sit in LDS for the whole kernel; a wave carries 16 samples
-// through every layer with v_mfma_f32_16x16x16_bf16 (A: 16 samples x 16
-// inputs, B: 16 inputs x 16 outputs, FP32 accumulators), bias and ReLU fused
-// into the epilogue.  Weight gradients reduce over samples in the MFMA
-// accumulators of each wave and are added to an FP32 gradient vector once per
-// workgroup.  Adam keeps FP32 master weights and writes the BF16 copy.
+// Layers 96 -> 64 -> 64 -> 64 -> 64 -> 48 (guide_core.h): BF16 weights and activations, FP32
+// accumulation, every product on v_mfma_f32_16x16x32_bf16.  Operand layout (wave64): lane l
+// holds A[l % 16][8 (l / 16) + i] and B[8 (l / 16) + i][l % 16] for i = 0..7, and the result
+// D[4 (l / 16) + i][l % 16] for i = 0..3.
 //
-// Operand layout of v_mfma_f32_16x16x16_bf16 (wave64): lane l holds
-// A[l % 16][4 (l / 16) + i] and B[4 (l / 16) + i][l % 16] for i = 0..3, and the
-// result D[4 (l / 16) + i][l % 16].
+// A layer runs in the transposed form D[unit][sample] = W[unit][:] . act[:][sample]: weights
+// are the A operand (LDS), a wave's activations the B operand (registers).  Lane (r, g) =
+// (l % 16, l / 16) of a result tile holds units 4g .. 4g + 3 of sample r, so two consecutive
+// result tiles concatenated per lane are a B operand of the next layer over 32 inputs in the
+// order
+//     position 32p + 8g + i  <->  unit 32p + 4g + i (i < 4), 32p + 16 + 4g + (i - 4) (i >= 4)
+// (guide_pair_unit); the next layer's weight columns are stored in that order and
+// activations never leave registers.  The 96 inputs are computed in registers as well: lane
+// group g evaluates the sin / cos features of frequencies 3g .. 3g + 2 on every axis and six
+// of the other 24 (guide_slot_feature), and the layer-0 columns follow that assignment.  The
+// bias is the accumulator's initial value and ReLU one packed 16-bit max on the BF16 pair.
+//
+// The weights reach LDS as images (rows padded by 8 BF16 so the rows one fragment read
+// touches fall in distinct banks, then the FP32 biases) that k_guide_pack writes from the
+// FP32 master weights after every update; a workgroup copies its image whole.
+//
+// Training computes a minibatch's gradient in one kernel.  A workgroup takes 128 samples at
+// a time and each wave carries 32 of them (two 16-sample tiles sharing every weight fragment)
+// through the forward pass, the loss head (a sample's four lane groups take two mixture lobes
+// each) and the backward pass.  Layer by layer, the layer's inputs and output gradients for
+// the 128 samples are staged in LDS (unit-major, samples contiguous) and the four waves add
+// the weight gradient dW[n][k] = sum_s dZ[n][s] A[k][s] into MFMA accumulators, each wave
+// owning fixed 16 x 16 tiles over every sample its workgroup sees; the bias gradient is the
+// same product with a B operand of ones.  Each workgroup writes its sums once and
+// k_guide_reduce adds them in workgroup order (deterministic), divides by the minibatch's
+// weight sum and takes the Adam step.
 #include "guide_core.h"
 #include "apg_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
 namespace apg {
-
-typedef short s4v __attribute__((ext_vector_type(4)));
-typedef float f4v __attribute__((ext_vector_type(4)));
 
 __host__ __device__ __forceinline__ uint16_t f2bf(float f)
 {
@@ -36,716 +53,649 @@ __host__ __device__ __forceinline__ uint16_t f2bf(float f)
     u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
     return uint16_t(u >> 16);
 }
-__host__ __device__ __forceinline__ float bf2f(uint16_t h)
-{
-    const uint32_t u = uint32_t(h) << 16;
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __uint_as_float(u);
-#else
-    float f;
-    __builtin_memcpy(&f, &u, 4);
-    return f;
-#endif
-}
 
-struct GuideNet
+// Position j (0..63) of a hidden layer's input -> unit of the previous layer's output.
+SHD constexpr int guide_pair_unit(int j)
 {
-    float* w32 = nullptr;   // master parameters (GUIDE_PARAMS)
-    uint16_t* w16 = nullptr; // BF16 weights (first GUIDE_B_OFF[0] entries)
-    float* m = nullptr;
-    float* v = nullptr;
-    float* grad = nullptr;
-    float* part = nullptr;     // WG_MAX x GUIDE_PARAMS partial gradients
-    float* scratch = nullptr;  // [0] sum of weights, [1] loss sum
-    bool ready = false;        // every buffer above is allocated and initialised
-    uint32_t step = 0;
-    float lr = 1e-3f;
-    // training buffers (capacity in samples)
-    uint32_t cap = 0;
-    uint16_t* acts = nullptr;  // cap x 4 x 64
-    uint16_t* feat16 = nullptr; // cap x 96 (fused training path)
-    float* out = nullptr;      // cap x 48
-    uint16_t* dz = nullptr;    // cap x (48 + 4 x 64)
-};
+    return 32 * (j >> 5) + ((j & 7) < 4 ? 4 * ((j & 31) >> 3) + (j & 7) : 16 + 4 * ((j & 31) >> 3) + (j & 7) - 4);
+}
+// Input slot s (0..23) of lane group g -> feature index (guide_feature): slots 0..17 are the
+// sin / cos pairs of frequency 3g + s / 6 on axis (s % 6) / 2, slots 18..23 six of the
+// position, normal, direction and zero-padding features.
+SHD constexpr int guide_slot_feature(int g, int s)
+{
+    return s < 18 ? 3 + 24 * ((s % 6) / 2) + 2 * (3 * g + s / 6) + (s & 1)
+                  : (g == 0 ? (s < 21 ? s - 18 : 75 + s - 21)
+                            : (g == 1 ? (s < 21 ? 78 + s - 18 : 81 + s - 21) : (g == 2 ? 84 + s - 18 : 90 + s - 18)));
+}
+// Position j (0..95) of the layer-0 input -> feature index.
+SHD constexpr int guide_in_feature(int j) { return guide_slot_feature((j & 31) >> 3, 8 * (j >> 5) + (j & 7)); }
+// Position u (0..47) of the output layer in the training layout -> output index: lane group
+// g's twelve result values are the outputs of lobes 2g, 2g + 1 and two unused outputs.
+SHD constexpr int guide_head_output(int u)
+{
+    return 4 * (u >> 4) + (u & 3) < 10 ? 10 * ((u & 15) >> 2) + 4 * (u >> 4) + (u & 3)
+                                         : 40 + 2 * ((u & 15) >> 2) + 4 * (u >> 4) + (u & 3) - 10;
+}
 
 namespace {
 
 constexpr int WAVES = 4;
 constexpr int BLOCK = 64 * WAVES;
-constexpr int WSZ = 21504;  // BF16 weights
+constexpr int WSZ = GUIDE_B_OFF[0];        // weights
+constexpr int NBIAS = GUIDE_PARAMS - WSZ;  // biases
+constexpr int LPAD = 8;                    // BF16 padding per LDS row
+constexpr int FROW0 = GUIDE_IN + LPAD, FROW = GUIDE_HID + LPAD;
+SHD constexpr int fw_row(int l) { return l == 0 ? FROW0 : FROW; }
+SHD constexpr int fw_off(int l) { return l == 0 ? 0 : GUIDE_HID * FROW0 + (l - 1) * GUIDE_HID * FROW; }
+constexpr int FW_ELEMS = fw_off(4) + GUIDE_OUT * FROW;  // forward weights, rows = output units
+SHD constexpr int bw_off(int l) { return (l - 1) * GUIDE_HID * FROW; }
+constexpr int BW_ELEMS = 4 * GUIDE_HID * FROW;          // W_l^T (l = 1..4), rows = input units
+constexpr int IMG_BIAS = FW_ELEMS * 2;                  // byte offsets in an image
+constexpr int IMG_BW = IMG_BIAS + NBIAS * 4;
+constexpr int IMG_INFER_BYTES = IMG_BW;
+constexpr int IMG_TRAIN_BYTES = IMG_BW + BW_ELEMS * 2;
+static_assert(IMG_INFER_BYTES % 16 == 0 && IMG_TRAIN_BYTES % 16 == 0, "images are copied as 16-byte vectors");
 
-__device__ __forceinline__ s4v load_a_row(const uint16_t* row)
+// training: 2 tiles per wave, 128 samples per workgroup round, staged unit-major
+constexpr int TQ = 2;
+constexpr int TS = WAVES * 16 * TQ;
+constexpr int SROW = TS + LPAD;                     // staging row (BF16)
+constexpr int ST_DZ = GUIDE_IN;                     // first output-gradient row (inputs: 0..95)
+constexpr int ST_ELEMS = (GUIDE_IN + GUIDE_HID) * SROW;
+constexpr int PART_STRIDE = GUIDE_PARAMS + 4;       // per workgroup: gradient sums, weight sum, loss sum
+constexpr int INFER_Q = 4;                          // 16-query tiles per wave (inference)
+
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+typedef short s2v __attribute__((ext_vector_type(2)));
+typedef unsigned short us2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t cvt2(float a, float b)  // v_cvt_pk_bf16_f32 (nearest even)
 {
-    // 4 consecutive BF16 values (8 bytes, 8-byte aligned)
-    const uint2 u = *reinterpret_cast<const uint2*>(row);
-    s4v r;
-    r[0] = short(u.x & 0xFFFFu);
-    r[1] = short(u.x >> 16);
-    r[2] = short(u.y & 0xFFFFu);
-    r[3] = short(u.y >> 16);
-    return r;
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f2v){ a, b }, bf2v));
+}
+__device__ __forceinline__ uint32_t relu2(uint32_t x)  // max(x, 0) of both BF16 halves
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s2v, x), (s2v){ 0, 0 }));
+}
+__device__ __forceinline__ uint32_t nonzero_mask(uint32_t x)  // 0xFFFF per non-zero half
+{
+    us2v m = __builtin_elementwise_min(__builtin_bit_cast(us2v, x), (us2v){ 1, 1 });
+    return __builtin_bit_cast(uint32_t, m * (us2v){ 0xFFFF, 0xFFFF });
+}
+__device__ __forceinline__ f4v mfma(u4v a, u4v b, f4v c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a), __builtin_bit_cast(bf8v, b), c, 0, 0, 0);
+}
+__device__ __forceinline__ u4v lds16(const uint16_t* p) { return *reinterpret_cast<const u4v*>(p); }
+
+__device__ __forceinline__ void copy_image(uint8_t* lds, const uint8_t* img, int bytes)
+{
+    const uint4* src = reinterpret_cast<const uint4*>(img);
+    uint4* dst = reinterpret_cast<uint4*>(lds);
+    for (int i = threadIdx.x; i < bytes / 16; i += BLOCK) dst[i] = src[i];
 }
 
-__device__ __forceinline__ s4v gather4(const uint16_t* base, int stride)
+// The 24 input slots of lane group g for one sample, as three B operands.
+template <bool FEAT>
+__device__ __forceinline__ void load_inputs(const float* q9, const float* feat, size_t src, bool valid, int g,
+                                            u4v (&in)[3])
 {
-    s4v r;
-    r[0] = short(base[0]);
-    r[1] = short(base[stride]);
-    r[2] = short(base[2 * stride]);
-    r[3] = short(base[3 * stride]);
-    return r;
-}
-
-__device__ __forceinline__ void load_weights(uint16_t* sw, const uint16_t* w16)
-{
-    const uint4* src = reinterpret_cast<const uint4*>(w16);
-    uint4* dst = reinterpret_cast<uint4*>(sw);
-    for (int i = threadIdx.x; i < WSZ / 8; i += BLOCK) dst[i] = src[i];
-}
-
-// One dense layer for a 16-sample tile: act_out[16][N] = act_in[16][K] . W^T (+ bias, ReLU).
-template <int K, int N, bool RELU, bool LAST>
-__device__ __forceinline__ void dense(const uint16_t* act_in, const uint16_t* W, const float* bias,
-                                      uint16_t* act_out, float* out_f32, uint32_t row0, uint32_t valid_rows,
-                                      uint16_t* acts_g, int acts_off)
-{
-    const int lane = threadIdx.x & 63;
-    const int r = lane & 15, kg = lane >> 4;
-#pragma unroll
-    for (int nt = 0; nt < N / 16; ++nt)
+    float s[24];
+    if (FEAT)
     {
-        f4v acc = { 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
-        for (int ks = 0; ks < K / 16; ++ks)
-        {
-            const s4v a = load_a_row(act_in + r * K + ks * 16 + 4 * kg);
-            const s4v b = load_a_row(W + (nt * 16 + r) * K + ks * 16 + 4 * kg);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-        }
-        const int col = nt * 16 + r;
-        const float bb = bias[col];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-        {
-            const int row = 4 * kg + i;
-            float v = acc[i] + bb;
-            if (RELU) v = v > 0.0f ? v : 0.0f;
-            if (LAST)
-            {
-                if (uint32_t(row) < valid_rows) out_f32[size_t(row0 + row) * GUIDE_OUT + col] = v;
-            }
-            else
-            {
-                const uint16_t h = f2bf(v);
-                act_out[row * N + col] = h;
-                if (acts_g && uint32_t(row) < valid_rows)
-                    acts_g[size_t(row0 + row) * (4 * GUIDE_HID) + acts_off + col] = h;
-            }
-        }
+        for (int k = 0; k < 24; ++k) s[k] = valid ? feat[src * GUIDE_IN + guide_slot_feature(g, k)] : 0.0f;
     }
-}
-
-struct FwdArgs
-{
-    const uint16_t* w16;
-    const float* w32;   // biases read from here
-    const float* feat;  // M x 96 fp32
-    uint32_t M;
-    const uint32_t* M_dev;
-    float* out;         // M x 48
-    uint16_t* acts;     // optional M x 256
-};
-
-__global__ __launch_bounds__(BLOCK) void k_guide_fwd(FwdArgs A)
-{
-    __shared__ __align__(16) uint16_t sw[WSZ];
-    __shared__ float sb[GUIDE_PARAMS - 21504];
-    __shared__ __align__(16) uint16_t sa[WAVES][2][16 * GUIDE_IN];
-    load_weights(sw, A.w16);
-    for (int i = threadIdx.x; i < GUIDE_PARAMS - 21504; i += BLOCK) sb[i] = A.w32[21504 + i];
-    __syncthreads();
-    const uint32_t M = A.M_dev ? *A.M_dev : A.M;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint16_t* a0 = sa[wave][0];
-    uint16_t* a1 = sa[wave][1];
-    for (uint32_t base = blockIdx.x * (16 * WAVES); base < M; base += gridDim.x * 16 * WAVES)
+    else
     {
-        const uint32_t row0 = base + 16 * wave;
-        const uint32_t valid = row0 < M ? min(16u, M - row0) : 0u;
-        // features -> BF16 tile
-        for (int i = lane; i < 16 * GUIDE_IN; i += 64)
-        {
-            const int rr = i / GUIDE_IN;
-            a0[i] = uint32_t(rr) < valid ? f2bf(A.feat[size_t(row0 + rr) * GUIDE_IN + (i % GUIDE_IN)]) : 0;
-        }
-        __syncthreads();
-        dense<GUIDE_IN, GUIDE_HID, true, false>(a0, sw + guide_w_off(0), sb + 0, a1, nullptr, row0, valid, A.acts, 0);
-        __syncthreads();
-        dense<GUIDE_HID, GUIDE_HID, true, false>(a1, sw + guide_w_off(1), sb + 64, a0, nullptr, row0, valid, A.acts, 64);
-        __syncthreads();
-        dense<GUIDE_HID, GUIDE_HID, true, false>(a0, sw + guide_w_off(2), sb + 128, a1, nullptr, row0, valid, A.acts, 128);
-        __syncthreads();
-        dense<GUIDE_HID, GUIDE_HID, true, false>(a1, sw + guide_w_off(3), sb + 192, a0, nullptr, row0, valid, A.acts, 192);
-        __syncthreads();
-        dense<GUIDE_HID, GUIDE_OUT, false, true>(a0, sw + guide_w_off(4), sb + 256, nullptr, A.out, row0, valid, nullptr, 0);
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Inference with the encoding fused in: queries are (p01, n, wo), 9 floats.
-// Each wave carries 16 queries through the network in the transposed form
-// D[unit][query] = W[unit][:] . act[query][:], so a layer's MFMA result tile
-// (lane (r, kg) holds units 4 kg .. 4 kg + 3 of query r) is exactly the next
-// layer's B operand for one 16-wide k-step: activations never leave
-// registers.  Weights live in LDS with rows padded by 4 BF16 so the 16 rows a
-// wave reads at once fall in different banks.
-// ---------------------------------------------------------------------------
-constexpr int IPAD = 4;
-__host__ __device__ constexpr int infer_row(int l) { return (l == 0 ? GUIDE_IN : GUIDE_HID) + IPAD; }
-__host__ __device__ constexpr int infer_off(int l)
-{
-    return l == 0 ? 0 : GUIDE_HID * infer_row(0) + (l - 1) * GUIDE_HID * infer_row(1);
-}
-constexpr int INFER_WSZ = infer_off(4) + GUIDE_OUT * infer_row(4);
-
-struct InferArgs
-{
-    const uint16_t* w16;
-    const float* w32;
-    const float* q;         // M x 9
-    uint32_t M;
-    const uint32_t* M_dev;
-    float* out;             // M x 48
-};
-
-__device__ __forceinline__ uint32_t pack_bf2(float a, float b)
-{
-    return uint32_t(f2bf(a)) | (uint32_t(f2bf(b)) << 16);
-}
-
-__device__ __forceinline__ s4v as_s4v(uint32_t lo, uint32_t hi)
-{
-    s4v r;
-    r[0] = short(lo & 0xFFFFu);
-    r[1] = short(lo >> 16);
-    r[2] = short(hi & 0xFFFFu);
-    r[3] = short(hi >> 16);
-    return r;
-}
-
-// One layer: act_out tiles (N / 16) from act_in tiles (K / 16).
-template <int L, int K, int N, bool RELU>
-__device__ __forceinline__ void infer_layer(const uint16_t* sw, const float* sb, const s4v (&in)[K / 16],
-                                            f4v (&res)[N / 16], int r, int kg)
-{
-    const uint16_t* W = sw + infer_off(L);
-#pragma unroll
-    for (int nt = 0; nt < N / 16; ++nt)
-    {
-        f4v acc = { 0.0f, 0.0f, 0.0f, 0.0f };
-#pragma unroll
-        for (int ks = 0; ks < K / 16; ++ks)
-        {
-            const s4v a = load_a_row(W + (nt * 16 + r) * infer_row(L) + ks * 16 + 4 * kg);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, in[ks], acc, 0, 0, 0);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-        {
-            float v = acc[i] + sb[guide_b_off(L) - 21504 + nt * 16 + 4 * kg + i];
-            if (RELU) v = v > 0.0f ? v : 0.0f;
-            acc[i] = v;
-        }
-        res[nt] = acc;
-    }
-}
-
-template <int T>
-__device__ __forceinline__ void to_bf16_tiles(const f4v (&x)[T], s4v (&y)[T])
-{
-#pragma unroll
-    for (int t = 0; t < T; ++t) y[t] = as_s4v(pack_bf2(x[t][0], x[t][1]), pack_bf2(x[t][2], x[t][3]));
-}
-
-__global__ __launch_bounds__(BLOCK) void k_guide_infer(InferArgs A)
-{
-    __shared__ __align__(16) uint16_t sw[INFER_WSZ];
-    __shared__ float sb[GUIDE_PARAMS - 21504];
-    for (int l = 0; l < GUIDE_LAYERS; ++l)
-    {
-        const int K = guide_k(l), N = guide_n(l);
-        for (int i = threadIdx.x; i < N * K / 4; i += BLOCK)
-        {
-            const int row = (4 * i) / K, col = (4 * i) % K;
-            *reinterpret_cast<uint2*>(sw + infer_off(l) + row * infer_row(l) + col) =
-                *reinterpret_cast<const uint2*>(A.w16 + guide_w_off(l) + 4 * i);
-        }
-    }
-    for (int i = threadIdx.x; i < GUIDE_PARAMS - 21504; i += BLOCK) sb[i] = A.w32[21504 + i];
-    __syncthreads();
-    const uint32_t M = A.M_dev ? *A.M_dev : A.M;
-    const int lane = threadIdx.x & 63, r = lane & 15, kg = lane >> 4;
-    const uint32_t nwaves = gridDim.x * WAVES;
-    for (uint32_t tile = blockIdx.x * WAVES + (threadIdx.x >> 6); tile * 16u < M; tile += nwaves)
-    {
-        const uint32_t qi = tile * 16u + uint32_t(r);
         float p[3] = { 0, 0, 0 }, n[3] = { 0, 0, 0 }, wo[3] = { 0, 0, 0 };
-        if (qi < M)
+        if (valid)
         {
-            const float* qq = A.q + size_t(qi) * 9;
+            const float* qq = q9 + src * 9;
             p[0] = qq[0]; p[1] = qq[1]; p[2] = qq[2];
             n[0] = qq[3]; n[1] = qq[4]; n[2] = qq[5];
             wo[0] = qq[6]; wo[1] = qq[7]; wo[2] = qq[8];
         }
-        s4v h0[GUIDE_IN / 16];
+        // sin and cos of 2^k pi p as sin(2 pi frac(2^(k-1) p + c / 4)), as guide_feature
 #pragma unroll
-        for (int ks = 0; ks < GUIDE_IN / 16; ++ks)
+        for (int j = 0; j < 3; ++j)
         {
-            const int f = ks * 16 + 4 * kg;
-            h0[ks] = as_s4v(pack_bf2(guide_feature(f, p, n, wo), guide_feature(f + 1, p, n, wo)),
-                            pack_bf2(guide_feature(f + 2, p, n, wo), guide_feature(f + 3, p, n, wo)));
+            const float sc = ldexpf(1.0f, 3 * g + j - 1);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+            {
+                const float x = p[a] * sc, xc = x + 0.25f;
+                s[6 * j + 2 * a] = sin_rev(x - floorf(x));
+                s[6 * j + 2 * a + 1] = sin_rev(xc - floorf(xc));
+            }
         }
-        f4v t1[4], t2[4];
-        s4v h1[4], h2[4];
-        infer_layer<0, GUIDE_IN, GUIDE_HID, true>(sw, sb, h0, t1, r, kg);
-        to_bf16_tiles(t1, h1);
-        infer_layer<1, GUIDE_HID, GUIDE_HID, true>(sw, sb, h1, t2, r, kg);
-        to_bf16_tiles(t2, h2);
-        infer_layer<2, GUIDE_HID, GUIDE_HID, true>(sw, sb, h2, t1, r, kg);
-        to_bf16_tiles(t1, h1);
-        infer_layer<3, GUIDE_HID, GUIDE_HID, true>(sw, sb, h1, t2, r, kg);
-        to_bf16_tiles(t2, h2);
-        f4v o[GUIDE_OUT / 16];
-        infer_layer<4, GUIDE_HID, GUIDE_OUT, false>(sw, sb, h2, o, r, kg);
-        if (qi < M)
-        {
 #pragma unroll
-            for (int nt = 0; nt < GUIDE_OUT / 16; ++nt)
-                *reinterpret_cast<float4*>(A.out + size_t(qi) * GUIDE_OUT + nt * 16 + 4 * kg) =
-                    make_float4(o[nt][0], o[nt][1], o[nt][2], o[nt][3]);
+        for (int t = 0; t < 3; ++t)
+        {
+            s[18 + t] = g == 0 ? p[t] : (g == 1 ? wo[t] : 0.0f);
+            s[21 + t] = g == 0 ? n[t] : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int kb = 0; kb < 3; ++kb)
+        in[kb] = (u4v){ cvt2(s[8 * kb], s[8 * kb + 1]), cvt2(s[8 * kb + 2], s[8 * kb + 3]),
+                        cvt2(s[8 * kb + 4], s[8 * kb + 5]), cvt2(s[8 * kb + 6], s[8 * kb + 7]) };
+}
+
+// One layer for Q tiles: acc[q][nt] = bias + W (16nt.., :) . in[q].
+template <int Q, int KB, int NT>
+__device__ __forceinline__ void layer_fwd(const uint16_t* W, int row, const float* bias, const u4v (&in)[Q][KB],
+                                          f4v (&acc)[Q][NT], int r, int g)
+{
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+    {
+        const f4v b = *reinterpret_cast<const f4v*>(bias + 16 * nt + 4 * g);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[q][nt] = b;
+    }
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+        {
+            const u4v a = lds16(W + (16 * nt + r) * row + 32 * kb + 8 * g);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) acc[q][nt] = mfma(a, in[q][kb], acc[q][nt]);
+        }
+}
+
+// Hidden result tiles -> the next layer's B operands (ReLU, BF16).
+template <int Q>
+__device__ __forceinline__ void to_hidden(const f4v (&t)[Q][4], u4v (&h)[Q][2])
+{
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+            h[q][p] = (u4v){ relu2(cvt2(t[q][2 * p][0], t[q][2 * p][1])), relu2(cvt2(t[q][2 * p][2], t[q][2 * p][3])),
+                             relu2(cvt2(t[q][2 * p + 1][0], t[q][2 * p + 1][1])),
+                             relu2(cvt2(t[q][2 * p + 1][2], t[q][2 * p + 1][3])) };
+}
+
+// ---------------------------------------------------------------------------
+// Weight images from the FP32 master parameters.
+// ---------------------------------------------------------------------------
+__global__ void k_guide_pack(const float* w32, uint8_t* infer, uint8_t* train)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < FW_ELEMS)
+    {
+        int l = 0;
+        while (l < 4 && i >= fw_off(l + 1)) ++l;
+        const int loc = i - fw_off(l), u = loc / fw_row(l), j = loc % fw_row(l), K = guide_k(l);
+        float vi = 0.0f, vt = 0.0f;
+        if (j < K)
+        {
+            const int k = l == 0 ? guide_in_feature(j) : guide_pair_unit(j);
+            vi = w32[guide_w_off(l) + u * K + k];
+            vt = w32[guide_w_off(l) + (l == 4 ? guide_head_output(u) : u) * K + k];
+        }
+        reinterpret_cast<uint16_t*>(infer)[i] = f2bf(vi);
+        reinterpret_cast<uint16_t*>(train)[i] = f2bf(vt);
+    }
+    if (i < NBIAS)
+    {
+        const int l = i / GUIDE_HID, u = i - GUIDE_HID * l;
+        reinterpret_cast<float*>(infer + IMG_BIAS)[i] = w32[WSZ + i];
+        reinterpret_cast<float*>(train + IMG_BIAS)[i] = w32[WSZ + GUIDE_HID * l + (l == 4 ? guide_head_output(u) : u)];
+    }
+    if (i < BW_ELEMS)
+    {
+        // W_l^T for the backward pass: row = input unit k, column = position of the output unit
+        const int l = 1 + i / (GUIDE_HID * FROW), loc = i % (GUIDE_HID * FROW), k = loc / FROW, j = loc % FROW;
+        float v = 0.0f;
+        if (j < GUIDE_HID)
+        {
+            const int u = guide_pair_unit(j);
+            if (u < guide_n(l)) v = w32[guide_w_off(l) + (l == 4 ? guide_head_output(u) : u) * GUIDE_HID + k];
+        }
+        reinterpret_cast<uint16_t*>(train + IMG_BW)[i] = f2bf(v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Inference: queries (p01, n, wo; 9 floats) or features (96 floats) -> 48 outputs.
+// ---------------------------------------------------------------------------
+struct InferArgs
+{
+    const uint8_t* img;
+    const float* q;      // M x 9 (queries) ...
+    const float* feat;   // ... or M x 96 (features)
+    uint32_t M;
+    const uint32_t* M_dev;
+    float* out;          // M x 48
+};
+
+template <int Q, bool FEAT>
+__global__ __launch_bounds__(BLOCK) void k_guide_infer(InferArgs A)
+{
+    __shared__ __align__(16) uint8_t lds[IMG_INFER_BYTES];
+    copy_image(lds, A.img, IMG_INFER_BYTES);
+    __syncthreads();
+    const uint16_t* sw = reinterpret_cast<const uint16_t*>(lds);
+    const float* sb = reinterpret_cast<const float*>(lds + IMG_BIAS);
+    const uint32_t M = A.M_dev ? *A.M_dev : A.M;
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const uint32_t step = gridDim.x * WAVES * Q;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform loop bounds
+    for (uint32_t t0 = (blockIdx.x * WAVES + wave) * Q; t0 * 16u < M; t0 += step)
+    {
+        u4v in0[Q][3];
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+        {
+            const uint32_t qi = (t0 + q) * 16u + uint32_t(r);
+            load_inputs<FEAT>(A.q, A.feat, qi, qi < M, g, in0[q]);
+        }
+        f4v t[Q][4];
+        u4v h[Q][2];
+        layer_fwd<Q, 3, 4>(sw + fw_off(0), FROW0, sb, in0, t, r, g);
+        to_hidden(t, h);
+        layer_fwd<Q, 2, 4>(sw + fw_off(1), FROW, sb + GUIDE_HID, h, t, r, g);
+        to_hidden(t, h);
+        layer_fwd<Q, 2, 4>(sw + fw_off(2), FROW, sb + 2 * GUIDE_HID, h, t, r, g);
+        to_hidden(t, h);
+        layer_fwd<Q, 2, 4>(sw + fw_off(3), FROW, sb + 3 * GUIDE_HID, h, t, r, g);
+        to_hidden(t, h);
+        f4v o[Q][3];
+        layer_fwd<Q, 2, 3>(sw + fw_off(4), FROW, sb + 4 * GUIDE_HID, h, o, r, g);
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+        {
+            const uint32_t qi = (t0 + q) * 16u + uint32_t(r);
+            if (qi < M)
+#pragma unroll
+                for (int nt = 0; nt < 3; ++nt)
+                    *reinterpret_cast<float4*>(A.out + size_t(qi) * GUIDE_OUT + 16 * nt + 4 * g) =
+                        make_float4(o[q][nt][0], o[q][nt][1], o[q][nt][2], o[q][nt][3]);
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Training step, fused: encoding, forward, loss head and backward through the
-// layers for 16-query tiles, one tile per wave at a time, activations and
-// gradients in registers in the transposed tile layout of k_guide_infer.
-// Writes what the weight-gradient kernel reads: the BF16 features, the hidden
-// activations and the pre-activation gradients of every layer.
+// Training.
 // ---------------------------------------------------------------------------
 struct TrainArgs
 {
-    const uint16_t* w16;
-    const float* w32;
-    const float* q;        // M x 9
-    const float* dir;      // M x 3
-    const float* weight;   // M
-    const float* wsum;     // device: sum of the minibatch's weights
-    uint32_t M;
-    uint16_t* feat16;      // M x 96
-    uint16_t* acts;        // M x 256
-    uint16_t* dz;          // M x (48 + 256)
-    float* loss_sum;       // optional
+    const uint8_t* img;     // training image
+    const float* q;         // queries (9 floats per input) ...
+    const float* feat;      // ... or features (96)
+    const float* dir;       // 3 per input
+    const float* weight;    // 1 per input
+    uint32_t M;             // minibatch size; sample i is input off + i * stride
+    uint32_t off, stride;
+    float* part;            // gridDim.x x PART_STRIDE
 };
 
-__global__ __launch_bounds__(BLOCK) void k_guide_train_fused(TrainArgs A)
+// Loss head of one sample on its four lane groups: group g holds outputs 10g .. 10g + 9
+// (lobes 2g and 2g + 1) in oo[0..9].  Returns -log q(d) and writes scale * d(-log q)/d out
+// into gg in the same layout (gg[10], gg[11] = 0): the arithmetic of guide_nll_grad, with the
+// sums over lobes taken across the lane groups.
+__device__ __forceinline__ float head_grad(const float (&oo)[12], f3 d, float scale, float (&gg)[12])
 {
-    __shared__ __align__(16) uint16_t sw[INFER_WSZ];
-    __shared__ float sb[GUIDE_PARAMS - 21504];
-    __shared__ float sh[WAVES][16 * GUIDE_OUT];
-    for (int l = 0; l < GUIDE_LAYERS; ++l)
+    float amax = fmaxf(oo[0], oo[5]);
+    amax = fmaxf(amax, __shfl_xor(amax, 16));
+    amax = fmaxf(amax, __shfl_xor(amax, 32));
+    float pi[2], kap[2], lv[2], cth[2], len[2];
+    f3 mu[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) pi[t] = expf(oo[5 * t] - amax);
+    float sum = pi[0] + pi[1];
+    sum += __shfl_xor(sum, 16);
+    sum += __shfl_xor(sum, 32);
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
     {
-        const int K = guide_k(l), N = guide_n(l);
-        for (int i = threadIdx.x; i < N * K / 4; i += BLOCK)
+        pi[t] /= sum;
+        kap[t] = expf(clampk(oo[5 * t + 1]));
+        const f3 v = mk(oo[5 * t + 2], oo[5 * t + 3], oo[5 * t + 4]);
+        len[t] = length(v);
+        mu[t] = len[t] > 1e-6f ? v / len[t] : mk(0.0f, 0.0f, 1.0f);
+        cth[t] = dot(mu[t], d);
+        lv[t] = logf(max_s(pi[t], 1e-30f)) + vmf_log(kap[t], cth[t]);
+    }
+    float lmax = fmaxf(lv[0], lv[1]);
+    lmax = fmaxf(lmax, __shfl_xor(lmax, 16));
+    lmax = fmaxf(lmax, __shfl_xor(lmax, 32));
+    float s2 = expf(lv[0] - lmax) + expf(lv[1] - lmax);
+    s2 += __shfl_xor(s2, 16);
+    s2 += __shfl_xor(s2, 32);
+    const float logq = lmax + logf(s2);
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+    {
+        const float r = expf(lv[t] - logq), k = kap[t];
+        gg[5 * t] = scale * (pi[t] - r);
+        const float e2 = expf(-2.0f * k);
+        const float dlk = 1.0f / k - 2.0f * e2 / (1.0f - e2) + (cth[t] - 1.0f);
+        const float ok = oo[5 * t + 1];
+        gg[5 * t + 1] = ok > GUIDE_LOGK_MIN && ok < GUIDE_LOGK_MAX ? scale * (-r * dlk * k) : 0.0f;
+        if (len[t] > 1e-6f)
         {
-            const int row = (4 * i) / K, col = (4 * i) % K;
-            *reinterpret_cast<uint2*>(sw + infer_off(l) + row * infer_row(l) + col) =
-                *reinterpret_cast<const uint2*>(A.w16 + guide_w_off(l) + 4 * i);
+            const f3 dmu = d * (-r * k);
+            const f3 dv = (dmu - mu[t] * dot(mu[t], dmu)) / len[t];
+            gg[5 * t + 2] = scale * dv.x;
+            gg[5 * t + 3] = scale * dv.y;
+            gg[5 * t + 4] = scale * dv.z;
+        }
+        else
+            gg[5 * t + 2] = gg[5 * t + 3] = gg[5 * t + 4] = 0.0f;
+    }
+    gg[10] = gg[11] = 0.0f;
+    return -logq;
+}
+
+// Staging: one dword per unit holds the unit's values of the lane's two samples (tiles 0, 1);
+// samples 32 w + 2 r + q of the workgroup round sit at column 32 w + 2 r + q.
+__device__ __forceinline__ uint32_t pair_word(uint32_t a, uint32_t b, int half)
+{
+    return half ? ((a >> 16) | (b & 0xFFFF0000u)) : ((a & 0xFFFFu) | (b << 16));
+}
+
+// Result-tile operands x[q][p] (units 16 tile + 4g + i, tiles < NT) to rows row0 + unit.
+template <int NT>
+__device__ __forceinline__ void stage_tiles(uint32_t* st32, int row0, const u4v (&x)[TQ][2], int g, int col_dw)
+{
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+        {
+            const int tile = 2 * p + (e >> 2);
+            if (tile < NT)
+                st32[(row0 + 16 * tile + 4 * g + (e & 3)) * (SROW / 2) + col_dw] =
+                    pair_word(x[0][p][e >> 1], x[1][p][e >> 1], e & 1);
+        }
+}
+
+// Layer-0 inputs (positions 32 kb + 8g + e) to rows 0..95.
+__device__ __forceinline__ void stage_inputs(uint32_t* st32, const u4v (&x)[TQ][3], int g, int col_dw)
+{
+#pragma unroll
+    for (int kb = 0; kb < 3; ++kb)
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            st32[(32 * kb + 8 * g + e) * (SROW / 2) + col_dw] = pair_word(x[0][kb][e >> 1], x[1][kb][e >> 1], e & 1);
+}
+
+constexpr u4v kOnes = { 0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u };  // BF16 1.0
+
+// Weight gradient of a layer whose n-tile w this wave owns (layers 0..3): KT input tiles.
+template <int KT>
+__device__ __forceinline__ void dw_rows(const uint16_t* st, f4v (&acc)[KT], f4v& accb, int w, int r, int g)
+{
+#pragma unroll
+    for (int jb = 0; jb < TS / 32; ++jb)
+    {
+        const int col = 32 * jb + 8 * g;
+        const u4v a = lds16(st + (ST_DZ + 16 * w + r) * SROW + col);
+#pragma unroll
+        for (int tk = 0; tk < KT; ++tk) acc[tk] = mfma(a, lds16(st + (16 * tk + r) * SROW + col), acc[tk]);
+        accb = mfma(a, kOnes, accb);
+    }
+}
+
+// Output layer: this wave owns input tile w and the three output tiles (bias: output tile w).
+__device__ __forceinline__ void dw_out(const uint16_t* st, f4v (&acc)[3], f4v& accb, int w, int r, int g)
+{
+#pragma unroll
+    for (int jb = 0; jb < TS / 32; ++jb)
+    {
+        const int col = 32 * jb + 8 * g;
+        const u4v b = lds16(st + (16 * w + r) * SROW + col);
+#pragma unroll
+        for (int tn = 0; tn < 3; ++tn)
+        {
+            const u4v a = lds16(st + (ST_DZ + 16 * tn + r) * SROW + col);
+            acc[tn] = mfma(a, b, acc[tn]);
+            if (tn == w) accb = mfma(a, kOnes, accb);
         }
     }
-    for (int i = threadIdx.x; i < GUIDE_PARAMS - 21504; i += BLOCK) sb[i] = A.w32[21504 + i];
-    __syncthreads();
-    const uint32_t M = A.M;
-    const float wsum = *A.wsum;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, kg = lane >> 4;
-    float* hw = sh[wave];
-    constexpr int DZS = GUIDE_OUT + 4 * GUIDE_HID;
-    float loss = 0.0f;
-    const uint32_t nwaves = gridDim.x * WAVES;
-    for (uint32_t tile = blockIdx.x * WAVES + wave; tile * 16u < M; tile += nwaves)
-    {
-        const uint32_t qi = tile * 16u + uint32_t(r);
-        const bool valid = qi < M;
-        float p[3] = { 0, 0, 0 }, n[3] = { 0, 0, 0 }, wo[3] = { 0, 0, 0 };
-        if (valid)
+}
+
+// Backward through layer L: dZ_{L-1} = (W_L^T dZ_L) masked by the layer's input h > 0.
+template <int L>
+__device__ __forceinline__ void back(const uint16_t* swb, const u4v (&dz)[TQ][2], const u4v (&h)[TQ][2],
+                                     u4v (&out)[TQ][2], int r, int g)
+{
+    f4v da[TQ][4];
+#pragma unroll
+    for (int q = 0; q < TQ; ++q)
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) da[q][kt] = (f4v){ 0.0f, 0.0f, 0.0f, 0.0f };
+    const uint16_t* W = swb + bw_off(L);
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt)
         {
-            const float* qq = A.q + size_t(qi) * 9;
-            p[0] = qq[0]; p[1] = qq[1]; p[2] = qq[2];
-            n[0] = qq[3]; n[1] = qq[4]; n[2] = qq[5];
-            wo[0] = qq[6]; wo[1] = qq[7]; wo[2] = qq[8];
+            const u4v a = lds16(W + (16 * kt + r) * FROW + 32 * p + 8 * g);
+#pragma unroll
+            for (int q = 0; q < TQ; ++q) da[q][kt] = mfma(a, dz[q][p], da[q][kt]);
         }
-        // ---- forward ----
-        s4v h0[GUIDE_IN / 16];
 #pragma unroll
-        for (int ks = 0; ks < GUIDE_IN / 16; ++ks)
-        {
-            const int f = ks * 16 + 4 * kg;
-            const uint32_t lo = pack_bf2(guide_feature(f, p, n, wo), guide_feature(f + 1, p, n, wo));
-            const uint32_t hi = pack_bf2(guide_feature(f + 2, p, n, wo), guide_feature(f + 3, p, n, wo));
-            h0[ks] = as_s4v(lo, hi);
-            if (valid) *reinterpret_cast<uint2*>(A.feat16 + size_t(qi) * GUIDE_IN + f) = make_uint2(lo, hi);
-        }
-        f4v t[4];
-        s4v h[4][4];  // hidden layer outputs (post-ReLU, BF16), layer-major
-        infer_layer<0, GUIDE_IN, GUIDE_HID, true>(sw, sb, h0, t, r, kg);
-        to_bf16_tiles(t, h[0]);
-        infer_layer<1, GUIDE_HID, GUIDE_HID, true>(sw, sb, h[0], t, r, kg);
-        to_bf16_tiles(t, h[1]);
-        infer_layer<2, GUIDE_HID, GUIDE_HID, true>(sw, sb, h[1], t, r, kg);
-        to_bf16_tiles(t, h[2]);
-        infer_layer<3, GUIDE_HID, GUIDE_HID, true>(sw, sb, h[2], t, r, kg);
-        to_bf16_tiles(t, h[3]);
-        if (valid)
-        {
+    for (int q = 0; q < TQ; ++q)
 #pragma unroll
-            for (int l = 0; l < 4; ++l)
+        for (int p = 0; p < 2; ++p)
 #pragma unroll
-                for (int nt = 0; nt < 4; ++nt)
-                {
-                    const s4v v = h[l][nt];
-                    const uint32_t lo = uint32_t(uint16_t(v[0])) | (uint32_t(uint16_t(v[1])) << 16);
-                    const uint32_t hi = uint32_t(uint16_t(v[2])) | (uint32_t(uint16_t(v[3])) << 16);
-                    *reinterpret_cast<uint2*>(A.acts + size_t(qi) * (4 * GUIDE_HID) + l * GUIDE_HID + nt * 16 + 4 * kg) =
-                        make_uint2(lo, hi);
-                }
-        }
-        f4v o[GUIDE_OUT / 16];
-        infer_layer<4, GUIDE_HID, GUIDE_OUT, false>(sw, sb, h[3], o, r, kg);
-        // ---- loss head: one lane per query ----
-#pragma unroll
-        for (int nt = 0; nt < GUIDE_OUT / 16; ++nt)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) hw[r * GUIDE_OUT + nt * 16 + 4 * kg + i] = o[nt][i];
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (kg == 0)
-        {
-            float g[GUIDE_OUT];
-            if (valid)
+            for (int e = 0; e < 4; ++e)
             {
-                const float scale = wsum > 0.0f ? A.weight[qi] / wsum : 0.0f;
-                const f3 d = mk(A.dir[3 * size_t(qi)], A.dir[3 * size_t(qi) + 1], A.dir[3 * size_t(qi) + 2]);
-                loss += guide_nll_grad(hw + r * GUIDE_OUT, d, scale, g) * scale;
+                const int kt = 2 * p + (e >> 1), i0 = 2 * (e & 1);
+                out[q][p][e] = cvt2(da[q][kt][i0], da[q][kt][i0 + 1]) & nonzero_mask(h[q][p][e]);
             }
-            else
-                for (int k = 0; k < GUIDE_OUT; ++k) g[k] = 0.0f;
-            for (int k = 0; k < GUIDE_OUT; ++k) hw[r * GUIDE_OUT + k] = g[k];
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        // dZ5 in the tile layout, rounded to BF16 as stored
-        s4v d5[GUIDE_OUT / 16];
-#pragma unroll
-        for (int nt = 0; nt < GUIDE_OUT / 16; ++nt)
-        {
-            const float* gp = hw + r * GUIDE_OUT + nt * 16 + 4 * kg;
-            const uint32_t lo = pack_bf2(gp[0], gp[1]), hi = pack_bf2(gp[2], gp[3]);
-            d5[nt] = as_s4v(lo, hi);
-            if (valid) *reinterpret_cast<uint2*>(A.dz + size_t(qi) * DZS + nt * 16 + 4 * kg) = make_uint2(lo, hi);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        // ---- backward through layers 4 .. 1: dA[k][q] = sum_n W[n][k] dZ[n][q] ----
-        s4v dcur[4];
-        s4v dnext[4];
-        auto back = [&](const uint16_t* W, int rowlen, int ksteps, const s4v* din, int hl) {
-#pragma unroll
-            for (int kt = 0; kt < 4; ++kt)
-            {
-                f4v acc = { 0.0f, 0.0f, 0.0f, 0.0f };
-                for (int ks = 0; ks < ksteps; ++ks)
-                {
-                    // A[m = input unit k][kdim = output unit n] = W[n][k]
-                    const uint16_t* wp = W + (ks * 16 + 4 * kg) * rowlen + kt * 16 + r;
-                    s4v a;
-                    a[0] = short(wp[0]);
-                    a[1] = short(wp[rowlen]);
-                    a[2] = short(wp[2 * rowlen]);
-                    a[3] = short(wp[3 * rowlen]);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, din[ks], acc, 0, 0, 0);
-                }
-                // lane holds dA for input units 16 kt + 4 kg + i of query r; ReLU mask of the
-                // hidden layer that produced them
-                const s4v act = h[hl][kt];
-                float v[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) v[i] = bf2f(uint16_t(act[i])) > 0.0f ? acc[i] : 0.0f;
-                const uint32_t lo = pack_bf2(v[0], v[1]), hi = pack_bf2(v[2], v[3]);
-                dnext[kt] = as_s4v(lo, hi);
-                if (valid)
-                    *reinterpret_cast<uint2*>(A.dz + size_t(qi) * DZS + GUIDE_OUT + hl * GUIDE_HID + kt * 16 + 4 * kg) =
-                        make_uint2(lo, hi);
-            }
-        };
-        back(sw + infer_off(4), infer_row(4), GUIDE_OUT / 16, d5, 3);
-        for (int k = 0; k < 4; ++k) dcur[k] = dnext[k];
-        back(sw + infer_off(3), infer_row(3), GUIDE_HID / 16, dcur, 2);
-        for (int k = 0; k < 4; ++k) dcur[k] = dnext[k];
-        back(sw + infer_off(2), infer_row(2), GUIDE_HID / 16, dcur, 1);
-        for (int k = 0; k < 4; ++k) dcur[k] = dnext[k];
-        back(sw + infer_off(1), infer_row(1), GUIDE_HID / 16, dcur, 0);
-    }
-    if (A.loss_sum)
-    {
-        for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off);
-        if (lane == 0) atomicAdd(A.loss_sum, loss);
-    }
 }
 
-// Loss head: dZ5 = d(-w log q)/d out, scaled by 1 / sum(w).
-__global__ void k_guide_wsum(const float* weight, uint32_t M, float* scratch)
+template <int L, int KT>
+__device__ __forceinline__ void put_rows(float* P, const f4v (&acc)[KT], const f4v& accb, int w, int r, int g)
 {
-    float s = 0.0f;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) s += weight[i];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&scratch[0], s);
-}
-
-// wsum: the minibatch's weight sum (device); loss_sum, when set, accumulates the
-// weighted mean negative log-likelihood.
-__global__ void k_guide_head(const float* out, const float* dir, const float* weight, uint32_t M,
-                             const float* wsum_p, float* loss_sum, uint16_t* dz)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    float nll = 0.0f;
-    if (i < M)
-    {
-        const float wsum = *wsum_p;
-        const float scale = wsum > 0.0f ? weight[i] / wsum : 0.0f;
-        float g[GUIDE_OUT];
-        nll = guide_nll_grad(out + size_t(i) * GUIDE_OUT, load3c(dir, i), scale, g) * scale;
-        uint16_t* d = dz + size_t(i) * (GUIDE_OUT + 4 * GUIDE_HID);
-        for (int k = 0; k < GUIDE_OUT; ++k) d[k] = f2bf(g[k]);
-    }
-    if (!loss_sum) return;
-    for (int off = 32; off > 0; off >>= 1) nll += __shfl_down(nll, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss_sum, nll);
-}
-
-__global__ void k_guide_zero(float* p, uint32_t n)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0f;
-}
-
-// Backward through the layers for 16-sample tiles: dA_{l-1} = dZ_l . W_l, dZ_{l-1} = dA (A > 0).
-__global__ __launch_bounds__(BLOCK) void k_guide_bwd_data(const uint16_t* w16, const uint16_t* acts, uint16_t* dz,
-                                                          uint32_t M)
-{
-    __shared__ __align__(16) uint16_t sw[WSZ];
-    __shared__ __align__(16) uint16_t sd[WAVES][2][16 * GUIDE_HID];
-    load_weights(sw, w16);
-    __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int r = lane & 15, kg = lane >> 4;
-    constexpr int DZS = GUIDE_OUT + 4 * GUIDE_HID;
-    for (uint32_t base = blockIdx.x * (16 * WAVES); base < M; base += gridDim.x * 16 * WAVES)
-    {
-        const uint32_t row0 = base + 16 * wave;
-        const uint32_t valid = row0 < M ? min(16u, M - row0) : 0u;
-        uint16_t* cur = sd[wave][0];
-        uint16_t* nxt = sd[wave][1];
-        for (int i = lane; i < 16 * GUIDE_OUT; i += 64)
-        {
-            const int rr = i / GUIDE_OUT;
-            cur[i] = uint32_t(rr) < valid ? dz[size_t(row0 + rr) * DZS + (i % GUIDE_OUT)] : 0;
-        }
-        __syncthreads();
-        // layer l (1-based 5..2): in width Kl = 64, out width Nl
+    constexpr int K = GUIDE_K[L];
 #pragma unroll
-        for (int l = 4; l >= 1; --l)
-        {
-            const int N = guide_n(l);          // gradient width of dZ_l
-            const uint16_t* W = sw + guide_w_off(l);  // W[n][k], k = 64
+    for (int tk = 0; tk < KT; ++tk)
 #pragma unroll
-            for (int nt = 0; nt < GUIDE_HID / 16; ++nt)
-            {
-                f4v acc = { 0.0f, 0.0f, 0.0f, 0.0f };
-                for (int ks = 0; ks < N / 16; ++ks)
-                {
-                    const s4v a = load_a_row(cur + r * N + ks * 16 + 4 * kg);
-                    // B[k = output unit][n = input unit] = W[k][n]
-                    const s4v b = gather4(W + (ks * 16 + 4 * kg) * GUIDE_HID + nt * 16 + r, GUIDE_HID);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-                }
-                const int col = nt * 16 + r;
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-                {
-                    const int row = 4 * kg + i;
-                    float v = 0.0f;
-                    if (uint32_t(row) < valid)
-                    {
-                        const uint16_t act = acts[size_t(row0 + row) * (4 * GUIDE_HID) + (l - 1) * GUIDE_HID + col];
-                        v = bf2f(act) > 0.0f ? acc[i] : 0.0f;
-                    }
-                    const uint16_t h = f2bf(v);
-                    nxt[row * GUIDE_HID + col] = h;
-                    if (uint32_t(row) < valid)
-                        dz[size_t(row0 + row) * DZS + GUIDE_OUT + (l - 1) * GUIDE_HID + col] = h;
-                }
-            }
-            __syncthreads();
-            uint16_t* t = cur;
-            cur = nxt;
-            nxt = t;
-        }
-    }
-}
-
-// dW_l[n][k] = sum_s dZ_l[s][n] A_{l-1}[s][k], db_l[n] = sum_s dZ_l[s][n] over a chunk of
-// samples per workgroup (blockIdx.y = layer); added to grad with one atomic per value.
-// Weight and bias gradients of one layer over a sample range, written as
-// this workgroup's partial sums (no atomics; k_guide_reduce adds the partials
-// in a fixed order).  64 samples are staged per round; each wave owns up to
-// six 16 x 16 output tiles and runs four MFMA k-steps (16 samples each) per
-// tile and round.
-constexpr int WG_MAX = 64;  // partial-sum slots
-constexpr int WG_STEP = 64;
-constexpr int SIN_LD = GUIDE_IN + 8, SDZ_LD = GUIDE_HID + 8;
-
-__device__ __forceinline__ s4v gather4_ld(const uint16_t* base, int ld)
-{
-    s4v r;
-    r[0] = short(base[0]);
-    r[1] = short(base[ld]);
-    r[2] = short(base[2 * ld]);
-    r[3] = short(base[3 * ld]);
-    return r;
-}
-
-__global__ __launch_bounds__(BLOCK) void k_guide_bwd_weight(const float* feat, const uint16_t* feat16,
-                                                            const uint16_t* acts, const uint16_t* dz,
-                                                            uint32_t M, uint32_t chunk, float* part)
-{
-    const int layer = blockIdx.y;
-    const int K = guide_k(layer), N = guide_n(layer);
-    __shared__ __align__(16) uint16_t s_in[WG_STEP * SIN_LD];
-    __shared__ __align__(16) uint16_t s_dz[WG_STEP * SDZ_LD];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int r = lane & 15, kg = lane >> 4;
-    constexpr int DZS = GUIDE_OUT + 4 * GUIDE_HID;
-    const int ntile_k = K / 16, ntiles = (N / 16) * ntile_k;
-    f4v acc[6];
-    for (int t = 0; t < 6; ++t) acc[t] = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
-    float bacc = 0.0f;
-    const uint32_t s0 = blockIdx.x * chunk;
-    const uint32_t s1 = min(M, s0 + chunk);
-    const int dz_off = layer == 4 ? 0 : GUIDE_OUT + layer * GUIDE_HID;
-    // rows of the layer's input and of its output gradient, moved as 8-BF16 vectors
-    const int in_vec = K / 8, dz_vec = N / 8;
-    for (uint32_t sb = s0; sb < s1; sb += WG_STEP)
-    {
-        __syncthreads();
-        for (int i = threadIdx.x; i < WG_STEP * in_vec; i += BLOCK)
-        {
-            const int rr = i / in_vec, c = (i - rr * in_vec) * 8;
-            const uint32_t smp = sb + rr;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (smp < s1)
-            {
-                if (layer > 0)
-                    v = *reinterpret_cast<const uint4*>(acts + size_t(smp) * (4 * GUIDE_HID) + (layer - 1) * GUIDE_HID + c);
-                else if (feat16)
-                    v = *reinterpret_cast<const uint4*>(feat16 + size_t(smp) * GUIDE_IN + c);
-                else
-                {
-                    const float* f = feat + size_t(smp) * GUIDE_IN + c;
-                    v.x = uint32_t(f2bf(f[0])) | (uint32_t(f2bf(f[1])) << 16);
-                    v.y = uint32_t(f2bf(f[2])) | (uint32_t(f2bf(f[3])) << 16);
-                    v.z = uint32_t(f2bf(f[4])) | (uint32_t(f2bf(f[5])) << 16);
-                    v.w = uint32_t(f2bf(f[6])) | (uint32_t(f2bf(f[7])) << 16);
-                }
-            }
-            *reinterpret_cast<uint4*>(&s_in[rr * SIN_LD + c]) = v;
-        }
-        for (int i = threadIdx.x; i < WG_STEP * dz_vec; i += BLOCK)
-        {
-            const int rr = i / dz_vec, c = (i - rr * dz_vec) * 8;
-            const uint32_t smp = sb + rr;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (smp < s1) v = *reinterpret_cast<const uint4*>(dz + size_t(smp) * DZS + dz_off + c);
-            *reinterpret_cast<uint4*>(&s_dz[rr * SDZ_LD + c]) = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int t = 0; t < 6; ++t)
-        {
-            const int tile = wave + WAVES * t;
-            if (tile >= ntiles) break;
-            const int tn = tile / ntile_k, tk = tile - tn * ntile_k;
-            // A[m = n unit][kdim = sample] = dZ[s][n]; B[kdim = sample][col = k unit] = in[s][k]
-#pragma unroll
-            for (int ks = 0; ks < WG_STEP / 16; ++ks)
-            {
-                const s4v a = gather4_ld(&s_dz[(16 * ks + 4 * kg) * SDZ_LD + tn * 16 + r], SDZ_LD);
-                const s4v b = gather4_ld(&s_in[(16 * ks + 4 * kg) * SIN_LD + tk * 16 + r], SIN_LD);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-            }
-        }
-        if (threadIdx.x < N)
-            for (int rr = 0; rr < WG_STEP; ++rr) bacc += bf2f(s_dz[rr * SDZ_LD + threadIdx.x]);
-    }
-    float* out = part + size_t(blockIdx.x) * GUIDE_PARAMS;
-    for (int t = 0; t < 6; ++t)
-    {
-        const int tile = wave + WAVES * t;
-        if (tile >= ntiles) break;
-        const int tn = tile / ntile_k, tk = tile - tn * ntile_k;
         for (int i = 0; i < 4; ++i)
         {
-            const int n = tn * 16 + 4 * kg + i;
-            const int k = tk * 16 + r;
-            out[guide_w_off(layer) + n * K + k] = acc[t][i];
+            const int kp = 16 * tk + r;
+            P[guide_w_off(L) + (16 * w + 4 * g + i) * K + (L == 0 ? guide_in_feature(kp) : kp)] = acc[tk][i];
         }
+    if (r == 0)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) P[guide_b_off(L) + 16 * w + 4 * g + i] = accb[i];
+}
+
+template <bool FEAT>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_guide_train(TrainArgs A)
+{
+    __shared__ __align__(16) uint8_t lds[IMG_TRAIN_BYTES];
+    __shared__ __align__(16) uint16_t st[ST_ELEMS];
+    __shared__ float red[2 * WAVES];
+    copy_image(lds, A.img, IMG_TRAIN_BYTES);
+    __syncthreads();
+    const uint16_t* sw = reinterpret_cast<const uint16_t*>(lds);
+    const float* sb = reinterpret_cast<const float*>(lds + IMG_BIAS);
+    const uint16_t* swb = reinterpret_cast<const uint16_t*>(lds + IMG_BW);
+    uint32_t* st32 = reinterpret_cast<uint32_t*>(st);
+    // the wave index in a scalar register: branches on it (dw_out) must be scalar branches,
+    // since an MFMA under a partial EXEC mask still runs on every lane
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const int col_dw = 16 * w + r;
+    const f4v z = { 0.0f, 0.0f, 0.0f, 0.0f };
+    f4v aw0[6], aw1[4], aw2[4], aw3[4], aw4[3], ab[5];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) aw0[k] = z;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) aw1[k] = aw2[k] = aw3[k] = z;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) aw4[k] = z;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) ab[k] = z;
+    float wsum = 0.0f, lsum = 0.0f;
+    const uint32_t rounds = (A.M + TS - 1) / TS;
+    for (uint32_t c = blockIdx.x; c < rounds; c += gridDim.x)
+    {
+        // ---- forward, this wave's 32 samples ----
+        u4v in0[TQ][3];
+        f3 dq[TQ];
+        float wq[TQ];
+        bool valid[TQ];
+#pragma unroll
+        for (int q = 0; q < TQ; ++q)
+        {
+            const uint32_t i = c * TS + 32u * w + 2u * r + q;
+            valid[q] = i < A.M;
+            const size_t src = size_t(A.off) + size_t(i) * A.stride;
+            load_inputs<FEAT>(A.q, A.feat, src, valid[q], g, in0[q]);
+            dq[q] = valid[q] ? load3c(A.dir, uint32_t(src)) : mk(0.0f, 0.0f, 0.0f);
+            wq[q] = valid[q] ? A.weight[src] : 0.0f;
+        }
+        f4v t[TQ][4];
+        u4v h0[TQ][2], h1[TQ][2], h2[TQ][2], h3[TQ][2];
+        layer_fwd<TQ, 3, 4>(sw + fw_off(0), FROW0, sb, in0, t, r, g);
+        to_hidden(t, h0);
+        layer_fwd<TQ, 2, 4>(sw + fw_off(1), FROW, sb + GUIDE_HID, h0, t, r, g);
+        to_hidden(t, h1);
+        layer_fwd<TQ, 2, 4>(sw + fw_off(2), FROW, sb + 2 * GUIDE_HID, h1, t, r, g);
+        to_hidden(t, h2);
+        layer_fwd<TQ, 2, 4>(sw + fw_off(3), FROW, sb + 3 * GUIDE_HID, h2, t, r, g);
+        to_hidden(t, h3);
+        f4v o[TQ][3];
+        layer_fwd<TQ, 2, 3>(sw + fw_off(4), FROW, sb + 4 * GUIDE_HID, h3, o, r, g);
+        // ---- loss head: dZ of the output layer (weight w_i; 1 / sum w comes in the reduction) ----
+        u4v dz[TQ][2], dn[TQ][2];
+#pragma unroll
+        for (int q = 0; q < TQ; ++q)
+        {
+            float oo[12], gg[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) oo[k] = o[q][k >> 2][k & 3];
+            const float nll = head_grad(oo, dq[q], wq[q], gg);
+            if (g == 0 && valid[q])
+            {
+                wsum += wq[q];
+                lsum += wq[q] * nll;
+            }
+            dz[q][0] = (u4v){ cvt2(gg[0], gg[1]), cvt2(gg[2], gg[3]), cvt2(gg[4], gg[5]), cvt2(gg[6], gg[7]) };
+            dz[q][1] = (u4v){ cvt2(gg[8], gg[9]), cvt2(gg[10], gg[11]), 0u, 0u };
+        }
+        // ---- layer 4 ----
+        __syncthreads();  // the previous round's layer-0 gradient has read the staging buffer
+        stage_tiles<4>(st32, 0, h3, g, col_dw);
+        stage_tiles<3>(st32, ST_DZ, dz, g, col_dw);
+        __syncthreads();
+        dw_out(st, aw4, ab[4], w, r, g);
+        back<4>(swb, dz, h3, dn, r, g);
+        // ---- layers 3, 2, 1 ----
+        __syncthreads();
+        stage_tiles<4>(st32, 0, h2, g, col_dw);
+        stage_tiles<4>(st32, ST_DZ, dn, g, col_dw);
+        __syncthreads();
+        dw_rows<4>(st, aw3, ab[3], w, r, g);
+        back<3>(swb, dn, h2, dz, r, g);
+        __syncthreads();
+        stage_tiles<4>(st32, 0, h1, g, col_dw);
+        stage_tiles<4>(st32, ST_DZ, dz, g, col_dw);
+        __syncthreads();
+        dw_rows<4>(st, aw2, ab[2], w, r, g);
+        back<2>(swb, dz, h1, dn, r, g);
+        __syncthreads();
+        stage_tiles<4>(st32, 0, h0, g, col_dw);
+        stage_tiles<4>(st32, ST_DZ, dn, g, col_dw);
+        __syncthreads();
+        dw_rows<4>(st, aw1, ab[1], w, r, g);
+        back<1>(swb, dn, h0, dz, r, g);
+        // ---- layer 0 ----
+        __syncthreads();
+        stage_inputs(st32, in0, g, col_dw);
+        stage_tiles<4>(st32, ST_DZ, dz, g, col_dw);
+        __syncthreads();
+        dw_rows<6>(st, aw0, ab[0], w, r, g);
     }
-    if (threadIdx.x < N) out[guide_b_off(layer) + threadIdx.x] = bacc;
+    // ---- this workgroup's sums ----
+    float* P = A.part + size_t(blockIdx.x) * PART_STRIDE;
+    put_rows<0, 6>(P, aw0, ab[0], w, r, g);
+    put_rows<1, 4>(P, aw1, ab[1], w, r, g);
+    put_rows<2, 4>(P, aw2, ab[2], w, r, g);
+    put_rows<3, 4>(P, aw3, ab[3], w, r, g);
+#pragma unroll
+    for (int tn = 0; tn < 3; ++tn)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            P[guide_w_off(4) + guide_head_output(16 * tn + 4 * g + i) * GUIDE_HID + 16 * w + r] = aw4[tn][i];
+    if (w < 3 && r == 0)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) P[guide_b_off(4) + guide_head_output(16 * w + 4 * g + i)] = ab[4][i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        wsum += __shfl_xor(wsum, off);
+        lsum += __shfl_xor(lsum, off);
+    }
+    if (lane == 0)
+    {
+        red[w] = wsum;
+        red[WAVES + w] = lsum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        float a = 0.0f, b = 0.0f;
+        for (int k = 0; k < WAVES; ++k)
+        {
+            a += red[k];
+            b += red[WAVES + k];
+        }
+        P[GUIDE_PARAMS] = a;
+        P[GUIDE_PARAMS + 1] = b;
+    }
 }
 
 struct AdamArgs
 {
     float* w32;
-    uint16_t* w16;
     float* m;
     float* v;
     float lr, b1, b2, eps, c1, c2;
 };
 
-// Sum of the partial gradients in slot order; then, when apply is set, the
-// Adam step with FP32 master weights and the BF16 copy the kernels read.
-__global__ void k_guide_reduce(const float* part, uint32_t G, float* grad, int apply, AdamArgs A)
+// Sum of the workgroup sums in workgroup order, divided by the minibatch's weight sum; then,
+// when apply is set, the Adam step on the FP32 master weights.  Thread GUIDE_PARAMS writes
+// scratch[0] = weight sum, scratch[1] = weighted mean negative log-likelihood.
+__global__ void k_guide_reduce(const float* part, uint32_t G, float* grad, float* scratch, int apply, AdamArgs A)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= uint32_t(GUIDE_PARAMS)) return;
-    float g = 0.0f;
-    for (uint32_t k = 0; k < G; ++k) g += part[size_t(k) * GUIDE_PARAMS + i];
-    grad[i] = g;
+    if (i > uint32_t(GUIDE_PARAMS)) return;
+    float ws = 0.0f;
+    for (uint32_t k = 0; k < G; ++k) ws += part[size_t(k) * PART_STRIDE + GUIDE_PARAMS];
+    const float inv = ws > 0.0f ? 1.0f / ws : 0.0f;
+    if (i == uint32_t(GUIDE_PARAMS))
+    {
+        float l = 0.0f;
+        for (uint32_t k = 0; k < G; ++k) l += part[size_t(k) * PART_STRIDE + GUIDE_PARAMS + 1];
+        scratch[0] = ws;
+        scratch[1] = l * inv;
+        return;
+    }
+    float s = 0.0f;
+    for (uint32_t k = 0; k < G; ++k) s += part[size_t(k) * PART_STRIDE + i];
+    const float gv = s * inv;
+    grad[i] = gv;
     if (!apply) return;
-    const float mm = A.b1 * A.m[i] + (1.0f - A.b1) * g;
-    const float vv = A.b2 * A.v[i] + (1.0f - A.b2) * g * g;
+    const float mm = A.b1 * A.m[i] + (1.0f - A.b1) * gv;
+    const float vv = A.b2 * A.v[i] + (1.0f - A.b2) * gv * gv;
     A.m[i] = mm;
     A.v[i] = vv;
-    const float p = A.w32[i] - A.lr * (mm / A.c1) / (sqrtf(vv / A.c2) + A.eps);
-    A.w32[i] = p;
-    if (i < uint32_t(WSZ)) A.w16[i] = f2bf(p);
-}
-
-__global__ void k_guide_sync16(const float* w32, uint16_t* w16)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < uint32_t(WSZ)) w16[i] = f2bf(w32[i]);
+    A.w32[i] = A.w32[i] - A.lr * (mm / A.c1) / (sqrtf(vv / A.c2) + A.eps);
 }
 
 // Deterministic initialisation: He-uniform weights, zero biases; the last
@@ -773,154 +723,174 @@ inline uint32_t nb(uint32_t n, uint32_t b) { return n ? (n + b - 1) / b : 1; }
 
 } // namespace
 
+struct GuideNet
+{
+    float* w32 = nullptr;        // master parameters (GUIDE_PARAMS)
+    uint8_t* img_infer = nullptr; // LDS images (k_guide_pack)
+    uint8_t* img_train = nullptr;
+    float* m = nullptr;
+    float* v = nullptr;
+    float* grad = nullptr;
+    float* part = nullptr;       // G_max x PART_STRIDE workgroup sums
+    float* scratch = nullptr;    // [0] weight sum, [1] loss of the last step
+    uint32_t G_max = 0;
+    uint32_t infer_blocks[2] = { 0, 0 };  // resident inference workgroups per CU (queries, features)
+    bool ready = false;          // every buffer above is allocated and initialised
+    uint32_t step = 0;
+    float lr = 1e-3f;
+};
+
 GuideNet* guide_of(Context* ctx)
 {
     if (!ctx->guide) ctx->guide = new GuideNet;
     return ctx->guide;
 }
 
+static void guide_pack(GuideNet* g, hipStream_t s)
+{
+    k_guide_pack<<<nb(FW_ELEMS, 256), 256, 0, s>>>(g->w32, g->img_infer, g->img_train);
+}
+
+// `cap` (training samples per step) is accepted for the callers' sizing; the network keeps no
+// per-sample buffers.
 int guide_ensure(Context* ctx, uint32_t cap)
 {
+    (void)cap;
     GuideNet* g = guide_of(ctx);
-    if (!g->ready)
-    {
-        // allocated in full or not at all (a failure part-way frees what it got, so the next
-        // call starts again instead of using a half-allocated network)
-        auto reset = [g]() {
-            for (void** p : { (void**)&g->w32, (void**)&g->w16, (void**)&g->m, (void**)&g->v, (void**)&g->grad,
-                              (void**)&g->part, (void**)&g->scratch })
-            {
-                if (*p) (void)hipFree(*p);
-                *p = nullptr;
-            }
-        };
-        hipError_t e = hipSuccess;
-        if (e == hipSuccess) e = hipMalloc(&g->w32, GUIDE_PARAMS * 4);
-        if (e == hipSuccess) e = hipMalloc(&g->w16, WSZ * 2);
-        if (e == hipSuccess) e = hipMalloc(&g->m, GUIDE_PARAMS * 4);
-        if (e == hipSuccess) e = hipMalloc(&g->v, GUIDE_PARAMS * 4);
-        if (e == hipSuccess) e = hipMalloc(&g->grad, GUIDE_PARAMS * 4);
-        if (e == hipSuccess) e = hipMalloc(&g->part, size_t(WG_MAX) * GUIDE_PARAMS * 4);
-        if (e == hipSuccess) e = hipMalloc(&g->scratch, 64);
-        std::vector<float> p;
-        init_params(p, 1);
-        if (e == hipSuccess) e = hipMemcpy(g->w32, p.data(), GUIDE_PARAMS * 4, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemset(g->m, 0, GUIDE_PARAMS * 4);
-        if (e == hipSuccess) e = hipMemset(g->v, 0, GUIDE_PARAMS * 4);
-        if (e == hipSuccess)
+    if (g->ready) return APG_OK;
+    // allocated in full or not at all (a failure part-way frees what it got, so the next call
+    // starts again instead of using a half-allocated network)
+    auto reset = [g]() {
+        for (void** p : { (void**)&g->w32, (void**)&g->img_infer, (void**)&g->img_train, (void**)&g->m, (void**)&g->v,
+                          (void**)&g->grad, (void**)&g->part, (void**)&g->scratch })
         {
-            k_guide_sync16<<<nb(WSZ, 256), 256>>>(g->w32, g->w16);
-            e = hipGetLastError();
+            if (*p) (void)hipFree(*p);
+            *p = nullptr;
         }
-        if (e == hipSuccess) e = hipDeviceSynchronize();
-        if (e != hipSuccess)
-        {
-            reset();
-            return ctx->fail(APG_E_HIP, std::string("guide network allocation: ") + hipGetErrorString(e));
-        }
-        g->ready = true;
-    }
-    if (cap > g->cap)
+    };
+    const uint32_t G = std::max(1, ctx->cu_count);
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = hipMalloc(&g->w32, GUIDE_PARAMS * 4);
+    if (e == hipSuccess) e = hipMalloc(&g->img_infer, IMG_INFER_BYTES);
+    if (e == hipSuccess) e = hipMalloc(&g->img_train, IMG_TRAIN_BYTES);
+    if (e == hipSuccess) e = hipMalloc(&g->m, GUIDE_PARAMS * 4);
+    if (e == hipSuccess) e = hipMalloc(&g->v, GUIDE_PARAMS * 4);
+    if (e == hipSuccess) e = hipMalloc(&g->grad, GUIDE_PARAMS * 4);
+    if (e == hipSuccess) e = hipMalloc(&g->part, size_t(G) * PART_STRIDE * 4);
+    if (e == hipSuccess) e = hipMalloc(&g->scratch, 64);
+    std::vector<float> p;
+    init_params(p, 1);
+    if (e == hipSuccess) e = hipMemcpy(g->w32, p.data(), GUIDE_PARAMS * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(g->m, 0, GUIDE_PARAMS * 4);
+    if (e == hipSuccess) e = hipMemset(g->v, 0, GUIDE_PARAMS * 4);
+    if (e == hipSuccess) e = hipMemset(g->scratch, 0, 64);
+    if (e == hipSuccess)
     {
-        APG_HIP_CHECK(ctx, hipDeviceSynchronize());
-        if (g->acts) (void)hipFree(g->acts);
-        if (g->out) (void)hipFree(g->out);
-        if (g->dz) (void)hipFree(g->dz);
-        if (g->feat16) (void)hipFree(g->feat16);
-        g->acts = nullptr; g->out = nullptr; g->dz = nullptr; g->feat16 = nullptr;
-        g->cap = 0;  // set again once all four are allocated
-        APG_HIP_CHECK(ctx, hipMalloc(&g->feat16, size_t(cap) * GUIDE_IN * 2));
-        APG_HIP_CHECK(ctx, hipMalloc(&g->acts, size_t(cap) * 4 * GUIDE_HID * 2));
-        APG_HIP_CHECK(ctx, hipMalloc(&g->out, size_t(cap) * GUIDE_OUT * 4));
-        APG_HIP_CHECK(ctx, hipMalloc(&g->dz, size_t(cap) * (GUIDE_OUT + 4 * GUIDE_HID) * 2));
-        g->cap = cap;
+        guide_pack(g, nullptr);
+        e = hipGetLastError();
     }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess)
+    {
+        reset();
+        return ctx->fail(APG_E_HIP, std::string("guide network allocation: ") + hipGetErrorString(e));
+    }
+    g->G_max = G;
+    g->ready = true;
     return APG_OK;
 }
 
 void guide_free(GuideNet* g)
 {
     if (!g) return;
-    for (void* p : { (void*)g->w32, (void*)g->w16, (void*)g->m, (void*)g->v, (void*)g->grad, (void*)g->part, (void*)g->scratch,
-                     (void*)g->acts, (void*)g->out, (void*)g->dz, (void*)g->feat16 })
+    for (void* p : { (void*)g->w32, (void*)g->img_infer, (void*)g->img_train, (void*)g->m, (void*)g->v, (void*)g->grad,
+                     (void*)g->part, (void*)g->scratch })
         if (p) (void)hipFree(p);
     delete g;
 }
 
-int guide_forward(Context* ctx, const float* feat, uint32_t M, const uint32_t* M_dev, float* out,
-                  uint16_t* acts, hipStream_t s)
-{
-    GuideNet* g = guide_of(ctx);
-    FwdArgs A{ g->w16, g->w32, feat, M, M_dev, out, acts };
-    const uint32_t grid = std::min<uint32_t>(nb(M, 16 * WAVES), uint32_t(ctx->cu_count) * 4);
-    k_guide_fwd<<<grid, BLOCK, 0, s>>>(A);
-    APG_HIP_CHECK(ctx, hipGetLastError());
-    return APG_OK;
-}
-
-// Forward + backward into g->grad (zeroed first); optionally the Adam step.
-int guide_infer(Context* ctx, const float* q9, uint32_t M, const uint32_t* M_dev, float* out, hipStream_t s)
+// Forward pass over M (or *M_dev) queries (q9) or feature vectors (feat).
+static int guide_run(Context* ctx, const float* q9, const float* feat, uint32_t M, const uint32_t* M_dev, float* out,
+                     hipStream_t s)
 {
     int rc = guide_ensure(ctx, 1);
     if (rc) return rc;
     GuideNet* g = guide_of(ctx);
-    InferArgs A{ g->w16, g->w32, q9, M, M_dev, out };
-    const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(nb(M, 16 * WAVES), uint32_t(ctx->cu_count) * 3));
-    k_guide_infer<<<grid, BLOCK, 0, s>>>(A);
+    InferArgs A{ g->img_infer, q9, feat, M, M_dev, out };
+    // a persistent grid of the workgroups that fit at once (the weights are loaded once per
+    // workgroup); with a device count the grid covers the capacity M
+    const void* fn = feat ? reinterpret_cast<const void*>(&k_guide_infer<INFER_Q, true>)
+                          : reinterpret_cast<const void*>(&k_guide_infer<INFER_Q, false>);
+    uint32_t& per_cu = g->infer_blocks[feat ? 1 : 0];
+    if (!per_cu)
+    {
+        int nblk = 0;
+        APG_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, fn, BLOCK, 0));
+        per_cu = uint32_t(std::max(1, nblk));
+    }
+    const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(nb(M, 16 * WAVES * INFER_Q), uint32_t(ctx->cu_count) * per_cu));
+    if (feat)
+        k_guide_infer<INFER_Q, true><<<grid, BLOCK, 0, s>>>(A);
+    else
+        k_guide_infer<INFER_Q, false><<<grid, BLOCK, 0, s>>>(A);
     APG_HIP_CHECK(ctx, hipGetLastError());
     return APG_OK;
 }
 
-int guide_train(Context* ctx, const float* feat, const float* dir, const float* weight, uint32_t M,
-                bool adam, float* loss, hipStream_t s, const float* wsum_dev, const float* q9)
+int guide_forward(Context* ctx, const float* feat, uint32_t M, const uint32_t* M_dev, float* out, hipStream_t s)
+{
+    return guide_run(ctx, nullptr, feat, M, M_dev, out, s);
+}
+
+int guide_infer(Context* ctx, const float* q9, uint32_t M, const uint32_t* M_dev, float* out, hipStream_t s)
+{
+    return guide_run(ctx, q9, nullptr, M, M_dev, out, s);
+}
+
+// One training step on M samples, sample i being input off + i * stride of q9 (queries) or feat
+// (features), dir and weight.  Enqueued on s; when loss is set, waits for the step and stores
+// its loss there.  The loss also stays in device memory for guide_last_loss.
+int guide_train(Context* ctx, const float* feat, const float* q9, const float* dir, const float* weight, uint32_t M,
+                uint32_t off, uint32_t stride, bool adam, float* loss, hipStream_t s)
 {
     int rc = guide_ensure(ctx, M);
     if (rc) return rc;
     GuideNet* g = guide_of(ctx);
-    // scratch[0]: weight sum when the caller has none, scratch[1]: loss sum
-    if (loss || !wsum_dev) k_guide_zero<<<1, 64, 0, s>>>(g->scratch, 2);
-    if (!wsum_dev)
-    {
-        k_guide_wsum<<<std::min<uint32_t>(nb(M, 256), 1024), 256, 0, s>>>(weight, M, g->scratch);
-        wsum_dev = g->scratch;
-    }
-    const uint16_t* feat16 = nullptr;
-    if (q9)
-    {
-        // queries: encoding, forward, head and backward-data in one kernel
-        TrainArgs A{ g->w16, g->w32, q9, dir, weight, wsum_dev, M, g->feat16, g->acts, g->dz,
-                     loss ? g->scratch + 1 : nullptr };
-        const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(nb(M, 16 * WAVES), uint32_t(ctx->cu_count) * 2));
-        k_guide_train_fused<<<grid, BLOCK, 0, s>>>(A);
-        feat16 = g->feat16;
-    }
+    const uint32_t G = std::max<uint32_t>(1u, std::min<uint32_t>(g->G_max, nb(M, TS)));
+    TrainArgs T{ g->img_train, q9, feat, dir, weight, M, off, stride, g->part };
+    if (feat)
+        k_guide_train<true><<<G, BLOCK, 0, s>>>(T);
     else
-    {
-        if ((rc = guide_forward(ctx, feat, M, nullptr, g->out, g->acts, s))) return rc;
-        k_guide_head<<<nb(M, 256), 256, 0, s>>>(g->out, dir, weight, M, wsum_dev, loss ? g->scratch + 1 : nullptr,
-                                                g->dz);
-        const uint32_t grid = std::min<uint32_t>(nb(M, 16 * WAVES), uint32_t(ctx->cu_count) * 4);
-        k_guide_bwd_data<<<grid, BLOCK, 0, s>>>(g->w16, g->acts, g->dz, M);
-    }
-    const uint32_t G = std::max<uint32_t>(1u, std::min<uint32_t>(WG_MAX, nb(M, WG_STEP)));
-    const uint32_t chunk = nb(nb(M, G), WG_STEP) * WG_STEP;
-    k_guide_bwd_weight<<<dim3(G, GUIDE_LAYERS), BLOCK, 0, s>>>(feat, feat16, g->acts, g->dz, M, chunk, g->part);
-    AdamArgs A{ g->w32, g->w16, g->m, g->v, g->lr, 0.9f, 0.999f, 1e-8f, 1.0f, 1.0f };
+        k_guide_train<false><<<G, BLOCK, 0, s>>>(T);
+    AdamArgs A{ g->w32, g->m, g->v, g->lr, 0.9f, 0.999f, 1e-8f, 1.0f, 1.0f };
     if (adam)
     {
         g->step++;
         A.c1 = 1.0f - std::pow(A.b1, float(g->step));
         A.c2 = 1.0f - std::pow(A.b2, float(g->step));
     }
-    k_guide_reduce<<<nb(GUIDE_PARAMS, 256), 256, 0, s>>>(g->part, G, g->grad, adam ? 1 : 0, A);
+    k_guide_reduce<<<nb(GUIDE_PARAMS + 1, 256), 256, 0, s>>>(g->part, G, g->grad, g->scratch, adam ? 1 : 0, A);
+    if (adam) guide_pack(g, s);
     APG_HIP_CHECK(ctx, hipGetLastError());
     if (loss)
     {
-        float h[2];
-        APG_HIP_CHECK(ctx, hipMemcpyAsync(h, g->scratch, 8, hipMemcpyDeviceToHost, s));
+        APG_HIP_CHECK(ctx, hipMemcpyAsync(loss, g->scratch + 1, 4, hipMemcpyDeviceToHost, s));
         APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
-        *loss = h[1];
     }
+    return APG_OK;
+}
+
+// Loss of the last training step (waits for the device).
+int guide_last_loss(Context* ctx, float* loss)
+{
+    GuideNet* g = ctx->guide;
+    if (!g || !g->ready)
+    {
+        *loss = 0.0f;
+        return APG_OK;
+    }
+    APG_HIP_CHECK(ctx, hipDeviceSynchronize());
+    APG_HIP_CHECK(ctx, hipMemcpy(loss, g->scratch + 1, 4, hipMemcpyDeviceToHost));
     return APG_OK;
 }
 
@@ -939,10 +909,12 @@ int apg_guide_init(apg_context* ctx, uint32_t seed, float lr)
     GuideNet* g = guide_of(ctx);
     std::vector<float> p;
     init_params(p, seed);
+    APG_HIP_CHECK(ctx, hipDeviceSynchronize());
     APG_HIP_CHECK(ctx, hipMemcpy(g->w32, p.data(), GUIDE_PARAMS * 4, hipMemcpyHostToDevice));
     APG_HIP_CHECK(ctx, hipMemset(g->m, 0, GUIDE_PARAMS * 4));
     APG_HIP_CHECK(ctx, hipMemset(g->v, 0, GUIDE_PARAMS * 4));
-    k_guide_sync16<<<nb(WSZ, 256), 256>>>(g->w32, g->w16);
+    guide_pack(g, nullptr);
+    APG_HIP_CHECK(ctx, hipGetLastError());
     APG_HIP_CHECK(ctx, hipDeviceSynchronize());
     g->step = 0;
     g->lr = lr > 0.0f ? lr : 1e-3f;
@@ -962,10 +934,12 @@ int apg_guide_params(apg_context* ctx, float* params, int write)
     int rc = guide_ensure(ctx, 1);
     if (rc) return rc;
     GuideNet* g = guide_of(ctx);
+    APG_HIP_CHECK(ctx, hipDeviceSynchronize());
     if (write)
     {
         APG_HIP_CHECK(ctx, hipMemcpy(g->w32, params, GUIDE_PARAMS * 4, hipMemcpyHostToDevice));
-        k_guide_sync16<<<nb(WSZ, 256), 256>>>(g->w32, g->w16);
+        guide_pack(g, nullptr);
+        APG_HIP_CHECK(ctx, hipGetLastError());
         APG_HIP_CHECK(ctx, hipDeviceSynchronize());
     }
     else
@@ -976,9 +950,22 @@ int apg_guide_params(apg_context* ctx, float* params, int write)
 int apg_guide_forward(apg_context* ctx, const float* feat, uint32_t n, float* out, void* stream)
 {
     if (!ctx || (n && (!feat || !out))) return APG_E_INVALID;
-    int rc = guide_ensure(ctx, 1);
-    if (rc || n == 0) return rc;
-    return guide_forward(ctx, feat, n, nullptr, out, nullptr, static_cast<hipStream_t>(stream));
+    if (n == 0) return APG_OK;
+    return guide_forward(ctx, feat, n, nullptr, out, static_cast<hipStream_t>(stream));
+}
+
+static int train_entry(apg_context* ctx, const float* feat, const float* q, const float* dir, const float* weight,
+                       uint32_t n, int apply, float* loss, float* grad_host, void* stream)
+{
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc = guide_train(ctx, feat, q, dir, weight, n, 0, 1, apply != 0, loss, s);
+    if (rc) return rc;
+    if (grad_host)
+    {
+        APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
+        APG_HIP_CHECK(ctx, hipMemcpy(grad_host, guide_of(ctx)->grad, GUIDE_PARAMS * 4, hipMemcpyDeviceToHost));
+    }
+    return APG_OK;
 }
 
 int apg_guide_train_step(apg_context* ctx, const float* feat, const float* dir, const float* weight,
@@ -986,15 +973,7 @@ int apg_guide_train_step(apg_context* ctx, const float* feat, const float* dir, 
 {
     if (!ctx || (n && (!feat || !dir || !weight))) return APG_E_INVALID;
     if (n == 0) return APG_OK;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    int rc = guide_train(ctx, feat, dir, weight, n, apply != 0, loss, s, nullptr, nullptr);
-    if (rc) return rc;
-    if (grad_host)
-    {
-        APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
-        APG_HIP_CHECK(ctx, hipMemcpy(grad_host, guide_of(ctx)->grad, GUIDE_PARAMS * 4, hipMemcpyDeviceToHost));
-    }
-    return APG_OK;
+    return train_entry(ctx, feat, nullptr, dir, weight, n, apply, loss, grad_host, stream);
 }
 
 int apg_guide_train_queries(apg_context* ctx, const float* q, const float* dir, const float* weight,
@@ -1002,15 +981,7 @@ int apg_guide_train_queries(apg_context* ctx, const float* q, const float* dir, 
 {
     if (!ctx || (n && (!q || !dir || !weight))) return APG_E_INVALID;
     if (n == 0) return APG_OK;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    int rc = guide_train(ctx, nullptr, dir, weight, n, apply != 0, loss, s, nullptr, q);
-    if (rc) return rc;
-    if (grad_host)
-    {
-        APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
-        APG_HIP_CHECK(ctx, hipMemcpy(grad_host, guide_of(ctx)->grad, GUIDE_PARAMS * 4, hipMemcpyDeviceToHost));
-    }
-    return APG_OK;
+    return train_entry(ctx, nullptr, q, dir, weight, n, apply, loss, grad_host, stream);
 }
 
 int apg_guide_param_count(void) { return GUIDE_PARAMS; }

@@ -27,10 +27,9 @@
 namespace apg {
 
 int guide_ensure(Context* ctx, uint32_t cap);
-int guide_forward(Context* ctx, const float* feat, uint32_t M, const uint32_t* M_dev, float* out,
-                  uint16_t* acts, hipStream_t s);
-int guide_train(Context* ctx, const float* feat, const float* dir, const float* weight, uint32_t M,
-                bool adam, float* loss, hipStream_t s, const float* wsum_dev, const float* q9);
+int guide_train(Context* ctx, const float* feat, const float* q9, const float* dir, const float* weight, uint32_t M,
+                uint32_t off, uint32_t stride, bool adam, float* loss, hipStream_t s);
+int guide_last_loss(Context* ctx, float* loss);
 int guide_infer(Context* ctx, const float* q9, uint32_t M, const uint32_t* M_dev, float* out, hipStream_t s);
 
 struct Queue
@@ -109,18 +108,13 @@ struct RenderState
     uint32_t vrec_depth = 0;
     uint32_t guide_P = 0;     // paths per pass the guide's per-path buffers are sized for
     float* tin = nullptr;     // tcap x 9 training inputs (p01, n, wo)
-    float* tfeat = nullptr;   // tcap x 9, minibatch order
     float* tdir = nullptr;    // tcap x 3
     float* tw = nullptr;      // tcap
-    float* tdir_p = nullptr;  // tcap x 3, minibatch order
-    float* tw_p = nullptr;    // tcap, minibatch order
-    float* twsum = nullptr;   // weight sum of each minibatch
     float train_keep = 1.0f;  // fraction of candidate samples kept in the last pass
     uint32_t tcap = 0;
     uint32_t* tcount = nullptr;
     std::vector<void*> guide_allocs;
     uint32_t last_train_samples = 0;
-    float last_loss = 0.0f;
     uint32_t guided_passes = 0;
     float scene_lo[3] = { 0, 0, 0 }, scene_hi[3] = { 1, 1, 1 };
     // shading attributes of the shapes (pools over all shapes, global triangle order)
@@ -762,20 +756,6 @@ __global__ void k_postprocess(const float* accum, uint32_t n3, float spp, uint32
     out[i] = uint8_t(c * 255.0f + 0.5f);
 }
 
-// Training inputs in minibatch order: sample i goes to minibatch i % S, so
-// each of the S minibatches spreads over the whole image.
-__global__ void k_train_encode(const float* tin, const float* tdir, const float* tw, uint32_t n, uint32_t S,
-                               uint32_t B, float* q_p, float* dir_p, float* w_p, float* wsum)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t o = (i % S) * B + i / S;
-    for (int k = 0; k < 9; ++k) q_p[9 * size_t(o) + k] = tin[9 * size_t(i) + k];
-    for (int k = 0; k < 3; ++k) dir_p[3 * size_t(o) + k] = tdir[3 * size_t(i) + k];
-    w_p[o] = tw[i];
-    atomicAdd(&wsum[i % S], tw[i]);
-}
-
 __global__ void k_zero_f32(float* p, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1350,7 +1330,7 @@ int apg_render_guide_stats(const apg_context* cctx, uint32_t* samples, float* lo
     if (!cctx) return APG_E_INVALID;
     RenderState* R = state_of(const_cast<apg_context*>(cctx));
     if (samples) *samples = R->last_train_samples;
-    if (loss) *loss = R->last_loss;
+    if (loss) return guide_last_loss(const_cast<apg_context*>(cctx), loss);
     return APG_OK;
 }
 
@@ -1366,7 +1346,7 @@ static int ensure_guide_buffers(apg_context* ctx, RenderState* R, uint32_t depth
     // allocation has succeeded: a failure part-way leaves the state "not allocated", so the
     // next guided render allocates again instead of reusing freed memory.
     R->gfeat = R->gout = R->vrec = nullptr;
-    R->tin = R->tfeat = R->tdir = R->tw = R->tdir_p = R->tw_p = R->twsum = nullptr;
+    R->tin = R->tdir = R->tw = nullptr;
     R->tcount = nullptr;
     R->vrec_depth = 0;
     R->tcap = 0;
@@ -1376,15 +1356,10 @@ static int ensure_guide_buffers(apg_context* ctx, RenderState* R, uint32_t depth
         if (rc == APG_OK) R->guide_allocs.push_back(ptr);
         return rc;
     };
-    // Minibatch order places S minibatches of B = ceil(n / S) slots each: S * B <= n + S - 1,
-    // so the minibatch-order arrays hold tcap + TRAIN_MAX_STEPS entries.
-    const size_t tpad = size_t(tcap) + TRAIN_MAX_STEPS;
     int rc;
     if ((rc = get(R->gfeat, size_t(P) * 9)) || (rc = get(R->gout, size_t(P) * GUIDE_OUT)) ||
         (rc = get(R->vrec, size_t(P) * depth * VREC)) || (rc = get(R->tin, size_t(tcap) * 9)) ||
-        (rc = get(R->tfeat, tpad * 9)) || (rc = get(R->tdir, size_t(tcap) * 3)) ||
-        (rc = get(R->tw, tcap)) || (rc = get(R->tdir_p, tpad * 3)) || (rc = get(R->tw_p, tpad)) ||
-        (rc = get(R->twsum, TRAIN_MAX_STEPS)) || (rc = get(R->tcount, 4)))
+        (rc = get(R->tdir, size_t(tcap) * 3)) || (rc = get(R->tw, tcap)) || (rc = get(R->tcount, 4)))
     {
         R->gfeat = nullptr;  // the check at the top then allocates again
         return rc;
@@ -1621,20 +1596,13 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
             R->last_train_samples = tn;
             if (tn)
             {
-                // one epoch over the pass's samples in train_steps minibatches
+                // one epoch over the pass's samples in train_steps minibatches: minibatch j
+                // takes samples j, j + S, j + 2S, ... so each spreads over the whole frame
                 const uint32_t S = std::min(std::min(train_steps, tn), TRAIN_MAX_STEPS);
-                const uint32_t B = (tn + S - 1) / S;
-                if (size_t(S) * B > size_t(R->tcap) + TRAIN_MAX_STEPS)
-                    return ctx->fail(APG_E_CAPACITY, "training minibatches exceed their buffers");
-                k_zero_f32<<<nblocks(S, 256), 256, 0, stream>>>(R->twsum, S);
-                k_train_encode<<<nblocks(tn, 256), 256, 0, stream>>>(R->tin, R->tdir, R->tw, tn, S, B, R->tfeat,
-                                                                     R->tdir_p, R->tw_p, R->twsum);
                 for (uint32_t j = 0; j < S; ++j)
                 {
                     const uint32_t mb = (tn - j + S - 1) / S;
-                    rc = guide_train(ctx, nullptr, R->tdir_p + size_t(j) * B * 3, R->tw_p + size_t(j) * B, mb, true,
-                                     j + 1 == S ? &R->last_loss : nullptr, stream, R->twsum + j,
-                                     R->tfeat + size_t(j) * B * 9);
+                    rc = guide_train(ctx, nullptr, R->tin, R->tdir, R->tw, mb, j, S, true, nullptr, stream);
                     if (rc) return rc;
                 }
             }
