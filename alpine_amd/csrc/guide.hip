@@ -112,7 +112,6 @@ typedef float f2v __attribute__((ext_vector_type(2)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef uint32_t u4v __attribute__((ext_vector_type(4)));
 typedef short s2v __attribute__((ext_vector_type(2)));
-typedef unsigned short us2v __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ uint32_t cvt2(float a, float b)  // v_cvt_pk_bf16_f32 (nearest even)
 {
@@ -122,10 +121,14 @@ __device__ __forceinline__ uint32_t relu2(uint32_t x)  // max(x, 0) of both BF16
 {
     return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s2v, x), (s2v){ 0, 0 }));
 }
-__device__ __forceinline__ uint32_t nonzero_mask(uint32_t x)  // 0xFFFF per non-zero half
+// Each 16-bit half of x kept where the same half of h (a ReLU output, >= 0) is non-zero:
+// x * min(h, 1) per half.  Written as the two packed instructions (ALU only), since the
+// compiler otherwise rewrites the product into per-half compares and selects.
+__device__ __forceinline__ uint32_t relu_grad(uint32_t x, uint32_t h)
 {
-    us2v m = __builtin_elementwise_min(__builtin_bit_cast(us2v, x), (us2v){ 1, 1 });
-    return __builtin_bit_cast(uint32_t, m * (us2v){ 0xFFFF, 0xFFFF });
+    uint32_t r;
+    asm("v_pk_min_u16 %0, %1, %3\n\tv_pk_mul_lo_u16 %0, %2, %0" : "=&v"(r) : "v"(h), "v"(x), "s"(0x00010001u));
+    return r;
 }
 __device__ __forceinline__ f4v mfma(u4v a, u4v b, f4v c)
 {
@@ -338,59 +341,69 @@ struct TrainArgs
     float* part;            // gridDim.x x PART_STRIDE
 };
 
+// Hardware transcendentals (v_exp_f32, v_log_f32, v_rcp_f32, v_rsq_f32; about 1 ulp) for the
+// training head, whose result only steers the gradient.
+__device__ __forceinline__ float hexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+__device__ __forceinline__ float hlog(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+
 // Loss head of one sample on its four lane groups: group g holds outputs 10g .. 10g + 9
 // (lobes 2g and 2g + 1) in oo[0..9].  Returns -log q(d) and writes scale * d(-log q)/d out
-// into gg in the same layout (gg[10], gg[11] = 0): the arithmetic of guide_nll_grad, with the
-// sums over lobes taken across the lane groups.
+// into gg in the same layout (gg[10], gg[11] = 0).  The terms are those of guide_nll_grad,
+// with the sums over lobes taken across the lane groups, log pi from the log-softmax and
+// log kappa = the clamped output.
 __device__ __forceinline__ float head_grad(const float (&oo)[12], f3 d, float scale, float (&gg)[12])
 {
     float amax = fmaxf(oo[0], oo[5]);
     amax = fmaxf(amax, __shfl_xor(amax, 16));
     amax = fmaxf(amax, __shfl_xor(amax, 32));
-    float pi[2], kap[2], lv[2], cth[2], len[2];
+    float pi[2], kap[2], lv[2], cth[2], il[2], e2[2];
+    bool nz[2];
     f3 mu[2];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) pi[t] = expf(oo[5 * t] - amax);
+    for (int t = 0; t < 2; ++t) pi[t] = hexp(oo[5 * t] - amax);
     float sum = pi[0] + pi[1];
     sum += __shfl_xor(sum, 16);
     sum += __shfl_xor(sum, 32);
+    const float isum = __builtin_amdgcn_rcpf(sum), lsum = hlog(sum);
+    constexpr float kLog2Pi = 1.83787706640934548f, kLogTiny = -69.0775527898213705f;  // log(2 pi), log(1e-30)
 #pragma unroll
     for (int t = 0; t < 2; ++t)
     {
-        pi[t] /= sum;
-        kap[t] = expf(clampk(oo[5 * t + 1]));
+        pi[t] *= isum;
+        const float lk = clampk(oo[5 * t + 1]);
+        kap[t] = hexp(lk);
         const f3 v = mk(oo[5 * t + 2], oo[5 * t + 3], oo[5 * t + 4]);
-        len[t] = length(v);
-        mu[t] = len[t] > 1e-6f ? v / len[t] : mk(0.0f, 0.0f, 1.0f);
+        const float l2 = dot(v, v);
+        il[t] = __builtin_amdgcn_rsqf(l2);
+        nz[t] = l2 * il[t] > 1e-6f;  // |v|
+        mu[t] = nz[t] ? v * il[t] : mk(0.0f, 0.0f, 1.0f);
         cth[t] = dot(mu[t], d);
-        lv[t] = logf(max_s(pi[t], 1e-30f)) + vmf_log(kap[t], cth[t]);
+        e2[t] = hexp(-2.0f * kap[t]);
+        // log pi + log vMF(kappa, c)
+        lv[t] = fmaxf(oo[5 * t] - amax - lsum, kLogTiny) + lk - kLog2Pi - hlog(1.0f - e2[t]) + kap[t] * (cth[t] - 1.0f);
     }
     float lmax = fmaxf(lv[0], lv[1]);
     lmax = fmaxf(lmax, __shfl_xor(lmax, 16));
     lmax = fmaxf(lmax, __shfl_xor(lmax, 32));
-    float s2 = expf(lv[0] - lmax) + expf(lv[1] - lmax);
+    float s2 = hexp(lv[0] - lmax) + hexp(lv[1] - lmax);
     s2 += __shfl_xor(s2, 16);
     s2 += __shfl_xor(s2, 32);
-    const float logq = lmax + logf(s2);
+    const float logq = lmax + hlog(s2);
 #pragma unroll
     for (int t = 0; t < 2; ++t)
     {
-        const float r = expf(lv[t] - logq), k = kap[t];
+        const float r = hexp(lv[t] - logq), k = kap[t];
         gg[5 * t] = scale * (pi[t] - r);
-        const float e2 = expf(-2.0f * k);
-        const float dlk = 1.0f / k - 2.0f * e2 / (1.0f - e2) + (cth[t] - 1.0f);
+        // d log vMF / d kappa = 1/k - 2 e^{-2k} / (1 - e^{-2k}) + (c - 1); kappa = exp(o)
+        const float dlk = __builtin_amdgcn_rcpf(k) - 2.0f * e2[t] * __builtin_amdgcn_rcpf(1.0f - e2[t]) + (cth[t] - 1.0f);
         const float ok = oo[5 * t + 1];
         gg[5 * t + 1] = ok > GUIDE_LOGK_MIN && ok < GUIDE_LOGK_MAX ? scale * (-r * dlk * k) : 0.0f;
-        if (len[t] > 1e-6f)
-        {
-            const f3 dmu = d * (-r * k);
-            const f3 dv = (dmu - mu[t] * dot(mu[t], dmu)) / len[t];
-            gg[5 * t + 2] = scale * dv.x;
-            gg[5 * t + 3] = scale * dv.y;
-            gg[5 * t + 4] = scale * dv.z;
-        }
-        else
-            gg[5 * t + 2] = gg[5 * t + 3] = gg[5 * t + 4] = 0.0f;
+        // d/d mu = -r k d, through mu = v / |v|
+        const f3 dmu = d * (-r * k);
+        const f3 dv = (dmu - mu[t] * dot(mu[t], dmu)) * (nz[t] ? scale * il[t] : 0.0f);
+        gg[5 * t + 2] = dv.x;
+        gg[5 * t + 3] = dv.y;
+        gg[5 * t + 4] = dv.z;
     }
     gg[10] = gg[11] = 0.0f;
     return -logq;
@@ -492,7 +505,7 @@ __device__ __forceinline__ void back(const uint16_t* swb, const u4v (&dz)[TQ][2]
             for (int e = 0; e < 4; ++e)
             {
                 const int kt = 2 * p + (e >> 1), i0 = 2 * (e & 1);
-                out[q][p][e] = cvt2(da[q][kt][i0], da[q][kt][i0 + 1]) & nonzero_mask(h[q][p][e]);
+                out[q][p][e] = relu_grad(cvt2(da[q][kt][i0], da[q][kt][i0 + 1]), h[q][p][e]);
             }
 }
 
@@ -668,27 +681,50 @@ struct AdamArgs
     float lr, b1, b2, eps, c1, c2;
 };
 
-// Sum of the workgroup sums in workgroup order, divided by the minibatch's weight sum; then,
-// when apply is set, the Adam step on the FP32 master weights.  Thread GUIDE_PARAMS writes
-// scratch[0] = weight sum, scratch[1] = weighted mean negative log-likelihood.
-__global__ void k_guide_reduce(const float* part, uint32_t G, float* grad, float* scratch, int apply, AdamArgs A)
+// Sum of the workgroup sums, divided by the minibatch's weight sum; then, when apply is set,
+// the Adam step on the FP32 master weights.  A block takes RED_P consecutive entries; its
+// threads add every RED_K-th workgroup's sums and the RED_K partial results are combined in
+// a fixed order, so the gradient does not depend on timing.  Entry GUIDE_PARAMS is the
+// weight sum, GUIDE_PARAMS + 1 the weighted loss: scratch[0] = weight sum, scratch[1] =
+// weighted mean negative log-likelihood.
+constexpr int RED_P = 64, RED_K = 4;
+__global__ __launch_bounds__(RED_P * RED_K) void k_guide_reduce(const float* part, uint32_t G, float* grad, float* scratch,
+                                                              int apply, AdamArgs A)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > uint32_t(GUIDE_PARAMS)) return;
-    float ws = 0.0f;
-    for (uint32_t k = 0; k < G; ++k) ws += part[size_t(k) * PART_STRIDE + GUIDE_PARAMS];
-    const float inv = ws > 0.0f ? 1.0f / ws : 0.0f;
-    if (i == uint32_t(GUIDE_PARAMS))
+    __shared__ float sp[RED_K][RED_P];
+    __shared__ float sw[RED_P * RED_K];
+    const int t = threadIdx.x, p = t % RED_P, k0 = t / RED_P;
+    // the weight sum, added in the same order by every block
+    float a = 0.0f;
+    for (uint32_t k = t; k < G; k += RED_P * RED_K) a += part[size_t(k) * PART_STRIDE + GUIDE_PARAMS];
+    sw[t] = a;
+    const uint32_t i = blockIdx.x * RED_P + p;
+    float s = 0.0f;
+    if (i < uint32_t(GUIDE_PARAMS + 2))
     {
-        float l = 0.0f;
-        for (uint32_t k = 0; k < G; ++k) l += part[size_t(k) * PART_STRIDE + GUIDE_PARAMS + 1];
-        scratch[0] = ws;
-        scratch[1] = l * inv;
+#pragma unroll 8
+        for (uint32_t k = k0; k < G; k += RED_K) s += part[size_t(k) * PART_STRIDE + i];
+    }
+    sp[k0][p] = s;
+    __syncthreads();
+    for (int h = RED_P * RED_K / 2; h > 0; h >>= 1)
+    {
+        if (t < h) sw[t] += sw[t + h];
+        __syncthreads();
+    }
+    if (k0 != 0 || i >= uint32_t(GUIDE_PARAMS + 2)) return;
+    const float ws = sw[0];
+    const float inv = ws > 0.0f ? 1.0f / ws : 0.0f;
+    float tot = sp[0][p];
+#pragma unroll
+    for (int k = 1; k < RED_K; ++k) tot += sp[k][p];
+    if (i >= uint32_t(GUIDE_PARAMS))
+    {
+        if (i == uint32_t(GUIDE_PARAMS)) scratch[0] = ws;
+        else scratch[1] = tot * inv;
         return;
     }
-    float s = 0.0f;
-    for (uint32_t k = 0; k < G; ++k) s += part[size_t(k) * PART_STRIDE + i];
-    const float gv = s * inv;
+    const float gv = tot * inv;
     grad[i] = gv;
     if (!apply) return;
     const float mm = A.b1 * A.m[i] + (1.0f - A.b1) * gv;
@@ -869,7 +905,7 @@ int guide_train(Context* ctx, const float* feat, const float* q9, const float* d
         A.c1 = 1.0f - std::pow(A.b1, float(g->step));
         A.c2 = 1.0f - std::pow(A.b2, float(g->step));
     }
-    k_guide_reduce<<<nb(GUIDE_PARAMS + 1, 256), 256, 0, s>>>(g->part, G, g->grad, g->scratch, adam ? 1 : 0, A);
+    k_guide_reduce<<<nb(GUIDE_PARAMS + 2, RED_P), RED_P * RED_K, 0, s>>>(g->part, G, g->grad, g->scratch, adam ? 1 : 0, A);
     if (adam) guide_pack(g, s);
     APG_HIP_CHECK(ctx, hipGetLastError());
     if (loss)
