@@ -164,17 +164,23 @@ __device__ __forceinline__ void load_inputs(const float* q9, const float* feat, 
             n[0] = qq[3]; n[1] = qq[4]; n[2] = qq[5];
             wo[0] = qq[6]; wo[1] = qq[7]; wo[2] = qq[8];
         }
-        // sin and cos of 2^k pi p as sin(2 pi frac(2^(k-1) p + c / 4)), as guide_feature
+        // sin and cos of 2^k pi p for k = 3g as sin(2 pi frac(2^(k-1) p + c / 4)), as
+        // guide_feature; k = 3g + 1, 3g + 2 by the double-angle formulas (each doubles the
+        // absolute error, about 1e-7 here, far below the BF16 rounding of the inputs)
+        const float sc = ldexpf(1.0f, 3 * g - 1);
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
+        for (int a = 0; a < 3; ++a)
         {
-            const float sc = ldexpf(1.0f, 3 * g + j - 1);
+            const float x = p[a] * sc, xc = x + 0.25f;
+            float sn = sin_rev(x - floorf(x)), cs = sin_rev(xc - floorf(xc));
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
+            for (int j = 0; j < 3; ++j)
             {
-                const float x = p[a] * sc, xc = x + 0.25f;
-                s[6 * j + 2 * a] = sin_rev(x - floorf(x));
-                s[6 * j + 2 * a + 1] = sin_rev(xc - floorf(xc));
+                s[6 * j + 2 * a] = sn;
+                s[6 * j + 2 * a + 1] = cs;
+                const float s2 = 2.0f * sn * cs;
+                cs = 1.0f - 2.0f * sn * sn;
+                sn = s2;
             }
         }
 #pragma unroll
@@ -281,19 +287,24 @@ struct InferArgs
 };
 
 template <int Q, bool FEAT>
-__global__ __launch_bounds__(BLOCK) void k_guide_infer(InferArgs A)
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_guide_infer(InferArgs A)
 {
     __shared__ __align__(16) uint8_t lds[IMG_INFER_BYTES];
     copy_image(lds, A.img, IMG_INFER_BYTES);
     __syncthreads();
-    const uint16_t* sw = reinterpret_cast<const uint16_t*>(lds);
-    const float* sb = reinterpret_cast<const float*>(lds + IMG_BIAS);
     const uint32_t M = A.M_dev ? *A.M_dev : A.M;
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     const uint32_t step = gridDim.x * WAVES * Q;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform loop bounds
     for (uint32_t t0 = (blockIdx.x * WAVES + wave) * Q; t0 * 16u < M; t0 += step)
     {
+        // weight fragments are read from LDS in every iteration rather than kept in registers:
+        // the register budget then allows several waves per SIMD, whose conversions and
+        // encoding overlap each other's MFMAs
+        uint8_t* base = lds;
+        asm volatile("" : "+v"(base));
+        const uint16_t* sw = reinterpret_cast<const uint16_t*>(base);
+        const float* sb = reinterpret_cast<const float*>(base + IMG_BIAS);
         u4v in0[Q][3];
 #pragma unroll
         for (int q = 0; q < Q; ++q)
@@ -320,8 +331,9 @@ __global__ __launch_bounds__(BLOCK) void k_guide_infer(InferArgs A)
             if (qi < M)
 #pragma unroll
                 for (int nt = 0; nt < 3; ++nt)
-                    *reinterpret_cast<float4*>(A.out + size_t(qi) * GUIDE_OUT + 16 * nt + 4 * g) =
-                        make_float4(o[q][nt][0], o[q][nt][1], o[q][nt][2], o[q][nt][3]);
+                    if (nt < 2 || g < 2)  // outputs 40..47 feed no lobe and are not written
+                        *reinterpret_cast<float4*>(A.out + size_t(qi) * GUIDE_OUT + 16 * nt + 4 * g) =
+                            make_float4(o[q][nt][0], o[q][nt][1], o[q][nt][2], o[q][nt][3]);
         }
     }
 }
