@@ -316,10 +316,13 @@ def mlp_benchmark(n_infer=1 << 20, n_train=1 << 19, reps=10):
 # online for TRAIN_SPP samples per pixel (these samples stay in the image) and then renders with
 # the frozen proposal until it reaches the target.  Both run SLOTS samples of each pixel side by
 # side and are timed around the GPU work only (the relMSE checks read the image between chunks).
-GUIDE_PLAIN_SPP, GUIDE_REF_SPP, GUIDE_SLOTS, GUIDE_CHUNK = 1024, 4096, 4, 32
+GUIDE_PLAIN_SPP, GUIDE_REF_SPP, GUIDE_SLOTS, GUIDE_CHUNK = 1024, 16384, 4, 32
 # FP64 references of the many-light frame and of the batched frame
 ML_REF_SPP, BATCH_REF_SPP = 1024, 256
-GUIDE_TRAIN_SPP, GUIDE_STEPS, GUIDE_LR, GUIDE_FRAC, GUIDE_SAMPLES = 64, 32, 5e-3, 0.5, 1 << 19
+GUIDE_TRAIN_SPP, GUIDE_STEPS, GUIDE_FRAC, GUIDE_SAMPLES = 64, 32, 0.5, 1 << 19
+# Adam learning rates: the workload's stated 1e-3 (the headline figure), and 5e-3, the best
+# of the settings swept on this scene, reported beside it
+GUIDE_LR, GUIDE_LR_TUNED = 1e-3, 5e-3
 GUIDE_DEPTH = 0   # path vertices below this depth sample the proposal (0 = every vertex)
 
 
@@ -339,6 +342,7 @@ def guided_benchmark(builder):
     acc.render_reset(sc.width, sc.height, 0, slots=GUIDE_SLOTS)
     acc.render(GUIDE_SLOTS, sc.max_depth)
     acc.render_reset(sc.width, sc.height, 4000, slots=GUIDE_SLOTS)
+    log(f"[bench] guided: {GUIDE_REF_SPP}-spp reference")
     t_ref = timed(lambda: acc.render(GUIDE_REF_SPP, sc.max_depth, fp64=True))
     _, r64, _ = acc.render_read(fp64=True)
     ref = r64 / GUIDE_REF_SPP
@@ -352,37 +356,49 @@ def guided_benchmark(builder):
         img, _, _ = acc.render_read()
         curve_plain.append((spp, t_plain, scenes.rel_mse(img / spp, ref)))
     target = curve_plain[-1][2]
-    # guided: online training, then the frozen proposal until the target is reached
-    acc.guide_init(seed=1, lr=GUIDE_LR)
-    acc.render_reset(sc.width, sc.height, 21, slots=GUIDE_SLOTS)
-    t_train = timed(lambda: acc.render(GUIDE_TRAIN_SPP, sc.max_depth, guide=1, guide_warmup=1,
-                                       train_steps=GUIDE_STEPS, guide_frac=GUIDE_FRAC,
-                                       train_samples=GUIDE_SAMPLES))
-    img, _, tot = acc.render_read()
-    t_g = t_train
-    curve_g = [(tot, t_g, scenes.rel_mse(img / tot, ref))]
-    reach = t_g if curve_g[0][2] <= target else None
-    while reach is None and tot < 4 * GUIDE_PLAIN_SPP:
-        t_g += timed(lambda: acc.render(GUIDE_CHUNK // 2, sc.max_depth, guide=2, guide_frac=GUIDE_FRAC,
-                                        guide_depth=GUIDE_DEPTH, graph=True))
+
+    def guided(lr, seed):
+        # online training, then the frozen proposal until the target is reached
+        acc.guide_init(seed=1, lr=lr)
+        acc.render_reset(sc.width, sc.height, seed, slots=GUIDE_SLOTS)
+        t_train = timed(lambda: acc.render(GUIDE_TRAIN_SPP, sc.max_depth, guide=1, guide_warmup=1,
+                                           train_steps=GUIDE_STEPS, guide_frac=GUIDE_FRAC,
+                                           train_samples=GUIDE_SAMPLES))
         img, _, tot = acc.render_read()
-        e = scenes.rel_mse(img / tot, ref)
-        curve_g.append((tot, t_g, e))
-        if e <= target:
-            reach = t_g
-    # equal-time comparison: plain's relMSE at the guided run's wall time
-    e_plain_eq = next((e for s_, t_, e in curve_plain if t_ >= t_g), curve_plain[-1][2])
+        t_g = t_train
+        curve_g = [(tot, t_g, scenes.rel_mse(img / tot, ref))]
+        reach = t_g if curve_g[0][2] <= target else None
+        while reach is None and tot < 4 * GUIDE_PLAIN_SPP:
+            t_g += timed(lambda: acc.render(GUIDE_CHUNK // 2, sc.max_depth, guide=2, guide_frac=GUIDE_FRAC,
+                                            guide_depth=GUIDE_DEPTH, graph=True))
+            img, _, tot = acc.render_read()
+            e = scenes.rel_mse(img / tot, ref)
+            curve_g.append((tot, t_g, e))
+            if e <= target:
+                reach = t_g
+        # equal-time comparison: plain's relMSE at the guided run's wall time
+        e_plain_eq = next((e for s_, t_, e in curve_plain if t_ >= t_g), curve_plain[-1][2])
+        return {
+            "lr": lr, "train_spp": GUIDE_TRAIN_SPP, "train_seconds": round(t_train, 4),
+            "spp_at_target": int(tot), "seconds_to_target": None if reach is None else round(reach, 4),
+            "rel_mse": round(curve_g[-1][2], 6),
+            "time_ratio_guided_over_plain": None if reach is None else round(reach / t_plain, 4),
+            "plain_rel_mse_at_guided_time": round(e_plain_eq, 6),
+        }
+
+    log(f"[bench] guided: plain reached relMSE {target:.5f} in {t_plain:.3f} s")
+    stated = guided(GUIDE_LR, 21)
+    log(f"[bench] guided: lr {GUIDE_LR}: {stated['time_ratio_guided_over_plain']} of plain's time")
+    tuned = guided(GUIDE_LR_TUNED, 31)
+    log(f"[bench] guided: lr {GUIDE_LR_TUNED}: {tuned['time_ratio_guided_over_plain']} of plain's time")
     return {
         "scene": "box grid (config-3 geometry) behind a 4 x 3 doorway, one 1 x 1 quad light",
         "triangles": int(len(sc.tris)), "frame": [sc.width, sc.height], "slots": GUIDE_SLOTS,
         "reference": f"{GUIDE_REF_SPP} spp plain, FP64 accumulation ({t_ref:.1f} s)",
         "plain_spp": GUIDE_PLAIN_SPP, "plain_seconds": round(t_plain, 4), "plain_rel_mse": round(target, 6),
-        "guided_train_spp": GUIDE_TRAIN_SPP, "guided_train_seconds": round(t_train, 4),
-        "guided_spp_at_target": int(tot), "guided_seconds_to_target": None if reach is None else round(reach, 4),
-        "guided_rel_mse": round(curve_g[-1][2], 6),
-        "time_ratio_guided_over_plain": None if reach is None else round(reach / t_plain, 4),
-        "plain_rel_mse_at_guided_time": round(e_plain_eq, 6),
-        "train": {"minibatches_per_pass": GUIDE_STEPS, "lr": GUIDE_LR, "proposal_fraction": GUIDE_FRAC,
+        "time_ratio_guided_over_plain": stated["time_ratio_guided_over_plain"],
+        "guided_lr_stated": stated, "guided_lr_tuned": tuned,
+        "train": {"minibatches_per_pass": GUIDE_STEPS, "proposal_fraction": GUIDE_FRAC,
                   "samples_per_pass": GUIDE_SAMPLES},
     }
 
