@@ -130,6 +130,11 @@ class TraceConfig(C.Structure):
 TRI_REFERENCE, TRI_WATERTIGHT = 0, 1
 
 
+class _RenderLaunch(C.Structure):
+    _fields_ = [("shade_blocks_per_cu", C.c_uint32), ("shadow_blocks_per_cu", C.c_uint32),
+                ("resolve_blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 _lib = None
 
 # Every symbol the C ABI headers declare (checked by tests/test_abi.py).
@@ -144,7 +149,7 @@ ABI_SYMBOLS = [
     "apg_guide_param_count",
     "apg_guide_init",
     "apg_guide_params", "apg_guide_forward", "apg_guide_infer", "apg_guide_train_step",
-    "apg_guide_train_queries",
+    "apg_guide_train_queries", "apg_render_set_launch", "apg_render_get_launch",
     "apg_render_guide_stats", "apg_mesh_set_shading", "apg_mesh_set_targets", "apg_mesh_morph",
     "apg_textures_set", "apg_render_read_aovs", "apg_render_set_environment",
     "apg_intersect_host", "apg_occluded_host", "apg_light_sampler_set",
@@ -226,6 +231,8 @@ def lib():
         "apg_guide_infer": (i32, [vp, vp, u32, vp, vp]),
         "apg_guide_train_step": (i32, [vp, vp, vp, vp, u32, i32, C.POINTER(C.c_float), vp, vp]),
         "apg_guide_train_queries": (i32, [vp, vp, vp, vp, u32, i32, C.POINTER(C.c_float), vp, vp]),
+        "apg_render_set_launch": (i32, [vp, vp]),
+        "apg_render_get_launch": (i32, [vp, vp]),
         "apg_debug_host_build": (i32, [vp, u32, vp, u32, u32, C.POINTER(vp)]),
         "apg_debug_scene_info": (i32, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(u32), C.POINTER(u32)]),
@@ -586,6 +593,16 @@ class GpuAccelerator:
             return
         e = np.ascontiguousarray(rgb, np.float32)
         self._rc(self._lib.apg_render_set_environment(self._h, _ptr(e), e.shape[1], e.shape[0]))
+
+    def render_launch(self):
+        """The wavefront stages' workgroups per CU (apg_render_launch)."""
+        r = _RenderLaunch()
+        self._rc(self._lib.apg_render_get_launch(self._h, C.byref(r)))
+        return {"shade": r.shade_blocks_per_cu, "shadow": r.shadow_blocks_per_cu, "resolve": r.resolve_blocks_per_cu}
+
+    def set_render_launch(self, shade=0, shadow=0, resolve=0):
+        r = _RenderLaunch(shade, shadow, resolve, 0)
+        self._rc(self._lib.apg_render_set_launch(self._h, C.byref(r)))
 
     def guide_stats(self):
         n, loss = C.c_uint32(), C.c_float()

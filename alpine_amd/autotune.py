@@ -1,11 +1,14 @@
-"""Launch-shape autotuner for the persistent traversal kernel.
+"""Launch-shape autotuner for the persistent traversal kernel and the render stages.
 
-Sweeps the `apg_trace_config` knobs -- workgroup size, persistent blocks per CU,
+`autotune` sweeps the `apg_trace_config` knobs -- workgroup size, persistent blocks per CU,
 the idle-lane threshold at which a wave refills itself from the ray cursor,
 the LDS depth of the per-lane traversal stack and the number of queued triangle
 tests that starts a wave-wide triangle round -- on a caller-supplied ray batch,
 timing each candidate with HIP events on the launch stream, and leaves the
-fastest configuration set on the accelerator.
+fastest configuration set on the accelerator.  `autotune_render` does the same for
+a render: the paths in flight (sample slots per pixel) and the persistent grids of
+the shade, shadow-visibility and resolve stages (`apg_render_launch`), timed over
+whole passes of the caller's frame.
 """
 from __future__ import annotations
 
@@ -69,3 +72,62 @@ def autotune(acc, org, dir, quick=False, reps=3, any_hit=False, log=None):
         log(f"[autotune] {len(rows)} configs in {time.perf_counter() - t0:.1f} s; best {best} "
             f"{rows[0][0]:.3f} ms, default-like worst {rows[-1][0]:.3f} ms")
     return best, rows
+
+
+# Wavefront render stages: persistent 256-thread workgroups per CU of the shade, shadow-visibility
+# and resolve stages (apg_render_launch), and the paths in flight (sample slots per pixel).
+SHADE_BLOCKS = (4, 6, 8, 12, 16)
+SHADOW_BLOCKS = (2, 4, 8)
+RESOLVE_BLOCKS = (2, 4, 8)
+
+
+def _time_render(acc, width, height, spp, max_depth, rr_depth, background, slots, reps, seed=9, **kw):
+    import torch
+    acc.render_reset(width, height, seed, slots=slots)
+    acc.render(spp, max_depth, rr_depth, background, **kw)  # warm-up (and capture, with graph=True)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        acc.render(spp, max_depth, rr_depth, background, **kw)
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def autotune_render(acc, width, height, spp, max_depth, rr_depth=0, background=(0.0, 0.0, 0.0),
+                    slots=(1,), reps=2, log=None, **render_kw):
+    """Tune the render launch on the frame the accelerator holds: first the sample slots per
+    pixel (paths in flight = width x height x slots) at the default stage grids, then the shade,
+    shadow and resolve grids one stage at a time at the best slots.  spp must be a multiple of
+    every slot count.  Leaves the best grids set and returns (best dict, [(ms, dict)]) with
+    samples per second per candidate; the caller resets the render before its own passes."""
+    t0 = time.perf_counter()
+    acc.set_render_launch()
+    rows = []
+
+    def run(sl, shade, shadow, resolve):
+        acc.set_render_launch(shade, shadow, resolve)
+        ms = _time_render(acc, width, height, spp, max_depth, rr_depth, background, sl, reps, **render_kw)
+        cfg = {"slots": sl, "shade": shade, "shadow": shadow, "resolve": resolve}
+        rows.append((ms, cfg))
+        return ms
+
+    d = acc.render_launch()
+    best_sl = min(slots, key=lambda sl: run(sl, d["shade"], d["shadow"], d["resolve"]))
+    best = {"shade": d["shade"], "shadow": d["shadow"], "resolve": d["resolve"]}
+    for stage, values in (("shade", SHADE_BLOCKS), ("shadow", SHADOW_BLOCKS), ("resolve", RESOLVE_BLOCKS)):
+        timed = []
+        for v in values:
+            trial = dict(best, **{stage: v})
+            timed.append((run(best_sl, trial["shade"], trial["shadow"], trial["resolve"]), v))
+        best[stage] = min(timed)[1]
+    acc.set_render_launch(best["shade"], best["shadow"], best["resolve"])
+    ms_best = min(ms for ms, c in rows if c == dict(best, slots=best_sl))
+    out = dict(best, slots=best_sl, ms_per_call=round(ms_best, 4),
+               samples_per_s=round(width * height * spp / (ms_best * 1e-3), 1))
+    if log:
+        log(f"[autotune] render: {len(rows)} launches in {time.perf_counter() - t0:.1f} s; best {out}")
+    rows.sort(key=lambda r: r[0])
+    return out, rows

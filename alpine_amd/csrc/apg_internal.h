@@ -107,6 +107,7 @@ struct Context
 
     // traversal launch state
     apg_trace_config cfg{};
+    apg_render_launch rlaunch{ 8u, 4u, 4u, 0u };  // wavefront stage grids (blocks per CU)
     std::mutex launch_mutex;
     hipEvent_t last_launch = nullptr;
     hipEvent_t noalloc_launch = nullptr;  // the last apg_intersect_noalloc outside graph capture

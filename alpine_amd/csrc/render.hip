@@ -1320,6 +1320,26 @@ int apg_render_reset_slots(apg_context* ctx, uint32_t width, uint32_t height, ui
     return APG_OK;
 }
 
+int apg_render_set_launch(apg_context* ctx, const apg_render_launch* launch)
+{
+    if (!ctx || !launch) return APG_E_INVALID;
+    const uint32_t v[3] = { launch->shade_blocks_per_cu, launch->shadow_blocks_per_cu, launch->resolve_blocks_per_cu };
+    for (uint32_t x : v)
+        if (x > 64u) return ctx->fail(APG_E_INVALID, "render stage grids take 1..64 workgroups per CU (0: default)");
+    std::lock_guard<std::mutex> lock(ctx->launch_mutex);
+    ctx->rlaunch.shade_blocks_per_cu = v[0] ? v[0] : 8u;
+    ctx->rlaunch.shadow_blocks_per_cu = v[1] ? v[1] : 4u;
+    ctx->rlaunch.resolve_blocks_per_cu = v[2] ? v[2] : 4u;
+    return APG_OK;
+}
+
+int apg_render_get_launch(const apg_context* ctx, apg_render_launch* launch)
+{
+    if (!ctx || !launch) return APG_E_INVALID;
+    *launch = ctx->rlaunch;
+    return APG_OK;
+}
+
 int apg_render_reset(apg_context* ctx, uint32_t width, uint32_t height, uint32_t seed)
 {
     return apg_render_reset_slots(ctx, width, height, seed, 1);
@@ -1446,9 +1466,10 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
     }
     const float frac = prm->guide_frac > 0.0f ? std::min(prm->guide_frac, 0.95f) : 0.5f;
     const uint32_t train_steps = prm->train_steps ? prm->train_steps : 4u;
-    const uint32_t sgrid = std::min<uint32_t>(nblocks(P, 256), uint32_t(ctx->cu_count) * 8);
-    const uint32_t rgrid = std::min<uint32_t>(nblocks(P, 256), uint32_t(ctx->cu_count) * 4);
-    const uint32_t vgrid = std::min<uint32_t>(nblocks(2 * P, 256), uint32_t(ctx->cu_count) * 4);
+    const apg_render_launch& rl = ctx->rlaunch;
+    const uint32_t sgrid = std::min<uint32_t>(nblocks(P, 256), uint32_t(ctx->cu_count) * rl.shade_blocks_per_cu);
+    const uint32_t rgrid = std::min<uint32_t>(nblocks(P, 256), uint32_t(ctx->cu_count) * rl.resolve_blocks_per_cu);
+    const uint32_t vgrid = std::min<uint32_t>(nblocks(2 * P, 256), uint32_t(ctx->cu_count) * rl.shadow_blocks_per_cu);
 
     std::lock_guard<std::mutex> lock(ctx->launch_mutex);
     if (!ctx->last_launch) APG_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->last_launch, hipEventDisableTiming));
@@ -1529,6 +1550,7 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
         put(&R->cam, sizeof(R->cam));
         put(&ctx->scene, sizeof(ctx->scene));
         put(&ctx->cfg, sizeof(ctx->cfg));
+        put(&ctx->rlaunch, sizeof(ctx->rlaunch));
         const uintptr_t extra[] = { P, R->W, R->H, uintptr_t(R->accum), uintptr_t(R->accum64), prm->fp64_accum,
                                     uintptr_t(R->gfeat), uintptr_t(R->gout), guide, uintptr_t(ctx->spill),
                                     prm->guide_depth, prm->count_fetches };

@@ -48,12 +48,12 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=262_144)
     ap.add_argument("--builder", choices=["gpu", "host"], default="gpu")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing configs")
-    ap.add_argument("--frames", type=int, default=3,
-                    help="frames of the batched animated config (60 for the full run)")
+    ap.add_argument("--frames", type=int, default=60,
+                    help="frames of the batched animated config (the workload states 60)")
     ap.add_argument("--autotune", action="store_true",
-                    help="sweep the launch shape first (the library default is the sweep's pick)")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape "
-                    "(the default behaviour)")
+                    help="sweep the launch shapes first (the default; kept for old command lines)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default launch shapes (no sweeps)")
     ap.add_argument("--render-config", choices=sorted(RENDER_CONFIGS),
                     help="run only this path-tracing config (or the MLP kernels) and print its "
                          "figures as one JSON line; for profiling, not the headline measurement")
@@ -162,16 +162,16 @@ def _timed_render(acc, sc, spp, seed, **kw):
     return time.perf_counter() - t0, img, img64
 
 
-def render_benchmarks(builder, frames=3):
+def render_benchmarks(builder, frames=60, tune=True):
     """Path-tracing configs: Cornell room (16 spp, 256^2), many-light GGX (1920x1080, 64 spp,
     light BVH, RR from depth 3), 32 animated scenes in one tiled frame, guided vs plain
     time to equal relMSE (box grid behind a doorway) and the guide network's kernels."""
     out = {}
     out.update(cornell_benchmark(builder))
     log("[bench] many-light scene")
-    out.update(many_light_benchmark(builder))
+    out.update(many_light_benchmark(builder, tune=tune))
     log("[bench] batched animated scenes")
-    out["batched_32_scenes_512_32spp"] = batched_benchmark(frames=frames)
+    out["batched_32_scenes_512_32spp"] = batched_benchmark(frames=frames, tune=tune)
     log("[bench] guided vs plain")
     out["guided_vs_plain"] = guided_benchmark(builder)
     log("[bench] guide network kernels")
@@ -209,13 +209,19 @@ def cornell_benchmark(builder):
     return out
 
 
-def many_light_benchmark(builder):
+def many_light_benchmark(builder, tune=True):
     """Config 3: 1920 x 1080, 64 spp, GGX materials, 10,000 emissive triangles behind the
-    light BVH, RR from depth 3; relMSE against a 1024-spp FP64 reference."""
+    light BVH, RR from depth 3; relMSE against a 1024-spp FP64 reference.  With tune, the
+    render stages' grids are swept on 4-spp passes of the same frame first."""
     from alpine_amd import scenes
     out = {}
     sc = scenes.many_light_scene()
     acc = _render_setup(sc, builder)
+    launch = None
+    if tune:
+        from alpine_amd import autotune
+        launch, _ = autotune.autotune_render(acc, sc.width, sc.height, 4, sc.max_depth, sc.rr_depth,
+                                             sc.background, log=log)
     _timed_render(acc, sc, 1, 0)
     dt, img, _ = _timed_render(acc, sc, sc.spp, sc.seed)
     c = acc.render_counters()
@@ -232,7 +238,8 @@ def many_light_benchmark(builder):
         "finite": bool(np.isfinite(img).all()),
         "rel_mse_vs_1024spp_fp64": round(scenes.rel_mse(img / sc.spp, ref), 6),
         "rel_mse_16spp_vs_1024spp_fp64": round(scenes.rel_mse(img16 / 16, ref), 6),
-        "bytes_per_sample": bps, "samples_frac_hbm": round(bps["total"] * sps / 1e9 / HBM_PEAK_GBS, 4)}
+        "bytes_per_sample": bps, "samples_frac_hbm": round(bps["total"] * sps / 1e9 / HBM_PEAK_GBS, 4),
+        "launch": launch or acc.render_launch()}
     return out
 
 
@@ -403,7 +410,7 @@ def guided_benchmark(builder):
     }
 
 
-def batched_benchmark(frames=3, spp=32, max_depth=5, rr_depth=3, seed=1):
+def batched_benchmark(frames=60, spp=32, max_depth=5, rr_depth=3, seed=1, tune=True):
     """Many independent scenes, animated, one tiled frame: 32 scenes of 10 K - 2 M triangles
     (log-uniform, about 12 M in all) in a per-shape forest, 512 x 512 tiles, 32 spp per frame.
     Each frame turns the last 25 % of every scene's triangles about y on the GPU, rebuilds all
@@ -427,6 +434,10 @@ def batched_benchmark(frames=3, spp=32, max_depth=5, rr_depth=3, seed=1):
     acc.set_lights([aa.light(**l) for l in B.lights], 0)
     acc.set_tiles([aa.camera(**c) for c in B.cams], B.tile, B.tile, roots=np.arange(len(B.cams)))
     W, H = B.tile * B.tiles_x, B.tile * B.tiles_y
+    launch = None
+    if tune:
+        from alpine_amd import autotune
+        launch, _ = autotune.autotune_render(acc, W, H, 4, max_depth, rr_depth, log=log, graph=True)
     rot = B.rotation(2.0 * np.pi / 120.0)
     builds, renders = [], []
     for f in range(frames):
@@ -465,6 +476,7 @@ def batched_benchmark(frames=3, spp=32, max_depth=5, rr_depth=3, seed=1):
                                   / float(np.median(builds)) / 1e9 / HBM_PEAK_GBS, 5),
         "finite": bool(np.isfinite(img).all()), "mean_radiance": round(float(img.mean() / total), 5),
         f"rel_mse_vs_{BATCH_REF_SPP}spp_fp64": round(rel, 6),
+        "launch": launch or acc.render_launch(),
     }
 
 
@@ -517,8 +529,9 @@ RENDER_CONFIGS = {
     "many_light_pass": lambda a: render_pass("many_light", _builder_id(a.builder)),
     "guided_pass": lambda a: render_pass("guided", _builder_id(a.builder)),
     "cornell": lambda a: cornell_benchmark(_builder_id(a.builder)),
-    "many_light": lambda a: many_light_benchmark(_builder_id(a.builder)),
-    "batched": lambda a: {"batched_32_scenes_512_32spp": batched_benchmark(frames=a.frames)},
+    "many_light": lambda a: many_light_benchmark(_builder_id(a.builder), tune=not a.no_autotune),
+    "batched": lambda a: {"batched_32_scenes_512_32spp": batched_benchmark(frames=a.frames,
+                                                                          tune=not a.no_autotune)},
     "guided": lambda a: {"guided_vs_plain": guided_benchmark(_builder_id(a.builder))},
     "mlp": lambda a: {"guide_mlp": mlp_benchmark()},
 }
@@ -642,11 +655,10 @@ def main():
     torch.cuda.synchronize()
 
     tuned = None
-    if args.autotune and not args.no_autotune:
+    if not args.no_autotune:
+        # swept on the timed ray set itself (54 shapes x 6 launches of ~2 ms)
         from alpine_amd import autotune
-        n_t = 1 << 20
-        tuned, _ = autotune.autotune(acc, o_d[:, :n_t].contiguous(), d_d[:, :n_t].contiguous(),
-                                     quick=True, log=log)
+        tuned, _ = autotune.autotune(acc, o_d, d_d, quick=True, reps=5, log=log)
     log("[bench] timing incoherent closest-hit")
     ms, wall_ms = timed_passes(lambda: acc.intersect(o_d, d_d, out=out), args.steps, args.warmup)
     log(f"[bench] incoherent closest {ms:.3f} ms/step")
@@ -729,7 +741,7 @@ def main():
     }
     if not args.no_render:
         log("[bench] render configs")
-        result["extra"]["render"] = render_benchmarks(builder, args.frames)
+        result["extra"]["render"] = render_benchmarks(builder, args.frames, tune=not args.no_autotune)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         cb, href, _ = cpu_baseline(verts, tris, org, dir, args.cpu_sample)

@@ -329,6 +329,19 @@ typedef struct apg_render_params {
 /* Allocate per-pixel samplers (seeded with pixel index + seed, alpine.cpp:259-269) and clear
  * the accumulation (Alpine::resetAccumulation). */
 int apg_render_reset(apg_context* ctx, uint32_t width, uint32_t height, uint32_t seed);
+/* Launch shape of the wavefront stages (no reference counterpart; the autotuner
+ * alpine_amd/autotune.py sets it): persistent 256-thread workgroups per CU of the shade,
+ * shadow-visibility and resolve stages, each 1..64; 0 keeps that stage's default
+ * (8, 4, 4).  Applies from the next apg_render call; a captured pass is captured again. */
+typedef struct apg_render_launch
+{
+    uint32_t shade_blocks_per_cu;
+    uint32_t shadow_blocks_per_cu;
+    uint32_t resolve_blocks_per_cu;
+    uint32_t reserved;
+} apg_render_launch;
+int apg_render_set_launch(apg_context* ctx, const apg_render_launch* launch);
+int apg_render_get_launch(const apg_context* ctx, apg_render_launch* launch);
 /* As apg_render_reset with `slots` samples of every pixel traced side by side in each pass
  * (width x height x slots paths), which fills the GPU on small frames.  Slot 0 of each pixel
  * keeps the reference's per-pixel random sequence; the other slots are seeded from a hash.

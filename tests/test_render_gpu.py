@@ -225,3 +225,26 @@ def test_counted_render_is_identical_and_counts_fetches():
         else:
             assert f == {"closest_nodes": 0, "closest_tris": 0, "shadow_nodes": 0, "shadow_tris": 0}
     np.testing.assert_array_equal(outs[False], outs[True])
+
+
+def test_render_launch_shape_does_not_change_the_image():
+    """The wavefront stages' grids (apg_render_launch) only change how paths are spread over
+    workgroups: each path keeps its own sampler and accumulation slot, so the image is
+    bit-identical under any launch shape, with stream launches and with graph replay."""
+    print("[gpu] render launch shape", flush=True)
+    sc = scenes.cornell_scene(96, 64, spp=4)
+    acc, ref, _, _ = gpu_render(sc)
+    assert acc.render_launch() == {"shade": 8, "shadow": 4, "resolve": 4}
+    for shade, shadow, resolve in ((1, 1, 1), (16, 8, 2), (64, 64, 64)):
+        acc.set_render_launch(shade, shadow, resolve)
+        assert acc.render_launch() == {"shade": shade, "shadow": shadow, "resolve": resolve}
+        for graph in (False, True):
+            acc.render_reset(sc.width, sc.height, sc.seed)
+            acc.render(sc.spp, sc.max_depth, sc.rr_depth, sc.background, graph=graph)
+            img, _, total = acc.render_read()
+            assert total == sc.spp
+            np.testing.assert_array_equal(img, ref)
+    with pytest.raises(aa.ApgError):
+        acc.set_render_launch(65, 4, 4)
+    acc.set_render_launch()  # zeros: the defaults
+    assert acc.render_launch() == {"shade": 8, "shadow": 4, "resolve": 4}
