@@ -379,7 +379,7 @@ class GpuAccelerator:
         self._rc(self._lib.apg_mesh_prim_offset(self._h, shape_id, C.byref(o)))
         return o.value
 
-    def update_scene(self, leaf_threshold: int = 2, builder: int = BUILDER_GPU,
+    def update_scene(self, leaf_threshold: int = 1, builder: int = BUILDER_GPU,
                      per_shape: bool = False, stats: bool = True):
         """Rebuild (Accelerator::updateScene).  per_shape: an independent tree per shape
         (GPU builder), wide node s the root of shape s; stats=False skips the host-side

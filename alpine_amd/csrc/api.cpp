@@ -277,7 +277,7 @@ int apg_scene_build(apg_context* ctx, const apg_build_params* params)
     if (!ctx) return APG_E_INVALID;
     APG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     auto t0 = std::chrono::steady_clock::now();
-    uint32_t leaf_threshold = params && params->leaf_threshold ? params->leaf_threshold : 2u;
+    uint32_t leaf_threshold = params && params->leaf_threshold ? params->leaf_threshold : 1u;
     uint32_t builder = params ? params->builder : APG_BUILDER_GPU;
     const uint32_t roots = params ? params->roots : APG_ROOTS_SCENE;
     const bool stats = !(params && (params->flags & APG_BUILD_NO_STATS));

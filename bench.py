@@ -33,6 +33,12 @@ METRIC = "Mrays/s incoherent closest-hit; time to equal relMSE, guided vs plain 
 HBM_PEAK_GBS = 8000.0
 N_RAYS = 4_194_304
 B_RAY_IN, B_HIT_OUT, B_OCC_OUT, B_NODE, B_TRI = 28, 16, 4, 64, 36
+# binned-SAH leaf threshold of the microbench tree: one triangle per leaf where the SAH allows
+# (a triangle test through the wave's queue costs more than the node visits it saves)
+LEAF_THRESHOLD = 1
+# the animated config rebuilds every frame: leaf threshold 2 builds its forest 7 ms faster for
+# the same render time
+BATCH_LEAF_THRESHOLD = 2
 
 
 def log(msg):
@@ -426,7 +432,7 @@ def batched_benchmark(frames=60, spp=32, max_depth=5, rr_depth=3, seed=1, tune=T
     for v, f in B.meshes:
         acc.append_mesh(v, f)
     t0 = time.perf_counter()
-    acc.update_scene(per_shape=True, stats=False)
+    acc.update_scene(leaf_threshold=BATCH_LEAF_THRESHOLD, per_shape=True, stats=False)
     torch.cuda.synchronize()
     t_first = time.perf_counter() - t0
     n_tris = B.triangles
@@ -445,7 +451,7 @@ def batched_benchmark(frames=60, spp=32, max_depth=5, rr_depth=3, seed=1, tune=T
         a = time.perf_counter()
         for k, (first, count) in enumerate(B.anim_range):
             acc.transform_vertices(k, rot, first=first, count=count)
-        acc.update_scene(per_shape=True, stats=False)
+        acc.update_scene(leaf_threshold=BATCH_LEAF_THRESHOLD, per_shape=True, stats=False)
         torch.cuda.synchronize()
         b = time.perf_counter()
         acc.render_reset(W, H, seed + f)
@@ -633,14 +639,14 @@ def main():
     acc = GpuAccelerator(0)
     acc.append_mesh(verts, tris)
     builder = _builder_id(args.builder)
-    acc.update_scene(leaf_threshold=2, builder=builder)
+    acc.update_scene(leaf_threshold=LEAF_THRESHOLD, builder=builder)
     st = acc.stats()
     # steady-state rebuild of the same geometry (the first build also loads the kernels)
     rebuild = []
     for _ in range(3):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        acc.update_scene(leaf_threshold=2, builder=builder, stats=False)
+        acc.update_scene(leaf_threshold=LEAF_THRESHOLD, builder=builder, stats=False)
         torch.cuda.synchronize()
         rebuild.append(time.perf_counter() - t0)
     rebuild_s = float(np.median(rebuild))

@@ -63,7 +63,7 @@ typedef struct apg_hits {
 
 /* Which builder and which tree shape scene_build produces. */
 typedef struct apg_build_params {
-    uint32_t leaf_threshold;  /* binned-SAH leaf threshold (reference Bvh: 8); 0 = default 2 */
+    uint32_t leaf_threshold;  /* binned-SAH leaf threshold (reference Bvh: 8); 0 = default 1 */
     uint32_t builder;         /* APG_BUILDER_GPU (default) or APG_BUILDER_HOST */
     uint32_t roots;           /* APG_ROOTS_SCENE: one tree over all shapes (the reference's
                                * updateScene); APG_ROOTS_PER_SHAPE: an independent tree per shape,
