@@ -60,6 +60,10 @@ def parse():
                     help="sweep the launch shapes first (the default; kept for old command lines)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch shapes (no sweeps)")
+    ap.add_argument("--trace-launch", metavar="B,P,F,S,T",
+                    help="use this traversal launch shape (block size, blocks per CU, fetch "
+                         "threshold, LDS stack entries, triangle batch) instead of the sweep, "
+                         "e.g. to profile exactly the shape a bench run picked")
     ap.add_argument("--render-config", choices=sorted(RENDER_CONFIGS),
                     help="run only this path-tracing config (or the MLP kernels) and print its "
                          "figures as one JSON line; for profiling, not the headline measurement")
@@ -661,7 +665,12 @@ def main():
     torch.cuda.synchronize()
 
     tuned = None
-    if not args.no_autotune:
+    if args.trace_launch:
+        from alpine_amd import autotune
+        vals = [int(x) for x in args.trace_launch.split(",")]
+        tuned = dict(zip(autotune.KNOBS, vals))
+        acc.set_trace_config(autotune.make_config(acc.trace_config(), **tuned))
+    elif not args.no_autotune:
         # swept on the timed ray set itself (54 shapes x 6 launches of ~2 ms)
         from alpine_amd import autotune
         tuned, _ = autotune.autotune(acc, o_d, d_d, quick=True, reps=5, log=log)
