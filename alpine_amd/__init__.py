@@ -159,7 +159,8 @@ DEBUG_SYMBOLS = ["apg_debug_host_build", "apg_debug_scene_info", "apg_debug_scen
                  "apg_debug_host_trace", "apg_debug_scene_free", "apg_debug_material",
                  "apg_debug_light_sampler", "apg_debug_light", "apg_debug_shading",
                  "apg_debug_environment", "apg_debug_triangles", "apg_debug_host_trace_limited",
-                 "apg_debug_scene_depth", "apg_debug_fail_allocations", "apg_debug_stream_read"]
+                 "apg_debug_scene_depth", "apg_debug_fail_allocations", "apg_debug_stream_read",
+                 "apg_debug_host_rate"]
 
 
 def lib():
@@ -207,6 +208,9 @@ def lib():
         "apg_lights_set": (i32, [vp, vp, u32, u32]),
         "apg_light_sampler_sample": (i32, [vp, vp, vp, vp, u32, vp, vp, vp]),
         "apg_light_sampler_pdf": (i32, [vp, vp, vp, vp, u32, vp, vp]),
+        "apg_light_sampler_sample_host": (i32, [vp, vp, vp, vp, u32, vp, vp]),
+        "apg_intersect_host": (i32, [vp, vp, u32, vp, vp, vp, vp]),
+        "apg_occluded_host": (i32, [vp, vp, vp, u32, vp]),
         "apg_render_reset": (i32, [vp, u32, u32, u32]),
         "apg_render_reset_slots": (i32, [vp, u32, u32, u32, u32]),
         "apg_render_set_camera": (i32, [vp, C.POINTER(Camera)]),
@@ -245,6 +249,7 @@ def lib():
         "apg_debug_scene_depth": (i32, [vp, C.POINTER(u32)]),
         "apg_debug_fail_allocations": (i32, [vp, i32]),
         "apg_debug_stream_read": (i32, [vp, C.c_uint64, vp, u32, vp]),
+        "apg_debug_host_rate": (i32, [vp, vp, u32, u32, u32, i32, vp, vp, vp, C.POINTER(C.c_double)]),
         "apg_debug_material": (i32, [C.POINTER(Material)] + [vp] * 3 + [u32] + [vp] * 6),
         "apg_debug_light_sampler": (i32, [vp, u32, u32, vp, vp, vp, u32, vp, vp, vp]),
         "apg_debug_light": (i32, [C.POINTER(Light), vp, vp, vp, u32] + [vp] * 7),
@@ -534,6 +539,20 @@ class GpuAccelerator:
                                                     _ptr(light), _ptr(pdf), self._stream(stream)))
         return light, pdf
 
+    def light_sampler_sample_host(self, u, hit, ns):
+        """LightSampler::sample for host arrays u (n), hit/ns (n, 3), synchronous; returns
+        numpy (light, pdf)."""
+        u = np.ascontiguousarray(u, np.float32)
+        hit = np.ascontiguousarray(hit, np.float32)
+        ns = np.ascontiguousarray(ns, np.float32)
+        n = u.shape[0]
+        assert hit.shape == (n, 3) and ns.shape == (n, 3)
+        light = np.empty(n, np.int32)
+        pdf = np.empty(n, np.float32)
+        self._rc(self._lib.apg_light_sampler_sample_host(self._h, _ptr(u), _ptr(hit), _ptr(ns), n,
+                                                         _ptr(light), _ptr(pdf)))
+        return light, pdf
+
     def light_sampler_pdf(self, light, hit, ns, stream=None):
         import torch
         n = light.shape[0]
@@ -696,6 +715,23 @@ class GpuAccelerator:
         ids = np.empty(n, np.uint32)
         self._rc(self._lib.apg_scene_triangles(self._h, _ptr(tri9), _ptr(ids), n))
         return tri9, ids
+
+    def host_rate(self, rays6, nthreads, rays_per_call=1, any_hit=False):
+        """The per-ray host entry points from `nthreads` native threads (apg_debug_host_rate):
+        (results, seconds); results are (prim, t) or the occlusion bytes."""
+        r = np.ascontiguousarray(rays6, np.float32).reshape(-1, 6)
+        n = len(r)
+        sec = C.c_double()
+        if any_hit:
+            occ = np.empty(n, np.uint8)
+            self._rc(self._lib.apg_debug_host_rate(self._h, _ptr(r), n, nthreads, rays_per_call, 1, None, None,
+                                                   _ptr(occ), C.byref(sec)))
+            return occ, sec.value
+        prim = np.empty(n, np.uint32)
+        t = np.empty(n, np.float32)
+        self._rc(self._lib.apg_debug_host_rate(self._h, _ptr(r), n, nthreads, rays_per_call, 0, _ptr(prim), _ptr(t),
+                                               None, C.byref(sec)))
+        return (prim, t), sec.value
 
     def wide_nodes(self) -> np.ndarray:
         n = self.stats()["wide_nodes"]

@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <mutex>
 #include <string>
@@ -65,11 +66,24 @@ void build_cache_free(BuildCache* c);
 struct HostStaging
 {
     hipStream_t stream = nullptr;
-    size_t cap = 0;           // items
-    float* d_in = nullptr;    // 8 floats per item
-    float* d_out = nullptr;   // 4 floats per item
-    float* h_in = nullptr;    // pinned
-    float* h_out = nullptr;   // pinned
+    size_t cap = 0;            // items
+    float* h_in = nullptr;     // pinned: 7 floats per item (rays, SoA)
+    float* h_out = nullptr;    // pinned: 4 floats per item (hits, SoA), then the overflow word
+    float* dev_in = nullptr;   // the same two buffers as the kernel addresses them
+    float* dev_out = nullptr;
+    // device: this staging's own traversal counters (32 words) and stack spill, so batches on
+    // different stagings need no ordering between them
+    uint64_t* trace_words = nullptr;
+    uint32_t trace_threads = 0;  // persistent threads the spill holds
+};
+
+// Pending host-memory calls of the per-ray entry points (host_calls.cpp), per kind (closest,
+// any hit): the calls waiting for a batch and the batches in flight.
+struct HostCoalescer
+{
+    std::mutex mutex;                     // guards both members
+    std::vector<void*> pending[2];        // calls not yet taken into a batch, oldest first
+    uint32_t leaders[2] = { 0, 0 };       // batches running or handed to a waiting caller
 };
 
 struct RenderState;  // render.hip
@@ -134,6 +148,7 @@ struct Context
     std::vector<HostStaging*> staging_free, staging_all;
     HostStaging* take_staging(size_t n, int& rc);
     void give_staging(HostStaging* s);
+    HostCoalescer host_batches;
 
     ~Context();
     int rebuild_shading();  // per-triangle normals after a scene build (render.hip)

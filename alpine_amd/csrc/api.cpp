@@ -40,10 +40,9 @@ Context::~Context()
     for (HostStaging* s : staging_all)
     {
         if (s->stream) (void)hipStreamDestroy(s->stream);
-        if (s->d_in) (void)hipFree(s->d_in);
-        if (s->d_out) (void)hipFree(s->d_out);
         if (s->h_in) (void)hipHostFree(s->h_in);
         if (s->h_out) (void)hipHostFree(s->h_out);
+        if (s->trace_words) (void)hipFree(s->trace_words);
         delete s;
     }
     if (last_launch) (void)hipEventDestroy(last_launch);

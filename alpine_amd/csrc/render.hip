@@ -1067,15 +1067,11 @@ int apg_light_sampler_sample_host(apg_context* ctx, const float* u, const float*
             h[4 * N + 3 * size_t(i) + a] = ns[3 * size_t(i) + a];
         }
     }
-    float* d = s->d_in;
-    hipError_t e = hipMemcpyAsync(d, h, 7 * N * sizeof(float), hipMemcpyHostToDevice, s->stream);
-    if (e == hipSuccess)
-    {
-        k_ls_sample<<<nblocks(n, 256), 256, 0, s->stream>>>(view_of(R), d, d + N, d + 4 * N, n,
-                                                             reinterpret_cast<int32_t*>(s->d_out), s->d_out + N);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipMemcpyAsync(s->h_out, s->d_out, 2 * N * sizeof(float), hipMemcpyDeviceToHost, s->stream);
+    // the kernel reads the queries from and writes the samples to the pinned staging buffers
+    float* d = s->dev_in;
+    k_ls_sample<<<nblocks(n, 256), 256, 0, s->stream>>>(view_of(R), d, d + N, d + 4 * N, n,
+                                                         reinterpret_cast<int32_t*>(s->dev_out), s->dev_out + N);
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess)
     {

@@ -76,6 +76,7 @@ struct TraceArgs
     uint32_t fetch_threshold;
     uint32_t tri_batch;               // queued triangle tests that start a round
     uint32_t stack_max;               // stack entries a lane may hold (<= APG_STACK_MAX)
+    uint32_t* flag_out;               // when set, the last wave stores the overflow flag here
 };
 
 // Inclusive prefix sum over the 64 lanes of a wave: shift-and-add inside each
@@ -464,6 +465,8 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
         {
             A.counters[0] = 0u;
             A.counters[2] = 0u;
+            if (A.flag_out)
+                A.flag_out[0] = __hip_atomic_load(&A.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -563,6 +566,7 @@ struct TraceScratch
     uint32_t spill_stride;
     uint32_t max_threads;   // persistent threads the spill has room for (0: any)
     uint32_t stack_max;     // stack entries per lane
+    uint32_t* flag_out;     // optional: receives counters[1] at the end of the launch
 };
 
 namespace {
@@ -632,6 +636,7 @@ int launch_trace_kernel(Context* ctx, const apg_rays& rays, uint32_t n, const ui
     A.fetch_threshold = cfg.fetch_threshold;
     A.tri_batch = cfg.tri_batch >= 1u && cfg.tri_batch <= 64u ? cfg.tri_batch : 48u;
     A.stack_max = scr.stack_max;
+    A.flag_out = scr.flag_out;
 
     // the watertight test reads the vertex-form triangles (8-entry LDS stack only)
     const bool wt = cfg.triangle_test == APG_TRI_WATERTIGHT;
@@ -663,6 +668,34 @@ int launch_trace_kernel(Context* ctx, const apg_rays& rays, uint32_t n, const ui
 }
 
 } // namespace
+
+// Launch one traversal of a host batch on the staging's stream with the staging's own counters
+// and spill (grown to the batch's persistent grid), so concurrent batches need no ordering
+// between them.  The kernel's last wave copies the staging's overflow flag (counters[1]) to
+// flag_out, a host-visible word.
+int launch_trace_staged(Context* ctx, const apg_rays& rays, uint32_t n, const apg_hits* hits, bool any,
+                        uint8_t* occ, uint32_t* flag_out, HostStaging* s)
+{
+    if (n == 0) return APG_OK;
+    if (!ctx->built) return ctx->fail(APG_E_STATE, "apg_scene_build has not been called");
+    if (n > 0x7fffffffu) return ctx->fail(APG_E_CAPACITY, "more than 2^31 - 1 rays in one call");
+    const uint32_t threads = trace_grid(ctx, n, 0u) * ctx->cfg.block_size;
+    if (s->trace_threads < threads)
+    {
+        if (s->trace_words) APG_HIP_CHECK(ctx, hipFree(s->trace_words));
+        s->trace_words = nullptr;
+        s->trace_threads = 0;
+        const size_t words = NOALLOC_COUNTER_WORDS + size_t(threads) * APG_SPILL_ALLOC;
+        APG_HIP_CHECK(ctx, hipMalloc(&s->trace_words, words * sizeof(uint64_t)));
+        // counters [0] and [2] return to zero at the end of every launch; [1] is cleared by
+        // the caller after it reports an overflow
+        APG_HIP_CHECK(ctx, hipMemset(s->trace_words, 0, NOALLOC_COUNTER_WORDS * sizeof(uint64_t)));
+        s->trace_threads = threads;
+    }
+    const TraceScratch scr{ reinterpret_cast<uint32_t*>(s->trace_words), s->trace_words + NOALLOC_COUNTER_WORDS,
+                            APG_SPILL_ALLOC, s->trace_threads, config_stack_max(ctx->cfg), flag_out };
+    return launch_trace_kernel(ctx, rays, n, nullptr, hits, any, occ, nullptr, s->stream, nullptr, scr);
+}
 
 int Context::take_overflow()
 {

@@ -103,6 +103,22 @@ def _profiled_traffic():
         return None
 
 
+def per_ray_host_rate(acc, org, dir, n=1 << 18, threads=256):
+    """The drop-in adapter's per-ray path: `threads` host threads (alpine's 256 tile tasks,
+    alpine.cpp:276-285) each calling apg_intersect_host / apg_occluded_host with one ray per
+    call over the first n incoherent rays (host memory in and out, coalesced into shared
+    launches inside the library)."""
+    rays6 = np.ascontiguousarray(np.concatenate([org[:n], dir[:n]], 1), np.float32)
+    acc.host_rate(rays6[:4096], threads)  # warm-up
+    _, s_c = acc.host_rate(rays6, threads)
+    _, s_a = acc.host_rate(rays6, threads, any_hit=True)
+    # the same threads handing over 16 rays per call (a tile row per call)
+    _, s_16 = acc.host_rate(rays6, threads, rays_per_call=16)
+    return {"threads": threads, "rays": n, "rays_per_call": 1,
+            "closest_mrays": round(n / s_c / 1e6, 3), "any_mrays": round(n / s_a / 1e6, 3),
+            "closest_mrays_16_per_call": round(n / s_16 / 1e6, 3)}
+
+
 def hbm_read_gbs(gib=4, reps=5):
     """Achievable HBM read bandwidth: the library's streaming-read kernel (16-byte
     non-temporal loads, 8 workgroups of 256 per CU) over `gib` GiB, HIP events on its stream."""
@@ -684,6 +700,8 @@ def main():
     ms_wt, _ = timed_passes(lambda: acc.intersect(o_d, d_d, out=out), args.steps, 2)
     acc.set_triangle_test(TRI_REFERENCE)
 
+    log("[bench] per-ray host calls (the adapter's Accelerator::intersect path)")
+    per_ray = per_ray_host_rate(acc, org, dir)
     log("[bench] instrumented run (byte model)")
     _, n_node, n_tri = acc.intersect_counted(o_d, d_d)
     _, cn_node, cn_tri = acc.intersect_counted(co_d, cd_d)
@@ -745,6 +763,7 @@ def main():
             "incoherent_closest_watertight_mrays": round(N_RAYS / (ms_wt * 1e-3) / 1e6, 2),
             "any_bytes_per_ray": round(b_any, 1),
             "any_frac_hbm": round(b_any * N_RAYS / (ms_any * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "per_ray_host": per_ray,
             "build_ms_first": round(st["build_ms"], 1), "rebuild_ms": round(rebuild_s * 1e3, 2),
             "rebuild_mtris_per_s": round(st["triangles"] / rebuild_s / 1e6, 1),
             "build_bytes": b_build,

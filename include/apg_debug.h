@@ -43,6 +43,14 @@ int apg_debug_host_trace_limited(const apg_debug_scene* s, const float* org3, co
  * written only if the data's XOR is a fixed constant.  The achievable-bandwidth measurement of
  * the roofline. */
 int apg_debug_stream_read(const void* buf, uint64_t bytes, uint32_t* sink, uint32_t blocks, void* stream);
+/* Per-ray callers measured: nthreads host threads, each over a contiguous share of the n rays
+ * (alpine's Ray layout, 6 floats each, host), call apg_intersect_host (any = 0; prim and t
+ * receive the results) or apg_occluded_host (any = 1; occluded) with rays_per_call rays per
+ * call, as alpine's tile tasks call Accelerator::intersect per ray (alpine.cpp:276-285);
+ * seconds receives the wall time from the threads' common start to the last one's end. */
+int apg_debug_host_rate(apg_context* ctx, const float* rays6, uint32_t n, uint32_t nthreads,
+                        uint32_t rays_per_call, int any, uint32_t* prim, float* t, uint8_t* occluded,
+                        double* seconds);
 /* Render-state device allocations of ctx (render.hip) fail with APG_E_HIP after `after` more
  * have succeeded; after < 0 restores normal allocation.  For tests of the failure paths. */
 int apg_debug_fail_allocations(apg_context* ctx, int after);

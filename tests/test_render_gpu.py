@@ -47,6 +47,10 @@ def test_light_sampler_matches_oracle(kind):
     li, lp = acc.light_sampler_sample(torch.from_numpy(u).cuda(), torch.from_numpy(hit).cuda(),
                                       torch.from_numpy(ns).cuda())
     li, lp = li.cpu().numpy(), lp.cpu().numpy()
+    # the synchronous host-memory entry point runs the same kernel
+    hi, hp = acc.light_sampler_sample_host(u, hit, ns)
+    np.testing.assert_array_equal(hi, li)
+    np.testing.assert_array_equal(hp.view(np.uint32), lp.view(np.uint32))
     ref = [ors.light_sampler_sample(float(u[i]), hit[i], ns[i]) for i in range(n)]
     ri = np.array([r[0] for r in ref])
     rp = np.array([r[1] if r[0] >= 0 else 0.0 for r in ref], np.float32)

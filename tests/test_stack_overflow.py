@@ -164,12 +164,19 @@ def test_overflow_is_reported_on_the_gpu(chain):
     from alpine_amd import _ptr
     prim = np.empty(n, np.uint32)
     t, b0, b1 = (np.empty(n, np.float32) for _ in range(3))
-    rc = acc._lib.apg_intersect_host(acc._h, _ptr(rays6), n, _ptr(prim), _ptr(t), _ptr(b0), _ptr(b1))
-    assert rc == -6
+    occ = np.empty(n, np.uint8)
+    host = lambda: acc._lib.apg_intersect_host(acc._h, _ptr(rays6), n, _ptr(prim), _ptr(t), _ptr(b0), _ptr(b1))
+    assert host() == -6
+    assert host() == -6, "each host batch reports its own overflow"
+    assert acc._lib.apg_occluded_host(acc._h, _ptr(rays6), None, n, _ptr(occ)) == -6
     cfg.stack_limit = 0
     acc.set_trace_config(cfg)
     print("overflow test: default stack again", flush=True)
     check()
+    assert host() == 0
+    np.testing.assert_array_equal(t.view(np.uint32), h.t[:n].view(np.uint32))
+    assert acc._lib.apg_occluded_host(acc._h, _ptr(rays6), None, n, _ptr(occ)) == 0
+    np.testing.assert_array_equal(occ, occ_ref[:n])
     acc.close()
 
 
