@@ -18,6 +18,8 @@
 //            allocated contiguously after the level.
 //   partition stable partition of each node's slot range: a global exclusive
 //            scan of the "below" flags gives every slot its new position.
+// A node of at most FIN slots leaves the level loop; after the loop, k_finish builds each
+// such subtree in one workgroup's LDS with the same rules (no per-level launches below).
 // Min/max are exact and order independent and the sweep is sequential per
 // node with the reference's operand order, so with the reference's leaf
 // threshold the leaf sets equal the reference builder's.
@@ -48,10 +50,13 @@ constexpr uint32_t RETIRED = 0xFFFFFFFFu;
 // Nodes of at most SMALL slots get one wave each for bounds and bins (no global
 // atomics); larger nodes are accumulated by chunks of CHUNK slots.
 constexpr uint32_t SMALL = CHUNK;
-// Nodes of at most TINY slots get one thread each: bounds by a serial loop, and
+// In k_finish, nodes of at most TINY slots get one thread each: bounds by a serial loop, and
 // the SAH sweep evaluates the splits from the slots directly (no histogram).
 constexpr uint32_t TINY = 16;
-constexpr uint32_t TINY_SLOT = 0xFFFFFFFEu;  // nbinslot of a tiny SAH candidate
+// Nodes of at most FIN slots leave the level loop: one workgroup builds each one's whole
+// subtree in LDS (k_finish).
+constexpr uint32_t FIN = 1024;
+static_assert(FIN >= TINY && FIN <= SMALL && FIN < 0x8000u, "finisher size");
 
 enum : uint32_t { NF_LEAF = 1u, NF_SAH_LEAF = 2u, NF_FORCED = 4u, NF_SPLIT = 8u };
 
@@ -112,10 +117,13 @@ struct BuildState
     uint32_t* bins;     // NODE_BIN_WORDS per candidate
     uint32_t* counters; // [0] node count, [1] bin slots, [2] tmp
     uint32_t* scan_tmp;
-    // the level being built: [0] first node, [1] end, [2] levels done, [3] listed large nodes
+    // the level being built: [0] first node, [1] end, [2] levels done, [3] listed large nodes,
+    // [4] nodes left to k_finish, [5] k_finish's cursor over them, [6] node ids allocated,
+    // [7] binary nodes built (allocated less k_finish's unused reservations)
     uint32_t* lvl;
     uint32_t* large;      // the level's nodes of more than SMALL slots
     uint32_t* large_off;  // exclusive scan of their chunk counts (total at [count])
+    uint32_t* fin;        // nodes of at most FIN slots, left to k_finish (count at lvl[4])
 };
 
 // ---------------------------------------------------------------------------
@@ -209,6 +217,13 @@ __global__ __launch_bounds__(256) void k_level_init(BuildState S)
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x; k < end; k += stride)
     {
+        if (S.ncount[k] <= FIN)
+        {
+            // built by k_finish after the level loop; every kernel of the level passes it by
+            S.nbinslot[k] = RETIRED;
+            S.fin[atomicAdd(&S.lvl[4], 1u)] = k;
+            continue;
+        }
         for (int a = 0; a < 3; ++a)
         {
             S.nbox[6 * size_t(k) + a] = ord_of(FLT_MAX);
@@ -339,7 +354,7 @@ __global__ __launch_bounds__(256) void k_decide(BuildState S, uint32_t threshold
         const uint32_t k = k0 + lane;
         bool cand = false;
         uint32_t cnt = 0;
-        if (k < end)
+        if (k < end && S.ncount[k] > FIN)
         {
             cnt = S.ncount[k];
             uint32_t fl = S.nflags[k];
@@ -369,8 +384,6 @@ __global__ __launch_bounds__(256) void k_decide(BuildState S, uint32_t threshold
                     fl |= cnt <= APG_MAX_LEAF ? NF_LEAF : (NF_FORCED | NF_SPLIT);
                     S.nflags[k] = fl;
                 }
-                else if (cnt <= TINY)
-                    S.nbinslot[k] = TINY_SLOT;
                 else
                     cand = true;
             }
@@ -460,7 +473,7 @@ __global__ __launch_bounds__(BLOCK) void k_bounds_small(BuildState S)
     for (uint32_t k = begin + (blockIdx.x * BLOCK + threadIdx.x) / 64u; k < end; k += waves)
     {
         const uint32_t cnt = S.ncount[k];
-        if (cnt <= TINY || cnt > SMALL) continue;
+        if (cnt <= FIN || cnt > SMALL) continue;
         const uint32_t first = S.nfirst[k];
         const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
         uint32_t r[12];
@@ -497,42 +510,6 @@ __global__ __launch_bounds__(BLOCK) void k_bounds_small(BuildState S)
     }
 }
 
-// Bounds of a node of at most TINY slots: one thread, a serial loop.
-__global__ __launch_bounds__(256) void k_bounds_tiny(BuildState S)
-{
-    const uint32_t begin = S.lvl[0], end = S.lvl[1];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x; k < end; k += stride)
-    {
-        const uint32_t cnt = S.ncount[k];
-        if (cnt == 0u || cnt > TINY) continue;
-        const uint32_t first = S.nfirst[k];
-        const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
-        uint32_t r[12];
-        for (int a = 0; a < 3; ++a)
-        {
-            r[a] = lo_ord; r[3 + a] = hi_ord; r[6 + a] = lo_ord; r[9 + a] = hi_ord;
-        }
-        for (uint32_t i = first; i < first + cnt; ++i)
-        {
-            const float* b = S.pbox + 6 * size_t(S.idx[i]);
-            for (int a = 0; a < 3; ++a)
-            {
-                const float lo = b[a], hi = b[3 + a], c = (lo + hi) * 0.5f;
-                r[a] = min(r[a], ord_of(lo));
-                r[3 + a] = max(r[3 + a], ord_of(hi));
-                r[6 + a] = min(r[6 + a], ord_of(c));
-                r[9 + a] = max(r[9 + a], ord_of(c));
-            }
-        }
-        for (int j = 0; j < 6; ++j)
-        {
-            S.nbox[6 * size_t(k) + j] = r[j];
-            S.ncbox[6 * size_t(k) + j] = r[6 + j];
-        }
-    }
-}
-
 // Bins of a node of at most SMALL slots: one wave, a histogram of its own in LDS,
 // written out whole.
 __global__ __launch_bounds__(BLOCK) void k_bin_small(BuildState S)
@@ -545,9 +522,9 @@ __global__ __launch_bounds__(BLOCK) void k_bin_small(BuildState S)
     for (uint32_t k = begin + (blockIdx.x * BLOCK + threadIdx.x) / 64u; k < end; k += waves)
     {
         const uint32_t cnt = S.ncount[k];
-        if (cnt == 0u || cnt > SMALL) continue;
+        if (cnt <= FIN || cnt > SMALL) continue;
         const uint32_t slot = S.nbinslot[k];
-        if (slot == RETIRED || slot == TINY_SLOT) continue;
+        if (slot == RETIRED) continue;
         for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u)
         {
             const uint32_t v = j % BIN_VALS;
@@ -631,89 +608,6 @@ __device__ __forceinline__ void split_node(BuildState& S, uint32_t k, uint32_t f
     S.nflags[l + 1] = 0;
 }
 
-// The SAH sweep of a node of at most TINY slots, with each split's below and
-// above boxes and counts taken from the slots directly.  Min / max are exact, so
-// the boxes, and with the same operation order the costs, equal those the bin
-// histograms give.
-__device__ void sweep_tiny(BuildState& S, uint32_t k, uint32_t fl, uint32_t cnt)
-{
-    const uint32_t first = S.nfirst[k];
-    float nlo[3], nhi[3];
-    for (int a = 0; a < 3; ++a)
-    {
-        nlo[a] = float_of(S.nbox[6 * size_t(k) + a]);
-        nhi[a] = float_of(S.nbox[6 * size_t(k) + 3 + a]);
-    }
-    float min_cost = FLT_MAX;
-    uint32_t best_dim = 0, best_idx = 0, best_below = 0;
-    float best_mn = 0.0f, best_mx = 0.0f;
-    bool found = false;
-    for (uint32_t d = 0; d < 3; ++d)
-    {
-        const float mn = float_of(S.ncbox[6 * size_t(k) + d]);
-        const float mx = float_of(S.ncbox[6 * size_t(k) + 3 + d]);
-        if (mn == mx) continue;
-        float blo[NSPLITS][3], bhi[NSPLITS][3], alo[NSPLITS][3], ahi[NSPLITS][3];
-        uint32_t bn[NSPLITS], an[NSPLITS];
-#pragma unroll
-        for (uint32_t s = 0; s < NSPLITS; ++s)
-        {
-            bn[s] = 0; an[s] = 0;
-            for (int a = 0; a < 3; ++a)
-            {
-                blo[s][a] = FLT_MAX; bhi[s][a] = -FLT_MAX; alo[s][a] = FLT_MAX; ahi[s][a] = -FLT_MAX;
-            }
-        }
-        for (uint32_t i = first; i < first + cnt; ++i)
-        {
-            const float* b = S.pbox + 6 * size_t(S.idx[i]);
-            float lo[3], hi[3];
-            for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; }
-            const uint32_t bi = bin_index((lo[d] + hi[d]) * 0.5f, mn, mx);
-#pragma unroll
-            for (uint32_t s = 0; s < NSPLITS; ++s)
-            {
-                if (bi <= s)
-                {
-                    bn[s]++;
-                    for (int a = 0; a < 3; ++a) { blo[s][a] = min_s(blo[s][a], lo[a]); bhi[s][a] = max_s(bhi[s][a], hi[a]); }
-                }
-                else
-                {
-                    an[s]++;
-                    for (int a = 0; a < 3; ++a) { alo[s][a] = min_s(alo[s][a], lo[a]); ahi[s][a] = max_s(ahi[s][a], hi[a]); }
-                }
-            }
-        }
-#pragma unroll
-        for (uint32_t s = 0; s < NSPLITS; ++s)
-        {
-            float cost = 0.0f;
-            cost += float(bn[s]) * box_area(blo[s], bhi[s]);
-            cost += float(an[s]) * box_area(alo[s], ahi[s]);
-            if (cost < min_cost)
-            {
-                min_cost = cost;
-                best_dim = d;
-                best_idx = s;
-                best_below = bn[s];
-                best_mn = mn;
-                best_mx = mx;
-                found = true;
-            }
-        }
-    }
-    const float leaf_cost = float(cnt) * box_area(nlo, nhi);
-    // no plane below FLT_MAX (surface areas overflowed): a leaf, where the reference's
-    // default split would send every primitive to one side and never terminate
-    if (!found || min_cost >= leaf_cost)
-    {
-        sah_leaf(S, k, fl, cnt);
-        return;
-    }
-    split_node(S, k, fl, cnt, best_below, best_dim, best_idx, best_mn, best_mx);
-}
-
 // Sweep, split choice and child allocation of one node (SAH candidates; positional
 // splits for forced nodes).
 __device__ void sweep_node(BuildState& S, uint32_t k)
@@ -737,11 +631,6 @@ __device__ void sweep_node(BuildState& S, uint32_t k)
     }
     const uint32_t slot = S.nbinslot[k];
     if (slot == RETIRED) return;  // leaf
-    if (slot == TINY_SLOT)
-    {
-        sweep_tiny(S, k, fl, cnt);
-        return;
-    }
     const uint32_t* b = S.bins + size_t(slot) * NODE_BIN_WORDS;
     float nlo[3], nhi[3];
     for (int a = 0; a < 3; ++a)
@@ -792,7 +681,9 @@ __device__ void sweep_node(BuildState& S, uint32_t k)
         }
     }
     const float leaf_cost = float(cnt) * box_area(nlo, nhi);
-    if (!found || min_cost >= leaf_cost)  // as in sweep_tiny
+    // no plane below FLT_MAX (surface areas overflowed): a leaf, where the reference's
+    // default split would send every primitive to one side and never terminate
+    if (!found || min_cost >= leaf_cost)
     {
         sah_leaf(S, k, fl, cnt);
         return;
@@ -896,11 +787,558 @@ __global__ void k_init_roots(BuildState S, const uint32_t* rfirst, uint32_t R)
         S.lvl[1] = R;
         S.lvl[2] = 0u;
         S.lvl[3] = 0u;
+        S.lvl[4] = 0u;
+        S.lvl[5] = 0u;
         S.counters[0] = R;
         S.counters[1] = 0u;
+        S.counters[3] = 0u;
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// subtree finisher: every node of at most FIN slots, one workgroup each
+// ---------------------------------------------------------------------------
+// The workgroup copies its node's triangle boxes into LDS once and builds the whole subtree
+// there, breadth first, with the level loop's rules (bounds, leaf tests, forced positional
+// splits, 9-bin SAH sweeps with the same operation order, stable partitions):
+//   wave nodes  (more than TINY slots) one wave each per round: bounds by a wave reduction,
+//               histograms by LDS atomics, the 24 (axis, split) costs one lane each and an
+//               argmin over the lanes in (axis, split) order (the sweep's strict-< rule);
+//   tiny nodes  (at most TINY slots) three steps per sub-level: a thread per node for bounds
+//               and leaf tests, a thread per (node, axis) for the sweep from the slots
+//               directly, a thread per node for the choice across axes and the partition.
+// Child ids come from one block of 2 * count - 2 ids reserved per subtree (a binary tree over
+// count slots has at most that many nodes below its root); ids left unused are counted in
+// counters[3].  The subtree's slot order is written back to idx at the end.
+constexpr uint32_t FIN_BLOCK = 256;
+constexpr uint32_t FIN_WAVES = FIN_BLOCK / 64u;
+constexpr uint32_t FIN_QCAP = FIN / (TINY + 1) + 1;  // wave nodes in a round (disjoint, > TINY slots)
+constexpr uint32_t FIN_CCAP = FIN / 2;               // tiny SAH candidates in a step (>= 2 slots each)
+
+// A node waiting in LDS: global id, and its first local slot | count << 16 | forced << 31.
+struct FinNode
+{
+    uint32_t node, pack;
+};
+
+__device__ __forceinline__ FinNode fin_node(uint32_t node, uint32_t first, uint32_t count, bool forced)
+{
+    return FinNode{ node, first | (count << 16) | (forced ? 0x80000000u : 0u) };
+}
+
+struct FinShared
+{
+    float box[FIN][6];                      // triangle boxes, by local triangle
+    uint32_t tri[FIN];                      // local triangle -> triangle
+    uint16_t lid[FIN];                      // slot -> local triangle
+    uint16_t tmp[FIN];                      // partition staging
+    FinNode queue[2][FIN_QCAP];             // wave nodes: this round's and the next round's
+    FinNode tiny[2][FIN];                   // tiny nodes: this sub-level's and the next one's
+    FinNode cand[FIN_CCAP];                 // this sub-level's tiny SAH candidates
+    float ccost[FIN_CCAP][3];               // their best cost per axis (+inf: none)
+    uint16_t cinfo[FIN_CCAP][3];            // split | below count << 8 of that cost
+    uint32_t hist[FIN_WAVES][NODE_BIN_WORDS];
+    uint32_t qn[2], tn[2], cn, next, base, item;
+};
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ uint32_t fin_count(FinNode e) { return (e.pack >> 16) & 0x7FFFu; }
+__device__ __forceinline__ uint32_t fin_first(FinNode e) { return e.pack & 0xFFFFu; }
+__device__ __forceinline__ bool fin_forced(FinNode e) { return (e.pack >> 31) != 0u; }
+
+__device__ __forceinline__ void fin_push(FinShared& L, uint32_t qnext, uint32_t tnext, FinNode e)
+{
+    if (fin_count(e) > TINY) L.queue[qnext][atomicAdd(&L.qn[qnext], 1u)] = e;
+    else L.tiny[tnext][atomicAdd(&L.tn[tnext], 1u)] = e;
+}
+
+// Positional split of a forced node: the first half rounded to whole leaves.
+__device__ __forceinline__ uint32_t forced_half(uint32_t cnt)
+{
+    uint32_t half = ((cnt + 1) / 2 + APG_MAX_LEAF - 1) / APG_MAX_LEAF * APG_MAX_LEAF;
+    return half < cnt - 1 ? half : cnt - 1;
+}
+
+// How a node is split: mode 0 none (a leaf), 1 positional, 2 SAH on axis d at bin si with nb
+// slots below and the node's centroid range [mn, mx] on that axis.
+struct FinSplit
+{
+    uint32_t mode, d, si, nb;
+    float mn, mx;
+};
+
+// Leaf test of the level loop (k_decide): forced nodes split positionally until they hold at
+// most APG_MAX_LEAF triangles; others are SAH leaves at most `thr` triangles or with a zero
+// centroid extent.  Returns true when the node is decided without a sweep.
+__device__ __forceinline__ bool fin_decide(uint32_t cnt, bool forced, uint32_t thr, const float cmn[3],
+                                           const float cmx[3], uint32_t& fl, FinSplit& sp)
+{
+    sp.mode = 0;
+    if (forced)
+    {
+        fl = NF_FORCED | (cnt <= APG_MAX_LEAF ? NF_LEAF : NF_SPLIT);
+        if (cnt > APG_MAX_LEAF) { sp.mode = 1; sp.nb = forced_half(cnt); }
+        return true;
+    }
+    bool leaf = cnt <= thr;
+    if (!leaf)
+    {
+        const float diag = max_s(cmx[0] - cmn[0], max_s(cmx[1] - cmn[1], cmx[2] - cmn[2]));
+        if (diag == 0.0f) leaf = true;
+    }
+    if (!leaf) return false;
+    fl = NF_SAH_LEAF | (cnt <= APG_MAX_LEAF ? NF_LEAF : (NF_FORCED | NF_SPLIT));
+    if (cnt > APG_MAX_LEAF) { sp.mode = 1; sp.nb = forced_half(cnt); }
+    return true;
+}
+
+// The choice after a sweep (sah_leaf / split_node): a leaf unless a split costs less than the
+// leaf.
+__device__ __forceinline__ void fin_choose(uint32_t cnt, bool found, float min_cost, float leaf_cost, uint32_t d,
+                                           uint32_t si, uint32_t nb, const float cmn[3], const float cmx[3],
+                                           uint32_t& fl, FinSplit& sp)
+{
+    // no plane below FLT_MAX (surface areas overflowed): a leaf, where the reference's
+    // default split would send every primitive to one side and never terminate
+    if (!found || min_cost >= leaf_cost)
+    {
+        fl = NF_SAH_LEAF | (cnt <= APG_MAX_LEAF ? NF_LEAF : (NF_FORCED | NF_SPLIT));
+        sp.mode = 0;
+        if (cnt > APG_MAX_LEAF) { sp.mode = 1; sp.nb = forced_half(cnt); }
+        return;
+    }
+    fl = NF_SPLIT;
+    sp.mode = 2;
+    sp.d = d;
+    sp.si = si;
+    sp.nb = nb;
+    sp.mn = d == 0 ? cmn[0] : (d == 1 ? cmn[1] : cmn[2]);
+    sp.mx = d == 0 ? cmx[0] : (d == 1 ? cmx[1] : cmx[2]);
+}
+
+// Children of a split node: ids from the subtree's block, counts, first slots (global).
+__device__ __forceinline__ uint32_t fin_children(BuildState& S, FinShared& L, uint32_t k, uint32_t gfirst,
+                                                 uint32_t f, uint32_t cnt, uint32_t nb)
+{
+    const uint32_t l = L.base + atomicAdd(&L.next, 2u);
+    S.nleft[k] = l;
+    S.ncount[l] = nb;
+    S.ncount[l + 1] = cnt - nb;
+    S.nfirst[l] = gfirst + f;
+    S.nfirst[l + 1] = gfirst + f + nb;
+    return l;
+}
+
+// Bounds and centroid bounds (ordered uints) of slots [f, f + cnt) by one thread.
+__device__ __forceinline__ void fin_bounds_serial(const FinShared& L, uint32_t f, uint32_t cnt, uint32_t r[12])
+{
+    const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
+    for (int a = 0; a < 3; ++a)
+    {
+        r[a] = lo_ord; r[3 + a] = hi_ord; r[6 + a] = lo_ord; r[9 + a] = hi_ord;
+    }
+    for (uint32_t j = 0; j < cnt; ++j)
+    {
+        const float* b = L.box[L.lid[f + j]];
+        for (int a = 0; a < 3; ++a)
+        {
+            const float lo = b[a], hi = b[3 + a], c = (lo + hi) * 0.5f;
+            r[a] = min(r[a], ord_of(lo));
+            r[3 + a] = max(r[3 + a], ord_of(hi));
+            r[6 + a] = min(r[6 + a], ord_of(c));
+            r[9 + a] = max(r[9 + a], ord_of(c));
+        }
+    }
+}
+
+__device__ __forceinline__ void fin_unpack_bounds(const uint32_t r[12], float nlo[3], float nhi[3], float cmn[3],
+                                                  float cmx[3])
+{
+    for (int a = 0; a < 3; ++a)
+    {
+        nlo[a] = float_of(r[a]); nhi[a] = float_of(r[3 + a]);
+        cmn[a] = float_of(r[6 + a]); cmx[a] = float_of(r[9 + a]);
+    }
+}
+
+// One node of more than TINY slots, by one wave.
+__device__ void fin_wave_node(BuildState& S, FinShared& L, uint32_t thr, uint32_t gfirst, FinNode e,
+                              uint32_t qnext, uint32_t* hist)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t k = e.node, f = fin_first(e), cnt = fin_count(e);
+    // bounds and centroid bounds, reduced across the wave (every lane ends with all 12)
+    const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
+    uint32_t r[12];
+    for (int a = 0; a < 3; ++a)
+    {
+        r[a] = lo_ord; r[3 + a] = hi_ord; r[6 + a] = lo_ord; r[9 + a] = hi_ord;
+    }
+    for (uint32_t j = lane; j < cnt; j += 64u)
+    {
+        const float* b = L.box[L.lid[f + j]];
+        for (int a = 0; a < 3; ++a)
+        {
+            const float lo = b[a], hi = b[3 + a], c = (lo + hi) * 0.5f;
+            r[a] = min(r[a], ord_of(lo));
+            r[3 + a] = max(r[3 + a], ord_of(hi));
+            r[6 + a] = min(r[6 + a], ord_of(c));
+            r[9 + a] = max(r[9 + a], ord_of(c));
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int a = 0; a < 3; ++a)
+        {
+            r[a] = min(r[a], uint32_t(__shfl_xor(int(r[a]), off)));
+            r[3 + a] = max(r[3 + a], uint32_t(__shfl_xor(int(r[3 + a]), off)));
+            r[6 + a] = min(r[6 + a], uint32_t(__shfl_xor(int(r[6 + a]), off)));
+            r[9 + a] = max(r[9 + a], uint32_t(__shfl_xor(int(r[9 + a]), off)));
+        }
+    if (lane == 0u)
+        for (int j = 0; j < 6; ++j) S.nbox[6 * size_t(k) + j] = r[j];
+    float nlo[3], nhi[3], cmn[3], cmx[3];
+    fin_unpack_bounds(r, nlo, nhi, cmn, cmx);
+    uint32_t fl = 0;
+    FinSplit sp;
+    if (!fin_decide(cnt, fin_forced(e), thr, cmn, cmx, fl, sp))
+    {
+        // 9-bin histograms of the three axes in the wave's own LDS block
+        for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u)
+        {
+            const uint32_t v = j % BIN_VALS;
+            hist[j] = v < 3 ? ord_of(FLT_MAX) : (v < 6 ? ord_of(-FLT_MAX) : 0u);
+        }
+        wave_sync_lds();
+        for (uint32_t j = lane; j < cnt; j += 64u)
+        {
+            const float* b = L.box[L.lid[f + j]];
+            float lo[3], hi[3];
+            for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; }
+            for (int d = 0; d < 3; ++d)
+            {
+                if (cmn[d] == cmx[d]) continue;
+                const float c = (lo[d] + hi[d]) * 0.5f;
+                uint32_t* h = hist + (d * NBINS + bin_index(c, cmn[d], cmx[d])) * BIN_VALS;
+                for (int a = 0; a < 3; ++a)
+                {
+                    atomicMin(&h[a], ord_of(lo[a]));
+                    atomicMax(&h[3 + a], ord_of(hi[a]));
+                }
+                atomicAdd(&h[6], 1u);
+            }
+        }
+        wave_sync_lds();
+        // the sweep (sweep_node): lane d * 8 + s prices split s of axis d, the below box
+        // accumulated over bins 0..s and the above box over bins 8..s+1 in sweep_node's order
+        float cost = __builtin_inff();
+        uint32_t key = 63u, below = 0u;
+        if (lane < 3u * NSPLITS)
+        {
+            const uint32_t d = lane / NSPLITS, s = lane % NSPLITS;
+            const float mn = d == 0 ? cmn[0] : (d == 1 ? cmn[1] : cmn[2]);
+            const float mx = d == 0 ? cmx[0] : (d == 1 ? cmx[1] : cmx[2]);
+            if (mn != mx)
+            {
+                float blo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, bhi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+                float alo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, ahi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+                uint32_t bn = 0, an = 0;
+                for (uint32_t b = 0; b <= s; ++b)
+                {
+                    const uint32_t* h = hist + (d * NBINS + b) * BIN_VALS;
+                    for (int a = 0; a < 3; ++a) { blo[a] = min_s(blo[a], float_of(h[a])); bhi[a] = max_s(bhi[a], float_of(h[3 + a])); }
+                    bn += h[6];
+                }
+                for (uint32_t b = NBINS - 1; b > s; --b)
+                {
+                    const uint32_t* h = hist + (d * NBINS + b) * BIN_VALS;
+                    for (int a = 0; a < 3; ++a) { alo[a] = min_s(alo[a], float_of(h[a])); ahi[a] = max_s(ahi[a], float_of(h[3 + a])); }
+                    an += h[6];
+                }
+                float c = 0.0f;
+                c += float(bn) * box_area(blo, bhi);
+                c += float(an) * box_area(alo, ahi);
+                if (c < FLT_MAX)
+                {
+                    cost = c;
+                    key = lane;
+                    below = bn;
+                }
+            }
+        }
+        // the first (axis, split) of least cost: the sequential sweep's strict-< choice
+        for (int off = 1; off < 32; off <<= 1)
+        {
+            const float oc = __shfl_xor(cost, off);
+            const uint32_t ok = uint32_t(__shfl_xor(int(key), off));
+            if (oc < cost || (oc == cost && ok < key)) { cost = oc; key = ok; }
+        }
+        const float min_cost = __shfl(cost, 0);
+        const uint32_t best = uint32_t(__shfl(int(key), 0));
+        const uint32_t nbelow = uint32_t(__shfl(int(below), int(best & 63u)));
+        const float leaf_cost = float(cnt) * box_area(nlo, nhi);
+        fin_choose(cnt, best != 63u, min_cost, leaf_cost, best / NSPLITS, best % NSPLITS, nbelow, cmn, cmx, fl, sp);
+        // the next node's clear of hist must follow every lane's reads above
+        wave_sync_lds();
+    }
+    if (lane == 0u) S.nflags[k] = fl;
+    if (sp.mode == 0) return;
+    const uint32_t nb = sp.nb;
+    if (sp.mode == 2)
+    {
+        // stable partition of the node's slots: below the split plane first
+        uint32_t run = 0;
+        for (uint32_t j0 = 0; j0 < cnt; j0 += 64u)
+        {
+            const uint32_t j = j0 + lane;
+            bool bl = false;
+            uint16_t v = 0;
+            if (j < cnt)
+            {
+                v = L.lid[f + j];
+                const float* b = L.box[v];
+                bl = bin_index((b[sp.d] + b[3 + sp.d]) * 0.5f, sp.mn, sp.mx) <= sp.si;
+            }
+            const uint64_t m = __ballot(bl);
+            const uint32_t before = run + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+            if (j < cnt) L.tmp[f + (bl ? before : nb + (j - before))] = v;
+            run += uint32_t(__popcll(m));
+        }
+        wave_sync_lds();
+        for (uint32_t j = lane; j < cnt; j += 64u) L.lid[f + j] = L.tmp[f + j];
+        wave_sync_lds();
+    }
+    if (lane == 0u)
+    {
+        const uint32_t l = fin_children(S, L, k, gfirst, f, cnt, nb);
+        fin_push(L, qnext, 0, fin_node(l, f, nb, sp.mode == 1));
+        fin_push(L, qnext, 0, fin_node(l + 1, f + nb, cnt - nb, sp.mode == 1));
+    }
+}
+
+// Sub-level step 2: the best split of one axis of a tiny candidate, from the slots directly
+// (each split's below and above boxes and counts; the sweep's strict-< rule within the axis).
+__device__ void fin_tiny_axis(FinShared& L, uint32_t c, uint32_t d)
+{
+    const FinNode e = L.cand[c];
+    const uint32_t f = fin_first(e), cnt = fin_count(e);
+    uint32_t omn = ord_of(FLT_MAX), omx = ord_of(-FLT_MAX);
+    for (uint32_t j = 0; j < cnt; ++j)
+    {
+        const float* b = L.box[L.lid[f + j]];
+        const uint32_t o = ord_of((b[d] + b[3 + d]) * 0.5f);
+        omn = min(omn, o);
+        omx = max(omx, o);
+    }
+    const float mn = float_of(omn), mx = float_of(omx);
+    float min_cost = FLT_MAX;
+    uint32_t best_idx = 0, best_below = 0;
+    bool found = false;
+    if (mn != mx)
+    {
+        float blo[NSPLITS][3], bhi[NSPLITS][3], alo[NSPLITS][3], ahi[NSPLITS][3];
+        uint32_t bn[NSPLITS], an[NSPLITS];
+#pragma unroll
+        for (uint32_t s = 0; s < NSPLITS; ++s)
+        {
+            bn[s] = 0; an[s] = 0;
+            for (int a = 0; a < 3; ++a)
+            {
+                blo[s][a] = FLT_MAX; bhi[s][a] = -FLT_MAX; alo[s][a] = FLT_MAX; ahi[s][a] = -FLT_MAX;
+            }
+        }
+        for (uint32_t j = 0; j < cnt; ++j)
+        {
+            const float* b = L.box[L.lid[f + j]];
+            float lo[3], hi[3];
+            for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; }
+            const uint32_t bi = bin_index((lo[d] + hi[d]) * 0.5f, mn, mx);
+#pragma unroll
+            for (uint32_t s = 0; s < NSPLITS; ++s)
+            {
+                if (bi <= s)
+                {
+                    bn[s]++;
+                    for (int a = 0; a < 3; ++a) { blo[s][a] = min_s(blo[s][a], lo[a]); bhi[s][a] = max_s(bhi[s][a], hi[a]); }
+                }
+                else
+                {
+                    an[s]++;
+                    for (int a = 0; a < 3; ++a) { alo[s][a] = min_s(alo[s][a], lo[a]); ahi[s][a] = max_s(ahi[s][a], hi[a]); }
+                }
+            }
+        }
+#pragma unroll
+        for (uint32_t s = 0; s < NSPLITS; ++s)
+        {
+            float cost = 0.0f;
+            cost += float(bn[s]) * box_area(blo[s], bhi[s]);
+            cost += float(an[s]) * box_area(alo[s], ahi[s]);
+            if (cost < min_cost)
+            {
+                min_cost = cost;
+                best_idx = s;
+                best_below = bn[s];
+                found = true;
+            }
+        }
+    }
+    L.ccost[c][d] = found ? min_cost : __builtin_inff();
+    L.cinfo[c][d] = uint16_t(best_idx | (best_below << 8));
+}
+
+// Sub-level step 3: the choice across axes (in axis order, strict <), the leaf test and the
+// stable partition of one tiny candidate, by one thread.
+__device__ void fin_tiny_choose(BuildState& S, FinShared& L, uint32_t gfirst, uint32_t c, uint32_t tnext)
+{
+    const FinNode e = L.cand[c];
+    const uint32_t k = e.node, f = fin_first(e), cnt = fin_count(e);
+    uint32_t r[12];
+    fin_bounds_serial(L, f, cnt, r);
+    float nlo[3], nhi[3], cmn[3], cmx[3];
+    fin_unpack_bounds(r, nlo, nhi, cmn, cmx);
+    float min_cost = FLT_MAX;
+    uint32_t best_dim = 0;
+    bool found = false;
+    for (uint32_t d = 0; d < 3; ++d)
+        if (L.ccost[c][d] < min_cost)
+        {
+            min_cost = L.ccost[c][d];
+            best_dim = d;
+            found = true;
+        }
+    const uint32_t info = L.cinfo[c][best_dim];
+    uint32_t fl = 0;
+    FinSplit sp;
+    fin_choose(cnt, found, min_cost, float(cnt) * box_area(nlo, nhi), best_dim, info & 0xFFu, info >> 8, cmn, cmx,
+               fl, sp);
+    S.nflags[k] = fl;
+    if (sp.mode == 0) return;
+    const uint32_t nb = sp.nb;
+    if (sp.mode == 2)
+    {
+        uint32_t nl = 0, nr = 0;
+        for (uint32_t j = 0; j < cnt; ++j)
+        {
+            const uint16_t v = L.lid[f + j];
+            const float* b = L.box[v];
+            if (bin_index((b[sp.d] + b[3 + sp.d]) * 0.5f, sp.mn, sp.mx) <= sp.si) L.tmp[f + nl++] = v;
+            else L.tmp[f + nb + nr++] = v;
+        }
+        for (uint32_t j = 0; j < cnt; ++j) L.lid[f + j] = L.tmp[f + j];
+    }
+    const uint32_t l = fin_children(S, L, k, gfirst, f, cnt, nb);
+    fin_push(L, 0, tnext, fin_node(l, f, nb, sp.mode == 1));
+    fin_push(L, 0, tnext, fin_node(l + 1, f + nb, cnt - nb, sp.mode == 1));
+}
+
+// Grid-stride over the nodes the level loop handed over, taken in turn from a device cursor.
+__global__ __launch_bounds__(FIN_BLOCK) void k_finish(BuildState S, uint32_t thr)
+{
+    __shared__ FinShared L;
+    const uint32_t tid = threadIdx.x, wave = tid / 64u;
+    const uint32_t total = S.lvl[4];
+    for (;;)
+    {
+        if (tid == 0) L.item = atomicAdd(&S.lvl[5], 1u);
+        __syncthreads();
+        const uint32_t item = L.item;
+        if (item >= total) break;
+        const uint32_t root = S.fin[item];
+        const uint32_t gfirst = S.nfirst[root], cnt = S.ncount[root];
+        const bool forced = (S.nflags[root] & NF_FORCED) != 0u;
+        for (uint32_t j = tid; j < cnt; j += FIN_BLOCK)
+        {
+            const uint32_t t = S.idx[gfirst + j];
+            L.tri[j] = t;
+            L.lid[j] = uint16_t(j);
+            const float* b = S.pbox + 6 * size_t(t);
+            for (int a = 0; a < 6; ++a) L.box[j][a] = b[a];
+        }
+        if (tid == 0)
+        {
+            L.base = cnt > 1 ? atomicAdd(&S.counters[0], 2 * cnt - 2) : 0u;
+            L.next = 0;
+            L.qn[0] = L.qn[1] = 0;
+            L.tn[0] = L.tn[1] = 0;
+            L.cn = 0;
+            fin_push(L, 0, 0, fin_node(root, 0, cnt, forced));
+        }
+        __syncthreads();
+        // wave nodes: rounds of one wave per node (their tiny children go to tiny[0])
+        uint32_t cur = 0;
+        for (;;)
+        {
+            const uint32_t qn = L.qn[cur];
+            if (qn == 0) break;
+            for (uint32_t q = wave; q < qn; q += FIN_WAVES)
+                fin_wave_node(S, L, thr, gfirst, L.queue[cur][q], cur ^ 1u, L.hist[wave]);
+            __syncthreads();
+            if (tid == 0) L.qn[cur] = 0;
+            cur ^= 1u;
+            __syncthreads();
+        }
+        // tiny nodes: sub-levels of three steps
+        uint32_t tc = 0;
+        for (;;)
+        {
+            const uint32_t tn = L.tn[tc];
+            if (tn == 0) break;
+            for (uint32_t t = tid; t < tn; t += FIN_BLOCK)
+            {
+                const FinNode e = L.tiny[tc][t];
+                const uint32_t k = e.node, f = fin_first(e), c = fin_count(e);
+                uint32_t r[12];
+                fin_bounds_serial(L, f, c, r);
+                for (int j = 0; j < 6; ++j) S.nbox[6 * size_t(k) + j] = r[j];
+                float nlo[3], nhi[3], cmn[3], cmx[3];
+                fin_unpack_bounds(r, nlo, nhi, cmn, cmx);
+                uint32_t fl = 0;
+                FinSplit sp;
+                if (!fin_decide(c, fin_forced(e), thr, cmn, cmx, fl, sp))
+                {
+                    L.cand[atomicAdd(&L.cn, 1u)] = e;
+                    continue;
+                }
+                S.nflags[k] = fl;
+                if (sp.mode == 1)
+                {
+                    const uint32_t l = fin_children(S, L, k, gfirst, f, c, sp.nb);
+                    fin_push(L, 0, tc ^ 1u, fin_node(l, f, sp.nb, true));
+                    fin_push(L, 0, tc ^ 1u, fin_node(l + 1, f + sp.nb, c - sp.nb, true));
+                }
+            }
+            __syncthreads();
+            const uint32_t cn = L.cn;
+            for (uint32_t u = tid; u < 3 * cn; u += FIN_BLOCK) fin_tiny_axis(L, u / 3, u % 3);
+            __syncthreads();
+            for (uint32_t c = tid; c < cn; c += FIN_BLOCK) fin_tiny_choose(S, L, gfirst, c, tc ^ 1u);
+            __syncthreads();
+            if (tid == 0)
+            {
+                L.tn[tc] = 0;
+                L.cn = 0;
+            }
+            tc ^= 1u;
+            __syncthreads();
+        }
+        for (uint32_t j = tid; j < cnt; j += FIN_BLOCK) S.idx[gfirst + j] = L.tri[L.lid[j]];
+        if (tid == 0 && cnt > 1) atomicAdd(&S.counters[3], 2 * cnt - 2 - L.next);
+        __syncthreads();
+    }
+}
+
+// After k_finish: the node ids allocated and the binary nodes built.
+__global__ void k_finish_done(BuildState S)
+{
+    S.lvl[6] = S.counters[0];
+    S.lvl[7] = S.counters[0] - S.counters[3];
+}
 
 // ---------------------------------------------------------------------------
 // exclusive scan (3 passes, recursive on the block sums)
@@ -1340,9 +1778,9 @@ struct BuildCache
     int ensure_sized(Context* ctx, uint32_t n, uint32_t R)
     {
         const size_t ncap = 2 * size_t(n) + 2 + R;
-        // histogram slots: only nodes of more than TINY slots take one, and a level's nodes
-        // own disjoint slots
-        const size_t bincap = size_t(n) / (TINY + 1) + 2;
+        // histogram slots: only the level loop's nodes (more than FIN slots) take one, and a
+        // level's nodes own disjoint slots
+        const size_t bincap = size_t(n) / (FIN + 1) + 2;
         const size_t lcap = size_t(n) / SMALL + 2;  // nodes of more than SMALL slots per level
         const size_t wcap = size_t(n) + R + 2;      // >= binary nodes / 2 + 2 + R
         int rc;
@@ -1355,6 +1793,7 @@ struct BuildCache
             (rc = get(ctx, S.nbinslot, ncap)) || (rc = get(ctx, S.bins, bincap * NODE_BIN_WORDS)) ||
             (rc = get(ctx, S.counters, 16)) || (rc = get(ctx, S.scan_tmp, scan_tmp_words(n + 1))) ||
             (rc = get(ctx, S.lvl, 16)) || (rc = get(ctx, S.large, lcap)) || (rc = get(ctx, S.large_off, lcap + 1)) ||
+            (rc = get(ctx, S.fin, size_t(n) + R)) ||
             (rc = get(ctx, rfirst, size_t(R) + 1)) || (rc = get(ctx, slack_bits, 1)) || (rc = get(ctx, slack, 1)) ||
             (rc = get(ctx, wbin, 2 * wcap + 8)) || (rc = get(ctx, wchild, 8 * wcap)) || (rc = get(ctx, n_inner, wcap)) ||
             (rc = get(ctx, n_tris, wcap)) || (rc = get(ctx, s_inner, wcap + 1)) || (rc = get(ctx, s_tris, wcap + 1)) ||
@@ -1452,16 +1891,16 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
 
     // ---- binary levels: enqueued in batches, the level range read back between batches ----
     const uint32_t g_node = std::min<uint32_t>(blocks_for(n, 256), 2048u);
-    const uint32_t g_wave = std::min<uint32_t>(blocks_for(n / (TINY + 1) + 1, BLOCK / 64u), 1024u);
+    const uint32_t g_wave = std::min<uint32_t>(blocks_for(n / (FIN + 1) + 1, BLOCK / 64u), 1024u);
     const uint32_t g_chunk = 2 * blocks_for(n, CHUNK) + 1;
     const uint32_t g_slot = blocks_for(n, 256);
+    const uint32_t g_fin = std::min<uint32_t>(blocks_for(n, 256), 3u * 256u);
     uint32_t* rb = C.rb;
     auto enqueue_level = [&]() {
         k_level_init<<<g_node, 256, 0, s>>>(S);
         k_large_scan<<<1, SCAN1, 0, s>>>(S);
         k_bounds_large<<<g_chunk, BLOCK, 0, s>>>(S);
         k_bounds_small<<<g_wave, BLOCK, 0, s>>>(S);
-        k_bounds_tiny<<<g_node, 256, 0, s>>>(S);
         k_decide<<<g_node, 256, 0, s>>>(S, thr);
         k_bin_large<<<g_chunk, BLOCK, 0, s>>>(S);
         k_bin_small<<<g_wave, BLOCK, 0, s>>>(S);
@@ -1474,11 +1913,12 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
         std::swap(S.idx, S.idx_tmp);
         std::swap(S.snode, S.snode_tmp);
     };
-    // most trees are a little deeper than log2(n): the first batch rarely runs empty levels
+    // the level loop ends where every node holds at most FIN slots, a little deeper than
+    // log2(n / FIN) levels: the first batch rarely runs empty levels
     uint32_t enqueued = 0;
     for (;;)
     {
-        const uint32_t batch = enqueued == 0 ? ceil_log2(n) + 2u : 4u;
+        const uint32_t batch = enqueued == 0 ? ceil_log2(n / FIN + 1) + 3u : 4u;
         for (uint32_t b = 0; b < batch; ++b) enqueue_level();
         enqueued += batch;
         APG_HIP_CHECK(ctx, hipMemcpyAsync(rb, S.lvl, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1487,7 +1927,13 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
         if (rb[0] == rb[1]) break;
         if (enqueued > 4096) return ctx->fail(APG_E_CAPACITY, "binary tree deeper than 4096 levels");
     }
-    const uint32_t bin_nodes = rb[1];
+    // the subtrees of at most FIN slots
+    k_finish<<<g_fin, FIN_BLOCK, 0, s>>>(S, thr);
+    k_finish_done<<<1, 1, 0, s>>>(S);
+    APG_HIP_CHECK(ctx, hipMemcpyAsync(rb, S.lvl, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    APG_HIP_CHECK(ctx, hipGetLastError());
+    const uint32_t bin_ids = rb[6], bin_nodes = rb[7];
 
     // ---- wide collapse ----
     k_fill_u32<<<1, 64, 0, s>>>(C.slack_bits, 1, 0u);
@@ -1554,13 +2000,13 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
     if (!stats) return APG_OK;
 
     // ---- binary-tree statistics and leaf keys (host, from the device tree) ----
-    std::vector<uint32_t> hbox(6 * size_t(bin_nodes)), hfirst(bin_nodes), hcount(bin_nodes),
-        hleft(bin_nodes), hflags(bin_nodes), hidx(n);
+    std::vector<uint32_t> hbox(6 * size_t(bin_ids)), hfirst(bin_ids), hcount(bin_ids),
+        hleft(bin_ids), hflags(bin_ids), hidx(n);
     APG_HIP_CHECK(ctx, hipMemcpy(hbox.data(), S.nbox, hbox.size() * 4, hipMemcpyDeviceToHost));
-    APG_HIP_CHECK(ctx, hipMemcpy(hfirst.data(), S.nfirst, bin_nodes * 4, hipMemcpyDeviceToHost));
-    APG_HIP_CHECK(ctx, hipMemcpy(hcount.data(), S.ncount, bin_nodes * 4, hipMemcpyDeviceToHost));
-    APG_HIP_CHECK(ctx, hipMemcpy(hleft.data(), S.nleft, bin_nodes * 4, hipMemcpyDeviceToHost));
-    APG_HIP_CHECK(ctx, hipMemcpy(hflags.data(), S.nflags, bin_nodes * 4, hipMemcpyDeviceToHost));
+    APG_HIP_CHECK(ctx, hipMemcpy(hfirst.data(), S.nfirst, bin_ids * 4, hipMemcpyDeviceToHost));
+    APG_HIP_CHECK(ctx, hipMemcpy(hcount.data(), S.ncount, bin_ids * 4, hipMemcpyDeviceToHost));
+    APG_HIP_CHECK(ctx, hipMemcpy(hleft.data(), S.nleft, bin_ids * 4, hipMemcpyDeviceToHost));
+    APG_HIP_CHECK(ctx, hipMemcpy(hflags.data(), S.nflags, bin_ids * 4, hipMemcpyDeviceToHost));
     APG_HIP_CHECK(ctx, hipMemcpy(hidx.data(), S.idx, size_t(n) * 4, hipMemcpyDeviceToHost));
     auto area = [&](uint32_t k) {
         float lo[3], hi[3];
