@@ -99,11 +99,8 @@ struct BuildState
     float* tri9_src;    // 9 per triangle: v0, e0, e1
     // slots
     uint32_t* idx;      // slot -> triangle
-    uint32_t* idx_tmp;
-    uint32_t* snode;    // slot -> node (RETIRED once in a leaf)
-    uint32_t* snode_tmp;
-    uint32_t* flag;     // partition flags / scan input
-    uint32_t* scan;     // exclusive scan of flag (n + 1)
+    uint32_t* idx_tmp;  // partition staging
+    uint8_t* flag8;     // per slot of the level's nodes: goes to the left child
     // nodes (capacity 2n)
     uint32_t* nbox;     // 6 ordered uints per node (bounds accumulators)
     uint32_t* ncbox;    // 6 ordered uints per node (centroid bounds)
@@ -115,14 +112,16 @@ struct BuildState
     float* nsplit_mm;   // mn, mx
     uint32_t* nbinslot; // histogram slot or RETIRED
     uint32_t* bins;     // NODE_BIN_WORDS per candidate
-    uint32_t* counters; // [0] node count, [1] bin slots, [2] tmp
-    uint32_t* scan_tmp;
+    uint32_t* counters; // [0] node count, [1] bin slots, [3] k_finish's unused node ids
     // the level being built: [0] first node, [1] end, [2] levels done, [3] listed large nodes,
     // [4] nodes left to k_finish, [5] k_finish's cursor over them, [6] node ids allocated,
-    // [7] binary nodes built (allocated less k_finish's unused reservations)
+    // [7] binary nodes built (allocated less k_finish's unused reservations), [9] levels that
+    // held nodes, [10 + level % 2] nodes k_level_init listed
     uint32_t* lvl;
-    uint32_t* large;      // the level's nodes of more than SMALL slots
+    uint32_t* large;      // the level's nodes (all of more than FIN slots)
     uint32_t* large_off;  // exclusive scan of their chunk counts (total at [count])
+    uint32_t* chunk_below;  // per chunk of the level: slots going to the left child
+    uint32_t* chunk_off;    // their exclusive scan over the level's chunks
     uint32_t* fin;        // nodes of at most FIN slots, left to k_finish (count at lvl[4])
 };
 
@@ -184,21 +183,11 @@ __global__ void k_gather_vertices(const uint32_t* tri_ids, uint32_t n, const uin
     }
 }
 
-// Slot i holds triangle i, in the root of its shape's tree (shape_first:
-// first triangle of each shape, R + 1 entries) or in the single root (R = 1).
-__global__ void k_init_slots(uint32_t n, uint32_t* idx, uint32_t* snode, const uint32_t* shape_first, uint32_t R)
+// Slot i holds triangle i (a shape's triangles are the slot range of its tree's root).
+__global__ void k_init_slots(uint32_t n, uint32_t* idx)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    idx[i] = i;
-    uint32_t lo = 0, hi = R;  // largest r with shape_first[r] <= i
-    while (hi - lo > 1)
-    {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (shape_first[mid] <= i) lo = mid;
-        else hi = mid;
-    }
-    snode[i] = lo;
+    if (i < n) idx[i] = i;
 }
 
 
@@ -207,13 +196,17 @@ __global__ void k_init_slots(uint32_t n, uint32_t* idx, uint32_t* snode, const u
 // ---------------------------------------------------------------------------
 // Every kernel of a level reads the level's node range [lvl[0], lvl[1]) from device memory
 // and walks it with a grid-stride loop over a grid fixed for the build, so the host enqueues
-// level after level without knowing their widths; k_next_level moves on to the children.
+// level after level without knowing their widths; the last workgroup of k_part_copy moves on
+// to the children.
 
-// Per node of the level: clear the bounds accumulators and list the nodes of more than SMALL
-// slots (their order in the list is arbitrary: what they accumulate is order independent).
+// Per node of the level: hand the nodes of at most FIN slots to k_finish; clear the others'
+// bounds accumulators and list them (their order in the list is arbitrary: what they
+// accumulate is order independent, and each node's chunks stay contiguous in chunk order).
 __global__ __launch_bounds__(256) void k_level_init(BuildState S)
 {
-    const uint32_t begin = S.lvl[0], end = S.lvl[1];
+    // the children the previous level allocated; the list count alternates between two words
+    const uint32_t begin = S.lvl[1], end = S.counters[0];
+    uint32_t* listed = &S.lvl[10 + (S.lvl[2] & 1u)];
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x; k < end; k += stride)
     {
@@ -231,46 +224,87 @@ __global__ __launch_bounds__(256) void k_level_init(BuildState S)
             S.ncbox[6 * size_t(k) + a] = ord_of(FLT_MAX);
             S.ncbox[6 * size_t(k) + 3 + a] = ord_of(-FLT_MAX);
         }
-        S.nbinslot[k] = RETIRED;
-        if (S.ncount[k] > SMALL) S.large[atomicAdd(&S.lvl[3], 1u)] = k;
+        const uint32_t j = atomicAdd(listed, 1u);
+        S.large[j] = k;
+        // the node's histogram (its list slot); forced nodes are split positionally, unbinned
+        if (S.nflags[k] & NF_FORCED)
+        {
+            S.nbinslot[k] = RETIRED;
+            continue;
+        }
+        S.nbinslot[k] = j;
+        uint32_t* hb = S.bins + size_t(j) * NODE_BIN_WORDS;
+        for (uint32_t e = 0; e < 3 * NBINS; ++e)
+        {
+            for (int a = 0; a < 3; ++a) { hb[e * BIN_VALS + a] = ord_of(FLT_MAX); hb[e * BIN_VALS + 3 + a] = ord_of(-FLT_MAX); }
+            hb[e * BIN_VALS + 6] = 0u;
+        }
     }
 }
 
-// One workgroup: large_off = exclusive scan of the listed nodes' chunk counts (total at
-// [count]); also clears the level's histogram-slot counter.
+// One workgroup of SCAN1 threads: out[j] = v(0) + ... + v(j - 1) for j < m; returns the total.
 constexpr uint32_t SCAN1 = 1024;
-__global__ __launch_bounds__(SCAN1) void k_large_scan(BuildState S)
+constexpr uint32_t SCAN1_ITEMS = 8;
+template <typename F>
+__device__ uint32_t wg_exclusive_scan(uint32_t m, F v, uint32_t* out)
 {
     __shared__ uint32_t sh[SCAN1];
-    const uint32_t m = S.lvl[3];
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < m; base += SCAN1)
+    for (uint32_t base = 0; base < m; base += SCAN1 * SCAN1_ITEMS)
     {
-        const uint32_t j = base + threadIdx.x;
-        const uint32_t c = j < m ? (S.ncount[S.large[j]] + CHUNK - 1) / CHUNK : 0u;
-        sh[threadIdx.x] = c;
+        const uint32_t j0 = base + threadIdx.x * SCAN1_ITEMS;
+        uint32_t x[SCAN1_ITEMS], tot = 0;
+        for (uint32_t k = 0; k < SCAN1_ITEMS; ++k)
+        {
+            x[k] = j0 + k < m ? v(j0 + k) : 0u;
+            tot += x[k];
+        }
+        sh[threadIdx.x] = tot;
         __syncthreads();
         for (uint32_t off = 1; off < SCAN1; off <<= 1)
         {
-            const uint32_t x = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
+            const uint32_t y = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
             __syncthreads();
-            sh[threadIdx.x] += x;
+            sh[threadIdx.x] += y;
             __syncthreads();
         }
-        if (j < m) S.large_off[j] = carry + sh[threadIdx.x] - c;
+        uint32_t run = carry + sh[threadIdx.x] - tot;
+        for (uint32_t k = 0; k < SCAN1_ITEMS; ++k)
+        {
+            if (j0 + k < m) out[j0 + k] = run;
+            run += x[k];
+        }
         carry += sh[SCAN1 - 1];
         __syncthreads();
     }
+    return carry;
+}
+
+// large_off = exclusive scan of the listed nodes' chunk counts (total at [count]); then the
+// level becomes current: [lvl[0], lvl[1]) = the nodes k_level_init took, lvl[3] = listed.
+__global__ __launch_bounds__(SCAN1) void k_large_scan(BuildState S)
+{
+    const uint32_t level = S.lvl[2];
+    const uint32_t m = S.lvl[10 + (level & 1u)];
+    const uint32_t total = wg_exclusive_scan(m, [&](uint32_t j) { return (S.ncount[S.large[j]] + CHUNK - 1) / CHUNK; },
+                                             S.large_off);
     if (threadIdx.x == 0)
     {
-        S.large_off[m] = carry;
-        S.counters[1] = 0u;
+        S.large_off[m] = total;
+        const uint32_t begin = S.lvl[1], end = S.counters[0];
+        S.lvl[0] = begin;
+        S.lvl[1] = end;
+        S.lvl[2] = level + 1u;
+        S.lvl[3] = m;
+        if (begin != end) S.lvl[9] += 1u;
+        S.lvl[10 + ((level + 1u) & 1u)] = 0u;  // the next level's list count
     }
 }
 
-// The node and slot range of chunk c of the listed large nodes (false past the last chunk).
+// The node, its list index and the slot range of chunk c of the listed nodes (false past the
+// last chunk).
 __device__ __forceinline__ bool chunk_range(const BuildState& S, uint32_t c, uint32_t& node, uint32_t& lo,
-                                            uint32_t& hi)
+                                            uint32_t& hi, uint32_t* list_index = nullptr)
 {
     const uint32_t m = S.lvl[3];
     if (c >= S.large_off[m]) return false;
@@ -282,6 +316,7 @@ __device__ __forceinline__ bool chunk_range(const BuildState& S, uint32_t c, uin
         else b = mid;
     }
     node = S.large[a];
+    if (list_index) *list_index = a;
     const uint32_t first = S.nfirst[node], cnt = S.ncount[node];
     lo = first + (c - S.large_off[a]) * CHUNK;
     hi = lo + CHUNK < first + cnt ? lo + CHUNK : first + cnt;
@@ -342,73 +377,6 @@ __device__ __forceinline__ float box_area(const float lo[3], const float hi[3])
     return (x * y + y * z + z * x) * 2.0f;
 }
 
-// Classify every node of the level.  The loop is wave-uniform (each wave takes 64
-// consecutive nodes per step) for the histogram-slot ballot.
-__global__ __launch_bounds__(256) void k_decide(BuildState S, uint32_t threshold)
-{
-    const uint32_t begin = S.lvl[0], end = S.lvl[1];
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t k0 = begin + blockIdx.x * blockDim.x + (threadIdx.x & ~63u); k0 < end; k0 += stride)
-    {
-        const uint32_t k = k0 + lane;
-        bool cand = false;
-        uint32_t cnt = 0;
-        if (k < end && S.ncount[k] > FIN)
-        {
-            cnt = S.ncount[k];
-            uint32_t fl = S.nflags[k];
-            if (fl & NF_FORCED)
-            {
-                if (cnt <= APG_MAX_LEAF) fl |= NF_LEAF;
-                else fl |= NF_SPLIT;
-                S.nflags[k] = fl;
-            }
-            else
-            {
-                bool leaf = cnt <= threshold;
-                if (!leaf)
-                {
-                    float clo[3], chi[3];
-                    for (int a = 0; a < 3; ++a)
-                    {
-                        clo[a] = float_of(S.ncbox[6 * size_t(k) + a]);
-                        chi[a] = float_of(S.ncbox[6 * size_t(k) + 3 + a]);
-                    }
-                    const float diag = max_s(chi[0] - clo[0], max_s(chi[1] - clo[1], chi[2] - clo[2]));
-                    if (diag == 0.0f) leaf = true;
-                }
-                if (leaf)
-                {
-                    fl |= NF_SAH_LEAF;
-                    fl |= cnt <= APG_MAX_LEAF ? NF_LEAF : (NF_FORCED | NF_SPLIT);
-                    S.nflags[k] = fl;
-                }
-                else
-                    cand = true;
-            }
-        }
-        // histogram slots for the SAH candidates: one counter atomic per wave
-        const uint64_t m = __ballot(cand);
-        if (m == 0ull) continue;
-        const uint32_t leader = __ffsll((unsigned long long)m) - 1u;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&S.counters[1], uint32_t(__popcll(m)));
-        base = uint32_t(__shfl(int(base), int(leader)));
-        if (!cand) continue;
-        const uint32_t slot = base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
-        S.nbinslot[k] = slot;
-        if (cnt <= SMALL) continue;  // k_bin_small writes the whole histogram
-        uint32_t* b = S.bins + size_t(slot) * NODE_BIN_WORDS;
-        for (uint32_t j = 0; j < 3 * NBINS; ++j)
-        {
-            uint32_t* e = b + j * BIN_VALS;
-            for (int a = 0; a < 3; ++a) { e[a] = ord_of(FLT_MAX); e[3 + a] = ord_of(-FLT_MAX); }
-            e[6] = 0;
-        }
-    }
-}
-
 // 9-bin histograms (box + count) per axis of a large SAH candidate's chunk: one
 // LDS histogram per wave, merged into the node's global histogram.
 __global__ __launch_bounds__(BLOCK) void k_bin_large(BuildState S)
@@ -463,312 +431,238 @@ __global__ __launch_bounds__(BLOCK) void k_bin_large(BuildState S)
     }
 }
 
-// Bounds of a node of at most SMALL slots: one wave walks the node's slot range
-// and reduces across lanes; the result is stored, not accumulated.
-__global__ __launch_bounds__(BLOCK) void k_bounds_small(BuildState S)
-{
-    const uint32_t begin = S.lvl[0], end = S.lvl[1];
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t waves = gridDim.x * (BLOCK / 64u);
-    for (uint32_t k = begin + (blockIdx.x * BLOCK + threadIdx.x) / 64u; k < end; k += waves)
-    {
-        const uint32_t cnt = S.ncount[k];
-        if (cnt <= FIN || cnt > SMALL) continue;
-        const uint32_t first = S.nfirst[k];
-        const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
-        uint32_t r[12];
-        for (int a = 0; a < 3; ++a)
-        {
-            r[a] = lo_ord; r[3 + a] = hi_ord; r[6 + a] = lo_ord; r[9 + a] = hi_ord;
-        }
-        for (uint32_t i = first + lane; i < first + cnt; i += 64u)
-        {
-            const float* b = S.pbox + 6 * size_t(S.idx[i]);
-            for (int a = 0; a < 3; ++a)
-            {
-                const float lo = b[a], hi = b[3 + a], c = (lo + hi) * 0.5f;
-                r[a] = min(r[a], ord_of(lo));
-                r[3 + a] = max(r[3 + a], ord_of(hi));
-                r[6 + a] = min(r[6 + a], ord_of(c));
-                r[9 + a] = max(r[9 + a], ord_of(c));
-            }
-        }
-        for (int off = 32; off > 0; off >>= 1)
-            for (int a = 0; a < 3; ++a)
-            {
-                r[a] = min(r[a], uint32_t(__shfl_xor(int(r[a]), off)));
-                r[3 + a] = max(r[3 + a], uint32_t(__shfl_xor(int(r[3 + a]), off)));
-                r[6 + a] = min(r[6 + a], uint32_t(__shfl_xor(int(r[6 + a]), off)));
-                r[9 + a] = max(r[9 + a], uint32_t(__shfl_xor(int(r[9 + a]), off)));
-            }
-        if (lane == 0u)
-            for (int j = 0; j < 6; ++j)
-            {
-                S.nbox[6 * size_t(k) + j] = r[j];
-                S.ncbox[6 * size_t(k) + j] = r[6 + j];
-            }
-    }
-}
-
-// Bins of a node of at most SMALL slots: one wave, a histogram of its own in LDS,
-// written out whole.
-__global__ __launch_bounds__(BLOCK) void k_bin_small(BuildState S)
-{
-    __shared__ uint32_t hist[BLOCK / 64u][NODE_BIN_WORDS];
-    const uint32_t begin = S.lvl[0], end = S.lvl[1];
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t waves = gridDim.x * (BLOCK / 64u);
-    uint32_t* acc = hist[threadIdx.x / 64u];
-    for (uint32_t k = begin + (blockIdx.x * BLOCK + threadIdx.x) / 64u; k < end; k += waves)
-    {
-        const uint32_t cnt = S.ncount[k];
-        if (cnt <= FIN || cnt > SMALL) continue;
-        const uint32_t slot = S.nbinslot[k];
-        if (slot == RETIRED) continue;
-        for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u)
-        {
-            const uint32_t v = j % BIN_VALS;
-            acc[j] = v < 3 ? ord_of(FLT_MAX) : (v < 6 ? ord_of(-FLT_MAX) : 0u);
-        }
-        float mn[3], mx[3];
-        for (int d = 0; d < 3; ++d)
-        {
-            mn[d] = float_of(S.ncbox[6 * size_t(k) + d]);
-            mx[d] = float_of(S.ncbox[6 * size_t(k) + 3 + d]);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        const uint32_t first = S.nfirst[k];
-        for (uint32_t i = first + lane; i < first + cnt; i += 64u)
-        {
-            const float* b = S.pbox + 6 * size_t(S.idx[i]);
-            float lo[3], hi[3];
-            for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; }
-            for (int d = 0; d < 3; ++d)
-            {
-                if (mn[d] == mx[d]) continue;
-                const float c = (lo[d] + hi[d]) * 0.5f;
-                uint32_t* e = acc + (d * NBINS + bin_index(c, mn[d], mx[d])) * BIN_VALS;
-                for (int a = 0; a < 3; ++a)
-                {
-                    atomicMin(&e[a], ord_of(lo[a]));
-                    atomicMax(&e[3 + a], ord_of(hi[a]));
-                }
-                atomicAdd(&e[6], 1u);
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        uint32_t* g = S.bins + size_t(slot) * NODE_BIN_WORDS;
-        for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u) g[j] = acc[j];
-        // the next node's clear of acc must follow every lane's read above
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-    }
-}
-
-
-
-// An SAH leaf; one of more than APG_MAX_LEAF triangles is split positionally
-// into halves rounded to whole leaves.
-__device__ __forceinline__ void sah_leaf(BuildState& S, uint32_t k, uint32_t fl, uint32_t cnt)
-{
-    fl |= NF_SAH_LEAF;
-    fl |= cnt <= APG_MAX_LEAF ? NF_LEAF : (NF_FORCED | NF_SPLIT);
-    S.nflags[k] = fl;
-    S.nbinslot[k] = RETIRED;
-    if (fl & NF_SPLIT)
-    {
-        uint32_t half = ((cnt + 1) / 2 + APG_MAX_LEAF - 1) / APG_MAX_LEAF * APG_MAX_LEAF;
-        half = half < cnt - 1 ? half : cnt - 1;
-        const uint32_t l = atomicAdd(&S.counters[0], 2u);
-        S.nleft[k] = l;
-        S.nsplit[k] = 0xFFu;
-        S.nsplit_mm[2 * size_t(k)] = __uint_as_float(half);
-        S.ncount[l] = half;
-        S.ncount[l + 1] = cnt - half;
-        S.nflags[l] = NF_FORCED;
-        S.nflags[l + 1] = NF_FORCED;
-    }
-}
-
-// Split a node at bin `idx` of axis `dim` into two children.
-__device__ __forceinline__ void split_node(BuildState& S, uint32_t k, uint32_t fl, uint32_t cnt, uint32_t nbelow,
-                                           uint32_t dim, uint32_t idx, float mn, float mx)
+// Two children of node k (first nb slots left, the rest right), allocated after the level.
+__device__ __forceinline__ void alloc_children(BuildState& S, uint32_t k, uint32_t cnt, uint32_t nb, uint32_t child_fl)
 {
     const uint32_t l = atomicAdd(&S.counters[0], 2u);
+    const uint32_t first = S.nfirst[k];
     S.nleft[k] = l;
-    S.nsplit[k] = dim | (idx << 8);
-    S.nsplit_mm[2 * size_t(k)] = mn;
-    S.nsplit_mm[2 * size_t(k) + 1] = mx;
-    S.nflags[k] = fl | NF_SPLIT;
-    S.ncount[l] = nbelow;
-    S.ncount[l + 1] = cnt - nbelow;
-    S.nflags[l] = 0;
-    S.nflags[l + 1] = 0;
+    S.ncount[l] = nb;
+    S.ncount[l + 1] = cnt - nb;
+    S.nfirst[l] = first;
+    S.nfirst[l + 1] = first + nb;
+    S.nflags[l] = child_fl;
+    S.nflags[l + 1] = child_fl;
 }
 
-// Sweep, split choice and child allocation of one node (SAH candidates; positional
-// splits for forced nodes).
-__device__ void sweep_node(BuildState& S, uint32_t k)
+// A positional split (a forced node, or an SAH leaf of more than APG_MAX_LEAF triangles): the
+// first half rounded to whole leaves goes left.
+__device__ __forceinline__ void positional_split(BuildState& S, uint32_t k, uint32_t fl, uint32_t cnt)
 {
-    uint32_t fl = S.nflags[k];
+    uint32_t half = ((cnt + 1) / 2 + APG_MAX_LEAF - 1) / APG_MAX_LEAF * APG_MAX_LEAF;
+    half = half < cnt - 1 ? half : cnt - 1;
+    S.nflags[k] = fl;
+    S.nsplit[k] = 0xFFu;
+    S.nsplit_mm[2 * size_t(k)] = __uint_as_float(half);
+    alloc_children(S, k, cnt, half, NF_FORCED);
+}
+
+// Sweep, split choice and child allocation of one node of the level, by one wave: lane
+// d * 8 + s prices split s of axis d from the node's histogram (the below box accumulated over
+// bins 0..s and the above box over bins 8..s+1, in the reference sweep's order), and the first
+// (axis, split) of least cost wins, as the sequential sweep's strict < picks it.
+__device__ void sweep_node_wave(BuildState& S, uint32_t k, uint32_t lane, uint32_t thr)
+{
     const uint32_t cnt = S.ncount[k];
-    if (fl & NF_SPLIT)
+    if (cnt <= FIN) return;  // k_finish's
+    // the leaf test (a node of the level loop holds more than FIN > APG_MAX_LEAF triangles, so
+    // forced nodes and leaves alike are split positionally)
+    uint32_t fl = S.nflags[k];
+    bool positional = (fl & NF_FORCED) != 0u;
+    if (positional) fl |= NF_SPLIT;
+    else
     {
-        // forced positional split: the first half rounded to whole leaves
-        uint32_t half = ((cnt + 1) / 2 + APG_MAX_LEAF - 1) / APG_MAX_LEAF * APG_MAX_LEAF;
-        half = half < cnt - 1 ? half : cnt - 1;
-        const uint32_t l = atomicAdd(&S.counters[0], 2u);
-        S.nleft[k] = l;
-        S.nsplit[k] = 0xFFu;  // positional
-        S.nsplit_mm[2 * size_t(k)] = __uint_as_float(half);
-        S.ncount[l] = half;
-        S.ncount[l + 1] = cnt - half;
-        S.nflags[l] = NF_FORCED;
-        S.nflags[l + 1] = NF_FORCED;
+        bool leaf = cnt <= thr;
+        if (!leaf)
+        {
+            float clo[3], chi[3];
+            for (int a = 0; a < 3; ++a)
+            {
+                clo[a] = float_of(S.ncbox[6 * size_t(k) + a]);
+                chi[a] = float_of(S.ncbox[6 * size_t(k) + 3 + a]);
+            }
+            const float diag = max_s(chi[0] - clo[0], max_s(chi[1] - clo[1], chi[2] - clo[2]));
+            if (diag == 0.0f) leaf = true;
+        }
+        if (leaf)
+        {
+            fl |= NF_SAH_LEAF | NF_FORCED | NF_SPLIT;
+            positional = true;
+        }
+    }
+    if (positional)
+    {
+        if (lane == 0u) positional_split(S, k, fl, cnt);
         return;
     }
-    const uint32_t slot = S.nbinslot[k];
-    if (slot == RETIRED) return;  // leaf
-    const uint32_t* b = S.bins + size_t(slot) * NODE_BIN_WORDS;
+    const uint32_t* hb = S.bins + size_t(S.nbinslot[k]) * NODE_BIN_WORDS;
+    float cost = __builtin_inff();
+    uint32_t key = 63u, below = 0u;
+    float mn = 0.0f, mx = 0.0f;
+    if (lane < 3u * NSPLITS)
+    {
+        const uint32_t d = lane / NSPLITS, s = lane % NSPLITS;
+        mn = float_of(S.ncbox[6 * size_t(k) + d]);
+        mx = float_of(S.ncbox[6 * size_t(k) + 3 + d]);
+        if (mn != mx)
+        {
+            float blo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, bhi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+            float alo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, ahi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+            uint32_t bn = 0, an = 0;
+            for (uint32_t b = 0; b <= s; ++b)
+            {
+                const uint32_t* e = hb + (d * NBINS + b) * BIN_VALS;
+                for (int a = 0; a < 3; ++a) { blo[a] = min_s(blo[a], float_of(e[a])); bhi[a] = max_s(bhi[a], float_of(e[3 + a])); }
+                bn += e[6];
+            }
+            for (uint32_t b = NBINS - 1; b > s; --b)
+            {
+                const uint32_t* e = hb + (d * NBINS + b) * BIN_VALS;
+                for (int a = 0; a < 3; ++a) { alo[a] = min_s(alo[a], float_of(e[a])); ahi[a] = max_s(ahi[a], float_of(e[3 + a])); }
+                an += e[6];
+            }
+            float c = 0.0f;
+            c += float(bn) * box_area(blo, bhi);
+            c += float(an) * box_area(alo, ahi);
+            if (c < FLT_MAX)
+            {
+                cost = c;
+                key = lane;
+                below = bn;
+            }
+        }
+    }
+    for (int off = 1; off < 32; off <<= 1)
+    {
+        const float oc = __shfl_xor(cost, off);
+        const uint32_t ok = uint32_t(__shfl_xor(int(key), off));
+        if (oc < cost || (oc == cost && ok < key)) { cost = oc; key = ok; }
+    }
+    const uint32_t best = uint32_t(__shfl(int(key), 0));
+    const uint32_t nbelow = uint32_t(__shfl(int(below), int(best & 63u)));
+    const float best_mn = __shfl(mn, int(best & 63u)), best_mx = __shfl(mx, int(best & 63u));
+    if (lane != 0u) return;
     float nlo[3], nhi[3];
     for (int a = 0; a < 3; ++a)
     {
         nlo[a] = float_of(S.nbox[6 * size_t(k) + a]);
         nhi[a] = float_of(S.nbox[6 * size_t(k) + 3 + a]);
     }
-    float min_cost = FLT_MAX;
-    uint32_t best_dim = 0, best_idx = 0;
-    float best_mn = 0.0f, best_mx = 0.0f;
-    bool found = false;
-    for (uint32_t d = 0; d < 3; ++d)
-    {
-        const float mn = float_of(S.ncbox[6 * size_t(k) + d]);
-        const float mx = float_of(S.ncbox[6 * size_t(k) + 3 + d]);
-        if (mn == mx) continue;
-        float costs[NSPLITS];
-        for (uint32_t s = 0; s < NSPLITS; ++s) costs[s] = 0.0f;
-        float alo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, ahi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
-        uint32_t an = 0;
-        for (uint32_t s = 0; s < NSPLITS; ++s)
-        {
-            const uint32_t* e = b + (d * NBINS + s) * BIN_VALS;
-            for (int a = 0; a < 3; ++a) { alo[a] = min_s(alo[a], float_of(e[a])); ahi[a] = max_s(ahi[a], float_of(e[3 + a])); }
-            an += e[6];
-            costs[s] += float(an) * box_area(alo, ahi);
-        }
-        for (int a = 0; a < 3; ++a) { alo[a] = FLT_MAX; ahi[a] = -FLT_MAX; }
-        an = 0;
-        for (int s = NSPLITS - 1; s >= 0; --s)
-        {
-            const uint32_t* e = b + (d * NBINS + s + 1) * BIN_VALS;
-            for (int a = 0; a < 3; ++a) { alo[a] = min_s(alo[a], float_of(e[a])); ahi[a] = max_s(ahi[a], float_of(e[3 + a])); }
-            an += e[6];
-            costs[s] += float(an) * box_area(alo, ahi);
-        }
-        for (uint32_t s = 0; s < NSPLITS; ++s)
-        {
-            if (costs[s] < min_cost)
-            {
-                min_cost = costs[s];
-                best_dim = d;
-                best_idx = s;
-                best_mn = mn;
-                best_mx = mx;
-                found = true;
-            }
-        }
-    }
     const float leaf_cost = float(cnt) * box_area(nlo, nhi);
     // no plane below FLT_MAX (surface areas overflowed): a leaf, where the reference's
     // default split would send every primitive to one side and never terminate
-    if (!found || min_cost >= leaf_cost)
+    if (best == 63u || cost >= leaf_cost)
     {
-        sah_leaf(S, k, fl, cnt);
+        positional_split(S, k, fl | NF_SAH_LEAF | NF_FORCED | NF_SPLIT, cnt);
         return;
     }
-    // child counts from the chosen axis' bins
-    uint32_t nbelow = 0;
-    for (uint32_t s = 0; s <= best_idx; ++s) nbelow += b[(best_dim * NBINS + s) * BIN_VALS + 6];
-    split_node(S, k, fl, cnt, nbelow, best_dim, best_idx, best_mn, best_mx);
+    S.nflags[k] = fl | NF_SPLIT;
+    S.nsplit[k] = (best / NSPLITS) | ((best % NSPLITS) << 8);
+    S.nsplit_mm[2 * size_t(k)] = best_mn;
+    S.nsplit_mm[2 * size_t(k) + 1] = best_mx;
+    alloc_children(S, k, cnt, nbelow, 0u);
 }
 
-__global__ __launch_bounds__(256) void k_sweep(BuildState S)
+__global__ __launch_bounds__(256) void k_sweep(BuildState S, uint32_t thr)
 {
     const uint32_t begin = S.lvl[0], end = S.lvl[1];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x; k < end; k += stride) sweep_node(S, k);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t waves = gridDim.x * (blockDim.x / 64u);
+    for (uint32_t k = begin + (blockIdx.x * blockDim.x + threadIdx.x) / 64u; k < end; k += waves)
+        sweep_node_wave(S, k, lane, thr);
 }
 
-// Partition flags: 1 = goes to the left child.
-__global__ void k_flags(BuildState S, uint32_t n)
+// ---------------------------------------------------------------------------
+// partition of the level's nodes, chunk by chunk (slots of other nodes are not touched)
+// ---------------------------------------------------------------------------
+// Whether slot i (triangle t) of split node nd goes to the left child.
+__device__ __forceinline__ bool goes_left(const BuildState& S, uint32_t nd, uint32_t first, uint32_t i, uint32_t t)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t begin = S.lvl[0], end = S.lvl[1];
-    const uint32_t nd = S.snode[i];
-    uint32_t f = 0;
-    if (nd != RETIRED && nd >= begin && nd < end && (S.nflags[nd] & NF_SPLIT))
+    const uint32_t sp = S.nsplit[nd];
+    if ((sp & 0xFFu) == 0xFFu) return (i - first) < __float_as_uint(S.nsplit_mm[2 * size_t(nd)]);
+    const uint32_t d = sp & 0xFFu, si = sp >> 8;
+    const float* b = S.pbox + 6 * size_t(t);
+    return bin_index((b[d] + b[3 + d]) * 0.5f, S.nsplit_mm[2 * size_t(nd)], S.nsplit_mm[2 * size_t(nd) + 1]) <= si;
+}
+
+// Per chunk: each slot's left flag (a byte in flag8) and the chunk's count of them.  Thread t
+// takes slots lo + t, lo + t + BLOCK, ... (coalesced).
+__global__ __launch_bounds__(BLOCK) void k_part_count(BuildState S)
+{
+    __shared__ uint32_t part[BLOCK / 64u];
+    uint32_t node, lo, hi;
+    if (!chunk_range(S, blockIdx.x, node, lo, hi)) return;
+    const uint32_t first = S.nfirst[node];
+    uint32_t cnt = 0;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += BLOCK)
     {
-        const uint32_t sp = S.nsplit[nd];
-        if ((sp & 0xFFu) == 0xFFu)
-            f = (i - S.nfirst[nd]) < __float_as_uint(S.nsplit_mm[2 * size_t(nd)]) ? 1u : 0u;
-        else
+        const bool l = goes_left(S, node, first, i, S.idx[i]);
+        S.flag8[i] = l ? 1u : 0u;
+        cnt += l ? 1u : 0u;
+    }
+    for (int off = 32; off > 0; off >>= 1) cnt += uint32_t(__shfl_xor(int(cnt), off));
+    if ((threadIdx.x & 63u) == 0u) part[threadIdx.x / 64u] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < BLOCK / 64u; ++w) t += part[w];
+        S.chunk_below[blockIdx.x] = t;
+    }
+}
+
+// One workgroup: chunk_off = exclusive scan of chunk_below over the level's chunks.
+__global__ __launch_bounds__(SCAN1) void k_part_scan(BuildState S)
+{
+    const uint32_t m = S.large_off[S.lvl[3]];
+    wg_exclusive_scan(m, [&](uint32_t c) { return S.chunk_below[c]; }, S.chunk_off);
+}
+
+// Per chunk: every slot's place in its node's stable partition (left slots first), in idx_tmp.
+// Slot lo + q * BLOCK + t is thread t's q-th; its rank among the chunk's left slots counts
+// the earlier rounds, the earlier waves of its round and the earlier lanes of its wave.
+__global__ __launch_bounds__(BLOCK) void k_part_scatter(BuildState S)
+{
+    constexpr uint32_t W = BLOCK / 64u;
+    __shared__ uint32_t wl[ITEMS][W];
+    uint32_t node, lo, hi, j;
+    if (!chunk_range(S, blockIdx.x, node, lo, hi, &j)) return;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x / 64u;
+    uint64_t masks[ITEMS];
+    for (uint32_t q = 0; q < ITEMS; ++q)
+    {
+        const uint32_t i = lo + q * BLOCK + threadIdx.x;
+        masks[q] = __ballot(i < hi && S.flag8[i] != 0u);
+        if (lane == 0u) wl[q][wave] = uint32_t(__popcll(masks[q]));
+    }
+    __syncthreads();
+    const uint32_t first = S.nfirst[node];
+    const uint32_t nleft_total = S.ncount[S.nleft[node]];
+    // left slots of the node in its earlier chunks
+    uint32_t left = S.chunk_off[blockIdx.x] - S.chunk_off[S.large_off[j]];
+    const uint64_t below_lane = (1ull << lane) - 1ull;
+    for (uint32_t q = 0; q < ITEMS; ++q)
+    {
+        const uint32_t i = lo + q * BLOCK + threadIdx.x;
+        uint32_t before = left;
+        for (uint32_t w = 0; w < wave; ++w) before += wl[q][w];
+        before += uint32_t(__popcll(masks[q] & below_lane));
+        if (i < hi)
         {
-            const uint32_t d = sp & 0xFFu, si = sp >> 8;
-            const float* b = S.pbox + 6 * size_t(S.idx[i]);
-            const float c = (b[d] + b[3 + d]) * 0.5f;
-            f = bin_index(c, S.nsplit_mm[2 * size_t(nd)], S.nsplit_mm[2 * size_t(nd) + 1]) <= si ? 1u : 0u;
+            const uint32_t pos = ((masks[q] >> lane) & 1ull) ? first + before
+                                                             : first + nleft_total + (i - first - before);
+            S.idx_tmp[pos] = S.idx[i];
         }
-    }
-    S.flag[i] = f;
-}
-
-__global__ void k_scatter(BuildState S, uint32_t n)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t begin = S.lvl[0], end = S.lvl[1];
-    const uint32_t nd = S.snode[i];
-    if (nd == RETIRED || nd < begin || nd >= end || !(S.nflags[nd] & NF_SPLIT))
-    {
-        S.idx_tmp[i] = S.idx[i];
-        S.snode_tmp[i] = (nd != RETIRED && nd >= begin && nd < end) ? RETIRED : nd;
-        return;
-    }
-    const uint32_t first = S.nfirst[nd], cnt = S.ncount[nd];
-    const uint32_t below_before = S.scan[i] - S.scan[first];
-    const uint32_t nbelow = S.scan[first + cnt] - S.scan[first];
-    const uint32_t l = S.nleft[nd];
-    uint32_t pos, child;
-    if (S.flag[i]) { pos = first + below_before; child = l; }
-    else { pos = first + nbelow + (i - first - below_before); child = l + 1; }
-    S.idx_tmp[pos] = S.idx[i];
-    S.snode_tmp[pos] = child;
-}
-
-// First slot of every child allocated in this level.
-__global__ __launch_bounds__(256) void k_child_first(BuildState S)
-{
-    const uint32_t begin = S.lvl[0], end = S.lvl[1];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t k = begin + blockIdx.x * blockDim.x + threadIdx.x; k < end; k += stride)
-    {
-        if (!(S.nflags[k] & NF_SPLIT)) continue;
-        const uint32_t l = S.nleft[k];
-        S.nfirst[l] = S.nfirst[k];
-        S.nfirst[l + 1] = S.nfirst[k] + S.ncount[l];
+        for (uint32_t w = 0; w < W; ++w) left += wl[q][w];
     }
 }
 
-// The next level is the children allocated by this one: [end, node count).
-__global__ void k_next_level(BuildState S)
+// Per chunk: the partitioned slots back into idx.
+__global__ __launch_bounds__(BLOCK) void k_part_copy(BuildState S)
 {
-    S.lvl[0] = S.lvl[1];
-    S.lvl[1] = S.counters[0];
-    S.lvl[2] += 1u;
-    S.lvl[3] = 0u;
+    uint32_t node, lo, hi;
+    if (!chunk_range(S, blockIdx.x, node, lo, hi)) return;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += BLOCK) S.idx[i] = S.idx_tmp[i];
 }
 
 // Roots: the first slot and slot count of each tree, the level control words and counters.
@@ -783,12 +677,16 @@ __global__ void k_init_roots(BuildState S, const uint32_t* rfirst, uint32_t R)
     }
     if (r == 0)
     {
+        // level 0 (the roots) is [lvl[1], counters[0]) for k_level_init
         S.lvl[0] = 0u;
-        S.lvl[1] = R;
+        S.lvl[1] = 0u;
         S.lvl[2] = 0u;
         S.lvl[3] = 0u;
         S.lvl[4] = 0u;
         S.lvl[5] = 0u;
+        S.lvl[9] = 0u;
+        S.lvl[10] = 0u;
+        S.lvl[11] = 0u;
         S.counters[0] = R;
         S.counters[1] = 0u;
         S.counters[3] = 0u;
@@ -873,7 +771,7 @@ struct FinSplit
     float mn, mx;
 };
 
-// Leaf test of the level loop (k_decide): forced nodes split positionally until they hold at
+// Leaf test of the level loop (sweep_node_wave): forced nodes split positionally until they hold at
 // most APG_MAX_LEAF triangles; others are SAH leaves at most `thr` triangles or with a zero
 // centroid extent.  Returns true when the node is decided without a sweep.
 __device__ __forceinline__ bool fin_decide(uint32_t cnt, bool forced, uint32_t thr, const float cmn[3],
@@ -898,7 +796,7 @@ __device__ __forceinline__ bool fin_decide(uint32_t cnt, bool forced, uint32_t t
     return true;
 }
 
-// The choice after a sweep (sah_leaf / split_node): a leaf unless a split costs less than the
+// The choice after a sweep (as in sweep_node_wave): a leaf unless a split costs less than the
 // leaf.
 __device__ __forceinline__ void fin_choose(uint32_t cnt, bool found, float min_cost, float leaf_cost, uint32_t d,
                                            uint32_t si, uint32_t nb, const float cmn[3], const float cmx[3],
@@ -1744,6 +1642,9 @@ struct BuildCache
     float* slack = nullptr;
     // wide collapse, capacity n + R + 2 wide nodes
     uint32_t wide_cap = 0;
+    uint32_t chunk_cap = 0;  // chunks a level can have (the chunk kernels' grid)
+    // the previous build: triangles, trees, binary levels that held nodes, wide levels
+    uint32_t last_n = 0, last_r = 0, last_levels = 0, last_wide = 0;
     uint32_t *wbin = nullptr, *wchild = nullptr, *n_inner = nullptr, *n_tris = nullptr;
     uint32_t *s_inner = nullptr, *s_tris = nullptr, *wscan_tmp = nullptr, *wl = nullptr;
     uint32_t* rb = nullptr;          // pinned read-back words
@@ -1754,7 +1655,7 @@ struct BuildCache
         ptrs.clear();
         if (rb) (void)hipHostFree(rb);
         rb = nullptr;
-        n_cap = r_cap = wide_cap = 0;
+        n_cap = r_cap = wide_cap = chunk_cap = 0;
     }
     ~BuildCache() { release(); }
 
@@ -1781,17 +1682,19 @@ struct BuildCache
         // histogram slots: only the level loop's nodes (more than FIN slots) take one, and a
         // level's nodes own disjoint slots
         const size_t bincap = size_t(n) / (FIN + 1) + 2;
-        const size_t lcap = size_t(n) / SMALL + 2;  // nodes of more than SMALL slots per level
+        const size_t lcap = size_t(n) / (FIN + 1) + 2;  // nodes of more than FIN slots per level
+        // chunks of a level: ceil(count / CHUNK) per listed node
+        const size_t ccap = size_t(n) / CHUNK + lcap + 1;
         const size_t wcap = size_t(n) + R + 2;      // >= binary nodes / 2 + 2 + R
         int rc;
         if ((rc = get(ctx, S.pbox, 6 * size_t(n))) || (rc = get(ctx, S.tri9_src, 9 * size_t(n))) ||
-            (rc = get(ctx, S.idx, n)) || (rc = get(ctx, S.idx_tmp, n)) || (rc = get(ctx, S.snode, n)) ||
-            (rc = get(ctx, S.snode_tmp, n)) || (rc = get(ctx, S.flag, n)) || (rc = get(ctx, S.scan, size_t(n) + 1)) ||
+            (rc = get(ctx, S.idx, n)) || (rc = get(ctx, S.idx_tmp, n)) || (rc = get(ctx, S.flag8, n)) ||
+            (rc = get(ctx, S.chunk_below, ccap)) || (rc = get(ctx, S.chunk_off, ccap)) ||
             (rc = get(ctx, S.nbox, 6 * ncap)) || (rc = get(ctx, S.ncbox, 6 * ncap)) || (rc = get(ctx, S.nfirst, ncap)) ||
             (rc = get(ctx, S.ncount, ncap)) || (rc = get(ctx, S.nleft, ncap)) || (rc = get(ctx, S.nflags, ncap)) ||
             (rc = get(ctx, S.nsplit, ncap)) || (rc = get(ctx, S.nsplit_mm, 2 * ncap)) ||
             (rc = get(ctx, S.nbinslot, ncap)) || (rc = get(ctx, S.bins, bincap * NODE_BIN_WORDS)) ||
-            (rc = get(ctx, S.counters, 16)) || (rc = get(ctx, S.scan_tmp, scan_tmp_words(n + 1))) ||
+            (rc = get(ctx, S.counters, 16)) ||
             (rc = get(ctx, S.lvl, 16)) || (rc = get(ctx, S.large, lcap)) || (rc = get(ctx, S.large_off, lcap + 1)) ||
             (rc = get(ctx, S.fin, size_t(n) + R)) ||
             (rc = get(ctx, rfirst, size_t(R) + 1)) || (rc = get(ctx, slack_bits, 1)) || (rc = get(ctx, slack, 1)) ||
@@ -1803,6 +1706,7 @@ struct BuildCache
         n_cap = n;
         r_cap = R;
         wide_cap = uint32_t(wcap);
+        chunk_cap = uint32_t(ccap);
         return APG_OK;
     }
 };
@@ -1886,54 +1790,51 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
         rfirst[R] = off;
     }
     APG_HIP_CHECK(ctx, hipMemcpyAsync(C.rfirst, rfirst.data(), 4 * (size_t(R) + 1), hipMemcpyHostToDevice, s));
-    k_init_slots<<<blocks_for(n, 256), 256, 0, s>>>(n, S.idx, S.snode, C.rfirst, R);
+    k_init_slots<<<blocks_for(n, 256), 256, 0, s>>>(n, S.idx);
     k_init_roots<<<blocks_for(R, 256), 256, 0, s>>>(S, C.rfirst, R);
 
     // ---- binary levels: enqueued in batches, the level range read back between batches ----
     const uint32_t g_node = std::min<uint32_t>(blocks_for(n, 256), 2048u);
-    const uint32_t g_wave = std::min<uint32_t>(blocks_for(n / (FIN + 1) + 1, BLOCK / 64u), 1024u);
-    const uint32_t g_chunk = 2 * blocks_for(n, CHUNK) + 1;
-    const uint32_t g_slot = blocks_for(n, 256);
+    const uint32_t g_chunk = C.chunk_cap;
     const uint32_t g_fin = std::min<uint32_t>(blocks_for(n, 256), 3u * 256u);
     uint32_t* rb = C.rb;
     auto enqueue_level = [&]() {
         k_level_init<<<g_node, 256, 0, s>>>(S);
         k_large_scan<<<1, SCAN1, 0, s>>>(S);
         k_bounds_large<<<g_chunk, BLOCK, 0, s>>>(S);
-        k_bounds_small<<<g_wave, BLOCK, 0, s>>>(S);
-        k_decide<<<g_node, 256, 0, s>>>(S, thr);
         k_bin_large<<<g_chunk, BLOCK, 0, s>>>(S);
-        k_bin_small<<<g_wave, BLOCK, 0, s>>>(S);
-        k_sweep<<<g_node, 256, 0, s>>>(S);
-        k_child_first<<<g_node, 256, 0, s>>>(S);
-        k_flags<<<g_slot, 256, 0, s>>>(S, n);
-        exclusive_scan(S.flag, n, S.scan, S.scan_tmp, s);
-        k_scatter<<<g_slot, 256, 0, s>>>(S, n);
-        k_next_level<<<1, 1, 0, s>>>(S);
-        std::swap(S.idx, S.idx_tmp);
-        std::swap(S.snode, S.snode_tmp);
+        k_sweep<<<g_node, 256, 0, s>>>(S, thr);
+        k_part_count<<<g_chunk, BLOCK, 0, s>>>(S);
+        k_part_scan<<<1, SCAN1, 0, s>>>(S);
+        k_part_scatter<<<g_chunk, BLOCK, 0, s>>>(S);
+        k_part_copy<<<g_chunk, BLOCK, 0, s>>>(S);
     };
     // the level loop ends where every node holds at most FIN slots, a little deeper than
-    // log2(n / FIN) levels: the first batch rarely runs empty levels
+    // log2(n / FIN) levels; a rebuild of the same triangle count starts with the levels the
+    // previous build took
+    const bool same = C.last_n == n && C.last_r == R;
     uint32_t enqueued = 0;
     for (;;)
     {
-        const uint32_t batch = enqueued == 0 ? ceil_log2(n / FIN + 1) + 3u : 4u;
+        const uint32_t batch = enqueued == 0 ? (same && C.last_levels ? C.last_levels : ceil_log2(n / FIN + 1) + 3u) : 2u;
         for (uint32_t b = 0; b < batch; ++b) enqueue_level();
         enqueued += batch;
+        // done when the last level allocated no children
         APG_HIP_CHECK(ctx, hipMemcpyAsync(rb, S.lvl, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        APG_HIP_CHECK(ctx, hipMemcpyAsync(rb + 4, S.counters, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
         APG_HIP_CHECK(ctx, hipGetLastError());
-        if (rb[0] == rb[1]) break;
+        if (rb[1] == rb[4]) break;
         if (enqueued > 4096) return ctx->fail(APG_E_CAPACITY, "binary tree deeper than 4096 levels");
     }
     // the subtrees of at most FIN slots
     k_finish<<<g_fin, FIN_BLOCK, 0, s>>>(S, thr);
     k_finish_done<<<1, 1, 0, s>>>(S);
-    APG_HIP_CHECK(ctx, hipMemcpyAsync(rb, S.lvl, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    APG_HIP_CHECK(ctx, hipMemcpyAsync(rb, S.lvl, 10 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     APG_HIP_CHECK(ctx, hipStreamSynchronize(s));
     APG_HIP_CHECK(ctx, hipGetLastError());
     const uint32_t bin_ids = rb[6], bin_nodes = rb[7];
+    C.last_levels = rb[9];
 
     // ---- wide collapse ----
     k_fill_u32<<<1, 64, 0, s>>>(C.slack_bits, 1, 0u);
@@ -1964,7 +1865,8 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
     enqueued = 0;
     for (;;)
     {
-        const uint32_t batch = enqueued == 0 ? std::max<uint32_t>(2u, (ceil_log2(n) + 2u) / 3u) : 2u;
+        const uint32_t batch =
+            enqueued == 0 ? (same && C.last_wide ? C.last_wide : std::max<uint32_t>(2u, (ceil_log2(n) + 2u) / 3u)) : 2u;
         for (uint32_t b = 0; b < batch; ++b) enqueue_wide();
         enqueued += batch;
         APG_HIP_CHECK(ctx, hipMemcpyAsync(rb, W.wl, 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1975,6 +1877,9 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
         if (enqueued > 4096) return ctx->fail(APG_E_CAPACITY, "wide tree deeper than 4096 levels");
     }
     const uint32_t wide_nodes = rb[1], tri_cursor = rb[2], wlevels = rb[3];
+    C.last_wide = wlevels;
+    C.last_n = n;
+    C.last_r = R;
     if (tri_cursor != n) return ctx->fail(APG_E_STATE, "GPU build lost triangles");
     sc.node_count = wide_nodes;
     sc.tri_count = tri_cursor;
