@@ -377,16 +377,19 @@ __device__ __forceinline__ float box_area(const float lo[3], const float hi[3])
     return (x * y + y * z + z * x) * 2.0f;
 }
 
-// 9-bin histograms (box + count) per axis of a large SAH candidate's chunk: one
-// LDS histogram per wave, merged into the node's global histogram.
+// 9-bin histograms (box + count) per axis of a large SAH candidate's chunk: one LDS histogram
+// per quarter wave (lanes that share a bin then hit four addresses, not one), merged into the
+// node's global histogram.
+constexpr uint32_t HCOPIES = 4;  // histogram copies per wave, by lane % HCOPIES
 __global__ __launch_bounds__(BLOCK) void k_bin_large(BuildState S)
 {
-    __shared__ uint32_t hist[BLOCK / 64u][NODE_BIN_WORDS];
+    constexpr uint32_t NH = (BLOCK / 64u) * HCOPIES;
+    __shared__ uint32_t hist[NH][NODE_BIN_WORDS];
     uint32_t node, lo_i, hi_i;
     if (!chunk_range(S, blockIdx.x, node, lo_i, hi_i)) return;
     const uint32_t slot = S.nbinslot[node];
     if (slot == RETIRED) return;
-    for (uint32_t j = threadIdx.x; j < (BLOCK / 64u) * NODE_BIN_WORDS; j += BLOCK)
+    for (uint32_t j = threadIdx.x; j < NH * NODE_BIN_WORDS; j += BLOCK)
     {
         const uint32_t v = (j % NODE_BIN_WORDS) % BIN_VALS;
         (&hist[0][0])[j] = v < 3 ? ord_of(FLT_MAX) : (v < 6 ? ord_of(-FLT_MAX) : 0u);
@@ -398,7 +401,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin_large(BuildState S)
         mx[d] = float_of(S.ncbox[6 * size_t(node) + 3 + d]);
     }
     __syncthreads();
-    uint32_t* acc = hist[threadIdx.x / 64u];
+    uint32_t* acc = hist[(threadIdx.x / 64u) * HCOPIES + (threadIdx.x % HCOPIES)];
     for (uint32_t i = lo_i + threadIdx.x; i < hi_i; i += BLOCK)
     {
         const float* b = S.pbox + 6 * size_t(S.idx[i]);
@@ -423,7 +426,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin_large(BuildState S)
     {
         const uint32_t v = j % BIN_VALS;
         uint32_t x = hist[0][j];
-        for (uint32_t w = 1; w < BLOCK / 64u; ++w)
+        for (uint32_t w = 1; w < NH; ++w)
             x = v == 6 ? x + hist[w][j] : (v < 3 ? min(x, hist[w][j]) : max(x, hist[w][j]));
         if (v == 6) { if (x) atomicAdd(&g[j], x); }
         else if (v < 3) atomicMin(&g[j], x);
@@ -736,7 +739,7 @@ struct FinShared
     FinNode cand[FIN_CCAP];                 // this sub-level's tiny SAH candidates
     float ccost[FIN_CCAP][3];               // their best cost per axis (+inf: none)
     uint16_t cinfo[FIN_CCAP][3];            // split | below count << 8 of that cost
-    uint32_t hist[FIN_WAVES][NODE_BIN_WORDS];
+    uint32_t hist[FIN_WAVES][HCOPIES][NODE_BIN_WORDS];  // per wave, one copy per lane % HCOPIES
     uint32_t qn[2], tn[2], cn, next, base, item;
 };
 
@@ -906,13 +909,15 @@ __device__ void fin_wave_node(BuildState& S, FinShared& L, uint32_t thr, uint32_
     FinSplit sp;
     if (!fin_decide(cnt, fin_forced(e), thr, cmn, cmx, fl, sp))
     {
-        // 9-bin histograms of the three axes in the wave's own LDS block
-        for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u)
+        // 9-bin histograms of the three axes in the wave's own LDS block (HCOPIES copies,
+        // merged into the first)
+        for (uint32_t j = lane; j < HCOPIES * NODE_BIN_WORDS; j += 64u)
         {
-            const uint32_t v = j % BIN_VALS;
+            const uint32_t v = (j % NODE_BIN_WORDS) % BIN_VALS;
             hist[j] = v < 3 ? ord_of(FLT_MAX) : (v < 6 ? ord_of(-FLT_MAX) : 0u);
         }
         wave_sync_lds();
+        uint32_t* hc = hist + (lane % HCOPIES) * NODE_BIN_WORDS;
         for (uint32_t j = lane; j < cnt; j += 64u)
         {
             const float* b = L.box[L.lid[f + j]];
@@ -922,7 +927,7 @@ __device__ void fin_wave_node(BuildState& S, FinShared& L, uint32_t thr, uint32_
             {
                 if (cmn[d] == cmx[d]) continue;
                 const float c = (lo[d] + hi[d]) * 0.5f;
-                uint32_t* h = hist + (d * NBINS + bin_index(c, cmn[d], cmx[d])) * BIN_VALS;
+                uint32_t* h = hc + (d * NBINS + bin_index(c, cmn[d], cmx[d])) * BIN_VALS;
                 for (int a = 0; a < 3; ++a)
                 {
                     atomicMin(&h[a], ord_of(lo[a]));
@@ -930,6 +935,18 @@ __device__ void fin_wave_node(BuildState& S, FinShared& L, uint32_t thr, uint32_
                 }
                 atomicAdd(&h[6], 1u);
             }
+        }
+        wave_sync_lds();
+        for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u)
+        {
+            const uint32_t v = j % BIN_VALS;
+            uint32_t x = hist[j];
+            for (uint32_t c = 1; c < HCOPIES; ++c)
+            {
+                const uint32_t y = hist[c * NODE_BIN_WORDS + j];
+                x = v == 6 ? x + y : (v < 3 ? min(x, y) : max(x, y));
+            }
+            hist[j] = x;
         }
         wave_sync_lds();
         // the sweep (sweep_node): lane d * 8 + s prices split s of axis d, the below box
@@ -1175,7 +1192,7 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finish(BuildState S, uint32_t thr
             const uint32_t qn = L.qn[cur];
             if (qn == 0) break;
             for (uint32_t q = wave; q < qn; q += FIN_WAVES)
-                fin_wave_node(S, L, thr, gfirst, L.queue[cur][q], cur ^ 1u, L.hist[wave]);
+                fin_wave_node(S, L, thr, gfirst, L.queue[cur][q], cur ^ 1u, &L.hist[wave][0][0]);
             __syncthreads();
             if (tid == 0) L.qn[cur] = 0;
             cur ^= 1u;
