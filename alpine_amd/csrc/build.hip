@@ -739,7 +739,7 @@ struct FinShared
     FinNode cand[FIN_CCAP];                 // this sub-level's tiny SAH candidates
     float ccost[FIN_CCAP][3];               // their best cost per axis (+inf: none)
     uint16_t cinfo[FIN_CCAP][3];            // split | below count << 8 of that cost
-    uint32_t hist[FIN_WAVES][HCOPIES][NODE_BIN_WORDS];  // per wave, one copy per lane % HCOPIES
+    uint32_t hist[FIN_WAVES][NODE_BIN_WORDS];
     uint32_t qn[2], tn[2], cn, next, base, item;
 };
 
@@ -909,15 +909,14 @@ __device__ void fin_wave_node(BuildState& S, FinShared& L, uint32_t thr, uint32_
     FinSplit sp;
     if (!fin_decide(cnt, fin_forced(e), thr, cmn, cmx, fl, sp))
     {
-        // 9-bin histograms of the three axes in the wave's own LDS block (HCOPIES copies,
-        // merged into the first)
-        for (uint32_t j = lane; j < HCOPIES * NODE_BIN_WORDS; j += 64u)
+        // 9-bin histograms of the three axes in the wave's own LDS block
+        for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u)
         {
-            const uint32_t v = (j % NODE_BIN_WORDS) % BIN_VALS;
+            const uint32_t v = j % BIN_VALS;
             hist[j] = v < 3 ? ord_of(FLT_MAX) : (v < 6 ? ord_of(-FLT_MAX) : 0u);
         }
         wave_sync_lds();
-        uint32_t* hc = hist + (lane % HCOPIES) * NODE_BIN_WORDS;
+        uint32_t* hc = hist;
         for (uint32_t j = lane; j < cnt; j += 64u)
         {
             const float* b = L.box[L.lid[f + j]];
@@ -935,18 +934,6 @@ __device__ void fin_wave_node(BuildState& S, FinShared& L, uint32_t thr, uint32_
                 }
                 atomicAdd(&h[6], 1u);
             }
-        }
-        wave_sync_lds();
-        for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u)
-        {
-            const uint32_t v = j % BIN_VALS;
-            uint32_t x = hist[j];
-            for (uint32_t c = 1; c < HCOPIES; ++c)
-            {
-                const uint32_t y = hist[c * NODE_BIN_WORDS + j];
-                x = v == 6 ? x + y : (v < 3 ? min(x, y) : max(x, y));
-            }
-            hist[j] = x;
         }
         wave_sync_lds();
         // the sweep (sweep_node): lane d * 8 + s prices split s of axis d, the below box
@@ -1056,16 +1043,16 @@ __device__ void fin_tiny_axis(FinShared& L, uint32_t c, uint32_t d)
     bool found = false;
     if (mn != mx)
     {
-        float blo[NSPLITS][3], bhi[NSPLITS][3], alo[NSPLITS][3], ahi[NSPLITS][3];
-        uint32_t bn[NSPLITS], an[NSPLITS];
+        // the slots' 9 bin boxes and counts, then each split's below box over bins 0..s and
+        // above box over bins 8..s+1 (min / max are exact: the boxes equal those the slots
+        // give directly, up to the sign of zero, which no area comparison sees)
+        float lo_[NBINS][3], hi_[NBINS][3];
+        uint32_t nbin[NBINS];
 #pragma unroll
-        for (uint32_t s = 0; s < NSPLITS; ++s)
+        for (uint32_t q = 0; q < NBINS; ++q)
         {
-            bn[s] = 0; an[s] = 0;
-            for (int a = 0; a < 3; ++a)
-            {
-                blo[s][a] = FLT_MAX; bhi[s][a] = -FLT_MAX; alo[s][a] = FLT_MAX; ahi[s][a] = -FLT_MAX;
-            }
+            nbin[q] = 0;
+            for (int a = 0; a < 3; ++a) { lo_[q][a] = FLT_MAX; hi_[q][a] = -FLT_MAX; }
         }
         for (uint32_t j = 0; j < cnt; ++j)
         {
@@ -1074,34 +1061,50 @@ __device__ void fin_tiny_axis(FinShared& L, uint32_t c, uint32_t d)
             for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; }
             const uint32_t bi = bin_index((lo[d] + hi[d]) * 0.5f, mn, mx);
 #pragma unroll
-            for (uint32_t s = 0; s < NSPLITS; ++s)
+            for (uint32_t q = 0; q < NBINS; ++q)
             {
-                if (bi <= s)
+                const bool in = bi == q;
+                nbin[q] += in ? 1u : 0u;
+                for (int a = 0; a < 3; ++a)
                 {
-                    bn[s]++;
-                    for (int a = 0; a < 3; ++a) { blo[s][a] = min_s(blo[s][a], lo[a]); bhi[s][a] = max_s(bhi[s][a], hi[a]); }
+                    lo_[q][a] = fminf(lo_[q][a], in ? lo[a] : FLT_MAX);
+                    hi_[q][a] = fmaxf(hi_[q][a], in ? hi[a] : -FLT_MAX);
                 }
-                else
-                {
-                    an[s]++;
-                    for (int a = 0; a < 3; ++a) { alo[s][a] = min_s(alo[s][a], lo[a]); ahi[s][a] = max_s(ahi[s][a], hi[a]); }
-                }
+            }
+        }
+        float cost[NSPLITS];
+        uint32_t bn[NSPLITS];
+        {
+            float plo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, phi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+            uint32_t pn = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < NSPLITS; ++q)
+            {
+                for (int a = 0; a < 3; ++a) { plo[a] = fminf(plo[a], lo_[q][a]); phi[a] = fmaxf(phi[a], hi_[q][a]); }
+                pn += nbin[q];
+                bn[q] = pn;
+                cost[q] = 0.0f;
+                cost[q] += float(pn) * box_area(plo, phi);
+            }
+        }
+        {
+            float plo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, phi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+#pragma unroll
+            for (int q = NSPLITS - 1; q >= 0; --q)
+            {
+                for (int a = 0; a < 3; ++a) { plo[a] = fminf(plo[a], lo_[q + 1][a]); phi[a] = fmaxf(phi[a], hi_[q + 1][a]); }
+                cost[q] += float(cnt - bn[q]) * box_area(plo, phi);
             }
         }
 #pragma unroll
-        for (uint32_t s = 0; s < NSPLITS; ++s)
-        {
-            float cost = 0.0f;
-            cost += float(bn[s]) * box_area(blo[s], bhi[s]);
-            cost += float(an[s]) * box_area(alo[s], ahi[s]);
-            if (cost < min_cost)
+        for (uint32_t q = 0; q < NSPLITS; ++q)
+            if (cost[q] < min_cost)
             {
-                min_cost = cost;
-                best_idx = s;
-                best_below = bn[s];
+                min_cost = cost[q];
+                best_idx = q;
+                best_below = bn[q];
                 found = true;
             }
-        }
     }
     L.ccost[c][d] = found ? min_cost : __builtin_inff();
     L.cinfo[c][d] = uint16_t(best_idx | (best_below << 8));
@@ -1192,7 +1195,7 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_finish(BuildState S, uint32_t thr
             const uint32_t qn = L.qn[cur];
             if (qn == 0) break;
             for (uint32_t q = wave; q < qn; q += FIN_WAVES)
-                fin_wave_node(S, L, thr, gfirst, L.queue[cur][q], cur ^ 1u, &L.hist[wave][0][0]);
+                fin_wave_node(S, L, thr, gfirst, L.queue[cur][q], cur ^ 1u, L.hist[wave]);
             __syncthreads();
             if (tid == 0) L.qn[cur] = 0;
             cur ^= 1u;
@@ -1256,99 +1259,13 @@ __global__ void k_finish_done(BuildState S)
 }
 
 // ---------------------------------------------------------------------------
-// exclusive scan (3 passes, recursive on the block sums)
-// ---------------------------------------------------------------------------
-// A single-workgroup scan (total != nullptr) also writes the total.
-__global__ __launch_bounds__(BLOCK) void k_scan_blocks(const uint32_t* in, uint32_t n, uint32_t* out,
-                                                       uint32_t* sums, uint32_t* total)
-{
-    __shared__ uint32_t sh[BLOCK];
-    const uint32_t base = blockIdx.x * CHUNK + threadIdx.x * ITEMS;
-    uint32_t v[ITEMS], tot = 0;
-    for (uint32_t k = 0; k < ITEMS; ++k)
-    {
-        v[k] = (base + k < n) ? in[base + k] : 0u;
-        tot += v[k];
-    }
-    sh[threadIdx.x] = tot;
-    __syncthreads();
-    for (uint32_t off = 1; off < BLOCK; off <<= 1)
-    {
-        uint32_t x = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
-        __syncthreads();
-        sh[threadIdx.x] += x;
-        __syncthreads();
-    }
-    uint32_t run = sh[threadIdx.x] - tot;
-    for (uint32_t k = 0; k < ITEMS; ++k)
-    {
-        if (base + k < n) out[base + k] = run;
-        run += v[k];
-    }
-    if (threadIdx.x == BLOCK - 1)
-    {
-        sums[blockIdx.x] = sh[threadIdx.x];
-        if (total) *total = sh[threadIdx.x];
-    }
-}
-
-// Adds each workgroup's offset; offs[gridDim.x] (the scan's total) goes to out[n].
-__global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* offs)
-{
-    const uint32_t i = blockIdx.x * CHUNK + threadIdx.x;
-    const uint32_t add = offs[blockIdx.x];
-    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = offs[gridDim.x];
-    for (uint32_t k = 0; k < ITEMS; ++k)
-    {
-        const uint32_t j = i + k * BLOCK;
-        if (j < n) out[j] += add;
-    }
-}
-
-__global__ void k_set_total(const uint32_t* in, const uint32_t* out, uint32_t n, uint32_t* total)
-{
-    *total = n ? out[n - 1] + in[n - 1] : 0u;
-}
-
-// out[0..n) = exclusive scan of in; out[n] = total.  tmp needs ceil(n / CHUNK) * 2 + 64 words.
-void exclusive_scan(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* tmp, hipStream_t s)
-{
-    const uint32_t nb = (n + CHUNK - 1) / CHUNK;
-    if (n == 0)
-    {
-        k_set_total<<<1, 1, 0, s>>>(in, out, 0, out);
-        return;
-    }
-    uint32_t* sums = tmp;
-    uint32_t* sums_scan = tmp + nb + 1;
-    k_scan_blocks<<<nb, BLOCK, 0, s>>>(in, n, out, sums, nb == 1 ? out + n : nullptr);
-    if (nb > 1)
-    {
-        exclusive_scan(sums, nb, sums_scan, sums_scan + nb + 1, s);   // sums_scan[nb] = total
-        k_scan_add<<<nb, BLOCK, 0, s>>>(out, n, sums_scan);
-    }
-}
-
-size_t scan_tmp_words(uint32_t n)
-{
-    size_t words = 64;
-    while (n > 1)
-    {
-        uint32_t nb = (n + CHUNK - 1) / CHUNK;
-        words += 2 * (size_t(nb) + 1);
-        n = nb;
-    }
-    return words;
-}
-
-// ---------------------------------------------------------------------------
 // wide collapse
 // ---------------------------------------------------------------------------
 // Breadth first over wide levels, like the binary levels: the level's wide nodes
-// [wl[0], wl[1]) are read from device memory.  Per-node counts are kept by absolute wide id
-// and are zero outside the level, so exclusive scans over the whole capacity give the level's
-// child and triangle offsets in wide-node order (deterministic) without the host knowing the
-// level's width.
+// [wl[0], wl[1]) are read from device memory.  Exclusive scans of the level's per-node counts
+// (over that range only, on a fixed grid) give its child and triangle offsets in wide-node
+// order (deterministic) without the host knowing the level's width; the leaf triangles are
+// copied in one pass after the last level (k_tri_gather).
 struct WideState
 {
     const uint32_t* nbox;   // ordered uints
@@ -1364,11 +1281,14 @@ struct WideState
     uint32_t* n_tris;       // per wide node: triangles of its leaf slots (0 outside the level)
     uint32_t* s_inner;      // exclusive scans over the capacity (wcap + 1)
     uint32_t* s_tris;
+    uint32_t* wpart;        // per workgroup of the level scan: sums, then offsets (4 * WSCAN_BLOCKS)
+    uint32_t* perm;         // triangle position -> slot (k_tri_gather then copies)
     ApgWideNode* nodes;
     float* tri9;
     uint32_t* tri_ids;
     const float* slack;     // device: the quantisation slack
-    uint32_t* wl;           // [0] level begin, [1] level end, [2] triangles placed, [3] levels, [4] error
+    uint32_t* wl;           // [0] level begin, [1] level end, [2] triangles placed, [3] levels, [4] error,
+                            // [5], [6] the level's inner children and leaf triangles
     uint32_t wcap;
 };
 
@@ -1503,11 +1423,12 @@ __device__ uint32_t quantise_axis(float lo_f, float hi_f, float slack, const flo
 }
 
 
-__device__ void wide_emit(const WideState& W, uint32_t w, uint32_t level_end, uint32_t tri_begin, float slack)
+__device__ void wide_emit(const WideState& W, uint32_t w, uint32_t level_end, uint32_t tri_begin, float slack,
+                          uint32_t seg_inner, uint32_t seg_tris)
 {
     ApgWideNode nd;
-    const uint32_t child_base = level_end + W.s_inner[w];
-    const uint32_t tri_base = tri_begin + W.s_tris[w];
+    const uint32_t child_base = level_end + seg_inner + W.s_inner[w];
+    const uint32_t tri_base = tri_begin + seg_tris + W.s_tris[w];
     uint32_t imask = 0, valid = 0, cmask = 0, inner_rank = 0, toff = 0;
     float clo[3][APG_WIDE], chi[3][APG_WIDE];
     float nlo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, nhi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
@@ -1530,15 +1451,7 @@ __device__ void wide_emit(const WideState& W, uint32_t w, uint32_t level_end, ui
         {
             const uint32_t cnt = W.ncount[c], first = W.nfirst[c];
             cmask |= apg_leaf_nibble(s, cnt);
-            for (uint32_t k = 0; k < cnt; ++k)
-            {
-                const uint32_t g = W.idx[first + k];
-                const float* src = W.tri9_src + 9 * size_t(g);
-                float* dst = W.tri9 + 9 * size_t(tri_base + toff);
-                for (int j = 0; j < 9; ++j) dst[j] = src[j];
-                W.tri_ids[tri_base + toff] = g;
-                toff++;
-            }
+            for (uint32_t k = 0; k < cnt; ++k) W.perm[tri_base + toff++] = first + k;
         }
         else
         {
@@ -1564,18 +1477,126 @@ __device__ void wide_emit(const WideState& W, uint32_t w, uint32_t level_end, ui
             apg_put_plane(nd.q, s, 3 + a, qhi[a][s]);
         }
     W.nodes[w] = nd;
-    // the scans of the next level must see zero outside that level
-    W.n_inner[w] = 0u;
-    W.n_tris[w] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scans of one wide level's counts over [wl[0], wl[1]) only: a grid fixed for the
+// build, each workgroup a contiguous segment of the level; the segment sums are scanned by
+// one workgroup and k_wide_emit adds its segment's offset.
+constexpr uint32_t WSCAN_BLOCKS = 256;
+
+__device__ __forceinline__ void wscan_segment(const WideState& W, uint32_t blk, uint32_t& s0, uint32_t& s1,
+                                              uint32_t& seg)
+{
+    const uint32_t b = W.wl[0], e = W.wl[1];
+    seg = (e - b + WSCAN_BLOCKS - 1) / WSCAN_BLOCKS;
+    s0 = b + blk * seg;
+    s1 = s0 + seg < e ? s0 + seg : e;
+    if (s0 > s1) s0 = s1;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_wscan_local(WideState W)
+{
+    __shared__ uint32_t sh[2][BLOCK];
+    uint32_t s0, s1, seg;
+    wscan_segment(W, blockIdx.x, s0, s1, seg);
+    uint32_t carry_i = 0, carry_t = 0;
+    for (uint32_t base = s0; base < s1; base += CHUNK)
+    {
+        const uint32_t j0 = base + threadIdx.x * ITEMS;
+        uint32_t vi[ITEMS], vt[ITEMS], ti = 0, tt = 0;
+        for (uint32_t k = 0; k < ITEMS; ++k)
+        {
+            const bool in = j0 + k < s1;
+            vi[k] = in ? W.n_inner[j0 + k] : 0u;
+            vt[k] = in ? W.n_tris[j0 + k] : 0u;
+            ti += vi[k];
+            tt += vt[k];
+        }
+        sh[0][threadIdx.x] = ti;
+        sh[1][threadIdx.x] = tt;
+        __syncthreads();
+        for (uint32_t off = 1; off < BLOCK; off <<= 1)
+        {
+            const uint32_t xi = threadIdx.x >= off ? sh[0][threadIdx.x - off] : 0u;
+            const uint32_t xt = threadIdx.x >= off ? sh[1][threadIdx.x - off] : 0u;
+            __syncthreads();
+            sh[0][threadIdx.x] += xi;
+            sh[1][threadIdx.x] += xt;
+            __syncthreads();
+        }
+        uint32_t ri = carry_i + sh[0][threadIdx.x] - ti, rt = carry_t + sh[1][threadIdx.x] - tt;
+        for (uint32_t k = 0; k < ITEMS; ++k)
+        {
+            if (j0 + k < s1)
+            {
+                W.s_inner[j0 + k] = ri;
+                W.s_tris[j0 + k] = rt;
+            }
+            ri += vi[k];
+            rt += vt[k];
+        }
+        carry_i += sh[0][BLOCK - 1];
+        carry_t += sh[1][BLOCK - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        W.wpart[blockIdx.x] = carry_i;
+        W.wpart[WSCAN_BLOCKS + blockIdx.x] = carry_t;
+    }
+}
+
+// One workgroup of WSCAN_BLOCKS threads: the segments' offsets and the level's totals.
+__global__ __launch_bounds__(WSCAN_BLOCKS) void k_wscan_top(WideState W)
+{
+    __shared__ uint32_t sh[2][WSCAN_BLOCKS];
+    const uint32_t t = threadIdx.x;
+    const uint32_t ti = W.wpart[t], tt = W.wpart[WSCAN_BLOCKS + t];
+    sh[0][t] = ti;
+    sh[1][t] = tt;
+    __syncthreads();
+    for (uint32_t off = 1; off < WSCAN_BLOCKS; off <<= 1)
+    {
+        const uint32_t xi = t >= off ? sh[0][t - off] : 0u;
+        const uint32_t xt = t >= off ? sh[1][t - off] : 0u;
+        __syncthreads();
+        sh[0][t] += xi;
+        sh[1][t] += xt;
+        __syncthreads();
+    }
+    W.wpart[2 * WSCAN_BLOCKS + t] = sh[0][t] - ti;
+    W.wpart[3 * WSCAN_BLOCKS + t] = sh[1][t] - tt;
+    if (t == WSCAN_BLOCKS - 1)
+    {
+        W.wl[5] = sh[0][t];
+        W.wl[6] = sh[1][t];
+    }
 }
 
 __global__ __launch_bounds__(128) void k_wide_emit(WideState W)
 {
     const uint32_t begin = W.wl[0], end = W.wl[1], tri_begin = W.wl[2];
+    const uint32_t seg = (end - begin + WSCAN_BLOCKS - 1) / WSCAN_BLOCKS;
     const float slack = *W.slack;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t w = begin + blockIdx.x * blockDim.x + threadIdx.x; w < end; w += stride)
-        wide_emit(W, w, end, tri_begin, slack);
+    {
+        const uint32_t blk = (w - begin) / seg;
+        wide_emit(W, w, end, tri_begin, slack, W.wpart[2 * WSCAN_BLOCKS + blk], W.wpart[3 * WSCAN_BLOCKS + blk]);
+    }
+}
+
+// The reordered triangles: position p holds the triangle of slot perm[p].
+__global__ void k_tri_gather(WideState W, uint32_t n)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t g = W.idx[W.perm[p]];
+    const float* src = W.tri9_src + 9 * size_t(g);
+    float* dst = W.tri9 + 9 * size_t(p);
+    for (int j = 0; j < 9; ++j) dst[j] = src[j];
+    W.tri_ids[p] = g;
 }
 
 // The next wide level is the inner children placed by this one.
@@ -1583,7 +1604,7 @@ __global__ void k_next_wide(WideState W)
 {
     const uint32_t begin = W.wl[0], end = W.wl[1];
     if (begin == end) return;  // past the last level: nothing placed, not a level
-    const uint32_t inner = W.s_inner[W.wcap], tris = W.s_tris[W.wcap];
+    const uint32_t inner = W.wl[5], tris = W.wl[6];
     const bool fits = end + inner <= W.wcap;
     W.wl[0] = end;
     W.wl[1] = fits ? end + inner : end;  // beyond the capacity: stop, with the error set
@@ -1663,7 +1684,7 @@ struct BuildCache
     // the previous build: triangles, trees, binary levels that held nodes, wide levels
     uint32_t last_n = 0, last_r = 0, last_levels = 0, last_wide = 0;
     uint32_t *wbin = nullptr, *wchild = nullptr, *n_inner = nullptr, *n_tris = nullptr;
-    uint32_t *s_inner = nullptr, *s_tris = nullptr, *wscan_tmp = nullptr, *wl = nullptr;
+    uint32_t *s_inner = nullptr, *s_tris = nullptr, *wpart = nullptr, *wl = nullptr;
     uint32_t* rb = nullptr;          // pinned read-back words
 
     void release()
@@ -1717,7 +1738,7 @@ struct BuildCache
             (rc = get(ctx, rfirst, size_t(R) + 1)) || (rc = get(ctx, slack_bits, 1)) || (rc = get(ctx, slack, 1)) ||
             (rc = get(ctx, wbin, 2 * wcap + 8)) || (rc = get(ctx, wchild, 8 * wcap)) || (rc = get(ctx, n_inner, wcap)) ||
             (rc = get(ctx, n_tris, wcap)) || (rc = get(ctx, s_inner, wcap + 1)) || (rc = get(ctx, s_tris, wcap + 1)) ||
-            (rc = get(ctx, wscan_tmp, scan_tmp_words(uint32_t(wcap) + 1))) || (rc = get(ctx, wl, 16)))
+            (rc = get(ctx, wpart, 4 * WSCAN_BLOCKS)) || (rc = get(ctx, wl, 16)))
             return rc;
         APG_HIP_CHECK(ctx, hipHostMalloc(&rb, 64 * sizeof(uint32_t)));
         n_cap = n;
@@ -1867,15 +1888,13 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
     W.s_inner = C.s_inner; W.s_tris = C.s_tris;
     W.nodes = sc.nodes; W.tri9 = sc.tris; W.tri_ids = sc.tri_ids;
     W.slack = C.slack; W.wl = C.wl; W.wcap = wcap;
-    const uint32_t g_fill = std::min<uint32_t>(blocks_for(wcap, 256), 1024u);
-    k_fill_u32<<<g_fill, 256, 0, s>>>(W.n_inner, wcap, 0u);
-    k_fill_u32<<<g_fill, 256, 0, s>>>(W.n_tris, wcap, 0u);
+    W.wpart = C.wpart; W.perm = S.idx_tmp;  // idx_tmp is free once the binary tree is built
     k_init_wide<<<blocks_for(R, 256), 256, 0, s>>>(W, R);
     const uint32_t g_wide = std::min<uint32_t>(blocks_for(wcap, 128), 2048u);
     auto enqueue_wide = [&]() {
         k_wide_gather<<<g_wide, 128, 0, s>>>(W);
-        exclusive_scan(W.n_inner, wcap, W.s_inner, C.wscan_tmp, s);
-        exclusive_scan(W.n_tris, wcap, W.s_tris, C.wscan_tmp, s);
+        k_wscan_local<<<WSCAN_BLOCKS, BLOCK, 0, s>>>(W);
+        k_wscan_top<<<1, WSCAN_BLOCKS, 0, s>>>(W);
         k_wide_emit<<<g_wide, 128, 0, s>>>(W);
         k_next_wide<<<1, 1, 0, s>>>(W);
     };
@@ -1894,6 +1913,7 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
         if (enqueued > 4096) return ctx->fail(APG_E_CAPACITY, "wide tree deeper than 4096 levels");
     }
     const uint32_t wide_nodes = rb[1], tri_cursor = rb[2], wlevels = rb[3];
+    if (tri_cursor == n) k_tri_gather<<<blocks_for(n, 256), 256, 0, s>>>(W, n);
     C.last_wide = wlevels;
     C.last_n = n;
     C.last_r = R;
