@@ -101,6 +101,7 @@ struct BuildState
     uint32_t* idx;      // slot -> triangle
     uint32_t* idx_tmp;  // partition staging
     uint8_t* flag8;     // per slot of the level's nodes: goes to the left child
+    uint16_t* sbin;     // per slot of the level's SAH candidates: bin on axis a in bits 4a..4a+3
     // nodes (capacity 2n)
     uint32_t* nbox;     // 6 ordered uints per node (bounds accumulators)
     uint32_t* ncbox;    // 6 ordered uints per node (centroid bounds)
@@ -407,11 +408,14 @@ __global__ __launch_bounds__(BLOCK) void k_bin_large(BuildState S)
         const float* b = S.pbox + 6 * size_t(S.idx[i]);
         float lo[3], hi[3];
         for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; }
+        uint32_t bins3 = 0;
         for (int d = 0; d < 3; ++d)
         {
             if (mn[d] == mx[d]) continue;
             const float c = (lo[d] + hi[d]) * 0.5f;
-            uint32_t* e = acc + (d * NBINS + bin_index(c, mn[d], mx[d])) * BIN_VALS;
+            const uint32_t bi = bin_index(c, mn[d], mx[d]);
+            bins3 |= bi << (4 * d);
+            uint32_t* e = acc + (d * NBINS + bi) * BIN_VALS;
             for (int a = 0; a < 3; ++a)
             {
                 atomicMin(&e[a], ord_of(lo[a]));
@@ -419,6 +423,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin_large(BuildState S)
             }
             atomicAdd(&e[6], 1u);
         }
+        S.sbin[i] = uint16_t(bins3);  // the partition's left test reads the split axis' bin
     }
     __syncthreads();
     uint32_t* g = S.bins + size_t(slot) * NODE_BIN_WORDS;
@@ -578,14 +583,15 @@ __global__ __launch_bounds__(256) void k_sweep(BuildState S, uint32_t thr)
 // ---------------------------------------------------------------------------
 // partition of the level's nodes, chunk by chunk (slots of other nodes are not touched)
 // ---------------------------------------------------------------------------
-// Whether slot i (triangle t) of split node nd goes to the left child.
-__device__ __forceinline__ bool goes_left(const BuildState& S, uint32_t nd, uint32_t first, uint32_t i, uint32_t t)
+// Whether slot i of split node nd goes to the left child: its bin on the split axis (recorded
+// by k_bin_large with the same centroid range) at most the split bin, or for a positional split
+// its place in the first half.
+__device__ __forceinline__ bool goes_left(const BuildState& S, uint32_t nd, uint32_t first, uint32_t i)
 {
     const uint32_t sp = S.nsplit[nd];
     if ((sp & 0xFFu) == 0xFFu) return (i - first) < __float_as_uint(S.nsplit_mm[2 * size_t(nd)]);
     const uint32_t d = sp & 0xFFu, si = sp >> 8;
-    const float* b = S.pbox + 6 * size_t(t);
-    return bin_index((b[d] + b[3 + d]) * 0.5f, S.nsplit_mm[2 * size_t(nd)], S.nsplit_mm[2 * size_t(nd) + 1]) <= si;
+    return ((uint32_t(S.sbin[i]) >> (4 * d)) & 15u) <= si;
 }
 
 // Per chunk: each slot's left flag (a byte in flag8) and the chunk's count of them.  Thread t
@@ -599,7 +605,7 @@ __global__ __launch_bounds__(BLOCK) void k_part_count(BuildState S)
     uint32_t cnt = 0;
     for (uint32_t i = lo + threadIdx.x; i < hi; i += BLOCK)
     {
-        const bool l = goes_left(S, node, first, i, S.idx[i]);
+        const bool l = goes_left(S, node, first, i);
         S.flag8[i] = l ? 1u : 0u;
         cnt += l ? 1u : 0u;
     }
@@ -1727,6 +1733,7 @@ struct BuildCache
         int rc;
         if ((rc = get(ctx, S.pbox, 6 * size_t(n))) || (rc = get(ctx, S.tri9_src, 9 * size_t(n))) ||
             (rc = get(ctx, S.idx, n)) || (rc = get(ctx, S.idx_tmp, n)) || (rc = get(ctx, S.flag8, n)) ||
+            (rc = get(ctx, S.sbin, n)) ||
             (rc = get(ctx, S.chunk_below, ccap)) || (rc = get(ctx, S.chunk_off, ccap)) ||
             (rc = get(ctx, S.nbox, 6 * ncap)) || (rc = get(ctx, S.ncbox, 6 * ncap)) || (rc = get(ctx, S.nfirst, ncap)) ||
             (rc = get(ctx, S.ncount, ncap)) || (rc = get(ctx, S.nleft, ncap)) || (rc = get(ctx, S.nflags, ncap)) ||
