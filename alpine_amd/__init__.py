@@ -50,6 +50,11 @@ class _BuildParams(C.Structure):
                 ("flags", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
 
+class FrameTransform(C.Structure):
+    _fields_ = [("shape_id", C.c_uint32), ("first", C.c_uint32), ("count", C.c_uint32),
+                ("m", C.c_float * 12)]
+
+
 ROOTS_SCENE, ROOTS_PER_SHAPE = 0, 1
 BUILD_NO_STATS = 1
 
@@ -154,13 +159,14 @@ ABI_SYMBOLS = [
     "apg_textures_set", "apg_render_read_aovs", "apg_render_set_environment",
     "apg_intersect_host", "apg_occluded_host", "apg_light_sampler_set",
     "apg_light_sampler_sample_host", "apg_trace_status", "apg_intersect_noalloc",
+    "apg_frame_render", "apg_frame_status",
 ]
 DEBUG_SYMBOLS = ["apg_debug_host_build", "apg_debug_scene_info", "apg_debug_scene_copy",
                  "apg_debug_host_trace", "apg_debug_scene_free", "apg_debug_material",
                  "apg_debug_light_sampler", "apg_debug_light", "apg_debug_shading",
                  "apg_debug_environment", "apg_debug_triangles", "apg_debug_host_trace_limited",
                  "apg_debug_scene_depth", "apg_debug_fail_allocations", "apg_debug_stream_read",
-                 "apg_debug_host_rate"]
+                 "apg_debug_host_rate", "apg_debug_frame_level_cut", "apg_debug_frame_counts"]
 
 
 def lib():
@@ -216,6 +222,10 @@ def lib():
         "apg_render_set_camera": (i32, [vp, C.POINTER(Camera)]),
         "apg_render_set_tiles": (i32, [vp, vp, vp, u32, u32, u32]),
         "apg_render": (i32, [vp, C.POINTER(RenderParams), vp]),
+        "apg_frame_render": (i32, [vp, vp, u32, C.POINTER(_BuildParams), C.POINTER(RenderParams), u32, vp]),
+        "apg_frame_status": (i32, [vp]),
+        "apg_debug_frame_level_cut": (i32, [vp, u32]),
+        "apg_debug_frame_counts": (i32, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
         "apg_render_read": (i32, [vp, vp, vp, C.POINTER(u32)]),
         "apg_render_read_aovs": (i32, [vp, vp, vp]),
         "apg_render_set_environment": (i32, [vp, vp, u32, u32]),
@@ -587,6 +597,38 @@ class GpuAccelerator:
                          train_steps, guide_warmup, guide_frac, 1 if graph else 0, train_samples,
                          guide_depth, 1 if aovs else 0, 1 if count_fetches else 0)
         self._rc(self._lib.apg_render(self._h, C.byref(p), self._stream(stream)))
+
+    def frame_render(self, transforms, spp, max_depth, rr_depth=0, seed=0, leaf_threshold=1,
+                     per_shape=False, stats=False, background=(0.0, 0.0, 0.0), fp64=False,
+                     stream=None):
+        """One animated frame (apg_frame_render): transforms = [(shape_id, first, count, m34)],
+        then the GPU rebuild, a reset at the current size with `seed` and `spp` samples; the
+        first frame runs eagerly, later ones replay one captured HIP graph."""
+        arr = (FrameTransform * max(1, len(transforms)))()
+        for i, (sid, first, count, m34) in enumerate(transforms):
+            arr[i].shape_id, arr[i].first, arr[i].count = int(sid), int(first), int(count)
+            arr[i].m = (C.c_float * 12)(*[float(x) for x in np.asarray(m34, np.float32).reshape(12)])
+        bp = _BuildParams(leaf_threshold, BUILDER_GPU, ROOTS_PER_SHAPE if per_shape else ROOTS_SCENE,
+                          0 if stats else BUILD_NO_STATS)
+        rp = RenderParams(spp, max_depth, rr_depth, 1 if fp64 else 0, _f3(background), 0, 4, 1, 0.5,
+                          0, 0, 0, 0, 0)
+        self._rc(self._lib.apg_frame_render(self._h, C.cast(arr, C.c_void_p), len(transforms),
+                                            C.byref(bp), C.byref(rp), seed, self._stream(stream)))
+
+    def frame_status(self) -> int:
+        """Waits for the last replayed frame: 0, or the APG_E_CAPACITY code when its rebuild ran
+        out of levels (that frame shows the previous tree)."""
+        return int(self._lib.apg_frame_status(self._h))
+
+    def frame_counts(self):
+        """(eager frames, captures, replays) of frame_render (test hook)."""
+        e, c, r = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._rc(self._lib.apg_debug_frame_counts(self._h, C.byref(e), C.byref(c), C.byref(r)))
+        return e.value, c.value, r.value
+
+    def frame_level_cut(self, cut):
+        """Test hook: captured frames run `cut` binary levels fewer than they need."""
+        self._rc(self._lib.apg_debug_frame_level_cut(self._h, int(cut)))
 
     def render_read_aovs(self):
         """Albedo and normal render-target sums (height, width, 3) of the aovs=True passes."""

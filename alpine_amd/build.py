@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["api.cpp", "host_build.cpp", "traverse.hip", "build.hip", "debug_host.hip",
-           "render_host.cpp", "render.hip", "guide.hip", "attributes.hip", "host_calls.cpp"]
+           "render_host.cpp", "render.hip", "guide.hip", "attributes.hip", "host_calls.cpp", "frame.hip"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",

@@ -60,6 +60,8 @@ struct DeviceScene
 
 struct BuildCache;  // build.hip: the GPU builder's device scratch, kept across builds
 void build_cache_free(BuildCache* c);
+struct FrameGraph;  // frame.hip: a captured frame (transforms, rebuild, render)
+void frame_graph_free(FrameGraph* f);
 
 // A stream with device and pinned host buffers for the host-memory entry points
 // (per-ray callers); the context keeps a pool, one per concurrently calling thread.
@@ -149,6 +151,11 @@ struct Context
     HostStaging* take_staging(size_t n, int& rc);
     void give_staging(HostStaging* s);
     HostCoalescer host_batches;
+
+    // scene buffers sized for any rebuild of the current triangles (graph-replayed frames)
+    bool scene_full_capacity = false;
+    bool capturing = false;  // apg_render enqueues into a frame graph being captured
+    FrameGraph* frame = nullptr;
 
     ~Context();
     int rebuild_shading();  // per-triangle normals after a scene build (render.hip)

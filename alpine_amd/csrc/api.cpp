@@ -22,6 +22,7 @@ int gather_vertex_triangles(Context* ctx);
 Context::~Context()
 {
     (void)hipSetDevice(device);
+    if (frame) frame_graph_free(frame);  // waits for its replays, which read what follows
     if (scene.nodes) (void)hipFree(scene.nodes);
     if (scene.tris) (void)hipFree(scene.tris);
     if (scene.tri_ids) (void)hipFree(scene.tri_ids);

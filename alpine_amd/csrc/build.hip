@@ -1264,6 +1264,16 @@ __global__ void k_finish_done(BuildState S)
     S.lvl[7] = S.counters[0] - S.counters[3];
 }
 
+// A graph-replayed build after its fixed number of levels: when the last level still allocated
+// children the tree is unfinished, so k_finish gets no subtrees and the wide collapse no level
+// (the scene keeps the previous tree); lvl[12] = 1 then.
+__global__ void k_build_gate(BuildState S)
+{
+    const bool done = S.lvl[1] == S.counters[0];
+    S.lvl[12] = done ? 0u : 1u;
+    if (!done) S.lvl[4] = 0u;
+}
+
 // ---------------------------------------------------------------------------
 // wide collapse
 // ---------------------------------------------------------------------------
@@ -1593,16 +1603,30 @@ __global__ __launch_bounds__(128) void k_wide_emit(WideState W)
     }
 }
 
-// The reordered triangles: position p holds the triangle of slot perm[p].
-__global__ void k_tri_gather(WideState W, uint32_t n)
+// The reordered triangles: position p holds the triangle of slot perm[p] (nothing when a
+// graph-replayed build's status word is set).
+__global__ void k_tri_gather(WideState W, uint32_t n, const uint32_t* fail)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    if (p >= n || (fail && *fail)) return;
     const uint32_t g = W.idx[W.perm[p]];
     const float* src = W.tri9_src + 9 * size_t(g);
     float* dst = W.tri9 + 9 * size_t(p);
     for (int j = 0; j < 9; ++j) dst[j] = src[j];
     W.tri_ids[p] = g;
+}
+
+// A graph-replayed build's status in lvl[13]: 0 a complete tree, 1 binary levels unfinished
+// (nothing replaced), 2 wide levels unfinished, 4 wide capacity exceeded.
+__global__ void k_build_status(uint32_t* lvl, const uint32_t* wl, uint32_t n)
+{
+    uint32_t st = lvl[12];
+    if (!st)
+    {
+        if (wl[0] != wl[1] || wl[2] != n) st |= 2u;
+        if (wl[4]) st |= 4u;
+    }
+    lvl[13] = st;
 }
 
 // The next wide level is the inner children placed by this one.
@@ -1619,15 +1643,17 @@ __global__ void k_next_wide(WideState W)
     W.wl[3] += 1u;
 }
 
-// Wide level 0: the roots' images (wide node r is the root of tree r).
-__global__ void k_init_wide(WideState W, uint32_t R)
+// Wide level 0: the roots' images (wide node r is the root of tree r); no level at all when a
+// graph-replayed build's binary levels did not finish (fail set).
+__global__ void k_init_wide(WideState W, uint32_t R, const uint32_t* fail)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < R) W.wbin[r] = r;
+    const bool skip = fail && *fail;
+    if (r < R && !skip) W.wbin[r] = r;
     if (r == 0)
     {
         W.wl[0] = 0u;
-        W.wl[1] = R;
+        W.wl[1] = skip ? 0u : R;
         W.wl[2] = 0u;
         W.wl[3] = 0u;
         W.wl[4] = 0u;
@@ -1797,6 +1823,50 @@ uint32_t ceil_log2(uint32_t n)
 
 } // namespace
 
+namespace {
+
+// One binary level of the level loop (every kernel reads the level's range on the device).
+void enqueue_binary_level(const BuildState& S, uint32_t thr, uint32_t g_node, uint32_t g_chunk, hipStream_t s)
+{
+    k_level_init<<<g_node, 256, 0, s>>>(S);
+    k_large_scan<<<1, SCAN1, 0, s>>>(S);
+    k_bounds_large<<<g_chunk, BLOCK, 0, s>>>(S);
+    k_bin_large<<<g_chunk, BLOCK, 0, s>>>(S);
+    k_sweep<<<g_node, 256, 0, s>>>(S, thr);
+    k_part_count<<<g_chunk, BLOCK, 0, s>>>(S);
+    k_part_scan<<<1, SCAN1, 0, s>>>(S);
+    k_part_scatter<<<g_chunk, BLOCK, 0, s>>>(S);
+    k_part_copy<<<g_chunk, BLOCK, 0, s>>>(S);
+}
+
+// One wide level of the collapse.
+void enqueue_wide_level(const WideState& W, uint32_t g_wide, hipStream_t s)
+{
+    k_wide_gather<<<g_wide, 128, 0, s>>>(W);
+    k_wscan_local<<<WSCAN_BLOCKS, BLOCK, 0, s>>>(W);
+    k_wscan_top<<<1, WSCAN_BLOCKS, 0, s>>>(W);
+    k_wide_emit<<<g_wide, 128, 0, s>>>(W);
+    k_next_wide<<<1, 1, 0, s>>>(W);
+}
+
+WideState wide_state(const BuildState& S, const BuildCache& C, const DeviceScene& sc, uint32_t wcap)
+{
+    WideState W;
+    W.nbox = S.nbox; W.nfirst = S.nfirst; W.ncount = S.ncount; W.nleft = S.nleft;
+    W.nflags = S.nflags; W.idx = S.idx; W.tri9_src = S.tri9_src;
+    W.wbin = C.wbin; W.wchild = C.wchild; W.n_inner = C.n_inner; W.n_tris = C.n_tris;
+    W.s_inner = C.s_inner; W.s_tris = C.s_tris;
+    W.nodes = sc.nodes; W.tri9 = sc.tris; W.tri_ids = sc.tri_ids;
+    W.slack = C.slack; W.wl = C.wl; W.wcap = wcap;
+    W.wpart = C.wpart; W.perm = S.idx_tmp;  // idx_tmp is free once the binary tree is built
+    return W;
+}
+
+uint32_t fin_grid(uint32_t n) { return std::min<uint32_t>(blocks_for(n, 256), 3u * 256u); }
+uint32_t node_grid(uint32_t n) { return std::min<uint32_t>(blocks_for(n, 256), 2048u); }
+
+} // namespace
+
 int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool stats, BinaryTree& bt)
 {
     const auto t_start = std::chrono::steady_clock::now();
@@ -1839,21 +1909,11 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
     k_init_roots<<<blocks_for(R, 256), 256, 0, s>>>(S, C.rfirst, R);
 
     // ---- binary levels: enqueued in batches, the level range read back between batches ----
-    const uint32_t g_node = std::min<uint32_t>(blocks_for(n, 256), 2048u);
+    const uint32_t g_node = node_grid(n);
     const uint32_t g_chunk = C.chunk_cap;
-    const uint32_t g_fin = std::min<uint32_t>(blocks_for(n, 256), 3u * 256u);
+    const uint32_t g_fin = fin_grid(n);
     uint32_t* rb = C.rb;
-    auto enqueue_level = [&]() {
-        k_level_init<<<g_node, 256, 0, s>>>(S);
-        k_large_scan<<<1, SCAN1, 0, s>>>(S);
-        k_bounds_large<<<g_chunk, BLOCK, 0, s>>>(S);
-        k_bin_large<<<g_chunk, BLOCK, 0, s>>>(S);
-        k_sweep<<<g_node, 256, 0, s>>>(S, thr);
-        k_part_count<<<g_chunk, BLOCK, 0, s>>>(S);
-        k_part_scan<<<1, SCAN1, 0, s>>>(S);
-        k_part_scatter<<<g_chunk, BLOCK, 0, s>>>(S);
-        k_part_copy<<<g_chunk, BLOCK, 0, s>>>(S);
-    };
+    auto enqueue_level = [&]() { enqueue_binary_level(S, thr, g_node, g_chunk, s); };
     // the level loop ends where every node holds at most FIN slots, a little deeper than
     // log2(n / FIN) levels; a rebuild of the same triangle count starts with the levels the
     // previous build took
@@ -1885,26 +1945,15 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
     k_fill_u32<<<1, 64, 0, s>>>(C.slack_bits, 1, 0u);
     k_scene_slack<<<256, 256, 0, s>>>(S.pbox, n, C.slack_bits);
     k_slack_final<<<1, 1, 0, s>>>(C.slack_bits, C.slack);
-    const uint32_t wcap = bin_nodes / 2 + 2 + R;  // each wide node holds >= 2 binary nodes or is a root
+    // each wide node holds >= 2 binary nodes or is a root; the full capacity when the scene
+    // buffers must hold any rebuild (graph-replayed frames)
+    const uint32_t wcap = ctx->scene_full_capacity ? C.wide_cap : bin_nodes / 2 + 2 + R;
     if (wcap > C.wide_cap) return ctx->fail(APG_E_CAPACITY, "wide node capacity exceeded");
     if ((rc = ensure_scene_buffers(ctx, wcap, n))) return rc;
-    WideState W;
-    W.nbox = S.nbox; W.nfirst = S.nfirst; W.ncount = S.ncount; W.nleft = S.nleft;
-    W.nflags = S.nflags; W.idx = S.idx; W.tri9_src = S.tri9_src;
-    W.wbin = C.wbin; W.wchild = C.wchild; W.n_inner = C.n_inner; W.n_tris = C.n_tris;
-    W.s_inner = C.s_inner; W.s_tris = C.s_tris;
-    W.nodes = sc.nodes; W.tri9 = sc.tris; W.tri_ids = sc.tri_ids;
-    W.slack = C.slack; W.wl = C.wl; W.wcap = wcap;
-    W.wpart = C.wpart; W.perm = S.idx_tmp;  // idx_tmp is free once the binary tree is built
-    k_init_wide<<<blocks_for(R, 256), 256, 0, s>>>(W, R);
+    const WideState W = wide_state(S, C, sc, wcap);
+    k_init_wide<<<blocks_for(R, 256), 256, 0, s>>>(W, R, nullptr);
     const uint32_t g_wide = std::min<uint32_t>(blocks_for(wcap, 128), 2048u);
-    auto enqueue_wide = [&]() {
-        k_wide_gather<<<g_wide, 128, 0, s>>>(W);
-        k_wscan_local<<<WSCAN_BLOCKS, BLOCK, 0, s>>>(W);
-        k_wscan_top<<<1, WSCAN_BLOCKS, 0, s>>>(W);
-        k_wide_emit<<<g_wide, 128, 0, s>>>(W);
-        k_next_wide<<<1, 1, 0, s>>>(W);
-    };
+    auto enqueue_wide = [&]() { enqueue_wide_level(W, g_wide, s); };
     enqueued = 0;
     for (;;)
     {
@@ -1920,7 +1969,7 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
         if (enqueued > 4096) return ctx->fail(APG_E_CAPACITY, "wide tree deeper than 4096 levels");
     }
     const uint32_t wide_nodes = rb[1], tri_cursor = rb[2], wlevels = rb[3];
-    if (tri_cursor == n) k_tri_gather<<<blocks_for(n, 256), 256, 0, s>>>(W, n);
+    if (tri_cursor == n) k_tri_gather<<<blocks_for(n, 256), 256, 0, s>>>(W, n, nullptr);
     C.last_wide = wlevels;
     C.last_n = n;
     C.last_r = R;
@@ -2004,8 +2053,48 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
     return APG_OK;
 }
 
-// After a build: the vertex-form triangles from the vertices the tree was built from.
-
+// A rebuild for capture into a HIP graph: the geometry on the device rebuilt with no host
+// read-back, running the previous eager build's binary and wide level counts plus `extra` of
+// each (less `cut` binary levels: a test hook).  *status (device) is the build's status word: 0 a complete tree; otherwise the scene
+// keeps the previous tree (binary levels unfinished) or holds an incomplete one (wide levels
+// unfinished) and must be rebuilt eagerly.  Needs a previous eager build of the same triangles
+// and trees with the scene buffers at full capacity (scene_full_capacity).
+int build_scene_gpu_enqueue(Context* ctx, uint32_t leaf_threshold, bool per_shape, uint32_t extra, uint32_t cut,
+                            hipStream_t s, uint32_t** status)
+{
+    const uint32_t n = ctx->total_tris;
+    const uint32_t R = per_shape ? uint32_t(ctx->shapes.size()) : 1u;
+    BuildCache* Cp = ctx->build_cache;
+    if (n == 0 || !Cp || Cp->last_n != n || Cp->last_r != R || !Cp->last_levels || !Cp->last_wide)
+        return ctx->fail(APG_E_STATE, "a replayed build needs an eager GPU build of the same scene first");
+    const BuildCache& C = *Cp;
+    const DeviceScene& sc = ctx->scene;
+    if (!sc.nodes || sc.node_cap < C.wide_cap || sc.tri_cap < n)
+        return ctx->fail(APG_E_STATE, "the scene buffers are not sized for replayed builds");
+    const BuildState S = C.S;
+    const uint32_t thr = std::max<uint32_t>(1, leaf_threshold);
+    k_prep<<<blocks_for(n, 256), 256, 0, s>>>(ctx->d_xyz, ctx->d_tri, n, S.tri9_src, S.pbox);
+    k_init_slots<<<blocks_for(n, 256), 256, 0, s>>>(n, S.idx);
+    k_init_roots<<<blocks_for(R, 256), 256, 0, s>>>(S, C.rfirst, R);
+    const uint32_t g_node = node_grid(n), g_chunk = C.chunk_cap;
+    const uint32_t levels = C.last_levels + extra > cut ? C.last_levels + extra - cut : 0u;
+    for (uint32_t l = 0; l < levels; ++l) enqueue_binary_level(S, thr, g_node, g_chunk, s);
+    k_build_gate<<<1, 1, 0, s>>>(S);
+    k_finish<<<fin_grid(n), FIN_BLOCK, 0, s>>>(S, thr);
+    k_finish_done<<<1, 1, 0, s>>>(S);
+    k_fill_u32<<<1, 64, 0, s>>>(C.slack_bits, 1, 0u);
+    k_scene_slack<<<256, 256, 0, s>>>(S.pbox, n, C.slack_bits);
+    k_slack_final<<<1, 1, 0, s>>>(C.slack_bits, C.slack);
+    const WideState W = wide_state(S, C, sc, C.wide_cap);
+    k_init_wide<<<blocks_for(R, 256), 256, 0, s>>>(W, R, S.lvl + 12);
+    const uint32_t g_wide = std::min<uint32_t>(blocks_for(C.wide_cap, 128), 2048u);
+    for (uint32_t l = 0; l < C.last_wide + extra; ++l) enqueue_wide_level(W, g_wide, s);
+    k_build_status<<<1, 1, 0, s>>>(S.lvl, C.wl, n);
+    k_tri_gather<<<blocks_for(n, 256), 256, 0, s>>>(W, n, S.lvl + 13);
+    APG_HIP_CHECK(ctx, hipGetLastError());
+    *status = S.lvl + 13;
+    return APG_OK;
+}
 
 // The vertex-form triangles from the vertices the tree was built from, which the watertight
 // test reads: gathered after a build when that test is selected, or when it is selected later

@@ -316,6 +316,27 @@ __global__ void k_reset_pixels(uint32_t P, uint32_t NPIX, uint32_t seed, uint32_
     }
 }
 
+// As k_reset_pixels with the seed read from device memory (a frame graph's per-frame seed).
+__global__ void k_reset_pixels_dev(uint32_t P, uint32_t NPIX, const uint32_t* seed, uint32_t* rng, float* accum,
+                                   double* accum64)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    uint32_t s[4];
+    sampler_reset(s, slot_seed(p % NPIX, p / NPIX, *seed));
+    for (int k = 0; k < 4; ++k) rng[4 * size_t(p) + k] = s[k];
+    for (int k = 0; k < 3; ++k)
+    {
+        accum[3 * size_t(p) + k] = 0.0f;
+        accum64[3 * size_t(p) + k] = 0.0;
+    }
+}
+
+__global__ void k_zero_words(uint32_t* p, size_t n)
+{
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = 0u;
+}
+
 // Per-pixel sums over the slots, in slot order.
 template <typename T>
 __global__ void k_sum_slots(const T* acc, uint32_t NPIX, uint32_t S, T* out)
@@ -896,6 +917,44 @@ int Context::rebuild_shading()
     return APG_OK;
 }
 
+// Frame graphs (frame.hip): apg_render_reset's clears with the seed read from device memory,
+// and the per-triangle normals of a scene build, as kernels on `s` only (capturable).
+int render_reset_enqueue(Context* ctx, const uint32_t* seed_dev, hipStream_t s)
+{
+    RenderState* R = state_of(ctx);
+    if (!R->P) return ctx->fail(APG_E_STATE, "apg_render_reset has not been called");
+    k_reset_pixels_dev<<<nblocks(R->P, 256), 256, 0, s>>>(R->P, R->NPIX, seed_dev, R->rng, R->accum, R->accum64);
+    k_zero_words<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t*>(R->stats), 16);
+    if (R->aov_albedo)
+    {
+        const size_t words = 3 * size_t(R->aov_P);
+        k_zero_words<<<256, 256, 0, s>>>(reinterpret_cast<uint32_t*>(R->aov_albedo), words);
+        k_zero_words<<<256, 256, 0, s>>>(reinterpret_cast<uint32_t*>(R->aov_normal), words);
+    }
+    APG_HIP_CHECK(ctx, hipGetLastError());
+    return APG_OK;
+}
+
+int render_tri_ng_enqueue(Context* ctx, hipStream_t s)
+{
+    RenderState* R = state_of(ctx);
+    const uint32_t n = ctx->scene.tri_count;
+    if (R->tri_ng_n != ctx->total_tris) return ctx->fail(APG_E_STATE, "no triangle normals for this scene");
+    if (n) k_tri_ng<<<nblocks(n, 256), 256, 0, s>>>(ctx->scene.tris, ctx->scene.tri_ids, n, R->tri_ng);
+    APG_HIP_CHECK(ctx, hipGetLastError());
+    return APG_OK;
+}
+
+uint32_t render_frame_spp(Context* ctx) { return state_of(ctx)->total_spp; }
+void render_set_total_spp(Context* ctx, uint32_t spp) { state_of(ctx)->total_spp = spp; }
+void render_size(Context* ctx, uint32_t& W, uint32_t& H, uint32_t& S)
+{
+    RenderState* R = state_of(ctx);
+    W = R->W;
+    H = R->H;
+    S = R->S;
+}
+
 namespace {
 
 // Pools of the shapes' shading attributes in global triangle order (rebuilt when a shape's
@@ -1139,6 +1198,67 @@ int install_lights(Context* ctx, const std::vector<DevLight>& dl, uint32_t sampl
     R->ls_kind = sampler;
     R->ls_set = true;
     return APG_OK;
+}
+
+// The stage kernels' arguments of a pass (no allocation: guide buffers exist already).
+void fill_shade_args(RenderState* R, const apg_render_params* prm, ShadeArgs& S)
+{
+    std::memset(&S, 0, sizeof(S));  // padding included: the graph key compares bytes
+    S.mats = R->mats;
+    S.tri_mat = R->tri_mat;
+    S.tri_ng = R->tri_ng;
+    S.ls = view_of(R);
+    S.has_ls = R->ls_set;
+    S.rng = R->rng;
+    S.T = R->T;
+    S.L = R->L;
+    S.state = R->state;
+    S.nee = R->nee;
+    S.vis = R->vis;
+    S.shadow = R->qs;
+    S.hprim = R->hprim;
+    S.ht = R->ht;
+    S.hb0 = R->hb0;
+    S.hb1 = R->hb1;
+    S.counts = R->counts;
+    S.max_depth = prm->max_depth;
+    S.rr_depth = prm->rr_depth;
+    for (int k = 0; k < 3; ++k) S.bg[k] = prm->background[k];
+    S.attr = R->attr;
+    S.env = R->env;
+    S.env_w = R->env_w;
+    S.env_h = R->env_h;
+    S.aov_albedo = prm->aovs ? R->aov_albedo : nullptr;
+    S.aov_normal = prm->aovs ? R->aov_normal : nullptr;
+    S.gout = R->gout;
+    S.vrec = R->vrec;
+    S.vrec_depth = R->vrec_depth;
+    for (int a = 0; a < 3; ++a)
+    {
+        S.lo[a] = R->scene_lo[a];
+        const float ext = R->scene_hi[a] - R->scene_lo[a];
+        S.inv[a] = ext > 0.0f ? 1.0f / ext : 0.0f;
+    }
+}
+
+// What a captured frame's passes baked in (frame.hip compares it before each replay): the
+// stage arguments, camera, tiles, launch shapes and buffers.
+void render_frame_key(Context* ctx, const apg_render_params* prm, std::vector<char>& key)
+{
+    RenderState* R = state_of(ctx);
+    ShadeArgs S;
+    fill_shade_args(R, prm, S);
+    auto put = [&key](const void* p, size_t n) {
+        key.insert(key.end(), static_cast<const char*>(p), static_cast<const char*>(p) + n);
+    };
+    put(&S, sizeof(S));
+    put(&R->cam, sizeof(R->cam));
+    put(&ctx->rlaunch, sizeof(ctx->rlaunch));
+    const uintptr_t extra[] = { R->P, R->W, R->H, R->S, uintptr_t(R->accum), uintptr_t(R->accum64), uintptr_t(R->stats),
+                                uintptr_t(R->tile_cams), uintptr_t(R->tile_roots), R->ntiles, R->tile_w, R->tile_h,
+                                uintptr_t(R->gfeat), uintptr_t(R->aov_albedo), R->aov_P, uintptr_t(ctx->spill),
+                                ctx->spill_threads, R->tri_ng_n };
+    put(extra, sizeof(extra));
 }
 
 } // namespace apg
@@ -1416,34 +1536,6 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
         R->aov_P = P;
     }
 
-    ShadeArgs S;
-    std::memset(&S, 0, sizeof(S));  // padding included: the graph key compares bytes
-    S.mats = R->mats;
-    S.tri_mat = R->tri_mat;
-    S.tri_ng = R->tri_ng;
-    S.ls = view_of(R);
-    S.has_ls = R->ls_set;
-    S.rng = R->rng;
-    S.T = R->T;
-    S.L = R->L;
-    S.state = R->state;
-    S.nee = R->nee;
-    S.vis = R->vis;
-    S.shadow = R->qs;
-    S.hprim = R->hprim;
-    S.ht = R->ht;
-    S.hb0 = R->hb0;
-    S.hb1 = R->hb1;
-    S.counts = R->counts;
-    S.max_depth = prm->max_depth;
-    S.rr_depth = prm->rr_depth;
-    for (int k = 0; k < 3; ++k) S.bg[k] = prm->background[k];
-    S.attr = R->attr;
-    S.env = R->env;
-    S.env_w = R->env_w;
-    S.env_h = R->env_h;
-    S.aov_albedo = prm->aovs ? R->aov_albedo : nullptr;
-    S.aov_normal = prm->aovs ? R->aov_normal : nullptr;
     const uint32_t guide = prm->guide;
     if (guide > 2) return ctx->fail(APG_E_INVALID, "guide must be 0, 1 or 2");
     if (guide)
@@ -1451,15 +1543,8 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
         int rc = ensure_guide_buffers(ctx, R, prm->max_depth, prm->train_samples);
         if (rc) return rc;
     }
-    S.gout = R->gout;
-    S.vrec = R->vrec;
-    S.vrec_depth = R->vrec_depth;
-    for (int a = 0; a < 3; ++a)
-    {
-        S.lo[a] = R->scene_lo[a];
-        const float ext = R->scene_hi[a] - R->scene_lo[a];
-        S.inv[a] = ext > 0.0f ? 1.0f / ext : 0.0f;
-    }
+    ShadeArgs S;
+    fill_shade_args(R, prm, S);
     const float frac = prm->guide_frac > 0.0f ? std::min(prm->guide_frac, 0.95f) : 0.5f;
     const uint32_t train_steps = prm->train_steps ? prm->train_steps : 4u;
     const apg_render_launch& rl = ctx->rlaunch;
@@ -1469,7 +1554,8 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
 
     std::lock_guard<std::mutex> lock(ctx->launch_mutex);
     if (!ctx->last_launch) APG_HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->last_launch, hipEventDisableTiming));
-    APG_HIP_CHECK(ctx, hipStreamWaitEvent(stream, ctx->last_launch, 0));
+    // a frame graph being captured is ordered after earlier work by its launch
+    if (!ctx->capturing) APG_HIP_CHECK(ctx, hipStreamWaitEvent(stream, ctx->last_launch, 0));
     if (prm->spp % R->S) return ctx->fail(APG_E_INVALID, "spp must be a multiple of the slots per pixel");
     Tiles tl;
     std::memset(&tl, 0, sizeof(tl));
@@ -1628,7 +1714,8 @@ int apg_render(apg_context* ctx, const apg_render_params* prm, void* stream_)
         }
     }
     APG_HIP_CHECK(ctx, hipGetLastError());
-    APG_HIP_CHECK(ctx, hipEventRecord(ctx->last_launch, stream));
+    // a frame graph being captured records its own completion after each replay
+    if (!ctx->capturing) APG_HIP_CHECK(ctx, hipEventRecord(ctx->last_launch, stream));
     R->total_spp += prm->spp;
     return APG_OK;
 }

@@ -383,6 +383,30 @@ int apg_render_fetch_counts(const apg_context* ctx, uint64_t* closest_nodes, uin
 /* Guided rendering: training samples of the last pass and the last training loss (host). */
 int apg_render_guide_stats(const apg_context* ctx, uint32_t* train_samples, float* loss);
 
+/* ---- animated frames as one HIP graph (no reference counterpart: the reference's frame is
+ * Alpine::updateScene then render, alpine.cpp:203-212 and 271-288, host-driven) ----
+ * One frame: apg_mesh_transform of each listed vertex range, apg_scene_build with `build` (GPU
+ * builder), apg_render_reset_slots at the current size and slots with `seed`, and apg_render
+ * with `render`.  The first frame runs eagerly through those entry points; the next one is
+ * captured into a HIP graph, which later frames replay (the matrices and the seed may change
+ * every frame; a change of anything else the graph baked in, such as the transform list's
+ * ranges, the parameters, camera, tiles, materials or launch shapes, makes that frame eager and
+ * the next one a new capture).  A replayed rebuild reads nothing back: it runs the last eager
+ * build's binary and wide level counts plus a margin; apg_frame_status reports a replay whose
+ * tree needed more levels, and the frame after it is built eagerly.  apg_scene_get_stats keeps
+ * the last eager build's figures.  Not with guide = 1 or the watertight triangle test. */
+typedef struct apg_frame_transform {
+    uint32_t shape_id;
+    uint32_t first, count;     /* vertex range of the shape */
+    float m[12];               /* row-major 3 x 4, as apg_mesh_transform */
+} apg_frame_transform;
+int apg_frame_render(apg_context* ctx, const apg_frame_transform* xf, uint32_t nxf,
+                     const apg_build_params* build, const apg_render_params* render, uint32_t seed,
+                     void* stream);
+/* Waits for the last replayed frame: APG_OK, or APG_E_CAPACITY when its rebuild ran out of
+ * levels (that frame's image shows the previous tree). */
+int apg_frame_status(apg_context* ctx);
+
 /* ---- learned path guiding (no reference counterpart) ----
  * An MLP (96 -> 4 x 64 -> 48, BF16 MFMA, FP32 accumulation) maps a shading point to an
  * 8-lobe von Mises-Fisher mixture over directions.  Features: 96 floats per query

@@ -54,6 +54,12 @@ int apg_debug_host_rate(apg_context* ctx, const float* rays6, uint32_t n, uint32
 /* Render-state device allocations of ctx (render.hip) fail with APG_E_HIP after `after` more
  * have succeeded; after < 0 restores normal allocation.  For tests of the failure paths. */
 int apg_debug_fail_allocations(apg_context* ctx, int after);
+/* Replayed frame graphs (apg_frame_render) captured from now on run `cut` binary levels fewer
+ * than the last eager build needed (0 restores the margin): their rebuild stops short, every
+ * kernel still ends normally, and apg_frame_status reports it.  For tests of that path. */
+int apg_debug_frame_level_cut(apg_context* ctx, uint32_t cut);
+/* Frames apg_frame_render ran eagerly, graphs it captured and frames it replayed. */
+int apg_debug_frame_counts(const apg_context* ctx, uint32_t* eager, uint32_t* captures, uint32_t* replays);
 /* Levels of 8-wide nodes of the scene's deepest root-to-leaf path. */
 int apg_debug_scene_depth(const apg_debug_scene* s, uint32_t* wide_depth);
 void apg_debug_scene_free(apg_debug_scene* s);
