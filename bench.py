@@ -39,6 +39,7 @@ LEAF_THRESHOLD = 1
 # the animated config rebuilds every frame: leaf threshold 2 builds its forest 7 ms faster for
 # the same render time
 BATCH_LEAF_THRESHOLD = 2
+EAGER_FRAMES = 5  # config 4 frames run stage by stage for the rebuild / render split
 
 
 def log(msg):
@@ -440,8 +441,9 @@ def batched_benchmark(frames=60, spp=32, max_depth=5, rr_depth=3, seed=1, tune=T
     """Many independent scenes, animated, one tiled frame: 32 scenes of 10 K - 2 M triangles
     (log-uniform, about 12 M in all) in a per-shape forest, 512 x 512 tiles, 32 spp per frame.
     Each frame turns the last 25 % of every scene's triangles about y on the GPU, rebuilds all
-    32 trees on the GPU in one forest build, and renders every tile in the same launches (the
-    pass replayed from a HIP graph)."""
+    32 trees on the GPU in one forest build, and renders every tile in the same launches.  A few
+    frames run stage by stage (the rebuild / render split); the stated `frames` frames run as
+    one HIP graph each (apg_frame_render: transforms, rebuild, reset and render replayed)."""
     import torch
     import alpine_amd as aa
     from alpine_amd import scenes
@@ -465,12 +467,14 @@ def batched_benchmark(frames=60, spp=32, max_depth=5, rr_depth=3, seed=1, tune=T
         from alpine_amd import autotune
         launch, _ = autotune.autotune_render(acc, W, H, 4, max_depth, rr_depth, log=log, graph=True)
     rot = B.rotation(2.0 * np.pi / 120.0)
+    xf = [(k, first, count, rot) for k, (first, count) in enumerate(B.anim_range)]
+    # the stage split, from eager frames: transforms + rebuild, then reset + render
     builds, renders = [], []
-    for f in range(frames):
+    for f in range(EAGER_FRAMES):
         torch.cuda.synchronize()
         a = time.perf_counter()
-        for k, (first, count) in enumerate(B.anim_range):
-            acc.transform_vertices(k, rot, first=first, count=count)
+        for k, first, count, m in xf:
+            acc.transform_vertices(k, m, first=first, count=count)
         acc.update_scene(leaf_threshold=BATCH_LEAF_THRESHOLD, per_shape=True, stats=False)
         torch.cuda.synchronize()
         b = time.perf_counter()
@@ -481,20 +485,49 @@ def batched_benchmark(frames=60, spp=32, max_depth=5, rr_depth=3, seed=1, tune=T
         builds.append(b - a)
         renders.append(c - b)
     cnt = acc.render_counters()
+    # the stated frame loop: every frame (transforms, rebuild, reset, render) one HIP graph
+    frame_s, short = [], 0
+    for f in range(frames):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        acc.frame_render(xf, spp, max_depth, rr_depth, seed=seed + EAGER_FRAMES + f,
+                         leaf_threshold=BATCH_LEAF_THRESHOLD, per_shape=True)
+        short += acc.frame_status() != 0
+        frame_s.append(time.perf_counter() - a)
+    eager_frames, captures, replays = acc.frame_counts()
     img, _, total = acc.render_read()
     # convergence of the last frame: a BATCH_REF_SPP FP64 reference of the same geometry
     acc.render_reset(W, H, 5000)
     acc.render(BATCH_REF_SPP, max_depth, rr_depth, fp64=True, graph=True)
     _, ref64, _ = acc.render_read(fp64=True)
     rel = scenes.rel_mse(img / total, ref64 / BATCH_REF_SPP)
+    # stage-by-stage frames again after the graph loop (clock drift over the run shows here)
+    after = []
+    for f in range(EAGER_FRAMES):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for k, first, count, m in xf:
+            acc.transform_vertices(k, m, first=first, count=count)
+        acc.update_scene(leaf_threshold=BATCH_LEAF_THRESHOLD, per_shape=True, stats=False)
+        acc.render_reset(W, H, seed + 1000 + f)
+        acc.render(spp, max_depth, rr_depth, graph=True)
+        torch.cuda.synchronize()
+        after.append(time.perf_counter() - a)
     r = float(np.median(renders))
+    fr = float(np.median(frame_s[1:]))  # the first graph frame is the eager one
     return {
         "scenes": len(B.meshes), "triangles": n_tris, "frame": [W, H], "spp": spp,
         "frames": frames, "max_depth": max_depth, "rr_depth": rr_depth,
         "generate_s": round(t_gen, 2), "first_build_upload_ms": round(t_first * 1e3, 1),
         "rebuild_ms_median": round(float(np.median(builds)) * 1e3, 1),
         "render_ms_median": round(r * 1e3, 1),
-        "frame_ms_median": round(float(np.median(np.array(builds) + np.array(renders))) * 1e3, 1),
+        "frame_ms_median_eager": round(float(np.median(np.array(builds) + np.array(renders))) * 1e3, 1),
+        "frame_ms_median_eager_after": round(float(np.median(after)) * 1e3, 1),
+        "frame_ms_median": round(fr * 1e3, 1),
+        "frame_graph": {"frames": frames, "eager": eager_frames, "captures": captures, "replays": replays,
+                        "short_rebuilds": int(short), "frame_ms_median": round(fr * 1e3, 1),
+                        "frames_per_s": round(1.0 / fr, 2),
+                        "samples_per_s": round(W * H * spp / fr, 1)},
         "samples_per_s": round(W * H * spp / r, 1),
         "rays_per_s": round((cnt["closest_rays"] + cnt["shadow_rays"]) / renders[-1], 1),
         "rebuild_mtris_per_s": round(n_tris / float(np.median(builds)) / 1e6, 2),
