@@ -39,6 +39,8 @@ constexpr uint32_t APG_SPILL_ALLOC = 56;  // spill entries allocated per thread 
 // Triangle-group queue entries per wave: room for 64 new groups on top of the
 // 32 a wave may hold between rounds.
 constexpr uint32_t TRACE_QCAP = 96;
+// Node visits per lane between the wave's fetch / triangle-round / finish steps.
+constexpr uint32_t TRACE_VISITS = 4;
 
 // LDS words per wave: the stack's top (2 words per entry and lane), 64 keys of
 // 2 words, 64 barycentric pairs, 64 pending counts and the triangle-group queue
@@ -205,64 +207,6 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
             continue;
         }
 
-        // ---- one inner node per lane ----
-        uint32_t t_base = 0, t_mask = 0, t_cmask = 0;
-        if (active && (g_bits >> 8) != 0u)
-        {
-            uint32_t hits = 0;
-            const uint32_t node_idx = group_next_node(g_base, g_bits, rb.oct, hits);
-            if (hits)
-            {
-                // push the remaining siblings
-                if (sp >= A.stack_max)
-                {
-                    // dropped: the host reports APG_E_OVERFLOW for this launch's results
-                    A.counters[1] = 1u;
-                    hits = 0;
-                }
-                else if (sp >= APG_STACK_LDS)
-                {
-                    const uint32_t deep = sp - APG_STACK_LDS;  // < APG_STACK_SPILL
-                    const uint32_t ring = (sp % APG_STACK_LDS) * 64u + lane;
-                    spill[deep] = (uint64_t(st_bits[ring]) << 32) | st_base[ring];
-                }
-                if (hits)
-                {
-                    const uint32_t ring = (sp % APG_STACK_LDS) * 64u + lane;
-                    st_base[ring] = g_base;
-                    st_bits[ring] = (hits << 8) | (g_bits & 0xffu);
-                    sp++;
-                }
-            }
-            const uint4* np = reinterpret_cast<const uint4*>(A.nodes + node_idx);
-            const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
-            const uint32_t w[16] = { w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w,
-                                     w2.x, w2.y, w2.z, w2.w, w3.x, w3.y, w3.z, w3.w };
-            if (COUNT) n_node++;
-            node_visit(w, rb, tmax, g_base, g_bits, t_base, t_mask, t_cmask);
-        }
-
-        // ---- queue this iteration's triangle groups ----
-        {
-            const uint32_t cnt = bit_popc(t_mask);
-            const uint64_t m = __ballot(cnt != 0u);
-            if (m)
-            {
-                if (cnt)
-                {
-                    const uint32_t pos = __popcll(m & ((1ull << lane) - 1ull));
-                    const uint32_t slot = queue_slot(q_head, q_count + pos);
-                    queue[3u * slot] = t_base;
-                    queue[3u * slot + 1u] = t_cmask;
-                    queue[3u * slot + 2u] = t_mask;
-                    q_owner[slot] = uint8_t(owner);
-                    atomicAdd(&pend[owner], cnt);
-                }
-                q_count += uint32_t(__popcll(m));
-                q_items += uint32_t(__builtin_amdgcn_readlane(int(wave_incl_scan(cnt)), 63));
-            }
-        }
-
         // ---- next node group from the stack ----
         auto pop = [&](uint32_t& b, uint32_t& bits) {
             sp--;
@@ -276,7 +220,73 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
                 st_bits[ring] = uint32_t(e >> 32);
             }
         };
-        if (active && (g_bits >> 8) == 0u && sp != 0u) pop(g_base, g_bits);
+        // ---- up to VISITS inner nodes per lane and iteration ----
+        // Each visit: one node per lane, its triangle groups queued, the next node group
+        // popped.  A further visit runs while the queue has room for another 64 groups and a
+        // lane has node work; the fetch, triangle rounds and finish steps below then run once
+        // for all of them.
+        for (uint32_t visit = 0; visit < TRACE_VISITS; ++visit)
+        {
+            if (visit && (q_count > TRACE_QCAP - 64u || __ballot(active && (g_bits >> 8) != 0u) == 0ull)) break;
+            // ---- one inner node per lane ----
+            uint32_t t_base = 0, t_mask = 0, t_cmask = 0;
+            if (active && (g_bits >> 8) != 0u)
+            {
+                uint32_t hits = 0;
+                const uint32_t node_idx = group_next_node(g_base, g_bits, rb.oct, hits);
+                if (hits)
+                {
+                    // push the remaining siblings
+                    if (sp >= A.stack_max)
+                    {
+                        // dropped: the host reports APG_E_OVERFLOW for this launch's results
+                        A.counters[1] = 1u;
+                        hits = 0;
+                    }
+                    else if (sp >= APG_STACK_LDS)
+                    {
+                        const uint32_t deep = sp - APG_STACK_LDS;  // < APG_STACK_SPILL
+                        const uint32_t ring = (sp % APG_STACK_LDS) * 64u + lane;
+                        spill[deep] = (uint64_t(st_bits[ring]) << 32) | st_base[ring];
+                    }
+                    if (hits)
+                    {
+                        const uint32_t ring = (sp % APG_STACK_LDS) * 64u + lane;
+                        st_base[ring] = g_base;
+                        st_bits[ring] = (hits << 8) | (g_bits & 0xffu);
+                        sp++;
+                    }
+                }
+                const uint4* np = reinterpret_cast<const uint4*>(A.nodes + node_idx);
+                const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+                const uint32_t w[16] = { w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w,
+                                         w2.x, w2.y, w2.z, w2.w, w3.x, w3.y, w3.z, w3.w };
+                if (COUNT) n_node++;
+                node_visit(w, rb, tmax, g_base, g_bits, t_base, t_mask, t_cmask);
+            }
+
+            // ---- queue this iteration's triangle groups ----
+            {
+                const uint32_t cnt = bit_popc(t_mask);
+                const uint64_t m = __ballot(cnt != 0u);
+                if (m)
+                {
+                    if (cnt)
+                    {
+                        const uint32_t pos = __popcll(m & ((1ull << lane) - 1ull));
+                        const uint32_t slot = queue_slot(q_head, q_count + pos);
+                        queue[3u * slot] = t_base;
+                        queue[3u * slot + 1u] = t_cmask;
+                        queue[3u * slot + 2u] = t_mask;
+                        q_owner[slot] = uint8_t(owner);
+                        atomicAdd(&pend[owner], cnt);
+                    }
+                    q_count += uint32_t(__popcll(m));
+                    q_items += uint32_t(__builtin_amdgcn_readlane(int(wave_incl_scan(cnt)), 63));
+                }
+            }
+            if (active && (g_bits >> 8) == 0u && sp != 0u) pop(g_base, g_bits);
+        }
 
         // ---- idle lanes take node groups from other lanes' stacks ----
         // Once no rays are left to fetch, the k-th idle lane takes the top stack
