@@ -353,11 +353,12 @@ def mlp_benchmark(n_infer=1 << 20, n_train=1 << 19, reps=10):
 GUIDE_PLAIN_SPP, GUIDE_REF_SPP, GUIDE_SLOTS, GUIDE_CHUNK = 1024, 16384, 4, 32
 # FP64 references of the many-light frame and of the batched frame
 ML_REF_SPP, BATCH_REF_SPP = 1024, 256
-GUIDE_TRAIN_SPP, GUIDE_STEPS, GUIDE_FRAC, GUIDE_SAMPLES = 64, 32, 0.5, 1 << 19
+GUIDE_TRAIN_SPP, GUIDE_STEPS, GUIDE_FRAC, GUIDE_SAMPLES = 64, 64, 0.5, 1 << 19
 # Adam learning rates: the workload's stated 1e-3 (the headline figure), and 5e-3, the best
 # of the settings swept on this scene, reported beside it
 GUIDE_LR, GUIDE_LR_TUNED = 1e-3, 5e-3
 GUIDE_DEPTH = 0   # path vertices below this depth sample the proposal (0 = every vertex)
+GUIDE_STEP_SPP = 8  # the guided run's relMSE is checked every 8 spp (two passes of 4 slots)
 
 
 def guided_benchmark(builder):
@@ -403,7 +404,7 @@ def guided_benchmark(builder):
         curve_g = [(tot, t_g, scenes.rel_mse(img / tot, ref))]
         reach = t_g if curve_g[0][2] <= target else None
         while reach is None and tot < 4 * GUIDE_PLAIN_SPP:
-            t_g += timed(lambda: acc.render(GUIDE_CHUNK // 2, sc.max_depth, guide=2, guide_frac=GUIDE_FRAC,
+            t_g += timed(lambda: acc.render(GUIDE_STEP_SPP, sc.max_depth, guide=2, guide_frac=GUIDE_FRAC,
                                             guide_depth=GUIDE_DEPTH, graph=True))
             img, _, tot = acc.render_read()
             e = scenes.rel_mse(img / tot, ref)
