@@ -718,7 +718,7 @@ __global__ void k_init_roots(BuildState S, const uint32_t* rfirst, uint32_t R)
 // Child ids come from one block of 2 * count - 2 ids reserved per subtree (a binary tree over
 // count slots has at most that many nodes below its root); ids left unused are counted in
 // counters[3].  The subtree's slot order is written back to idx at the end.
-constexpr uint32_t FIN_BLOCK = 256;
+constexpr uint32_t FIN_BLOCK = 512;
 constexpr uint32_t FIN_WAVES = FIN_BLOCK / 64u;
 constexpr uint32_t FIN_QCAP = FIN / (TINY + 1) + 1;  // wave nodes in a round (disjoint, > TINY slots)
 constexpr uint32_t FIN_CCAP = FIN / 2;               // tiny SAH candidates in a step (>= 2 slots each)
@@ -733,6 +733,14 @@ __device__ __forceinline__ FinNode fin_node(uint32_t node, uint32_t first, uint3
 {
     return FinNode{ node, first | (count << 16) | (forced ? 0x80000000u : 0u) };
 }
+
+// How a node is split: mode 0 none (a leaf), 1 positional, 2 SAH on axis d at bin si with nb
+// slots below and the node's centroid range [mn, mx] on that axis.
+struct FinSplit
+{
+    uint32_t mode, d, si, nb;
+    float mn, mx;
+};
 
 struct FinShared
 {
@@ -772,13 +780,6 @@ __device__ __forceinline__ uint32_t forced_half(uint32_t cnt)
     return half < cnt - 1 ? half : cnt - 1;
 }
 
-// How a node is split: mode 0 none (a leaf), 1 positional, 2 SAH on axis d at bin si with nb
-// slots below and the node's centroid range [mn, mx] on that axis.
-struct FinSplit
-{
-    uint32_t mode, d, si, nb;
-    float mn, mx;
-};
 
 // Leaf test of the level loop (sweep_node_wave): forced nodes split positionally until they hold at
 // most APG_MAX_LEAF triangles; others are SAH leaves at most `thr` triangles or with a zero
@@ -874,20 +875,24 @@ __device__ __forceinline__ void fin_unpack_bounds(const uint32_t r[12], float nl
     }
 }
 
-// One node of more than TINY slots, by one wave.
-__device__ void fin_wave_node(BuildState& S, FinShared& L, uint32_t thr, uint32_t gfirst, FinNode e,
-                              uint32_t qnext, uint32_t* hist)
+// ---- wave-level steps of a node of more than TINY slots ----
+
+__device__ __forceinline__ void ord_bounds_init(uint32_t r[12])
 {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t k = e.node, f = fin_first(e), cnt = fin_count(e);
-    // bounds and centroid bounds, reduced across the wave (every lane ends with all 12)
     const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
-    uint32_t r[12];
     for (int a = 0; a < 3; ++a)
     {
         r[a] = lo_ord; r[3 + a] = hi_ord; r[6 + a] = lo_ord; r[9 + a] = hi_ord;
     }
-    for (uint32_t j = lane; j < cnt; j += 64u)
+}
+
+// Bounds and centroid bounds of slots [f + j0, f + j1), reduced across the wave (every lane
+// ends with all 12 ordered values).
+__device__ void fin_bounds_wave(const FinShared& L, uint32_t f, uint32_t j0, uint32_t j1, uint32_t r[12])
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    ord_bounds_init(r);
+    for (uint32_t j = j0 + lane; j < j1; j += 64u)
     {
         const float* b = L.box[L.lid[f + j]];
         for (int a = 0; a < 3; ++a)
@@ -907,6 +912,136 @@ __device__ void fin_wave_node(BuildState& S, FinShared& L, uint32_t thr, uint32_
             r[6 + a] = min(r[6 + a], uint32_t(__shfl_xor(int(r[6 + a]), off)));
             r[9 + a] = max(r[9 + a], uint32_t(__shfl_xor(int(r[9 + a]), off)));
         }
+}
+
+// The 9-bin histograms of the three axes of slots [f + j0, f + j1) in the wave's own LDS block.
+__device__ void fin_bin_wave(const FinShared& L, uint32_t f, uint32_t j0, uint32_t j1, uint32_t* hist,
+                             const float cmn[3], const float cmx[3])
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u)
+    {
+        const uint32_t v = j % BIN_VALS;
+        hist[j] = v < 3 ? ord_of(FLT_MAX) : (v < 6 ? ord_of(-FLT_MAX) : 0u);
+    }
+    wave_sync_lds();
+    for (uint32_t j = j0 + lane; j < j1; j += 64u)
+    {
+        const float* b = L.box[L.lid[f + j]];
+        float lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; }
+        for (int d = 0; d < 3; ++d)
+        {
+            if (cmn[d] == cmx[d]) continue;
+            const float c = (lo[d] + hi[d]) * 0.5f;
+            uint32_t* h = hist + (d * NBINS + bin_index(c, cmn[d], cmx[d])) * BIN_VALS;
+            for (int a = 0; a < 3; ++a)
+            {
+                atomicMin(&h[a], ord_of(lo[a]));
+                atomicMax(&h[3 + a], ord_of(hi[a]));
+            }
+            atomicAdd(&h[6], 1u);
+        }
+    }
+    wave_sync_lds();
+}
+
+// The sweep (sweep_node_wave's): lane d * 8 + s prices split s of axis d from the histogram,
+// the below box accumulated over bins 0..s and the above box over bins 8..s+1 in the reference
+// sweep's order; the first (axis, split) of least cost wins; then the leaf test.
+__device__ void fin_sweep_wave(const uint32_t* hist, uint32_t cnt, const float nlo[3], const float nhi[3],
+                               const float cmn[3], const float cmx[3], uint32_t& fl, FinSplit& sp)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    float cost = __builtin_inff();
+    uint32_t key = 63u, below = 0u;
+    if (lane < 3u * NSPLITS)
+    {
+        const uint32_t d = lane / NSPLITS, s = lane % NSPLITS;
+        const float mn = d == 0 ? cmn[0] : (d == 1 ? cmn[1] : cmn[2]);
+        const float mx = d == 0 ? cmx[0] : (d == 1 ? cmx[1] : cmx[2]);
+        if (mn != mx)
+        {
+            float blo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, bhi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+            float alo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, ahi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+            uint32_t bn = 0, an = 0;
+            for (uint32_t b = 0; b <= s; ++b)
+            {
+                const uint32_t* h = hist + (d * NBINS + b) * BIN_VALS;
+                for (int a = 0; a < 3; ++a) { blo[a] = min_s(blo[a], float_of(h[a])); bhi[a] = max_s(bhi[a], float_of(h[3 + a])); }
+                bn += h[6];
+            }
+            for (uint32_t b = NBINS - 1; b > s; --b)
+            {
+                const uint32_t* h = hist + (d * NBINS + b) * BIN_VALS;
+                for (int a = 0; a < 3; ++a) { alo[a] = min_s(alo[a], float_of(h[a])); ahi[a] = max_s(ahi[a], float_of(h[3 + a])); }
+                an += h[6];
+            }
+            float c = 0.0f;
+            c += float(bn) * box_area(blo, bhi);
+            c += float(an) * box_area(alo, ahi);
+            if (c < FLT_MAX)
+            {
+                cost = c;
+                key = lane;
+                below = bn;
+            }
+        }
+    }
+    // the first (axis, split) of least cost: the sequential sweep's strict-< choice
+    for (int off = 1; off < 32; off <<= 1)
+    {
+        const float oc = __shfl_xor(cost, off);
+        const uint32_t ok = uint32_t(__shfl_xor(int(key), off));
+        if (oc < cost || (oc == cost && ok < key)) { cost = oc; key = ok; }
+    }
+    const float min_cost = __shfl(cost, 0);
+    const uint32_t best = uint32_t(__shfl(int(key), 0));
+    const uint32_t nbelow = uint32_t(__shfl(int(below), int(best & 63u)));
+    const float leaf_cost = float(cnt) * box_area(nlo, nhi);
+    fin_choose(cnt, best != 63u, min_cost, leaf_cost, best / NSPLITS, best % NSPLITS, nbelow, cmn, cmx, fl, sp);
+    // the next use of hist must follow every lane's reads above
+    wave_sync_lds();
+}
+
+__device__ __forceinline__ bool fin_goes_left(const FinShared& L, uint16_t v, const FinSplit& sp)
+{
+    const float* b = L.box[v];
+    return bin_index((b[sp.d] + b[3 + sp.d]) * 0.5f, sp.mn, sp.mx) <= sp.si;
+}
+
+// Stable partition of slots [f + j0, f + j1) of a node into tmp: `left` left slots of the node
+// precede j0; the node has sp.nb left slots in all.
+__device__ void fin_scatter_wave(FinShared& L, uint32_t f, uint32_t j0, uint32_t j1, const FinSplit& sp,
+                                 uint32_t left)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t run = left;
+    for (uint32_t b = j0; b < j1; b += 64u)
+    {
+        const uint32_t j = b + lane;
+        bool bl = false;
+        uint16_t v = 0;
+        if (j < j1)
+        {
+            v = L.lid[f + j];
+            bl = fin_goes_left(L, v, sp);
+        }
+        const uint64_t m = __ballot(bl);
+        const uint32_t before = run + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+        if (j < j1) L.tmp[f + (bl ? before : sp.nb + (j - before))] = v;
+        run += uint32_t(__popcll(m));
+    }
+}
+
+// One node of more than TINY slots, by one wave.
+__device__ void fin_wave_node(BuildState& S, FinShared& L, uint32_t thr, uint32_t gfirst, FinNode e,
+                              uint32_t qnext, uint32_t* hist)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t k = e.node, f = fin_first(e), cnt = fin_count(e);
+    uint32_t r[12];
+    fin_bounds_wave(L, f, 0, cnt, r);
     if (lane == 0u)
         for (int j = 0; j < 6; ++j) S.nbox[6 * size_t(k) + j] = r[j];
     float nlo[3], nhi[3], cmn[3], cmx[3];
@@ -915,108 +1050,15 @@ __device__ void fin_wave_node(BuildState& S, FinShared& L, uint32_t thr, uint32_
     FinSplit sp;
     if (!fin_decide(cnt, fin_forced(e), thr, cmn, cmx, fl, sp))
     {
-        // 9-bin histograms of the three axes in the wave's own LDS block
-        for (uint32_t j = lane; j < NODE_BIN_WORDS; j += 64u)
-        {
-            const uint32_t v = j % BIN_VALS;
-            hist[j] = v < 3 ? ord_of(FLT_MAX) : (v < 6 ? ord_of(-FLT_MAX) : 0u);
-        }
-        wave_sync_lds();
-        uint32_t* hc = hist;
-        for (uint32_t j = lane; j < cnt; j += 64u)
-        {
-            const float* b = L.box[L.lid[f + j]];
-            float lo[3], hi[3];
-            for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; }
-            for (int d = 0; d < 3; ++d)
-            {
-                if (cmn[d] == cmx[d]) continue;
-                const float c = (lo[d] + hi[d]) * 0.5f;
-                uint32_t* h = hc + (d * NBINS + bin_index(c, cmn[d], cmx[d])) * BIN_VALS;
-                for (int a = 0; a < 3; ++a)
-                {
-                    atomicMin(&h[a], ord_of(lo[a]));
-                    atomicMax(&h[3 + a], ord_of(hi[a]));
-                }
-                atomicAdd(&h[6], 1u);
-            }
-        }
-        wave_sync_lds();
-        // the sweep (sweep_node): lane d * 8 + s prices split s of axis d, the below box
-        // accumulated over bins 0..s and the above box over bins 8..s+1 in sweep_node's order
-        float cost = __builtin_inff();
-        uint32_t key = 63u, below = 0u;
-        if (lane < 3u * NSPLITS)
-        {
-            const uint32_t d = lane / NSPLITS, s = lane % NSPLITS;
-            const float mn = d == 0 ? cmn[0] : (d == 1 ? cmn[1] : cmn[2]);
-            const float mx = d == 0 ? cmx[0] : (d == 1 ? cmx[1] : cmx[2]);
-            if (mn != mx)
-            {
-                float blo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, bhi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
-                float alo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, ahi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
-                uint32_t bn = 0, an = 0;
-                for (uint32_t b = 0; b <= s; ++b)
-                {
-                    const uint32_t* h = hist + (d * NBINS + b) * BIN_VALS;
-                    for (int a = 0; a < 3; ++a) { blo[a] = min_s(blo[a], float_of(h[a])); bhi[a] = max_s(bhi[a], float_of(h[3 + a])); }
-                    bn += h[6];
-                }
-                for (uint32_t b = NBINS - 1; b > s; --b)
-                {
-                    const uint32_t* h = hist + (d * NBINS + b) * BIN_VALS;
-                    for (int a = 0; a < 3; ++a) { alo[a] = min_s(alo[a], float_of(h[a])); ahi[a] = max_s(ahi[a], float_of(h[3 + a])); }
-                    an += h[6];
-                }
-                float c = 0.0f;
-                c += float(bn) * box_area(blo, bhi);
-                c += float(an) * box_area(alo, ahi);
-                if (c < FLT_MAX)
-                {
-                    cost = c;
-                    key = lane;
-                    below = bn;
-                }
-            }
-        }
-        // the first (axis, split) of least cost: the sequential sweep's strict-< choice
-        for (int off = 1; off < 32; off <<= 1)
-        {
-            const float oc = __shfl_xor(cost, off);
-            const uint32_t ok = uint32_t(__shfl_xor(int(key), off));
-            if (oc < cost || (oc == cost && ok < key)) { cost = oc; key = ok; }
-        }
-        const float min_cost = __shfl(cost, 0);
-        const uint32_t best = uint32_t(__shfl(int(key), 0));
-        const uint32_t nbelow = uint32_t(__shfl(int(below), int(best & 63u)));
-        const float leaf_cost = float(cnt) * box_area(nlo, nhi);
-        fin_choose(cnt, best != 63u, min_cost, leaf_cost, best / NSPLITS, best % NSPLITS, nbelow, cmn, cmx, fl, sp);
-        // the next node's clear of hist must follow every lane's reads above
-        wave_sync_lds();
+        fin_bin_wave(L, f, 0, cnt, hist, cmn, cmx);
+        fin_sweep_wave(hist, cnt, nlo, nhi, cmn, cmx, fl, sp);
     }
     if (lane == 0u) S.nflags[k] = fl;
     if (sp.mode == 0) return;
     const uint32_t nb = sp.nb;
     if (sp.mode == 2)
     {
-        // stable partition of the node's slots: below the split plane first
-        uint32_t run = 0;
-        for (uint32_t j0 = 0; j0 < cnt; j0 += 64u)
-        {
-            const uint32_t j = j0 + lane;
-            bool bl = false;
-            uint16_t v = 0;
-            if (j < cnt)
-            {
-                v = L.lid[f + j];
-                const float* b = L.box[v];
-                bl = bin_index((b[sp.d] + b[3 + sp.d]) * 0.5f, sp.mn, sp.mx) <= sp.si;
-            }
-            const uint64_t m = __ballot(bl);
-            const uint32_t before = run + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
-            if (j < cnt) L.tmp[f + (bl ? before : nb + (j - before))] = v;
-            run += uint32_t(__popcll(m));
-        }
+        fin_scatter_wave(L, f, 0, cnt, sp, 0u);
         wave_sync_lds();
         for (uint32_t j = lane; j < cnt; j += 64u) L.lid[f + j] = L.tmp[f + j];
         wave_sync_lds();
@@ -1162,7 +1204,10 @@ __device__ void fin_tiny_choose(BuildState& S, FinShared& L, uint32_t gfirst, ui
 }
 
 // Grid-stride over the nodes the level loop handed over, taken in turn from a device cursor.
-__global__ __launch_bounds__(FIN_BLOCK) void k_finish(BuildState S, uint32_t thr)
+// Two workgroups of 512 threads per CU (the LDS holds two): the register budget is set for
+// four waves per SIMD.
+__global__ __launch_bounds__(FIN_BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_finish(BuildState S,
+                                                                                                  uint32_t thr)
 {
     __shared__ FinShared L;
     const uint32_t tid = threadIdx.x, wave = tid / 64u;
