@@ -95,11 +95,13 @@ __device__ __forceinline__ uint8_t bin_index(float c, float mn, float mx)
 struct BuildState
 {
     // triangles (original order)
-    float* pbox;        // 6 per triangle: lo xyz, hi xyz
     float* tri9_src;    // 9 per triangle: v0, e0, e1
-    // slots
-    uint32_t* idx;      // slot -> triangle
-    uint32_t* idx_tmp;  // partition staging
+    // slots, in two buffers: buffer 0 is (idx, pbox), buffer 1 (idx_tmp, pbox_tmp); level l
+    // reads its nodes' slots from buffer l % 2 and partitions them into buffer (l + 1) % 2
+    uint32_t* idx;      // slot -> triangle (in the end every slot's, written by k_finish)
+    uint32_t* idx_tmp;
+    float* pbox;        // 6 per slot: the triangle's lo xyz, hi xyz (k_prep: triangle order)
+    float* pbox_tmp;
     uint8_t* flag8;     // per slot of the level's nodes: goes to the left child
     uint16_t* sbin;     // per slot of the level's SAH candidates: bin on axis a in bits 4a..4a+3
     // nodes (capacity 2n)
@@ -123,8 +125,14 @@ struct BuildState
     uint32_t* large_off;  // exclusive scan of their chunk counts (total at [count])
     uint32_t* chunk_below;  // per chunk of the level: slots going to the left child
     uint32_t* chunk_off;    // their exclusive scan over the level's chunks
-    uint32_t* fin;        // nodes of at most FIN slots, left to k_finish (count at lvl[4])
+    uint32_t* fin;        // nodes of at most FIN slots, left to k_finish (count at lvl[4]), with
+                          // the buffer of their slots in bit 31
 };
+
+// The slot buffer a kernel of the current level reads (k_large_scan has advanced lvl[2]).
+__device__ __forceinline__ uint32_t level_buffer(const BuildState& S) { return (S.lvl[2] - 1u) & 1u; }
+__device__ __forceinline__ const uint32_t* slots_of(const BuildState& S, uint32_t b) { return b ? S.idx_tmp : S.idx; }
+__device__ __forceinline__ const float* boxes_of(const BuildState& S, uint32_t b) { return b ? S.pbox_tmp : S.pbox; }
 
 // ---------------------------------------------------------------------------
 // triangle preparation (Primitive::updateVertices, bvh_common.cpp:105-125)
@@ -197,8 +205,7 @@ __global__ void k_init_slots(uint32_t n, uint32_t* idx)
 // ---------------------------------------------------------------------------
 // Every kernel of a level reads the level's node range [lvl[0], lvl[1]) from device memory
 // and walks it with a grid-stride loop over a grid fixed for the build, so the host enqueues
-// level after level without knowing their widths; the last workgroup of k_part_copy moves on
-// to the children.
+// level after level without knowing their widths; k_large_scan moves on to the children.
 
 // Per node of the level: hand the nodes of at most FIN slots to k_finish; clear the others'
 // bounds accumulators and list them (their order in the list is arbitrary: what they
@@ -215,7 +222,7 @@ __global__ __launch_bounds__(256) void k_level_init(BuildState S)
         {
             // built by k_finish after the level loop; every kernel of the level passes it by
             S.nbinslot[k] = RETIRED;
-            S.fin[atomicAdd(&S.lvl[4], 1u)] = k;
+            S.fin[atomicAdd(&S.lvl[4], 1u)] = k | ((S.lvl[2] & 1u) << 31);
             continue;
         }
         for (int a = 0; a < 3; ++a)
@@ -338,9 +345,10 @@ __global__ __launch_bounds__(BLOCK) void k_bounds_large(BuildState S)
     {
         r[a] = lo_ord; r[3 + a] = hi_ord; r[6 + a] = lo_ord; r[9 + a] = hi_ord;
     }
+    const float* box = boxes_of(S, level_buffer(S));
     for (uint32_t i = lo_i + threadIdx.x; i < hi_i; i += BLOCK)
     {
-        const float* b = S.pbox + 6 * size_t(S.idx[i]);
+        const float* b = box + 6 * size_t(i);
         for (int a = 0; a < 3; ++a)
         {
             const float lo = b[a], hi = b[3 + a], c = (lo + hi) * 0.5f;
@@ -403,9 +411,10 @@ __global__ __launch_bounds__(BLOCK) void k_bin_large(BuildState S)
     }
     __syncthreads();
     uint32_t* acc = hist[(threadIdx.x / 64u) * HCOPIES + (threadIdx.x % HCOPIES)];
+    const float* box = boxes_of(S, level_buffer(S));
     for (uint32_t i = lo_i + threadIdx.x; i < hi_i; i += BLOCK)
     {
-        const float* b = S.pbox + 6 * size_t(S.idx[i]);
+        const float* b = box + 6 * size_t(i);
         float lo[3], hi[3];
         for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; }
         uint32_t bins3 = 0;
@@ -465,10 +474,93 @@ __device__ __forceinline__ void positional_split(BuildState& S, uint32_t k, uint
     alloc_children(S, k, cnt, half, NF_FORCED);
 }
 
-// Sweep, split choice and child allocation of one node of the level, by one wave: lane
-// d * 8 + s prices split s of axis d from the node's histogram (the below box accumulated over
-// bins 0..s and the above box over bins 8..s+1, in the reference sweep's order), and the first
-// (axis, split) of least cost wins, as the sequential sweep's strict < picks it.
+// The SAH sweep of a node's 3 x 9-bin histograms (hb: per bin lo xyz and hi xyz as ordered
+// uints, then a count) by one wave, for the axes in the mask `axes` (those of non-zero centroid
+// extent).  Lane d * 9 + b loads bin b of axis d once; inclusive scans over the axis' nine lanes
+// give each split s its below box and count (bins 0..s) and its above box and count (bins
+// s+1..8).  Min and max are exact, so the boxes are those the reference's sequential sweep
+// accumulates, up to the sign of a zero (which changes no area), and so are the costs
+// (bn * area(below) + an * area(above), in that order).  Splits whose cost is not below
+// FLT_MAX are not candidates; the first (axis, split) of least cost wins, as the sequential
+// sweep's strict < picks it.  Every lane returns the winner: key = d * 8 + s (63: none), its
+// cost and its below count; `at` is the lane that priced it.
+struct SweepBest
+{
+    float cost;
+    uint32_t key, below, at;
+};
+
+__device__ SweepBest sweep_bins_wave(const uint32_t* hb, uint32_t axes)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool in = lane < 3u * NBINS;
+    const uint32_t d = lane / NBINS, b = lane % NBINS;
+    float blo[3], bhi[3], alo[3], ahi[3];
+    uint32_t bn = 0;
+    for (int a = 0; a < 3; ++a) { blo[a] = FLT_MAX; bhi[a] = -FLT_MAX; }
+    if (in)
+    {
+        const uint32_t* e = hb + lane * BIN_VALS;
+        for (int a = 0; a < 3; ++a) { blo[a] = float_of(e[a]); bhi[a] = float_of(e[3 + a]); }
+        bn = e[6];
+    }
+    for (int a = 0; a < 3; ++a) { alo[a] = blo[a]; ahi[a] = bhi[a]; }
+    uint32_t an = bn;
+    // below: bins 0..b; above: bins b..8 (shifted to s + 1 afterwards)
+    for (uint32_t off = 1; off < NBINS; off <<= 1)
+    {
+        const int up = int((lane - off) & 63u), dn = int((lane + off) & 63u);
+        const bool take_up = b >= off, take_dn = b + off < NBINS;
+        float t[12];
+        for (int a = 0; a < 3; ++a)
+        {
+            t[a] = __shfl(blo[a], up); t[3 + a] = __shfl(bhi[a], up);
+            t[6 + a] = __shfl(alo[a], dn); t[9 + a] = __shfl(ahi[a], dn);
+        }
+        const uint32_t tbn = uint32_t(__shfl(int(bn), up)), tan = uint32_t(__shfl(int(an), dn));
+        if (take_up)
+        {
+            for (int a = 0; a < 3; ++a) { blo[a] = min_s(blo[a], t[a]); bhi[a] = max_s(bhi[a], t[3 + a]); }
+            bn += tbn;
+        }
+        if (take_dn)
+        {
+            for (int a = 0; a < 3; ++a) { alo[a] = min_s(alo[a], t[6 + a]); ahi[a] = max_s(ahi[a], t[9 + a]); }
+            an += tan;
+        }
+    }
+    {
+        const int nx = int((lane + 1u) & 63u);
+        for (int a = 0; a < 3; ++a) { alo[a] = __shfl(alo[a], nx); ahi[a] = __shfl(ahi[a], nx); }
+        an = uint32_t(__shfl(int(an), nx));
+    }
+    SweepBest r{ __builtin_inff(), 63u, 0u, 0u };
+    if (in && b < NSPLITS && ((axes >> d) & 1u))
+    {
+        float c = 0.0f;
+        c += float(bn) * box_area(blo, bhi);
+        c += float(an) * box_area(alo, ahi);
+        if (c < FLT_MAX)
+        {
+            r.cost = c;
+            r.key = d * NSPLITS + b;
+        }
+    }
+    for (int off = 1; off < 32; off <<= 1)
+    {
+        const float oc = __shfl_xor(r.cost, off);
+        const uint32_t ok = uint32_t(__shfl_xor(int(r.key), off));
+        if (oc < r.cost || (oc == r.cost && ok < r.key)) { r.cost = oc; r.key = ok; }
+    }
+    r.cost = __shfl(r.cost, 0);
+    r.key = uint32_t(__shfl(int(r.key), 0));
+    r.at = r.key == 63u ? 0u : (r.key / NSPLITS) * NBINS + r.key % NSPLITS;
+    r.below = uint32_t(__shfl(int(bn), int(r.at)));
+    return r;
+}
+
+// Sweep, split choice and child allocation of one node of the level, by one wave
+// (sweep_bins_wave over the node's global histogram).
 __device__ void sweep_node_wave(BuildState& S, uint32_t k, uint32_t lane, uint32_t thr)
 {
     const uint32_t cnt = S.ncount[k];
@@ -504,51 +596,12 @@ __device__ void sweep_node_wave(BuildState& S, uint32_t k, uint32_t lane, uint32
         return;
     }
     const uint32_t* hb = S.bins + size_t(S.nbinslot[k]) * NODE_BIN_WORDS;
-    float cost = __builtin_inff();
-    uint32_t key = 63u, below = 0u;
-    float mn = 0.0f, mx = 0.0f;
-    if (lane < 3u * NSPLITS)
-    {
-        const uint32_t d = lane / NSPLITS, s = lane % NSPLITS;
-        mn = float_of(S.ncbox[6 * size_t(k) + d]);
-        mx = float_of(S.ncbox[6 * size_t(k) + 3 + d]);
-        if (mn != mx)
-        {
-            float blo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, bhi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
-            float alo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, ahi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
-            uint32_t bn = 0, an = 0;
-            for (uint32_t b = 0; b <= s; ++b)
-            {
-                const uint32_t* e = hb + (d * NBINS + b) * BIN_VALS;
-                for (int a = 0; a < 3; ++a) { blo[a] = min_s(blo[a], float_of(e[a])); bhi[a] = max_s(bhi[a], float_of(e[3 + a])); }
-                bn += e[6];
-            }
-            for (uint32_t b = NBINS - 1; b > s; --b)
-            {
-                const uint32_t* e = hb + (d * NBINS + b) * BIN_VALS;
-                for (int a = 0; a < 3; ++a) { alo[a] = min_s(alo[a], float_of(e[a])); ahi[a] = max_s(ahi[a], float_of(e[3 + a])); }
-                an += e[6];
-            }
-            float c = 0.0f;
-            c += float(bn) * box_area(blo, bhi);
-            c += float(an) * box_area(alo, ahi);
-            if (c < FLT_MAX)
-            {
-                cost = c;
-                key = lane;
-                below = bn;
-            }
-        }
-    }
-    for (int off = 1; off < 32; off <<= 1)
-    {
-        const float oc = __shfl_xor(cost, off);
-        const uint32_t ok = uint32_t(__shfl_xor(int(key), off));
-        if (oc < cost || (oc == cost && ok < key)) { cost = oc; key = ok; }
-    }
-    const uint32_t best = uint32_t(__shfl(int(key), 0));
-    const uint32_t nbelow = uint32_t(__shfl(int(below), int(best & 63u)));
-    const float best_mn = __shfl(mn, int(best & 63u)), best_mx = __shfl(mx, int(best & 63u));
+    uint32_t axes = 0;
+    for (int a = 0; a < 3; ++a)
+        if (float_of(S.ncbox[6 * size_t(k) + a]) != float_of(S.ncbox[6 * size_t(k) + 3 + a])) axes |= 1u << a;
+    const SweepBest sb = sweep_bins_wave(hb, axes);
+    const float cost = sb.cost;
+    const uint32_t best = sb.key, nbelow = sb.below;
     if (lane != 0u) return;
     float nlo[3], nhi[3];
     for (int a = 0; a < 3; ++a)
@@ -566,8 +619,8 @@ __device__ void sweep_node_wave(BuildState& S, uint32_t k, uint32_t lane, uint32
     }
     S.nflags[k] = fl | NF_SPLIT;
     S.nsplit[k] = (best / NSPLITS) | ((best % NSPLITS) << 8);
-    S.nsplit_mm[2 * size_t(k)] = best_mn;
-    S.nsplit_mm[2 * size_t(k) + 1] = best_mx;
+    S.nsplit_mm[2 * size_t(k)] = float_of(S.ncbox[6 * size_t(k) + best / NSPLITS]);
+    S.nsplit_mm[2 * size_t(k) + 1] = float_of(S.ncbox[6 * size_t(k) + 3 + best / NSPLITS]);
     alloc_children(S, k, cnt, nbelow, 0u);
 }
 
@@ -627,7 +680,8 @@ __global__ __launch_bounds__(SCAN1) void k_part_scan(BuildState S)
     wg_exclusive_scan(m, [&](uint32_t c) { return S.chunk_below[c]; }, S.chunk_off);
 }
 
-// Per chunk: every slot's place in its node's stable partition (left slots first), in idx_tmp.
+// Per chunk: every slot's place in its node's stable partition (left slots first), in the
+// other slot buffer (triangle and box move together).
 // Slot lo + q * BLOCK + t is thread t's q-th; its rank among the chunk's left slots counts
 // the earlier rounds, the earlier waves of its round and the earlier lanes of its wave.
 __global__ __launch_bounds__(BLOCK) void k_part_scatter(BuildState S)
@@ -647,6 +701,11 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(BuildState S)
     __syncthreads();
     const uint32_t first = S.nfirst[node];
     const uint32_t nleft_total = S.ncount[S.nleft[node]];
+    const uint32_t buf = level_buffer(S);
+    const uint32_t* idx_in = slots_of(S, buf);
+    const float* box_in = boxes_of(S, buf);
+    uint32_t* idx_out = buf ? S.idx : S.idx_tmp;
+    float* box_out = buf ? S.pbox : S.pbox_tmp;
     // left slots of the node in its earlier chunks
     uint32_t left = S.chunk_off[blockIdx.x] - S.chunk_off[S.large_off[j]];
     const uint64_t below_lane = (1ull << lane) - 1ull;
@@ -660,18 +719,13 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(BuildState S)
         {
             const uint32_t pos = ((masks[q] >> lane) & 1ull) ? first + before
                                                              : first + nleft_total + (i - first - before);
-            S.idx_tmp[pos] = S.idx[i];
+            idx_out[pos] = idx_in[i];
+            float b[6];
+            for (int a = 0; a < 6; ++a) b[a] = box_in[6 * size_t(i) + a];
+            for (int a = 0; a < 6; ++a) box_out[6 * size_t(pos) + a] = b[a];
         }
         for (uint32_t w = 0; w < W; ++w) left += wl[q][w];
     }
-}
-
-// Per chunk: the partitioned slots back into idx.
-__global__ __launch_bounds__(BLOCK) void k_part_copy(BuildState S)
-{
-    uint32_t node, lo, hi;
-    if (!chunk_range(S, blockIdx.x, node, lo, hi)) return;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += BLOCK) S.idx[i] = S.idx_tmp[i];
 }
 
 // Roots: the first slot and slot count of each tree, the level control words and counters.
@@ -751,8 +805,11 @@ struct FinShared
     FinNode queue[2][FIN_QCAP];             // wave nodes: this round's and the next round's
     FinNode tiny[2][FIN];                   // tiny nodes: this sub-level's and the next one's
     FinNode cand[FIN_CCAP];                 // this sub-level's tiny SAH candidates
-    float ccost[FIN_CCAP][3];               // their best cost per axis (+inf: none)
+    float ccost[FIN_CCAP][3];               // centroid minimum per axis, then the best cost (+inf: none)
+    float ccmx[FIN_CCAP][3];                // centroid maximum per axis
     uint16_t cinfo[FIN_CCAP][3];            // split | below count << 8 of that cost
+    uint16_t order[FIN_CCAP];               // the candidates by slot count (lanes of a wave run alike loops)
+    uint32_t csize[TINY + 1];               // candidates per slot count, then their offsets
     uint32_t hist[FIN_WAVES][NODE_BIN_WORDS];
     uint32_t qn[2], tn[2], cn, next, base, item;
 };
@@ -946,60 +1003,17 @@ __device__ void fin_bin_wave(const FinShared& L, uint32_t f, uint32_t j0, uint32
     wave_sync_lds();
 }
 
-// The sweep (sweep_node_wave's): lane d * 8 + s prices split s of axis d from the histogram,
-// the below box accumulated over bins 0..s and the above box over bins 8..s+1 in the reference
-// sweep's order; the first (axis, split) of least cost wins; then the leaf test.
+// The sweep (sweep_bins_wave over the wave's histogram), then the leaf test.
 __device__ void fin_sweep_wave(const uint32_t* hist, uint32_t cnt, const float nlo[3], const float nhi[3],
                                const float cmn[3], const float cmx[3], uint32_t& fl, FinSplit& sp)
 {
-    const uint32_t lane = threadIdx.x & 63u;
-    float cost = __builtin_inff();
-    uint32_t key = 63u, below = 0u;
-    if (lane < 3u * NSPLITS)
-    {
-        const uint32_t d = lane / NSPLITS, s = lane % NSPLITS;
-        const float mn = d == 0 ? cmn[0] : (d == 1 ? cmn[1] : cmn[2]);
-        const float mx = d == 0 ? cmx[0] : (d == 1 ? cmx[1] : cmx[2]);
-        if (mn != mx)
-        {
-            float blo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, bhi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
-            float alo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, ahi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
-            uint32_t bn = 0, an = 0;
-            for (uint32_t b = 0; b <= s; ++b)
-            {
-                const uint32_t* h = hist + (d * NBINS + b) * BIN_VALS;
-                for (int a = 0; a < 3; ++a) { blo[a] = min_s(blo[a], float_of(h[a])); bhi[a] = max_s(bhi[a], float_of(h[3 + a])); }
-                bn += h[6];
-            }
-            for (uint32_t b = NBINS - 1; b > s; --b)
-            {
-                const uint32_t* h = hist + (d * NBINS + b) * BIN_VALS;
-                for (int a = 0; a < 3; ++a) { alo[a] = min_s(alo[a], float_of(h[a])); ahi[a] = max_s(ahi[a], float_of(h[3 + a])); }
-                an += h[6];
-            }
-            float c = 0.0f;
-            c += float(bn) * box_area(blo, bhi);
-            c += float(an) * box_area(alo, ahi);
-            if (c < FLT_MAX)
-            {
-                cost = c;
-                key = lane;
-                below = bn;
-            }
-        }
-    }
-    // the first (axis, split) of least cost: the sequential sweep's strict-< choice
-    for (int off = 1; off < 32; off <<= 1)
-    {
-        const float oc = __shfl_xor(cost, off);
-        const uint32_t ok = uint32_t(__shfl_xor(int(key), off));
-        if (oc < cost || (oc == cost && ok < key)) { cost = oc; key = ok; }
-    }
-    const float min_cost = __shfl(cost, 0);
-    const uint32_t best = uint32_t(__shfl(int(key), 0));
-    const uint32_t nbelow = uint32_t(__shfl(int(below), int(best & 63u)));
+    uint32_t axes = 0;
+    for (int a = 0; a < 3; ++a)
+        if (cmn[a] != cmx[a]) axes |= 1u << a;
+    const SweepBest sb = sweep_bins_wave(hist, axes);
     const float leaf_cost = float(cnt) * box_area(nlo, nhi);
-    fin_choose(cnt, best != 63u, min_cost, leaf_cost, best / NSPLITS, best % NSPLITS, nbelow, cmn, cmx, fl, sp);
+    fin_choose(cnt, sb.key != 63u, sb.cost, leaf_cost, sb.key / NSPLITS, sb.key % NSPLITS, sb.below, cmn, cmx, fl,
+               sp);
     // the next use of hist must follow every lane's reads above
     wave_sync_lds();
 }
@@ -1077,15 +1091,8 @@ __device__ void fin_tiny_axis(FinShared& L, uint32_t c, uint32_t d)
 {
     const FinNode e = L.cand[c];
     const uint32_t f = fin_first(e), cnt = fin_count(e);
-    uint32_t omn = ord_of(FLT_MAX), omx = ord_of(-FLT_MAX);
-    for (uint32_t j = 0; j < cnt; ++j)
-    {
-        const float* b = L.box[L.lid[f + j]];
-        const uint32_t o = ord_of((b[d] + b[3 + d]) * 0.5f);
-        omn = min(omn, o);
-        omx = max(omx, o);
-    }
-    const float mn = float_of(omn), mx = float_of(omx);
+    // the centroid range on axis d, left by step 1 (ccost holds the minimum until this step)
+    const float mn = L.ccost[c][d], mx = L.ccmx[c][d];
     float min_cost = FLT_MAX;
     uint32_t best_idx = 0, best_below = 0;
     bool found = false;
@@ -1108,6 +1115,9 @@ __device__ void fin_tiny_axis(FinShared& L, uint32_t c, uint32_t d)
             float lo[3], hi[3];
             for (int a = 0; a < 3; ++a) { lo[a] = b[a]; hi[a] = b[3 + a]; }
             const uint32_t bi = bin_index((lo[d] + hi[d]) * 0.5f, mn, mx);
+            // compare-and-select updates (two instructions a value; fminf / fmaxf would also
+            // quieten their operands every time): the boxes are finite, and which zero a tie
+            // between -0 and +0 keeps changes no area
 #pragma unroll
             for (uint32_t q = 0; q < NBINS; ++q)
             {
@@ -1115,8 +1125,8 @@ __device__ void fin_tiny_axis(FinShared& L, uint32_t c, uint32_t d)
                 nbin[q] += in ? 1u : 0u;
                 for (int a = 0; a < 3; ++a)
                 {
-                    lo_[q][a] = fminf(lo_[q][a], in ? lo[a] : FLT_MAX);
-                    hi_[q][a] = fmaxf(hi_[q][a], in ? hi[a] : -FLT_MAX);
+                    lo_[q][a] = (in && lo[a] < lo_[q][a]) ? lo[a] : lo_[q][a];
+                    hi_[q][a] = (in && hi_[q][a] < hi[a]) ? hi[a] : hi_[q][a];
                 }
             }
         }
@@ -1128,7 +1138,7 @@ __device__ void fin_tiny_axis(FinShared& L, uint32_t c, uint32_t d)
 #pragma unroll
             for (uint32_t q = 0; q < NSPLITS; ++q)
             {
-                for (int a = 0; a < 3; ++a) { plo[a] = fminf(plo[a], lo_[q][a]); phi[a] = fmaxf(phi[a], hi_[q][a]); }
+                for (int a = 0; a < 3; ++a) { plo[a] = min_s(plo[a], lo_[q][a]); phi[a] = max_s(phi[a], hi_[q][a]); }
                 pn += nbin[q];
                 bn[q] = pn;
                 cost[q] = 0.0f;
@@ -1140,7 +1150,7 @@ __device__ void fin_tiny_axis(FinShared& L, uint32_t c, uint32_t d)
 #pragma unroll
             for (int q = NSPLITS - 1; q >= 0; --q)
             {
-                for (int a = 0; a < 3; ++a) { plo[a] = fminf(plo[a], lo_[q + 1][a]); phi[a] = fmaxf(phi[a], hi_[q + 1][a]); }
+                for (int a = 0; a < 3; ++a) { plo[a] = min_s(plo[a], lo_[q + 1][a]); phi[a] = max_s(phi[a], hi_[q + 1][a]); }
                 cost[q] += float(cnt - bn[q]) * box_area(plo, phi);
             }
         }
@@ -1218,15 +1228,16 @@ __global__ __launch_bounds__(FIN_BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)
         __syncthreads();
         const uint32_t item = L.item;
         if (item >= total) break;
-        const uint32_t root = S.fin[item];
+        const uint32_t root = S.fin[item] & 0x7FFFFFFFu, buf = S.fin[item] >> 31;
         const uint32_t gfirst = S.nfirst[root], cnt = S.ncount[root];
+        const uint32_t* idx_in = slots_of(S, buf);
+        const float* box_in = boxes_of(S, buf);
         const bool forced = (S.nflags[root] & NF_FORCED) != 0u;
         for (uint32_t j = tid; j < cnt; j += FIN_BLOCK)
         {
-            const uint32_t t = S.idx[gfirst + j];
-            L.tri[j] = t;
+            L.tri[j] = idx_in[gfirst + j];
             L.lid[j] = uint16_t(j);
-            const float* b = S.pbox + 6 * size_t(t);
+            const float* b = box_in + 6 * size_t(gfirst + j);
             for (int a = 0; a < 6; ++a) L.box[j][a] = b[a];
         }
         if (tid == 0)
@@ -1238,6 +1249,7 @@ __global__ __launch_bounds__(FIN_BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)
             L.cn = 0;
             fin_push(L, 0, 0, fin_node(root, 0, cnt, forced));
         }
+        if (tid <= TINY) L.csize[tid] = 0;
         __syncthreads();
         // wave nodes: rounds of one wave per node (their tiny children go to tiny[0])
         uint32_t cur = 0;
@@ -1271,7 +1283,10 @@ __global__ __launch_bounds__(FIN_BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)
                 FinSplit sp;
                 if (!fin_decide(c, fin_forced(e), thr, cmn, cmx, fl, sp))
                 {
-                    L.cand[atomicAdd(&L.cn, 1u)] = e;
+                    const uint32_t j = atomicAdd(&L.cn, 1u);
+                    L.cand[j] = e;
+                    for (int a = 0; a < 3; ++a) { L.ccost[j][a] = cmn[a]; L.ccmx[j][a] = cmx[a]; }
+                    atomicAdd(&L.csize[c], 1u);
                     continue;
                 }
                 S.nflags[k] = fl;
@@ -1284,15 +1299,30 @@ __global__ __launch_bounds__(FIN_BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)
             }
             __syncthreads();
             const uint32_t cn = L.cn;
-            for (uint32_t u = tid; u < 3 * cn; u += FIN_BLOCK) fin_tiny_axis(L, u / 3, u % 3);
+            if (tid == 0)
+            {
+                uint32_t run = 0;
+                for (uint32_t k = 0; k <= TINY; ++k)
+                {
+                    const uint32_t h = L.csize[k];
+                    L.csize[k] = run;
+                    run += h;
+                }
+            }
             __syncthreads();
-            for (uint32_t c = tid; c < cn; c += FIN_BLOCK) fin_tiny_choose(S, L, gfirst, c, tc ^ 1u);
+            for (uint32_t c = tid; c < cn; c += FIN_BLOCK)
+                L.order[atomicAdd(&L.csize[fin_count(L.cand[c])], 1u)] = uint16_t(c);
+            __syncthreads();
+            for (uint32_t u = tid; u < 3 * cn; u += FIN_BLOCK) fin_tiny_axis(L, L.order[u / 3], u % 3);
+            __syncthreads();
+            for (uint32_t c = tid; c < cn; c += FIN_BLOCK) fin_tiny_choose(S, L, gfirst, L.order[c], tc ^ 1u);
             __syncthreads();
             if (tid == 0)
             {
                 L.tn[tc] = 0;
                 L.cn = 0;
             }
+            if (tid <= TINY) L.csize[tid] = 0;
             tc ^= 1u;
             __syncthreads();
         }
@@ -1370,29 +1400,39 @@ __device__ __forceinline__ void node_box(const WideState& W, uint32_t b, float l
 __device__ void wide_gather(const WideState& W, uint32_t w)
 {
     const uint32_t root = W.wbin[w];
+    // the children so far, with each one's leaf flag, surface area and box centre read once
     uint32_t ch[APG_WIDE];
+    bool leaf[APG_WIDE];
+    float sa[APG_WIDE], ctr[APG_WIDE][3];
     uint32_t nch = 0;
-    if (is_leaf_node(W, root)) ch[nch++] = root;
+    auto take = [&](uint32_t i, uint32_t b) {
+        float lo[3], hi[3];
+        node_box(W, b, lo, hi);
+        ch[i] = b;
+        leaf[i] = is_leaf_node(W, b);
+        sa[i] = box_area(lo, hi);
+        for (int a = 0; a < 3; ++a) ctr[i][a] = 0.5f * (lo[a] + hi[a]);
+    };
+    if (is_leaf_node(W, root)) take(nch++, root);
     else
     {
-        ch[nch++] = W.nleft[root];
-        ch[nch++] = W.nleft[root] + 1;
+        const uint32_t l = W.nleft[root];
+        take(nch++, l);
+        take(nch++, l + 1);
+        // the inner child of largest surface area (the first such) opens into its two children
         while (nch < APG_WIDE)
         {
             int best = -1;
             float best_sa = 0.0f;
             for (uint32_t i = 0; i < nch; ++i)
             {
-                if (is_leaf_node(W, ch[i])) continue;
-                float lo[3], hi[3];
-                node_box(W, ch[i], lo, hi);
-                const float sa = box_area(lo, hi);
-                if (best < 0 || sa > best_sa) { best = int(i); best_sa = sa; }
+                if (leaf[i]) continue;
+                if (best < 0 || sa[i] > best_sa) { best = int(i); best_sa = sa[i]; }
             }
             if (best < 0) break;
-            const uint32_t c = ch[best];
-            ch[nch++] = W.nleft[c];
-            ch[best] = W.nleft[c] + 1;
+            const uint32_t c = W.nleft[ch[best]];
+            take(nch++, c);
+            take(uint32_t(best), c + 1);
         }
     }
     // octant slot assignment (greedy over the 8 x 8 cost table)
@@ -1402,13 +1442,10 @@ __device__ void wide_gather(const WideState& W, uint32_t w)
     for (int a = 0; a < 3; ++a) pc[a] = 0.5f * (plo[a] + phi[a]);
     float cd[APG_WIDE][3];
     for (uint32_t c = 0; c < nch; ++c)
-    {
-        float lo[3], hi[3];
-        node_box(W, ch[c], lo, hi);
-        for (int a = 0; a < 3; ++a) cd[c][a] = 0.5f * (lo[a] + hi[a]) - pc[a];
-    }
+        for (int a = 0; a < 3; ++a) cd[c][a] = ctr[c][a] - pc[a];
     uint32_t slot_child[APG_WIDE];
-    for (uint32_t s = 0; s < APG_WIDE; ++s) slot_child[s] = RETIRED;
+    bool slot_leaf[APG_WIDE];
+    for (uint32_t s = 0; s < APG_WIDE; ++s) { slot_child[s] = RETIRED; slot_leaf[s] = false; }
     uint32_t done = 0, used = 0;
     for (uint32_t r = 0; r < nch; ++r)
     {
@@ -1426,6 +1463,7 @@ __device__ void wide_gather(const WideState& W, uint32_t w)
             }
         }
         slot_child[bs] = ch[bcc];
+        slot_leaf[bs] = leaf[bcc];
         done |= 1u << bcc;
         used |= 1u << bs;
     }
@@ -1435,7 +1473,7 @@ __device__ void wide_gather(const WideState& W, uint32_t w)
         const uint32_t c = slot_child[s];
         W.wchild[8 * size_t(w) + s] = c;
         if (c == RETIRED) continue;
-        if (is_leaf_node(W, c)) nt += W.ncount[c];
+        if (slot_leaf[s]) nt += W.ncount[c];
         else ni++;
     }
     W.n_inner[w] = ni;
@@ -1657,7 +1695,11 @@ __global__ void k_tri_gather(WideState W, uint32_t n, const uint32_t* fail)
     const uint32_t g = W.idx[W.perm[p]];
     const float* src = W.tri9_src + 9 * size_t(g);
     float* dst = W.tri9 + 9 * size_t(p);
-    for (int j = 0; j < 9; ++j) dst[j] = src[j];
+    // all nine loads before the first store (the buffers may alias as far as the compiler
+    // knows, so interleaved copies would wait on memory once per value)
+    float v[9];
+    for (int j = 0; j < 9; ++j) v[j] = src[j];
+    for (int j = 0; j < 9; ++j) dst[j] = v[j];
     W.tri_ids[p] = g;
 }
 
@@ -1803,6 +1845,7 @@ struct BuildCache
         const size_t wcap = size_t(n) + R + 2;      // >= binary nodes / 2 + 2 + R
         int rc;
         if ((rc = get(ctx, S.pbox, 6 * size_t(n))) || (rc = get(ctx, S.tri9_src, 9 * size_t(n))) ||
+            (rc = get(ctx, S.pbox_tmp, 6 * size_t(n))) ||
             (rc = get(ctx, S.idx, n)) || (rc = get(ctx, S.idx_tmp, n)) || (rc = get(ctx, S.flag8, n)) ||
             (rc = get(ctx, S.sbin, n)) ||
             (rc = get(ctx, S.chunk_below, ccap)) || (rc = get(ctx, S.chunk_off, ccap)) ||
@@ -1881,7 +1924,6 @@ void enqueue_binary_level(const BuildState& S, uint32_t thr, uint32_t g_node, ui
     k_part_count<<<g_chunk, BLOCK, 0, s>>>(S);
     k_part_scan<<<1, SCAN1, 0, s>>>(S);
     k_part_scatter<<<g_chunk, BLOCK, 0, s>>>(S);
-    k_part_copy<<<g_chunk, BLOCK, 0, s>>>(S);
 }
 
 // One wide level of the collapse.
@@ -1938,6 +1980,10 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
     const uint32_t thr = std::max<uint32_t>(1, leaf_threshold);
 
     k_prep<<<blocks_for(n, 256), 256, 0, s>>>(d_xyz, d_tri, n, S.tri9_src, S.pbox);
+    // the quantisation slack from every triangle's box, before the level loop permutes them
+    k_fill_u32<<<1, 64, 0, s>>>(C.slack_bits, 1, 0u);
+    k_scene_slack<<<256, 256, 0, s>>>(S.pbox, n, C.slack_bits);
+    k_slack_final<<<1, 1, 0, s>>>(C.slack_bits, C.slack);
     // roots: one per shape (forest) or one for the scene
     std::vector<uint32_t> rfirst(R + 1);
     {
@@ -1987,9 +2033,6 @@ int build_scene_gpu(Context* ctx, uint32_t leaf_threshold, bool per_shape, bool 
     C.last_levels = rb[9];
 
     // ---- wide collapse ----
-    k_fill_u32<<<1, 64, 0, s>>>(C.slack_bits, 1, 0u);
-    k_scene_slack<<<256, 256, 0, s>>>(S.pbox, n, C.slack_bits);
-    k_slack_final<<<1, 1, 0, s>>>(C.slack_bits, C.slack);
     // each wide node holds >= 2 binary nodes or is a root; the full capacity when the scene
     // buffers must hold any rebuild (graph-replayed frames)
     const uint32_t wcap = ctx->scene_full_capacity ? C.wide_cap : bin_nodes / 2 + 2 + R;
@@ -2119,6 +2162,9 @@ int build_scene_gpu_enqueue(Context* ctx, uint32_t leaf_threshold, bool per_shap
     const BuildState S = C.S;
     const uint32_t thr = std::max<uint32_t>(1, leaf_threshold);
     k_prep<<<blocks_for(n, 256), 256, 0, s>>>(ctx->d_xyz, ctx->d_tri, n, S.tri9_src, S.pbox);
+    k_fill_u32<<<1, 64, 0, s>>>(C.slack_bits, 1, 0u);
+    k_scene_slack<<<256, 256, 0, s>>>(S.pbox, n, C.slack_bits);
+    k_slack_final<<<1, 1, 0, s>>>(C.slack_bits, C.slack);
     k_init_slots<<<blocks_for(n, 256), 256, 0, s>>>(n, S.idx);
     k_init_roots<<<blocks_for(R, 256), 256, 0, s>>>(S, C.rfirst, R);
     const uint32_t g_node = node_grid(n), g_chunk = C.chunk_cap;
@@ -2127,9 +2173,6 @@ int build_scene_gpu_enqueue(Context* ctx, uint32_t leaf_threshold, bool per_shap
     k_build_gate<<<1, 1, 0, s>>>(S);
     k_finish<<<fin_grid(n), FIN_BLOCK, 0, s>>>(S, thr);
     k_finish_done<<<1, 1, 0, s>>>(S);
-    k_fill_u32<<<1, 64, 0, s>>>(C.slack_bits, 1, 0u);
-    k_scene_slack<<<256, 256, 0, s>>>(S.pbox, n, C.slack_bits);
-    k_slack_final<<<1, 1, 0, s>>>(C.slack_bits, C.slack);
     const WideState W = wide_state(S, C, sc, C.wide_cap);
     k_init_wide<<<blocks_for(R, 256), 256, 0, s>>>(W, R, S.lvl + 12);
     const uint32_t g_wide = std::min<uint32_t>(blocks_for(C.wide_cap, 128), 2048u);
