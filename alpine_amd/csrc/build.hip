@@ -225,13 +225,16 @@ __global__ __launch_bounds__(256) void k_level_init(BuildState S)
             S.fin[atomicAdd(&S.lvl[4], 1u)] = k | ((S.lvl[2] & 1u) << 31);
             continue;
         }
-        for (int a = 0; a < 3; ++a)
-        {
-            S.nbox[6 * size_t(k) + a] = ord_of(FLT_MAX);
-            S.nbox[6 * size_t(k) + 3 + a] = ord_of(-FLT_MAX);
-            S.ncbox[6 * size_t(k) + a] = ord_of(FLT_MAX);
-            S.ncbox[6 * size_t(k) + 3 + a] = ord_of(-FLT_MAX);
-        }
+        // the roots' bounds are accumulated by k_bounds_large; every later node's by the
+        // partition that placed its slots
+        if (S.lvl[2] == 0u)
+            for (int a = 0; a < 3; ++a)
+            {
+                S.nbox[6 * size_t(k) + a] = ord_of(FLT_MAX);
+                S.nbox[6 * size_t(k) + 3 + a] = ord_of(-FLT_MAX);
+                S.ncbox[6 * size_t(k) + a] = ord_of(FLT_MAX);
+                S.ncbox[6 * size_t(k) + 3 + a] = ord_of(-FLT_MAX);
+            }
         const uint32_t j = atomicAdd(listed, 1u);
         S.large[j] = k;
         // the node's histogram (its list slot); forced nodes are split positionally, unbinned
@@ -337,6 +340,7 @@ __device__ __forceinline__ bool chunk_range(const BuildState& S, uint32_t c, uin
 __global__ __launch_bounds__(BLOCK) void k_bounds_large(BuildState S)
 {
     __shared__ uint32_t part[BLOCK / 64u][12];
+    if (S.lvl[2] != 1u) return;  // past level 0 the partition accumulates the bounds
     uint32_t node, lo_i, hi_i;
     if (!chunk_range(S, blockIdx.x, node, lo_i, hi_i)) return;
     const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
@@ -460,6 +464,15 @@ __device__ __forceinline__ void alloc_children(BuildState& S, uint32_t k, uint32
     S.nfirst[l + 1] = first + nb;
     S.nflags[l] = child_fl;
     S.nflags[l + 1] = child_fl;
+    // the children's bounds accumulators (k_part_scatter fills them as it moves the slots)
+    for (uint32_t c = l; c < l + 2; ++c)
+        for (int a = 0; a < 3; ++a)
+        {
+            S.nbox[6 * size_t(c) + a] = ord_of(FLT_MAX);
+            S.nbox[6 * size_t(c) + 3 + a] = ord_of(-FLT_MAX);
+            S.ncbox[6 * size_t(c) + a] = ord_of(FLT_MAX);
+            S.ncbox[6 * size_t(c) + 3 + a] = ord_of(-FLT_MAX);
+        }
 }
 
 // A positional split (a forced node, or an SAH leaf of more than APG_MAX_LEAF triangles): the
@@ -688,6 +701,7 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(BuildState S)
 {
     constexpr uint32_t W = BLOCK / 64u;
     __shared__ uint32_t wl[ITEMS][W];
+    __shared__ uint32_t part[W][24];
     uint32_t node, lo, hi, j;
     if (!chunk_range(S, blockIdx.x, node, lo, hi, &j)) return;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x / 64u;
@@ -699,13 +713,21 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(BuildState S)
         if (lane == 0u) wl[q][wave] = uint32_t(__popcll(masks[q]));
     }
     __syncthreads();
-    const uint32_t first = S.nfirst[node];
-    const uint32_t nleft_total = S.ncount[S.nleft[node]];
+    const uint32_t first = S.nfirst[node], l = S.nleft[node];
+    const uint32_t nleft_total = S.ncount[l];
     const uint32_t buf = level_buffer(S);
     const uint32_t* idx_in = slots_of(S, buf);
     const float* box_in = boxes_of(S, buf);
     uint32_t* idx_out = buf ? S.idx : S.idx_tmp;
     float* box_out = buf ? S.pbox : S.pbox_tmp;
+    // bounds and centroid bounds of the two children (ordered uints: left 0..11, right 12..23)
+    uint32_t r[24];
+    for (int c = 0; c < 2; ++c)
+        for (int a = 0; a < 3; ++a)
+        {
+            r[12 * c + a] = ord_of(FLT_MAX); r[12 * c + 3 + a] = ord_of(-FLT_MAX);
+            r[12 * c + 6 + a] = ord_of(FLT_MAX); r[12 * c + 9 + a] = ord_of(-FLT_MAX);
+        }
     // left slots of the node in its earlier chunks
     uint32_t left = S.chunk_off[blockIdx.x] - S.chunk_off[S.large_off[j]];
     const uint64_t below_lane = (1ull << lane) - 1ull;
@@ -717,14 +739,53 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(BuildState S)
         before += uint32_t(__popcll(masks[q] & below_lane));
         if (i < hi)
         {
-            const uint32_t pos = ((masks[q] >> lane) & 1ull) ? first + before
-                                                             : first + nleft_total + (i - first - before);
+            const bool is_left = ((masks[q] >> lane) & 1ull) != 0ull;
+            const uint32_t pos = is_left ? first + before : first + nleft_total + (i - first - before);
             idx_out[pos] = idx_in[i];
-            float b[6];
-            for (int a = 0; a < 6; ++a) b[a] = box_in[6 * size_t(i) + a];
-            for (int a = 0; a < 6; ++a) box_out[6 * size_t(pos) + a] = b[a];
+            // a box is 6 floats at a multiple of 24 bytes: three 8-byte moves
+            const float2* src = reinterpret_cast<const float2*>(box_in + 6 * size_t(i));
+            const float2 b0 = src[0], b1 = src[1], b2 = src[2];
+            float2* dst = reinterpret_cast<float2*>(box_out + 6 * size_t(pos));
+            dst[0] = b0;
+            dst[1] = b1;
+            dst[2] = b2;
+            const float lo3[3] = { b0.x, b0.y, b1.x }, hi3[3] = { b1.y, b2.x, b2.y };
+            const uint32_t o = is_left ? 0u : 12u;
+            for (int a = 0; a < 3; ++a)
+            {
+                const uint32_t c = ord_of((lo3[a] + hi3[a]) * 0.5f);
+                const uint32_t vlo = ord_of(lo3[a]), vhi = ord_of(hi3[a]);
+                // both children's accumulators, the other one unchanged
+                r[a] = o ? r[a] : min(r[a], vlo);
+                r[12 + a] = o ? min(r[12 + a], vlo) : r[12 + a];
+                r[3 + a] = o ? r[3 + a] : max(r[3 + a], vhi);
+                r[15 + a] = o ? max(r[15 + a], vhi) : r[15 + a];
+                r[6 + a] = o ? r[6 + a] : min(r[6 + a], c);
+                r[18 + a] = o ? min(r[18 + a], c) : r[18 + a];
+                r[9 + a] = o ? r[9 + a] : max(r[9 + a], c);
+                r[21 + a] = o ? max(r[21 + a], c) : r[21 + a];
+            }
         }
         for (uint32_t w = 0; w < W; ++w) left += wl[q][w];
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int k = 0; k < 24; ++k)
+        {
+            const uint32_t v = uint32_t(__shfl_xor(int(r[k]), off));
+            r[k] = (k % 6) < 3 ? min(r[k], v) : max(r[k], v);
+        }
+    if (lane == 0u)
+        for (int k = 0; k < 24; ++k) part[wave][k] = r[k];
+    __syncthreads();
+    if (threadIdx.x < 24)
+    {
+        const uint32_t k = threadIdx.x;
+        uint32_t v = part[0][k];
+        for (uint32_t w = 1; w < W; ++w) v = (k % 6) < 3 ? min(v, part[w][k]) : max(v, part[w][k]);
+        const uint32_t c = l + k / 12, e = k % 12;
+        uint32_t* dst = e < 6 ? &S.nbox[6 * size_t(c) + e] : &S.ncbox[6 * size_t(c) + (e - 6)];
+        if ((e % 6) < 3) atomicMin(dst, v);
+        else atomicMax(dst, v);
     }
 }
 
