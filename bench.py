@@ -585,6 +585,36 @@ def render_pass(which, builder):
     return out
 
 
+def forest_build_benchmark(steps=20, warmup=3):
+    """Config 4's per-frame rebuild alone: the 32-scene forest (about 12 M triangles), each step
+    turning the last 25 % of every scene's triangles on the GPU and rebuilding all 32 trees in
+    one GPU forest build (leaf threshold 2), timed on the host clock around synchronised
+    steps."""
+    import torch
+    import alpine_amd as aa
+    from alpine_amd import scenes
+    B = scenes.BatchedScenes()
+    acc = aa.GpuAccelerator(0)
+    for v, f in B.meshes:
+        acc.append_mesh(v, f)
+    acc.update_scene(leaf_threshold=BATCH_LEAF_THRESHOLD, per_shape=True, stats=False)
+    rot = B.rotation(2.0 * np.pi / 120.0)
+    times = []
+    for i in range(warmup + steps):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for k, (first, count) in enumerate(B.anim_range):
+            acc.transform_vertices(k, rot, first=first, count=count)
+        acc.update_scene(leaf_threshold=BATCH_LEAF_THRESHOLD, per_shape=True, stats=False)
+        torch.cuda.synchronize()
+        if i >= warmup:
+            times.append(time.perf_counter() - a)
+    med = float(np.median(times))
+    return {"scenes": len(B.meshes), "triangles": B.triangles, "steps": steps,
+            "rebuild_ms_median": round(med * 1e3, 2), "rebuild_ms_min": round(min(times) * 1e3, 2),
+            "rebuild_mtris_per_s": round(B.triangles / med / 1e6, 1)}
+
+
 RENDER_CONFIGS = {
     "many_light_pass": lambda a: render_pass("many_light", _builder_id(a.builder)),
     "guided_pass": lambda a: render_pass("guided", _builder_id(a.builder)),
@@ -594,6 +624,7 @@ RENDER_CONFIGS = {
                                                                           tune=not a.no_autotune)},
     "guided": lambda a: {"guided_vs_plain": guided_benchmark(_builder_id(a.builder))},
     "mlp": lambda a: {"guide_mlp": mlp_benchmark()},
+    "forest_build": lambda a: {"forest_rebuild": forest_build_benchmark(a.steps, a.warmup)},
 }
 
 
