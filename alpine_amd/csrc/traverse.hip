@@ -41,6 +41,8 @@ constexpr uint32_t APG_SPILL_ALLOC = 56;  // spill entries allocated per thread 
 constexpr uint32_t TRACE_QCAP = 96;
 // Node visits per lane between the wave's fetch / triangle-round / finish steps.
 constexpr uint32_t TRACE_VISITS = 4;
+// Rays per taking-part wave: a launch over n rays runs min(grid, ceil(n / 64)) waves.
+constexpr uint32_t TRACE_WAVE_RAYS = 64;
 
 // LDS words per wave: the stack's top (2 words per entry and lane), 64 keys of
 // 2 words, 64 barycentric pairs, 64 pending counts and the triangle-group queue
@@ -130,11 +132,18 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
     uint64_t* spill = A.spill + size_t(blockIdx.x * blockDim.x + threadIdx.x) * A.spill_stride;
 
     const uint32_t nrays = A.n_dev ? *A.n_dev : A.n;
+    // Only one wave per TRACE_WAVE_RAYS rays takes part (the first ones, spread over the
+    // CUs as the blocks are): the others leave before touching the ray cursor, so a launch
+    // over few rays (a late bounce of a render, whose count only the device knows) does not
+    // serialise a claim and a finish atomic for every wave of the persistent grid.
+    const uint32_t grid_waves = gridDim.x * (blockDim.x >> 6);
+    const uint32_t want = (nrays + TRACE_WAVE_RAYS - 1u) / TRACE_WAVE_RAYS;
+    const uint32_t total_waves = want == 0u ? 1u : (want < grid_waves ? want : grid_waves);
+    if (blockIdx.x * (blockDim.x >> 6) + wave >= total_waves) return;
     const uint32_t batch = A.tri_batch;
     bool active = false;
     bool exhausted = false;  // wave-uniform: the cursor has passed the last ray
     uint32_t pool_next = 0, pool_end = 0, seen = 0;  // wave-uniform: claimed indices
-    const uint32_t total_waves = gridDim.x * (blockDim.x >> 6);
     uint32_t rid = 0;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0;
     RayBox rb{};
@@ -466,12 +475,11 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
         atomicAdd(&A.fetch_counts[0], (unsigned long long)n_node);
         atomicAdd(&A.fetch_counts[1], (unsigned long long)n_tri);
     }
-    // The last wave to finish rewinds the ray cursor for the next launch.
+    // The last taking-part wave to finish rewinds the ray cursor for the next launch.
     if (lane == 0u)
     {
-        const uint32_t waves = gridDim.x * (blockDim.x >> 6);
         __threadfence();
-        if (atomicAdd(&A.counters[2], 1u) == waves - 1u)
+        if (atomicAdd(&A.counters[2], 1u) == total_waves - 1u)
         {
             A.counters[0] = 0u;
             A.counters[2] = 0u;
