@@ -139,11 +139,17 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
     const uint32_t grid_waves = gridDim.x * (blockDim.x >> 6);
     const uint32_t want = (nrays + TRACE_WAVE_RAYS - 1u) / TRACE_WAVE_RAYS;
     const uint32_t total_waves = want == 0u ? 1u : (want < grid_waves ? want : grid_waves);
-    if (blockIdx.x * (blockDim.x >> 6) + wave >= total_waves) return;
+    const uint32_t gwave = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (gwave >= total_waves) return;
     const uint32_t batch = A.tri_batch;
     bool active = false;
     bool exhausted = false;  // wave-uniform: the cursor has passed the last ray
-    uint32_t pool_next = 0, pool_end = 0, seen = 0;  // wave-uniform: claimed indices
+    // Each wave's first pool is a static slice of half its share (whole runs of 64 rays), so
+    // the waves of a large launch do not all queue on the cursor at the start; the cursor
+    // hands out the rays after the slices.
+    const uint32_t slice = (nrays / (2u * total_waves)) & ~63u;
+    const uint32_t sliced = slice * total_waves;
+    uint32_t pool_next = gwave * slice, pool_end = pool_next + slice, seen = sliced;  // wave-uniform
     uint32_t rid = 0;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0;
     RayBox rb{};
@@ -178,7 +184,7 @@ __global__ __launch_bounds__(256, 6) void trace_kernel(TraceArgs A)
                 chunk = chunk < 64u ? 64u : (chunk > 512u ? 512u : chunk);
                 uint32_t base = 0;
                 if (lane == 0u) base = atomicAdd(&A.counters[0], chunk);
-                base = uint32_t(__builtin_amdgcn_readlane(int(base), 0));
+                base = sliced + uint32_t(__builtin_amdgcn_readlane(int(base), 0));
                 seen = base + chunk;
                 pool_next = base < nrays ? base : nrays;
                 pool_end = seen < nrays ? seen : nrays;
