@@ -390,10 +390,11 @@ __device__ __forceinline__ float box_area(const float lo[3], const float hi[3])
     return (x * y + y * z + z * x) * 2.0f;
 }
 
-// 9-bin histograms (box + count) per axis of a large SAH candidate's chunk: one LDS histogram
-// per quarter wave (lanes that share a bin then hit four addresses, not one), merged into the
+// 9-bin histograms (box + count) per axis of a large SAH candidate's chunk: eight LDS
+// histograms per wave (lane % 8: lanes that share a bin then hit eight addresses, not one),
+// merged into the
 // node's global histogram.
-constexpr uint32_t HCOPIES = 4;  // histogram copies per wave, by lane % HCOPIES
+constexpr uint32_t HCOPIES = 8;  // histogram copies per wave, by lane % HCOPIES
 __global__ __launch_bounds__(BLOCK) void k_bin_large(BuildState S)
 {
     constexpr uint32_t NH = (BLOCK / 64u) * HCOPIES;
