@@ -1147,12 +1147,23 @@ __device__ void fin_wave_node(BuildState& S, FinShared& L, uint32_t thr, uint32_
     }
 }
 
+// Exchange a word with the other lane of the pair (lanes 2k and 2k + 1): DPP quad_perm [1,0,3,2].
+__device__ __forceinline__ uint32_t pair_swap(uint32_t v)
+{
+    return uint32_t(__builtin_amdgcn_mov_dpp(int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float pair_swap(float v) { return __uint_as_float(pair_swap(__float_as_uint(v))); }
+
 // Sub-level step 2: the best split of one axis of a tiny candidate, from the slots directly
-// (each split's below and above boxes and counts; the sweep's strict-< rule within the axis).
-__device__ void fin_tiny_axis(FinShared& L, uint32_t c, uint32_t d)
+// (each split's below and above boxes and counts; the sweep's strict-< rule within the axis),
+// by the lane pair 2k, 2k + 1: each lane bins half of the slots, the pair merges the bins, the
+// even lane prices the 8 below boxes and the odd lane the 8 above boxes, and with the halves
+// exchanged both lanes hold every cost (0 + below term) + above term.  Both lanes of a pair
+// are active together (the callers hand out pairs).
+__device__ void fin_tiny_axis(FinShared& L, uint32_t c, uint32_t d, uint32_t half)
 {
     const FinNode e = L.cand[c];
-    const uint32_t f = fin_first(e), cnt = fin_count(e);
+    const uint32_t cnt = fin_count(e), f = fin_first(e);
     // the centroid range on axis d, left by step 1 (ccost holds the minimum until this step)
     const float mn = L.ccost[c][d], mx = L.ccmx[c][d];
     float min_cost = FLT_MAX;
@@ -1160,9 +1171,8 @@ __device__ void fin_tiny_axis(FinShared& L, uint32_t c, uint32_t d)
     bool found = false;
     if (mn != mx)
     {
-        // the slots' 9 bin boxes and counts, then each split's below box over bins 0..s and
-        // above box over bins 8..s+1 (min / max are exact: the boxes equal those the slots
-        // give directly, up to the sign of zero, which no area comparison sees)
+        // the slots' 9 bin boxes and counts (min / max are exact: the boxes equal those the
+        // slots give directly, up to the sign of zero, which no area comparison sees)
         float lo_[NBINS][3], hi_[NBINS][3];
         uint32_t nbin[NBINS];
 #pragma unroll
@@ -1171,7 +1181,9 @@ __device__ void fin_tiny_axis(FinShared& L, uint32_t c, uint32_t d)
             nbin[q] = 0;
             for (int a = 0; a < 3; ++a) { lo_[q][a] = FLT_MAX; hi_[q][a] = -FLT_MAX; }
         }
-        for (uint32_t j = 0; j < cnt; ++j)
+        const uint32_t split = (cnt + 1u) / 2u;
+        const uint32_t j0 = half ? split : 0u, j1 = half ? cnt : split;
+        for (uint32_t j = j0; j < j1; ++j)
         {
             const float* b = L.box[L.lid[f + j]];
             float lo[3], hi[3];
@@ -1192,42 +1204,66 @@ __device__ void fin_tiny_axis(FinShared& L, uint32_t c, uint32_t d)
                 }
             }
         }
-        float cost[NSPLITS];
+        // the pair's bins
+#pragma unroll
+        for (uint32_t q = 0; q < NBINS; ++q)
+        {
+            nbin[q] += pair_swap(nbin[q]);
+            for (int a = 0; a < 3; ++a)
+            {
+                lo_[q][a] = min_s(lo_[q][a], pair_swap(lo_[q][a]));
+                hi_[q][a] = max_s(hi_[q][a], pair_swap(hi_[q][a]));
+            }
+        }
         uint32_t bn[NSPLITS];
         {
-            float plo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, phi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
             uint32_t pn = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < NSPLITS; ++q) { pn += nbin[q]; bn[q] = pn; }
+        }
+        // this lane's terms: below (even lane) or above (odd lane)
+        float part[NSPLITS];
+        float plo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, phi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+        if (!half)
+        {
 #pragma unroll
             for (uint32_t q = 0; q < NSPLITS; ++q)
             {
                 for (int a = 0; a < 3; ++a) { plo[a] = min_s(plo[a], lo_[q][a]); phi[a] = max_s(phi[a], hi_[q][a]); }
-                pn += nbin[q];
-                bn[q] = pn;
-                cost[q] = 0.0f;
-                cost[q] += float(pn) * box_area(plo, phi);
+                part[q] = float(bn[q]) * box_area(plo, phi);
             }
         }
+        else
         {
-            float plo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, phi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
 #pragma unroll
             for (int q = NSPLITS - 1; q >= 0; --q)
             {
                 for (int a = 0; a < 3; ++a) { plo[a] = min_s(plo[a], lo_[q + 1][a]); phi[a] = max_s(phi[a], hi_[q + 1][a]); }
-                cost[q] += float(cnt - bn[q]) * box_area(plo, phi);
+                part[q] = float(cnt - bn[q]) * box_area(plo, phi);
             }
         }
 #pragma unroll
         for (uint32_t q = 0; q < NSPLITS; ++q)
-            if (cost[q] < min_cost)
+        {
+            const float other = pair_swap(part[q]);
+            const float below = half ? other : part[q], above = half ? part[q] : other;
+            float cost = 0.0f;
+            cost += below;
+            cost += above;
+            if (cost < min_cost)
             {
-                min_cost = cost[q];
+                min_cost = cost;
                 best_idx = q;
                 best_below = bn[q];
                 found = true;
             }
+        }
     }
-    L.ccost[c][d] = found ? min_cost : __builtin_inff();
-    L.cinfo[c][d] = uint16_t(best_idx | (best_below << 8));
+    if (!half)
+    {
+        L.ccost[c][d] = found ? min_cost : __builtin_inff();
+        L.cinfo[c][d] = uint16_t(best_idx | (best_below << 8));
+    }
 }
 
 // Sub-level step 3: the choice across axes (in axis order, strict <), the leaf test and the
@@ -1375,7 +1411,9 @@ __global__ __launch_bounds__(FIN_BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)
             for (uint32_t c = tid; c < cn; c += FIN_BLOCK)
                 L.order[atomicAdd(&L.csize[fin_count(L.cand[c])], 1u)] = uint16_t(c);
             __syncthreads();
-            for (uint32_t u = tid; u < 3 * cn; u += FIN_BLOCK) fin_tiny_axis(L, L.order[u / 3], u % 3);
+            // lane pairs: both lanes of a pair fall in the same pass (FIN_BLOCK and 6 cn are even)
+            for (uint32_t u = tid; u < 6 * cn; u += FIN_BLOCK)
+                fin_tiny_axis(L, L.order[(u >> 1) / 3], (u >> 1) % 3, u & 1u);
             __syncthreads();
             for (uint32_t c = tid; c < cn; c += FIN_BLOCK) fin_tiny_choose(S, L, gfirst, L.order[c], tc ^ 1u);
             __syncthreads();
