@@ -962,28 +962,6 @@ __device__ __forceinline__ uint32_t fin_children(BuildState& S, FinShared& L, ui
     return l;
 }
 
-// Bounds and centroid bounds (ordered uints) of slots [f, f + cnt) by one thread.
-__device__ __forceinline__ void fin_bounds_serial(const FinShared& L, uint32_t f, uint32_t cnt, uint32_t r[12])
-{
-    const uint32_t lo_ord = ord_of(FLT_MAX), hi_ord = ord_of(-FLT_MAX);
-    for (int a = 0; a < 3; ++a)
-    {
-        r[a] = lo_ord; r[3 + a] = hi_ord; r[6 + a] = lo_ord; r[9 + a] = hi_ord;
-    }
-    for (uint32_t j = 0; j < cnt; ++j)
-    {
-        const float* b = L.box[L.lid[f + j]];
-        for (int a = 0; a < 3; ++a)
-        {
-            const float lo = b[a], hi = b[3 + a], c = (lo + hi) * 0.5f;
-            r[a] = min(r[a], ord_of(lo));
-            r[3 + a] = max(r[3 + a], ord_of(hi));
-            r[6 + a] = min(r[6 + a], ord_of(c));
-            r[9 + a] = max(r[9 + a], ord_of(c));
-        }
-    }
-}
-
 __device__ __forceinline__ void fin_unpack_bounds(const uint32_t r[12], float nlo[3], float nhi[3], float cmn[3],
                                                   float cmx[3])
 {
@@ -1154,6 +1132,32 @@ __device__ __forceinline__ uint32_t pair_swap(uint32_t v)
 }
 __device__ __forceinline__ float pair_swap(float v) { return __uint_as_float(pair_swap(__float_as_uint(v))); }
 
+// Bounds and centroid bounds (ordered uints) of a tiny node's slots [f, f + cnt) by a lane pair:
+// the even lane reduces the first half, the odd lane the second, and both end with the node's.
+__device__ void fin_pair_bounds(const FinShared& L, uint32_t f, uint32_t cnt, uint32_t half, uint32_t r[12])
+{
+    const uint32_t split = (cnt + 1u) / 2u;
+    const uint32_t j0 = half ? split : 0u, j1 = half ? cnt : split;
+    ord_bounds_init(r);
+    for (uint32_t j = j0; j < j1; ++j)
+    {
+        const float* b = L.box[L.lid[f + j]];
+        for (int a = 0; a < 3; ++a)
+        {
+            const float lo = b[a], hi = b[3 + a], cc = (lo + hi) * 0.5f;
+            r[a] = min(r[a], ord_of(lo));
+            r[3 + a] = max(r[3 + a], ord_of(hi));
+            r[6 + a] = min(r[6 + a], ord_of(cc));
+            r[9 + a] = max(r[9 + a], ord_of(cc));
+        }
+    }
+    for (int i = 0; i < 12; ++i)
+    {
+        const uint32_t o = pair_swap(r[i]);
+        r[i] = (i % 6) < 3 ? min(r[i], o) : max(r[i], o);
+    }
+}
+
 // Sub-level step 2: the best split of one axis of a tiny candidate, from the slots directly
 // (each split's below and above boxes and counts; the sweep's strict-< rule within the axis),
 // by the lane pair 2k, 2k + 1: each lane bins half of the slots, the pair merges the bins, the
@@ -1267,13 +1271,18 @@ __device__ void fin_tiny_axis(FinShared& L, uint32_t c, uint32_t d, uint32_t hal
 }
 
 // Sub-level step 3: the choice across axes (in axis order, strict <), the leaf test and the
-// stable partition of one tiny candidate, by one thread.
-__device__ void fin_tiny_choose(BuildState& S, FinShared& L, uint32_t gfirst, uint32_t c, uint32_t tnext)
+// stable partition of one tiny candidate, by the lane pair 2k, 2k + 1: each lane reduces the
+// bounds of half of the slots (the pair merges them), both make the same choice, and each lane
+// places its half's slots (the odd lane after the even lane's left and right ones).
+__device__ void fin_tiny_choose(BuildState& S, FinShared& L, uint32_t gfirst, uint32_t c, uint32_t tnext,
+                                uint32_t half)
 {
     const FinNode e = L.cand[c];
     const uint32_t k = e.node, f = fin_first(e), cnt = fin_count(e);
+    const uint32_t split = (cnt + 1u) / 2u;
+    const uint32_t j0 = half ? split : 0u, j1 = half ? cnt : split;
     uint32_t r[12];
-    fin_bounds_serial(L, f, cnt, r);
+    fin_pair_bounds(L, f, cnt, half, r);
     float nlo[3], nhi[3], cmn[3], cmx[3];
     fin_unpack_bounds(r, nlo, nhi, cmn, cmx);
     float min_cost = FLT_MAX;
@@ -1291,24 +1300,36 @@ __device__ void fin_tiny_choose(BuildState& S, FinShared& L, uint32_t gfirst, ui
     FinSplit sp;
     fin_choose(cnt, found, min_cost, float(cnt) * box_area(nlo, nhi), best_dim, info & 0xFFu, info >> 8, cmn, cmx,
                fl, sp);
-    S.nflags[k] = fl;
+    if (!half) S.nflags[k] = fl;
     if (sp.mode == 0) return;
     const uint32_t nb = sp.nb;
     if (sp.mode == 2)
     {
-        uint32_t nl = 0, nr = 0;
-        for (uint32_t j = 0; j < cnt; ++j)
+        uint32_t flags = 0, nl = 0;
+        for (uint32_t j = j0; j < j1; ++j)
+        {
+            const float* b = L.box[L.lid[f + j]];
+            const bool left = bin_index((b[sp.d] + b[3 + sp.d]) * 0.5f, sp.mn, sp.mx) <= sp.si;
+            flags |= (left ? 1u : 0u) << (j - j0);
+            nl += left ? 1u : 0u;
+        }
+        const uint32_t nl_even = pair_swap(nl);  // the odd lane's view: the even lane's left count
+        uint32_t wl = half ? nl_even : 0u, wr = half ? split - nl_even : 0u;
+        for (uint32_t j = j0; j < j1; ++j)
         {
             const uint16_t v = L.lid[f + j];
-            const float* b = L.box[v];
-            if (bin_index((b[sp.d] + b[3 + sp.d]) * 0.5f, sp.mn, sp.mx) <= sp.si) L.tmp[f + nl++] = v;
-            else L.tmp[f + nb + nr++] = v;
+            if ((flags >> (j - j0)) & 1u) L.tmp[f + wl++] = v;
+            else L.tmp[f + nb + wr++] = v;
         }
-        for (uint32_t j = 0; j < cnt; ++j) L.lid[f + j] = L.tmp[f + j];
+        wave_sync_lds();
+        for (uint32_t j = j0; j < j1; ++j) L.lid[f + j] = L.tmp[f + j];
     }
-    const uint32_t l = fin_children(S, L, k, gfirst, f, cnt, nb);
-    fin_push(L, 0, tnext, fin_node(l, f, nb, sp.mode == 1));
-    fin_push(L, 0, tnext, fin_node(l + 1, f + nb, cnt - nb, sp.mode == 1));
+    if (!half)
+    {
+        const uint32_t l = fin_children(S, L, k, gfirst, f, cnt, nb);
+        fin_push(L, 0, tnext, fin_node(l, f, nb, sp.mode == 1));
+        fin_push(L, 0, tnext, fin_node(l + 1, f + nb, cnt - nb, sp.mode == 1));
+    }
 }
 
 // Grid-stride over the nodes the level loop handed over, taken in turn from a device cursor.
@@ -1368,12 +1389,14 @@ __global__ __launch_bounds__(FIN_BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)
         {
             const uint32_t tn = L.tn[tc];
             if (tn == 0) break;
-            for (uint32_t t = tid; t < tn; t += FIN_BLOCK)
+            // a lane pair per tiny node for its bounds; the even lane does the rest
+            for (uint32_t u = tid; u < 2 * tn; u += FIN_BLOCK)
             {
-                const FinNode e = L.tiny[tc][t];
+                const FinNode e = L.tiny[tc][u >> 1];
                 const uint32_t k = e.node, f = fin_first(e), c = fin_count(e);
                 uint32_t r[12];
-                fin_bounds_serial(L, f, c, r);
+                fin_pair_bounds(L, f, c, u & 1u, r);
+                if (u & 1u) continue;
                 for (int j = 0; j < 6; ++j) S.nbox[6 * size_t(k) + j] = r[j];
                 float nlo[3], nhi[3], cmn[3], cmx[3];
                 fin_unpack_bounds(r, nlo, nhi, cmn, cmx);
@@ -1415,7 +1438,8 @@ __global__ __launch_bounds__(FIN_BLOCK) __attribute__((amdgpu_waves_per_eu(4, 4)
             for (uint32_t u = tid; u < 6 * cn; u += FIN_BLOCK)
                 fin_tiny_axis(L, L.order[(u >> 1) / 3], (u >> 1) % 3, u & 1u);
             __syncthreads();
-            for (uint32_t c = tid; c < cn; c += FIN_BLOCK) fin_tiny_choose(S, L, gfirst, L.order[c], tc ^ 1u);
+            for (uint32_t u = tid; u < 2 * cn; u += FIN_BLOCK)
+                fin_tiny_choose(S, L, gfirst, L.order[u >> 1], tc ^ 1u, u & 1u);
             __syncthreads();
             if (tid == 0)
             {
