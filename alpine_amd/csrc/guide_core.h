@@ -89,13 +89,16 @@ SHD VmfMix guide_decode(const float* o)
 {
     VmfMix m;
     float amax = -FLT_MAX;
+#pragma unroll
     for (int i = 0; i < GUIDE_LOBES; ++i) amax = fmaxf(amax, o[5 * i]);
     float sum = 0.0f;
+#pragma unroll
     for (int i = 0; i < GUIDE_LOBES; ++i)
     {
         m.pi[i] = expf(o[5 * i] - amax);
         sum += m.pi[i];
     }
+#pragma unroll
     for (int i = 0; i < GUIDE_LOBES; ++i)
     {
         m.pi[i] /= sum;
@@ -116,6 +119,8 @@ SHD float vmf_log(float kappa, float c)
 SHD float guide_pdf(const VmfMix& m, f3 w)
 {
     float q = 0.0f;
+    // unrolled: a loop index into the mixture's arrays would move them to per-lane memory
+#pragma unroll
     for (int i = 0; i < GUIDE_LOBES; ++i) q += m.pi[i] * expf(vmf_log(m.kappa[i], dot(m.mu[i], w)));
     return q;
 }
@@ -124,20 +129,31 @@ SHD float guide_pdf(const VmfMix& m, f3 w)
 // sampling of the von Mises-Fisher distribution on S^2").
 SHD f3 guide_sample(const VmfMix& m, float u0, float u1, float u2)
 {
-    int lobe = GUIDE_LOBES - 1;
+    // the first lobe whose running weight passes u0 (the last if none does), picked by
+    // selects over the unrolled lobes: an index into the mixture's arrays would move them
+    // from registers to per-lane memory
+    float k = m.kappa[GUIDE_LOBES - 1];
+    float mx = m.mu[GUIDE_LOBES - 1].x, my = m.mu[GUIDE_LOBES - 1].y, mz = m.mu[GUIDE_LOBES - 1].z;
     float acc = 0.0f;
+    bool taken = false;
+#pragma unroll
     for (int i = 0; i < GUIDE_LOBES; ++i)
     {
         acc += m.pi[i];
-        if (u0 < acc) { lobe = i; break; }
+        const bool take = !taken && u0 < acc;
+        k = take ? m.kappa[i] : k;
+        mx = take ? m.mu[i].x : mx;
+        my = take ? m.mu[i].y : my;
+        mz = take ? m.mu[i].z : mz;
+        taken = taken || take;
     }
-    const float k = m.kappa[lobe];
+    const f3 mu = mk(mx, my, mz);
     const float w = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * k)) / k;
     const float s = sqrtf(max_s(0.0f, 1.0f - w * w));
     const float phi = 2.0f * kPi * u2;
     f3 b1, b2;
-    get_basis(m.mu[lobe], b1, b2);
-    return normalize(b1 * (s * cosf(phi)) + b2 * (s * sinf(phi)) + m.mu[lobe] * w);
+    get_basis(mu, b1, b2);
+    return normalize(b1 * (s * cosf(phi)) + b2 * (s * sinf(phi)) + mu * w);
 }
 
 // Gradient of -log q(w) with respect to the 40 outputs, scaled by `scale`.
