@@ -179,9 +179,9 @@ def _render_setup(sc, builder):
     return acc
 
 
-def _timed_render(acc, sc, spp, seed, **kw):
+def _timed_render(acc, sc, spp, seed, slots=1, **kw):
     import torch
-    acc.render_reset(sc.width, sc.height, seed)
+    acc.render_reset(sc.width, sc.height, seed, slots=slots)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     acc.render(spp, sc.max_depth, sc.rr_depth, sc.background, **kw)
@@ -245,16 +245,20 @@ def many_light_benchmark(builder, tune=True):
     sc = scenes.many_light_scene()
     acc = _render_setup(sc, builder)
     launch = None
+    sl = 1
     if tune:
         from alpine_amd import autotune
+        # samples in flight per pixel too: the late bounces of a pass hold few paths, so more
+        # samples per pass keep those launches fuller
         launch, _ = autotune.autotune_render(acc, sc.width, sc.height, 4, sc.max_depth, sc.rr_depth,
-                                             sc.background, log=log)
-    _timed_render(acc, sc, 1, 0)
-    dt, img, _ = _timed_render(acc, sc, sc.spp, sc.seed)
+                                             sc.background, slots=ML_SLOTS, log=log)
+        sl = launch["slots"]
+    _timed_render(acc, sc, sl, 0, slots=sl)
+    dt, img, _ = _timed_render(acc, sc, sc.spp, sc.seed, slots=sl)
     c = acc.render_counters()
     # convergence: the same estimator at 16 spp and a 1024-spp FP64 reference (other seeds)
-    _, img16, _ = _timed_render(acc, sc, 16, sc.seed + 7)
-    _, _, ref64 = _timed_render(acc, sc, ML_REF_SPP, 1000, fp64=True, graph=True)
+    _, img16, _ = _timed_render(acc, sc, 16, sc.seed + 7, slots=sl)
+    _, _, ref64 = _timed_render(acc, sc, ML_REF_SPP, 1000, slots=sl, fp64=True, graph=True)
     ref = ref64 / ML_REF_SPP
     bps = render_bytes_per_sample(acc, sc)
     sps = sc.width * sc.height * sc.spp / dt
@@ -353,6 +357,8 @@ def mlp_benchmark(n_infer=1 << 20, n_train=1 << 19, reps=10):
 GUIDE_PLAIN_SPP, GUIDE_REF_SPP, GUIDE_SLOTS, GUIDE_CHUNK = 1024, 16384, 4, 32
 # FP64 references of the many-light frame and of the batched frame
 ML_REF_SPP, BATCH_REF_SPP = 1024, 256
+ML_SLOTS = (1, 2, 4)  # samples in flight per pixel the many-light config's autotuning tries
+BATCH_SLOTS = (1, 2)  # and the batched config's (4096 x 2048 paths per slot)
 GUIDE_TRAIN_SPP, GUIDE_STEPS, GUIDE_FRAC, GUIDE_SAMPLES = 64, 64, 0.5, 1 << 19
 # Adam learning rates: the workload's stated 1e-3 (the headline figure), and 5e-3, the best
 # of the settings swept on this scene, reported beside it
@@ -464,9 +470,12 @@ def batched_benchmark(frames=60, spp=32, max_depth=5, rr_depth=3, seed=1, tune=T
     acc.set_tiles([aa.camera(**c) for c in B.cams], B.tile, B.tile, roots=np.arange(len(B.cams)))
     W, H = B.tile * B.tiles_x, B.tile * B.tiles_y
     launch = None
+    sl = 1
     if tune:
         from alpine_amd import autotune
-        launch, _ = autotune.autotune_render(acc, W, H, 4, max_depth, rr_depth, log=log, graph=True)
+        launch, _ = autotune.autotune_render(acc, W, H, 4, max_depth, rr_depth, slots=BATCH_SLOTS, log=log,
+                                             graph=True)
+        sl = launch["slots"]
     rot = B.rotation(2.0 * np.pi / 120.0)
     xf = [(k, first, count, rot) for k, (first, count) in enumerate(B.anim_range)]
     # the stage split, from eager frames: transforms + rebuild, then reset + render
@@ -479,7 +488,7 @@ def batched_benchmark(frames=60, spp=32, max_depth=5, rr_depth=3, seed=1, tune=T
         acc.update_scene(leaf_threshold=BATCH_LEAF_THRESHOLD, per_shape=True, stats=False)
         torch.cuda.synchronize()
         b = time.perf_counter()
-        acc.render_reset(W, H, seed + f)
+        acc.render_reset(W, H, seed + f, slots=sl)
         acc.render(spp, max_depth, rr_depth, graph=True)
         torch.cuda.synchronize()
         c = time.perf_counter()
@@ -498,7 +507,7 @@ def batched_benchmark(frames=60, spp=32, max_depth=5, rr_depth=3, seed=1, tune=T
     eager_frames, captures, replays = acc.frame_counts()
     img, _, total = acc.render_read()
     # convergence of the last frame: a BATCH_REF_SPP FP64 reference of the same geometry
-    acc.render_reset(W, H, 5000)
+    acc.render_reset(W, H, 5000, slots=sl)
     acc.render(BATCH_REF_SPP, max_depth, rr_depth, fp64=True, graph=True)
     _, ref64, _ = acc.render_read(fp64=True)
     rel = scenes.rel_mse(img / total, ref64 / BATCH_REF_SPP)
@@ -510,7 +519,7 @@ def batched_benchmark(frames=60, spp=32, max_depth=5, rr_depth=3, seed=1, tune=T
         for k, first, count, m in xf:
             acc.transform_vertices(k, m, first=first, count=count)
         acc.update_scene(leaf_threshold=BATCH_LEAF_THRESHOLD, per_shape=True, stats=False)
-        acc.render_reset(W, H, seed + 1000 + f)
+        acc.render_reset(W, H, seed + 1000 + f, slots=sl)
         acc.render(spp, max_depth, rr_depth, graph=True)
         torch.cuda.synchronize()
         after.append(time.perf_counter() - a)
