@@ -250,8 +250,8 @@ def many_light_benchmark(builder, tune=True):
         from alpine_amd import autotune
         # samples in flight per pixel too: the late bounces of a pass hold few paths, so more
         # samples per pass keep those launches fuller
-        launch, _ = autotune.autotune_render(acc, sc.width, sc.height, 4, sc.max_depth, sc.rr_depth,
-                                             sc.background, slots=ML_SLOTS, log=log)
+        launch, _ = autotune.autotune_render(acc, sc.width, sc.height, max(ML_SLOTS), sc.max_depth,
+                                             sc.rr_depth, sc.background, slots=ML_SLOTS, log=log)
         sl = launch["slots"]
     _timed_render(acc, sc, sl, 0, slots=sl)
     dt, img, _ = _timed_render(acc, sc, sc.spp, sc.seed, slots=sl)
@@ -357,7 +357,7 @@ def mlp_benchmark(n_infer=1 << 20, n_train=1 << 19, reps=10):
 GUIDE_PLAIN_SPP, GUIDE_REF_SPP, GUIDE_SLOTS, GUIDE_CHUNK = 1024, 16384, 4, 32
 # FP64 references of the many-light frame and of the batched frame
 ML_REF_SPP, BATCH_REF_SPP = 1024, 256
-ML_SLOTS = (1, 2, 4)  # samples in flight per pixel the many-light config's autotuning tries
+ML_SLOTS = (1, 2, 4, 8)  # samples in flight per pixel the many-light config's autotuning tries
 BATCH_SLOTS = (1, 2)  # and the batched config's (4096 x 2048 paths per slot)
 GUIDE_TRAIN_SPP, GUIDE_STEPS, GUIDE_FRAC, GUIDE_SAMPLES = 64, 64, 0.5, 1 << 19
 # Adam learning rates: the workload's stated 1e-3 (the headline figure), and 5e-3, the best
