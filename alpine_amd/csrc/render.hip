@@ -235,6 +235,35 @@ __device__ __forceinline__ uint32_t wave_scan(uint32_t x)
     return x;
 }
 
+// The shade loop's 256 entries of one pass of a block, ordered by a small key (0..SORT_KEYS-1,
+// stable within a key): returns the entry (0..255) this thread takes.  With the key the
+// branch the shading takes (a miss, a conductor, a Lambert or a coated dielectric), the lanes
+// of a wave mostly run the same material code instead of every branch at part occupancy.
+// The order changes no result: each path keeps its own state and sample sequence.
+constexpr uint32_t SORT_KEYS = 5;
+__device__ __forceinline__ uint32_t block_sort_keys(uint32_t key)
+{
+    __shared__ uint32_t s_kc[SORT_KEYS][4];
+    __shared__ uint16_t s_src[256];
+    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint64_t mine = 0;
+    for (uint32_t k = 0; k < SORT_KEYS; ++k)
+    {
+        const uint64_t m = __ballot(key == k);
+        if (key == k) mine = m;
+        if (lane == 0u) s_kc[k][w] = uint32_t(__popcll(m));
+    }
+    __syncthreads();
+    uint32_t off = 0;
+    for (uint32_t k = 0; k < SORT_KEYS; ++k)
+        for (uint32_t v = 0; v < 4u; ++v)
+            if (k < key || (k == key && v < w)) off += s_kc[k][v];
+    s_src[off + uint32_t(__popcll(mine & below))] = uint16_t(threadIdx.x);
+    __syncthreads();
+    return s_src[threadIdx.x];
+}
+
 // Block-aggregated appends for k_shade: the shadow queue takes both NEE rays of
 // a path (a then b), the continuation queue one ray.  Device-scope atomics on
 // one address serialise across the chip (~11 ns each), so the whole 256-thread
@@ -401,7 +430,19 @@ __global__ __launch_bounds__(256, GUIDED ? 1 : 4) void k_shade(ShadeArgs A)
     uint32_t buf = 0;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride)
     {
-        const uint32_t q = base + threadIdx.x;
+        // this pass's entries ordered by the material branch they take (entries past the end last)
+        uint32_t key = SORT_KEYS - 1u;
+        if (base + threadIdx.x < count)
+        {
+            const uint32_t pr = A.hprim[base + threadIdx.x];
+            if (pr == 0xFFFFFFFFu) key = 0u;
+            else
+            {
+                const DevMaterial& mm = A.mats[A.tri_mat[pr]];
+                key = mm.kind == MAT_CONDUCTOR ? 1u : (mm.nbsdf > 1u ? 3u : 2u);
+            }
+        }
+        const uint32_t q = base + block_sort_keys(key);
         const bool valid = q < count;
         bool emit_a = false, emit_b = false, emit_c = false;
         f3 sa_o{}, sa_d{}, sb_d{}, c_o{}, c_d{};
