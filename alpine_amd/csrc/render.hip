@@ -563,7 +563,7 @@ __global__ __launch_bounds__(256, GUIDED ? 1 : 4) void k_shade(ShadeArgs A)
                 if (GUIDED)
                 {
                     // one-sample MIS of BSDF sampling and the learned proposal (true densities)
-                    vr = A.vrec + (size_t(p) * A.vrec_depth + A.bounce) * VREC;
+                    if (A.bounce < A.vrec_depth) vr = A.vrec + (size_t(p) * A.vrec_depth + A.bounce) * VREC;
                     const VmfMix mix = guide_decode(A.gout + size_t(q) * GUIDE_OUT);
                     const float c = A.guide_frac;
                     if (get1d(s) < c)
@@ -588,12 +588,15 @@ __global__ __launch_bounds__(256, GUIDED ? 1 : 4) void k_shade(ShadeArgs A)
                         ok = wi.z > 0.0f && pm > 0.0f;
                         if (ok) est = f * wi.z / pm;
                     }
-                    const f3 p01 = to_unit_box(A, hit);
-                    vr[0] = p01.x; vr[1] = p01.y; vr[2] = p01.z;
-                    vr[3] = ns.x; vr[4] = ns.y; vr[5] = ns.z;
-                    vr[6] = -rd.x; vr[7] = -rd.y; vr[8] = -rd.z;
-                    vr[15] = 0.0f; vr[16] = 0.0f; vr[17] = 0.0f;
-                    vr[18] = 0.0f;
+                    if (vr)
+                    {
+                        const f3 p01 = to_unit_box(A, hit);
+                        vr[0] = p01.x; vr[1] = p01.y; vr[2] = p01.z;
+                        vr[3] = ns.x; vr[4] = ns.y; vr[5] = ns.z;
+                        vr[6] = -rd.x; vr[7] = -rd.y; vr[8] = -rd.z;
+                        vr[15] = 0.0f; vr[16] = 0.0f; vr[17] = 0.0f;
+                        vr[18] = 0.0f;
+                    }
                 }
                 else
                 {
@@ -620,7 +623,7 @@ __global__ __launch_bounds__(256, GUIDED ? 1 : 4) void k_shade(ShadeArgs A)
                             c_o = hit + ng * kRayOffset;
                             c_d = wiw;
                             st = (st & ~ST_DEPTH) | (depth + 1);
-                            if (GUIDED)
+                            if (GUIDED && vr)
                             {
                                 vr[9] = wiw.x; vr[10] = wiw.y; vr[11] = wiw.z;
                                 vr[12] = est.x; vr[13] = est.y; vr[14] = est.z;
@@ -1272,8 +1275,10 @@ void fill_shade_args(RenderState* R, const apg_render_params* prm, ShadeArgs& S)
     S.aov_albedo = prm->aovs ? R->aov_albedo : nullptr;
     S.aov_normal = prm->aovs ? R->aov_normal : nullptr;
     S.gout = R->gout;
-    S.vrec = R->vrec;
-    S.vrec_depth = R->vrec_depth;
+    // only training passes record path vertices (the resolve stage reads their radiance
+    // targets back); plain and frozen-proposal passes write none
+    S.vrec = prm->guide == 1 ? R->vrec : nullptr;
+    S.vrec_depth = prm->guide == 1 ? R->vrec_depth : 0u;
     for (int a = 0; a < 3; ++a)
     {
         S.lo[a] = R->scene_lo[a];
