@@ -569,10 +569,10 @@ def render_pass(which, builder):
     acc = _render_setup(sc, builder)
     out = {"config": which, "frame": [sc.width, sc.height]}
     if which == "many_light":
-        # 4 samples in flight per pixel, the slots the bench's autotuning picks for this frame
-        _timed_render(acc, sc, 4, 0, slots=4)
-        dt, img, _ = _timed_render(acc, sc, sc.spp, sc.seed, slots=4)
-        out.update(spp=sc.spp, slots=4, seconds=round(dt, 4),
+        # 8 samples in flight per pixel, the slots the bench's autotuning picks for this frame
+        _timed_render(acc, sc, 8, 0, slots=8)
+        dt, img, _ = _timed_render(acc, sc, sc.spp, sc.seed, slots=8)
+        out.update(spp=sc.spp, slots=8, seconds=round(dt, 4),
                    samples_per_s=round(sc.width * sc.height * sc.spp / dt, 1))
         return out
     acc.render_reset(sc.width, sc.height, 11, slots=GUIDE_SLOTS)
