@@ -358,7 +358,7 @@ GUIDE_PLAIN_SPP, GUIDE_REF_SPP, GUIDE_SLOTS, GUIDE_CHUNK = 1024, 16384, 4, 32
 # FP64 references of the many-light frame and of the batched frame
 ML_REF_SPP, BATCH_REF_SPP = 1024, 256
 ML_SLOTS = (1, 2, 4, 8)  # samples in flight per pixel the many-light config's autotuning tries
-BATCH_SLOTS = (1, 2)  # and the batched config's (4096 x 2048 paths per slot)
+BATCH_SLOTS = (2, 4)  # and the batched config's (4096 x 2048 paths per slot)
 GUIDE_TRAIN_SPP, GUIDE_STEPS, GUIDE_FRAC, GUIDE_SAMPLES = 64, 64, 0.5, 1 << 19
 # Adam learning rates: the workload's stated 1e-3 (the headline figure), and 5e-3, the best
 # of the settings swept on this scene, reported beside it
