@@ -357,7 +357,7 @@ def mlp_benchmark(n_infer=1 << 20, n_train=1 << 19, reps=10):
 GUIDE_PLAIN_SPP, GUIDE_REF_SPP, GUIDE_SLOTS, GUIDE_CHUNK = 1024, 16384, 4, 32
 # FP64 references of the many-light frame and of the batched frame
 ML_REF_SPP, BATCH_REF_SPP = 1024, 256
-ML_SLOTS = (1, 2, 4, 8)  # samples in flight per pixel the many-light config's autotuning tries
+ML_SLOTS = (2, 4, 8, 16)  # samples in flight per pixel the many-light config's autotuning tries
 BATCH_SLOTS = (2, 4)  # and the batched config's (4096 x 2048 paths per slot)
 GUIDE_TRAIN_SPP, GUIDE_STEPS, GUIDE_FRAC, GUIDE_SAMPLES = 64, 64, 0.5, 1 << 19
 # Adam learning rates: the workload's stated 1e-3 (the headline figure), and 5e-3, the best
@@ -569,7 +569,8 @@ def render_pass(which, builder):
     acc = _render_setup(sc, builder)
     out = {"config": which, "frame": [sc.width, sc.height]}
     if which == "many_light":
-        # 8 samples in flight per pixel, the slots the bench's autotuning picks for this frame
+        # 8 samples in flight per pixel (the bench's autotuning picks 16 for this frame, 5 %
+        # faster; the stage profile is taken at 8)
         _timed_render(acc, sc, 8, 0, slots=8)
         dt, img, _ = _timed_render(acc, sc, sc.spp, sc.seed, slots=8)
         out.update(spp=sc.spp, slots=8, seconds=round(dt, 4),
